@@ -1,0 +1,220 @@
+"""Synthetic cases of the benchmark configurations, built in memory in the
+reference's input layout (and writable as a reference case directory).
+
+* ``plate2d``  -- single-block steady plate (uniform Cartesian, 1 cell in z):
+  fixed-T faces 1/2, adiabatic 301 q=0 faces 3/4, null faces 5/6, constant
+  properties, seeded uniform IC noise.  128x128, k=50, rho=7800, cp=460,
+  T=500/300 is the 2-D parity configuration.
+* ``cube3d``   -- single-block steady cube with seeded smooth per-axis
+  stretching, a k(T) table from 32 noisy knots resampled to 1 K rows,
+  uniform qvol and convective 303 walls; 256^3 with IRS and 3 levels is the
+  3-D throughput configuration.
+* ``lattice``  -- multi-block 4x4x4 lattice of curvilinear blocks joined by
+  1-to-1 connections (101), three tabulated materials, mixed 302/301/304 walls.
+* ``plate_adjoint`` -- 2-D transient plate with a segmented bottom heat flux.
+* ``sweep``    -- independent single-block steady problems as one multi-block
+  domain (no connections).
+
+Tables follow the reference's properties.dat convention: one row per kelvin
+from T=1, columns cp, rho, k, h with h(T) = sum_{t<=T} rho(t) cp(t) (e.g.
+test/steady-state/temperature_Plate/INPUT/properties.dat: h(1)=rho*cp).
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from . import case as cs
+from . import formats as fm
+
+NT_DEFAULT = 2000
+
+
+def const_table(k, rho, cp, nT=NT_DEFAULT, nmat=1):
+    k = np.broadcast_to(np.asarray(k, np.float64).reshape(-1, 1), (nmat, nT)).copy()
+    rho = np.broadcast_to(np.asarray(rho, np.float64).reshape(-1, 1), (nmat, nT)).copy()
+    cp = np.broadcast_to(np.asarray(cp, np.float64).reshape(-1, 1), (nmat, nT)).copy()
+    h = np.cumsum(rho * cp, axis=1)
+    return fm.PropertyTable(1, cp, rho, k, h, [f"mat{m+1}" for m in range(nmat)])
+
+
+def knot_table(knots_T, knots_v, nT=NT_DEFAULT):
+    """Resample knots to 1 K rows T=1..nT by linear interpolation, clamped ends."""
+    T = np.arange(1, nT + 1, dtype=np.float64)
+    return np.interp(T, knots_T, knots_v)
+
+
+def nodes_from_axes(x, y, z):
+    """Structured node array (nk+1, nj+1, ni+1, 3) from 1-D axis coordinates."""
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    return np.ascontiguousarray(np.stack([X, Y, Z], axis=-1))
+
+
+def stretch(n, L, a, phi):
+    """Smooth monotone map of n cells on [0, L]: spacing ~ 1 + a cos(2 pi xi + phi),
+    so the largest/smallest spacing ratio is (1+a)/(1-a)."""
+    xi = np.linspace(0.0, 1.0, n + 1)
+    s = xi + a / (2 * np.pi) * (np.sin(2 * np.pi * xi + phi) - np.sin(phi))
+    return L * s
+
+
+def face_entries(b, dims, f, typ, rrec, ib_extra=None):
+    """BC entries of face f of block b (1-based), i/j fastest over the face."""
+    ni, nj, nk = dims
+    ib, rb = [], []
+    if f <= 2:
+        i = 1 if f == 1 else ni
+        cells = [(i, j, k) for k in range(1, nk + 1) for j in range(1, nj + 1)]
+    elif f <= 4:
+        j = 1 if f == 3 else nj
+        cells = [(i, j, k) for k in range(1, nk + 1) for i in range(1, ni + 1)]
+    else:
+        k = 1 if f == 5 else nk
+        cells = [(i, j, k) for j in range(1, nj + 1) for i in range(1, ni + 1)]
+    for n, (i, j, k) in enumerate(cells):
+        q = [b, i, j, k, f, typ] + [0] * 9
+        if ib_extra is not None:
+            q[6:15] = ib_extra(i, j, k)
+        ib.append(q)
+        rb.append(list(rrec))
+    return ib, rb
+
+
+def single_block_bc(b, dims, faces):
+    """faces: list of 6 (type, rrec) for faces 1..6."""
+    ib, rb = [], []
+    for f in range(1, 7):
+        typ, rrec = faces[f - 1]
+        a, c = face_entries(b, dims, f, typ, rrec)
+        ib += a
+        rb += c
+    return ib, rb
+
+
+def bclist(ib, rb):
+    ib = np.asarray(ib, np.int32).reshape(-1, fm.BC_NI)
+    rb = np.asarray(rb, np.float64).reshape(-1, fm.BC_NR)
+    return fm.BCList(ib, rb, [None] * ib.shape[0])
+
+
+def R(qw=0.0, Tw=0.0, hconv=0.0, qrad=0.0, Tref=0.0, eps=0.0):
+    return (qw, Tw, hconv, qrad, Tref, eps)
+
+
+def plate2d(n=128, L=1.0, k=50.0, rho=7800.0, cp=460.0, T_left=500.0, T_right=300.0, T0=300.0,
+            noise=1.0, seed=0, n_rk=3, vnn=0.8, prim=True, irs_beta=0.0, res_threshold=1e-10,
+            mgl=1, level_iter=None):
+    dx = L / n
+    x = np.linspace(0.0, L, n + 1)
+    nodes = nodes_from_axes(x, x.copy(), np.array([0.0, dx]))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    T = (T0 + rng.uniform(-noise, noise, size=n * n)).reshape(1, n, n)
+    blk = cs.Block((n, n, 1), nodes, T, np.ones_like(T), np.zeros_like(T))
+    faces = [(302, R(Tw=T_left)), (302, R(Tw=T_right)), (301, R(qw=0.0)), (301, R(qw=0.0)),
+             (0, R()), (0, R())]
+    bcs = []
+    for m in range(mgl):
+        d = cs.coarse_dims((n, n, 1), mgl)[m]
+        bcs.append(bclist(*single_block_bc(1, d, faces)))
+    p = cs.Params(n_rk=n_rk, vnn=vnn, prim=prim, irs_beta=irs_beta, mgl=mgl,
+                  level_iter=level_iter or [0] * mgl, res_threshold=res_threshold)
+    if mgl > 1 and level_iter is None:
+        p.level_iter = [10**9] * mgl
+    return cs.Case([blk], bcs, const_table(k, rho, cp), p, [], [np.full(b.n, -1, np.int32) for b in bcs],
+                   name=f"plate2d_{n}")
+
+
+def cube_table(seed=1, nT=NT_DEFAULT, rho=7800.0, cp=460.0):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kT = np.linspace(250.0, 1500.0, 32)
+    kv = (20.0 + 0.02 * (kT - 300.0)) * (1.0 + 0.05 * rng.uniform(-1.0, 1.0, size=32))
+    k = knot_table(kT, kv, nT)
+    t = const_table(1.0, rho, cp, nT)
+    t.kap[0] = k
+    return t
+
+
+def cube3d(n=256, L=0.1, seed=1, qvol=1e6, hconv=50.0, Tinf=300.0, T0=300.0, n_rk=3, vnn=2.0,
+           prim=True, irs_beta=0.5, mgl=3, level_iter=None, res_threshold=1e-10, stretch_max=0.6):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    axes = []
+    for _ in range(3):
+        a = rng.uniform(0.2, stretch_max)
+        phi = rng.uniform(0.0, 2 * np.pi)
+        axes.append(stretch(n, L, a, phi))
+    nodes = nodes_from_axes(*axes)
+    T = np.full((n, n, n), T0)
+    blk = cs.Block((n, n, n), nodes, T, np.ones_like(T), np.full_like(T, qvol))
+    faces = [(303, R(hconv=hconv, Tref=Tinf))] * 6
+    bcs = [bclist(*single_block_bc(1, d, faces)) for d in cs.coarse_dims((n, n, n), mgl)]
+    p = cs.Params(n_rk=n_rk, vnn=vnn, prim=prim, irs_beta=irs_beta, mgl=mgl,
+                  level_iter=level_iter or [10**9] * mgl, res_threshold=res_threshold)
+    return cs.Case([blk], bcs, cube_table(seed), p, [], [np.full(b.n, -1, np.int32) for b in bcs],
+                   name=f"cube3d_{n}")
+
+
+def sweep(ncase=256, lo=32, hi=96, L=0.1, seed=5, n_rk=3, vnn=2.0, prim=True, irs_beta=0.5, mgl=3,
+          res_threshold=1e-10, T0=300.0, Tinf=300.0):
+    """Independent steady cubes as one multi-block domain: per-case k
+    (constant table row per case), h and qvol."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    blocks, faces_all = [], []
+    ks = rng.uniform(5.0, 200.0, size=ncase)
+    hs = rng.uniform(10.0, 500.0, size=ncase)
+    qs = rng.uniform(0.0, 1e6, size=ncase)
+    for c in range(ncase):
+        dims = tuple(int(4 * rng.integers(lo // 4, hi // 4 + 1)) for _ in range(3))
+        axes = [np.linspace(0.0, L, d + 1) for d in dims]
+        nodes = nodes_from_axes(*axes)
+        T = np.full(dims[::-1], T0)
+        blocks.append(cs.Block(dims, nodes, T, np.full_like(T, float(c + 1)), np.full_like(T, qs[c])))
+        faces_all.append([(303, R(hconv=hs[c], Tref=Tinf))] * 6)
+    bcs = []
+    for m in range(mgl):
+        ib, rb = [], []
+        for c, b in enumerate(blocks):
+            d = cs.coarse_dims(b.dims, mgl)[m]
+            a, r = single_block_bc(c + 1, d, faces_all[c])
+            ib += a
+            rb += r
+        bcs.append(bclist(ib, rb))
+    p = cs.Params(n_rk=n_rk, vnn=vnn, prim=prim, irs_beta=irs_beta, mgl=mgl, level_iter=[10**9] * mgl,
+                  res_threshold=res_threshold)
+    tab = const_table(ks, 7800.0, 460.0, nmat=ncase)
+    return cs.Case(blocks, bcs, tab, p, [], [np.full(b.n, -1, np.int32) for b in bcs], name=f"sweep_{ncase}")
+
+
+def write_case_dir(case, path, ini_extra=""):
+    """Write a Case as a reference case directory (input.ini, INPUT/)."""
+    os.makedirs(os.path.join(path, "INPUT"), exist_ok=True)
+    zones, stz = [], []
+    for b, blk in enumerate(case.blocks):
+        ni, nj, nk = blk.dims
+        d = {"x": blk.nodes[..., 0], "y": blk.nodes[..., 1], "z": blk.nodes[..., 2], "T": blk.T,
+             "matID": blk.matID}
+        zones.append((f"B{b+1}-SP", ni + 1, nj + 1, nk + 1, d))
+        stz.append((f"B{b+1}-ST", ni + 1, nj + 1, nk + 1, {"x": d["x"], "y": d["y"], "z": d["z"], "qvol": blk.qvol}))
+    fm.write_tecplot(os.path.join(path, "INPUT", "ic.tec"), zones, ["x", "y", "z", "T", "matID"], {3, 4})
+    fm.write_tecplot(os.path.join(path, "INPUT", "st.tec"), stz, ["x", "y", "z", "qvol"], {3})
+    for m, bcl in enumerate(case.bcs):
+        fm.write_bc(os.path.join(path, "INPUT", "bc.txt" if m == 0 else f"bc{m+1}.txt"), bcl)
+    fm.write_properties(os.path.join(path, "INPUT", "properties.dat"), case.table)
+    with open(os.path.join(path, "INPUT", "phase.txt"), "w") as fh:
+        fh.write("solid phase\n")
+        for m in range(case.table.nmat):
+            fh.write(f"mat{m+1} 1\n")
+    p = case.params
+    scheme = {1: "euler", 2: "RK2", 3: "RK3"}[p.n_rk]
+    with open(os.path.join(path, "input.ini"), "w") as fh:
+        fh.write("[FUSS-Parameters]\nnewrun = true\n")
+        fh.write(f"res-threshold = {p.res_threshold!r}\ntime-threshold = {p.time_threshold!r}\n")
+        fh.write(f"iter-threshold = {p.iter_threshold}\n\n[FUSS-Numerics]\n")
+        fh.write(f"time-scheme = {scheme}\nvnn = {p.vnn!r}\nvnn-rise-threshold = {p.rampa_iter}\n")
+        fh.write(f"time-accurate = {'true' if p.time_accurate else 'false'}\n")
+        fh.write(f"integration-variables = {'prim' if p.prim else 'cons'}\n")
+        fh.write(f"irs = {'true' if p.irs_beta > 0 else 'false'}\nirs-beta = {p.irs_beta!r}\n\n")
+        fh.write(f"[FUSS-Multigrid]\nlevels = {p.mgl}\n")
+        for m in range(p.mgl):
+            fh.write(f"level{m+1}-iter = {p.level_iter[m]}\n")
+        fh.write(ini_extra)

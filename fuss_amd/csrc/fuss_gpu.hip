@@ -1,0 +1,1753 @@
+// fuss_gpu.hip -- MI355X (gfx950) explicit solver step of FUSS behind the
+// C-ABI of include/fuss_gpu.h.
+//
+// One HIP stream per context.  Per multigrid level all blocks share one
+// arena per field; kernels walk a work-list of 64x4 cell tiles covering every
+// block, so uneven multi-block levels run in one launch per pass.
+//
+// Passes of one Explicit_Step (src/lib/numerics/time/explicit/Mod_Explicit.f90:10-111):
+//   k_dt        local dt from T^n + global min                (Mod_dt.f90:10-84)
+//   per stage:
+//     k_ghost   layer-1 ghosts: mirror or block-to-block copy (Lib_Ghost.f90:11-189)
+//     k_stage   gather-form interior fluxes + BC fluxes + RK   (Mod_Fluxes.f90, Mod_BC_Fluxes.f90,
+//               update (or the scaled residual when IRS is on)  Lib_Newstate.f90)
+//     k_irs     ndir line passes of 2 Jacobi sweeps, the last   (Lib_IRS.f90)
+//               fused with the RK update
+//   k_sum       Sum (T^{n+1}-T^n)^2, deterministic two-level   (Mod_Diagnostic.f90:13-49)
+// Copy_State is free: T^n, and the two stage buffers rotate.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fuss_gpu.h"
+#include "fuss_kernels.h"
+
+using namespace fuss;
+
+#define HIPCHK(x)                                                     \
+  do {                                                                \
+    hipError_t e_ = (x);                                              \
+    if (e_ != hipSuccess) {                                           \
+      fprintf(stderr, "fuss_gpu: HIP error %s at %s:%d\n",            \
+              hipGetErrorString(e_), __FILE__, __LINE__);             \
+      return FUSS_EHIP;                                               \
+    }                                                                 \
+  } while (0)
+
+// ------------------------------------------------------------------------
+// kernels: geometry (setup)
+// ------------------------------------------------------------------------
+struct GeomArgs {
+  const BlockInfo* blk;
+  const Tile* tiles;
+  const double* nodes;   // node arena, 3 per node
+  double* M;             // 9 per padded cell
+  double* vol;
+  double* invL2;         // sum_{d<=ndir} 1/dl_d^2 (Mod_dt.f90:280-283)
+  int ndir;
+  int* status;
+};
+
+__device__ __forceinline__ const double* node_at(const double* nodes, const BlockInfo& B, int i, int j,
+                                                 int k) {
+  return nodes + 3 * (B.node_off + i + (long)(B.ni + 1) * (j + (long)(B.nj + 1) * k));
+}
+
+// Setup_Metrics interior loop, src/lib/numerics/space/Mod_Metrics.f90:23-41
+__global__ __launch_bounds__(TILE_T) void k_cell_metrics(GeomArgs a) {
+  const Tile t = a.tiles[blockIdx.x];
+  const BlockInfo B = a.blk[t.b];
+  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
+  if (i > B.ni || j > B.nj) return;
+  const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
+  double m[9], dl[3], v;
+  const int e = metric_tensor(node_at(a.nodes, B, i - 1, j - 1, k - 1), node_at(a.nodes, B, i - 1, j - 1, k),
+                              node_at(a.nodes, B, i - 1, j, k - 1), node_at(a.nodes, B, i - 1, j, k),
+                              node_at(a.nodes, B, i, j - 1, k - 1), node_at(a.nodes, B, i, j - 1, k),
+                              node_at(a.nodes, B, i, j, k - 1), node_at(a.nodes, B, i, j, k), m, dl, &v);
+  if (e) flag_error(a.status, FUSS_EMETRIC, t.b, i, j, k);
+  for (int q = 0; q < 9; q++) a.M[9 * p + q] = m[q];
+  a.vol[p] = v;
+  double inv = 0.0;
+  {
+#pragma clang fp contract(off)
+    for (int d = 0; d < a.ndir; d++) inv = inv + 1.0 / (dl[d] * dl[d]);
+  }
+  a.invL2[p] = inv;
+}
+
+struct FaceArgs {
+  BlockInfo B;
+  int bidx;
+  int D;                 // direction 0..2
+  double sign;
+  const double* nodes;
+  const double* M;
+  double* F;             // 4 per padded cell: N0,N1,N2,A of the face (cell, cell+1 in D)
+  double* W[3];          // weights of direction D
+  int* tang;             // set when a tangential weight is nonzero
+};
+
+// Compute_Norm_Area, Lib_Metrics.f90:69-247, plus the flux weights of the
+// interior faces (Lib_Diffusive.f90:34-40 regrouped per computational direction).
+__global__ void k_face_metrics(FaceArgs a) {
+  const BlockInfo& B = a.B;
+  const int D = a.D;
+  const long ext0 = B.ni + (D == 0), ext1 = B.nj + (D == 1), ext2 = B.nk + (D == 2);
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= ext0 * ext1 * ext2) return;
+  const int ii = (int)(idx % ext0), jj = (int)((idx / ext0) % ext1), kk = (int)(idx / (ext0 * ext1));
+  // Fortran face indices: dir1 i=0..ni, j=1..nj, k=1..nk, etc.
+  const int i = D == 0 ? ii : ii + 1, j = D == 1 ? jj : jj + 1, k = D == 2 ? kk : kk + 1;
+  const double *p, *q, *r, *s;
+  if (D == 0) {
+    p = node_at(a.nodes, B, i, j, k); q = node_at(a.nodes, B, i, j - 1, k - 1);
+    r = node_at(a.nodes, B, i, j - 1, k); s = node_at(a.nodes, B, i, j, k - 1);
+  } else if (D == 1) {
+    p = node_at(a.nodes, B, i - 1, j, k); q = node_at(a.nodes, B, i, j, k - 1);
+    r = node_at(a.nodes, B, i, j, k); s = node_at(a.nodes, B, i - 1, j, k - 1);
+  } else {
+    p = node_at(a.nodes, B, i - 1, j, k); q = node_at(a.nodes, B, i, j - 1, k);
+    r = node_at(a.nodes, B, i - 1, j - 1, k); s = node_at(a.nodes, B, i, j, k);
+  }
+  double N[3], A;
+  face_normal(p, q, r, s, a.sign, N, &A);
+  // the face (i,j,k) of direction D lies between cells c and c+e_D, c = (i,j,k)
+  // with the D index as given (0 = ghost side of the min face)
+  const long c = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
+  a.F[4 * c + 0] = N[0];
+  a.F[4 * c + 1] = N[1];
+  a.F[4 * c + 2] = N[2];
+  a.F[4 * c + 3] = A;
+  const int idD = D == 0 ? i : (D == 1 ? j : k);
+  const int nD = D == 0 ? B.ni : (D == 1 ? B.nj : B.nk);
+  if (idD >= 1 && idD < nD) {
+#pragma clang fp contract(off)
+    const long st = D == 0 ? 1 : (D == 1 ? (long)B.sy : B.sz);
+    const double* M1 = a.M + 9 * c;
+    const double* M2 = a.M + 9 * (c + st);
+    double Mb[9];
+    for (int e = 0; e < 9; e++) Mb[e] = 0.5 * (M1[e] + M2[e]);
+    int tg = 0;
+    for (int cc = 0; cc < 3; cc++) {
+      const double w = A * (Mb[cc] * N[0] + Mb[cc + 3] * N[1] + Mb[cc + 6] * N[2]);
+      a.W[cc][c] = w;
+      if (cc != D && w != 0.0) tg = 1;
+    }
+    if (tg) atomicOr(a.tang, 1);
+  }
+}
+
+// Fill geometry of BC records from the cell metrics and faces.
+__global__ void k_bc_geometry(BCRec* bc, int n, const BlockInfo* blk, const double* M, const double* F0,
+                              const double* F1, const double* F2) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  BCRec& r = bc[e];
+  const BlockInfo& B = blk[r.b];
+  // Face_Index, Lib_BC_Fluxes.f90:27-45: the face of a min face lies at cell-1
+  int fi = r.i, fj = r.j, fk = r.k;
+  if (r.f == 1) fi -= 1;
+  if (r.f == 3) fj -= 1;
+  if (r.f == 5) fk -= 1;
+  const long fc = B.base + (fi - 1) + (long)(fj - 1) * B.sy + (long)(fk - 1) * B.sz;
+  const double* F = r.dir == 1 ? F0 : (r.dir == 2 ? F1 : F2);
+  r.N[0] = F[4 * fc + 0];
+  r.N[1] = F[4 * fc + 1];
+  r.N[2] = F[4 * fc + 2];
+  r.A = F[4 * fc + 3];
+  for (int q = 0; q < 9; q++) r.M[q] = M[9 * r.cell + q];
+  {
+#pragma clang fp contract(off)
+    for (int rr = 0; rr < 3; rr++)
+      r.mc[rr] = r.Mg[rr] * r.N[0] + r.Mg[rr + 3] * r.N[1] + r.Mg[rr + 6] * r.N[2];
+  }
+}
+
+// ------------------------------------------------------------------------
+// kernels: ghost fill (per stage)
+// ------------------------------------------------------------------------
+// Ghost_Connection / Ghost_Symmetry (Lib_Ghost.f90:132-189), first layer:
+// the explicit step's stencils read no cell beyond it.
+template <typename R>
+__global__ void k_ghost(R* __restrict__ T, const BCRec* __restrict__ bc, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const long d = bc[e].gdst;
+  if (d >= 0) T[d] = T[bc[e].gsrc];
+}
+
+// ------------------------------------------------------------------------
+// kernels: local dt (per iteration)
+// ------------------------------------------------------------------------
+template <typename R>
+struct DtArgs {
+  const BlockInfo* blk;
+  const Tile* tiles;
+  const R* T;
+  const int* mat;
+  const double* invL2;
+  R* dt;
+  Tab<R> tab;
+  double vnn;
+  int iter, rampa;
+  unsigned long long* dtmin;  // bits of the positive double minimum
+  int* status;
+};
+
+// Compute_dt, Mod_dt.f90:26-84
+template <typename R>
+__global__ __launch_bounds__(TILE_T) void k_dt(DtArgs<R> a) {
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Tile t = a.tiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
+  double dtc = 1e300;
+  if (i <= B.ni && j <= B.nj) {
+    const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
+    const R T = a.T[p];
+    const int m = a.mat[p];
+    int bad = 0;
+    const long row = (long)m * a.tab.nT;
+    const R kap = tab_interp(a.tab.kap + row, T, a.tab.Ti1, a.tab.Ti2, &bad);
+    const R rho = tab_interp(a.tab.rho + row, T, a.tab.Ti1, a.tab.Ti2, &bad);
+    const R cp = tab_interp(a.tab.cp + row, T, a.tab.Ti1, a.tab.Ti2, &bad);
+    {
+#pragma clang fp contract(off)
+      const double alpha = (double)kap / (double)rho / (double)cp;
+      double d = 0.5 * a.vnn / (alpha * a.invL2[p]);
+      d = d < 1e8 ? d : 1e8;
+      if (a.iter < a.rampa) d = d * (double)a.iter / (double)a.rampa;
+      dtc = d;
+    }
+    a.dt[p] = (R)dtc;
+    if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+  }
+  // workgroup min, then one atomic per workgroup
+  __shared__ double sm[TILE_T / 64];
+  for (int o = 32; o > 0; o >>= 1) dtc = fmin(dtc, __shfl_xor(dtc, o, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = dtc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = sm[0];
+    for (int q = 1; q < TILE_T / 64; q++) v = fmin(v, sm[q]);
+    if (v < 1e300) atomicMin(a.dtmin, (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+// ------------------------------------------------------------------------
+// kernels: stage (fluxes + BC fluxes + newstate)
+// ------------------------------------------------------------------------
+template <typename R>
+struct StageArgs {
+  const BlockInfo* blk;
+  const Tile* tiles;
+  const R* T;          // stage input (ghosts filled)
+  const R* TO;         // T^n
+  R* Tout;             // stage output (update mode)
+  R* Rres;             // scaled residual (IRS mode)
+  const R* dt;
+  const double* dtg;   // device dtglobal (time-accurate)
+  int use_dtg;
+  const R* qvol;
+  const R* vol;
+  const int* mat;
+  const R* W[9];
+  const BCRec* bc;
+  const int* fmap;
+  Tab<R> tab;
+  double sval[MAX_SERIES];
+  R rkc;
+  int last;
+  double* partials;
+  int* status;
+};
+
+// One interior face flux between cells q and q+e_D (left cell q), in the
+// gather form of Diffusive_Flux (Lib_Diffusive.f90:10-45).
+template <int D, int NDIR, bool TANG, typename R>
+__device__ __forceinline__ R face_flux(const StageArgs<R>& a, const BlockInfo& B, long q) {
+  const R* __restrict__ T = a.T;
+  const long sy = B.sy, sz = B.sz;
+  const long s = D == 0 ? 1 : (D == 1 ? sy : sz);
+  const R Tl = T[q], Tr = T[q + s];
+  R G0, G1 = 0, G2 = 0;
+  G0 = Tr - Tl;
+  if (TANG) {
+    // tangential differences, Tangential_Gradient (Lib_Diffusive.f90:48-55)
+    const long t1 = D == 0 ? sy : 1;
+    const long t2 = D == 2 ? sy : sz;
+    G1 = (T[q + t1] - T[q - t1] + T[q + s + t1] - T[q + s - t1]) * (R)0.25;
+    if (NDIR == 3 || D == 2) G2 = (T[q + t2] - T[q - t2] + T[q + s + t2] - T[q + s - t2]) * (R)0.25;
+  }
+  int bad = 0;
+  const R kap = tab_interp(a.tab.kap + (long)a.mat[q] * a.tab.nT, (R)0.5 * (Tl + Tr), a.tab.Ti1,
+                           a.tab.Ti2, &bad);
+  if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
+  // weights in computational-direction order; the normal direction is D
+  const R wn = a.W[3 * D + D][q];
+  if (!TANG) return kap * (G0 * wn);
+  const int c1 = D == 0 ? 1 : 0, c2 = D == 2 ? 1 : 2;
+  const R w1 = a.W[3 * D + c1][q], w2 = a.W[3 * D + c2][q];
+  // sum over c = 0,1,2 in order of the computational index
+  R g[3], w[3];
+  g[D] = G0; w[D] = wn;
+  g[c1] = G1; w[c1] = w1;
+  g[c2] = G2; w[c2] = w2;
+  return kap * ((g[0] * w[0] + g[1] * w[1]) + g[2] * w[2]);
+}
+
+template <typename R>
+__device__ __forceinline__ R dotrow(const double* M, int r, const double* N) {
+  // dot_product(M(r,:), N), r = 1..3
+  return (R)(M[r - 1] * N[0] + M[r - 1 + 3] * N[1] + M[r - 1 + 6] * N[2]);
+}
+
+// BC_Fluxes for one entry (Mod_BC_Fluxes.f90:43-79 and the per-type files
+// src/lib/numerics/fluxes/bc/Lib_BC_Fluxes_*.f90), in the reference's
+// arithmetic order.
+template <int NDIR, typename R>
+__device__ R bc_flux(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e, long p, R Rv) {
+#pragma clang fp contract(off)
+  const R* __restrict__ T = a.T;
+  const int Fm = e.f;
+  const int modfm2 = 1 - 2 * (Fm % 2);
+  const int modfm3 = 2 * (2 * (Fm % 2) - 1);
+  const R Tc = T[p];
+  const long sy = B.sy, sz = B.sz;
+  const double* M = e.M;
+  const double* Nr = e.N;
+  int bad = 0;
+  const R* krow = a.tab.kap + (long)a.mat[p] * a.tab.nT;
+  switch (e.type) {
+    case 101: {
+      // BC_Connection, Lib_BC_Fluxes_Connection.f90:12-113
+      const R Tip = T[p + 1], Tim = T[p - 1], Tjp = T[p + sy], Tjm = T[p - sy];
+      R Tkp = 0, Tkm = 0;
+      if (NDIR == 3) { Tkp = T[p + sz]; Tkm = T[p - sz]; }
+      const R Tcon = T[e.gsrc];
+      const R kl = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
+      const R kc = tab_interp(a.tab.kap + (long)e.matg * a.tab.nT, Tcon, a.tab.Ti1, a.tab.Ti2, &bad);
+      const R g3 = T[e.dn[0]], g4 = T[e.dn[1]], g5 = T[e.dn[2]], g6 = T[e.dn[3]];
+      const R ml0 = dotrow<R>(M, 1, Nr), ml1 = dotrow<R>(M, 2, Nr), ml2 = dotrow<R>(M, 3, Nr);
+      const R mc0 = (R)e.mc[0], mc1 = (R)e.mc[1], mc2 = (R)e.mc[2];
+      R Dl, Nl, Dc, Nc;
+      if (Fm <= 2) {
+        Dl = (R)2.0 * ml0 * modfm2;
+        Nl = (R)2.0 * Tc * ml0 * modfm2 - (R)0.5 * (Tjp - Tjm) * ml1 - (R)0.5 * (Tkp - Tkm) * ml2;
+        Dc = (R)2.0 * mc0 * modfm2;
+        Nc = (R)2.0 * Tcon * mc0 * modfm2 + (R)0.5 * (g4 - g3) * mc1 + (R)0.5 * (g6 - g5) * mc2;
+      } else if (Fm <= 4) {
+        Dl = (R)2.0 * ml1 * modfm2;
+        Nl = -(R)0.5 * (Tip - Tim) * ml0 + (R)2.0 * Tc * ml1 * modfm2 - (R)0.5 * (Tkp - Tkm) * ml2;
+        Dc = (R)2.0 * mc1 * modfm2;
+        Nc = (R)0.5 * (g4 - g3) * mc0 + (R)2.0 * Tcon * mc1 * modfm2 + (R)0.5 * (g6 - g5) * mc2;
+      } else {
+        Dl = (R)2.0 * ml2 * modfm2;
+        Nl = -(R)0.5 * (Tip - Tim) * ml0 - (R)0.5 * (Tjp - Tjm) * ml1 + (R)2.0 * Tc * ml2 * modfm2;
+        Dc = (R)2.0 * mc2 * modfm2;
+        Nc = (R)0.5 * (g4 - g3) * mc0 + (R)0.5 * (g6 - g5) * mc1 + (R)2.0 * Tcon * mc2 * modfm2;
+      }
+      const R Tint = (kl * Nl + kc * Nc) / (kl * Dl + kc * Dc);
+      R G[3];
+      if (Fm <= 2) { G[0] = (R)2.0 * (Tint - Tc) * modfm2; G[1] = (Tjp - Tjm) * (R)0.5; G[2] = (Tkp - Tkm) * (R)0.5; }
+      else if (Fm <= 4) { G[0] = (Tip - Tim) * (R)0.5; G[1] = (R)2.0 * (Tint - Tc) * modfm2; G[2] = (Tkp - Tkm) * (R)0.5; }
+      else { G[0] = (Tip - Tim) * (R)0.5; G[1] = (Tjp - Tjm) * (R)0.5; G[2] = (R)2.0 * (Tint - Tc) * modfm2; }
+      if (NDIR == 2) G[2] = 0;
+      R gr[3];
+      for (int x = 0; x < 3; x++) gr[x] = G[0] * (R)M[3 * x] + G[1] * (R)M[3 * x + 1] + G[2] * (R)M[3 * x + 2];
+      const R Flux = (R)e.A * kl * (gr[0] * (R)Nr[0] + gr[1] * (R)Nr[1] + gr[2] * (R)Nr[2]);
+      Rv = Rv - (R)modfm2 * Flux;
+    } break;
+    case 300: {
+      // BC_Symmetry, Lib_BC_Fluxes_Symmetry.f90:12-67
+      R G[3] = {0, 0, 0};
+      if (Fm > 2) G[0] = (R)0.5 * (T[p + 1] - T[p - 1]) * modfm2;
+      if (Fm <= 2 || Fm > 4) G[1] = (R)0.5 * (T[p + sy] - T[p - sy]) * modfm2;
+      if (NDIR == 3 && Fm <= 4) G[2] = (R)0.5 * (T[p + sz] - T[p - sz]) * modfm2;
+      R gr[3];
+      for (int x = 0; x < 3; x++) gr[x] = G[0] * (R)M[3 * x] + G[1] * (R)M[3 * x + 1] + G[2] * (R)M[3 * x + 2];
+      const R kappa = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
+      const R Flux = (R)e.A * kappa * (gr[0] * (R)Nr[0] + gr[1] * (R)Nr[1] + gr[2] * (R)Nr[2]);
+      Rv = Rv - (R)modfm2 * Flux;
+    } break;
+    case 301: {
+      // BC_Wall_Heat, Lib_BC_Fluxes_Wall_Heat.f90:12-54
+      const R qw = e.series >= 0 ? (R)a.sval[e.series] : (R)e.qw;
+      const R Flux = (R)e.A * qw;
+      Rv = Rv - (R)modfm2 * Flux;
+    } break;
+    case 302: {
+      // BC_Wall_Temperature, Lib_BC_Fluxes_Wall_Temperature.f90:12-68
+      const R Tw = e.series >= 0 ? (R)a.sval[e.series] : (R)e.Tw;
+      const R kappa = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
+      R G[3] = {0, 0, 0};
+      G[e.dir - 1] = (Tc - Tw) * (R)modfm3;
+      if (Fm <= 2) {
+        G[1] = (T[p - sy] - T[p + sy]) / (R)2.0;
+        if (NDIR == 3) G[2] = (T[p - sz] - T[p + sz]) / (R)2.0;
+      } else if (Fm <= 4) {
+        G[0] = (T[p - 1] - T[p + 1]) / (R)2.0;
+        if (NDIR == 3) G[2] = (T[p - sz] - T[p + sz]) / (R)2.0;
+      } else {
+        G[0] = (T[p - 1] - T[p + 1]) / (R)2.0;
+        G[1] = (T[p - sy] - T[p + sy]) / (R)2.0;
+      }
+      if (NDIR == 2) G[2] = 0;
+      R gr[3];
+      for (int x = 0; x < 3; x++) gr[x] = G[0] * (R)M[3 * x] + G[1] * (R)M[3 * x + 1] + G[2] * (R)M[3 * x + 2];
+      const R Flux = (R)e.A * kappa * (gr[0] * (R)Nr[0] + gr[1] * (R)Nr[1] + gr[2] * (R)Nr[2]);
+      Rv = Rv - (R)modfm2 * Flux;
+    } break;
+    case 303: {
+      // BC_Wall_HeatTransfer, Lib_BC_Fluxes_Wall_HeatTransfer.f90:12-52
+      // (the reference passes bc%qw, never set for 303: 0 here)
+      const R kappa = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
+      const R mn = dotrow<R>(M, e.dir, Nr);
+      const R H = (R)1.0 / ((R)2.0 * kappa * mn) + (R)1.0 / (R)e.hconv;
+      const R Hrad = (R)e.hconv / ((R)2.0 * kappa * mn) + (R)1.0;
+      const R Flux = (R)e.A * ((R)1.0 / H * ((R)e.Tref - Tc) + (R)1.0 / Hrad * (R)0.0);
+      Rv = Rv - Flux;
+    } break;
+    case 304:
+    case 305: {
+      // BC_Wall_Radiation(+Convection), Lib_BC_Fluxes_Wall_Radiation.f90:12-65 and
+      // Lib_BC_Fluxes_Wall_RadiationConvection.f90:12-65.  The fixed point stays in
+      // FP64: its tolerance (1e-3 W/m^2) is below the FP32 resolution of q.
+      const double kappa = (double)tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
+      const double mn = M[e.dir - 1] * Nr[0] + M[e.dir - 1 + 3] * Nr[1] + M[e.dir - 1 + 6] * Nr[2];
+      const double Tref = e.Tref, Tcd = (double)Tc;
+      const double Tr2 = Tref * Tref, Tr4 = Tr2 * Tr2;
+      double Tw = Tcd, qw = 0.0, err = 1.0e6, Flux = 0.0;
+      int it = 0;
+      while (err > 1.0e-3) {
+        const double Tw2 = Tw * Tw;
+        const double K = SIGMA_SB * e.eps * (Tw2 * Tw2 - Tr4) / (Tw - Tref);
+        const double H = e.type == 304 ? 1.0 / (2.0 * kappa * mn) + 1.0 / K
+                                       : 1.0 / (2.0 * kappa * mn) + 1.0 / (K + e.hconv);
+        Flux = e.A * (1.0 / H * (Tref - Tcd));
+        const double q_wall = K * (Tref - Tw);
+        err = fabs(qw - q_wall);
+        qw = q_wall;
+        Tw = Tref - 1.0 / (H * K) * (Tref - Tcd);
+        if (++it > 1000 || err != err) {
+          flag_error(a.status, FUSS_ESTATE, e.b, e.i, e.j, e.k);
+          break;
+        }
+      }
+      Rv = Rv - (R)Flux;
+    } break;
+    default:
+      break;
+  }
+  if (bad) flag_error(a.status, FUSS_ETABLE, e.b, e.i, e.j, e.k);
+  return Rv;
+}
+
+template <typename R>
+__device__ __forceinline__ int face_map_index(const BlockInfo& B, int f, int i, int j, int k) {
+  if (f <= 2) return B.fmap_off[f - 1] + (j - 1) + B.nj * (k - 1);
+  if (f <= 4) return B.fmap_off[f - 1] + (i - 1) + B.ni * (k - 1);
+  return B.fmap_off[f - 1] + (i - 1) + B.ni * (j - 1);
+}
+
+// RK_stage, Lib_RK.f90:15-26
+template <typename R>
+__device__ __forceinline__ R rk_stage(R state, R old, R res, R c) {
+#pragma clang fp contract(off)
+  const R r_ = (state - old + res) * c;
+  return old + r_;
+}
+
+// Newstate for one cell from its scaled residual.  PRIM: Newstate_Primitive
+// (Lib_Newstate.f90:241-270); else Newstate_Conservative (:99-146).
+template <bool PRIM, typename R>
+__device__ __forceinline__ R new_state(const Tab<R>& tab, int m, R T, R TO, R res, R c, int* bad) {
+#pragma clang fp contract(off)
+  if (PRIM) return rk_stage(T, TO, res, c);
+  const long row = (long)m * tab.nT;
+  const R hO = tab_interp(tab.h + row, TO, tab.Ti1, tab.Ti2, bad);
+  R h = tab_interp(tab.h + row, T, tab.Ti1, tab.Ti2, bad);
+  h = rk_stage(h, hO, res, c);
+  return tab_inverse(tab.h + row, h, T, tab.Ti1, tab.Ti2, bad);
+}
+
+template <bool PRIM, typename R>
+__device__ __forceinline__ R scaled_residual(const Tab<R>& tab, int m, R T, R Rv, R vol, R qvol, R dt,
+                                             int* bad) {
+#pragma clang fp contract(off)
+  if (!PRIM) return (-Rv / vol + qvol) * dt;  // Scale_Residual_Cons, Lib_Newstate.f90:99-104
+  const long row = (long)m * tab.nT;
+  const R rho = tab_interp(tab.rho + row, T, tab.Ti1, tab.Ti2, bad);
+  const R cp = tab_interp(tab.cp + row, T, tab.Ti1, tab.Ti2, bad);
+  const R drho = tab_slope(tab.rho + row, T, tab.Ti1, tab.Ti2);
+  const R dcp = tab_slope(tab.cp + row, T, tab.Ti1, tab.Ti2);
+  // Compute_Prim_Residual, Lib_Newstate.f90:241-256
+  return (-Rv / vol + qvol) * dt / (T * cp * drho + rho * cp + T * rho * dcp);
+}
+
+__device__ __forceinline__ void block_sum(double v, double* partials, int wg) {
+  __shared__ double sm[TILE_T / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = sm[0];
+    for (int q = 1; q < TILE_T / 64; q++) s += sm[q];
+    partials[wg] = s;
+  }
+}
+
+// IRSMODE: write the scaled residual to Rres (smoothing follows);
+// otherwise apply the RK update to Tout.
+template <int NDIR, bool TANG, bool PRIM, bool IRSMODE, typename R>
+__global__ __launch_bounds__(TILE_T) void k_stage(StageArgs<R> a) {
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Tile t = a.tiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
+  double rsq = 0.0;
+  if (i <= B.ni && j <= B.nj) {
+    const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
+    R Rv = 0;
+    {
+#pragma clang fp contract(off)
+      // Fluxes_blk ordering (Mod_Fluxes.f90:54-192): i faces, then j, then k;
+      // Res2 += F for the face on the min side, Res1 -= F on the max side.
+      if (i > 1) Rv = Rv + face_flux<0, NDIR, TANG>(a, B, p - 1);
+      if (i < B.ni) Rv = Rv - face_flux<0, NDIR, TANG>(a, B, p);
+      if (j > 1) Rv = Rv + face_flux<1, NDIR, TANG>(a, B, p - B.sy);
+      if (j < B.nj) Rv = Rv - face_flux<1, NDIR, TANG>(a, B, p);
+      if (NDIR == 3) {
+        if (k > 1) Rv = Rv + face_flux<2, NDIR, TANG>(a, B, p - B.sz);
+        if (k < B.nk) Rv = Rv - face_flux<2, NDIR, TANG>(a, B, p);
+      }
+    }
+    if (B.face_active) {
+      // BC entries of the cell, in the (block, face) order of the BC list
+      const int on[6] = {i == 1, i == B.ni, j == 1, j == B.nj, k == 1, k == B.nk};
+      for (int f = 1; f <= 6; f++) {
+        if (on[f - 1] && (B.face_active >> (f - 1) & 1)) {
+          const int e = a.fmap[face_map_index<R>(B, f, i, j, k)];
+          Rv = bc_flux<NDIR>(a, B, a.bc[e], p, Rv);
+        }
+      }
+    }
+    const R T = a.T[p];
+    const int m = a.mat[p];
+    const R dt = a.use_dtg ? (R)(*a.dtg) : a.dt[p];
+    int bad = 0;
+    const R res = scaled_residual<PRIM>(a.tab, m, T, Rv, a.vol[p], a.qvol[p], dt, &bad);
+    if (IRSMODE) {
+      a.Rres[p] = res;
+    } else {
+      const R TO = a.TO[p];
+      const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, &bad);
+      a.Tout[p] = Tn;
+      // Check_And_Fix_State, Lib_Newstate.f90:278-290
+      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+      const double d = (double)Tn - (double)TO;
+      rsq = d * d;
+    }
+    if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+  }
+  if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
+}
+
+// ------------------------------------------------------------------------
+// kernels: implicit residual smoothing
+// ------------------------------------------------------------------------
+template <typename R>
+struct IrsArgs {
+  const BlockInfo* blk;
+  const Tile* tiles;
+  const R* Rin;
+  R* Rout;
+  const BCRec* bc;
+  const int* fmap;
+  R beta, den;         // den = 1 + 2 beta
+  // fused update (last direction)
+  const R* T;
+  const R* TO;
+  R* Tout;
+  const int* mat;
+  Tab<R> tab;
+  R rkc;
+  int last;
+  double* partials;
+  int* status;
+};
+
+// Residual_Smoothing_Blk along DIR (Lib_IRS.f90:43-134): two Jacobi sweeps
+// r*_new(s) = (r(s) + beta (r*(s-1) + r*(s+1))) / (1 + 2 beta), r* = r
+// initially, ghosts from Ghost_Residual_Extrapolation (:137-187).  The
+// second sweep needs r(s-2..s+2) only, so each cell is computed directly.
+template <int DIR, int NDIR, bool UPDATE, bool PRIM, typename R>
+__global__ __launch_bounds__(TILE_T) void k_irs(IrsArgs<R> a) {
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Tile t = a.tiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
+  double rsq = 0.0;
+  if (i <= B.ni && j <= B.nj) {
+#pragma clang fp contract(off)
+    const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
+    const long st = DIR == 0 ? 1 : (DIR == 1 ? (long)B.sy : B.sz);
+    const int s = DIR == 0 ? i : (DIR == 1 ? j : k);
+    const int n = DIR == 0 ? B.ni : (DIR == 1 ? B.nj : B.nk);
+    const R* __restrict__ Rin = a.Rin;
+    R glo = 0, ghi = 0;
+    if (s <= 2) {
+      const int f = 2 * DIR + 1;
+      const int i1 = DIR == 0 ? 1 : i, j1 = DIR == 1 ? 1 : j, k1 = DIR == 2 ? 1 : k;
+      const BCRec& e = a.bc[a.fmap[face_map_index<R>(B, f, i1, j1, k1)]];
+      glo = e.type == 101 ? Rin[e.rdonor] : Rin[p - (long)(s - 1) * st];
+    }
+    if (s >= n - 1) {
+      const int f = 2 * DIR + 2;
+      const int i1 = DIR == 0 ? B.ni : i, j1 = DIR == 1 ? B.nj : j, k1 = DIR == 2 ? B.nk : k;
+      const BCRec& e = a.bc[a.fmap[face_map_index<R>(B, f, i1, j1, k1)]];
+      ghi = e.type == 101 ? Rin[e.rdonor] : Rin[p + (long)(n - s) * st];
+    }
+    auto r = [&](int d) -> R {
+      const int q = s + d;
+      if (q < 1) return glo;
+      if (q > n) return ghi;
+      return Rin[p + d * st];
+    };
+    const R beta = a.beta, den = a.den;
+    // first sweep at s-1 and s+1 (ghost cells keep r*)
+    const R rm = (s - 1 < 1) ? glo : (r(-1) + beta * (r(-2) + r(0))) / den;
+    const R rp = (s + 1 > n) ? ghi : (r(1) + beta * (r(0) + r(2))) / den;
+    const R rs = (r(0) + beta * (rm + rp)) / den;
+    if (!UPDATE) {
+      a.Rout[p] = rs;
+    } else {
+      const R T = a.T[p], TO = a.TO[p];
+      int bad = 0;
+      const R Tn = new_state<PRIM>(a.tab, a.mat[p], T, TO, rs, a.rkc, &bad);
+      a.Tout[p] = Tn;
+      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+      const double d = (double)Tn - (double)TO;
+      rsq = d * d;
+    }
+  }
+  if (UPDATE && a.last) block_sum(rsq, a.partials, wg);
+}
+
+// ------------------------------------------------------------------------
+// kernels: reductions
+// ------------------------------------------------------------------------
+// Deterministic sum of the per-workgroup partials (fixed order).
+__global__ __launch_bounds__(256) void k_sum(const double* __restrict__ partials, int n,
+                                             double* __restrict__ out) {
+  __shared__ double sm[256];
+  double s = 0.0;
+  for (int q = threadIdx.x; q < n; q += 256) s += partials[q];
+  sm[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sm[0];
+}
+
+__global__ void k_init_scalars(unsigned long long* dtmin, unsigned long long bits, double* scal,
+                               double time) {
+  *dtmin = bits;
+  scal[1] = time;
+}
+
+// dtglobal from the min bits; the time advance of time-accurate runs
+// (Mod_Explicit.f90:40-45).
+__global__ void k_dt_finish(const unsigned long long* dtmin, double* scal, int time_accurate) {
+  const double m = __longlong_as_double((long long)*dtmin);
+  scal[0] = m;
+  if (time_accurate) scal[1] = scal[1] + m;
+}
+
+// ------------------------------------------------------------------------
+// kernels: multigrid
+// ------------------------------------------------------------------------
+struct MGArgs {
+  BlockInfo F, Cb;
+  double* fT;
+  const int* fmat;
+  const double* fvol;
+  const double* fqvol;
+  double* cT;
+  int* cmat;
+  const double* cvol;
+  double* cqvol;
+  Tab<double> tab;
+  int* status;
+  int bidx;
+};
+
+// fine2coarse_prim, Lib_Multigrid.f90:38-125
+__global__ void k_restrict(MGArgs a) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const BlockInfo& C = a.Cb;
+  const BlockInfo& F = a.F;
+  if (idx >= (long)C.ni * C.nj * C.nk) return;
+  const int i = 1 + (int)(idx % C.ni), j = 1 + (int)((idx / C.ni) % C.nj), k = 1 + (int)(idx / ((long)C.ni * C.nj));
+  const bool two_d = F.nk == 1;
+  const int i2 = 2 * i, j2 = 2 * j, k2 = two_d ? 1 : 2 * k, i2d = i2 - 1, j2d = j2 - 1, k2d = two_d ? 1 : k2 - 1;
+  auto fp = [&](int ii, int jj, int kk) { return F.base + (ii - 1) + (long)(jj - 1) * F.sy + (long)(kk - 1) * F.sz; };
+  int bad = 0;
+  auto fC = [&](int ii, int jj, int kk) {
+    const long q = fp(ii, jj, kk);
+    return tab_interp(a.tab.h + (long)a.fmat[q] * a.tab.nT, a.fT[q], a.tab.Ti1, a.tab.Ti2, &bad);
+  };
+  double cc, cq, tv;
+  {
+#pragma clang fp contract(off)
+    const long q1 = fp(i2d, j2d, k2d), q2 = fp(i2, j2d, k2d), q3 = fp(i2d, j2, k2d), q4 = fp(i2, j2, k2d);
+    if (two_d) {
+      cc = fC(i2d, j2d, 1) * a.fvol[q1] + fC(i2, j2d, 1) * a.fvol[q2] + fC(i2d, j2, 1) * a.fvol[q3] +
+           fC(i2, j2, 1) * a.fvol[q4];
+      cq = a.fqvol[q1] * a.fvol[q1] + a.fqvol[q2] * a.fvol[q2] + a.fqvol[q3] * a.fvol[q3] + a.fqvol[q4] * a.fvol[q4];
+      tv = a.fvol[q1] + a.fvol[q2] + a.fvol[q3] + a.fvol[q4];
+    } else {
+      const long q5 = fp(i2d, j2d, k2), q6 = fp(i2, j2d, k2), q7 = fp(i2d, j2, k2), q8 = fp(i2, j2, k2);
+      cc = fC(i2d, j2d, k2d) * a.fvol[q1] + fC(i2, j2d, k2d) * a.fvol[q2] + fC(i2d, j2, k2d) * a.fvol[q3] +
+           fC(i2, j2, k2d) * a.fvol[q4] + fC(i2d, j2d, k2) * a.fvol[q5] + fC(i2, j2d, k2) * a.fvol[q6] +
+           fC(i2d, j2, k2) * a.fvol[q7] + fC(i2, j2, k2) * a.fvol[q8];
+      cq = a.fqvol[q1] * a.fvol[q1] + a.fqvol[q2] * a.fvol[q2] + a.fqvol[q3] * a.fvol[q3] +
+           a.fqvol[q4] * a.fvol[q4] + a.fqvol[q5] * a.fvol[q5] + a.fqvol[q6] * a.fvol[q6] +
+           a.fqvol[q7] * a.fvol[q7] + a.fqvol[q8] * a.fvol[q8];
+      tv = a.fvol[q1] + a.fvol[q2] + a.fvol[q3] + a.fvol[q4] + a.fvol[q5] + a.fvol[q6] + a.fvol[q7] +
+           a.fvol[q8];
+    }
+  }
+  const long pc = C.base + (i - 1) + (long)(j - 1) * C.sy + (long)(k - 1) * C.sz;
+  a.cqvol[pc] = cq / tv;
+  // cmatID = fmatID(2i,2j,2k); in 2-D the k=2 plane is the mirror of plane 1
+  const int m = a.fmat[fp(i2, j2, two_d ? 1 : k2)];
+  a.cmat[pc] = m;
+  const double ccv = cc / a.cvol[pc];
+  a.cT[pc] = tab_inverse_search(a.tab.h + (long)m * a.tab.nT, ccv, a.tab.Ti1, a.tab.Ti2, &bad);
+  if (bad) flag_error(a.status, FUSS_ETABLE, a.bidx, i, j, k);
+}
+
+// coarse2fine_prim, Lib_Multigrid.f90:128-220
+__global__ void k_prolong(MGArgs a) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const BlockInfo& C = a.Cb;
+  const BlockInfo& F = a.F;
+  if (idx >= (long)C.ni * C.nj * C.nk) return;
+  const int i = 1 + (int)(idx % C.ni), j = 1 + (int)((idx / C.ni) % C.nj), k = 1 + (int)(idx / ((long)C.ni * C.nj));
+  const bool two_d = F.nk == 1;
+  double co[8];
+  if (two_d) {
+    const double a1 = 9.0 / 16.0, a2 = 3.0 / 16.0, a3 = 1.0 / 16.0, a4 = 0.0;
+    co[0] = a1; co[1] = a2; co[2] = a2; co[3] = a4; co[4] = a3; co[5] = a4; co[6] = a4; co[7] = a4;
+  } else {
+    const double a1 = 27.0 / 64.0, a2 = 9.0 / 64.0, a3 = 3.0 / 64.0, a4 = 1.0 / 64.0;
+    co[0] = a1; co[1] = a2; co[2] = a2; co[3] = a2; co[4] = a3; co[5] = a3; co[6] = a3; co[7] = a4;
+  }
+  const int id[6] = {i - 1 > 1 ? i - 1 : 1, j - 1 > 1 ? j - 1 : 1, k - 1 > 1 ? k - 1 : 1,
+                     i + 1 < C.ni ? i + 1 : C.ni, j + 1 < C.nj ? j + 1 : C.nj, k + 1 < C.nk ? k + 1 : C.nk};
+  auto cP = [&](int ii, int jj, int kk) { return a.cT[C.base + (ii - 1) + (long)(jj - 1) * C.sy + (long)(kk - 1) * C.sz]; };
+  const int i2 = 2 * i, j2 = 2 * j, k2 = two_d ? 1 : 2 * k;
+  const int i2d = i2 - 1, j2d = j2 - 1, k2d = two_d ? 1 : k2 - 1;
+  int counter = 0;
+  for (int kk = k2d; kk <= k2; kk++)
+    for (int jj = j2d; jj <= j2; jj++)
+      for (int ii = i2d; ii <= i2; ii++) {
+        const int m0 = (counter & 1) ? 3 : 0, m1 = (counter & 2) ? 4 : 1, m2 = (counter & 4) ? 5 : 2;
+        const int A0 = id[m0], B1 = id[m1], C2 = id[m2];
+        double v;
+        {
+#pragma clang fp contract(off)
+          v = co[0] * cP(i, j, k) + co[1] * cP(A0, j, k) + co[2] * cP(i, B1, k) + co[3] * cP(i, j, C2) +
+              co[4] * cP(A0, B1, k) + co[5] * cP(A0, j, C2) + co[6] * cP(i, B1, C2) + co[7] * cP(A0, B1, C2);
+        }
+        a.fT[F.base + (ii - 1) + (long)(jj - 1) * F.sy + (long)(kk - 1) * F.sz] = v;
+        counter++;
+      }
+}
+
+// ------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    free();
+    n = count;
+    if (!count) return 0;
+    if (hipMalloc(&p, sizeof(T) * count) != hipSuccess) { p = nullptr; return FUSS_EHIP; }
+    if (hipMemset(p, 0, sizeof(T) * count) != hipSuccess) return FUSS_EHIP;
+    return 0;
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct HostBlock {
+  int n[3] = {0, 0, 0};
+  std::vector<double> nodes, T, matID, qvol;
+};
+
+struct Level {
+  int nb = 0;
+  int kp = 2;                       // k ghost layers (0 in 2-D)
+  std::vector<BlockInfo> hblk;
+  std::vector<HostBlock> hb;        // host nodes (every level), fields (level 1)
+  std::vector<Tile> htiles;
+  long npad = 0, ncell = 0, nnodes = 0;
+  DBuf<BlockInfo> blk;
+  DBuf<Tile> tiles;
+  DBuf<double> nodes;
+  DBuf<double> Tb[3];               // T^n and two stage buffers
+  int cur = 0;                      // index of T^n in Tb
+  DBuf<double> R, R2, dt, qvol, vol, invL2;
+  DBuf<int> mat;
+  DBuf<double> M, F[3], W[9];
+  DBuf<BCRec> bc;
+  DBuf<int> fmap;
+  int nbound = 0;
+  std::vector<int> hib;             // nbound x 15
+  std::vector<double> hrb;          // nbound x 6
+  std::vector<int> hser;
+  int tang = 0, any_active = 0;
+  double time = 0.0;
+  int iter = 0;
+  DBuf<double> partials;
+};
+
+}  // namespace
+
+struct fuss_ctx {
+  int nlev = 0, nb = 0, precision = FUSS_FP64, ndir = 0;
+  std::vector<Level> lev;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int nmat = 0, Ti1 = 0, nT = 0;
+  std::vector<double> tcp, trho, tkap, th;
+  DBuf<double> dcp, drho, dkap, dh;
+  struct Series { int periodic; std::vector<double> t, v; };
+  std::vector<Series> series;
+  DBuf<int> status;                 // code, b, i, j, k
+  DBuf<double> scal;                // dtglobal, time, res_sq
+  DBuf<unsigned long long> dtmin;
+  double* hscal = nullptr;          // pinned readback
+  int setup_done = 0;
+  int* hstatus = nullptr;
+};
+
+namespace {
+
+long cidx(const BlockInfo& B, int i, int j, int k) {
+  return B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
+}
+
+// Compute_Norm_Area sign per direction (Lib_Metrics.f90:86-153), host side.
+double norm_sign(const HostBlock& h, int dir) {
+  auto N = [&](int i, int j, int k) { return &h.nodes[3 * (i + (long)(h.n[0] + 1) * (j + (long)(h.n[1] + 1) * k))]; };
+  double d1[3], d2[3], sn[3], c1[3], c2[3];
+  const double *a, *b, *c, *d;
+  if (dir == 0) { for (int x = 0; x < 3; x++) { d1[x] = N(1, 1, 1)[x] - N(1, 0, 0)[x]; d2[x] = N(1, 0, 1)[x] - N(1, 1, 0)[x]; } }
+  else if (dir == 1) { for (int x = 0; x < 3; x++) { d1[x] = N(0, 1, 1)[x] - N(1, 1, 0)[x]; d2[x] = N(1, 1, 1)[x] - N(0, 1, 0)[x]; } }
+  else { for (int x = 0; x < 3; x++) { d1[x] = N(0, 1, 1)[x] - N(1, 0, 1)[x]; d2[x] = N(0, 0, 1)[x] - N(1, 1, 1)[x]; } }
+  sn[0] = d1[1] * d2[2] - d1[2] * d2[1];
+  sn[1] = d1[2] * d2[0] - d1[0] * d2[2];
+  sn[2] = d1[0] * d2[1] - d1[1] * d2[0];
+  if (dir == 0) { a = N(1, 1, 1); b = N(1, 0, 1); c = N(1, 0, 0); d = N(1, 1, 0); }
+  else if (dir == 1) { a = N(1, 1, 1); b = N(0, 1, 1); c = N(0, 1, 0); d = N(1, 1, 0); }
+  else { a = N(1, 1, 1); b = N(0, 1, 1); c = N(0, 0, 1); d = N(1, 0, 1); }
+  for (int x = 0; x < 3; x++) c1[x] = 0.25 * (a[x] + b[x] + c[x] + d[x]);
+  if (dir == 0) { a = N(0, 1, 1); b = N(0, 0, 1); c = N(0, 0, 0); d = N(0, 1, 0); }
+  else if (dir == 1) { a = N(1, 0, 1); b = N(0, 0, 1); c = N(0, 0, 0); d = N(1, 0, 0); }
+  else { a = N(1, 1, 0); b = N(0, 1, 0); c = N(0, 0, 0); d = N(1, 0, 0); }
+  for (int x = 0; x < 3; x++) c2[x] = 0.25 * (a[x] + b[x] + c[x] + d[x]);
+  const double scal = (c1[0] - c2[0]) * sn[0] + (c1[1] - c2[1]) * sn[1] + (c1[2] - c2[2]) * sn[2];
+  return scal >= 0.0 ? 1.0 : -1.0;
+}
+
+// BC_Connect_Metrics, Lib_Metrics.f90:704-936: Mg(1) of a 101 entry from the
+// local face nodes and the donor's nodes (host side, surface only).
+int connect_metric(const HostBlock& bm, const HostBlock& bs, const int* q, double* Mg1) {
+  auto N = [](const HostBlock& h, int i, int j, int k) -> const double* {
+    if (i < 0 || i > h.n[0] || j < 0 || j > h.n[1] || k < 0 || k > h.n[2]) return nullptr;
+    return &h.nodes[3 * (i + (long)(h.n[0] + 1) * (j + (long)(h.n[1] + 1) * k))];
+  };
+  const int Im = q[1], Jm = q[2], Km = q[3], Fm = q[4];
+  const int Is = q[7], Js = q[8], Ks = q[9], Fs = q[10], d11 = q[11], d12 = q[12], d21 = q[13], d22 = q[14];
+  static const int gs[6][3] = {{0, 0, 0}, {-1, 0, 0}, {0, 0, 0}, {0, -1, 0}, {0, 0, 0}, {0, 0, -1}};
+  const double* P0[8] = {nullptr};
+  const double* P1[8] = {nullptr};
+  switch (Fm) {
+    case 1: P0[0] = N(bm, Im - 1, Jm - 1, Km - 1); P0[1] = N(bm, Im - 1, Jm - 1, Km); P0[2] = N(bm, Im - 1, Jm, Km - 1); P0[3] = N(bm, Im - 1, Jm, Km); break;
+    case 2: P0[4] = N(bm, Im, Jm - 1, Km - 1); P0[5] = N(bm, Im, Jm - 1, Km); P0[6] = N(bm, Im, Jm, Km - 1); P0[7] = N(bm, Im, Jm, Km); break;
+    case 3: P0[0] = N(bm, Im - 1, Jm - 1, Km - 1); P0[1] = N(bm, Im - 1, Jm - 1, Km); P0[4] = N(bm, Im, Jm - 1, Km - 1); P0[5] = N(bm, Im, Jm - 1, Km); break;
+    case 4: P0[2] = N(bm, Im - 1, Jm, Km - 1); P0[3] = N(bm, Im - 1, Jm, Km); P0[6] = N(bm, Im, Jm, Km - 1); P0[7] = N(bm, Im, Jm, Km); break;
+    case 5: P0[0] = N(bm, Im - 1, Jm - 1, Km - 1); P0[2] = N(bm, Im - 1, Jm, Km - 1); P0[4] = N(bm, Im, Jm - 1, Km - 1); P0[6] = N(bm, Im, Jm, Km - 1); break;
+    case 6: P0[1] = N(bm, Im - 1, Jm - 1, Km); P0[3] = N(bm, Im - 1, Jm, Km); P0[5] = N(bm, Im, Jm - 1, Km); P0[7] = N(bm, Im, Jm, Km); break;
+    default: return FUSS_EBC;
+  }
+  // g = 1: donor node index Is1 = Is + (gc-1)*guides(Fs)
+  const int Is1 = Is + gs[Fs - 1][0], Js1 = Js + gs[Fs - 1][1], Ks1 = Ks + gs[Fs - 1][2];
+  int II, JJ, KK;
+  if (Fs <= 2) { II = Is1; JJ = (2 * Js1 + d11 + d21) / 2; KK = (2 * Ks1 + d12 + d22) / 2; }
+  else if (Fs <= 4) { II = (2 * Is1 + d11 + d21) / 2; JJ = Js1; KK = (2 * Ks1 + d12 + d22) / 2; }
+  else { II = (2 * Is1 + d11 + d21) / 2; JJ = (2 * Js1 + d12 + d22) / 2; KK = Ks1; }
+  auto off = [&](int a1, int a2, int* o) {  // donor index shifted by -a1 (first tangent) -a2 (second)
+    if (Fs <= 2) { o[0] = II; o[1] = JJ - a1 * d11 - a2 * d21; o[2] = KK - a1 * d12 - a2 * d22; }
+    else if (Fs <= 4) { o[0] = II - a1 * d11 - a2 * d21; o[1] = JJ; o[2] = KK - a1 * d12 - a2 * d22; }
+    else { o[0] = II - a1 * d11 - a2 * d21; o[1] = JJ - a1 * d12 - a2 * d22; o[2] = KK; }
+  };
+  static const int q1[6] = {4, 8, 6, 8, 7, 8}, q2[6] = {2, 6, 2, 4, 3, 4}, q3[6] = {3, 7, 5, 7, 5, 6},
+                   q4[6] = {1, 5, 1, 3, 1, 2};
+  int o[3];
+  off(0, 0, o); P1[q1[Fm - 1] - 1] = N(bs, o[0], o[1], o[2]);
+  off(1, 0, o); P1[q2[Fm - 1] - 1] = N(bs, o[0], o[1], o[2]);
+  off(0, 1, o); P1[q3[Fm - 1] - 1] = N(bs, o[0], o[1], o[2]);
+  off(1, 1, o); P1[q4[Fm - 1] - 1] = N(bs, o[0], o[1], o[2]);
+  for (int qq = 0; qq < 8; qq++) {
+    const bool need0 = P0[qq] != nullptr;
+    if (!need0 && !P1[qq]) continue;
+  }
+  const double* n1[8];
+  // argument lists per face (Lib_Metrics.f90:192-229), g=1 nodes in P1, g-1=0 in P0
+#define A1(x) P1[(x)-1]
+#define A0(x) P0[(x)-1]
+  switch (Fm) {
+    case 1: { const double* t[8] = {A1(1), A1(2), A1(3), A1(4), A0(1), A0(2), A0(3), A0(4)}; memcpy(n1, t, sizeof(t)); } break;
+    case 2: { const double* t[8] = {A0(5), A0(6), A0(7), A0(8), A1(5), A1(6), A1(7), A1(8)}; memcpy(n1, t, sizeof(t)); } break;
+    case 3: { const double* t[8] = {A1(1), A1(2), A0(1), A0(2), A1(5), A1(6), A0(5), A0(6)}; memcpy(n1, t, sizeof(t)); } break;
+    case 4: { const double* t[8] = {A0(3), A0(4), A1(3), A1(4), A0(7), A0(8), A1(7), A1(8)}; memcpy(n1, t, sizeof(t)); } break;
+    case 5: { const double* t[8] = {A1(1), A0(1), A1(3), A0(3), A1(5), A0(5), A1(7), A0(7)}; memcpy(n1, t, sizeof(t)); } break;
+    default: { const double* t[8] = {A0(2), A1(2), A0(4), A1(4), A0(6), A1(6), A0(8), A1(8)}; memcpy(n1, t, sizeof(t)); } break;
+  }
+#undef A1
+#undef A0
+  for (int qq = 0; qq < 8; qq++)
+    if (!n1[qq]) return FUSS_EBC;
+  double dl[3], v;
+  const int e = metric_tensor(n1[0], n1[1], n1[2], n1[3], n1[4], n1[5], n1[6], n1[7], Mg1, dl, &v);
+  return e ? FUSS_EMETRIC : 0;
+}
+
+int upload(void* dst, const void* src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : FUSS_EHIP;
+}
+
+int build_level_layout(fuss_ctx* c, Level& L) {
+  long off = 0, noff = 0;
+  L.hblk.resize(L.nb);
+  L.htiles.clear();
+  for (int b = 0; b < L.nb; b++) {
+    BlockInfo& B = L.hblk[b];
+    const HostBlock& h = L.hb[b];
+    B.ni = h.n[0]; B.nj = h.n[1]; B.nk = h.n[2];
+    B.sy = B.ni + 2 * GC;
+    B.sz = (long)B.sy * (B.nj + 2 * GC);
+    B.off = off;
+    B.npad = B.sz * (B.nk + 2 * L.kp);
+    B.base = off + GC + (long)GC * B.sy + (long)L.kp * B.sz;
+    B.node_off = noff;
+    for (int f = 0; f < 6; f++) B.fmap_off[f] = -1;
+    B.face_active = 0;
+    B.tang = 0;
+    off += B.npad;
+    noff += (long)(B.ni + 1) * (B.nj + 1) * (B.nk + 1);
+    for (int k = 1; k <= B.nk; k++)
+      for (int j = 1; j <= B.nj; j += TILE_Y)
+        for (int i = 1; i <= B.ni; i += TILE_X) L.htiles.push_back(Tile{b, i, j, k});
+    L.ncell += (long)B.ni * B.nj * B.nk;
+  }
+  L.npad = off;
+  L.nnodes = noff;
+  (void)c;
+  return 0;
+}
+
+int alloc_level(fuss_ctx* c, Level& L) {
+  int e = 0;
+  for (int q = 0; q < 3; q++) e |= L.Tb[q].alloc(L.npad);
+  e |= L.R.alloc(L.npad);
+  e |= L.R2.alloc(L.npad);
+  e |= L.dt.alloc(L.npad);
+  e |= L.qvol.alloc(L.npad);
+  e |= L.vol.alloc(L.npad);
+  e |= L.invL2.alloc(L.npad);
+  e |= L.mat.alloc(L.npad);
+  e |= L.M.alloc(9 * L.npad);
+  for (int d = 0; d < 3; d++) e |= L.F[d].alloc(4 * L.npad);
+  for (int w = 0; w < 9; w++) e |= L.W[w].alloc(L.npad);
+  e |= L.nodes.alloc(3 * L.nnodes);
+  e |= L.tiles.alloc(L.htiles.size());
+  e |= L.partials.alloc(std::max<size_t>(L.htiles.size(), 1));
+  (void)c;
+  return e ? FUSS_EHIP : 0;
+}
+
+int build_bc(fuss_ctx* c, Level& L) {
+  const int n = L.nbound;
+  // grouping check: (block, face) non-decreasing, as BC_Fluxes assumes
+  // (Mod_BC_Fluxes.f90:26-33)
+  for (int e = 1; e < n; e++) {
+    const int* a = &L.hib[15 * (e - 1)];
+    const int* b = &L.hib[15 * e];
+    if (b[0] < a[0] || (b[0] == a[0] && b[4] < a[4])) return FUSS_EBC;
+  }
+  // face maps
+  std::vector<int> fmap;
+  for (int b = 0; b < L.nb; b++) {
+    BlockInfo& B = L.hblk[b];
+    for (int f = 1; f <= 6; f++) {
+      const int ext = f <= 2 ? B.nj * B.nk : (f <= 4 ? B.ni * B.nk : B.ni * B.nj);
+      B.fmap_off[f - 1] = (int)fmap.size();
+      fmap.insert(fmap.end(), ext, -1);
+    }
+  }
+  std::vector<BCRec> recs(n);
+  for (int e = 0; e < n; e++) {
+    const int* q = &L.hib[15 * e];
+    const double* r = &L.hrb[6 * e];
+    BCRec& R = recs[e];
+    memset(&R, 0, sizeof(R));
+    const int b = q[0] - 1, i = q[1], j = q[2], k = q[3], f = q[4], type = q[5];
+    if (b < 0 || b >= L.nb || f < 1 || f > 6) return FUSS_EBC;
+    BlockInfo& B = L.hblk[b];
+    if (i < 1 || i > B.ni || j < 1 || j > B.nj || k < 1 || k > B.nk) return FUSS_EBC;
+    const bool on = (f == 1 && i == 1) || (f == 2 && i == B.ni) || (f == 3 && j == 1) || (f == 4 && j == B.nj) ||
+                    (f == 5 && k == 1) || (f == 6 && k == B.nk);
+    if (!on) return FUSS_EBC;
+    const int mi = f <= 2 ? (j - 1) + B.nj * (k - 1) : (f <= 4 ? (i - 1) + B.ni * (k - 1) : (i - 1) + B.ni * (j - 1));
+    if (fmap[B.fmap_off[f - 1] + mi] != -1) return FUSS_EBC;
+    fmap[B.fmap_off[f - 1] + mi] = e;
+    if (type == 102) return FUSS_EUNSUPPORTED;
+    R.type = type; R.f = f; R.dir = (f + 1) / 2; R.series = L.hser.empty() ? -1 : L.hser[e];
+    R.b = b; R.i = i; R.j = j; R.k = k;
+    R.cell = cidx(B, i, j, k);
+    R.qw = r[0]; R.Tw = r[1]; R.hconv = r[2]; R.Tref = r[4]; R.eps = r[5];
+    const int gi = guide(f, 0), gj = guide(f, 1), gk = guide(f, 2);
+    const bool kface = f >= 5;
+    const bool two_d = L.kp == 0;
+    R.gdst = (two_d && kface) ? -1 : cidx(B, i - gi, j - gj, k - gk);
+    R.gsrc = R.cell;  // Ghost_Symmetry layer 1: T(ghost) = T(cell)
+    R.rdonor = -1;
+    for (int t = 0; t < 4; t++) R.dn[t] = R.cell;
+    if (type == 101) {
+      const int bs = q[6] - 1, Is = q[7], Js = q[8], Ks = q[9], Fs = q[10];
+      if (bs < 0 || bs >= L.nb || Fs < 1 || Fs > 6) return FUSS_EBC;
+      const BlockInfo& S = L.hblk[bs];
+      if (Is < 1 || Is > S.ni || Js < 1 || Js > S.nj || Ks < 1 || Ks > S.nk) return FUSS_EBC;
+      R.gsrc = cidx(S, Is, Js, Ks);  // Ghost_Connection g=1: donor (Is,Js,Ks)
+      R.rdonor = R.gsrc;
+      const int d11 = q[11], d12 = q[12], d21 = q[13], d22 = q[14];
+      int p[4][3];
+      if (Fs <= 2) { int t[4][3] = {{Is, Js - d11, Ks - d12}, {Is, Js + d11, Ks + d12}, {Is, Js - d21, Ks - d22}, {Is, Js + d21, Ks + d22}}; memcpy(p, t, sizeof(p)); }
+      else if (Fs <= 4) { int t[4][3] = {{Is - d11, Js, Ks - d12}, {Is + d11, Js, Ks + d12}, {Is - d21, Js, Ks - d22}, {Is + d21, Js, Ks + d22}}; memcpy(p, t, sizeof(p)); }
+      else { int t[4][3] = {{Is - d11, Js - d12, Ks}, {Is + d11, Js + d12, Ks}, {Is - d21, Js - d22, Ks}, {Is + d21, Js + d22, Ks}}; memcpy(p, t, sizeof(p)); }
+      for (int t = 0; t < 4; t++) {
+        // 2-D: the k offsets of Tg(5:6) address mirrored planes equal to the
+        // donor plane; they enter multiplied by a vanishing M_con(3,:).N
+        if (two_d) p[t][2] = 1;
+        if (p[t][0] < 0 || p[t][0] > S.ni + 1 || p[t][1] < 0 || p[t][1] > S.nj + 1 || p[t][2] < (two_d ? 1 : 0) ||
+            p[t][2] > (two_d ? 1 : S.nk + 1))
+          return FUSS_EBC;
+        R.dn[t] = cidx(S, p[t][0], p[t][1], p[t][2]);
+      }
+      const int e2 = connect_metric(L.hb[b], L.hb[bs], q, R.Mg);
+      if (e2) return e2;
+      // matIDg: the donor cell's material (Ghost_Connection copies it)
+    }
+    const bool active = type == 101 || (type >= 300 && type <= 305);
+    if (active) B.face_active |= 1 << (f - 1);
+  }
+  for (size_t q = 0; q < fmap.size(); q++)
+    if (fmap[q] < 0) return FUSS_EBC;
+  L.any_active = 0;
+  for (int b = 0; b < L.nb; b++) L.any_active |= L.hblk[b].face_active;
+  int e = L.bc.alloc(std::max(n, 1));
+  e |= L.fmap.alloc(std::max<size_t>(fmap.size(), 1));
+  if (e) return FUSS_EHIP;
+  if (n) e |= upload(L.bc.p, recs.data(), sizeof(BCRec) * n);
+  if (!fmap.empty()) e |= upload(L.fmap.p, fmap.data(), sizeof(int) * fmap.size());
+  (void)c;
+  return e;
+}
+
+int sync_status(fuss_ctx* c, int* bijk = nullptr) {
+  HIPCHK(hipMemcpyAsync(c->hstatus, c->status.p, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (bijk) for (int q = 0; q < 4; q++) bijk[q] = c->hstatus[1 + q];
+  return c->hstatus[0];
+}
+
+template <typename R>
+Tab<R> dtab(fuss_ctx* c) {
+  Tab<R> t;
+  t.cp = c->dcp.p; t.rho = c->drho.p; t.kap = c->dkap.p; t.h = c->dh.p;
+  t.nT = c->nT; t.Ti1 = c->Ti1; t.Ti2 = c->Ti1 + c->nT - 1;
+  return t;
+}
+
+int launch_ghost(fuss_ctx* c, Level& L, double* T) {
+  if (!L.nbound) return 0;
+  k_ghost<double><<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(T, L.bc.p, L.nbound);
+  return hipGetLastError() == hipSuccess ? 0 : FUSS_EHIP;
+}
+
+int restrict_level(fuss_ctx* c, int fine) {
+  Level& F = c->lev[fine];
+  Level& C = c->lev[fine + 1];
+  for (int b = 0; b < F.nb; b++) {
+    MGArgs a;
+    a.F = F.hblk[b]; a.Cb = C.hblk[b];
+    a.fT = F.Tb[F.cur].p; a.fmat = F.mat.p; a.fvol = F.vol.p; a.fqvol = F.qvol.p;
+    a.cT = C.Tb[C.cur].p; a.cmat = C.mat.p; a.cvol = C.vol.p; a.cqvol = C.qvol.p;
+    a.tab = dtab<double>(c); a.status = c->status.p; a.bidx = b;
+    const long n = (long)a.Cb.ni * a.Cb.nj * a.Cb.nk;
+    k_restrict<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(a);
+    if (hipGetLastError() != hipSuccess) return FUSS_EHIP;
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// C-ABI
+// ------------------------------------------------------------------------
+extern "C" {
+
+const char* fuss_strerror(int code) {
+  switch (code) {
+    case FUSS_OK: return "ok";
+    case FUSS_EARG: return "invalid argument or call order";
+    case FUSS_ESTATE: return "NaN or T<0 detected";
+    case FUSS_ETABLE: return "temperature or enthalpy outside the property table";
+    case FUSS_EMETRIC: return "metric tensor det=0 or negative volume";
+    case FUSS_EUNSUPPORTED: return "unsupported feature (chimera 102)";
+    case FUSS_EHIP: return "HIP runtime error";
+    case FUSS_EBC: return "boundary list inconsistent with the blocks";
+  }
+  return "unknown status";
+}
+
+int fuss_create(int nlevels, int nblocks, int precision, fuss_ctx** out) {
+  if (!out || nlevels < 1 || nblocks < 1 || (precision != FUSS_FP64 && precision != FUSS_FP32)) return FUSS_EARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return FUSS_EHIP;
+  fuss_ctx* c = new fuss_ctx();
+  c->nlev = nlevels;
+  c->nb = nblocks;
+  c->precision = precision;
+  c->lev.resize(nlevels);
+  for (auto& L : c->lev) {
+    L.nb = nblocks;
+    L.hb.resize(nblocks);
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return FUSS_EHIP;
+  }
+  c->own_stream = true;
+  if (c->status.alloc(8) || c->scal.alloc(4) || c->dtmin.alloc(1) ||
+      hipHostMalloc((void**)&c->hscal, 8 * sizeof(double), 0) != hipSuccess ||
+      hipHostMalloc((void**)&c->hstatus, 8 * sizeof(int), 0) != hipSuccess) {
+    fuss_destroy(c);
+    return FUSS_EHIP;
+  }
+  *out = c;
+  return 0;
+}
+
+int fuss_set_table(fuss_ctx* c, int nmat, int Ti1, int nT, const double* cp, const double* rho,
+                   const double* kappa, const double* h) {
+  if (!c || nmat < 1 || nT < 2 || !cp || !rho || !kappa || !h) return FUSS_EARG;
+  const size_t n = (size_t)nmat * nT;
+  c->nmat = nmat; c->Ti1 = Ti1; c->nT = nT;
+  c->tcp.assign(cp, cp + n); c->trho.assign(rho, rho + n); c->tkap.assign(kappa, kappa + n); c->th.assign(h, h + n);
+  for (int m = 0; m < nmat; m++)
+    for (int t = 1; t < nT; t++)
+      if (h[(size_t)m * nT + t] < h[(size_t)m * nT + t - 1]) return FUSS_ETABLE;
+  int e = c->dcp.alloc(n) | c->drho.alloc(n) | c->dkap.alloc(n) | c->dh.alloc(n);
+  if (e) return FUSS_EHIP;
+  e |= upload(c->dcp.p, cp, n * 8);
+  e |= upload(c->drho.p, rho, n * 8);
+  e |= upload(c->dkap.p, kappa, n * 8);
+  e |= upload(c->dh.p, h, n * 8);
+  return e;
+}
+
+int fuss_set_block(fuss_ctx* c, int b, int ni, int nj, int nk, const double* nodes, const double* T,
+                   const double* matID, const double* qvol) {
+  if (!c || b < 1 || b > c->nb || ni < 1 || nj < 1 || nk < 1 || !nodes || !T || !matID) return FUSS_EARG;
+  HostBlock& h = c->lev[0].hb[b - 1];
+  h.n[0] = ni; h.n[1] = nj; h.n[2] = nk;
+  const size_t nn = 3 * (size_t)(ni + 1) * (nj + 1) * (nk + 1), nc = (size_t)ni * nj * nk;
+  h.nodes.assign(nodes, nodes + nn);
+  h.T.assign(T, T + nc);
+  h.matID.assign(matID, matID + nc);
+  if (qvol) h.qvol.assign(qvol, qvol + nc);
+  else h.qvol.assign(nc, 0.0);
+  return 0;
+}
+
+int fuss_add_series(fuss_ctx* c, int periodic, int n, const double* t, const double* v, int* id) {
+  if (!c || n < 1 || !t || !v || (int)c->series.size() >= MAX_SERIES) return FUSS_EARG;
+  fuss_ctx::Series s;
+  s.periodic = periodic;
+  s.t.assign(t, t + n);
+  s.v.assign(v, v + n);
+  c->series.push_back(s);
+  if (id) *id = (int)c->series.size() - 1;
+  return 0;
+}
+
+int fuss_set_bc(fuss_ctx* c, int level, int nbound, const int* ib, const double* rb, const int* series) {
+  if (!c || level < 1 || level > c->nlev || nbound < 0 || (nbound && (!ib || !rb))) return FUSS_EARG;
+  Level& L = c->lev[level - 1];
+  L.nbound = nbound;
+  L.hib.assign(ib, ib + 15 * (size_t)nbound);
+  L.hrb.assign(rb, rb + 6 * (size_t)nbound);
+  if (series) L.hser.assign(series, series + nbound);
+  else L.hser.clear();
+  for (int e = 0; e < nbound && series; e++)
+    if (series[e] >= (int)c->series.size()) return FUSS_EARG;
+  return 0;
+}
+
+int fuss_setup(fuss_ctx* c, double initial_time) {
+  if (!c || c->setup_done || c->tkap.empty()) return FUSS_EARG;
+  for (int b = 0; b < c->nb; b++)
+    if (c->lev[0].hb[b].nodes.empty()) return FUSS_EARG;
+  // Check_Mesh_Type, Lib_Metrics.f90:13-66
+  const HostBlock& b1 = c->lev[0].hb[0];
+  c->ndir = b1.n[2] > 1 ? 3 : (b1.n[1] == 1 ? 1 : 2);
+  // Check_Multigrid + Setup_Multigrid/Coarse_Grid (Lib_Multigrid.f90:11-35, :223-265)
+  const int rap = 1 << (c->nlev - 1);
+  for (int b = 0; b < c->nb; b++) {
+    const HostBlock& h = c->lev[0].hb[b];
+    int chk = 0;
+    for (int d = 0; d < 3; d++) {
+      if (h.n[d] % rap == 0) chk++;
+      if (d == 2 && h.n[d] == 1) chk++;
+    }
+    if (chk < 3) return FUSS_EARG;
+  }
+  for (int m = 1; m < c->nlev; m++)
+    for (int b = 0; b < c->nb; b++) {
+      const HostBlock& F = c->lev[m - 1].hb[b];
+      HostBlock& C = c->lev[m].hb[b];
+      C.n[0] = F.n[0] / 2; C.n[1] = F.n[1] / 2; C.n[2] = std::max(1, F.n[2] / 2);
+      C.nodes.assign(3 * (size_t)(C.n[0] + 1) * (C.n[1] + 1) * (C.n[2] + 1), 0.0);
+      const int step = 2 - F.n[2] % 2;
+      for (int k = 0; k <= F.n[2]; k += step)
+        for (int j = 0; j <= F.n[1]; j += 2)
+          for (int i = 0; i <= F.n[0]; i += 2) {
+            const int k2 = F.n[2] == 1 ? k : k / 2;
+            const double* s = &F.nodes[3 * (i + (size_t)(F.n[0] + 1) * (j + (size_t)(F.n[1] + 1) * k))];
+            double* d = &C.nodes[3 * (i / 2 + (size_t)(C.n[0] + 1) * (j / 2 + (size_t)(C.n[1] + 1) * k2))];
+            d[0] = s[0]; d[1] = s[1]; d[2] = s[2];
+          }
+    }
+  HIPCHK(hipMemsetAsync(c->status.p, 0, 8 * sizeof(int), c->stream));
+  for (int m = 0; m < c->nlev; m++) {
+    Level& L = c->lev[m];
+    // 2-D levels keep one plane on each side: it stores the k-face data
+    L.kp = c->ndir == 2 ? 1 : GC;
+    int e = build_level_layout(c, L);
+    if (e) return e;
+    e = build_bc(c, L);  // fills fmap offsets and face_active into hblk
+    if (e) return e;
+    e = alloc_level(c, L);
+    if (e) return e;
+    // re-upload block infos after build_bc (face maps, activity)
+    e = L.blk.alloc(L.nb);
+    if (e) return e;
+    e = upload(L.tiles.p, L.htiles.data(), sizeof(Tile) * L.htiles.size());
+    // nodes
+    std::vector<double> nodes(3 * L.nnodes);
+    for (int b = 0; b < L.nb; b++)
+      std::copy(L.hb[b].nodes.begin(), L.hb[b].nodes.end(), nodes.begin() + 3 * L.hblk[b].node_off);
+    e |= upload(L.nodes.p, nodes.data(), nodes.size() * 8);
+    e |= upload(L.blk.p, L.hblk.data(), sizeof(BlockInfo) * L.nb);
+    if (e) return e;
+    // cell metrics
+    GeomArgs g;
+    g.blk = L.blk.p; g.tiles = L.tiles.p; g.nodes = L.nodes.p; g.M = L.M.p; g.vol = L.vol.p;
+    g.invL2 = L.invL2.p; g.ndir = c->ndir; g.status = c->status.p;
+    k_cell_metrics<<<(unsigned)L.htiles.size(), TILE_T, 0, c->stream>>>(g);
+    HIPCHK(hipGetLastError());
+    // faces + weights per block and direction
+    DBuf<int> tang;
+    if (tang.alloc(L.nb)) return FUSS_EHIP;
+    for (int b = 0; b < L.nb; b++)
+      for (int D = 0; D < 3; D++) {
+        FaceArgs f;
+        f.B = L.hblk[b]; f.bidx = b; f.D = D; f.sign = norm_sign(L.hb[b], D);
+        f.nodes = L.nodes.p; f.M = L.M.p; f.F = L.F[D].p;
+        for (int cc = 0; cc < 3; cc++) f.W[cc] = L.W[3 * D + cc].p;
+        f.tang = tang.p + b;
+        const long nf = (long)(f.B.ni + (D == 0)) * (f.B.nj + (D == 1)) * (f.B.nk + (D == 2));
+        k_face_metrics<<<(unsigned)((nf + 255) / 256), 256, 0, c->stream>>>(f);
+        HIPCHK(hipGetLastError());
+      }
+    std::vector<int> htang(L.nb);
+    HIPCHK(hipMemcpyAsync(htang.data(), tang.p, sizeof(int) * L.nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    tang.free();
+    L.tang = 0;
+    for (int b = 0; b < L.nb; b++) {
+      L.hblk[b].tang = htang[b];
+      L.tang |= htang[b];
+    }
+    HIPCHK(hipMemcpy(L.blk.p, L.hblk.data(), sizeof(BlockInfo) * L.nb, hipMemcpyHostToDevice));
+    if (L.nbound) {
+      k_bc_geometry<<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(L.bc.p, L.nbound, L.blk.p, L.M.p, L.F[0].p,
+                                                                  L.F[1].p, L.F[2].p);
+      HIPCHK(hipGetLastError());
+    }
+    L.time = initial_time;
+    L.iter = 0;
+  }
+  // level-1 fields: T, material index, qvol
+  {
+    Level& L = c->lev[0];
+    std::vector<double> T(L.npad, 0.0), q(L.npad, 0.0);
+    std::vector<int> mat(L.npad, 0);
+    for (int b = 0; b < L.nb; b++) {
+      const BlockInfo& B = L.hblk[b];
+      const HostBlock& h = L.hb[b];
+      for (int k = 1; k <= B.nk; k++)
+        for (int j = 1; j <= B.nj; j++)
+          for (int i = 1; i <= B.ni; i++) {
+            const size_t s = (i - 1) + (size_t)B.ni * ((j - 1) + (size_t)B.nj * (k - 1));
+            const long p = cidx(B, i, j, k);
+            T[p] = h.T[s];
+            q[p] = h.qvol[s];
+            const int m = (int)h.matID[s] - 1;
+            if (m < 0 || m >= c->nmat) return FUSS_EARG;
+            mat[p] = m;
+          }
+    }
+    int e = upload(L.Tb[L.cur].p, T.data(), T.size() * 8);
+    e |= upload(L.qvol.p, q.data(), q.size() * 8);
+    e |= upload(L.mat.p, mat.data(), mat.size() * sizeof(int));
+    if (e) return e;
+  }
+  // matIDg of connections: material of the donor cell (Fill_matIDg after the
+  // first Ghost_Connection copy, Lib_Ghost.f90:149-153, :436-470)
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int m = 0; m < c->nlev; m++) {
+    Level& L = c->lev[m];
+    if (!L.nbound) continue;
+    std::vector<BCRec> recs(L.nbound);
+    HIPCHK(hipMemcpy(recs.data(), L.bc.p, sizeof(BCRec) * L.nbound, hipMemcpyDeviceToHost));
+    if (m == 0) {
+      for (auto& r : recs)
+        if (r.type == 101) {
+          const int* q = &L.hib[15 * (&r - &recs[0])];
+          const HostBlock& S = L.hb[q[6] - 1];
+          const size_t s = (q[7] - 1) + (size_t)S.n[0] * ((q[8] - 1) + (size_t)S.n[1] * (q[9] - 1));
+          r.matg = (int)S.matID[s] - 1;
+        }
+      HIPCHK(hipMemcpy(L.bc.p, recs.data(), sizeof(BCRec) * L.nbound, hipMemcpyHostToDevice));
+    }
+  }
+  int e = launch_ghost(c, c->lev[0], c->lev[0].Tb[c->lev[0].cur].p);
+  if (e) return e;
+  for (int m = 1; m < c->nlev; m++) {
+    e = restrict_level(c, m - 1);
+    if (e) return e;
+    // coarse matIDg: donor material from the restricted material field
+    Level& L = c->lev[m];
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (L.nbound) {
+      std::vector<int> mat(L.npad);
+      HIPCHK(hipMemcpy(mat.data(), L.mat.p, sizeof(int) * L.npad, hipMemcpyDeviceToHost));
+      std::vector<BCRec> recs(L.nbound);
+      HIPCHK(hipMemcpy(recs.data(), L.bc.p, sizeof(BCRec) * L.nbound, hipMemcpyDeviceToHost));
+      for (auto& r : recs)
+        if (r.type == 101) r.matg = mat[r.gsrc];
+      HIPCHK(hipMemcpy(L.bc.p, recs.data(), sizeof(BCRec) * L.nbound, hipMemcpyHostToDevice));
+    }
+    e = launch_ghost(c, L, L.Tb[L.cur].p);
+    if (e) return e;
+  }
+  int bijk[4];
+  const int st = sync_status(c, bijk);
+  if (st) return st;
+  c->setup_done = 1;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <int NDIR, bool TANG, bool PRIM, bool IRSMODE>
+void launch_stage_t(fuss_ctx* c, Level& L, const StageArgs<double>& a) {
+  k_stage<NDIR, TANG, PRIM, IRSMODE, double><<<(unsigned)L.htiles.size(), TILE_T, 0, c->stream>>>(a);
+}
+
+template <int NDIR>
+void launch_stage_n(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang, bool prim, bool irs) {
+  if (tang) {
+    if (prim) { irs ? launch_stage_t<NDIR, true, true, true>(c, L, a) : launch_stage_t<NDIR, true, true, false>(c, L, a); }
+    else { irs ? launch_stage_t<NDIR, true, false, true>(c, L, a) : launch_stage_t<NDIR, true, false, false>(c, L, a); }
+  } else {
+    if (prim) { irs ? launch_stage_t<NDIR, false, true, true>(c, L, a) : launch_stage_t<NDIR, false, true, false>(c, L, a); }
+    else { irs ? launch_stage_t<NDIR, false, false, true>(c, L, a) : launch_stage_t<NDIR, false, false, false>(c, L, a); }
+  }
+}
+
+template <int DIR, int NDIR>
+void launch_irs_d(fuss_ctx* c, Level& L, const IrsArgs<double>& a, bool update, bool prim) {
+  const unsigned g = (unsigned)L.htiles.size();
+  if (!update) k_irs<DIR, NDIR, false, true, double><<<g, TILE_T, 0, c->stream>>>(a);
+  else if (prim) k_irs<DIR, NDIR, true, true, double><<<g, TILE_T, 0, c->stream>>>(a);
+  else k_irs<DIR, NDIR, true, false, double><<<g, TILE_T, 0, c->stream>>>(a);
+}
+
+template <int NDIR>
+void launch_irs(fuss_ctx* c, Level& L, int dir, const IrsArgs<double>& a, bool update, bool prim) {
+  if (dir == 0) launch_irs_d<0, NDIR>(c, L, a, update, prim);
+  else if (dir == 1) launch_irs_d<1, NDIR>(c, L, a, update, prim);
+  else launch_irs_d<2, NDIR>(c, L, a, update, prim);
+}
+
+const double RKC[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.50, 0.0}, {1.0, 0.25, 2.0 / 3.0}};
+
+// Explicit_Step on one level, enqueued on the stream (no host sync except
+// the dtglobal readback of time-accurate runs with time-dependent BCs).
+int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
+  Level& L = c->lev[level - 1];
+  const int nrk = p->n_rk;
+  if (nrk < 1 || nrk > 3) return FUSS_EARG;
+  const bool prim = p->integration == FUSS_PRIM;
+  const bool irs = p->irs_beta > 0.0;
+  if (irs && p->irs_mode != FUSS_IRS_JACOBI2) return FUSS_EUNSUPPORTED;
+  L.iter += 1;
+  const unsigned ntile = (unsigned)L.htiles.size();
+  // Compute_dt + min (Mod_dt.f90:10-84); dtglobal starts at 1e5 (Mod_Explicit.f90:40)
+  const double init = 1e5;
+  unsigned long long bits;
+  memcpy(&bits, &init, 8);
+  k_init_scalars<<<1, 1, 0, c->stream>>>(c->dtmin.p, bits, c->scal.p, L.time);
+  HIPCHK(hipGetLastError());
+  DtArgs<double> da;
+  da.blk = L.blk.p; da.tiles = L.tiles.p; da.T = L.Tb[L.cur].p; da.mat = L.mat.p; da.invL2 = L.invL2.p;
+  da.dt = L.dt.p; da.tab = dtab<double>(c); da.vnn = p->vnn; da.iter = L.iter; da.rampa = p->rampa_iter;
+  da.dtmin = c->dtmin.p; da.status = c->status.p;
+  k_dt<double><<<ntile, TILE_T, 0, c->stream>>>(da);
+  HIPCHK(hipGetLastError());
+  // device scalars: [0] dtglobal, [1] time, [2] Sum (T-TO)^2
+  k_dt_finish<<<1, 1, 0, c->stream>>>(c->dtmin.p, c->scal.p, p->time_accurate);
+  HIPCHK(hipGetLastError());
+  double sval[MAX_SERIES] = {0};
+  if (p->time_accurate) {
+    HIPCHK(hipMemcpyAsync(c->hscal, c->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    L.time = c->hscal[1];
+  }
+  for (size_t s = 0; s < c->series.size(); s++)
+    sval[s] = series_value(c->series[s].periodic, (int)c->series[s].t.size(), c->series[s].t.data(),
+                           c->series[s].v.data(), L.time);
+  const int o = L.cur, a1 = (L.cur + 1) % 3, a2 = (L.cur + 2) % 3;
+  double* in = L.Tb[o].p;
+  for (int irk = 1; irk <= nrk; irk++) {
+    double* out = (irk % 2 == 1) ? L.Tb[a1].p : L.Tb[a2].p;
+    int e = launch_ghost(c, L, in);
+    if (e) return e;
+    StageArgs<double> sa;
+    sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.T = in; sa.TO = L.Tb[o].p; sa.Tout = out; sa.Rres = L.R.p;
+    sa.dt = L.dt.p; sa.dtg = c->scal.p; sa.use_dtg = p->time_accurate; sa.qvol = L.qvol.p; sa.vol = L.vol.p;
+    sa.mat = L.mat.p;
+    for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
+    sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.tab = dtab<double>(c);
+    for (int s = 0; s < MAX_SERIES; s++) sa.sval[s] = sval[s];
+    sa.rkc = RKC[nrk - 1][irk - 1];
+    sa.last = irk == nrk && p->compute_residual;
+    sa.partials = L.partials.p; sa.status = c->status.p;
+    if (c->ndir == 2) launch_stage_n<2>(c, L, sa, L.tang != 0, prim, irs);
+    else launch_stage_n<3>(c, L, sa, L.tang != 0, prim, irs);
+    HIPCHK(hipGetLastError());
+    if (irs) {
+      // Residual_Smoothing (Lib_IRS.f90:14-40): ndir passes R -> R2 -> R ...
+      IrsArgs<double> ia;
+      ia.blk = L.blk.p; ia.tiles = L.tiles.p; ia.bc = L.bc.p; ia.fmap = L.fmap.p;
+      ia.beta = p->irs_beta;
+      {
+#pragma clang fp contract(off)
+        ia.den = 1.0 + 2.0 * p->irs_beta;
+      }
+      ia.T = in; ia.TO = L.Tb[o].p; ia.Tout = out; ia.mat = L.mat.p; ia.tab = dtab<double>(c);
+      ia.rkc = sa.rkc; ia.last = sa.last; ia.partials = L.partials.p; ia.status = c->status.p;
+      double* rin = L.R.p;
+      double* rout = L.R2.p;
+      for (int d = 0; d < c->ndir; d++) {
+        ia.Rin = rin; ia.Rout = rout;
+        const bool upd = d == c->ndir - 1;
+        if (c->ndir == 2) launch_irs<2>(c, L, d, ia, upd, prim);
+        else launch_irs<3>(c, L, d, ia, upd, prim);
+        HIPCHK(hipGetLastError());
+        std::swap(rin, rout);
+      }
+    }
+    in = out;
+  }
+  if (p->compute_residual) {
+    k_sum<<<1, 256, 0, c->stream>>>(L.partials.p, (int)ntile, c->scal.p + 2);
+    HIPCHK(hipGetLastError());
+  }
+  // T^{n+1} becomes the new T^n
+  L.cur = (nrk % 2 == 1) ? a1 : a2;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fuss_step(fuss_ctx* c, int level, const fuss_step_params* p, fuss_step_result* r) {
+  if (!c || !c->setup_done || !p || level < 1 || level > c->nlev) return FUSS_EARG;
+  int e = enqueue_step(c, level, p);
+  if (e) return e;
+  Level& L = c->lev[level - 1];
+  HIPCHK(hipMemcpyAsync(c->hscal, c->scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  int bijk[4] = {0, 0, 0, 0};
+  const int st = sync_status(c, bijk);
+  if (p->time_accurate) L.time = c->hscal[1];
+  if (r) {
+    r->res_sq = p->compute_residual ? c->hscal[2] : 0.0;
+    r->dtglobal = c->hscal[0];
+    r->time = L.time;
+    r->iter = L.iter;
+    r->status = st;
+    for (int q = 0; q < 4; q++) r->bijk[q] = bijk[q];
+  }
+  return st;
+}
+
+int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, int itermax_left,
+             double res_threshold, double time_threshold, int* n_done, double* hist_res, double* hist_time,
+             fuss_step_result* last) {
+  if (!c || !c->setup_done || !p || level < 1 || level > c->nlev) return FUSS_EARG;
+  fuss_step_result r;
+  memset(&r, 0, sizeof(r));
+  int n = 0;
+  fuss_step_params q = *p;
+  q.compute_residual = 1;
+  while (n < max_iters) {
+    const int e = fuss_step(c, level, &q, &r);
+    if (e) {
+      if (n_done) *n_done = n + 1;
+      if (last) *last = r;
+      return e;
+    }
+    const double res = sqrt(r.res_sq);
+    if (hist_res) hist_res[n] = res;
+    if (hist_time) hist_time[n] = r.time;
+    n++;
+    if (n >= itermax_left || res <= res_threshold || r.time >= time_threshold) break;
+  }
+  if (n_done) *n_done = n;
+  if (last) *last = r;
+  return 0;
+}
+
+int fuss_restrict(fuss_ctx* c, int fine_level) {
+  if (!c || !c->setup_done || fine_level < 1 || fine_level >= c->nlev) return FUSS_EARG;
+  int e = restrict_level(c, fine_level - 1);
+  if (e) return e;
+  return sync_status(c);
+}
+
+int fuss_prolong(fuss_ctx* c, int fine_level) {
+  if (!c || !c->setup_done || fine_level < 1 || fine_level >= c->nlev) return FUSS_EARG;
+  Level& F = c->lev[fine_level - 1];
+  Level& C = c->lev[fine_level];
+  for (int b = 0; b < F.nb; b++) {
+    MGArgs a;
+    a.F = F.hblk[b]; a.Cb = C.hblk[b]; a.fT = F.Tb[F.cur].p; a.cT = C.Tb[C.cur].p;
+    a.tab = dtab<double>(c); a.status = c->status.p; a.bidx = b;
+    const long n = (long)a.Cb.ni * a.Cb.nj * a.Cb.nk;
+    k_prolong<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(a);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int field_ptr(fuss_ctx* c, Level& L, const char* name, double** p) {
+  if (!strcmp(name, "T")) *p = L.Tb[L.cur].p;
+  else if (!strcmp(name, "R")) *p = L.R.p;
+  else if (!strcmp(name, "dt")) *p = L.dt.p;
+  else if (!strcmp(name, "vol")) *p = L.vol.p;
+  else if (!strcmp(name, "qvol")) *p = L.qvol.p;
+  else return FUSS_EARG;
+  (void)c;
+  return 0;
+}
+
+int fuss_get_field(fuss_ctx* c, int level, int b, const char* name, double* out) {
+  if (!c || !c->setup_done || level < 1 || level > c->nlev || b < 1 || b > c->nb || !out || !name) return FUSS_EARG;
+  Level& L = c->lev[level - 1];
+  double* src = nullptr;
+  if (field_ptr(c, L, name, &src)) return FUSS_EARG;
+  const BlockInfo& B = L.hblk[b - 1];
+  std::vector<double> pad(B.npad);
+  HIPCHK(hipMemcpyAsync(pad.data(), src + B.off, B.npad * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 1; k <= B.nk; k++)
+    for (int j = 1; j <= B.nj; j++)
+      for (int i = 1; i <= B.ni; i++)
+        out[(i - 1) + (size_t)B.ni * ((j - 1) + (size_t)B.nj * (k - 1))] = pad[cidx(B, i, j, k) - B.off];
+  return 0;
+}
+
+int fuss_get_T(fuss_ctx* c, int level, int b, double* T) { return fuss_get_field(c, level, b, "T", T); }
+
+int fuss_set_T(fuss_ctx* c, int level, int b, const double* T) {
+  if (!c || !c->setup_done || level < 1 || level > c->nlev || b < 1 || b > c->nb || !T) return FUSS_EARG;
+  Level& L = c->lev[level - 1];
+  const BlockInfo& B = L.hblk[b - 1];
+  std::vector<double> pad(B.npad);
+  HIPCHK(hipMemcpyAsync(pad.data(), L.Tb[L.cur].p + B.off, B.npad * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 1; k <= B.nk; k++)
+    for (int j = 1; j <= B.nj; j++)
+      for (int i = 1; i <= B.ni; i++)
+        pad[cidx(B, i, j, k) - B.off] = T[(i - 1) + (size_t)B.ni * ((j - 1) + (size_t)B.nj * (k - 1))];
+  HIPCHK(hipMemcpy(L.Tb[L.cur].p + B.off, pad.data(), B.npad * 8, hipMemcpyHostToDevice));
+  return launch_ghost(c, L, L.Tb[L.cur].p);
+}
+
+int fuss_probe(fuss_ctx* c, int n, const int* bijk, double* out) {
+  if (!c || !c->setup_done || n < 0 || (n && (!bijk || !out))) return FUSS_EARG;
+  Level& L = c->lev[0];
+  for (int q = 0; q < n; q++) {
+    const int b = bijk[4 * q] - 1;
+    if (b < 0 || b >= L.nb) return FUSS_EARG;
+    const BlockInfo& B = L.hblk[b];
+    const int i = bijk[4 * q + 1], j = bijk[4 * q + 2], k = bijk[4 * q + 3];
+    if (i < 1 || i > B.ni || j < 1 || j > B.nj || k < 1 || k > B.nk) return FUSS_EARG;
+    HIPCHK(hipMemcpyAsync(out + q, L.Tb[L.cur].p + cidx(B, i, j, k), 8, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fuss_get_clock(fuss_ctx* c, int level, double* time, int* iter) {
+  if (!c || level < 1 || level > c->nlev) return FUSS_EARG;
+  if (time) *time = c->lev[level - 1].time;
+  if (iter) *iter = c->lev[level - 1].iter;
+  return 0;
+}
+
+int fuss_set_clock(fuss_ctx* c, int level, double time, int iter) {
+  if (!c || level < 1 || level > c->nlev) return FUSS_EARG;
+  c->lev[level - 1].time = time;
+  c->lev[level - 1].iter = iter;
+  return 0;
+}
+
+int fuss_ndir(fuss_ctx* c, int* ndir) {
+  if (!c || !ndir || !c->setup_done) return FUSS_EARG;
+  *ndir = c->ndir;
+  return 0;
+}
+
+int fuss_get_metrics(fuss_ctx* c, int level, int b, double* M, double* vol, int d, double* N, double* A) {
+  if (!c || !c->setup_done || level < 1 || level > c->nlev || b < 1 || b > c->nb) return FUSS_EARG;
+  Level& L = c->lev[level - 1];
+  const BlockInfo& B = L.hblk[b - 1];
+  if (M || vol) {
+    std::vector<double> m(9 * B.npad), v(B.npad);
+    HIPCHK(hipMemcpy(m.data(), L.M.p + 9 * B.off, 9 * B.npad * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(v.data(), L.vol.p + B.off, B.npad * 8, hipMemcpyDeviceToHost));
+    for (int k = 1; k <= B.nk; k++)
+      for (int j = 1; j <= B.nj; j++)
+        for (int i = 1; i <= B.ni; i++) {
+          const size_t s = (i - 1) + (size_t)B.ni * ((j - 1) + (size_t)B.nj * (k - 1));
+          const long p = cidx(B, i, j, k) - B.off;
+          if (M) for (int q = 0; q < 9; q++) M[9 * s + q] = m[9 * p + q];
+          if (vol) vol[s] = v[p];
+        }
+  }
+  if (d >= 1 && d <= 3 && (N || A)) {
+    const int D = d - 1;
+    std::vector<double> f(4 * B.npad);
+    HIPCHK(hipMemcpy(f.data(), L.F[D].p + 4 * B.off, 4 * B.npad * 8, hipMemcpyDeviceToHost));
+    const int e0 = B.ni + (D == 0), e1 = B.nj + (D == 1), e2 = B.nk + (D == 2);
+    for (int kk = 0; kk < e2; kk++)
+      for (int jj = 0; jj < e1; jj++)
+        for (int ii = 0; ii < e0; ii++) {
+          const int i = D == 0 ? ii : ii + 1, j = D == 1 ? jj : jj + 1, k = D == 2 ? kk : kk + 1;
+          const long p = cidx(B, i, j, k) - B.off;
+          const size_t s = ii + (size_t)e0 * (jj + (size_t)e1 * kk);
+          if (N) for (int q = 0; q < 3; q++) N[3 * s + q] = f[4 * p + q];
+          if (A) A[s] = f[4 * p + 3];
+        }
+  }
+  return 0;
+}
+
+int fuss_get_stream(fuss_ctx* c, void** s) {
+  if (!c || !s) return FUSS_EARG;
+  *s = (void*)c->stream;
+  return 0;
+}
+
+int fuss_set_stream(fuss_ctx* c, void* s) {
+  if (!c) return FUSS_EARG;
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  c->stream = (hipStream_t)s;
+  c->own_stream = false;
+  return 0;
+}
+
+int fuss_memory(fuss_ctx* c, double* bytes) {
+  if (!c || !bytes) return FUSS_EARG;
+  double s = 0;
+  for (auto& L : c->lev) {
+    s += 8.0 * (3 + 6) * L.npad + 4.0 * L.npad + 8.0 * (9 + 12 + 9) * L.npad + 24.0 * L.nnodes;
+    s += sizeof(BCRec) * (double)L.nbound;
+  }
+  *bytes = s;
+  return 0;
+}
+
+void fuss_destroy(fuss_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& L : c->lev) {
+    L.blk.free(); L.tiles.free(); L.nodes.free();
+    for (auto& t : L.Tb) t.free();
+    L.R.free(); L.R2.free(); L.dt.free(); L.qvol.free(); L.vol.free(); L.invL2.free(); L.mat.free();
+    L.M.free();
+    for (auto& f : L.F) f.free();
+    for (auto& w : L.W) w.free();
+    L.bc.free(); L.fmap.free(); L.partials.free();
+  }
+  c->dcp.free(); c->drho.free(); c->dkap.free(); c->dh.free();
+  c->status.free(); c->scal.free(); c->dtmin.free();
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->hstatus) (void)hipHostFree(c->hstatus);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

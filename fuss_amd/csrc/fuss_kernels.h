@@ -1,0 +1,163 @@
+// fuss_kernels.h -- device data structures and kernels of the FUSS explicit
+// step on gfx950 (MI355X).  Included by fuss_gpu.hip only.
+//
+// Layout (per multigrid level, all blocks of the level in one arena):
+//   cell arrays are padded by GC=2 ghost layers in i and j and, in 3-D, in k
+//   (2-D levels keep a single k plane on each side, which only stores the
+//   k-face geometry: the reference zeroes every k term when ndir == 2,
+//   Lib_Diffusive.f90:32).  Cell (i,j,k) of block b lives at
+//   blk[b].base + (i-1) + (j-1)*sy + (k-1)*sz (doubles), i fastest.
+//   Face weights: W[3*D+c][cell] holds, for the face between cell and its
+//   +1 neighbour in direction D, A * sum_x Mbar(c,x) n_x with
+//   Mbar = (M_cell + M_nb)/2 -- the flux is then kappa * sum_c G_c W[3D+c].
+#pragma once
+
+#include "fuss_common.h"
+
+namespace fuss {
+
+constexpr int TILE_X = 64;
+constexpr int TILE_Y = 4;
+constexpr int TILE_T = TILE_X * TILE_Y;
+constexpr int MAX_SERIES = 16;
+
+struct BlockInfo {
+  int ni, nj, nk;
+  int sy;            // ni + 2*GC
+  long sz;           // sy * (nj + 2*GC)
+  long off;          // arena index of the padded origin
+  long base;         // arena index of cell (1,1,1)
+  long npad;         // padded size
+  long node_off;     // node arena offset (in nodes)
+  int fmap_off[6];   // face-map offsets, -1 when the face has no entries
+  int face_active;   // bit f-1 set when face f has flux-producing entries
+  int tang;          // nonzero when any tangential weight is nonzero
+};
+
+struct Tile {
+  int b, i0, j0, k0;
+};
+
+// One BC entry, geometry gathered at setup.  Reference fields: bc_type /
+// FUSS_bc_type (src/lib/base/Advanced_Types_m.f90:83-114).
+struct BCRec {
+  int type, f, dir, series;
+  int b, i, j, k;      // 0-based block, 1-based cell
+  long cell;           // arena index of the boundary cell
+  long gdst, gsrc;     // layer-1 ghost of the cell and its source (mirror or donor)
+  long dn[4];          // donor cells of Tg(3..6) (connections)
+  long rdonor;         // donor cell whose R fills the IRS ghost (connections)
+  int matg, pad0;      // material of the connection ghost (matIDg)
+  double qw, Tw, hconv, Tref, eps;
+  double A, N[3];
+  double M[9];         // metric tensor of the boundary cell, c(3,3) column-major
+  double Mg[9];        // ghost-cell metric Mg(1) (connections)
+  double mc[3];        // dot(Mg(r,:), N), r = 1..3
+};
+
+// Property tables, one row per kelvin from Ti1 (Load_Table.f90:61-71).
+template <typename R>
+struct Tab {
+  const R* cp;
+  const R* rho;
+  const R* kap;
+  const R* h;
+  int nT, Ti1, Ti2;
+};
+
+__device__ __forceinline__ void flag_error(int* st, int code, int b, int i, int j, int k) {
+  if (atomicCAS(st, 0, code) == 0) {
+    st[1] = b + 1;
+    st[2] = i;
+    st[3] = j;
+    st[4] = k;
+  }
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin; give each XCD a
+// contiguous run of tiles so that neighbouring k-planes share its L2.
+__device__ __forceinline__ int xcd_remap(int bid, int n) {
+  const int q = n >> 3, r = n & 7;
+  const int x = bid & 7, s = bid >> 3;
+  return x * q + (x < r ? x : r) + s;
+}
+
+// co_Kappa / co_Rho / co_Cp / co_H, Lib_Solid.f90:11-76:
+// v = v(Tm) + (T - Tm) * (v(Tm+1) - v(Tm)), Tm = int(T).
+template <typename R>
+__device__ __forceinline__ R tab_interp(const R* __restrict__ row, R T, int Ti1, int Ti2,
+                                        int* bad) {
+  if (!(T >= (R)Ti1 && T < (R)Ti2)) {
+    *bad = 1;
+    T = T < (R)Ti1 || T != T ? (R)Ti1 : (R)(Ti2 - 1);
+  }
+  const int Tm = (int)T;
+  const R vm = row[Tm - Ti1], vp = row[Tm - Ti1 + 1];
+  return vm + (T - (R)Tm) * (vp - vm);
+}
+// co_DrhoDT / co_DcpDT, Lib_Solid.f90:101-132
+template <typename R>
+__device__ __forceinline__ R tab_slope(const R* __restrict__ row, R T, int Ti1, int Ti2) {
+  if (!(T >= (R)Ti1 && T < (R)Ti2)) T = T < (R)Ti1 || T != T ? (R)Ti1 : (R)(Ti2 - 1);
+  const int Tm = (int)T;
+  return row[Tm - Ti1 + 1] - row[Tm - Ti1];
+}
+// co_T, Lib_Solid.f90:79-98: the first row s with h(s) > hv brackets
+// [s-1, s].  The reference scans from the first row; on a non-decreasing
+// table the local walk from the previous temperature finds the same row.
+template <typename R>
+__device__ __forceinline__ R tab_inverse(const R* __restrict__ row, R hv, R Tguess, int Ti1,
+                                         int Ti2, int* bad) {
+  int s = (Tguess >= (R)Ti1 && Tguess < (R)Ti2) ? (int)Tguess + 1 : Ti1 + 1;
+  // invariant sought: h(s-1) <= hv < h(s), Ti1+1 <= s <= Ti2
+  int guard = 0;
+  while (s > Ti1 + 1 && row[s - 1 - Ti1] > hv && guard < 1 << 20) {
+    --s;
+    ++guard;
+  }
+  while (s <= Ti2 && !(row[s - Ti1] > hv) && guard < 1 << 20) {
+    ++s;
+    ++guard;
+  }
+  if (s > Ti2 || !(row[s - 1 - Ti1] <= hv) || hv != hv) {
+    *bad = 1;
+    s = s > Ti2 ? Ti2 : (s < Ti1 + 1 ? Ti1 + 1 : s);
+  }
+  const R hm = row[s - 1 - Ti1], hp = row[s - Ti1];
+  return (R)(s - 1) + (hv - hm) / (hp - hm);
+}
+
+// co_T by bisection over the whole table (same first row s with h(s) > hv).
+template <typename R>
+__device__ __forceinline__ R tab_inverse_search(const R* __restrict__ row, R hv, int Ti1, int Ti2, int* bad) {
+  int lo = Ti1 + 1, hi = Ti2;
+  if (!(row[Ti2 - Ti1] > hv) || !(row[0] <= hv)) {
+    *bad = 1;
+    return (R)Ti1;
+  }
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (row[mid - Ti1] > hv) hi = mid;
+    else lo = mid + 1;
+  }
+  const R hm = row[lo - 1 - Ti1], hp = row[lo - Ti1];
+  return (R)(lo - 1) + (hv - hm) / (hp - hm);
+}
+
+// time_series_type%update, src/lib/base/Series_Data_m.f90:55-86 (host)
+inline double series_value(int periodic, int n, const double* t, const double* v, double time) {
+  double t_ = time;
+  if (periodic) {
+    const double period = t[n - 1] - t[0];
+    if (period > 0.0) {
+      const double a = t_ - t[0];
+      t_ = t[0] + (a - floor(a / period) * period);
+    }
+  }
+  if (t_ == t[0]) return v[0];
+  for (int i = 1; i < n; i++)
+    if (t_ > t[i - 1] && t_ <= t[i]) return (v[i] - v[i - 1]) / (t[i] - t[i - 1]) * (t_ - t[i - 1]) + v[i - 1];
+  return t_ > t[n - 1] ? v[n - 1] : 0.0;
+}
+
+}  // namespace fuss

@@ -140,7 +140,7 @@ def read_tecplot(path):
 
 
 def _fmt15(a):
-    return "\n".join(f" {x: .15E}" for x in np.asarray(a, dtype=np.float64).ravel())
+    return "\n".join(f" {x: .16E}" for x in np.asarray(a, dtype=np.float64).ravel())
 
 
 def write_tecplot(path, zones, varnames, cell_vars, solutiontime=0.0):
@@ -341,7 +341,7 @@ def write_properties(path, tab: PropertyTable):
             fh.write(f'ZONE T="{nm}"\nI={tab.nT}, F=POINT\n')
             T = tab.Ti1 + np.arange(tab.nT)
             for q in range(tab.nT):
-                fh.write(f"{T[q]:.1f} {tab.cp[m, q]!r} {tab.rho[m, q]!r} {tab.kap[m, q]!r} {tab.h[m, q]!r}\n")
+                fh.write(f"{T[q]:.1f} {float(tab.cp[m, q])!r} {float(tab.rho[m, q])!r} {float(tab.kap[m, q])!r} {float(tab.h[m, q])!r}\n")
 
 
 def read_phase(path):

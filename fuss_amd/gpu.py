@@ -1,0 +1,253 @@
+"""ctypes binding of libfuss_gpu.so and the FUSS driver loop on top of it.
+
+``Solver`` mirrors the reference's library API ``FUSS_type%setup / solve``
+(src/lib/driver/Procedures_m.f90:8-14): ``Solver(case)`` is FUSS_setup,
+``explicit_step`` is Explicit_Step (src/lib/numerics/time/explicit/Mod_Explicit.f90:10-111)
+and ``solve`` runs the FUSS_solve / FUSS_postprocess control of
+src/lib/driver/Wrap_Solve.f90:10-33 and Wrap_Postprocess.f90:183-187
+(coarse-to-fine multigrid sequencing, residual/time/iteration stops).
+
+Errors are raised as ``FussError`` carrying the C-ABI status and, for a failed
+state check, the (block, i, j, k) of the cell -- the information the reference
+prints before ``stop "NaN or T<0 detected"`` (Lib_Newstate.f90:283-287).
+There is no CPU fallback: a missing library is an ImportError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfuss_gpu.so")
+
+FUSS_FP64, FUSS_FP32 = 0, 1
+FUSS_CONS, FUSS_PRIM = 0, 1
+FUSS_IRS_JACOBI2, FUSS_IRS_TRIDIAG = 0, 1
+FUSS_ESTATE = 2
+
+
+class StepParams(C.Structure):
+    _fields_ = [("n_rk", C.c_int), ("vnn", C.c_double), ("rampa_iter", C.c_int), ("time_accurate", C.c_int),
+                ("integration", C.c_int), ("irs_beta", C.c_double), ("irs_mode", C.c_int),
+                ("compute_residual", C.c_int)]
+
+
+class StepResult(C.Structure):
+    _fields_ = [("res_sq", C.c_double), ("dtglobal", C.c_double), ("time", C.c_double), ("iter", C.c_int),
+                ("status", C.c_int), ("bijk", C.c_int * 4)]
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lib = None
+
+
+def load():
+    """Load libfuss_gpu.so; raise ImportError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.isfile(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    L.fuss_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(P)]
+    L.fuss_set_table.argtypes = [P, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+    L.fuss_set_block.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+    L.fuss_add_series.argtypes = [P, C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
+    L.fuss_set_bc.argtypes = [P, C.c_int, C.c_int, _ip, _dp, _ip]
+    L.fuss_setup.argtypes = [P, C.c_double]
+    L.fuss_step.argtypes = [P, C.c_int, C.POINTER(StepParams), C.POINTER(StepResult)]
+    L.fuss_run.argtypes = [P, C.c_int, C.POINTER(StepParams), C.c_int, C.c_int, C.c_double, C.c_double,
+                           C.POINTER(C.c_int), C.c_void_p, C.c_void_p, C.POINTER(StepResult)]
+    L.fuss_restrict.argtypes = [P, C.c_int]
+    L.fuss_prolong.argtypes = [P, C.c_int]
+    L.fuss_get_T.argtypes = [P, C.c_int, C.c_int, _dp]
+    L.fuss_set_T.argtypes = [P, C.c_int, C.c_int, _dp]
+    L.fuss_probe.argtypes = [P, C.c_int, _ip, _dp]
+    L.fuss_get_clock.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.fuss_set_clock.argtypes = [P, C.c_int, C.c_double, C.c_int]
+    L.fuss_ndir.argtypes = [P, C.POINTER(C.c_int)]
+    L.fuss_get_field.argtypes = [P, C.c_int, C.c_int, C.c_char_p, _dp]
+    L.fuss_get_metrics.argtypes = [P, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    L.fuss_get_stream.argtypes = [P, C.POINTER(P)]
+    L.fuss_set_stream.argtypes = [P, P]
+    L.fuss_memory.argtypes = [P, C.POINTER(C.c_double)]
+    L.fuss_strerror.restype = C.c_char_p
+    L.fuss_strerror.argtypes = [C.c_int]
+    L.fuss_destroy.argtypes = [P]
+    _lib = L
+    return L
+
+
+class FussError(RuntimeError):
+    def __init__(self, code, what="", bijk=None):
+        self.code = code
+        self.bijk = tuple(bijk) if bijk is not None else None
+        msg = f"fuss_gpu: {load().fuss_strerror(code).decode()} (status {code})"
+        if what:
+            msg += f" in {what}"
+        if bijk is not None and code == FUSS_ESTATE:
+            msg += f"; Integration failed at b, i, j, k: {tuple(bijk)}"
+        super().__init__(msg)
+
+
+def _chk(code, what=""):
+    if code:
+        raise FussError(code, what)
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def step_params(p, compute_residual=True, irs_mode=FUSS_IRS_JACOBI2, n_rk=None):
+    return StepParams(n_rk or p.n_rk, p.vnn, p.rampa_iter, int(p.time_accurate),
+                      FUSS_PRIM if p.prim else FUSS_CONS, p.irs_beta, irs_mode, int(compute_residual))
+
+
+class Solver:
+    """A case resident on the GPU: FUSS_setup done at construction."""
+
+    def __init__(self, case, precision=FUSS_FP64, params=None):
+        self.L = load()
+        self.case = case
+        self.p = params or case.params
+        self.h = C.c_void_p()
+        _chk(self.L.fuss_create(self.p.mgl, case.nb, precision, C.byref(self.h)), "fuss_create")
+        t = case.table
+        _chk(self.L.fuss_set_table(self.h, t.nmat, t.Ti1, t.nT, _c(t.cp), _c(t.rho), _c(t.kap), _c(t.h)),
+             "fuss_set_table")
+        for bi, b in enumerate(case.blocks):
+            ni, nj, nk = b.dims
+            _chk(self.L.fuss_set_block(self.h, bi + 1, ni, nj, nk, _c(b.nodes), _c(b.T), _c(b.matID), _c(b.qvol)),
+                 "fuss_set_block")
+        for s in case.series:
+            sid = C.c_int(0)
+            _chk(self.L.fuss_add_series(self.h, int(s[0]), len(s[1]), _c(s[1]), _c(s[2]), C.byref(sid)))
+        for m in range(self.p.mgl):
+            bcl = case.bcs[m]
+            sid = case.series_id[m] if m < len(case.series_id) else np.full(bcl.n, -1, np.int32)
+            _chk(self.L.fuss_set_bc(self.h, m + 1, bcl.n, _c(bcl.ib, np.int32), _c(bcl.rb), _c(sid, np.int32)),
+                 f"fuss_set_bc level {m+1}")
+        _chk(self.L.fuss_setup(self.h, 0.0 if self.p.time_accurate else -1.0), "fuss_setup")
+        self.level = self.p.mgl
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.fuss_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state access ------------------------------------------------------
+    def dims(self, level, b):
+        ni, nj, nk = self.case.blocks[b - 1].dims
+        for _ in range(1, level):
+            ni, nj, nk = ni // 2, nj // 2, max(1, nk // 2)
+        return ni, nj, nk
+
+    def get(self, name, level=1, b=1):
+        ni, nj, nk = self.dims(level, b)
+        out = np.zeros(ni * nj * nk)
+        _chk(self.L.fuss_get_field(self.h, level, b, name.encode(), out), f"fuss_get_field {name}")
+        return out.reshape(nk, nj, ni)
+
+    def T(self, level=1):
+        return [self.get("T", level, b + 1) for b in range(self.case.nb)]
+
+    def set_T(self, T, level=1, b=1):
+        _chk(self.L.fuss_set_T(self.h, level, b, _c(T)), "fuss_set_T")
+
+    def probe(self, bijk):
+        q = _c(np.asarray(bijk).reshape(-1, 4), np.int32)
+        out = np.zeros(q.shape[0])
+        _chk(self.L.fuss_probe(self.h, q.shape[0], q, out), "fuss_probe")
+        return out
+
+    def clock(self, level=1):
+        t = C.c_double(0)
+        it = C.c_int(0)
+        _chk(self.L.fuss_get_clock(self.h, level, C.byref(t), C.byref(it)))
+        return t.value, it.value
+
+    def metrics(self, level=1, b=1):
+        ni, nj, nk = self.dims(level, b)
+        M = np.zeros(9 * ni * nj * nk)
+        vol = np.zeros(ni * nj * nk)
+        _chk(self.L.fuss_get_metrics(self.h, level, b, M.ctypes.data_as(C.c_void_p), vol.ctypes.data_as(C.c_void_p),
+                                     0, None, None))
+        faces = []
+        for d in range(3):
+            e = [ni, nj, nk]
+            e[d] += 1
+            N = np.zeros(3 * e[0] * e[1] * e[2])
+            A = np.zeros(e[0] * e[1] * e[2])
+            _chk(self.L.fuss_get_metrics(self.h, level, b, None, None, d + 1, N.ctypes.data_as(C.c_void_p),
+                                         A.ctypes.data_as(C.c_void_p)))
+            faces.append((N.reshape(e[2], e[1], e[0], 3), A.reshape(e[2], e[1], e[0])))
+        return M.reshape(nk, nj, ni, 3, 3).swapaxes(-1, -2), vol.reshape(nk, nj, ni), faces
+
+    def stream(self):
+        s = C.c_void_p()
+        _chk(self.L.fuss_get_stream(self.h, C.byref(s)))
+        return s.value
+
+    # -- hot path ----------------------------------------------------------
+    def explicit_step(self, level=None, compute_residual=True, irs_mode=FUSS_IRS_JACOBI2):
+        """One Explicit_Step; returns the StepResult (res_sq, dtglobal, time, iter)."""
+        lv = level or self.level
+        sp = step_params(self.p, compute_residual, irs_mode)
+        r = StepResult()
+        code = self.L.fuss_step(self.h, lv, C.byref(sp), C.byref(r))
+        if code:
+            raise FussError(code, "fuss_step", list(r.bijk))
+        return r
+
+    def restrict(self, fine_level):
+        _chk(self.L.fuss_restrict(self.h, fine_level), "fuss_restrict")
+
+    def prolong(self, fine_level):
+        _chk(self.L.fuss_prolong(self.h, fine_level), "fuss_prolong")
+
+    def run_level(self, level, max_iters, itermax_left, res_threshold, time_threshold, hist=True,
+                  irs_mode=FUSS_IRS_JACOBI2):
+        sp = step_params(self.p, True, irs_mode)
+        n = C.c_int(0)
+        hr = np.zeros(max(max_iters, 1)) if hist else None
+        ht = np.zeros(max(max_iters, 1)) if hist else None
+        r = StepResult()
+        code = self.L.fuss_run(self.h, level, C.byref(sp), int(max_iters), int(itermax_left), res_threshold,
+                               time_threshold, C.byref(n), hr.ctypes.data_as(C.c_void_p) if hist else None,
+                               ht.ctypes.data_as(C.c_void_p) if hist else None, C.byref(r))
+        if code:
+            raise FussError(code, "fuss_run", list(r.bijk))
+        k = n.value
+        return k, (hr[:k] if hist else None), (ht[:k] if hist else None), r
+
+    def solve(self, max_iters=10**9, irs_mode=FUSS_IRS_JACOBI2):
+        """FUSS_solve loop to the end of the run: coarse-to-fine sequencing,
+        then level 1 until a stop condition.  Returns (n_iter, history) with
+        history rows (iter_general, time, residual)."""
+        p = self.p
+        itermax = p.itermax()
+        rows = []
+        done = 0
+        while done < max_iters:
+            lv = self.level
+            k, hr, ht, _ = self.run_level(lv, max_iters - done, itermax[lv - 1], p.res_threshold,
+                                          p.time_threshold, True, irs_mode)
+            rows.append(np.stack([done + 1 + np.arange(k, dtype=np.float64), ht, hr], axis=1))
+            done += k
+            stop = (k >= itermax[lv - 1] or (k and hr[-1] <= p.res_threshold) or (k and ht[-1] >= p.time_threshold))
+            if lv == 1 or not stop or done >= max_iters:
+                break
+            self.prolong(lv - 1)
+            self.level = lv - 1
+        return done, (np.concatenate(rows) if rows else np.zeros((0, 3)))

@@ -1,0 +1,160 @@
+/*
+ * fuss_gpu.h -- C-ABI of the MI355X (gfx950) explicit solver step of FUSS.
+ *
+ * Drop-in boundary for the reference's hot path (open-hydra/FUSS, Fortran):
+ * the driver keeps FUSS_setup / FUSS_solve / FUSS_postprocess, the INI
+ * configuration and the BC / material files, and calls these entry points
+ * through iso_c_binding (fuss_amd/fortran/fuss_gpu_m.f90) instead of the
+ * OpenMP loops of Explicit_Step.  Plain C types only: int, double, pointers
+ * and sizes.  Every entry returns 0 on success or a FUSS_E* status.
+ *
+ * Index conventions are the reference's: blocks, cells and levels are
+ * 1-based; faces are 1=i-min, 2=i-max, 3=j-min, 4=j-max, 5=k-min, 6=k-max;
+ * cell arrays passed in or out hold the interior ni*nj*nk cells with i
+ * fastest (Fortran T(1:ni,1:nj,1:nk)); node arrays hold (ni+1)(nj+1)(nk+1)
+ * points x,y,z with the coordinate fastest (Fortran node(0:ni,0:nj,0:nk)%c).
+ */
+#ifndef FUSS_GPU_H
+#define FUSS_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fuss_ctx fuss_ctx;
+
+/* status codes */
+#define FUSS_OK 0
+#define FUSS_EARG 1          /* invalid argument or call order */
+#define FUSS_ESTATE 2        /* NaN or T<0 after an update: Check_And_Fix_State */
+#define FUSS_ETABLE 3        /* T or h outside the property table */
+#define FUSS_EMETRIC 4       /* det(M)=0 or vol<=0 in the metrics */
+#define FUSS_EUNSUPPORTED 5  /* e.g. chimera (102) entries */
+#define FUSS_EHIP 6          /* HIP runtime error */
+#define FUSS_EBC 7           /* BC list inconsistent with the blocks */
+
+/* precision modes of a context */
+#define FUSS_FP64 0          /* plain FP64 path: parity with the reference */
+#define FUSS_FP32 1          /* FP32 sweeps (flux, RK, IRS) + FP64 state/reductions */
+
+/* integration variables (Mod_Newstate.f90:191-205) */
+#define FUSS_CONS 0
+#define FUSS_PRIM 1
+
+/* IRS flavours */
+#define FUSS_IRS_JACOBI2 0   /* reference: 2 Jacobi sweeps per direction */
+#define FUSS_IRS_TRIDIAG 1   /* exact line-tridiagonal solve per direction */
+
+/* Numerics of one step.  Replaces the obj_time_scheme / obj_irs globals that
+ * Explicit_Step reads (src/lib/config/Config_Types_m.f90, Register_Numerics.f90:30-48). */
+typedef struct {
+  int n_rk;            /* 1 euler, 2 RK2, 3 RK3 (Lib_RK.f90:8-11), 4 = four-stage extension */
+  double vnn;          /* VNN number (Mod_dt.f90:262-288) */
+  int rampa_iter;      /* vnn-rise-threshold */
+  int time_accurate;   /* global dt (Set_Global_dt) when nonzero */
+  int integration;     /* FUSS_CONS or FUSS_PRIM */
+  double irs_beta;     /* IRS enabled when > 0 */
+  int irs_mode;        /* FUSS_IRS_JACOBI2 or FUSS_IRS_TRIDIAG */
+  int compute_residual;/* nonzero: return Sum (T-TO)^2 (Mod_Explicit.f90:65-78) */
+} fuss_step_params;
+
+/* Outputs of one step. */
+typedef struct {
+  double res_sq;       /* Sum over the level of (T^{n+1}-T^n)^2 */
+  double dtglobal;     /* global min of the local dt (1e5 cap), Mod_dt.f90:228-258 */
+  double time;         /* domain(level)%time after the step */
+  int iter;            /* domain(level)%iter after the step */
+  int status;          /* FUSS_OK or FUSS_ESTATE / FUSS_ETABLE */
+  int bijk[4];         /* first failing cell (block, i, j, k) */
+} fuss_step_result;
+
+/* --- construction (replaces FUSS_setup's Setup_Data_Structure, Setup_BC,
+ *     Setup_Multigrid, Setup_Metrics and first Fill_Ghost_Cell:
+ *     src/lib/driver/Wrap_Setup.f90:10-165) ------------------------------ */
+
+/* New context for nlevels multigrid levels of nblocks blocks. */
+int fuss_create(int nlevels, int nblocks, int precision, fuss_ctx **out);
+
+/* Property tables, Load_Table (src/lib/io/Load_Table.f90:10-75): nmat
+ * materials x nT rows of 1 K starting at Ti1; arrays nmat*nT, T fastest. */
+int fuss_set_table(fuss_ctx *ctx, int nmat, int Ti1, int nT, const double *cp,
+                   const double *rho, const double *kappa, const double *h);
+
+/* Finest-level block b: dims, nodes, initial T, matID (material number as a
+ * real, FUSS_block_type%matID), qvol (may be NULL).
+ * Replaces Setup_Data_Structure (src/lib/driver/Mod_Allocate_Data.f90:7-60). */
+int fuss_set_block(fuss_ctx *ctx, int b, int ni, int nj, int nk, const double *nodes,
+                   const double *T, const double *matID, const double *qvol);
+
+/* Time series of time-dependent 301/302 values, time_series_type
+ * (src/lib/base/Series_Data_m.f90:20-86).  *id receives its index. */
+int fuss_add_series(fuss_ctx *ctx, int periodic, int n, const double *t, const double *v,
+                    int *id);
+
+/* BC list of a level as read by Read_BCfile (src/lib/io/IO_BC.f90:121-245),
+ * entries grouped by (block, face) ascending.  ib: nbound x 15 ints
+ * (b,i,j,k,f,type,bs,is,js,ks,fs,d11,d12,d21,d22); rb: nbound x 6 doubles
+ * (qw,Tw,hconv,qrad,Tref,eps); series: nbound ids or -1 (may be NULL). */
+int fuss_set_bc(fuss_ctx *ctx, int level, int nbound, const int *ib, const double *rb,
+                const int *series);
+
+/* Coarse grids, metrics, ghost metrics, initial ghost fill and restriction
+ * of the initial state to coarse levels.  initial_time: 0 for time-accurate
+ * runs, -1 for steady runs (Wrap_Setup.f90:123-144). */
+int fuss_setup(fuss_ctx *ctx, double initial_time);
+
+/* --- hot path -------------------------------------------------------- */
+
+/* One Explicit_Step on `level` (src/lib/numerics/time/explicit/Mod_Explicit.f90:10-111):
+ * Compute_dt + min, Copy_State, n_rk x (Fill_Ghost_Cell, Fluxes, BC_Fluxes,
+ * RK_Newstate [with Residual_Smoothing]), Compute_Residual. */
+int fuss_step(fuss_ctx *ctx, int level, const fuss_step_params *p, fuss_step_result *r);
+
+/* Up to max_iters Explicit_Steps on `level`, stopping after the first step
+ * whose residual sqrt(res_sq) <= res_threshold, whose time >= time_threshold,
+ * or when iter_from_call reaches itermax (the endsim/endmg tests of
+ * Mod_Explicit.f90:81-90).  hist_res/hist_time (may be NULL) receive one row
+ * per step; *n_done the number of steps taken. */
+int fuss_run(fuss_ctx *ctx, int level, const fuss_step_params *p, int max_iters,
+             int itermax_left, double res_threshold, double time_threshold, int *n_done,
+             double *hist_res, double *hist_time, fuss_step_result *last);
+
+/* Restriction (Mod_Multigrid.f90:59-77): coarse T, qvol, matID from level
+ * fine_level to fine_level+1. */
+int fuss_restrict(fuss_ctx *ctx, int fine_level);
+/* Prolongation (Mod_Multigrid.f90:80-95): fine T of fine_level replaced by
+ * the interpolation of level fine_level+1. */
+int fuss_prolong(fuss_ctx *ctx, int fine_level);
+
+/* --- state access (output cadence, restart, probes) ------------------- */
+int fuss_get_T(fuss_ctx *ctx, int level, int b, double *T);
+int fuss_set_T(fuss_ctx *ctx, int level, int b, const double *T);
+/* Probes (src/lib/io/IO_Probes.f90:124-166): T of n level-1 cells bijk[4n]. */
+int fuss_probe(fuss_ctx *ctx, int n, const int *bijk, double *out);
+/* Level time / iteration counters (domain(level)%time, %iter). */
+int fuss_get_clock(fuss_ctx *ctx, int level, double *time, int *iter);
+int fuss_set_clock(fuss_ctx *ctx, int level, double time, int iter);
+
+/* --- diagnostics / introspection ----------------------------------------- */
+/* Number of dimensions from Check_Mesh_Type (Lib_Metrics.f90:13-66). */
+int fuss_ndir(fuss_ctx *ctx, int *ndir);
+/* Interior field copy of a level/block: "T","TO","R","dt","vol","qvol". */
+int fuss_get_field(fuss_ctx *ctx, int level, int b, const char *name, double *out);
+/* Cell metric tensor M (9 per interior cell, c(3,3) column-major), cell
+ * volumes, and face normal/area of direction d (1..3) of a block, interior
+ * ranges of Allocate_Block (Mod_Allocate_Data.f90:78-84).  NULL skips. */
+int fuss_get_metrics(fuss_ctx *ctx, int level, int b, double *M, double *vol, int d,
+                     double *N, double *A);
+/* The HIP stream the context launches on (hipStream_t as void*), and a
+ * caller-provided stream to use instead (borrowed, not destroyed). */
+int fuss_get_stream(fuss_ctx *ctx, void **stream);
+int fuss_set_stream(fuss_ctx *ctx, void *stream);
+/* Device memory held by the context, bytes. */
+int fuss_memory(fuss_ctx *ctx, double *bytes);
+const char *fuss_strerror(int code);
+void fuss_destroy(fuss_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

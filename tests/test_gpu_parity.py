@@ -1,0 +1,142 @@
+"""Parity of the HIP path (libfuss_gpu.so, FP64) with the oracle and with the
+reference's stored outputs.
+
+Tolerances: the north-star bar for the plain FP64 path is 1e-12 relative
+(L-inf on T).  Metrics are computed on the device with the reference's
+operation order and FP contraction off, so they must agree bit for bit.
+The interior fluxes regroup A * kappa * (grad . n) as kappa * sum_c G_c w_c
+with precomputed face weights w (a different rounding, ~1 ulp per face), so
+fields agree to ~1e-15 per step rather than bitwise.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import gpu_available
+from tests.golden import fixtures
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-12
+ALL = list(fixtures.NAMES)
+STEADY = ["temperature_Plate", "qvol_Plate", "convective_Plate", "radiative_Rod"]
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    if not gpu_available():
+        pytest.skip("no GPU")
+    from fuss_amd import gpu as g
+
+    g.load()
+    return g
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+@pytest.mark.parametrize("name", ["temperature_Plate", "multimat_Plate", "mg_irs"])
+def test_metrics_bitwise(gpu, oracle_mod, name):
+    c, _ = fixtures.load(name)
+    s = gpu.Solver(c)
+    o = oracle_mod.Oracle(c)
+    for lv in range(1, c.params.mgl + 1):
+        for b in range(1, c.nb + 1):
+            M, vol, faces = s.metrics(lv, b)
+            Mo, dlo, volo, faceso = o.metrics(lv, b)
+            ni, nj, nk = s.dims(lv, b)
+            sl = (slice(2, 2 + nk), slice(2, 2 + nj), slice(2, 2 + ni))
+            np.testing.assert_array_equal(M, Mo[sl])
+            np.testing.assert_array_equal(vol, volo[sl])
+            for d in range(3):
+                np.testing.assert_array_equal(faces[d][0], faceso[d][0])
+                np.testing.assert_array_equal(faces[d][1], faceso[d][1])
+    s.close()
+
+
+@pytest.mark.parametrize("name", ALL)
+def test_steps_vs_oracle(gpu, oracle_mod, name):
+    """20 Explicit_Steps on the coarsest level (and the finest, via
+    prolongation), field and residual against the oracle."""
+    c, _ = fixtures.load(name)
+    s = gpu.Solver(c)
+    o = oracle_mod.Oracle(c)
+    lv = c.params.mgl
+    for n in range(20):
+        r = s.explicit_step(lv)
+        ro = o.step(lv)
+        if n == 0:
+            for b in range(c.nb):
+                assert relerr(s.get("T", lv, b + 1), o.get("T", lv, b + 1)) < 1e-14
+        assert abs(np.sqrt(r.res_sq) - np.sqrt(ro)) <= 1e-9 * np.sqrt(ro) + 1e-12
+    for b in range(c.nb):
+        assert relerr(s.get("T", lv, b + 1), o.get("T", lv, b + 1)) < RTOL
+    t, it = s.clock(lv)
+    assert it == 20 and abs(t - o.time(lv)) <= 1e-14 * max(1.0, abs(t))
+    if lv > 1:
+        s.prolong(lv - 1)
+        o.prolong(lv - 1)
+        for n in range(5):
+            s.explicit_step(lv - 1)
+            o.step(lv - 1)
+        for b in range(c.nb):
+            assert relerr(s.get("T", lv - 1, b + 1), o.get("T", lv - 1, b + 1)) < RTOL
+    s.close()
+
+
+@pytest.mark.parametrize("name", STEADY)
+def test_steady_to_convergence(gpu, name):
+    """Run to res-threshold on the GPU: same stop iteration as the reference
+    and its stored field to 1e-12."""
+    c, z = fixtures.load(name)
+    s = gpu.Solver(c)
+    n, hist = s.solve()
+    assert n == int(z["stop_iter"]), (n, int(z["stop_iter"]))
+    for b in range(c.nb):
+        err = relerr(s.get("T", 1, b + 1), z[f"ref_T{b}"])
+        print(f"{name}: stop {n}, block {b+1} max rel err vs reference {err:.3e}", flush=True)
+        assert err < RTOL
+    s.close()
+
+
+def test_multimat_transient(gpu):
+    c, z = fixtures.load("multimat_Plate")
+    s = gpu.Solver(c)
+    it = 0
+    for q, target in enumerate(z["snap_iter"]):
+        while it < target:
+            s.explicit_step(1, compute_residual=False)
+            it += 1
+        t, _ = s.clock(1)
+        assert abs(t - float(z[f"snap{q}_time"])) <= 1e-13 * abs(t)
+        for b in range(c.nb):
+            assert relerr(s.get("T", 1, b + 1), z[f"snap{q}_T{b}"]) < RTOL
+    s.close()
+
+
+def test_oscillating_probes(gpu):
+    c, z = fixtures.load("oscillating_Rod")
+    s = gpu.Solver(c)
+    p1, p2 = z["p1"], z["p2"]
+    for row in range(p1.shape[0]):
+        r = s.explicit_step(1, compute_residual=False)
+        v = s.probe(z["probes"])
+        assert abs(r.time - p1[row, 0]) <= 5e-14 * p1[row, 0]
+        assert abs(v[0] - p1[row, 1]) <= RTOL * p1[row, 1]
+        assert abs(v[1] - p2[row, 1]) <= RTOL * p2[row, 1]
+    s.close()
+
+
+def test_mg_irs_trajectory(gpu):
+    """All 80603 residuals of the 2-level multigrid + IRS + connection run."""
+    from tests.test_oracle_golden import residual_tolerance
+
+    c, z = fixtures.load("mg_irs")
+    ref = z["res_hist"]
+    s = gpu.Solver(c)
+    n, hist = s.solve(max_iters=ref.size + 10)
+    assert n == ref.size
+    tol = residual_tolerance(ref, 2000.0, c.ncell)
+    bad = np.abs(hist[:, 2] - ref) > tol
+    print(f"mg_irs: {n} iterations, max |dres|/tol = {np.max(np.abs(hist[:,2]-ref)/tol):.3f}", flush=True)
+    assert not bad.any(), f"first mismatch at iteration {np.argmax(bad)+1}"
+    s.close()

@@ -1,0 +1,84 @@
+"""Pin the CPU oracle to the reference's own stored outputs.
+
+Each fixture (tests/golden/*.npz, written by make_fixtures.py from the
+reference's test/ directory) runs through the oracle with the reference's
+inputs and numerics; the oracle must reproduce the stored solution:
+
+* steady cases: the stop iteration (first residual <= res-threshold) and the
+  final T field (Tecplot output, 15 significant digits);
+* multimat_Plate: six transient snapshots (time and T);
+* oscillating_Rod: both probe histories (time, T) of a periodic 302 series;
+* mg_irs: all 80603 residuals of a 2-level multigrid + IRS + connection run.
+"""
+import numpy as np
+import pytest
+
+from tests.golden import fixtures
+
+STEADY = ["temperature_Plate", "qvol_Plate", "convective_Plate", "radiative_Rod"]
+# Tecplot stores 15 significant digits; the oracle follows the reference's
+# operation order, so agreement is at that level.
+FIELD_RTOL = 1e-13
+
+
+@pytest.mark.parametrize("name", STEADY)
+def test_steady_field(oracle_mod, name):
+    c, z = fixtures.load(name)
+    stop = int(z["stop_iter"])
+    o = oracle_mod.Oracle(c)
+    n, hist, _ = o.run(max_iters=stop + 5)
+    assert n == stop, f"oracle stopped at {n}, reference at {stop}"
+    assert hist[-1, 2] <= c.params.res_threshold < hist[-2, 2]
+    for b in range(c.nb):
+        ref = z[f"ref_T{b}"]
+        err = np.max(np.abs(o.get("T", 1, b + 1) - ref) / np.abs(ref))
+        assert err < FIELD_RTOL, f"block {b+1}: max rel err {err:.3e}"
+
+
+def test_multimat_transient(oracle_mod):
+    c, z = fixtures.load("multimat_Plate")
+    o = oracle_mod.Oracle(c)
+    it = 0
+    for q, target in enumerate(z["snap_iter"]):
+        while it < target:
+            o.step(1)
+            it += 1
+        assert abs(o.time(1) - float(z[f"snap{q}_time"])) <= 1e-14 * abs(float(z[f"snap{q}_time"])) + 1e-16
+        for b in range(c.nb):
+            ref = z[f"snap{q}_T{b}"]
+            err = np.max(np.abs(o.get("T", 1, b + 1) - ref) / np.abs(ref))
+            assert err < FIELD_RTOL, f"snapshot {q} block {b+1}: {err:.3e}"
+
+
+def test_oscillating_probes(oracle_mod):
+    c, z = fixtures.load("oscillating_Rod")
+    o = oracle_mod.Oracle(c)
+    n, hist, pr = o.run(max_iters=10**6, probes=z["probes"])
+    p1, p2 = z["p1"], z["p2"]
+    k = p1.shape[0]
+    assert pr.shape[0] >= k
+    np.testing.assert_allclose(pr[:k, 0], p1[:, 0], rtol=5e-14)
+    np.testing.assert_allclose(pr[:k, 1], p1[:, 1], rtol=FIELD_RTOL)
+    np.testing.assert_allclose(pr[:k, 2], p2[:, 1], rtol=FIELD_RTOL)
+
+
+def residual_tolerance(res, Tmax, ncell):
+    """The stored residual has 10 significant digits; late in the run the
+    residual sqrt(sum dT^2) is itself rounding-limited: every cell's dT is
+    only known to ~ulp(T), so allow sqrt(ncell)*ulp(Tmax) on top."""
+    return 6e-10 * res + 4.0 * np.sqrt(ncell) * np.spacing(Tmax)
+
+
+@pytest.mark.slow
+def test_mg_irs_trajectory(oracle_mod):
+    c, z = fixtures.load("mg_irs")
+    ref = z["res_hist"]
+    o = oracle_mod.Oracle(c)
+    n, hist, _ = o.run(max_iters=ref.size + 10)
+    assert n == ref.size, f"oracle took {n} iterations, reference {ref.size}"
+    np.testing.assert_array_equal(hist[:, 0], np.arange(1, n + 1))
+    tol = residual_tolerance(ref, 2000.0, c.ncell)
+    bad = np.abs(hist[:, 2] - ref) > tol
+    assert not bad.any(), f"{bad.sum()} residuals off, first at iteration {np.argmax(bad)+1}"
+    # the first 10000 iterations agree to the stored 10 digits
+    np.testing.assert_allclose(hist[:10000, 2], ref[:10000], rtol=6e-10)
