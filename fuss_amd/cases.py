@@ -185,6 +185,101 @@ def sweep(ncase=256, lo=32, hi=96, L=0.1, seed=5, n_rk=3, vnn=2.0, prim=True, ir
     return cs.Case(blocks, bcs, tab, p, [], [np.full(b.n, -1, np.int32) for b in bcs], name=f"sweep_{ncase}")
 
 
+def material_tables(nmat=3, seed=3, nT=NT_DEFAULT, npts=16):
+    """nmat materials, each with npts-point rho(T), cp(T), k(T) knots on
+    [200, 1800] K (seeded), resampled to 1 K rows; h = cumulative rho*cp."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kT = np.linspace(200.0, 1800.0, npts)
+    cp, rho, kap = [], [], []
+    for _ in range(nmat):
+        k0, r0, c0 = rng.uniform(5.0, 150.0), rng.uniform(2000.0, 9000.0), rng.uniform(300.0, 900.0)
+        ph = rng.uniform(0.0, 2 * np.pi)
+        kap.append(knot_table(kT, k0 * (1.0 + 0.3 * np.sin(2 * np.pi * (kT - 200.0) / 1600.0 + ph)), nT))
+        rho.append(knot_table(kT, r0 * (1.0 - 2e-5 * (kT - 300.0)), nT))
+        cp.append(knot_table(kT, c0 * (1.0 + 3e-4 * (kT - 300.0)), nT))
+    cp, rho, kap = np.array(cp), np.array(rho), np.array(kap)
+    return fm.PropertyTable(1, cp, rho, kap, np.cumsum(rho * cp, axis=1), [f"mat{m+1}" for m in range(nmat)])
+
+
+def lattice(nl=4, lo=16, hi=128, seed=7, h=1e-3, amp=0.05, nmat=3, T0=500.0, n_rk=3, vnn=0.8,
+            prim=True, irs_beta=0.0, time_accurate=True, mgl=1, walls=None, res_threshold=1e-10):
+    """nl^3 blocks in a lattice with per-slab dimensions drawn from U{lo..hi}
+    (so neighbouring blocks share face dimensions), joined by 1-to-1 101
+    connections.  Nodes are a smooth global curvilinear map: base spacing h,
+    each coordinate perturbed by amp*h*sin(2 pi (k . X/Lg) + phi) (seeded), so
+    the faces of neighbouring blocks coincide.  Exterior faces by global side:
+    x-min 302, x-max 304, y-min 301, y-max 304, z-min 302, z-max 301 unless
+    `walls` overrides them (6 (type, rrec) pairs)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    step = 2 ** (mgl - 1)
+    dims = [rng.integers(lo // step, hi // step + 1, size=nl) * step for _ in range(3)]
+    off = [np.concatenate([[0], np.cumsum(d)]) for d in dims]
+    Lg = [off[d][-1] * h for d in range(3)]
+    kv = rng.integers(1, 3, size=(3, 3))
+    ph = rng.uniform(0.0, 2 * np.pi, size=3)
+
+    def node_map(I, J, K):
+        X = np.stack([I * h, J * h, K * h], axis=-1)
+        xi = X / np.asarray(Lg)
+        out = X.copy()
+        for c in range(3):
+            out[..., c] += amp * h * np.sin(2 * np.pi * (xi @ kv[c]) + ph[c])
+        return out
+
+    walls = walls or [(302, R(Tw=450.0)), (304, R(eps=0.8, Tref=300.0)), (301, R(qw=-5e3)),
+                      (304, R(eps=0.8, Tref=300.0)), (302, R(Tw=550.0)), (301, R(qw=0.0))]
+    blocks, bid = [], {}
+    mats = rng.integers(1, nmat + 1, size=nl ** 3)
+    for c in range(nl):
+        for b in range(nl):
+            for a in range(nl):
+                n = len(blocks)
+                bid[(a, b, c)] = n + 1
+                ni, nj, nk = int(dims[0][a]), int(dims[1][b]), int(dims[2][c])
+                I = off[0][a] + np.arange(ni + 1)
+                J = off[1][b] + np.arange(nj + 1)
+                K = off[2][c] + np.arange(nk + 1)
+                KK, JJ, II = np.meshgrid(K, J, I, indexing="ij")
+                nodes = np.ascontiguousarray(node_map(II.astype(float), JJ.astype(float), KK.astype(float)))
+                T = np.full((nk, nj, ni), T0)
+                blocks.append(cs.Block((ni, nj, nk), nodes, T, np.full_like(T, float(mats[n])), np.zeros_like(T)))
+
+    def lattice_bc(level):
+        ib, rb = [], []
+        for c in range(nl):
+            for b in range(nl):
+                for a in range(nl):
+                    me = bid[(a, b, c)]
+                    d0 = cs.coarse_dims(blocks[me - 1].dims, level + 1)[level]
+                    for f in range(1, 7):
+                        ax, hi_side = (f - 1) // 2, (f - 1) % 2
+                        pos = [a, b, c]
+                        pos[ax] += 1 if hi_side else -1
+                        if 0 <= pos[ax] < nl:
+                            nb_ = bid[tuple(pos)]
+                            dn = cs.coarse_dims(blocks[nb_ - 1].dims, level + 1)[level]
+                            fs = f + 1 if f % 2 == 1 else f - 1
+
+                            def extra(i, j, k, ax=ax, hi_side=hi_side, nb_=nb_, dn=dn, fs=fs):
+                                q = [i, j, k]
+                                q[ax] = dn[ax] if not hi_side else 1
+                                return [nb_, q[0], q[1], q[2], fs, 1, 0, 0, 1]
+
+                            x, y = face_entries(me, d0, f, 101, R(), extra)
+                        else:
+                            typ, rr = walls[f - 1]
+                            x, y = face_entries(me, d0, f, typ, rr)
+                        ib += x
+                        rb += y
+        return bclist(ib, rb)
+
+    bcs = [lattice_bc(m) for m in range(mgl)]
+    p = cs.Params(n_rk=n_rk, vnn=vnn, prim=prim, irs_beta=irs_beta, time_accurate=time_accurate, mgl=mgl,
+                  level_iter=[10**9] * mgl, res_threshold=res_threshold)
+    return cs.Case(blocks, bcs, material_tables(nmat, seed + 1), p, [], [np.full(x.n, -1, np.int32) for x in bcs],
+                   name=f"lattice{nl}")
+
+
 def write_case_dir(case, path, ini_extra=""):
     """Write a Case as a reference case directory (input.ini, INPUT/)."""
     os.makedirs(os.path.join(path, "INPUT"), exist_ok=True)

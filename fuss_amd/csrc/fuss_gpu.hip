@@ -13,7 +13,7 @@
 //               update (or the scaled residual when IRS is on)  Lib_Newstate.f90)
 //     k_irs     ndir line passes of 2 Jacobi sweeps, the last   (Lib_IRS.f90)
 //               fused with the RK update
-//   k_sum       Sum (T^{n+1}-T^n)^2, deterministic two-level   (Mod_Diagnostic.f90:13-49)
+//   k_reduce    Sum (T^{n+1}-T^n)^2, deterministic two-pass     (Mod_Diagnostic.f90:13-49)
 // Copy_State is free: T^n, and the two stage buffers rotate.
 #include <hip/hip_runtime.h>
 
@@ -196,7 +196,7 @@ struct DtArgs {
   Tab<R> tab;
   double vnn;
   int iter, rampa;
-  unsigned long long* dtmin;  // bits of the positive double minimum
+  double* pmin;        // per-work-group minimum
   int* status;
 };
 
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(TILE_T) void k_dt(DtArgs<R> a) {
   if (threadIdx.x == 0) {
     double v = sm[0];
     for (int q = 1; q < TILE_T / 64; q++) v = fmin(v, sm[q]);
-    if (v < 1e300) atomicMin(a.dtmin, (unsigned long long)__double_as_longlong(v));
+    a.pmin[wg] = v;
   }
 }
 
@@ -247,6 +247,7 @@ template <typename R>
 struct StageArgs {
   const BlockInfo* blk;
   const Tile* tiles;
+  const Tile* ctiles;  // column tiles (k0 = first plane of a KC-plane chunk)
   const R* T;          // stage input (ghosts filled)
   const R* TO;         // T^n
   R* Tout;             // stage output (update mode)
@@ -260,6 +261,7 @@ struct StageArgs {
   const R* W[9];
   const BCRec* bc;
   const int* fmap;
+  R* bcv;              // per BC entry: value subtracted from the cell residual
   Tab<R> tab;
   double sval[MAX_SERIES];
   R rkc;
@@ -311,9 +313,13 @@ __device__ __forceinline__ R dotrow(const double* M, int r, const double* N) {
 // BC_Fluxes for one entry (Mod_BC_Fluxes.f90:43-79 and the per-type files
 // src/lib/numerics/fluxes/bc/Lib_BC_Fluxes_*.f90), in the reference's
 // arithmetic order.
+// Returns v such that the reference's residual update is R = R - v; the stage
+// kernels apply the v of a cell's faces in face order, which reproduces the
+// reference's accumulation exactly.
 template <int NDIR, typename R>
-__device__ R bc_flux(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e, long p, R Rv) {
+__device__ R bc_value(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e, long p) {
 #pragma clang fp contract(off)
+  R Rv = 0;
   const R* __restrict__ T = a.T;
   const int Fm = e.f;
   const int modfm2 = 1 - 2 * (Fm % 2);
@@ -362,7 +368,7 @@ __device__ R bc_flux(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e, 
       R gr[3];
       for (int x = 0; x < 3; x++) gr[x] = G[0] * (R)M[3 * x] + G[1] * (R)M[3 * x + 1] + G[2] * (R)M[3 * x + 2];
       const R Flux = (R)e.A * kl * (gr[0] * (R)Nr[0] + gr[1] * (R)Nr[1] + gr[2] * (R)Nr[2]);
-      Rv = Rv - (R)modfm2 * Flux;
+      Rv = (R)modfm2 * Flux;
     } break;
     case 300: {
       // BC_Symmetry, Lib_BC_Fluxes_Symmetry.f90:12-67
@@ -374,20 +380,23 @@ __device__ R bc_flux(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e, 
       for (int x = 0; x < 3; x++) gr[x] = G[0] * (R)M[3 * x] + G[1] * (R)M[3 * x + 1] + G[2] * (R)M[3 * x + 2];
       const R kappa = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
       const R Flux = (R)e.A * kappa * (gr[0] * (R)Nr[0] + gr[1] * (R)Nr[1] + gr[2] * (R)Nr[2]);
-      Rv = Rv - (R)modfm2 * Flux;
+      Rv = (R)modfm2 * Flux;
     } break;
     case 301: {
       // BC_Wall_Heat, Lib_BC_Fluxes_Wall_Heat.f90:12-54
       const R qw = e.series >= 0 ? (R)a.sval[e.series] : (R)e.qw;
       const R Flux = (R)e.A * qw;
-      Rv = Rv - (R)modfm2 * Flux;
+      Rv = (R)modfm2 * Flux;
     } break;
     case 302: {
       // BC_Wall_Temperature, Lib_BC_Fluxes_Wall_Temperature.f90:12-68
       const R Tw = e.series >= 0 ? (R)a.sval[e.series] : (R)e.Tw;
       const R kappa = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
       R G[3] = {0, 0, 0};
-      G[e.dir - 1] = (Tc - Tw) * (R)modfm3;
+      const R gn = (Tc - Tw) * (R)modfm3;
+      if (e.dir == 1) G[0] = gn;
+      else if (e.dir == 2) G[1] = gn;
+      else G[2] = gn;
       if (Fm <= 2) {
         G[1] = (T[p - sy] - T[p + sy]) / (R)2.0;
         if (NDIR == 3) G[2] = (T[p - sz] - T[p + sz]) / (R)2.0;
@@ -402,7 +411,7 @@ __device__ R bc_flux(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e, 
       R gr[3];
       for (int x = 0; x < 3; x++) gr[x] = G[0] * (R)M[3 * x] + G[1] * (R)M[3 * x + 1] + G[2] * (R)M[3 * x + 2];
       const R Flux = (R)e.A * kappa * (gr[0] * (R)Nr[0] + gr[1] * (R)Nr[1] + gr[2] * (R)Nr[2]);
-      Rv = Rv - (R)modfm2 * Flux;
+      Rv = (R)modfm2 * Flux;
     } break;
     case 303: {
       // BC_Wall_HeatTransfer, Lib_BC_Fluxes_Wall_HeatTransfer.f90:12-52
@@ -412,7 +421,7 @@ __device__ R bc_flux(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e, 
       const R H = (R)1.0 / ((R)2.0 * kappa * mn) + (R)1.0 / (R)e.hconv;
       const R Hrad = (R)e.hconv / ((R)2.0 * kappa * mn) + (R)1.0;
       const R Flux = (R)e.A * ((R)1.0 / H * ((R)e.Tref - Tc) + (R)1.0 / Hrad * (R)0.0);
-      Rv = Rv - Flux;
+      Rv = Flux;
     } break;
     case 304:
     case 305: {
@@ -440,7 +449,7 @@ __device__ R bc_flux(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e, 
           break;
         }
       }
-      Rv = Rv - (R)Flux;
+      Rv = (R)Flux;
     } break;
     default:
       break;
@@ -454,6 +463,34 @@ __device__ __forceinline__ int face_map_index(const BlockInfo& B, int f, int i, 
   if (f <= 2) return B.fmap_off[f - 1] + (j - 1) + B.nj * (k - 1);
   if (f <= 4) return B.fmap_off[f - 1] + (i - 1) + B.ni * (k - 1);
   return B.fmap_off[f - 1] + (i - 1) + B.ni * (j - 1);
+}
+
+// BC entries of a boundary cell in the (block, face) order of the BC list
+// (Mod_BC_Fluxes.f90:29-33): faces 1..6 in turn.
+template <typename R>
+__device__ __forceinline__ R apply_bc_faces(const StageArgs<R>& a, const BlockInfo& B, int i, int j, int k, R Rv) {
+#pragma clang fp contract(off)
+  const int fa = B.face_active;
+  if (!fa) return Rv;
+  if ((fa & 1) && i == 1) Rv = Rv - a.bcv[a.fmap[B.fmap_off[0] + (j - 1) + B.nj * (k - 1)]];
+  if ((fa & 2) && i == B.ni) Rv = Rv - a.bcv[a.fmap[B.fmap_off[1] + (j - 1) + B.nj * (k - 1)]];
+  if ((fa & 4) && j == 1) Rv = Rv - a.bcv[a.fmap[B.fmap_off[2] + (i - 1) + B.ni * (k - 1)]];
+  if ((fa & 8) && j == B.nj) Rv = Rv - a.bcv[a.fmap[B.fmap_off[3] + (i - 1) + B.ni * (k - 1)]];
+  if ((fa & 16) && k == 1) Rv = Rv - a.bcv[a.fmap[B.fmap_off[4] + (i - 1) + B.ni * (j - 1)]];
+  if ((fa & 32) && k == B.nk) Rv = Rv - a.bcv[a.fmap[B.fmap_off[5] + (i - 1) + B.ni * (j - 1)]];
+  return Rv;
+}
+
+// BC_Fluxes (Mod_BC_Fluxes.f90:10-86) as a surface pass: one thread per
+// entry stores the value its face subtracts from the cell residual.
+template <int NDIR, typename R>
+__global__ void k_bcflux(StageArgs<R> a, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const BCRec& r = a.bc[e];
+  if (!(r.type == 101 || (r.type >= 300 && r.type <= 305))) return;
+  const BlockInfo B = a.blk[r.b];
+  a.bcv[e] = bc_value<NDIR>(a, B, r, r.cell);
 }
 
 // RK_stage, Lib_RK.f90:15-26
@@ -482,11 +519,18 @@ __device__ __forceinline__ R scaled_residual(const Tab<R>& tab, int m, R T, R Rv
                                              int* bad) {
 #pragma clang fp contract(off)
   if (!PRIM) return (-Rv / vol + qvol) * dt;  // Scale_Residual_Cons, Lib_Newstate.f90:99-104
-  const long row = (long)m * tab.nT;
-  const R rho = tab_interp(tab.rho + row, T, tab.Ti1, tab.Ti2, bad);
-  const R cp = tab_interp(tab.cp + row, T, tab.Ti1, tab.Ti2, bad);
-  const R drho = tab_slope(tab.rho + row, T, tab.Ti1, tab.Ti2);
-  const R dcp = tab_slope(tab.cp + row, T, tab.Ti1, tab.Ti2);
+  // co_Rho / co_Cp / co_DrhoDT / co_DcpDT from one (rho, drho, cp, dcp) row
+  R Tc = T;
+  if (!(Tc >= (R)tab.Ti1 && Tc < (R)tab.Ti2)) {
+    *bad = 1;
+    Tc = Tc < (R)tab.Ti1 || Tc != Tc ? (R)tab.Ti1 : (R)(tab.Ti2 - 1);
+  }
+  const int Tm = (int)Tc;
+  const typename Vec4<R>::T v =
+      ldu(reinterpret_cast<const typename Vec4<R>::T*>(tab.pr4), (unsigned)((long)m * tab.nT + (Tm - tab.Ti1)));
+  const R rho = v.x + (Tc - (R)Tm) * v.y;
+  const R cp = v.z + (Tc - (R)Tm) * v.w;
+  const R drho = v.y, dcp = v.w;
   // Compute_Prim_Residual, Lib_Newstate.f90:241-256
   return (-Rv / vol + qvol) * dt / (T * cp * drho + rho * cp + T * rho * dcp);
 }
@@ -528,16 +572,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage(StageArgs<R> a) {
         if (k < B.nk) Rv = Rv - face_flux<2, NDIR, TANG>(a, B, p);
       }
     }
-    if (B.face_active) {
-      // BC entries of the cell, in the (block, face) order of the BC list
-      const int on[6] = {i == 1, i == B.ni, j == 1, j == B.nj, k == 1, k == B.nk};
-      for (int f = 1; f <= 6; f++) {
-        if (on[f - 1] && (B.face_active >> (f - 1) & 1)) {
-          const int e = a.fmap[face_map_index<R>(B, f, i, j, k)];
-          Rv = bc_flux<NDIR>(a, B, a.bc[e], p, Rv);
-        }
-      }
-    }
+    Rv = apply_bc_faces(a, B, i, j, k, Rv);
     const R T = a.T[p];
     const int m = a.mat[p];
     const R dt = a.use_dtg ? (R)(*a.dtg) : a.dt[p];
@@ -555,6 +590,203 @@ __global__ __launch_bounds__(TILE_T) void k_stage(StageArgs<R> a) {
       rsq = d * d;
     }
     if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+  }
+  if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
+}
+
+
+// ------------------------------------------------------------------------
+// 2.5-D stage kernel for 3-D levels: a work-group owns a 64x4 column of
+// cells over KC consecutive k-planes and marches up in k.  Three planes of T
+// (k-1, k, k+1, each with a one-cell halo in i and j) rotate through LDS;
+// every face flux is evaluated once: +x / +y fluxes go through LDS to the
+// neighbouring cell, the +z flux is carried in a register to the next plane.
+// The residual accumulation keeps the Fluxes_blk order (i, then j, then k
+// faces; Mod_Fluxes.f90:54-192).
+// ------------------------------------------------------------------------
+constexpr int KC = 16;
+constexpr int PX = TILE_X + 2, PY = TILE_Y + 2;
+
+template <typename R>
+__device__ __forceinline__ void load_plane(R (*dst)[PX], const R* __restrict__ T, const BlockInfo& B, int i0,
+                                           int j0, int kk) {
+  const long base = B.base + (long)(kk - 1) * B.sz;
+  for (int q = threadIdx.x; q < PX * PY; q += TILE_T) {
+    const int ii = q % PX, jj = q / PX;
+    const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
+    R v = 0;
+    if (gi <= B.ni + 1 && gj <= B.nj + 1) v = T[base + (gi - 1) + (long)(gj - 1) * B.sy];
+    dst[jj][ii] = v;
+  }
+}
+
+// A plane with halo has PX*PY = 396 values: thread t holds elements t and
+// t + TILE_T (when < 396) in registers between the global load and the LDS
+// store, so the load of plane k+2 overlaps the flux work of plane k.
+template <typename R>
+struct PlaneRegs {
+  R v0, v1;
+};
+template <typename R>
+__device__ __forceinline__ void fetch_plane(PlaneRegs<R>& r, const R* __restrict__ T, const BlockInfo& B, int i0,
+                                            int j0, int kk) {
+  // arena index of the halo corner (i0-1, j0-1, kk); all offsets stay 32-bit
+  const unsigned corner = (unsigned)(B.base + (i0 - 2) + (long)(j0 - 2) * B.sy + (long)(kk - 1) * B.sz);
+  {
+    const int q = threadIdx.x, ii = q % PX, jj = q / PX;
+    r.v0 = (i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1) ? ldu(T, corner + ii + jj * B.sy) : (R)0;
+  }
+  {
+    const int q = threadIdx.x + TILE_T, ii = q % PX, jj = q / PX;
+    r.v1 = (q < PX * PY && i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1) ? ldu(T, corner + ii + jj * B.sy)
+                                                                                : (R)0;
+  }
+}
+template <typename R>
+__device__ __forceinline__ void store_plane(R (*dst)[PX], const PlaneRegs<R>& r) {
+  (&dst[0][0])[threadIdx.x] = r.v0;
+  if (threadIdx.x + TILE_T < PX * PY) (&dst[0][0])[threadIdx.x + TILE_T] = r.v1;
+}
+
+// kappa row of the left cell of a face: blocks of one material skip the
+// per-cell material load
+template <typename R>
+__device__ __forceinline__ const R* kap_row(const StageArgs<R>& a, const BlockInfo& B, unsigned q) {
+  const int m = B.umat >= 0 ? B.umat : ldu(a.mat, q);
+  return a.tab.kap2 + 2 * (long)m * a.tab.nT;
+}
+
+// Face fluxes from the LDS planes (m = k-1, c = k, p = k+1); (x, y) are the
+// LDS coordinates of the left cell, q its arena index.  Same algebra as
+// face_flux (Lib_Diffusive.f90:10-45 regrouped per computational direction).
+template <bool TANG, typename R>
+__device__ __forceinline__ R flux_x(const StageArgs<R>& a, const BlockInfo& B, R (*m)[PX], R (*c)[PX], R (*p)[PX],
+                                    int x, int y, unsigned q) {
+  const R Tl = c[y][x], Tr = c[y][x + 1];
+  int bad = 0;
+  const R kap = tab_interp2(kap_row(a, B, q), (R)0.5 * (Tl + Tr), a.tab.Ti1, a.tab.Ti2, &bad);
+  if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
+  const R G0 = Tr - Tl;
+  if (!TANG) return kap * (G0 * ldu(a.W[0], q));
+  const R G1 = (c[y + 1][x] - c[y - 1][x] + c[y + 1][x + 1] - c[y - 1][x + 1]) * (R)0.25;
+  const R G2 = (p[y][x] - m[y][x] + p[y][x + 1] - m[y][x + 1]) * (R)0.25;
+  return kap * ((G0 * ldu(a.W[0], q) + G1 * ldu(a.W[1], q)) + G2 * ldu(a.W[2], q));
+}
+
+template <bool TANG, typename R>
+__device__ __forceinline__ R flux_y(const StageArgs<R>& a, const BlockInfo& B, R (*m)[PX], R (*c)[PX], R (*p)[PX],
+                                    int x, int y, unsigned q) {
+  const R Tl = c[y][x], Tr = c[y + 1][x];
+  int bad = 0;
+  const R kap = tab_interp2(kap_row(a, B, q), (R)0.5 * (Tl + Tr), a.tab.Ti1, a.tab.Ti2, &bad);
+  if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
+  const R G1 = Tr - Tl;
+  if (!TANG) return kap * (G1 * ldu(a.W[4], q));
+  const R G0 = (c[y][x + 1] - c[y][x - 1] + c[y + 1][x + 1] - c[y + 1][x - 1]) * (R)0.25;
+  const R G2 = (p[y][x] - m[y][x] + p[y + 1][x] - m[y + 1][x]) * (R)0.25;
+  return kap * ((G0 * ldu(a.W[3], q) + G1 * ldu(a.W[4], q)) + G2 * ldu(a.W[5], q));
+}
+
+template <bool TANG, typename R>
+__device__ __forceinline__ R flux_z(const StageArgs<R>& a, const BlockInfo& B, R (*c)[PX], R (*p)[PX], int x, int y,
+                                    unsigned q) {
+  const R Tl = c[y][x], Tr = p[y][x];
+  int bad = 0;
+  const R kap = tab_interp2(kap_row(a, B, q), (R)0.5 * (Tl + Tr), a.tab.Ti1, a.tab.Ti2, &bad);
+  if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
+  const R G2 = Tr - Tl;
+  if (!TANG) return kap * (G2 * ldu(a.W[8], q));
+  const R G0 = (c[y][x + 1] - c[y][x - 1] + p[y][x + 1] - p[y][x - 1]) * (R)0.25;
+  const R G1 = (c[y + 1][x] - c[y - 1][x] + p[y + 1][x] - p[y - 1][x]) * (R)0.25;
+  return kap * ((G0 * ldu(a.W[6], q) + G1 * ldu(a.W[7], q)) + G2 * ldu(a.W[8], q));
+}
+
+template <bool TANG, bool PRIM, bool IRSMODE, typename R>
+__global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Tile t = a.ctiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int tx = threadIdx.x % TILE_X, ty = threadIdx.x / TILE_X;
+  const int i = t.i0 + tx, j = t.j0 + ty;
+  const int k0 = t.k0, k1 = min(B.nk, k0 + KC - 1);
+  const bool live = i <= B.ni && j <= B.nj;
+  const int x = tx + 1, y = ty + 1;  // LDS coordinates of the thread's cell
+  // min-side halo faces of the tile: wave 0 computes the y-halo (one face per
+  // lane), wave 1 lanes 0..TILE_Y-1 the x-halo of rows 0..TILE_Y-1
+  const bool yhalo = ty == 0 && t.j0 > 1 && i <= B.ni;
+  const int hr = tx;  // x-halo row handled by this lane of wave 1
+  const bool xhalo = ty == 1 && tx < TILE_Y && t.i0 > 1 && t.j0 + hr <= B.nj;
+  __shared__ R ring[3][PY][PX];
+  __shared__ R fx[TILE_Y][TILE_X + 1];
+  __shared__ R fy[TILE_Y + 1][TILE_X];
+  int sm = 0, sc = 1, sp = 2;
+  PlaneRegs<R> pre;
+  fetch_plane(pre, a.T, B, t.i0, t.j0, k0 - 1);
+  store_plane(ring[sm], pre);
+  fetch_plane(pre, a.T, B, t.i0, t.j0, k0);
+  store_plane(ring[sc], pre);
+  fetch_plane(pre, a.T, B, t.i0, t.j0, k0 + 1);
+  __syncthreads();
+  const unsigned sy = (unsigned)B.sy, sz = (unsigned)B.sz;
+  unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k0 - 1) * B.sz);
+  // arena index of the x-halo cell (i0-1, j0+hr, k) of wave 1
+  unsigned ph = (unsigned)(B.base + (t.i0 - 2) + (long)(t.j0 + hr - 1) * B.sy + (long)(k0 - 1) * B.sz);
+  R fz_prev = 0;
+  if (live && k0 > 1) {
+#pragma clang fp contract(off)
+    fz_prev = flux_z<TANG>(a, B, ring[sm], ring[sc], x, y, p - sz);
+  }
+  double rsq = 0.0;
+  for (int k = k0; k <= k1; k++, p += sz, ph += sz) {
+    store_plane(ring[sp], pre);
+    __syncthreads();
+    if (k < k1) fetch_plane(pre, a.T, B, t.i0, t.j0, k + 2);
+    R fz = 0;
+    {
+#pragma clang fp contract(off)
+      if (live) {
+        if (i < B.ni) fx[ty][tx + 1] = flux_x<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, y, p);
+        if (j < B.nj) fy[ty + 1][tx] = flux_y<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, y, p);
+        if (k < B.nk) fz = flux_z<TANG>(a, B, ring[sc], ring[sp], x, y, p);
+      }
+      if (yhalo) fy[0][tx] = flux_y<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, 0, p - sy);
+      if (xhalo) fx[hr][0] = flux_x<TANG>(a, B, ring[sm], ring[sc], ring[sp], 0, hr + 1, ph);
+    }
+    __syncthreads();
+    if (live) {
+      R Rv = 0;
+      {
+#pragma clang fp contract(off)
+        if (i > 1) Rv = Rv + fx[ty][tx];
+        if (i < B.ni) Rv = Rv - fx[ty][tx + 1];
+        if (j > 1) Rv = Rv + fy[ty][tx];
+        if (j < B.nj) Rv = Rv - fy[ty + 1][tx];
+        if (k > 1) Rv = Rv + fz_prev;
+        if (k < B.nk) Rv = Rv - fz;
+      }
+      Rv = apply_bc_faces(a, B, i, j, k, Rv);
+      const R T = ring[sc][y][x];
+      const int m = B.umat >= 0 ? B.umat : ldu(a.mat, p);
+      const R dt = a.use_dtg ? (R)(*a.dtg) : ldu(a.dt, p);
+      int bad = 0;
+      const R res = scaled_residual<PRIM>(a.tab, m, T, Rv, ldu(a.vol, p), ldu(a.qvol, p), dt, &bad);
+      if (IRSMODE) {
+        stu(a.Rres, p, res);
+      } else {
+        const R TO = ldu(a.TO, p);
+        const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, &bad);
+        stu(a.Tout, p, Tn);
+        if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+        const double d = (double)Tn - (double)TO;
+        rsq += d * d;
+      }
+      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+    }
+    fz_prev = fz;
+    const int s0 = sm;
+    sm = sc;
+    sc = sp;
+    sp = s0;
   }
   if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
 }
@@ -641,36 +873,173 @@ __global__ __launch_bounds__(TILE_T) void k_irs(IrsArgs<R> a) {
   if (UPDATE && a.last) block_sum(rsq, a.partials, wg);
 }
 
+// Two Jacobi sweeps at line position s of n (Lib_IRS.f90:63-89) from
+// r(s-2..s+2); positions 0 and n+1 are the ghosts glo/ghi, which the sweeps
+// leave unchanged (Residual_Smoothing_Blk smooths interior cells only).
+template <typename R>
+__device__ __forceinline__ R jacobi2(R rm2, R rm1, R r0, R rp1, R rp2, int s, int n, R glo, R ghi, R beta, R den) {
+#pragma clang fp contract(off)
+  const R am1 = s - 1 >= 1 ? rm1 : glo;
+  const R ap1 = s + 1 <= n ? rp1 : ghi;
+  const R am2 = s - 2 >= 1 ? rm2 : glo;
+  const R ap2 = s + 2 <= n ? rp2 : ghi;
+  const R sm = (s - 1 < 1) ? glo : (am1 + beta * (am2 + r0)) / den;
+  const R sp = (s + 1 > n) ? ghi : (ap1 + beta * (r0 + ap2)) / den;
+  return (r0 + beta * (sm + sp)) / den;
+}
+
+// IRS ghost of the line end on face f (Ghost_Residual_Extrapolation,
+// Lib_IRS.f90:170-184): the donor's R for a connection, else the boundary
+// cell's own R (zero gradient).
+template <typename R>
+__device__ __forceinline__ R irs_ghost(const IrsArgs<R>& a, const BlockInfo& B, int f, int i, int j, int k,
+                                       R own) {
+  const int e = a.fmap[face_map_index<R>(B, f, i, j, k)];
+  const BCRec& r = a.bc[e];
+  return r.type == 101 ? ldu(a.Rin, (unsigned)r.rdonor) : own;
+}
+
+constexpr int XY_TJ = 16;                 // rows of an x-y IRS tile
+constexpr int XY_RJ = XY_TJ + 4, XY_RI = TILE_X + 4;
+
+// x then y smoothing of one plane tile of 64 x 16 cells (Residual_Smoothing
+// directions 1 and 2, Lib_IRS.f90:25-38) through LDS: R is read once.  Used
+// when no block has a connection on a j face (the y ghosts are then the
+// x-smoothed boundary rows).  2-D levels finish the stage here (UPDATE).
+template <int NDIR, bool UPDATE, bool PRIM, typename R>
+__global__ __launch_bounds__(TILE_T) void k_irs_xy(IrsArgs<R> a) {
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Tile t = a.tiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int k = t.k0;
+  __shared__ R rin[XY_RJ][XY_RI];
+  __shared__ R xs[XY_RJ][TILE_X];
+  const unsigned corner = (unsigned)(B.base + (t.i0 - 3) + (long)(t.j0 - 3) * B.sy + (long)(k - 1) * B.sz);
+  for (int q = threadIdx.x; q < XY_RJ * XY_RI; q += TILE_T) {
+    const int ii = q % XY_RI, jj = q / XY_RI;
+    const int gi = t.i0 - 2 + ii, gj = t.j0 - 2 + jj;
+    R v = 0;
+    if (gi >= 1 && gi <= B.ni && gj >= 1 && gj <= B.nj) v = ldu(a.Rin, corner + ii + jj * B.sy);
+    rin[jj][ii] = v;
+  }
+  __syncthreads();
+  const R beta = a.beta, den = a.den;
+  // x smoothing of rows j0-2 .. j0+17 inside the block
+  for (int q = threadIdx.x; q < XY_RJ * TILE_X; q += TILE_T) {
+    const int ii = q % TILE_X, jj = q / TILE_X;
+    const int i = t.i0 + ii, j = t.j0 - 2 + jj;
+    if (i > B.ni || j < 1 || j > B.nj) continue;
+    R glo = 0, ghi = 0;
+    if (i <= 2) glo = irs_ghost(a, B, 1, 1, j, k, rin[jj][ii + 2 - (i - 1)]);
+    if (i >= B.ni - 1) ghi = irs_ghost(a, B, 2, B.ni, j, k, rin[jj][ii + 2 + (B.ni - i)]);
+    xs[jj][ii] = jacobi2(rin[jj][ii], rin[jj][ii + 1], rin[jj][ii + 2], rin[jj][ii + 3], rin[jj][ii + 4], i, B.ni,
+                         glo, ghi, beta, den);
+  }
+  __syncthreads();
+  // y smoothing of the 16 output rows; y ghosts are the x-smoothed rows 1, nj
+  double rsq = 0.0;
+  for (int q = threadIdx.x; q < XY_TJ * TILE_X; q += TILE_T) {
+    const int ii = q % TILE_X, jr = q / TILE_X;
+    const int i = t.i0 + ii, j = t.j0 + jr;
+    if (i > B.ni || j > B.nj) continue;
+    const int jj = jr + 2;  // LDS row of j
+    const R glo = (j <= 2) ? xs[jj - (j - 1)][ii] : (R)0;
+    const R ghi = (j >= B.nj - 1) ? xs[jj + (B.nj - j)][ii] : (R)0;
+    const R rs = jacobi2(xs[jj - 2][ii], xs[jj - 1][ii], xs[jj][ii], xs[jj + 1][ii], xs[jj + 2][ii], j, B.nj, glo, ghi,
+                         beta, den);
+    const unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz);
+    if (!UPDATE) {
+      stu(a.Rout, p, rs);
+    } else {
+      const R T = ldu(a.T, p), TO = ldu(a.TO, p);
+      int bad = 0;
+      const int m = B.umat >= 0 ? B.umat : ldu(a.mat, p);
+      const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, &bad);
+      stu(a.Tout, p, Tn);
+      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+      const double d = (double)Tn - (double)TO;
+      rsq += d * d;
+    }
+  }
+  if (UPDATE && a.last) block_sum(rsq, a.partials, wg);
+}
+
+// z smoothing (direction 3) fused with the RK update: a 64x4 column marches
+// through KC planes with r(k-2..k+2) in registers, so R is read once.
+template <bool PRIM, typename R>
+__global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Tile t = a.tiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X);
+  const int k0 = t.k0, k1 = min(B.nk, k0 + KC - 1), n = B.nk;
+  double rsq = 0.0;
+  if (i <= B.ni && j <= B.nj) {
+    const unsigned sz = (unsigned)B.sz;
+    const unsigned p1 = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy);  // plane k = 1
+    const R glo = (k0 <= 2) ? irs_ghost(a, B, 5, i, j, 1, ldu(a.Rin, p1)) : (R)0;
+    const R ghi = (k1 >= n - 1) ? irs_ghost(a, B, 6, i, j, n, ldu(a.Rin, p1 + (n - 1) * sz)) : (R)0;
+    auto rd = [&](int kk) -> R { return (kk >= 1 && kk <= n) ? ldu(a.Rin, p1 + (kk - 1) * sz) : (R)0; };
+    R w0 = rd(k0 - 2), w1 = rd(k0 - 1), w2 = rd(k0), w3 = rd(k0 + 1), w4 = rd(k0 + 2);
+    const int m = B.umat >= 0 ? B.umat : -1;
+    for (int k = k0; k <= k1; k++) {
+      const R rs = jacobi2(w0, w1, w2, w3, w4, k, n, glo, ghi, a.beta, a.den);
+      const unsigned p = p1 + (k - 1) * sz;
+      const R T = ldu(a.T, p), TO = ldu(a.TO, p);
+      int bad = 0;
+      const R Tn = new_state<PRIM>(a.tab, m >= 0 ? m : ldu(a.mat, p), T, TO, rs, a.rkc, &bad);
+      stu(a.Tout, p, Tn);
+      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+      const double d = (double)Tn - (double)TO;
+      rsq += d * d;
+      w0 = w1; w1 = w2; w2 = w3; w3 = w4;
+      w4 = rd(k + 3);
+    }
+  }
+  if (a.last) block_sum(rsq, a.partials, wg);
+}
+
 // ------------------------------------------------------------------------
-// kernels: reductions
+// kernels: reductions (deterministic: fixed partition and order)
 // ------------------------------------------------------------------------
-// Deterministic sum of the per-workgroup partials (fixed order).
-__global__ __launch_bounds__(256) void k_sum(const double* __restrict__ partials, int n,
-                                             double* __restrict__ out) {
+constexpr int RED_CHUNK = 2048;
+
+// Pass 1: work-group b reduces in[b*RED_CHUNK, ...) to out[b].
+template <bool MIN>
+__global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ in, int n, double* __restrict__ out) {
   __shared__ double sm[256];
-  double s = 0.0;
-  for (int q = threadIdx.x; q < n; q += 256) s += partials[q];
-  sm[threadIdx.x] = s;
+  const int lo = blockIdx.x * RED_CHUNK, hi = min(n, lo + RED_CHUNK);
+  double v = MIN ? 1e300 : 0.0;
+  for (int q = lo + threadIdx.x; q < hi; q += 256) v = MIN ? fmin(v, in[q]) : v + in[q];
+  sm[threadIdx.x] = v;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+    if (threadIdx.x < w) sm[threadIdx.x] = MIN ? fmin(sm[threadIdx.x], sm[threadIdx.x + w]) : sm[threadIdx.x] + sm[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = sm[0];
+  if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
 }
 
-__global__ void k_init_scalars(unsigned long long* dtmin, unsigned long long bits, double* scal,
-                               double time) {
-  *dtmin = bits;
-  scal[1] = time;
-}
-
-// dtglobal from the min bits; the time advance of time-accurate runs
-// (Mod_Explicit.f90:40-45).
-__global__ void k_dt_finish(const unsigned long long* dtmin, double* scal, int time_accurate) {
-  const double m = __longlong_as_double((long long)*dtmin);
-  scal[0] = m;
-  if (time_accurate) scal[1] = scal[1] + m;
+// Pass 2 of the dt minimum: dtglobal = min(1e5, min dt) (Mod_Explicit.f90:40,
+// Mod_dt.f90:250-258) and the time advance of time-accurate runs (:42-45).
+__global__ __launch_bounds__(256) void k_dt_final(const double* __restrict__ in, int n, double* scal, double time,
+                                                  int time_accurate) {
+  __shared__ double sm[256];
+  double v = 1e300;
+  for (int q = threadIdx.x; q < n; q += 256) v = fmin(v, in[q]);
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double m = fmin(1e5, sm[0]);
+    scal[0] = m;
+    scal[1] = time_accurate ? time + m : time;
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -809,10 +1178,13 @@ struct Level {
   int kp = 2;                       // k ghost layers (0 in 2-D)
   std::vector<BlockInfo> hblk;
   std::vector<HostBlock> hb;        // host nodes (every level), fields (level 1)
-  std::vector<Tile> htiles;
+  std::vector<Tile> htiles;         // one 64x4 cell tile per plane
+  std::vector<Tile> hctiles;        // 64x4 columns of KC planes (3-D levels)
+  std::vector<Tile> hxytiles;       // 64x16 plane tiles of the x-y IRS pass
+  int no_ycon = 1;                  // no connection on a j face (fused x-y IRS allowed)
   long npad = 0, ncell = 0, nnodes = 0;
   DBuf<BlockInfo> blk;
-  DBuf<Tile> tiles;
+  DBuf<Tile> tiles, ctiles, xytiles;
   DBuf<double> nodes;
   DBuf<double> Tb[3];               // T^n and two stage buffers
   int cur = 0;                      // index of T^n in Tb
@@ -821,6 +1193,7 @@ struct Level {
   DBuf<double> M, F[3], W[9];
   DBuf<BCRec> bc;
   DBuf<int> fmap;
+  DBuf<double> bcv;
   int nbound = 0;
   std::vector<int> hib;             // nbound x 15
   std::vector<double> hrb;          // nbound x 6
@@ -828,7 +1201,7 @@ struct Level {
   int tang = 0, any_active = 0;
   double time = 0.0;
   int iter = 0;
-  DBuf<double> partials;
+  DBuf<double> partials, pmin, red2;
 };
 
 }  // namespace
@@ -840,15 +1213,21 @@ struct fuss_ctx {
   bool own_stream = false;
   int nmat = 0, Ti1 = 0, nT = 0;
   std::vector<double> tcp, trho, tkap, th;
-  DBuf<double> dcp, drho, dkap, dh;
+  DBuf<double> dcp, drho, dkap, dh, dkap2, dpr4;
   struct Series { int periodic; std::vector<double> t, v; };
   std::vector<Series> series;
   DBuf<int> status;                 // code, b, i, j, k
   DBuf<double> scal;                // dtglobal, time, res_sq
-  DBuf<unsigned long long> dtmin;
   double* hscal = nullptr;          // pinned readback
   int setup_done = 0;
   int* hstatus = nullptr;
+  // per-kernel-class timing with HIP events on the context stream
+  struct Prof {
+    int enabled = 0;
+    std::vector<hipEvent_t> pool;
+    std::vector<int> cls;       // class of each recorded (start, end) pair
+    size_t used = 0;
+  } prof;
 };
 
 namespace {
@@ -952,6 +1331,8 @@ int build_level_layout(fuss_ctx* c, Level& L) {
   long off = 0, noff = 0;
   L.hblk.resize(L.nb);
   L.htiles.clear();
+  L.hctiles.clear();
+  L.hxytiles.clear();
   for (int b = 0; b < L.nb; b++) {
     BlockInfo& B = L.hblk[b];
     const HostBlock& h = L.hb[b];
@@ -965,16 +1346,26 @@ int build_level_layout(fuss_ctx* c, Level& L) {
     for (int f = 0; f < 6; f++) B.fmap_off[f] = -1;
     B.face_active = 0;
     B.tang = 0;
+    B.umat = -1;
     off += B.npad;
     noff += (long)(B.ni + 1) * (B.nj + 1) * (B.nk + 1);
     for (int k = 1; k <= B.nk; k++)
       for (int j = 1; j <= B.nj; j += TILE_Y)
         for (int i = 1; i <= B.ni; i += TILE_X) L.htiles.push_back(Tile{b, i, j, k});
+    for (int k = 1; k <= B.nk; k++)
+      for (int j = 1; j <= B.nj; j += XY_TJ)
+        for (int i = 1; i <= B.ni; i += TILE_X) L.hxytiles.push_back(Tile{b, i, j, k});
+    if (L.kp == GC)
+      for (int k = 1; k <= B.nk; k += KC)
+        for (int j = 1; j <= B.nj; j += TILE_Y)
+          for (int i = 1; i <= B.ni; i += TILE_X) L.hctiles.push_back(Tile{b, i, j, k});
     L.ncell += (long)B.ni * B.nj * B.nk;
   }
   L.npad = off;
   L.nnodes = noff;
   (void)c;
+  // kernels address a level's arrays with 32-bit byte offsets
+  if (L.npad * 8 >= (1L << 32)) return FUSS_EUNSUPPORTED;
   return 0;
 }
 
@@ -993,13 +1384,19 @@ int alloc_level(fuss_ctx* c, Level& L) {
   for (int w = 0; w < 9; w++) e |= L.W[w].alloc(L.npad);
   e |= L.nodes.alloc(3 * L.nnodes);
   e |= L.tiles.alloc(L.htiles.size());
-  e |= L.partials.alloc(std::max<size_t>(L.htiles.size(), 1));
+  e |= L.ctiles.alloc(std::max<size_t>(L.hctiles.size(), 1));
+  e |= L.xytiles.alloc(std::max<size_t>(L.hxytiles.size(), 1));
+  const size_t np = std::max<size_t>(std::max(L.htiles.size(), L.hctiles.size()), 1);
+  e |= L.partials.alloc(np);
+  e |= L.pmin.alloc(np);
+  e |= L.red2.alloc(np / RED_CHUNK + 2);
   (void)c;
   return e ? FUSS_EHIP : 0;
 }
 
 int build_bc(fuss_ctx* c, Level& L) {
   const int n = L.nbound;
+  L.no_ycon = 1;
   // grouping check: (block, face) non-decreasing, as BC_Fluxes assumes
   // (Mod_BC_Fluxes.f90:26-33)
   for (int e = 1; e < n; e++) {
@@ -1070,6 +1467,7 @@ int build_bc(fuss_ctx* c, Level& L) {
       if (e2) return e2;
       // matIDg: the donor cell's material (Ghost_Connection copies it)
     }
+    if (type == 101 && (f == 3 || f == 4)) L.no_ycon = 0;
     const bool active = type == 101 || (type >= 300 && type <= 305);
     if (active) B.face_active |= 1 << (f - 1);
   }
@@ -1078,6 +1476,7 @@ int build_bc(fuss_ctx* c, Level& L) {
   L.any_active = 0;
   for (int b = 0; b < L.nb; b++) L.any_active |= L.hblk[b].face_active;
   int e = L.bc.alloc(std::max(n, 1));
+  e |= L.bcv.alloc(std::max(n, 1));
   e |= L.fmap.alloc(std::max<size_t>(fmap.size(), 1));
   if (e) return FUSS_EHIP;
   if (n) e |= upload(L.bc.p, recs.data(), sizeof(BCRec) * n);
@@ -1093,17 +1492,41 @@ int sync_status(fuss_ctx* c, int* bijk = nullptr) {
   return c->hstatus[0];
 }
 
+// Event pair around one launch of a kernel class (profiling mode only).
+int prof_begin(fuss_ctx* c, int cls) {
+  auto& P = c->prof;
+  if (!P.enabled) return -1;
+  if (2 * (P.used + 1) > P.pool.size()) {
+    for (int q = 0; q < 256; q++) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return -1;
+      P.pool.push_back(e);
+    }
+  }
+  const int id = (int)P.used++;
+  P.cls.resize(P.used);
+  P.cls[id] = cls;
+  (void)hipEventRecord(P.pool[2 * id], c->stream);
+  return id;
+}
+void prof_end(fuss_ctx* c, int id) {
+  if (id >= 0) (void)hipEventRecord(c->prof.pool[2 * id + 1], c->stream);
+}
+
 template <typename R>
 Tab<R> dtab(fuss_ctx* c) {
   Tab<R> t;
   t.cp = c->dcp.p; t.rho = c->drho.p; t.kap = c->dkap.p; t.h = c->dh.p;
+  t.kap2 = c->dkap2.p; t.pr4 = c->dpr4.p;
   t.nT = c->nT; t.Ti1 = c->Ti1; t.Ti2 = c->Ti1 + c->nT - 1;
   return t;
 }
 
 int launch_ghost(fuss_ctx* c, Level& L, double* T) {
   if (!L.nbound) return 0;
+  const int pid = prof_begin(c, FUSS_K_GHOST);
   k_ghost<double><<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(T, L.bc.p, L.nbound);
+  prof_end(c, pid);
   return hipGetLastError() == hipSuccess ? 0 : FUSS_EHIP;
 }
 
@@ -1162,7 +1585,7 @@ int fuss_create(int nlevels, int nblocks, int precision, fuss_ctx** out) {
     return FUSS_EHIP;
   }
   c->own_stream = true;
-  if (c->status.alloc(8) || c->scal.alloc(4) || c->dtmin.alloc(1) ||
+  if (c->status.alloc(8) || c->scal.alloc(4) ||
       hipHostMalloc((void**)&c->hscal, 8 * sizeof(double), 0) != hipSuccess ||
       hipHostMalloc((void**)&c->hstatus, 8 * sizeof(int), 0) != hipSuccess) {
     fuss_destroy(c);
@@ -1187,6 +1610,22 @@ int fuss_set_table(fuss_ctx* c, int nmat, int Ti1, int nT, const double* cp, con
   e |= upload(c->drho.p, rho, n * 8);
   e |= upload(c->dkap.p, kappa, n * 8);
   e |= upload(c->dh.p, h, n * 8);
+  // (v, v(T+1)-v(T)) rows; the last row has no successor and is never used
+  std::vector<double> k2(2 * n, 0.0), p4(4 * n, 0.0);
+  for (int m = 0; m < nmat; m++)
+    for (int t = 0; t < nT; t++) {
+      const size_t q = (size_t)m * nT + t;
+      const bool nx = t + 1 < nT;
+      k2[2 * q] = kappa[q];
+      k2[2 * q + 1] = nx ? kappa[q + 1] - kappa[q] : 0.0;
+      p4[4 * q] = rho[q];
+      p4[4 * q + 1] = nx ? rho[q + 1] - rho[q] : 0.0;
+      p4[4 * q + 2] = cp[q];
+      p4[4 * q + 3] = nx ? cp[q + 1] - cp[q] : 0.0;
+    }
+  if (c->dkap2.alloc(2 * n) || c->dpr4.alloc(4 * n)) return FUSS_EHIP;
+  e |= upload(c->dkap2.p, k2.data(), k2.size() * 8);
+  e |= upload(c->dpr4.p, p4.data(), p4.size() * 8);
   return e;
 }
 
@@ -1277,6 +1716,8 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     e = L.blk.alloc(L.nb);
     if (e) return e;
     e = upload(L.tiles.p, L.htiles.data(), sizeof(Tile) * L.htiles.size());
+    if (!L.hctiles.empty()) e |= upload(L.ctiles.p, L.hctiles.data(), sizeof(Tile) * L.hctiles.size());
+    e |= upload(L.xytiles.p, L.hxytiles.data(), sizeof(Tile) * L.hxytiles.size());
     // nodes
     std::vector<double> nodes(3 * L.nnodes);
     for (int b = 0; b < L.nb; b++)
@@ -1330,6 +1771,7 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     for (int b = 0; b < L.nb; b++) {
       const BlockInfo& B = L.hblk[b];
       const HostBlock& h = L.hb[b];
+      int umat = (int)h.matID[0] - 1;
       for (int k = 1; k <= B.nk; k++)
         for (int j = 1; j <= B.nj; j++)
           for (int i = 1; i <= B.ni; i++) {
@@ -1340,8 +1782,15 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
             const int m = (int)h.matID[s] - 1;
             if (m < 0 || m >= c->nmat) return FUSS_EARG;
             mat[p] = m;
+            if (m != umat) umat = -1;
           }
+      // a block of one material keeps it on every level (restriction copies
+      // a child's material, Lib_Multigrid.f90:113)
+      for (int lv = 0; lv < c->nlev; lv++) c->lev[lv].hblk[b].umat = umat;
     }
+    for (int lv = 0; lv < c->nlev; lv++)
+      HIPCHK(hipMemcpy(c->lev[lv].blk.p, c->lev[lv].hblk.data(), sizeof(BlockInfo) * c->lev[lv].nb,
+                       hipMemcpyHostToDevice));
     int e = upload(L.Tb[L.cur].p, T.data(), T.size() * 8);
     e |= upload(L.qvol.p, q.data(), q.size() * 8);
     e |= upload(L.mat.p, mat.data(), mat.size() * sizeof(int));
@@ -1413,6 +1862,17 @@ void launch_stage_n(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang
   }
 }
 
+void launch_stage3d(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang, bool prim, bool irs) {
+  const unsigned g = (unsigned)L.hctiles.size();
+  if (tang) {
+    if (prim) { irs ? k_stage3d<true, true, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, true, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
+    else { irs ? k_stage3d<true, false, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, false, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
+  } else {
+    if (prim) { irs ? k_stage3d<false, true, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, true, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
+    else { irs ? k_stage3d<false, false, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, false, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
+  }
+}
+
 template <int DIR, int NDIR>
 void launch_irs_d(fuss_ctx* c, Level& L, const IrsArgs<double>& a, bool update, bool prim) {
   const unsigned g = (unsigned)L.htiles.size();
@@ -1442,19 +1902,24 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   L.iter += 1;
   const unsigned ntile = (unsigned)L.htiles.size();
   // Compute_dt + min (Mod_dt.f90:10-84); dtglobal starts at 1e5 (Mod_Explicit.f90:40)
-  const double init = 1e5;
-  unsigned long long bits;
-  memcpy(&bits, &init, 8);
-  k_init_scalars<<<1, 1, 0, c->stream>>>(c->dtmin.p, bits, c->scal.p, L.time);
-  HIPCHK(hipGetLastError());
   DtArgs<double> da;
   da.blk = L.blk.p; da.tiles = L.tiles.p; da.T = L.Tb[L.cur].p; da.mat = L.mat.p; da.invL2 = L.invL2.p;
   da.dt = L.dt.p; da.tab = dtab<double>(c); da.vnn = p->vnn; da.iter = L.iter; da.rampa = p->rampa_iter;
-  da.dtmin = c->dtmin.p; da.status = c->status.p;
+  da.pmin = L.pmin.p; da.status = c->status.p;
+  int pid = prof_begin(c, FUSS_K_DT);
   k_dt<double><<<ntile, TILE_T, 0, c->stream>>>(da);
+  prof_end(c, pid);
   HIPCHK(hipGetLastError());
   // device scalars: [0] dtglobal, [1] time, [2] Sum (T-TO)^2
-  k_dt_finish<<<1, 1, 0, c->stream>>>(c->dtmin.p, c->scal.p, p->time_accurate);
+  pid = prof_begin(c, FUSS_K_REDUCE);
+  if (ntile > (unsigned)RED_CHUNK) {
+    const int nb1 = (int)((ntile + RED_CHUNK - 1) / RED_CHUNK);
+    k_reduce<true><<<nb1, 256, 0, c->stream>>>(L.pmin.p, (int)ntile, L.red2.p);
+    k_dt_final<<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p, L.time, p->time_accurate);
+  } else {
+    k_dt_final<<<1, 256, 0, c->stream>>>(L.pmin.p, (int)ntile, c->scal.p, L.time, p->time_accurate);
+  }
+  prof_end(c, pid);
   HIPCHK(hipGetLastError());
   double sval[MAX_SERIES] = {0};
   if (p->time_accurate) {
@@ -1467,6 +1932,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
                            c->series[s].v.data(), L.time);
   const int o = L.cur, a1 = (L.cur + 1) % 3, a2 = (L.cur + 2) % 3;
   double* in = L.Tb[o].p;
+  int npart = (int)ntile;  // partial sums written by the kernel that applies the last update
   for (int irk = 1; irk <= nrk; irk++) {
     double* out = (irk % 2 == 1) ? L.Tb[a1].p : L.Tb[a2].p;
     int e = launch_ghost(c, L, in);
@@ -1476,13 +1942,27 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     sa.dt = L.dt.p; sa.dtg = c->scal.p; sa.use_dtg = p->time_accurate; sa.qvol = L.qvol.p; sa.vol = L.vol.p;
     sa.mat = L.mat.p;
     for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
-    sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.tab = dtab<double>(c);
+    sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.tab = dtab<double>(c);
     for (int s = 0; s < MAX_SERIES; s++) sa.sval[s] = sval[s];
+    if (L.any_active && L.nbound) {
+      pid = prof_begin(c, FUSS_K_BC);
+      if (L.kp == GC) k_bcflux<3, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound);
+      else k_bcflux<2, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound);
+      prof_end(c, pid);
+      HIPCHK(hipGetLastError());
+    }
     sa.rkc = RKC[nrk - 1][irk - 1];
     sa.last = irk == nrk && p->compute_residual;
     sa.partials = L.partials.p; sa.status = c->status.p;
-    if (c->ndir == 2) launch_stage_n<2>(c, L, sa, L.tang != 0, prim, irs);
-    else launch_stage_n<3>(c, L, sa, L.tang != 0, prim, irs);
+    sa.ctiles = L.ctiles.p;
+    pid = prof_begin(c, FUSS_K_STAGE);
+    if (L.kp == GC) {
+      launch_stage3d(c, L, sa, L.tang != 0, prim, irs);
+      if (!irs) npart = (int)L.hctiles.size();
+    } else {
+      launch_stage_n<2>(c, L, sa, L.tang != 0, prim, irs);
+    }
+    prof_end(c, pid);
     HIPCHK(hipGetLastError());
     if (irs) {
       // Residual_Smoothing (Lib_IRS.f90:14-40): ndir passes R -> R2 -> R ...
@@ -1495,21 +1975,68 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
       }
       ia.T = in; ia.TO = L.Tb[o].p; ia.Tout = out; ia.mat = L.mat.p; ia.tab = dtab<double>(c);
       ia.rkc = sa.rkc; ia.last = sa.last; ia.partials = L.partials.p; ia.status = c->status.p;
-      double* rin = L.R.p;
-      double* rout = L.R2.p;
-      for (int d = 0; d < c->ndir; d++) {
-        ia.Rin = rin; ia.Rout = rout;
-        const bool upd = d == c->ndir - 1;
-        if (c->ndir == 2) launch_irs<2>(c, L, d, ia, upd, prim);
-        else launch_irs<3>(c, L, d, ia, upd, prim);
+      if (L.no_ycon && c->ndir >= 2) {
+        // x and y in one LDS pass; 2-D levels update there, 3-D levels then
+        // march the z pass with the update
+        ia.Rin = L.R.p;
+        ia.Rout = L.R2.p;
+        ia.tiles = L.xytiles.p;
+        const unsigned gxy = (unsigned)L.hxytiles.size();
+        pid = prof_begin(c, c->ndir == 2 ? FUSS_K_IRS_UPDATE : FUSS_K_IRS);
+        if (c->ndir == 2) {
+          prim ? k_irs_xy<2, true, true, double><<<gxy, TILE_T, 0, c->stream>>>(ia)
+               : k_irs_xy<2, true, false, double><<<gxy, TILE_T, 0, c->stream>>>(ia);
+          if (sa.last) npart = (int)gxy;
+        } else {
+          prim ? k_irs_xy<3, false, true, double><<<gxy, TILE_T, 0, c->stream>>>(ia)
+               : k_irs_xy<3, false, false, double><<<gxy, TILE_T, 0, c->stream>>>(ia);
+        }
+        prof_end(c, pid);
         HIPCHK(hipGetLastError());
-        std::swap(rin, rout);
+        if (c->ndir == 3) {
+          ia.Rin = L.R2.p;
+          ia.tiles = L.ctiles.p;
+          pid = prof_begin(c, FUSS_K_IRS_UPDATE);
+          prim ? k_irs_z<true, double><<<(unsigned)L.hctiles.size(), TILE_T, 0, c->stream>>>(ia)
+               : k_irs_z<false, double><<<(unsigned)L.hctiles.size(), TILE_T, 0, c->stream>>>(ia);
+          prof_end(c, pid);
+          HIPCHK(hipGetLastError());
+          if (sa.last) npart = (int)L.hctiles.size();
+        }
+      } else {
+        double* rin = L.R.p;
+        double* rout = L.R2.p;
+        const int nline = c->ndir == 3 ? 2 : c->ndir;  // 3-D: z pass marched below
+        for (int d = 0; d < nline; d++) {
+          ia.Rin = rin; ia.Rout = rout;
+          const bool upd = d == c->ndir - 1;
+          pid = prof_begin(c, upd ? FUSS_K_IRS_UPDATE : FUSS_K_IRS);
+          if (c->ndir == 2) launch_irs<2>(c, L, d, ia, upd, prim);
+          else launch_irs<3>(c, L, d, ia, upd, prim);
+          prof_end(c, pid);
+          HIPCHK(hipGetLastError());
+          std::swap(rin, rout);
+        }
+        if (c->ndir == 3) {
+          ia.Rin = rin;
+          ia.tiles = L.ctiles.p;
+          pid = prof_begin(c, FUSS_K_IRS_UPDATE);
+          prim ? k_irs_z<true, double><<<(unsigned)L.hctiles.size(), TILE_T, 0, c->stream>>>(ia)
+               : k_irs_z<false, double><<<(unsigned)L.hctiles.size(), TILE_T, 0, c->stream>>>(ia);
+          prof_end(c, pid);
+          HIPCHK(hipGetLastError());
+          if (sa.last) npart = (int)L.hctiles.size();
+        }
       }
     }
     in = out;
   }
   if (p->compute_residual) {
-    k_sum<<<1, 256, 0, c->stream>>>(L.partials.p, (int)ntile, c->scal.p + 2);
+    pid = prof_begin(c, FUSS_K_REDUCE);
+    const int nb1 = (npart + RED_CHUNK - 1) / RED_CHUNK;
+    k_reduce<false><<<nb1, 256, 0, c->stream>>>(L.partials.p, npart, L.red2.p);
+    k_reduce<false><<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p + 2);
+    prof_end(c, pid);
     HIPCHK(hipGetLastError());
   }
   // T^{n+1} becomes the new T^n
@@ -1719,6 +2246,35 @@ int fuss_set_stream(fuss_ctx* c, void* s) {
   return 0;
 }
 
+int fuss_profile(fuss_ctx* c, int enable) {
+  if (!c) return FUSS_EARG;
+  c->prof.enabled = enable != 0;
+  c->prof.used = 0;
+  c->prof.cls.clear();
+  return 0;
+}
+
+int fuss_profile_read(fuss_ctx* c, int nclass, double* ms, int* count) {
+  if (!c || nclass < 1 || !ms || !count) return FUSS_EARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int q = 0; q < nclass; q++) {
+    ms[q] = 0.0;
+    count[q] = 0;
+  }
+  auto& P = c->prof;
+  for (size_t id = 0; id < P.used; id++) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, P.pool[2 * id], P.pool[2 * id + 1]));
+    if (P.cls[id] < nclass) {
+      ms[P.cls[id]] += t;
+      count[P.cls[id]] += 1;
+    }
+  }
+  P.used = 0;
+  P.cls.clear();
+  return 0;
+}
+
 int fuss_memory(fuss_ctx* c, double* bytes) {
   if (!c || !bytes) return FUSS_EARG;
   double s = 0;
@@ -1740,12 +2296,13 @@ void fuss_destroy(fuss_ctx* c) {
     L.M.free();
     for (auto& f : L.F) f.free();
     for (auto& w : L.W) w.free();
-    L.bc.free(); L.fmap.free(); L.partials.free();
+    L.bc.free(); L.bcv.free(); L.fmap.free(); L.xytiles.free(); L.partials.free(); L.pmin.free(); L.red2.free(); L.ctiles.free();
   }
-  c->dcp.free(); c->drho.free(); c->dkap.free(); c->dh.free();
-  c->status.free(); c->scal.free(); c->dtmin.free();
+  c->dcp.free(); c->drho.free(); c->dkap.free(); c->dh.free(); c->dkap2.free(); c->dpr4.free();
+  c->status.free(); c->scal.free();
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hstatus) (void)hipHostFree(c->hstatus);
+  for (auto e : c->prof.pool) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -32,6 +32,8 @@ struct BlockInfo {
   int fmap_off[6];   // face-map offsets, -1 when the face has no entries
   int face_active;   // bit f-1 set when face f has flux-producing entries
   int tang;          // nonzero when any tangential weight is nonzero
+  int umat;          // material index when the whole block has one material, else -1
+  int pad_;
 };
 
 struct Tile {
@@ -62,8 +64,42 @@ struct Tab {
   const R* rho;
   const R* kap;
   const R* h;
+  // the same tables as (v(T), v(T+1) - v(T)) rows: one vector load per lookup
+  const R* kap2;       // 2 per row
+  const R* pr4;        // rho, drho, cp, dcp per row
   int nT, Ti1, Ti2;
 };
+
+template <typename R> struct Vec2;
+template <> struct Vec2<double> { using T = double2; };
+template <> struct Vec2<float> { using T = float2; };
+template <typename R> struct Vec4;
+template <> struct Vec4<double> { using T = double4; };
+template <> struct Vec4<float> { using T = float4; };
+
+// 32-bit element offsets from a uniform base: lets the compiler use the
+// SGPR-base + 32-bit VGPR-offset form of global loads and stores.
+template <typename T>
+__device__ __forceinline__ T ldu(const T* __restrict__ base, unsigned idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void stu(T* __restrict__ base, unsigned idx, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T)) = v;
+}
+
+// Interpolation from a (v, dv) row: v(Tm) + (T - Tm) * (v(Tm+1) - v(Tm)),
+// the same operations as tab_interp (the difference is formed once at setup).
+template <typename R>
+__device__ __forceinline__ R tab_interp2(const R* __restrict__ t2, R T, int Ti1, int Ti2, int* bad) {
+  if (!(T >= (R)Ti1 && T < (R)Ti2)) {
+    *bad = 1;
+    T = T < (R)Ti1 || T != T ? (R)Ti1 : (R)(Ti2 - 1);
+  }
+  const int Tm = (int)T;
+  const typename Vec2<R>::T v = ldu(reinterpret_cast<const typename Vec2<R>::T*>(t2), (unsigned)(Tm - Ti1));
+  return v.x + (T - (R)Tm) * v.y;
+}
 
 __device__ __forceinline__ void flag_error(int* st, int code, int b, int i, int j, int k) {
   if (atomicCAS(st, 0, code) == 0) {

@@ -149,6 +149,21 @@ int fuss_get_metrics(fuss_ctx *ctx, int level, int b, double *M, double *vol, in
  * caller-provided stream to use instead (borrowed, not destroyed). */
 int fuss_get_stream(fuss_ctx *ctx, void **stream);
 int fuss_set_stream(fuss_ctx *ctx, void *stream);
+/* Per-kernel timing (no reference counterpart: replaces the reference's
+ * Cpu_Time bracket, src/lib/driver/Wrap_Setup.f90:164, with HIP events on the
+ * context stream).  fuss_profile(ctx, 1) starts recording an event pair
+ * around every launch; fuss_profile_read synchronizes the stream and returns
+ * the summed milliseconds and launch count per kernel class FUSS_K_*. */
+#define FUSS_K_DT 0          /* local dt + global min */
+#define FUSS_K_GHOST 1       /* ghost fill */
+#define FUSS_K_STAGE 2       /* fused fluxes + BC fluxes + newstate */
+#define FUSS_K_IRS 3         /* IRS line pass */
+#define FUSS_K_IRS_UPDATE 4  /* last IRS line pass fused with the RK update */
+#define FUSS_K_REDUCE 5      /* dt minimum and residual sum */
+#define FUSS_K_BC 6          /* boundary-face fluxes (surface pass) */
+#define FUSS_K_NCLASS 8
+int fuss_profile(fuss_ctx *ctx, int enable);
+int fuss_profile_read(fuss_ctx *ctx, int nclass, double *ms, int *count);
 /* Device memory held by the context, bytes. */
 int fuss_memory(fuss_ctx *ctx, double *bytes);
 const char *fuss_strerror(int code);
