@@ -33,12 +33,31 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the stage kernel measured with rocprofv3 PMC
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB device copy);
+# source: profiles/explicit_step_cube256_fp64.md (256^3 cube, FP64).
+STAGE_TRAFFIC_MEASURED = {"k_stage3d<false, true, true, double>": 1023e6 + 142e6}
 
 
 def device_sync():
     hip = ctypes.CDLL("libamdhip64.so")
     if hip.hipDeviceSynchronize() != 0:
         raise RuntimeError("hipDeviceSynchronize failed")
+
+
+def calibration_copy(nbytes=1 << 30, reps=3):
+    """Device-to-device copies of a known size: the reference kernel for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def cpu_baseline(n, iters):
@@ -67,6 +86,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true", help="also run 1 GiB D2D copies (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
@@ -123,7 +143,8 @@ def main():
                                "finest-level Explicit_Step",
                    "cells": ncell, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": STAGE_TRAFFIC_MEASURED["k_stage3d<false, true, true, double>"] if args.n == 256 else None,
                      "kernel": "k_stage (fused interior+BC fluxes, scaled residual)",
                      "bytes_per_cell": perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True),
                      "ms_avg": stage_ms},
@@ -131,6 +152,8 @@ def main():
         "iteration_bytes_per_cell": perfmodel.iteration_bytes(3, 3, False, True, True),
     }
     s.close()
+    if args.calib_copy:
+        calibration_copy()
     if not args.no_cpu_baseline:
         print(f"[bench] cpu baseline: oracle {args.cpu_n}^3 x {args.cpu_iters}", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_iters)

@@ -27,6 +27,33 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+def flang():
+    for cand in (os.environ.get("FLANG"), shutil.which("flang"), "/opt/rocm/lib/llvm/bin/flang"):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+DEMO = os.path.join(FORTRAN, "fuss_gpu_demo")
+
+
+def build_fortran(verbose=False):
+    """Compile the iso_c_binding module and the Fortran driver against the
+    library (skipped when no Fortran compiler is installed)."""
+    fc = flang()
+    if fc is None:
+        return None
+    bdir = os.path.join(FORTRAN, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    cmd = [fc, "-O2", "-J", bdir, os.path.join(FORTRAN, "fuss_gpu_m.f90"), os.path.join(FORTRAN, "fuss_gpu_demo.f90"),
+           "-L", HERE, "-lfuss_gpu", "-Wl,-rpath,$ORIGIN/..", "-o", DEMO]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=bdir)
+    return DEMO
+
+
 def build(verbose=False):
     """Compile fuss_amd/csrc/*.hip into fuss_amd/libfuss_gpu.so (always rebuilds)."""
     srcs = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")]
@@ -41,3 +68,4 @@ def build(verbose=False):
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))
