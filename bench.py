@@ -3,12 +3,20 @@
 
 Workload: the steady 3-D cube at 256^3 cells (0.1 m, seeded smooth per-axis
 stretching, 32-knot k(T) table resampled to 1 K rows, qvol = 1e6 W/m^3,
-convective 303 walls h = 50 W/m^2K, T_inf = 300 K), SSP-RK3 with local time
-stepping, implicit residual smoothing beta = 0.5 (2 Jacobi sweeps per
+convective 303 walls h = 50 W/m^2K, T_inf = 300 K), four-stage RK with local
+time stepping, implicit residual smoothing beta = 0.5 (2 Jacobi sweeps per
 direction), 3 multigrid levels allocated.  One step = one Explicit_Step on
-the finest level: local dt + global min, 3 x (ghost fill, fused
+the finest level: local dt + global min, 4 x (ghost fill, fused
 flux/BC/newstate stage, 3 IRS line passes), residual sum -- inputs resident
 in HBM.  value = interior cells x steps / wall time (cell-updates/s).
+
+Time to residual 1e-10 (||T^{n+1}-T^n||_2 of an FP64 Explicit_Step, the
+reference's stopping test): measured for the FP64 defect correction with
+FP32 inner sweeps (fuss_refine) and with FP64 inner sweeps, from the initial
+state T = 300 K.  The plain FP64 path needs ~3e6 iterations at 256^3 (hours),
+so its time is the measured time per iteration x an iteration count
+extrapolated from plain runs to 1e-10 at 32^3, 48^3, 64^3 (table below; the
+32^3 count is re-measured live unless --plain-conv-n 0).
 
 Also reported: the fused stage kernel's achieved bandwidth (algorithmic
 bytes per launch / average launch time from HIP events on the library's
@@ -37,6 +45,11 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB device copy);
 # source: profiles/explicit_step_cube256_fp64.md (256^3 cube, FP64).
 STAGE_TRAFFIC_MEASURED = {"k_stage3d<false, true, true, double>": 1023e6 + 142e6}
+# Plain FP64 path (four-stage RK + LTS + IRS 0.5, 3-level sequencing) run to
+# residual 1e-10 on the GPU: iterations per cube size (profiles/
+# steady_convergence_cube.md).  Fit: iters = 230218 (n/64)^PLAIN_EXP.
+PLAIN_ITERS_MEASURED = {32: 64506, 48: 135672, 64: 230218}
+PLAIN_EXP = 1.835
 
 
 def device_sync():
@@ -66,15 +79,71 @@ def cpu_baseline(n, iters):
     from fuss_amd import cases
     from oracle import pyoracle
 
-    c = cases.cube3d(n=n, mgl=1)
+    c = cases.cube3d(n=n, mgl=1, n_rk=4)
     o = pyoracle.Oracle(c)
     t0 = time.perf_counter()
     for _ in range(iters):
         o.step(1)
     dt = time.perf_counter() - t0
     return {"value": c.ncell * iters / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (scalar C, FP64), {iters} iteration(s) of the {n}^3 cube, RK3+IRS, "
+            "sample": f"oracle (scalar C, FP64), {iters} iteration(s) of the {n}^3 cube, RK4+IRS, "
                       f"{dt:.1f} s"}
+
+
+def time_to_converge(s, case, args, ms_step):
+    """Wall time to the reference's residual 1e-10 from T = 300 K: defect
+    correction with FP32 and with FP64 inner sweeps (measured), plain FP64
+    (measured time per iteration x extrapolated iteration count)."""
+    from fuss_amd import cases, gpu
+
+    T0 = case.blocks[0].T
+    out = {"threshold": 1e-10, "from": "initial state T = 300 K"}
+    finals = {}
+    for prec, name in ((gpu.FUSS_FP32, "fp32_refined"), (gpu.FUSS_FP64, "fp64_refined")):
+        rp = gpu.refine_params(precision=prec, max_outer=60, res_threshold=1e-10)
+        gpu._chk(s.L.fuss_refine_setup(s.h, gpu.C.byref(rp)), "fuss_refine_setup")
+        s.set_T(T0)
+        s.refine(rp, setup=False)        # warm-up (module load)
+        s.set_T(T0)
+        device_sync()
+        t0 = time.perf_counter()
+        r, hist = s.refine(rp, setup=False)
+        device_sync()
+        dt = time.perf_counter() - t0
+        finals[name] = s.get("T")
+        out[name] = {"seconds": dt, "converged": bool(r.converged), "outer": r.outer, "vcycles": r.vcycles,
+                     "levels": r.levels, "final_residual": r.res, "defect_W": r.defect,
+                     "residual_history": [float(x) for x in hist]}
+        print(f"[bench] {name}: {dt*1e3:.1f} ms, {r.outer} outer, residual {r.res:.3e}", file=sys.stderr,
+              flush=True)
+    a, b = finals["fp32_refined"], finals["fp64_refined"]
+    out["fp32_vs_fp64_refined_max_rel"] = float(np.max(np.abs(a - b) / np.abs(b)))
+    iters = PLAIN_ITERS_MEASURED[64] * (args.n / 64.0) ** PLAIN_EXP
+    plain_s = iters * ms_step * 1e-3
+    out["fp64_plain"] = {"seconds_est": plain_s, "iterations_est": iters, "ms_per_iteration": ms_step,
+                         "iterations_measured": PLAIN_ITERS_MEASURED,
+                         "fit": f"230218 (n/64)^{PLAIN_EXP}"}
+    out["speedup_fp32_refined_vs_plain"] = plain_s / out["fp32_refined"]["seconds"]
+    out["speedup_fp32_vs_fp64_refined"] = out["fp64_refined"]["seconds"] / out["fp32_refined"]["seconds"]
+    if args.plain_conv_n:
+        # live check at a small size: plain FP64 to 1e-10 vs the refined solution
+        m = args.plain_conv_n
+        c = cases.cube3d(n=m, mgl=3, n_rk=args.rk)
+        sp = gpu.Solver(c)
+        nit, hist = sp.solve()
+        Tp = sp.get("T")
+        sp.close()
+        c1 = cases.cube3d(n=m, mgl=1, n_rk=args.rk)
+        sr = gpu.Solver(c1)
+        r, _ = sr.refine(gpu.refine_params(precision=gpu.FUSS_FP32, res_threshold=1e-10))
+        Tr = sr.get("T")
+        sr.close()
+        out["live_check"] = {"n": m, "plain_iterations": int(nit), "plain_final_residual": float(hist[-1, 2]),
+                             "fp32_refined_outer": r.outer,
+                             "fp32_refined_vs_plain_max_rel": float(np.max(np.abs(Tr - Tp) / np.abs(Tp)))}
+        print(f"[bench] live check {m}^3: plain {nit} iterations; refined vs plain max rel "
+              f"{out['live_check']['fp32_refined_vs_plain_max_rel']:.2e}", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -87,6 +156,9 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="also run 1 GiB D2D copies (counter calibration)")
+    ap.add_argument("--rk", type=int, default=4)
+    ap.add_argument("--no-refine", action="store_true", help="skip the time-to-1e-10 runs")
+    ap.add_argument("--plain-conv-n", type=int, default=32, help="cube size of the live plain run to 1e-10 (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
@@ -96,7 +168,7 @@ def main():
     from fuss_amd import cases, gpu, perfmodel
 
     print(f"[bench] building the {args.n}^3 cube", file=sys.stderr, flush=True)
-    case = cases.cube3d(n=args.n, mgl=3)
+    case = cases.cube3d(n=args.n, mgl=3, n_rk=args.rk)
     t0 = time.perf_counter()
     s = gpu.Solver(case)
     print(f"[bench] setup {time.perf_counter()-t0:.1f} s", file=sys.stderr, flush=True)
@@ -138,7 +210,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"steady cube {args.n}^3, plain FP64, RK3 + LTS + IRS(jacobi2, 0.5), "
+        "config": {"workload": f"steady cube {args.n}^3, plain FP64, RK{args.rk} + LTS + IRS(jacobi2, 0.5), "
                                "qvol 1e6, 303 h=50, k(T) 32-knot table, 3 MG levels; step = one "
                                "finest-level Explicit_Step",
                    "cells": ncell, "parallelism": "single-gpu"},
@@ -149,8 +221,11 @@ def main():
                      "bytes_per_cell": perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True),
                      "ms_avg": stage_ms},
         "kernels": kern,
-        "iteration_bytes_per_cell": perfmodel.iteration_bytes(3, 3, False, True, True),
+        "iteration_bytes_per_cell": perfmodel.iteration_bytes(args.rk, 3, False, True, True),
     }
+    ms_step = wall / args.steps * 1e3
+    if not args.no_refine:
+        out["time_to_1e-10"] = time_to_converge(s, case, args, ms_step)
     s.close()
     if args.calib_copy:
         calibration_copy()

@@ -35,7 +35,7 @@ class Block:
 class Params:
     """Numerics of the INI file (src/lib/config/Register_Numerics.f90:30-48,
     Register_Sim_Param.f90:121-124, Register_IO.f90:31)."""
-    n_rk: int = 1                 # euler=1, RK2=2, RK3=3
+    n_rk: int = 1                 # euler=1, RK2=2, RK3=3, RK4=4 (Jameson four-stage, no reference counterpart)
     vnn: float = 0.3
     rampa_iter: int = 0           # vnn-rise-threshold
     time_accurate: bool = False

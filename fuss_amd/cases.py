@@ -300,7 +300,7 @@ def write_case_dir(case, path, ini_extra=""):
         for m in range(case.table.nmat):
             fh.write(f"mat{m+1} 1\n")
     p = case.params
-    scheme = {1: "euler", 2: "RK2", 3: "RK3"}[p.n_rk]
+    scheme = {1: "euler", 2: "RK2", 3: "RK3", 4: "RK4"}[p.n_rk]
     with open(os.path.join(path, "input.ini"), "w") as fh:
         fh.write("[FUSS-Parameters]\nnewrun = true\n")
         fh.write(f"res-threshold = {p.res_threshold!r}\ntime-threshold = {p.time_threshold!r}\n")

@@ -265,6 +265,7 @@ struct StageArgs {
   Tab<R> tab;
   double sval[MAX_SERIES];
   R rkc;
+  int jam;             // four-stage Jameson update (n_rk = 4)
   int last;
   double* partials;
   int* status;
@@ -493,10 +494,13 @@ __global__ void k_bcflux(StageArgs<R> a, int n) {
   a.bcv[e] = bc_value<NDIR>(a, B, r, r.cell);
 }
 
-// RK_stage, Lib_RK.f90:15-26
+// RK_stage, Lib_RK.f90:15-26.  jam: the four-stage scheme of Jameson,
+// Schmidt and Turkel (new = old + a_k res, a = 1/4, 1/3, 1/2, 1), which has
+// no counterpart in the reference (its RKcoeff stops at three stages).
 template <typename R>
-__device__ __forceinline__ R rk_stage(R state, R old, R res, R c) {
+__device__ __forceinline__ R rk_stage(R state, R old, R res, R c, int jam) {
 #pragma clang fp contract(off)
+  if (jam) return old + res * c;
   const R r_ = (state - old + res) * c;
   return old + r_;
 }
@@ -504,13 +508,13 @@ __device__ __forceinline__ R rk_stage(R state, R old, R res, R c) {
 // Newstate for one cell from its scaled residual.  PRIM: Newstate_Primitive
 // (Lib_Newstate.f90:241-270); else Newstate_Conservative (:99-146).
 template <bool PRIM, typename R>
-__device__ __forceinline__ R new_state(const Tab<R>& tab, int m, R T, R TO, R res, R c, int* bad) {
+__device__ __forceinline__ R new_state(const Tab<R>& tab, int m, R T, R TO, R res, R c, int jam, int* bad) {
 #pragma clang fp contract(off)
-  if (PRIM) return rk_stage(T, TO, res, c);
+  if (PRIM) return rk_stage(T, TO, res, c, jam);
   const long row = (long)m * tab.nT;
   const R hO = tab_interp(tab.h + row, TO, tab.Ti1, tab.Ti2, bad);
   R h = tab_interp(tab.h + row, T, tab.Ti1, tab.Ti2, bad);
-  h = rk_stage(h, hO, res, c);
+  h = rk_stage(h, hO, res, c, jam);
   return tab_inverse(tab.h + row, h, T, tab.Ti1, tab.Ti2, bad);
 }
 
@@ -582,7 +586,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage(StageArgs<R> a) {
       a.Rres[p] = res;
     } else {
       const R TO = a.TO[p];
-      const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, &bad);
+      const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, a.jam, &bad);
       a.Tout[p] = Tn;
       // Check_And_Fix_State, Lib_Newstate.f90:278-290
       if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
@@ -774,7 +778,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
         stu(a.Rres, p, res);
       } else {
         const R TO = ldu(a.TO, p);
-        const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, &bad);
+        const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, a.jam, &bad);
         stu(a.Tout, p, Tn);
         if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
         const double d = (double)Tn - (double)TO;
@@ -810,6 +814,7 @@ struct IrsArgs {
   const int* mat;
   Tab<R> tab;
   R rkc;
+  int jam;
   int last;
   double* partials;
   int* status;
@@ -862,7 +867,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs(IrsArgs<R> a) {
     } else {
       const R T = a.T[p], TO = a.TO[p];
       int bad = 0;
-      const R Tn = new_state<PRIM>(a.tab, a.mat[p], T, TO, rs, a.rkc, &bad);
+      const R Tn = new_state<PRIM>(a.tab, a.mat[p], T, TO, rs, a.rkc, a.jam, &bad);
       a.Tout[p] = Tn;
       if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
       if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
@@ -954,7 +959,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xy(IrsArgs<R> a) {
       const R T = ldu(a.T, p), TO = ldu(a.TO, p);
       int bad = 0;
       const int m = B.umat >= 0 ? B.umat : ldu(a.mat, p);
-      const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, &bad);
+      const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
       stu(a.Tout, p, Tn);
       if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
       if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
@@ -988,7 +993,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
       const unsigned p = p1 + (k - 1) * sz;
       const R T = ldu(a.T, p), TO = ldu(a.TO, p);
       int bad = 0;
-      const R Tn = new_state<PRIM>(a.tab, m >= 0 ? m : ldu(a.mat, p), T, TO, rs, a.rkc, &bad);
+      const R Tn = new_state<PRIM>(a.tab, m >= 0 ? m : ldu(a.mat, p), T, TO, rs, a.rkc, a.jam, &bad);
       stu(a.Tout, p, Tn);
       if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
       if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
@@ -1144,6 +1149,8 @@ __global__ void k_prolong(MGArgs a) {
       }
 }
 
+#include "fuss_refine.h"
+
 // ------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------
@@ -1204,6 +1211,24 @@ struct Level {
   DBuf<double> partials, pmin, red2;
 };
 
+// One level of the correction multigrid (arrays in the inner precision).
+struct CLevel {
+  int ncell = 0, maxcells = 0;
+  std::vector<CBlk> hblk;
+  DBuf<CBlk> blk;
+  DBuf<unsigned char> mem;
+  void* d[3] = {nullptr, nullptr, nullptr};
+  void *b = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *D = nullptr, *s = nullptr, *s2 = nullptr;
+  int cur = 0;
+};
+
+struct Refine {
+  int ready = 0;
+  fuss_refine_params p{};
+  std::vector<CLevel> lv;
+  DBuf<double> bcd, red;
+};
+
 }  // namespace
 
 struct fuss_ctx {
@@ -1221,6 +1246,7 @@ struct fuss_ctx {
   double* hscal = nullptr;          // pinned readback
   int setup_done = 0;
   int* hstatus = nullptr;
+  Refine ref;                       // steady defect correction (fuss_refine)
   // per-kernel-class timing with HIP events on the context stream
   struct Prof {
     int enabled = 0;
@@ -1889,13 +1915,14 @@ void launch_irs(fuss_ctx* c, Level& L, int dir, const IrsArgs<double>& a, bool u
 }
 
 const double RKC[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.50, 0.0}, {1.0, 0.25, 2.0 / 3.0}};
+const double RK4A[4] = {0.25, 1.0 / 3.0, 0.5, 1.0};
 
 // Explicit_Step on one level, enqueued on the stream (no host sync except
 // the dtglobal readback of time-accurate runs with time-dependent BCs).
 int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   Level& L = c->lev[level - 1];
   const int nrk = p->n_rk;
-  if (nrk < 1 || nrk > 3) return FUSS_EARG;
+  if (nrk < 1 || nrk > 4) return FUSS_EARG;
   const bool prim = p->integration == FUSS_PRIM;
   const bool irs = p->irs_beta > 0.0;
   if (irs && p->irs_mode != FUSS_IRS_JACOBI2) return FUSS_EUNSUPPORTED;
@@ -1951,7 +1978,8 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
       prof_end(c, pid);
       HIPCHK(hipGetLastError());
     }
-    sa.rkc = RKC[nrk - 1][irk - 1];
+    sa.rkc = nrk == 4 ? RK4A[irk - 1] : RKC[nrk - 1][irk - 1];
+    sa.jam = nrk == 4;
     sa.last = irk == nrk && p->compute_residual;
     sa.partials = L.partials.p; sa.status = c->status.p;
     sa.ctiles = L.ctiles.p;
@@ -1974,7 +2002,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
         ia.den = 1.0 + 2.0 * p->irs_beta;
       }
       ia.T = in; ia.TO = L.Tb[o].p; ia.Tout = out; ia.mat = L.mat.p; ia.tab = dtab<double>(c);
-      ia.rkc = sa.rkc; ia.last = sa.last; ia.partials = L.partials.p; ia.status = c->status.p;
+      ia.rkc = sa.rkc; ia.jam = sa.jam; ia.last = sa.last; ia.partials = L.partials.p; ia.status = c->status.p;
       if (L.no_ycon && c->ndir >= 2) {
         // x and y in one LDS pass; 2-D levels update there, 3-D levels then
         // march the z pass with the update
@@ -2044,7 +2072,296 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   return 0;
 }
 
+// ------------------------------------------------------------------------
+// steady defect correction (fuss_refine)
+// ------------------------------------------------------------------------
+
+// Stage coefficients a_1..a_m of the multistage smoother whose polynomial is
+// the Chebyshev one on [lo, hi]: P(z) = prod_i (1 + z / l_i)
+// = 1 + a_m z (1 + a_{m-1} z (... (1 + a_1 z))), z = -lambda.
+std::vector<double> cheb_stages(int m, double lo, double hi) {
+  std::vector<double> cf(1, 1.0);
+  for (int i = 1; i <= m; i++) {
+    const double l = 0.5 * (hi + lo) + 0.5 * (hi - lo) * cos(M_PI * (2 * i - 1) / (2.0 * m));
+    std::vector<double> nc(cf.size() + 1, 0.0);
+    for (size_t q = 0; q < cf.size(); q++) {
+      nc[q] += cf[q];
+      nc[q + 1] += cf[q] / l;
+    }
+    cf.swap(nc);
+  }
+  std::vector<double> a(m);
+  for (int q = 1; q <= m; q++) a[m - q] = cf[q] / cf[q - 1];
+  return a;
+}
+
+void refine_free(Refine& R) {
+  for (auto& l : R.lv) {
+    l.blk.free();
+    l.mem.free();
+  }
+  R.lv.clear();
+  R.bcd.free();
+  R.red.free();
+  R.ready = 0;
+}
+
+int refine_build(fuss_ctx* c, const fuss_refine_params* rp) {
+  Refine& R = c->ref;
+  Level& L = c->lev[0];
+  for (int e = 0; e < L.nbound; e++) {
+    const int t = L.hib[15 * e + 5];
+    if (t == 101 || t == 102) return FUSS_EUNSUPPORTED;
+  }
+  if (rp->precision != FUSS_FP32 && rp->precision != FUSS_FP64) return FUSS_EARG;
+  if (rp->n_stage < 1 || rp->n_stage > 8 || rp->nu1 < 0 || rp->nu2 < 0 || rp->vcycles < 1 || rp->max_outer < 0 ||
+      rp->max_levels < 0 || rp->ncoarse < 0 || !(rp->lam_lo > 0.0) || !(rp->lam_hi > rp->lam_lo) ||
+      !(rp->irs_beta >= 0.0))
+    return FUSS_EARG;
+  refine_free(R);
+  const size_t es = rp->precision == FUSS_FP32 ? 4 : 8;
+  const int nb = L.nb;
+  std::vector<CBlk> prev;
+  for (int l = 0;; l++) {
+    CLevel cl;
+    cl.hblk.resize(nb);
+    long off = 0;
+    bool coarsened = false;
+    for (int b = 0; b < nb; b++) {
+      CBlk& B = cl.hblk[b];
+      B = CBlk{};
+      const int fn[3] = {l ? prev[b].n[0] : L.hblk[b].ni, l ? prev[b].n[1] : L.hblk[b].nj,
+                         l ? prev[b].n[2] : L.hblk[b].nk};
+      for (int d = 0; d < 3; d++) {
+        B.r[d] = (l > 0 && fn[d] > 2) ? 2 : 1;
+        if (B.r[d] == 2) coarsened = true;
+        B.n[d] = (fn[d] + B.r[d] - 1) / B.r[d];
+        B.nf[d] = fn[d];
+      }
+      B.off = (int)off;
+      B.foff = l ? prev[b].off : 0;
+      const long nc = (long)B.n[0] * B.n[1] * B.n[2];
+      cl.maxcells = (int)std::max<long>(cl.maxcells, nc);
+      off += nc;
+    }
+    if (l > 0 && !coarsened) break;
+    if (off >= (1L << 31) / 8) return FUSS_EUNSUPPORTED;
+    cl.ncell = (int)off;
+    prev = cl.hblk;
+    R.lv.push_back(std::move(cl));
+    if (rp->max_levels > 0 && (int)R.lv.size() >= rp->max_levels) break;
+  }
+  for (auto& cl : R.lv) {
+    const size_t one = ((size_t)cl.ncell * es + 255) / 256 * 256;
+    if (cl.mem.alloc(10 * one)) return FUSS_EHIP;
+    unsigned char* base = cl.mem.p;
+    void** slots[10] = {&cl.d[0], &cl.d[1], &cl.d[2], &cl.b, &cl.cx, &cl.cy, &cl.cz, &cl.D, &cl.s, &cl.s2};
+    for (int q = 0; q < 10; q++) *slots[q] = base + q * one;
+    if (cl.blk.alloc(cl.hblk.size())) return FUSS_EHIP;
+    HIPCHK(hipMemcpy(cl.blk.p, cl.hblk.data(), cl.hblk.size() * sizeof(CBlk), hipMemcpyHostToDevice));
+    cl.cur = 0;
+  }
+  if (R.bcd.alloc(std::max(L.nbound, 1)) || R.red.alloc(2)) return FUSS_EHIP;
+  R.p = *rp;
+  R.ready = 1;
+  return 0;
+}
+
+template <typename S>
+COp<S> cop(const CLevel& l) {
+  COp<S> o;
+  o.blk = l.blk.p;
+  o.nb = (int)l.hblk.size();
+  o.ncell = l.ncell;
+  o.cx = (const S*)l.cx;
+  o.cy = (const S*)l.cy;
+  o.cz = (const S*)l.cz;
+  o.D = (const S*)l.D;
+  o.b = (const S*)l.b;
+  return o;
+}
+
+inline unsigned cgrid(int n) { return (unsigned)((n + 255) / 256); }
+
+// nu iterations of the multistage smoother on one correction level.
+template <typename S>
+int cmg_smooth(fuss_ctx* c, CLevel& l, int nu, const std::vector<double>& al, double beta) {
+  const COp<S> o = cop<S>(l);
+  const unsigned gs = cgrid(l.ncell);
+  const S bt = (S)beta;
+  S den;
+  {
+#pragma clang fp contract(off)
+    den = (S)(1.0 + 2.0 * beta);
+  }
+  const int m = (int)al.size();
+  for (int it = 0; it < nu; it++) {
+    const int base = l.cur, x = (base + 1) % 3, y = (base + 2) % 3;
+    const S* d0 = (const S*)l.d[base];
+    const S* din = d0;
+    for (int st = 0; st < m; st++) {
+      S* out = (S*)l.d[st % 2 == 0 ? x : y];
+      const S a = (S)al[st];
+      if (beta <= 0.0) {
+        k_cmg_stage<S, false><<<gs, 256, 0, c->stream>>>(o, din, d0, out, a);
+      } else {
+        S* s1 = (S*)l.s;
+        S* s2 = (S*)l.s2;
+        k_cmg_stage<S, true><<<gs, 256, 0, c->stream>>>(o, din, nullptr, s1, (S)0);
+        if (c->ndir == 3) {
+          k_cmg_irs<0, false, S><<<gs, 256, 0, c->stream>>>(o, s1, s2, bt, den, nullptr, (S)0);
+          k_cmg_irs<1, false, S><<<gs, 256, 0, c->stream>>>(o, s2, s1, bt, den, nullptr, (S)0);
+          k_cmg_irs<2, true, S><<<gs, 256, 0, c->stream>>>(o, s1, out, bt, den, d0, a);
+        } else {
+          k_cmg_irs<0, false, S><<<gs, 256, 0, c->stream>>>(o, s1, s2, bt, den, nullptr, (S)0);
+          k_cmg_irs<1, true, S><<<gs, 256, 0, c->stream>>>(o, s2, out, bt, den, d0, a);
+        }
+      }
+      HIPCHK(hipGetLastError());
+      din = out;
+    }
+    l.cur = (m % 2 == 1) ? x : y;
+  }
+  return 0;
+}
+
+template <typename S>
+int cmg_vcycle(fuss_ctx* c, int l, const std::vector<double>& al) {
+  Refine& R = c->ref;
+  CLevel& L = R.lv[l];
+  const int nl = (int)R.lv.size();
+  if (l == nl - 1) {
+    if (L.maxcells <= CMG_DIRECT) {
+      const int nb = (int)L.hblk.size();
+      k_cmg_direct<S><<<(nb + 63) / 64, 64, 0, c->stream>>>(cop<S>(L), (S*)L.d[L.cur]);
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
+    return cmg_smooth<S>(c, L, R.p.ncoarse, al, R.p.irs_beta);
+  }
+  int e = cmg_smooth<S>(c, L, R.p.nu1, al, R.p.irs_beta);
+  if (e) return e;
+  CLevel& C = R.lv[l + 1];
+  k_cmg_restrict<S><<<cgrid(C.ncell), 256, 0, c->stream>>>(cop<S>(L), (const S*)L.d[L.cur], cop<S>(C), (S*)C.b,
+                                                          (S*)C.d[C.cur]);
+  HIPCHK(hipGetLastError());
+  e = cmg_vcycle<S>(c, l + 1, al);
+  if (e) return e;
+  k_cmg_prolong<S><<<cgrid(L.ncell), 256, 0, c->stream>>>(cop<S>(L), (S*)L.d[L.cur], cop<S>(C),
+                                                         (const S*)C.d[C.cur]);
+  HIPCHK(hipGetLastError());
+  return cmg_smooth<S>(c, L, R.p.nu2, al, R.p.irs_beta);
+}
+
+template <int NDIR, bool TANG, typename S>
+void launch_defect(fuss_ctx* c, Level& L, const StageArgs<double>& sa, CLevel& C0) {
+  k_defect<NDIR, TANG, S><<<(unsigned)L.htiles.size(), TILE_T, 0, c->stream>>>(
+      sa, c->ref.bcd.p, C0.blk.p, (S*)C0.b, (S*)C0.cx, (S*)C0.cy, (S*)C0.cz, (S*)C0.D, L.partials.p);
+}
+
+template <typename S>
+int refine_run(fuss_ctx* c, const fuss_step_params* mon, fuss_refine_result* out, double* hist) {
+  Refine& R = c->ref;
+  Level& L = c->lev[0];
+  const fuss_refine_params& rp = R.p;
+  const std::vector<double> al = cheb_stages(rp.n_stage, rp.lam_lo, rp.lam_hi);
+  fuss_step_params pm = *mon;
+  pm.compute_residual = 1;
+  CLevel& C0 = R.lv[0];
+  const int nl = (int)R.lv.size();
+  const unsigned ntile = (unsigned)L.htiles.size();
+  *out = fuss_refine_result{};
+  out->levels = nl;
+  bool have_def = false;
+  for (int m = 0;; m++) {
+    // monitor: one FP64 Explicit_Step; its residual is the reference's
+    int e = enqueue_step(c, 1, &pm);
+    if (e) return e;
+    HIPCHK(hipMemcpyAsync(c->hscal, c->scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (have_def) HIPCHK(hipMemcpyAsync(c->hscal + 4, R.red.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    int bijk[4] = {0, 0, 0, 0};
+    const int st = sync_status(c, bijk);
+    if (st) return st;
+    const double res = sqrt(c->hscal[2]);
+    if (have_def) out->defect = sqrt(c->hscal[4]);
+    if (hist) hist[m] = res;
+    out->res = res;
+    out->outer = m;
+    if (res <= rp.res_threshold) {
+      out->converged = 1;
+      break;
+    }
+    if (m >= rp.max_outer) break;
+    // FP64 defect, conductances and BC diagonal at the current T
+    double* T = L.Tb[L.cur].p;
+    e = launch_ghost(c, L, T);
+    if (e) return e;
+    StageArgs<double> sa;
+    memset(&sa, 0, sizeof(sa));
+    sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.ctiles = L.ctiles.p; sa.T = T; sa.TO = T;
+    sa.qvol = L.qvol.p; sa.vol = L.vol.p; sa.mat = L.mat.p;
+    for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
+    sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.tab = dtab<double>(c);
+    for (size_t s = 0; s < c->series.size() && s < (size_t)MAX_SERIES; s++)
+      sa.sval[s] = series_value(c->series[s].periodic, (int)c->series[s].t.size(), c->series[s].t.data(),
+                                c->series[s].v.data(), L.time);
+    sa.partials = L.partials.p; sa.status = c->status.p;
+    const bool d3 = L.kp == GC;
+    if (L.nbound) {
+      if (L.any_active) {
+        d3 ? k_bcflux<3, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound)
+           : k_bcflux<2, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound);
+      }
+      d3 ? k_bcderiv<3><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, R.bcd.p, L.nbound)
+         : k_bcderiv<2><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, R.bcd.p, L.nbound);
+      HIPCHK(hipGetLastError());
+    }
+    if (d3) L.tang ? launch_defect<3, true, S>(c, L, sa, C0) : launch_defect<3, false, S>(c, L, sa, C0);
+    else L.tang ? launch_defect<2, true, S>(c, L, sa, C0) : launch_defect<2, false, S>(c, L, sa, C0);
+    HIPCHK(hipGetLastError());
+    {
+      const int nb1 = (int)((ntile + RED_CHUNK - 1) / RED_CHUNK);
+      k_reduce<false><<<nb1, 256, 0, c->stream>>>(L.partials.p, (int)ntile, L.red2.p);
+      k_reduce<false><<<1, 256, 0, c->stream>>>(L.red2.p, nb1, R.red.p);
+      HIPCHK(hipGetLastError());
+    }
+    have_def = true;
+    for (int l = 1; l < nl; l++) {
+      CLevel& Cl = R.lv[l];
+      k_cmg_coarsen<S><<<cgrid(Cl.ncell), 256, 0, c->stream>>>(cop<S>(R.lv[l - 1]), Cl.blk.p, (int)Cl.hblk.size(),
+                                                              Cl.ncell, (S*)Cl.cx, (S*)Cl.cy, (S*)Cl.cz, (S*)Cl.D);
+      HIPCHK(hipGetLastError());
+    }
+    k_cmg_fill<S><<<cgrid(C0.ncell), 256, 0, c->stream>>>((S*)C0.d[C0.cur], C0.ncell, (S)0);
+    HIPCHK(hipGetLastError());
+    for (int v = 0; v < rp.vcycles; v++) {
+      e = cmg_vcycle<S>(c, 0, al);
+      if (e) return e;
+      out->vcycles++;
+    }
+    k_cmg_apply<S><<<ntile, TILE_T, 0, c->stream>>>(L.blk.p, L.tiles.p, C0.blk.p, (const S*)C0.d[C0.cur], T);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
 }  // namespace
+
+extern "C" {
+
+int fuss_refine_setup(fuss_ctx* c, const fuss_refine_params* rp) {
+  if (!c || !c->setup_done || !rp) return FUSS_EARG;
+  return refine_build(c, rp);
+}
+
+int fuss_refine(fuss_ctx* c, const fuss_step_params* monitor, fuss_refine_result* out, double* hist_res) {
+  if (!c || !c->setup_done || !monitor || !out || !c->ref.ready) return FUSS_EARG;
+  if (monitor->time_accurate) return FUSS_EARG;
+  return c->ref.p.precision == FUSS_FP32 ? refine_run<float>(c, monitor, out, hist_res)
+                                         : refine_run<double>(c, monitor, out, hist_res);
+}
+
+}  // extern "C"
 
 extern "C" {
 
@@ -2300,6 +2617,7 @@ void fuss_destroy(fuss_ctx* c) {
   }
   c->dcp.free(); c->drho.free(); c->dkap.free(); c->dh.free(); c->dkap2.free(); c->dpr4.free();
   c->status.free(); c->scal.free();
+  refine_free(c->ref);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hstatus) (void)hipHostFree(c->hstatus);
   for (auto e : c->prof.pool) (void)hipEventDestroy(e);

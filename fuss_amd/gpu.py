@@ -39,6 +39,26 @@ class StepResult(C.Structure):
                 ("status", C.c_int), ("bijk", C.c_int * 4)]
 
 
+class RefineParams(C.Structure):
+    _fields_ = [("precision", C.c_int), ("n_stage", C.c_int), ("lam_lo", C.c_double), ("lam_hi", C.c_double),
+                ("irs_beta", C.c_double), ("nu1", C.c_int), ("nu2", C.c_int), ("vcycles", C.c_int),
+                ("max_levels", C.c_int), ("ncoarse", C.c_int), ("max_outer", C.c_int), ("res_threshold", C.c_double)]
+
+
+class RefineResult(C.Structure):
+    _fields_ = [("outer", C.c_int), ("vcycles", C.c_int), ("levels", C.c_int), ("converged", C.c_int),
+                ("res", C.c_double), ("defect", C.c_double)]
+
+
+def refine_params(precision=FUSS_FP32, n_stage=4, lam_lo=0.3, lam_hi=2.0, irs_beta=0.0, nu1=1, nu2=1, vcycles=2,
+                  max_levels=0, ncoarse=20, max_outer=200, res_threshold=1e-10):
+    """Defaults: FP32 sweeps, four-stage Chebyshev smoother on [0.3, 2], one
+    pre- and one post-smoothing iteration, two V-cycles per outer iteration,
+    coarsening down to <= 2 cells per direction."""
+    return RefineParams(precision, n_stage, lam_lo, lam_hi, irs_beta, nu1, nu2, vcycles, max_levels, ncoarse,
+                        max_outer, res_threshold)
+
+
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _lib = None
@@ -75,6 +95,8 @@ def load():
     L.fuss_get_stream.argtypes = [P, C.POINTER(P)]
     L.fuss_set_stream.argtypes = [P, P]
     L.fuss_memory.argtypes = [P, C.POINTER(C.c_double)]
+    L.fuss_refine_setup.argtypes = [P, C.POINTER(RefineParams)]
+    L.fuss_refine.argtypes = [P, C.POINTER(StepParams), C.POINTER(RefineResult), C.c_void_p]
     L.fuss_strerror.restype = C.c_char_p
     L.fuss_strerror.argtypes = [C.c_int]
     L.fuss_destroy.argtypes = [P]
@@ -251,3 +273,18 @@ class Solver:
             self.prolong(lv - 1)
             self.level = lv - 1
         return done, (np.concatenate(rows) if rows else np.zeros((0, 3)))
+
+    # -- steady state by defect correction --------------------------------
+    def refine(self, rp=None, monitor=None, setup=True):
+        """Steady state of the finest level by FP64 defect correction
+        (fuss_refine).  monitor: StepParams of the FP64 monitor step (default:
+        the case's numerics).  Returns (RefineResult, residual history)."""
+        rp = rp or refine_params(res_threshold=self.p.res_threshold)
+        if setup:
+            _chk(self.L.fuss_refine_setup(self.h, C.byref(rp)), "fuss_refine_setup")
+        mon = monitor or step_params(self.p)
+        hist = np.zeros(rp.max_outer + 1)
+        r = RefineResult()
+        _chk(self.L.fuss_refine(self.h, C.byref(mon), C.byref(r), hist.ctypes.data_as(C.c_void_p)), "fuss_refine")
+        self.level = 1
+        return r, hist[: r.outer + 1]

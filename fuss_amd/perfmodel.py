@@ -44,3 +44,9 @@ def iteration_bytes(n_rk=3, ndir=3, tang=False, irs=True, prim=True, fp=8):
     if irs:
         per_stage += (ndir - 1) * irs_bytes(False, fp) + irs_bytes(True, fp)
     return b + n_rk * per_stage
+
+
+def cmg_stage_bytes(fp=4):
+    """One stage of the correction smoother (no IRS): read d, d_0, b, the
+    cell's three low-face conductances and the BC diagonal, write d."""
+    return fp * 8

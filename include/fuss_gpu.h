@@ -126,6 +126,46 @@ int fuss_restrict(fuss_ctx *ctx, int fine_level);
  * the interpolation of level fine_level+1. */
 int fuss_prolong(fuss_ctx *ctx, int fine_level);
 
+/* --- steady state by FP64 defect correction (no reference counterpart) -----
+ * The reference reaches a steady state only by marching Explicit_Step until
+ * its residual ||T^{n+1}-T^n||_2 (Mod_Explicit.f90:65-90) drops below
+ * res-threshold.  fuss_refine reaches the same stopping test faster: each
+ * outer iteration takes one FP64 Explicit_Step (the monitor: its residual is
+ * the reference's), evaluates the FP64 defect -R + qvol V of the finest
+ * level, solves the linearised correction equation with a multigrid V-cycle
+ * whose multistage smoother sweeps run in `precision`, and adds the
+ * correction to T in FP64.  Finest level only; blocks must not be joined by
+ * connections (FUSS_EUNSUPPORTED otherwise). */
+typedef struct {
+  int precision;       /* FUSS_FP32 or FUSS_FP64: inner sweeps */
+  int n_stage;         /* stages of the multistage smoother (1..8) */
+  double lam_lo, lam_hi; /* interval of the Jacobi-scaled spectrum the stages damp */
+  double irs_beta;     /* IRS (two Jacobi sweeps per direction) on each stage, 0 = off */
+  int nu1, nu2;        /* pre / post smoothing iterations */
+  int vcycles;         /* V-cycles per outer iteration */
+  int max_levels;      /* correction levels, 0 = coarsen to <= 2 cells per direction */
+  int ncoarse;         /* smoothing iterations on a coarsest level too large for the direct solve */
+  int max_outer;       /* outer iterations (monitor steps - 1) */
+  double res_threshold;/* stop when the monitor residual sqrt(res_sq) <= this */
+} fuss_refine_params;
+
+typedef struct {
+  int outer;           /* monitor steps taken - 1 (= corrections applied) */
+  int vcycles;         /* V-cycles run */
+  int levels;          /* correction levels */
+  int converged;       /* 1 when the monitor residual reached res_threshold */
+  double res;          /* last monitor residual sqrt(res_sq) */
+  double defect;       /* ||-R + qvol V||_2 (W) before the last correction */
+} fuss_refine_result;
+
+/* Builds the correction hierarchy (allocates device memory). */
+int fuss_refine_setup(fuss_ctx *ctx, const fuss_refine_params *rp);
+/* Runs the outer iteration; `monitor` are the Explicit_Step numerics of the
+ * monitor step (steady: time_accurate = 0).  hist_res (may be NULL) receives
+ * the monitor residual of each outer iteration (max_outer + 1 entries). */
+int fuss_refine(fuss_ctx *ctx, const fuss_step_params *monitor, fuss_refine_result *out,
+                double *hist_res);
+
 /* --- state access (output cadence, restart, probes) ------------------- */
 int fuss_get_T(fuss_ctx *ctx, int level, int b, double *T);
 int fuss_set_T(fuss_ctx *ctx, int level, int b, const double *T);

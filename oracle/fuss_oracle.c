@@ -35,6 +35,12 @@ static const double SIGMA_SB = 5.67e-8;
 static const double RKC[3][3] = {{1.0, 0.0, 0.0},
                                  {1.0, 0.50, 0.0},
                                  {1.0, 0.25, 2.0 / 3.0}};
+/* n_rk = 4 (no counterpart in the reference, which stops at RK3): the
+ * four-stage scheme of Jameson, Schmidt and Turkel (AIAA 81-1259),
+ * new = old + a_k * residual(state), a = 1/4, 1/3, 1/2, 1.  For a linear
+ * autonomous problem its amplification polynomial is the classical RK4 one,
+ * 1 + z + z^2/2 + z^3/6 + z^4/24. */
+static const double RK4A[4] = {0.25, 1.0 / 3.0, 0.5, 1.0};
 
 typedef struct {
   int n[3];
@@ -849,6 +855,7 @@ static void residual_smoothing(ob_ctx *c, ob_dom *d) {
 /* ------------------------------------------------------------------ */
 static inline double rk_stage(ob_ctx *c, int irk, double state, double old, double res) {
   /* RK_stage, Lib_RK.f90:15-26 */
+  if (c->n_rk == 4) return old + res * RK4A[irk - 1];
   double r_ = (state - old + res) * RKC[c->n_rk - 1][irk - 1];
   return old + r_;
 }
@@ -1232,7 +1239,7 @@ int ob_set_bc(ob_ctx *c, int level, int nbound, const int *ib, const double *rb,
 
 int ob_set_params(ob_ctx *c, int n_rk, double vnn, int rampa_iter, int time_accurate,
                   int prim, double irs_beta) {
-  if (!c || n_rk < 1 || n_rk > 3) return OB_ERR_ARG;
+  if (!c || n_rk < 1 || n_rk > 4) return OB_ERR_ARG;
   c->n_rk = n_rk; c->vnn = vnn; c->rampa = rampa_iter; c->time_accurate = time_accurate;
   c->prim = prim; c->beta = irs_beta; c->irs = irs_beta > 0.0;
   return 0;

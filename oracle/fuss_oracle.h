@@ -59,7 +59,7 @@ int ob_add_series(ob_ctx *c, int periodic, int n, const double *t, const double 
 int ob_set_bc(ob_ctx *c, int level, int nbound, const int *ib, const double *rb,
               const int *series);
 
-/* Numerics: n_rk in {1,2,3}; prim=1 primitive, 0 conservative; irs_beta>0
+/* Numerics: n_rk in {1,2,3} (reference RK), 4 = Jameson four-stage; prim=1 primitive, 0 conservative; irs_beta>0
  * enables implicit residual smoothing. */
 int ob_set_params(ob_ctx *c, int n_rk, double vnn, int rampa_iter,
                   int time_accurate, int prim, double irs_beta);

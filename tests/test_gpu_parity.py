@@ -101,6 +101,10 @@ SYNTH = {
     "lattice_mg3": lambda: cases.lattice(nl=2, lo=8, hi=20, irs_beta=0.5, time_accurate=False, mgl=3),
     # 2-D plate parity configuration (reduced size)
     "plate2d_64": lambda: cases.plate2d(64),
+    # four-stage (Jameson) extension: the cube with IRS, and the transient
+    # lattice in conservative variables
+    "cube3d_rk4_irs": lambda: cases.cube3d(24, mgl=1, n_rk=4),
+    "lattice_rk4_cons": lambda: cases.lattice(nl=2, lo=4, hi=8, n_rk=4, prim=False),
 }
 
 

@@ -10,7 +10,8 @@ order) would break:
 * a linear temperature field between matching fixed-T walls is steady: the
   302 wall gradient sign and the interior flux cancel exactly;
 * one SSP-RK3 step of a single convectively cooled cell reproduces the
-  stability polynomial 1 + z + z^2/2 + z^3/6, z = -lambda dt, with
+  stability polynomial 1 + z + z^2/2 + z^3/6, z = -lambda dt (and the
+  four-stage extension the classical RK4 polynomial up to z^4/24), with
   lambda from the 303 conductance 1/(1/(2k/dx) + 1/h) (Lib_RK.f90:8-11,
   Lib_BC_Fluxes_Wall_HeatTransfer.f90:31-34);
 * IRS equals two textbook Jacobi iterations of the tridiagonal system
@@ -91,7 +92,8 @@ def test_linear_field_is_steady(oracle_mod):
     assert o2.get("T")[0, 0, 0] > T[0, 0, 0] + 1e-3  # a warmer wall heats the cell
 
 
-@pytest.mark.parametrize("n_rk,poly", [(1, [1, 1]), (2, [1, 1, 0.5]), (3, [1, 1, 0.5, 1 / 6])])
+@pytest.mark.parametrize("n_rk,poly", [(1, [1, 1]), (2, [1, 1, 0.5]), (3, [1, 1, 0.5, 1 / 6]),
+                                       (4, [1, 1, 0.5, 1 / 6, 1 / 24])])
 def test_rk_polynomial(oracle_mod, n_rk, poly):
     dx, k, rho, cp, h, Tinf, T0 = 0.1, 10.0, 1000.0, 500.0, 20.0, 300.0, 500.0
     # one cell, convective on face 2 only: dT/dt = -lambda (T - Tinf)
