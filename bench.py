@@ -146,6 +146,80 @@ def time_to_converge(s, case, args, ms_step):
     return out
 
 
+def adjoint_check(args):
+    """Heat-flux inverse problem on the 512^2 transient plate: forward and
+    reverse-sweep wall times, the adjoint gradient against central finite
+    differences (h = 1e-6) on 8 seeded directions, the FP32-field adjoint
+    against the FP64 one, and a short L-BFGS recovery of q and c."""
+    import torch
+
+    from fuss_amd import adjoint, cases, gpu
+
+    case, info = cases.flux_plate(n=args.adj_n, nseg=64, nprobe=32)
+    nseg = info["q_true"].size
+    prob = adjoint.FluxProblem(case, info["seg_entries"], info["probes"], args.adj_steps, 20, gpu.FUSS_FP64)
+    obs = prob.forward(info["q_true"], 1.0)
+    q0, c0 = np.full(nseg, 3.0e4), 1.15
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = prob.forward(q0, c0)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    g = 2.0 * (hist - obs)
+    t0 = time.perf_counter()
+    gq, gk = prob.backward(g)
+    t_bwd = time.perf_counter() - t0
+    grad = np.concatenate([gq, [gk]])
+    loss0 = float(((hist - obs) ** 2).sum())
+    rng = np.random.default_rng(2024)
+    scale = np.concatenate([np.full(nseg, 1.0e4), [1.0]])
+    h = 1e-6
+    errs = []
+    for _ in range(8):
+        v = rng.normal(size=nseg + 1) * scale
+        Lp = float(((prob.forward(q0 + h * v[:-1], c0 + h * v[-1]) - obs) ** 2).sum())
+        Lm = float(((prob.forward(q0 - h * v[:-1], c0 - h * v[-1]) - obs) ** 2).sum())
+        fd = (Lp - Lm) / (2 * h)
+        ad = float(grad @ v)
+        errs.append(abs(fd - ad) / abs(ad))
+    p32 = adjoint.FluxProblem(case, info["seg_entries"], info["probes"], args.adj_steps, 20, gpu.FUSS_FP32)
+    h32 = p32.forward(q0, c0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gq32, gk32 = p32.backward(2.0 * (h32 - obs))
+    t_bwd32 = time.perf_counter() - t0
+    p32.close()
+    grad32 = np.concatenate([gq32, [gk32]])
+    x = torch.tensor(np.concatenate([q0 / 1e4, [c0]]), dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.LBFGS([x], lr=1.0, max_iter=args.adj_lbfgs, line_search_fn="strong_wolfe")
+
+    def closure():
+        opt.zero_grad()
+        hh = adjoint.ProbeHistory.apply(x[:nseg] * 1e4, x[nseg], prob)
+        loss = ((hh - obs) ** 2).sum()
+        loss.backward()
+        return loss
+
+    t0 = time.perf_counter()
+    opt.step(closure)
+    t_inv = time.perf_counter() - t0
+    loss1 = float(closure().detach())
+    xq = x.detach().numpy()
+    prob.close()
+    out = {"workload": f"transient plate {args.adj_n}^2, 64 flux segments + k scale, 32 probes, "
+                       f"{args.adj_steps} SSP-RK3 steps (global dt), checkpoints every 20",
+           "forward_s": t_fwd, "backward_s": t_bwd, "backward_fp32_fields_s": t_bwd32,
+           "fd_directions": 8, "fd_h": h, "fd_max_rel_err": max(errs), "fd_rel_errs": errs,
+           "fp32_vs_fp64_grad_rel": float(np.linalg.norm(grad32 - grad) / np.linalg.norm(grad)),
+           "inverse": {"lbfgs_max_iter": args.adj_lbfgs, "seconds": t_inv, "loss_start": loss0, "loss_end": loss1,
+                       "q_rel_err": float(np.linalg.norm(xq[:nseg] * 1e4 - info["q_true"]) /
+                                          np.linalg.norm(info["q_true"])),
+                       "k_scale": float(xq[nseg]), "k_scale_true": 1.0}}
+    print(f"[bench] adjoint: fwd {t_fwd:.3f} s, bwd {t_bwd:.3f} s, FD max rel err {max(errs):.2e}, "
+          f"inverse loss {loss0:.3e} -> {loss1:.3e}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +233,10 @@ def main():
     ap.add_argument("--rk", type=int, default=4)
     ap.add_argument("--no-refine", action="store_true", help="skip the time-to-1e-10 runs")
     ap.add_argument("--plain-conv-n", type=int, default=32, help="cube size of the live plain run to 1e-10 (0: skip)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (inverse problem) measurement")
+    ap.add_argument("--adj-n", type=int, default=512)
+    ap.add_argument("--adj-steps", type=int, default=400)
+    ap.add_argument("--adj-lbfgs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
@@ -227,6 +305,8 @@ def main():
     if not args.no_refine:
         out["time_to_1e-10"] = time_to_converge(s, case, args, ms_step)
     s.close()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_check(args)
     if args.calib_copy:
         calibration_copy()
     if not args.no_cpu_baseline:

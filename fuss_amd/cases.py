@@ -201,6 +201,50 @@ def material_tables(nmat=3, seed=3, nT=NT_DEFAULT, npts=16):
     return fm.PropertyTable(1, cp, rho, kap, np.cumsum(rho * cp, axis=1), [f"mat{m+1}" for m in range(nmat)])
 
 
+def flux_plate(n=512, nseg=64, L=1.0, seed=11, nprobe=32, probe_rows=8, n_rk=3, vnn=0.8, T0=300.0,
+               q_lo=1e4, q_hi=5e4, mat_seed=3):
+    """Transient 2-D plate of the heat-flux inverse problem.
+
+    n x n cells on [0, L]^2; one material with tabulated rho(T), cp(T), k(T)
+    (16 knots, resampled to 1 K rows); walls: face 1 302 T = 300 K, face 2
+    303 h = 50, face 3 (min j) 301 with the unknown flux q per segment of
+    n / nseg cells -- the bc.txt value is -q, which heats through a min face
+    (the 301 value is signed along +xi) --, face 4 303 h = 20; T_inf = 300 K.
+    Global dt (time-accurate).  Returns (case, info) with info: seg_entries
+    (nseg x n/nseg 0-based BC entry indices of face 3), q_true (smooth seeded
+    field in [q_lo, q_hi]), probes (nprobe x 4 1-based b,i,j,k, rows j <=
+    probe_rows, seeded), kscale_true = 1."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    assert n % nseg == 0
+    x = np.linspace(0.0, L, n + 1)
+    dx = L / n
+    nodes = nodes_from_axes(x, x.copy(), np.array([0.0, dx]))
+    T = np.full((1, n, n), T0)
+    blk = cs.Block((n, n, 1), nodes, T, np.ones_like(T), np.zeros_like(T))
+    xs = (np.arange(nseg) + 0.5) / nseg
+    f = np.zeros(nseg)
+    for kk in range(1, 5):
+        f += rng.normal() / kk * np.sin(2 * np.pi * kk * xs + rng.uniform(0, 2 * np.pi))
+    f = (f - f.min()) / max(f.max() - f.min(), 1e-30)
+    q_true = q_lo + (q_hi - q_lo) * f
+    faces = [(302, R(Tw=T0)), (303, R(hconv=50.0, Tref=T0)), (301, R(qw=-q_true.mean())),
+             (303, R(hconv=20.0, Tref=T0)), (0, R()), (0, R())]
+    ib, rb = single_block_bc(1, (n, n, 1), faces)
+    off3 = 2 * n  # faces 1 and 2 hold n entries each
+    seg_entries = (off3 + np.arange(n)).reshape(nseg, n // nseg).astype(np.int32)
+    for sgi in range(nseg):
+        for e in seg_entries[sgi]:
+            rb[e][0] = -q_true[sgi]
+    bcs = [bclist(ib, rb)]
+    pi = rng.integers(1, n + 1, size=nprobe)
+    pj = rng.integers(1, probe_rows + 1, size=nprobe)
+    probes = np.stack([np.ones(nprobe, int), pi, pj, np.ones(nprobe, int)], axis=1).astype(np.int32)
+    p = cs.Params(n_rk=n_rk, vnn=vnn, prim=True, irs_beta=0.0, time_accurate=True, mgl=1, level_iter=[10**9])
+    tab = material_tables(nmat=1, seed=mat_seed)
+    case = cs.Case([blk], bcs, tab, p, [], [np.full(bcs[0].n, -1, np.int32)], name=f"flux_plate_{n}")
+    return case, {"seg_entries": seg_entries, "q_true": q_true, "probes": probes, "kscale_true": 1.0}
+
+
 def lattice(nl=4, lo=16, hi=128, seed=7, h=1e-3, amp=0.05, nmat=3, T0=500.0, n_rk=3, vnn=0.8,
             prim=True, irs_beta=0.0, time_accurate=True, mgl=1, walls=None, res_threshold=1e-10):
     """nl^3 blocks in a lattice with per-slab dimensions drawn from U{lo..hi}

@@ -1150,6 +1150,7 @@ __global__ void k_prolong(MGArgs a) {
 }
 
 #include "fuss_refine.h"
+#include "fuss_adjoint.h"
 
 // ------------------------------------------------------------------------
 // host side
@@ -1229,6 +1230,23 @@ struct Refine {
   DBuf<double> bcd, red;
 };
 
+// Checkpointed forward and reverse sweep of the time-accurate step.
+struct Adjoint {
+  int ready = 0, prec = FUSS_FP64;
+  int nsteps = 0, K = 0, nprobe = 0, forward_done = 0;
+  std::vector<long> hprobe;
+  DBuf<long> probe, coff;
+  DBuf<double> ckpt;                // (nsteps / K + 1) states
+  DBuf<double> seg;                 // the K states of one segment
+  DBuf<double> stage[4];            // stage inputs of the recomputed step
+  DBuf<unsigned char> lam;          // lam, lin, lTO in the adjoint precision
+  DBuf<double> part, red, acc;      // partial sums; acc[0] dL/d(dt), acc[1] dL/dc
+  DBuf<int> amin;
+  std::vector<double> times;        // level time before each step
+  std::vector<int> iters;
+  fuss_step_params p{};
+};
+
 }  // namespace
 
 struct fuss_ctx {
@@ -1247,6 +1265,11 @@ struct fuss_ctx {
   int setup_done = 0;
   int* hstatus = nullptr;
   Refine ref;                       // steady defect correction (fuss_refine)
+  Adjoint adj;                      // discrete adjoint (fuss_adj_*)
+  int adj_store_on = 0;             // enqueue_step copies each stage input to adj.stage[]
+  double kscale = 1.0;              // conductivity table scale (fuss_set_kscale)
+  DBuf<int> stage_i;                // staging for fuss_set_bc_values
+  DBuf<double> stage_d;
   // per-kernel-class timing with HIP events on the context stream
   struct Prof {
     int enabled = 0;
@@ -1964,6 +1987,9 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     double* out = (irk % 2 == 1) ? L.Tb[a1].p : L.Tb[a2].p;
     int e = launch_ghost(c, L, in);
     if (e) return e;
+    if (c->adj_store_on)
+      HIPCHK(hipMemcpyAsync(c->adj.stage[irk - 1].p, in, L.npad * sizeof(double), hipMemcpyDeviceToDevice,
+                            c->stream));
     StageArgs<double> sa;
     sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.T = in; sa.TO = L.Tb[o].p; sa.Tout = out; sa.Rres = L.R.p;
     sa.dt = L.dt.p; sa.dtg = c->scal.p; sa.use_dtg = p->time_accurate; sa.qvol = L.qvol.p; sa.vol = L.vol.p;
@@ -2345,9 +2371,255 @@ int refine_run(fuss_ctx* c, const fuss_step_params* mon, fuss_refine_result* out
   return 0;
 }
 
+
+// ------------------------------------------------------------------------
+// discrete adjoint of the time-accurate step (fuss_adj_*)
+// ------------------------------------------------------------------------
+void adj_free(Adjoint& A) {
+  A.probe.free(); A.coff.free(); A.ckpt.free(); A.seg.free();
+  for (auto& b : A.stage) b.free();
+  A.lam.free(); A.part.free(); A.red.free(); A.acc.free(); A.amin.free();
+  A.ready = 0;
+  A.forward_done = 0;
+}
+
+int adj_setup(fuss_ctx* c, int nsteps, int K, int nprobe, const int* bijk, int prec) {
+  Adjoint& A = c->adj;
+  Level& L = c->lev[0];
+  if (nsteps < 1 || K < 1 || nprobe < 0 || (nprobe && !bijk)) return FUSS_EARG;
+  if (prec != FUSS_FP64 && prec != FUSS_FP32) return FUSS_EARG;
+  if (L.tang) return FUSS_EUNSUPPORTED;
+  for (int e = 0; e < L.nbound; e++) {
+    const int t = L.hib[15 * e + 5];
+    if (!(t == 0 || t == 301 || t == 302 || t == 303)) return FUSS_EUNSUPPORTED;
+    if (t == 302 && L.hser.size() > (size_t)e && L.hser[e] >= 0) return FUSS_EUNSUPPORTED;
+  }
+  adj_free(A);
+  A.nsteps = nsteps; A.K = K; A.nprobe = nprobe; A.prec = prec;
+  A.hprobe.resize(nprobe);
+  for (int q = 0; q < nprobe; q++) {
+    const int b = bijk[4 * q] - 1, i = bijk[4 * q + 1], j = bijk[4 * q + 2], k = bijk[4 * q + 3];
+    if (b < 0 || b >= L.nb) return FUSS_EARG;
+    const BlockInfo& B = L.hblk[b];
+    if (i < 1 || i > B.ni || j < 1 || j > B.nj || k < 1 || k > B.nk) return FUSS_EARG;
+    A.hprobe[q] = cidx(B, i, j, k);
+  }
+  std::vector<long> coff(L.nb);
+  long o = 0;
+  for (int b = 0; b < L.nb; b++) { coff[b] = o; o += (long)L.hblk[b].ni * L.hblk[b].nj * L.hblk[b].nk; }
+  const size_t np = (size_t)L.npad;
+  const size_t es = prec == FUSS_FP32 ? 4 : 8;
+  const size_t one = (np * es + 255) / 256 * 256;
+  const size_t ntile = L.htiles.size();
+  int e = A.probe.alloc(std::max(nprobe, 1)) | A.coff.alloc(L.nb) | A.ckpt.alloc(((nsteps - 1) / K + 2) * np) |
+          A.seg.alloc((size_t)K * np) | A.lam.alloc(3 * one) | A.part.alloc(2 * ntile) |
+          A.red.alloc(2 * (ntile / RED_CHUNK + 2)) | A.acc.alloc(2) | A.amin.alloc(ntile);
+  for (auto& b : A.stage) e |= b.alloc(np);
+  if (e) { adj_free(A); return FUSS_EHIP; }
+  if (nprobe) HIPCHK(hipMemcpy(A.probe.p, A.hprobe.data(), nprobe * sizeof(long), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(A.coff.p, coff.data(), L.nb * sizeof(long), hipMemcpyHostToDevice));
+  A.ready = 1;
+  return 0;
+}
+
+int adj_forward(fuss_ctx* c, const fuss_step_params* p, double* hist) {
+  Adjoint& A = c->adj;
+  Level& L = c->lev[0];
+  if (p->integration != FUSS_PRIM || p->irs_beta > 0.0 || !p->time_accurate || p->n_rk < 1 || p->n_rk > 4)
+    return FUSS_EUNSUPPORTED;
+  A.p = *p;
+  A.p.compute_residual = 0;
+  A.times.assign(A.nsteps, 0.0);
+  A.iters.assign(A.nsteps, 0);
+  const size_t np = (size_t)L.npad;
+  for (int n = 0; n < A.nsteps; n++) {
+    if (n % A.K == 0)
+      HIPCHK(hipMemcpyAsync(A.ckpt.p + (size_t)(n / A.K) * np, L.Tb[L.cur].p, np * sizeof(double),
+                            hipMemcpyDeviceToDevice, c->stream));
+    A.times[n] = L.time;
+    A.iters[n] = L.iter;
+    const int e = enqueue_step(c, 1, &A.p);
+    if (e) return e;
+    if (A.nprobe && hist) {
+      k_probe_rec<<<(A.nprobe + 63) / 64, 64, 0, c->stream>>>(L.Tb[L.cur].p, A.probe.p, hist + (size_t)n * A.nprobe,
+                                                            A.nprobe);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  // keep T^N for restoring the level after a reverse sweep
+  HIPCHK(hipMemcpyAsync(A.ckpt.p + (size_t)((A.nsteps - 1) / A.K + 1) * np, L.Tb[L.cur].p, np * sizeof(double),
+                        hipMemcpyDeviceToDevice, c->stream));
+  const int st = sync_status(c);
+  if (st) return st;
+  A.forward_done = 1;
+  return 0;
+}
+
+template <int NDIR, typename S>
+int adj_backward_t(fuss_ctx* c, const double* g, double* gqw, double* gk, double* gT0) {
+  Adjoint& A = c->adj;
+  Level& L = c->lev[0];
+  const size_t np = (size_t)L.npad;
+  const size_t one = (np * sizeof(S) + 255) / 256 * 256;
+  S* lam = (S*)A.lam.p;
+  S* lin = (S*)(A.lam.p + one);
+  S* lTO = (S*)(A.lam.p + 2 * one);
+  const unsigned fb = (unsigned)((np + 255) / 256);
+  k_adj_fill<S><<<fb, 256, 0, c->stream>>>(lam, (long)np);
+  k_adj_fill<S><<<fb, 256, 0, c->stream>>>(lin, (long)np);
+  k_adj_fill<S><<<fb, 256, 0, c->stream>>>(lTO, (long)np);
+  k_adj_fill<double><<<1, 64, 0, c->stream>>>(A.acc.p, 2);
+  if (gqw && L.nbound) k_adj_fill<double><<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(gqw, L.nbound);
+  HIPCHK(hipGetLastError());
+  DBuf<double> gq_own;
+  if (!gqw) {
+    if (gq_own.alloc(std::max(L.nbound, 1))) return FUSS_EHIP;
+    gqw = gq_own.p;
+  }
+  const unsigned ntile = (unsigned)L.htiles.size();
+  const int nb1 = (int)((ntile + RED_CHUNK - 1) / RED_CHUNK);
+  const int nrk = A.p.n_rk;
+  const double t_end = L.time;
+  const int it_end = L.iter;
+  const int N = A.nsteps, K = A.K;
+  if (A.nprobe && g)
+    k_adj_seed<S><<<1, 64, 0, c->stream>>>(lam, A.probe.p, g + (size_t)(N - 1) * A.nprobe, A.nprobe);
+  const int nseg = (N + K - 1) / K;
+  int err = 0;
+  for (int sg = nseg - 1; sg >= 0 && !err; sg--) {
+    const int n0 = sg * K, n1 = std::min(N, n0 + K);
+    HIPCHK(hipMemcpyAsync(L.Tb[L.cur].p, A.ckpt.p + (size_t)sg * np, np * sizeof(double), hipMemcpyDeviceToDevice,
+                          c->stream));
+    for (int n = n0; n < n1; n++) {
+      HIPCHK(hipMemcpyAsync(A.seg.p + (size_t)(n - n0) * np, L.Tb[L.cur].p, np * sizeof(double),
+                            hipMemcpyDeviceToDevice, c->stream));
+      if (n < n1 - 1) {
+        L.time = A.times[n];
+        L.iter = A.iters[n];
+        if ((err = enqueue_step(c, 1, &A.p))) break;
+      }
+    }
+    for (int n = n1 - 1; n >= n0 && !err; n--) {
+      const double* Tn = A.seg.p + (size_t)(n - n0) * np;
+      HIPCHK(hipMemcpyAsync(L.Tb[L.cur].p, Tn, np * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      L.time = A.times[n];
+      L.iter = A.iters[n];
+      c->adj_store_on = 1;
+      err = enqueue_step(c, 1, &A.p);
+      c->adj_store_on = 0;
+      if (err) break;
+      double sval[MAX_SERIES] = {0};
+      for (size_t q = 0; q < c->series.size() && q < (size_t)MAX_SERIES; q++)
+        sval[q] = series_value(c->series[q].periodic, (int)c->series[q].t.size(), c->series[q].t.data(),
+                               c->series[q].v.data(), L.time);
+      for (int sidx = nrk; sidx >= 1; sidx--) {
+        StageArgs<double> sa;
+        memset(&sa, 0, sizeof(sa));
+        sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.T = A.stage[sidx - 1].p; sa.qvol = L.qvol.p; sa.vol = L.vol.p;
+        sa.mat = L.mat.p;
+        for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
+        sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.tab = dtab<double>(c); sa.status = c->status.p;
+        for (int q = 0; q < MAX_SERIES; q++) sa.sval[q] = sval[q];
+        if (L.nbound && L.any_active)
+          k_bcflux<NDIR, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound);
+        AdjArgs<S> aa;
+        aa.blk = L.blk.p; aa.tiles = L.tiles.p; aa.T = A.stage[sidx - 1].p; aa.qvol = L.qvol.p; aa.vol = L.vol.p;
+        aa.mat = L.mat.p;
+        for (int d = 0; d < 3; d++) aa.W[d] = L.W[4 * d].p;
+        aa.bc = L.bc.p; aa.fmap = L.fmap.p; aa.bcv = L.bcv.p; aa.tab = dtab<double>(c); aa.dtg = c->scal.p;
+        aa.coef = nrk == 4 ? RK4A[sidx - 1] : RKC[nrk - 1][sidx - 1];
+        aa.jam = nrk == 4;
+        aa.kscale = c->kscale;
+        aa.lout = lam; aa.lin = lin; aa.lTO = lTO;
+        aa.part = A.part.p; aa.gqw = gqw;
+        k_adj_stage<NDIR, S><<<ntile, TILE_T, 0, c->stream>>>(aa);
+        k_adj_sum2<<<nb1, 256, 0, c->stream>>>(A.part.p, (int)ntile, A.red.p);
+        k_adj_acc2<<<1, 64, 0, c->stream>>>(A.red.p, nb1, A.acc.p);
+        HIPCHK(hipGetLastError());
+        std::swap(lam, lin);
+      }
+      k_adj_combine<S><<<ntile, TILE_T, 0, c->stream>>>(L.blk.p, L.tiles.p, lam, lTO);
+      k_adj_argmin<<<ntile, TILE_T, 0, c->stream>>>(L.blk.p, L.tiles.p, L.dt.p, c->scal.p, A.amin.p);
+      k_adj_dtpath<<<1, 64, 0, c->stream>>>(A.amin.p, (int)ntile, Tn, L.mat.p, c->scal.p, dtab<double>(c), c->kscale,
+                                           sizeof(S) == 8 ? (double*)lam : nullptr,
+                                           sizeof(S) == 4 ? (float*)lam : nullptr, A.acc.p);
+      if (n >= 1 && A.nprobe && g)
+        k_adj_seed<S><<<1, 64, 0, c->stream>>>(lam, A.probe.p, g + (size_t)(n - 1) * A.nprobe, A.nprobe);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  if (!err) {
+    if (gT0) k_adj_extract<S><<<ntile, TILE_T, 0, c->stream>>>(L.blk.p, L.tiles.p, A.coff.p, lam, gT0);
+    if (gk) HIPCHK(hipMemcpyAsync(gk, A.acc.p + 1, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipGetLastError());
+  }
+  // the level returns to T^N and its clock
+  HIPCHK(hipMemcpyAsync(L.Tb[L.cur].p, A.ckpt.p + (size_t)((N - 1) / K + 1) * np, np * sizeof(double),
+                        hipMemcpyDeviceToDevice, c->stream));
+  L.time = t_end;
+  L.iter = it_end;
+  const int st = sync_status(c);
+  gq_own.free();
+  return err ? err : st;
+}
 }  // namespace
 
 extern "C" {
+
+int fuss_set_kscale(fuss_ctx* c, double scale) {
+  if (!c || !c->nT || !(scale > 0.0)) return FUSS_EARG;
+  const size_t n = (size_t)c->nmat * c->nT;
+  std::vector<double> k(n), k2(2 * n, 0.0);
+  for (size_t q = 0; q < n; q++) k[q] = c->tkap[q] * scale;
+  for (int m = 0; m < c->nmat; m++)
+    for (int t = 0; t < c->nT; t++) {
+      const size_t q = (size_t)m * c->nT + t;
+      k2[2 * q] = k[q];
+      k2[2 * q + 1] = t + 1 < c->nT ? k[q + 1] - k[q] : 0.0;
+    }
+  int e = upload(c->dkap.p, k.data(), n * 8) | upload(c->dkap2.p, k2.data(), 2 * n * 8);
+  if (e) return FUSS_EHIP;
+  c->kscale = scale;
+  return 0;
+}
+
+int fuss_set_bc_values(fuss_ctx* c, int level, int n, const int* entries, int column, const double* values) {
+  if (!c || !c->setup_done || level < 1 || level > c->nlev || n < 0 || (n && (!entries || !values))) return FUSS_EARG;
+  if (!(column == 0 || column == 1 || column == 2 || column == 4 || column == 5)) return FUSS_EARG;
+  Level& L = c->lev[level - 1];
+  for (int q = 0; q < n; q++)
+    if (entries[q] < 0 || entries[q] >= L.nbound) return FUSS_EARG;
+  if (!n) return 0;
+  for (int q = 0; q < n; q++) L.hrb[6 * entries[q] + column] = values[q];
+  if ((int)c->stage_i.n < n && (c->stage_i.alloc(n) || c->stage_d.alloc(n))) return FUSS_EHIP;
+  HIPCHK(hipMemcpyAsync(c->stage_i.p, entries, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->stage_d.p, values, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_set_bcval<<<(n + 255) / 256, 256, 0, c->stream>>>(L.bc.p, c->stage_i.p, c->stage_d.p, n, column);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fuss_adj_setup(fuss_ctx* c, int nsteps, int ckpt_every, int nprobe, const int* probe_bijk, int precision) {
+  if (!c || !c->setup_done) return FUSS_EARG;
+  return adj_setup(c, nsteps, ckpt_every, nprobe, probe_bijk, precision);
+}
+
+int fuss_adj_forward(fuss_ctx* c, const fuss_step_params* p, double* probe_hist) {
+  if (!c || !c->setup_done || !p || !c->adj.ready) return FUSS_EARG;
+  return adj_forward(c, p, probe_hist);
+}
+
+int fuss_adj_backward(fuss_ctx* c, const double* probe_adj, double* grad_qw, double* grad_kscale,
+                      double* grad_T0) {
+  if (!c || !c->adj.ready || !c->adj.forward_done) return FUSS_EARG;
+  const bool d3 = c->lev[0].kp == GC;
+  if (c->adj.prec == FUSS_FP32)
+    return d3 ? adj_backward_t<3, float>(c, probe_adj, grad_qw, grad_kscale, grad_T0)
+              : adj_backward_t<2, float>(c, probe_adj, grad_qw, grad_kscale, grad_T0);
+  return d3 ? adj_backward_t<3, double>(c, probe_adj, grad_qw, grad_kscale, grad_T0)
+            : adj_backward_t<2, double>(c, probe_adj, grad_qw, grad_kscale, grad_T0);
+}
 
 int fuss_refine_setup(fuss_ctx* c, const fuss_refine_params* rp) {
   if (!c || !c->setup_done || !rp) return FUSS_EARG;
@@ -2618,6 +2890,8 @@ void fuss_destroy(fuss_ctx* c) {
   c->dcp.free(); c->drho.free(); c->dkap.free(); c->dh.free(); c->dkap2.free(); c->dpr4.free();
   c->status.free(); c->scal.free();
   refine_free(c->ref);
+  adj_free(c->adj);
+  c->stage_i.free(); c->stage_d.free();
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hstatus) (void)hipHostFree(c->hstatus);
   for (auto e : c->prof.pool) (void)hipEventDestroy(e);

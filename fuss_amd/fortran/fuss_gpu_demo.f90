@@ -2,9 +2,10 @@
 !> (n x n cells on [0,1]^2, k=50, rho=7800, cp=460, 302 walls T=500/300 on
 !> faces 1/2, 301 q=0 on faces 3/4, null faces 5/6, IC T=300 K uniform),
 !> built in Fortran arrays with the reference's layouts and stepped with
-!> fuss_step, then fuss_run.  Prints one line per step and the final state
-!> checksum; tests/test_fortran_driver.py compares it with the Python
-!> binding on the same case.
+!> fuss_step, then fuss_run, then the defect-correction steady solve
+!> (fuss_refine).  Prints one line per step, the run and refine outcomes and
+!> the final state checksum; tests/test_fortran_driver.py compares it with
+!> the Python binding on the same case.
 program fuss_gpu_demo
   use iso_c_binding
   use fuss_gpu_m
@@ -17,6 +18,8 @@ program fuss_gpu_demo
   real(c_double), allocatable :: rb(:,:)
   type(fuss_step_params) :: p
   type(fuss_step_result) :: r
+  type(fuss_refine_params) :: rp
+  type(fuss_refine_result) :: rr
   integer(c_int) :: st, nb, e, i, j, k, f, it, ndone
   real(c_double) :: dx, s
 
@@ -78,6 +81,9 @@ program fuss_gpu_demo
   end do
   st = fuss_run(ctx, 1, p, 200, 200, 1d-10, 1d30, ndone, hist_r, hist_t, r); call check(st, 'fuss_run')
   write(*, '(A,I6,ES24.16)') 'run', r%iter, sqrt(r%res_sq)
+  st = fuss_refine_setup(ctx, rp);                                       call check(st, 'fuss_refine_setup')
+  st = fuss_refine(ctx, p, rr, c_null_ptr);                              call check(st, 'fuss_refine')
+  write(*, '(A,2I6,ES24.16)') 'refine', rr%outer, rr%converged, rr%res
   st = fuss_get_T(ctx, 1, 1, T);                                         call check(st, 'fuss_get_T')
   s = sum(T)
   write(*, '(A,ES24.16)') 'sumT', s

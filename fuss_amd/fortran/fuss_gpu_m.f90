@@ -36,8 +36,29 @@ module fuss_gpu_m
     integer(c_int) :: bijk(4)
   end type
 
+  !> defect-correction steady solve (fuss_refine_params)
+  type, bind(c), public :: fuss_refine_params
+    integer(c_int) :: precision = FUSS_FP32
+    integer(c_int) :: n_stage = 4
+    real(c_double) :: lam_lo = 0.3d0, lam_hi = 2.0d0
+    real(c_double) :: irs_beta = 0d0
+    integer(c_int) :: nu1 = 1, nu2 = 1
+    integer(c_int) :: vcycles = 2
+    integer(c_int) :: max_levels = 0
+    integer(c_int) :: ncoarse = 20
+    integer(c_int) :: max_outer = 200
+    real(c_double) :: res_threshold = 1d-10
+  end type
+
+  !> outcome of fuss_refine (fuss_refine_result)
+  type, bind(c), public :: fuss_refine_result
+    integer(c_int) :: outer, vcycles, levels, converged
+    real(c_double) :: res, defect
+  end type
+
   public :: fuss_create, fuss_set_table, fuss_set_block, fuss_add_series, fuss_set_bc, fuss_setup
   public :: fuss_step, fuss_run, fuss_restrict, fuss_prolong
+  public :: fuss_refine_setup, fuss_refine
   public :: fuss_get_T, fuss_set_T, fuss_probe, fuss_get_clock, fuss_set_clock, fuss_ndir, fuss_destroy
 
   interface
@@ -102,6 +123,20 @@ module fuss_gpu_m
       integer(c_int), intent(out) :: n_done
       real(c_double), intent(out) :: hist_res(*), hist_time(*)
       type(fuss_step_result), intent(out) :: last
+    end function
+
+    integer(c_int) function fuss_refine_setup(ctx, rp) bind(c, name='fuss_refine_setup')
+      import :: c_int, c_ptr, fuss_refine_params
+      type(c_ptr), value :: ctx
+      type(fuss_refine_params), intent(in) :: rp
+    end function
+
+    integer(c_int) function fuss_refine(ctx, monitor, res, hist_res) bind(c, name='fuss_refine')
+      import :: c_int, c_ptr, fuss_step_params, fuss_refine_result
+      type(c_ptr), value :: ctx
+      type(fuss_step_params), intent(in) :: monitor
+      type(fuss_refine_result), intent(out) :: res
+      type(c_ptr), value :: hist_res
     end function
 
     integer(c_int) function fuss_restrict(ctx, fine_level) bind(c, name='fuss_restrict')

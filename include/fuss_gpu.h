@@ -166,6 +166,35 @@ int fuss_refine_setup(fuss_ctx *ctx, const fuss_refine_params *rp);
 int fuss_refine(fuss_ctx *ctx, const fuss_step_params *monitor, fuss_refine_result *out,
                 double *hist_res);
 
+/* --- discrete adjoint of the time-accurate step (no reference counterpart) ---
+ * Parameters of the forward model: wall values of BC entries (the reference
+ * keeps them in bc%qw / bc%Tw ..., set once by Read_BCfile, IO_BC.f90:121-245)
+ * and a scale of the conductivity table (Load_Table, Load_Table.f90:61-71). */
+/* kappa table := scale x the table given to fuss_set_table. */
+int fuss_set_kscale(fuss_ctx *ctx, double scale);
+/* Sets column `column` of rb (0 qw, 1 Tw, 2 hconv, 4 Tref, 5 eps) for the
+ * 0-based entries[n] of a level's BC list. */
+int fuss_set_bc_values(fuss_ctx *ctx, int level, int n, const int *entries, int column,
+                       const double *values);
+/* Reverse-mode sweep over nsteps time-accurate Explicit_Steps of level 1 with
+ * a checkpoint every ckpt_every steps; probe_bijk[4 nprobe] (1-based b,i,j,k)
+ * are the cells whose T after every step the loss may depend on.  precision:
+ * FUSS_FP64 or FUSS_FP32 adjoint fields (the forward recompute is FP64).
+ * Scope: primitive variables, no IRS, orthogonal blocks, walls 0/301/302/303,
+ * no connections (FUSS_EUNSUPPORTED otherwise). */
+int fuss_adj_setup(fuss_ctx *ctx, int nsteps, int ckpt_every, int nprobe, const int *probe_bijk,
+                   int precision);
+/* nsteps steps from the current state, storing checkpoints; probe_hist
+ * (device, nsteps x nprobe, may be NULL) receives T at the probes after each
+ * step.  Leaves T^nsteps as the level state. */
+int fuss_adj_forward(fuss_ctx *ctx, const fuss_step_params *p, double *probe_hist);
+/* Given dL/d(probe_hist) (device, nsteps x nprobe), writes dL/dqw per BC entry
+ * of level 1 (device, nbound; 301 entries without a time series), dL/dscale
+ * (device, 1) and dL/dT^0 (device, interior cells block after block, i
+ * fastest; may be NULL).  The level keeps T^nsteps. */
+int fuss_adj_backward(fuss_ctx *ctx, const double *probe_adj, double *grad_qw, double *grad_kscale,
+                      double *grad_T0);
+
 /* --- state access (output cadence, restart, probes) ------------------- */
 int fuss_get_T(fuss_ctx *ctx, int level, int b, double *T);
 int fuss_set_T(fuss_ctx *ctx, int level, int b, const double *T);

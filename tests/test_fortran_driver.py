@@ -1,7 +1,8 @@
 """The Fortran iso_c_binding module drives the library like the reference's
 driver would: the demo program (fuss_amd/fortran/fuss_gpu_demo.f90) builds a
-plate in Fortran arrays and steps it through fuss_step / fuss_run; the Python
-binding on the same case must print the same residuals and state sum."""
+plate in Fortran arrays and steps it through fuss_step / fuss_run, then
+solves the steady state with fuss_refine; the Python binding on the same case
+must print the same residuals, refine outcome and state sum."""
 import os
 import subprocess
 
@@ -24,6 +25,7 @@ def test_fortran_driver_matches_python():
     lines = [ln.split() for ln in out.stdout.splitlines() if ln.strip()]
     fres = [float(x[2]) for x in lines if x[0] == "step"]
     frun = [x for x in lines if x[0] == "run"][0]
+    fref = [x for x in lines if x[0] == "refine"][0]
     fsum = float([x for x in lines if x[0] == "sumT"][0][1])
 
     from fuss_amd import gpu
@@ -35,5 +37,9 @@ def test_fortran_driver_matches_python():
     np.testing.assert_allclose(fres, pres, rtol=1e-15)
     assert int(frun[1]) == r.iter
     np.testing.assert_allclose(float(frun[2]), np.sqrt(r.res_sq), rtol=1e-15)
-    np.testing.assert_allclose(fsum, float(np.sum(s.get("T"))), rtol=1e-15)
+    rr, _ = s.refine(gpu.refine_params(res_threshold=1e-10))
+    assert int(fref[1]) == rr.outer and int(fref[2]) == 1 == rr.converged
+    np.testing.assert_allclose(float(fref[3]), rr.res, rtol=1e-15)
+    # the checksum is summed in a different order by Fortran's sum and numpy (~1e-13)
+    np.testing.assert_allclose(fsum, float(np.sum(s.get("T"))), rtol=1e-12)
     s.close()
