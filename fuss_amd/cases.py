@@ -60,36 +60,36 @@ def stretch(n, L, a, phi):
 
 
 def face_entries(b, dims, f, typ, rrec, ib_extra=None):
-    """BC entries of face f of block b (1-based), i/j fastest over the face."""
+    """BC entries of face f of block b (1-based), i/j fastest over the face:
+    (ib (n, 15) int32, rb (n, 6) float64).  ib_extra(i, j, k) maps the cell
+    index arrays to the (n, 9) donor columns (connections)."""
     ni, nj, nk = dims
-    ib, rb = [], []
     if f <= 2:
-        i = 1 if f == 1 else ni
-        cells = [(i, j, k) for k in range(1, nk + 1) for j in range(1, nj + 1)]
+        kk, jj = np.meshgrid(np.arange(1, nk + 1), np.arange(1, nj + 1), indexing="ij")
+        ii = np.full_like(jj, 1 if f == 1 else ni)
     elif f <= 4:
-        j = 1 if f == 3 else nj
-        cells = [(i, j, k) for k in range(1, nk + 1) for i in range(1, ni + 1)]
+        kk, ii = np.meshgrid(np.arange(1, nk + 1), np.arange(1, ni + 1), indexing="ij")
+        jj = np.full_like(ii, 1 if f == 3 else nj)
     else:
-        k = 1 if f == 5 else nk
-        cells = [(i, j, k) for j in range(1, nj + 1) for i in range(1, ni + 1)]
-    for n, (i, j, k) in enumerate(cells):
-        q = [b, i, j, k, f, typ] + [0] * 9
-        if ib_extra is not None:
-            q[6:15] = ib_extra(i, j, k)
-        ib.append(q)
-        rb.append(list(rrec))
+        jj, ii = np.meshgrid(np.arange(1, nj + 1), np.arange(1, ni + 1), indexing="ij")
+        kk = np.full_like(ii, 1 if f == 5 else nk)
+    ii, jj, kk = ii.reshape(-1), jj.reshape(-1), kk.reshape(-1)
+    n = ii.size
+    ib = np.zeros((n, fm.BC_NI), np.int32)
+    ib[:, 0] = b
+    ib[:, 1], ib[:, 2], ib[:, 3] = ii, jj, kk
+    ib[:, 4] = f
+    ib[:, 5] = typ
+    if ib_extra is not None:
+        ib[:, 6:15] = ib_extra(ii, jj, kk)
+    rb = np.tile(np.asarray(rrec, np.float64), (n, 1))
     return ib, rb
 
 
 def single_block_bc(b, dims, faces):
     """faces: list of 6 (type, rrec) for faces 1..6."""
-    ib, rb = [], []
-    for f in range(1, 7):
-        typ, rrec = faces[f - 1]
-        a, c = face_entries(b, dims, f, typ, rrec)
-        ib += a
-        rb += c
-    return ib, rb
+    parts = [face_entries(b, dims, f, *faces[f - 1]) for f in range(1, 7)]
+    return np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts])
 
 
 def bclist(ib, rb):
@@ -176,9 +176,9 @@ def sweep(ncase=256, lo=32, hi=96, L=0.1, seed=5, n_rk=3, vnn=2.0, prim=True, ir
         for c, b in enumerate(blocks):
             d = cs.coarse_dims(b.dims, mgl)[m]
             a, r = single_block_bc(c + 1, d, faces_all[c])
-            ib += a
-            rb += r
-        bcs.append(bclist(ib, rb))
+            ib.append(a)
+            rb.append(r)
+        bcs.append(bclist(np.concatenate(ib), np.concatenate(rb)))
     p = cs.Params(n_rk=n_rk, vnn=vnn, prim=prim, irs_beta=irs_beta, mgl=mgl, level_iter=[10**9] * mgl,
                   res_threshold=res_threshold)
     tab = const_table(ks, 7800.0, 460.0, nmat=ncase)
@@ -233,8 +233,7 @@ def flux_plate(n=512, nseg=64, L=1.0, seed=11, nprobe=32, probe_rows=8, n_rk=3, 
     off3 = 2 * n  # faces 1 and 2 hold n entries each
     seg_entries = (off3 + np.arange(n)).reshape(nseg, n // nseg).astype(np.int32)
     for sgi in range(nseg):
-        for e in seg_entries[sgi]:
-            rb[e][0] = -q_true[sgi]
+        rb[seg_entries[sgi], 0] = -q_true[sgi]
     bcs = [bclist(ib, rb)]
     pi = rng.integers(1, n + 1, size=nprobe)
     pj = rng.integers(1, probe_rows + 1, size=nprobe)
@@ -305,17 +304,19 @@ def lattice(nl=4, lo=16, hi=128, seed=7, h=1e-3, amp=0.05, nmat=3, T0=500.0, n_r
                             fs = f + 1 if f % 2 == 1 else f - 1
 
                             def extra(i, j, k, ax=ax, hi_side=hi_side, nb_=nb_, dn=dn, fs=fs):
-                                q = [i, j, k]
-                                q[ax] = dn[ax] if not hi_side else 1
-                                return [nb_, q[0], q[1], q[2], fs, 1, 0, 0, 1]
+                                q = [i.copy(), j.copy(), k.copy()]
+                                q[ax][:] = dn[ax] if not hi_side else 1
+                                cols = [np.full_like(i, nb_), q[0], q[1], q[2], np.full_like(i, fs),
+                                        np.ones_like(i), np.zeros_like(i), np.zeros_like(i), np.ones_like(i)]
+                                return np.stack(cols, axis=1)
 
                             x, y = face_entries(me, d0, f, 101, R(), extra)
                         else:
                             typ, rr = walls[f - 1]
                             x, y = face_entries(me, d0, f, typ, rr)
-                        ib += x
-                        rb += y
-        return bclist(ib, rb)
+                        ib.append(x)
+                        rb.append(y)
+        return bclist(np.concatenate(ib), np.concatenate(rb))
 
     bcs = [lattice_bc(m) for m in range(mgl)]
     p = cs.Params(n_rk=n_rk, vnn=vnn, prim=prim, irs_beta=irs_beta, time_accurate=time_accurate, mgl=mgl,

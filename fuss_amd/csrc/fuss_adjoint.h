@@ -318,6 +318,14 @@ __global__ __launch_bounds__(TILE_T) void k_adj_extract(const BlockInfo* __restr
   out[coff[t.b] + (i - 1) + (long)B.ni * ((j - 1) + (long)B.nj * (k - 1))] = (double)lam[p];
 }
 
+// Level time and iteration before a step (forward of the adjoint).
+__global__ void k_clock_rec(const double* __restrict__ clk, const int* __restrict__ cnt, double* t, int* it) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *t = clk[0];
+    *it = cnt[0];
+  }
+}
+
 // Wall values of selected BC entries (fuss_set_bc_values): column 0 qw,
 // 1 Tw, 2 hconv, 4 Tref, 5 eps.
 __global__ void k_set_bcval(BCRec* __restrict__ bc, const int* __restrict__ idx, const double* __restrict__ v, int n,

@@ -175,9 +175,9 @@ __global__ void k_bc_geometry(BCRec* bc, int n, const BlockInfo* blk, const doub
 // Ghost_Connection / Ghost_Symmetry (Lib_Ghost.f90:132-189), first layer:
 // the explicit step's stencils read no cell beyond it.
 template <typename R>
-__global__ void k_ghost(R* __restrict__ T, const BCRec* __restrict__ bc, int n) {
+__global__ void k_ghost(R* __restrict__ T, const BCRec* __restrict__ bc, int n, const int* __restrict__ halt) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+  if (e >= n || (halt && *halt)) return;
   const long d = bc[e].gdst;
   if (d >= 0) T[d] = T[bc[e].gsrc];
 }
@@ -195,7 +195,8 @@ struct DtArgs {
   R* dt;
   Tab<R> tab;
   double vnn;
-  int iter, rampa;
+  const int* cnt;      // level counters: [0] iteration before this step, [1] halt
+  int rampa;
   double* pmin;        // per-work-group minimum
   int* status;
 };
@@ -203,6 +204,8 @@ struct DtArgs {
 // Compute_dt, Mod_dt.f90:26-84
 template <typename R>
 __global__ __launch_bounds__(TILE_T) void k_dt(DtArgs<R> a) {
+  if (a.cnt[1]) return;
+  const int iter = a.cnt[0] + 1;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const Tile t = a.tiles[wg];
   const BlockInfo B = a.blk[t.b];
@@ -222,7 +225,7 @@ __global__ __launch_bounds__(TILE_T) void k_dt(DtArgs<R> a) {
       const double alpha = (double)kap / (double)rho / (double)cp;
       double d = 0.5 * a.vnn / (alpha * a.invL2[p]);
       d = d < 1e8 ? d : 1e8;
-      if (a.iter < a.rampa) d = d * (double)a.iter / (double)a.rampa;
+      if (iter < a.rampa) d = d * (double)iter / (double)a.rampa;
       dtc = d;
     }
     a.dt[p] = (R)dtc;
@@ -263,7 +266,9 @@ struct StageArgs {
   const int* fmap;
   R* bcv;              // per BC entry: value subtracted from the cell residual
   Tab<R> tab;
-  double sval[MAX_SERIES];
+  SerDev ser;          // 301/302 time series, evaluated at *clk
+  const double* clk;   // level time (device)
+  const int* halt;     // nonzero: the step is skipped (graph replay past a stop)
   R rkc;
   int jam;             // four-stage Jameson update (n_rk = 4)
   int last;
@@ -318,7 +323,7 @@ __device__ __forceinline__ R dotrow(const double* M, int r, const double* N) {
 // kernels apply the v of a cell's faces in face order, which reproduces the
 // reference's accumulation exactly.
 template <int NDIR, typename R>
-__device__ R bc_value(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e, long p) {
+__device__ R bc_value(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e, long p, double sv) {
 #pragma clang fp contract(off)
   R Rv = 0;
   const R* __restrict__ T = a.T;
@@ -385,13 +390,13 @@ __device__ R bc_value(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e,
     } break;
     case 301: {
       // BC_Wall_Heat, Lib_BC_Fluxes_Wall_Heat.f90:12-54
-      const R qw = e.series >= 0 ? (R)a.sval[e.series] : (R)e.qw;
+      const R qw = e.series >= 0 ? (R)sv : (R)e.qw;
       const R Flux = (R)e.A * qw;
       Rv = (R)modfm2 * Flux;
     } break;
     case 302: {
       // BC_Wall_Temperature, Lib_BC_Fluxes_Wall_Temperature.f90:12-68
-      const R Tw = e.series >= 0 ? (R)a.sval[e.series] : (R)e.Tw;
+      const R Tw = e.series >= 0 ? (R)sv : (R)e.Tw;
       const R kappa = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
       R G[3] = {0, 0, 0};
       const R gn = (Tc - Tw) * (R)modfm3;
@@ -487,11 +492,16 @@ __device__ __forceinline__ R apply_bc_faces(const StageArgs<R>& a, const BlockIn
 template <int NDIR, typename R>
 __global__ void k_bcflux(StageArgs<R> a, int n) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+  if (e >= n || (a.halt && *a.halt)) return;
   const BCRec& r = a.bc[e];
   if (!(r.type == 101 || (r.type >= 300 && r.type <= 305))) return;
   const BlockInfo B = a.blk[r.b];
-  a.bcv[e] = bc_value<NDIR>(a, B, r, r.cell);
+  double sv = 0.0;
+  if (r.series >= 0) {
+    const int q = r.series;
+    sv = series_value(a.ser.per[q], a.ser.n[q], a.ser.t + a.ser.off[q], a.ser.v + a.ser.off[q], *a.clk);
+  }
+  a.bcv[e] = bc_value<NDIR>(a, B, r, r.cell, sv);
 }
 
 // RK_stage, Lib_RK.f90:15-26.  jam: the four-stage scheme of Jameson,
@@ -555,6 +565,7 @@ __device__ __forceinline__ void block_sum(double v, double* partials, int wg) {
 // otherwise apply the RK update to Tout.
 template <int NDIR, bool TANG, bool PRIM, bool IRSMODE, typename R>
 __global__ __launch_bounds__(TILE_T) void k_stage(StageArgs<R> a) {
+  if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const Tile t = a.tiles[wg];
   const BlockInfo B = a.blk[t.b];
@@ -707,6 +718,7 @@ __device__ __forceinline__ R flux_z(const StageArgs<R>& a, const BlockInfo& B, R
 
 template <bool TANG, bool PRIM, bool IRSMODE, typename R>
 __global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
+  if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const Tile t = a.ctiles[wg];
   const BlockInfo B = a.blk[t.b];
@@ -818,6 +830,7 @@ struct IrsArgs {
   int last;
   double* partials;
   int* status;
+  const int* halt;
 };
 
 // Residual_Smoothing_Blk along DIR (Lib_IRS.f90:43-134): two Jacobi sweeps
@@ -826,6 +839,7 @@ struct IrsArgs {
 // second sweep needs r(s-2..s+2) only, so each cell is computed directly.
 template <int DIR, int NDIR, bool UPDATE, bool PRIM, typename R>
 __global__ __launch_bounds__(TILE_T) void k_irs(IrsArgs<R> a) {
+  if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const Tile t = a.tiles[wg];
   const BlockInfo B = a.blk[t.b];
@@ -913,6 +927,7 @@ constexpr int XY_RJ = XY_TJ + 4, XY_RI = TILE_X + 4;
 // x-smoothed boundary rows).  2-D levels finish the stage here (UPDATE).
 template <int NDIR, bool UPDATE, bool PRIM, typename R>
 __global__ __launch_bounds__(TILE_T) void k_irs_xy(IrsArgs<R> a) {
+  if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const Tile t = a.tiles[wg];
   const BlockInfo B = a.blk[t.b];
@@ -974,6 +989,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xy(IrsArgs<R> a) {
 // through KC planes with r(k-2..k+2) in registers, so R is read once.
 template <bool PRIM, typename R>
 __global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
+  if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const Tile t = a.tiles[wg];
   const BlockInfo B = a.blk[t.b];
@@ -1013,7 +1029,9 @@ constexpr int RED_CHUNK = 2048;
 
 // Pass 1: work-group b reduces in[b*RED_CHUNK, ...) to out[b].
 template <bool MIN>
-__global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ in, int n, double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ in, int n, double* __restrict__ out,
+                                                const int* __restrict__ halt = nullptr) {
+  if (halt && *halt) return;
   __shared__ double sm[256];
   const int lo = blockIdx.x * RED_CHUNK, hi = min(n, lo + RED_CHUNK);
   double v = MIN ? 1e300 : 0.0;
@@ -1029,8 +1047,11 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ in, i
 
 // Pass 2 of the dt minimum: dtglobal = min(1e5, min dt) (Mod_Explicit.f90:40,
 // Mod_dt.f90:250-258) and the time advance of time-accurate runs (:42-45).
-__global__ __launch_bounds__(256) void k_dt_final(const double* __restrict__ in, int n, double* scal, double time,
-                                                  int time_accurate) {
+// dtglobal = min(1e5, .) (Set_Global_dt, Mod_dt.f90:86-107); the level clock
+// advances on the device: time (+ dtglobal when time-accurate), iteration + 1.
+__global__ __launch_bounds__(256) void k_dt_final(const double* __restrict__ in, int n, double* scal, double* clk,
+                                                  int* cnt, int time_accurate) {
+  if (cnt[1]) return;
   __shared__ double sm[256];
   double v = 1e300;
   for (int q = threadIdx.x; q < n; q += 256) v = fmin(v, in[q]);
@@ -1042,9 +1063,46 @@ __global__ __launch_bounds__(256) void k_dt_final(const double* __restrict__ in,
   }
   if (threadIdx.x == 0) {
     const double m = fmin(1e5, sm[0]);
+    const double t = clk[0];
+    const double tn = time_accurate ? t + m : t;
     scal[0] = m;
-    scal[1] = time_accurate ? time + m : time;
+    scal[1] = tn;
+    clk[0] = tn;
+    cnt[0] += 1;
   }
+}
+
+// Level clock from the host (setup, fuss_set_clock, adjoint recompute).
+__global__ void k_set_clock(double* clk, int* cnt, double time, int iter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    clk[0] = time;
+    cnt[0] = iter;
+    cnt[1] = 0;
+  }
+}
+
+// fuss_run on a captured graph: counters cnt[2] (steps of this run), cnt[3]
+// (steps before this chunk); a step that meets a stop test raises the halt
+// flag cnt[1], which turns every later kernel of the graph into a no-op.
+__global__ void k_run_begin(int* cnt) {
+  if (threadIdx.x == 0) {
+    cnt[1] = 0;
+    cnt[2] = 0;
+  }
+}
+__global__ void k_chunk_begin(int* cnt) {
+  if (threadIdx.x == 0) cnt[3] = cnt[2];
+}
+__global__ void k_step_end(const double* __restrict__ scal, const double* __restrict__ clk, int* cnt,
+                           const int* __restrict__ status, double* hres, double* htime, double thr, double tthr,
+                           int maxcount) {
+  if (threadIdx.x != 0 || cnt[1]) return;
+  const double res = sqrt(scal[2]);
+  const int n = cnt[2];
+  hres[n - cnt[3]] = res;
+  htime[n - cnt[3]] = clk[0];
+  cnt[2] = n + 1;
+  if (status[0] || n + 1 >= maxcount || res <= thr || clk[0] >= tthr) cnt[1] = 1;
 }
 
 // ------------------------------------------------------------------------
@@ -1207,8 +1265,10 @@ struct Level {
   std::vector<double> hrb;          // nbound x 6
   std::vector<int> hser;
   int tang = 0, any_active = 0;
-  double time = 0.0;
+  double time = 0.0;                // host mirrors of the device clock
   int iter = 0;
+  DBuf<double> clk;                 // [0] level time
+  DBuf<int> cnt;                    // [0] iteration, [1] halt, [2] run steps, [3] chunk start
   DBuf<double> partials, pmin, red2;
 };
 
@@ -1244,6 +1304,8 @@ struct Adjoint {
   DBuf<int> amin;
   std::vector<double> times;        // level time before each step
   std::vector<int> iters;
+  DBuf<double> dtimes;              // the same, recorded on the device during the forward
+  DBuf<int> diters;
   fuss_step_params p{};
 };
 
@@ -1267,9 +1329,15 @@ struct fuss_ctx {
   Refine ref;                       // steady defect correction (fuss_refine)
   Adjoint adj;                      // discrete adjoint (fuss_adj_*)
   int adj_store_on = 0;             // enqueue_step copies each stage input to adj.stage[]
+  int graphs_off = 0;               // fuss_run without graph capture (fuss_graphs)
   double kscale = 1.0;              // conductivity table scale (fuss_set_kscale)
   DBuf<int> stage_i;                // staging for fuss_set_bc_values
   DBuf<double> stage_d;
+  DBuf<double> ser_t, ser_v;        // time series on the device
+  DBuf<int> ser_off, ser_n, ser_per;
+  DBuf<double> run_hist;            // per-chunk residual / time history of fuss_run
+  double* hrun = nullptr;           // pinned: run history readback
+  int* hcnt = nullptr;              // pinned: level counters readback
   // per-kernel-class timing with HIP events on the context stream
   struct Prof {
     int enabled = 0;
@@ -1280,6 +1348,12 @@ struct fuss_ctx {
 };
 
 namespace {
+
+SerDev serdev(fuss_ctx* c) {
+  SerDev d;
+  d.t = c->ser_t.p; d.v = c->ser_v.p; d.off = c->ser_off.p; d.n = c->ser_n.p; d.per = c->ser_per.p;
+  return d;
+}
 
 long cidx(const BlockInfo& B, int i, int j, int k) {
   return B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
@@ -1438,6 +1512,8 @@ int alloc_level(fuss_ctx* c, Level& L) {
   const size_t np = std::max<size_t>(std::max(L.htiles.size(), L.hctiles.size()), 1);
   e |= L.partials.alloc(np);
   e |= L.pmin.alloc(np);
+  e |= L.clk.alloc(2);
+  e |= L.cnt.alloc(4);
   e |= L.red2.alloc(np / RED_CHUNK + 2);
   (void)c;
   return e ? FUSS_EHIP : 0;
@@ -1541,6 +1617,24 @@ int sync_status(fuss_ctx* c, int* bijk = nullptr) {
   return c->hstatus[0];
 }
 
+// Host mirrors of a level's device clock (synchronises the stream).
+int read_clock(fuss_ctx* c, Level& L) {
+  HIPCHK(hipMemcpyAsync(c->hscal + 6, L.clk.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hcnt, L.cnt.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  L.time = c->hscal[6];
+  L.iter = c->hcnt[0];
+  return 0;
+}
+
+int set_clock(fuss_ctx* c, Level& L, double time, int iter) {
+  k_set_clock<<<1, 64, 0, c->stream>>>(L.clk.p, L.cnt.p, time, iter);
+  HIPCHK(hipGetLastError());
+  L.time = time;
+  L.iter = iter;
+  return 0;
+}
+
 // Event pair around one launch of a kernel class (profiling mode only).
 int prof_begin(fuss_ctx* c, int cls) {
   auto& P = c->prof;
@@ -1574,7 +1668,7 @@ Tab<R> dtab(fuss_ctx* c) {
 int launch_ghost(fuss_ctx* c, Level& L, double* T) {
   if (!L.nbound) return 0;
   const int pid = prof_begin(c, FUSS_K_GHOST);
-  k_ghost<double><<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(T, L.bc.p, L.nbound);
+  k_ghost<double><<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(T, L.bc.p, L.nbound, L.cnt.p + 1);
   prof_end(c, pid);
   return hipGetLastError() == hipSuccess ? 0 : FUSS_EHIP;
 }
@@ -1634,9 +1728,11 @@ int fuss_create(int nlevels, int nblocks, int precision, fuss_ctx** out) {
     return FUSS_EHIP;
   }
   c->own_stream = true;
-  if (c->status.alloc(8) || c->scal.alloc(4) ||
+  if (c->status.alloc(8) || c->scal.alloc(4) || c->run_hist.alloc(2 * RUN_G) ||
       hipHostMalloc((void**)&c->hscal, 8 * sizeof(double), 0) != hipSuccess ||
-      hipHostMalloc((void**)&c->hstatus, 8 * sizeof(int), 0) != hipSuccess) {
+      hipHostMalloc((void**)&c->hstatus, 8 * sizeof(int), 0) != hipSuccess ||
+      hipHostMalloc((void**)&c->hrun, 2 * RUN_G * sizeof(double), 0) != hipSuccess ||
+      hipHostMalloc((void**)&c->hcnt, 8 * sizeof(int), 0) != hipSuccess) {
     fuss_destroy(c);
     return FUSS_EHIP;
   }
@@ -1811,6 +1907,31 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     }
     L.time = initial_time;
     L.iter = 0;
+    k_set_clock<<<1, 64, 0, c->stream>>>(L.clk.p, L.cnt.p, initial_time, 0);
+    HIPCHK(hipGetLastError());
+  }
+  // time series of 301/302 values, evaluated on the device at the level time
+  {
+    std::vector<double> st, sv;
+    std::vector<int> so, sn, sp;
+    for (const auto& q : c->series) {
+      so.push_back((int)st.size());
+      sn.push_back((int)q.t.size());
+      sp.push_back(q.periodic);
+      st.insert(st.end(), q.t.begin(), q.t.end());
+      sv.insert(sv.end(), q.v.begin(), q.v.end());
+    }
+    const size_t ns = std::max<size_t>(so.size(), 1), np_ = std::max<size_t>(st.size(), 1);
+    if (c->ser_t.alloc(np_) || c->ser_v.alloc(np_) || c->ser_off.alloc(ns) || c->ser_n.alloc(ns) ||
+        c->ser_per.alloc(ns))
+      return FUSS_EHIP;
+    if (!st.empty()) {
+      HIPCHK(hipMemcpy(c->ser_t.p, st.data(), st.size() * 8, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->ser_v.p, sv.data(), sv.size() * 8, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->ser_off.p, so.data(), so.size() * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->ser_n.p, sn.data(), sn.size() * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->ser_per.p, sp.data(), sp.size() * 4, hipMemcpyHostToDevice));
+    }
   }
   // level-1 fields: T, material index, qvol
   {
@@ -1949,12 +2070,12 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   const bool prim = p->integration == FUSS_PRIM;
   const bool irs = p->irs_beta > 0.0;
   if (irs && p->irs_mode != FUSS_IRS_JACOBI2) return FUSS_EUNSUPPORTED;
-  L.iter += 1;
   const unsigned ntile = (unsigned)L.htiles.size();
+  const int* halt = L.cnt.p + 1;
   // Compute_dt + min (Mod_dt.f90:10-84); dtglobal starts at 1e5 (Mod_Explicit.f90:40)
   DtArgs<double> da;
   da.blk = L.blk.p; da.tiles = L.tiles.p; da.T = L.Tb[L.cur].p; da.mat = L.mat.p; da.invL2 = L.invL2.p;
-  da.dt = L.dt.p; da.tab = dtab<double>(c); da.vnn = p->vnn; da.iter = L.iter; da.rampa = p->rampa_iter;
+  da.dt = L.dt.p; da.tab = dtab<double>(c); da.vnn = p->vnn; da.cnt = L.cnt.p; da.rampa = p->rampa_iter;
   da.pmin = L.pmin.p; da.status = c->status.p;
   int pid = prof_begin(c, FUSS_K_DT);
   k_dt<double><<<ntile, TILE_T, 0, c->stream>>>(da);
@@ -1964,22 +2085,13 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   pid = prof_begin(c, FUSS_K_REDUCE);
   if (ntile > (unsigned)RED_CHUNK) {
     const int nb1 = (int)((ntile + RED_CHUNK - 1) / RED_CHUNK);
-    k_reduce<true><<<nb1, 256, 0, c->stream>>>(L.pmin.p, (int)ntile, L.red2.p);
-    k_dt_final<<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p, L.time, p->time_accurate);
+    k_reduce<true><<<nb1, 256, 0, c->stream>>>(L.pmin.p, (int)ntile, L.red2.p, halt);
+    k_dt_final<<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p, L.clk.p, L.cnt.p, p->time_accurate);
   } else {
-    k_dt_final<<<1, 256, 0, c->stream>>>(L.pmin.p, (int)ntile, c->scal.p, L.time, p->time_accurate);
+    k_dt_final<<<1, 256, 0, c->stream>>>(L.pmin.p, (int)ntile, c->scal.p, L.clk.p, L.cnt.p, p->time_accurate);
   }
   prof_end(c, pid);
   HIPCHK(hipGetLastError());
-  double sval[MAX_SERIES] = {0};
-  if (p->time_accurate) {
-    HIPCHK(hipMemcpyAsync(c->hscal, c->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    L.time = c->hscal[1];
-  }
-  for (size_t s = 0; s < c->series.size(); s++)
-    sval[s] = series_value(c->series[s].periodic, (int)c->series[s].t.size(), c->series[s].t.data(),
-                           c->series[s].v.data(), L.time);
   const int o = L.cur, a1 = (L.cur + 1) % 3, a2 = (L.cur + 2) % 3;
   double* in = L.Tb[o].p;
   int npart = (int)ntile;  // partial sums written by the kernel that applies the last update
@@ -1996,7 +2108,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     sa.mat = L.mat.p;
     for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
     sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.tab = dtab<double>(c);
-    for (int s = 0; s < MAX_SERIES; s++) sa.sval[s] = sval[s];
+    sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = halt;
     if (L.any_active && L.nbound) {
       pid = prof_begin(c, FUSS_K_BC);
       if (L.kp == GC) k_bcflux<3, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound);
@@ -2029,6 +2141,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
       }
       ia.T = in; ia.TO = L.Tb[o].p; ia.Tout = out; ia.mat = L.mat.p; ia.tab = dtab<double>(c);
       ia.rkc = sa.rkc; ia.jam = sa.jam; ia.last = sa.last; ia.partials = L.partials.p; ia.status = c->status.p;
+      ia.halt = halt;
       if (L.no_ycon && c->ndir >= 2) {
         // x and y in one LDS pass; 2-D levels update there, 3-D levels then
         // march the z pass with the update
@@ -2088,8 +2201,8 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   if (p->compute_residual) {
     pid = prof_begin(c, FUSS_K_REDUCE);
     const int nb1 = (npart + RED_CHUNK - 1) / RED_CHUNK;
-    k_reduce<false><<<nb1, 256, 0, c->stream>>>(L.partials.p, npart, L.red2.p);
-    k_reduce<false><<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p + 2);
+    k_reduce<false><<<nb1, 256, 0, c->stream>>>(L.partials.p, npart, L.red2.p, halt);
+    k_reduce<false><<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p + 2, halt);
     prof_end(c, pid);
     HIPCHK(hipGetLastError());
   }
@@ -2328,9 +2441,7 @@ int refine_run(fuss_ctx* c, const fuss_step_params* mon, fuss_refine_result* out
     sa.qvol = L.qvol.p; sa.vol = L.vol.p; sa.mat = L.mat.p;
     for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
     sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.tab = dtab<double>(c);
-    for (size_t s = 0; s < c->series.size() && s < (size_t)MAX_SERIES; s++)
-      sa.sval[s] = series_value(c->series[s].periodic, (int)c->series[s].t.size(), c->series[s].t.data(),
-                                c->series[s].v.data(), L.time);
+    sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = L.cnt.p + 1;
     sa.partials = L.partials.p; sa.status = c->status.p;
     const bool d3 = L.kp == GC;
     if (L.nbound) {
@@ -2378,7 +2489,7 @@ int refine_run(fuss_ctx* c, const fuss_step_params* mon, fuss_refine_result* out
 void adj_free(Adjoint& A) {
   A.probe.free(); A.coff.free(); A.ckpt.free(); A.seg.free();
   for (auto& b : A.stage) b.free();
-  A.lam.free(); A.part.free(); A.red.free(); A.acc.free(); A.amin.free();
+  A.lam.free(); A.part.free(); A.red.free(); A.acc.free(); A.amin.free(); A.dtimes.free(); A.diters.free();
   A.ready = 0;
   A.forward_done = 0;
 }
@@ -2413,7 +2524,8 @@ int adj_setup(fuss_ctx* c, int nsteps, int K, int nprobe, const int* bijk, int p
   const size_t ntile = L.htiles.size();
   int e = A.probe.alloc(std::max(nprobe, 1)) | A.coff.alloc(L.nb) | A.ckpt.alloc(((nsteps - 1) / K + 2) * np) |
           A.seg.alloc((size_t)K * np) | A.lam.alloc(3 * one) | A.part.alloc(2 * ntile) |
-          A.red.alloc(2 * (ntile / RED_CHUNK + 2)) | A.acc.alloc(2) | A.amin.alloc(ntile);
+          A.red.alloc(2 * (ntile / RED_CHUNK + 2)) | A.acc.alloc(2) | A.amin.alloc(ntile) |
+          A.dtimes.alloc(nsteps) | A.diters.alloc(nsteps);
   for (auto& b : A.stage) e |= b.alloc(np);
   if (e) { adj_free(A); return FUSS_EHIP; }
   if (nprobe) HIPCHK(hipMemcpy(A.probe.p, A.hprobe.data(), nprobe * sizeof(long), hipMemcpyHostToDevice));
@@ -2436,8 +2548,7 @@ int adj_forward(fuss_ctx* c, const fuss_step_params* p, double* hist) {
     if (n % A.K == 0)
       HIPCHK(hipMemcpyAsync(A.ckpt.p + (size_t)(n / A.K) * np, L.Tb[L.cur].p, np * sizeof(double),
                             hipMemcpyDeviceToDevice, c->stream));
-    A.times[n] = L.time;
-    A.iters[n] = L.iter;
+    k_clock_rec<<<1, 64, 0, c->stream>>>(L.clk.p, L.cnt.p, A.dtimes.p + n, A.diters.p + n);
     const int e = enqueue_step(c, 1, &A.p);
     if (e) return e;
     if (A.nprobe && hist) {
@@ -2451,6 +2562,9 @@ int adj_forward(fuss_ctx* c, const fuss_step_params* p, double* hist) {
                         hipMemcpyDeviceToDevice, c->stream));
   const int st = sync_status(c);
   if (st) return st;
+  HIPCHK(hipMemcpy(A.times.data(), A.dtimes.p, A.nsteps * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(A.iters.data(), A.diters.p, A.nsteps * sizeof(int), hipMemcpyDeviceToHost));
+  if (read_clock(c, L)) return FUSS_EHIP;
   A.forward_done = 1;
   return 0;
 }
@@ -2494,24 +2608,18 @@ int adj_backward_t(fuss_ctx* c, const double* g, double* gqw, double* gk, double
       HIPCHK(hipMemcpyAsync(A.seg.p + (size_t)(n - n0) * np, L.Tb[L.cur].p, np * sizeof(double),
                             hipMemcpyDeviceToDevice, c->stream));
       if (n < n1 - 1) {
-        L.time = A.times[n];
-        L.iter = A.iters[n];
+        set_clock(c, L, A.times[n], A.iters[n]);
         if ((err = enqueue_step(c, 1, &A.p))) break;
       }
     }
     for (int n = n1 - 1; n >= n0 && !err; n--) {
       const double* Tn = A.seg.p + (size_t)(n - n0) * np;
       HIPCHK(hipMemcpyAsync(L.Tb[L.cur].p, Tn, np * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-      L.time = A.times[n];
-      L.iter = A.iters[n];
+      set_clock(c, L, A.times[n], A.iters[n]);
       c->adj_store_on = 1;
       err = enqueue_step(c, 1, &A.p);
       c->adj_store_on = 0;
       if (err) break;
-      double sval[MAX_SERIES] = {0};
-      for (size_t q = 0; q < c->series.size() && q < (size_t)MAX_SERIES; q++)
-        sval[q] = series_value(c->series[q].periodic, (int)c->series[q].t.size(), c->series[q].t.data(),
-                               c->series[q].v.data(), L.time);
       for (int sidx = nrk; sidx >= 1; sidx--) {
         StageArgs<double> sa;
         memset(&sa, 0, sizeof(sa));
@@ -2519,7 +2627,7 @@ int adj_backward_t(fuss_ctx* c, const double* g, double* gqw, double* gk, double
         sa.mat = L.mat.p;
         for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
         sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.tab = dtab<double>(c); sa.status = c->status.p;
-        for (int q = 0; q < MAX_SERIES; q++) sa.sval[q] = sval[q];
+        sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = L.cnt.p + 1;
         if (L.nbound && L.any_active)
           k_bcflux<NDIR, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound);
         AdjArgs<S> aa;
@@ -2556,8 +2664,7 @@ int adj_backward_t(fuss_ctx* c, const double* g, double* gqw, double* gk, double
   // the level returns to T^N and its clock
   HIPCHK(hipMemcpyAsync(L.Tb[L.cur].p, A.ckpt.p + (size_t)((N - 1) / K + 1) * np, np * sizeof(double),
                         hipMemcpyDeviceToDevice, c->stream));
-  L.time = t_end;
-  L.iter = it_end;
+  set_clock(c, L, t_end, it_end);
   const int st = sync_status(c);
   gq_own.free();
   return err ? err : st;
@@ -2565,6 +2672,12 @@ int adj_backward_t(fuss_ctx* c, const double* g, double* gqw, double* gk, double
 }  // namespace
 
 extern "C" {
+
+int fuss_graphs(fuss_ctx* c, int enable) {
+  if (!c) return FUSS_EARG;
+  c->graphs_off = !enable;
+  return 0;
+}
 
 int fuss_set_kscale(fuss_ctx* c, double scale) {
   if (!c || !c->nT || !(scale > 0.0)) return FUSS_EARG;
@@ -2643,9 +2756,12 @@ int fuss_step(fuss_ctx* c, int level, const fuss_step_params* p, fuss_step_resul
   if (e) return e;
   Level& L = c->lev[level - 1];
   HIPCHK(hipMemcpyAsync(c->hscal, c->scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hscal + 6, L.clk.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hcnt, L.cnt.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   int bijk[4] = {0, 0, 0, 0};
   const int st = sync_status(c, bijk);
-  if (p->time_accurate) L.time = c->hscal[1];
+  L.time = c->hscal[6];
+  L.iter = c->hcnt[0];
   if (r) {
     r->res_sq = p->compute_residual ? c->hscal[2] : 0.0;
     r->dtglobal = c->hscal[0];
@@ -2666,22 +2782,96 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
   int n = 0;
   fuss_step_params q = *p;
   q.compute_residual = 1;
-  while (n < max_iters) {
-    const int e = fuss_step(c, level, &q, &r);
-    if (e) {
-      if (n_done) *n_done = n + 1;
-      if (last) *last = r;
-      return e;
+  // steps this call may take: the loop below stops after the first step with
+  // n >= itermax_left, so at least one step runs when max_iters > 0
+  const int maxcount = max_iters <= 0 ? 0 : std::min(max_iters, std::max(1, itermax_left));
+  if (c->prof.enabled || maxcount < RUN_G || c->graphs_off) {
+    while (n < max_iters) {
+      const int e = fuss_step(c, level, &q, &r);
+      if (e) {
+        if (n_done) *n_done = n + 1;
+        if (last) *last = r;
+        return e;
+      }
+      const double res = sqrt(r.res_sq);
+      if (hist_res) hist_res[n] = res;
+      if (hist_time) hist_time[n] = r.time;
+      n++;
+      if (n >= itermax_left || res <= res_threshold || r.time >= time_threshold) break;
     }
-    const double res = sqrt(r.res_sq);
-    if (hist_res) hist_res[n] = res;
-    if (hist_time) hist_time[n] = r.time;
-    n++;
-    if (n >= itermax_left || res <= res_threshold || r.time >= time_threshold) break;
+    if (n_done) *n_done = n;
+    if (last) *last = r;
+    return 0;
   }
-  if (n_done) *n_done = n;
+  // Graph mode: RUN_G steps (a multiple of 3, so the rotating T buffers come
+  // back to their starting roles) captured once and replayed; the stop tests
+  // run on the device (k_step_end) and later steps of the chunk become no-ops.
+  Level& L = c->lev[level - 1];
+  k_run_begin<<<1, 64, 0, c->stream>>>(L.cnt.p);
+  HIPCHK(hipGetLastError());
+  int cur_seq[RUN_G + 1];
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int e = 0;
+  {
+    k_chunk_begin<<<1, 64, 0, c->stream>>>(L.cnt.p);
+    cur_seq[0] = L.cur;
+    for (int s = 0; s < RUN_G && !e; s++) {
+      e = enqueue_step(c, level, &q);
+      k_step_end<<<1, 64, 0, c->stream>>>(c->scal.p, L.clk.p, L.cnt.p, c->status.p, c->run_hist.p,
+                                          c->run_hist.p + RUN_G, res_threshold, time_threshold, maxcount);
+      cur_seq[s + 1] = L.cur;
+    }
+  }
+  const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+  if (e || ce != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    L.cur = cur_seq[0];
+    return e ? e : FUSS_EHIP;
+  }
+  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    L.cur = cur_seq[0];
+    return FUSS_EHIP;
+  }
+  L.cur = cur_seq[0];
+  int done = 0, halted = 0;
+  while (!halted) {
+    if (hipGraphLaunch(exec, c->stream) != hipSuccess) { e = FUSS_EHIP; break; }
+    HIPCHK(hipMemcpyAsync(c->hcnt, L.cnt.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hrun, c->run_hist.p, 2 * RUN_G * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int k = c->hcnt[2] - done;
+    for (int s = 0; s < k; s++) {
+      if (hist_res) hist_res[done + s] = c->hrun[s];
+      if (hist_time) hist_time[done + s] = c->hrun[RUN_G + s];
+    }
+    done += k;
+    halted = c->hcnt[1];
+    L.cur = cur_seq[halted ? k : RUN_G];
+  }
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (e) return e;
+  HIPCHK(hipMemcpyAsync(c->hscal, c->scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hcnt, L.cnt.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hscal + 6, L.clk.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  int bijk[4] = {0, 0, 0, 0};
+  const int st = sync_status(c, bijk);
+  // leave the halt flag down for the next caller
+  k_run_begin<<<1, 64, 0, c->stream>>>(L.cnt.p);
+  L.time = c->hscal[6];
+  L.iter = c->hcnt[0];
+  r.res_sq = c->hscal[2];
+  r.dtglobal = c->hscal[0];
+  r.time = L.time;
+  r.iter = L.iter;
+  r.status = st;
+  for (int q2 = 0; q2 < 4; q2++) r.bijk[q2] = bijk[q2];
+  if (n_done) *n_done = done;
   if (last) *last = r;
-  return 0;
+  return st;
 }
 
 int fuss_restrict(fuss_ctx* c, int fine_level) {
@@ -2768,6 +2958,7 @@ int fuss_probe(fuss_ctx* c, int n, const int* bijk, double* out) {
 
 int fuss_get_clock(fuss_ctx* c, int level, double* time, int* iter) {
   if (!c || level < 1 || level > c->nlev) return FUSS_EARG;
+  if (c->setup_done && read_clock(c, c->lev[level - 1])) return FUSS_EHIP;
   if (time) *time = c->lev[level - 1].time;
   if (iter) *iter = c->lev[level - 1].iter;
   return 0;
@@ -2777,6 +2968,7 @@ int fuss_set_clock(fuss_ctx* c, int level, double time, int iter) {
   if (!c || level < 1 || level > c->nlev) return FUSS_EARG;
   c->lev[level - 1].time = time;
   c->lev[level - 1].iter = iter;
+  if (c->setup_done) return set_clock(c, c->lev[level - 1], time, iter);
   return 0;
 }
 
@@ -2892,6 +3084,10 @@ void fuss_destroy(fuss_ctx* c) {
   refine_free(c->ref);
   adj_free(c->adj);
   c->stage_i.free(); c->stage_d.free();
+  c->ser_t.free(); c->ser_v.free(); c->ser_off.free(); c->ser_n.free(); c->ser_per.free(); c->run_hist.free();
+  for (auto& L : c->lev) { L.clk.free(); L.cnt.free(); }
+  if (c->hrun) (void)hipHostFree(c->hrun);
+  if (c->hcnt) (void)hipHostFree(c->hcnt);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hstatus) (void)hipHostFree(c->hstatus);
   for (auto e : c->prof.pool) (void)hipEventDestroy(e);

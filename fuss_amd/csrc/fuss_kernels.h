@@ -20,6 +20,7 @@ constexpr int TILE_X = 64;
 constexpr int TILE_Y = 4;
 constexpr int TILE_T = TILE_X * TILE_Y;
 constexpr int MAX_SERIES = 16;
+constexpr int RUN_G = 30;            // steps per captured graph of fuss_run (multiple of 3)
 
 struct BlockInfo {
   int ni, nj, nk;
@@ -38,6 +39,15 @@ struct BlockInfo {
 
 struct Tile {
   int b, i0, j0, k0;
+};
+
+// Time series of 301/302 values on the device (one concatenated table).
+struct SerDev {
+  const double* t;
+  const double* v;
+  const int* off;      // first point of series s
+  const int* n;        // points of series s
+  const int* per;      // periodic flag
 };
 
 // One BC entry, geometry gathered at setup.  Reference fields: bc_type /
@@ -180,8 +190,8 @@ __device__ __forceinline__ R tab_inverse_search(const R* __restrict__ row, R hv,
   return (R)(lo - 1) + (hv - hm) / (hp - hm);
 }
 
-// time_series_type%update, src/lib/base/Series_Data_m.f90:55-86 (host)
-inline double series_value(int periodic, int n, const double* t, const double* v, double time) {
+// time_series_type%update, src/lib/base/Series_Data_m.f90:55-86
+__host__ __device__ inline double series_value(int periodic, int n, const double* t, const double* v, double time) {
   double t_ = time;
   if (periodic) {
     const double period = t[n - 1] - t[0];

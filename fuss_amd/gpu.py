@@ -95,6 +95,7 @@ def load():
     L.fuss_get_stream.argtypes = [P, C.POINTER(P)]
     L.fuss_set_stream.argtypes = [P, P]
     L.fuss_memory.argtypes = [P, C.POINTER(C.c_double)]
+    L.fuss_graphs.argtypes = [P, C.c_int]
     L.fuss_refine_setup.argtypes = [P, C.POINTER(RefineParams)]
     L.fuss_refine.argtypes = [P, C.POINTER(StepParams), C.POINTER(RefineResult), C.c_void_p]
     L.fuss_strerror.restype = C.c_char_p
@@ -156,6 +157,10 @@ class Solver:
                  f"fuss_set_bc level {m+1}")
         _chk(self.L.fuss_setup(self.h, 0.0 if self.p.time_accurate else -1.0), "fuss_setup")
         self.level = self.p.mgl
+
+    def graphs(self, enable):
+        """fuss_run on captured HIP graphs (default) or step by step."""
+        _chk(self.L.fuss_graphs(self.h, int(bool(enable))))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:

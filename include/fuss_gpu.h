@@ -119,6 +119,12 @@ int fuss_run(fuss_ctx *ctx, int level, const fuss_step_params *p, int max_iters,
              int itermax_left, double res_threshold, double time_threshold, int *n_done,
              double *hist_res, double *hist_time, fuss_step_result *last);
 
+/* fuss_run replays its steps from a captured HIP graph (30 steps per graph,
+ * stop tests evaluated on the device, so it stops at the same iteration as a
+ * step-by-step loop) unless profiling is on, fewer than 30 steps remain, or
+ * graphs are switched off here (enable = 0).  Default: on. */
+int fuss_graphs(fuss_ctx *ctx, int enable);
+
 /* Restriction (Mod_Multigrid.f90:59-77): coarse T, qvol, matID from level
  * fine_level to fine_level+1. */
 int fuss_restrict(fuss_ctx *ctx, int fine_level);

@@ -170,3 +170,23 @@ def test_mg_irs_trajectory(gpu):
     print(f"mg_irs: {n} iterations, max |dres|/tol = {np.max(np.abs(hist[:,2]-ref)/tol):.3f}", flush=True)
     assert not bad.any(), f"first mismatch at iteration {np.argmax(bad)+1}"
     s.close()
+
+
+@pytest.mark.parametrize("name", ["convective_Plate", "oscillating_Rod"])
+def test_graph_run_equals_step_loop(gpu, name):
+    """fuss_run replayed from captured graphs (device-side stop tests and
+    time series) stops at the same step with the same state, bit for bit, as
+    the step-by-step loop."""
+    c, _ = fixtures.load(name)
+    out = []
+    for graphs in (True, False):
+        s = gpu.Solver(c)
+        s.graphs(graphs)
+        k, hr, ht, r = s.run_level(1, 500, 500 if name == "oscillating_Rod" else 10**9, c.params.res_threshold,
+                                   c.params.time_threshold)
+        out.append((k, hr.copy(), ht.copy(), s.get("T"), s.clock()))
+        s.close()
+    (k1, hr1, ht1, T1, c1), (k2, hr2, ht2, T2, c2) = out
+    assert k1 == k2 and c1 == c2
+    assert np.array_equal(hr1, hr2) and np.array_equal(ht1, ht2)
+    assert np.array_equal(T1, T2)
