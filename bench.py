@@ -220,6 +220,83 @@ def adjoint_check(args):
     return out
 
 
+def sweep_check(args):
+    """Parameter sweep of independent steady cubes (k, h, qvol drawn per case,
+    uneven 32^3-96^3 blocks) as one batched context: the plain FP64
+    Explicit_Step over all cases in one launch per pass, and the steady solve
+    of every case to its own residual 1e-10 by FP64 defect correction with
+    FP32 (and, as the FP64 solution, FP64) inner sweeps."""
+    from fuss_amd import cases, gpu
+
+    t0 = time.perf_counter()
+    case = cases.sweep(ncase=args.sweep_cases, lo=32, hi=96, mgl=3, n_rk=args.rk)
+    s = gpu.Solver(case)
+    setup_s = time.perf_counter() - t0
+    ncell = case.ncell
+    for _ in range(2):
+        s.explicit_step(1)
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.sweep_steps):
+        s.explicit_step(1)
+    device_sync()
+    step_s = (time.perf_counter() - t0) / args.sweep_steps
+    T0 = [b.T for b in case.blocks]
+    out = {"workload": f"{args.sweep_cases} independent steady cubes, dims 32..96 (multiples of 4), "
+                       f"k 5-200, h 10-500, qvol 0-1e6, RK{args.rk} + IRS 0.5, one context",
+           "cells": ncell, "setup_s": setup_s, "plain_fp64_ms_per_step": step_s * 1e3,
+           "plain_fp64_cell_updates_per_s": ncell / step_s}
+    finals = {}
+    for prec, name in ((gpu.FUSS_FP32, "fp32_refined"), (gpu.FUSS_FP64, "fp64_refined")):
+        rp = gpu.refine_params(precision=prec, max_outer=80, res_threshold=1e-10, per_block=True)
+        gpu._chk(s.L.fuss_refine_setup(s.h, gpu.C.byref(rp)), "fuss_refine_setup")
+        for b, T in enumerate(T0):
+            s.set_T(T, 1, b + 1)
+        device_sync()
+        t0 = time.perf_counter()
+        r, _ = s.refine(rp, setup=False)
+        device_sync()
+        dt = time.perf_counter() - t0
+        conv, res = s.refine_blocks()
+        finals[name] = s.T()
+        out[name] = {"seconds": dt, "all_converged": bool(r.converged), "outer": r.outer, "vcycles": r.vcycles,
+                     "levels": r.levels, "max_case_residual": float(res.max()),
+                     "case_outer_at_convergence": {"min": int(conv.min()), "median": float(np.median(conv)),
+                                                   "max": int(conv.max())}}
+        print(f"[bench] sweep {name}: {dt*1e3:.1f} ms, {r.outer} outer, max case residual {res.max():.2e}",
+              file=sys.stderr, flush=True)
+    errs = np.array([float(np.max(np.abs(a - b) / np.abs(b))) for a, b in zip(finals["fp32_refined"],
+                                                                               finals["fp64_refined"])])
+    out["per_case_max_rel_err_fp32_vs_fp64"] = {"max": float(errs.max()), "median": float(np.median(errs))}
+    s.close()
+    return out
+
+
+def lattice_check(args):
+    """Transient 4x4x4 block lattice (uneven per-slab dims 16-128, 1-to-1
+    connections, curvilinear, 3 tabulated materials, 302/301/304 walls),
+    global-dt RK4 for `lattice_steps` steps through fuss_run (captured graphs)."""
+    from fuss_amd import cases, gpu
+
+    t0 = time.perf_counter()
+    case = cases.lattice(nl=4, lo=16, hi=128, n_rk=4, time_accurate=True)
+    s = gpu.Solver(case)
+    setup_s = time.perf_counter() - t0
+    s.run_level(1, 30, 30, -1.0, 1e300, hist=False)   # warm-up (graph capture, module load)
+    device_sync()
+    t0 = time.perf_counter()
+    k, _, _, r = s.run_level(1, args.lattice_steps, args.lattice_steps, -1.0, 1e300, hist=False)
+    device_sync()
+    dt = time.perf_counter() - t0
+    out = {"workload": "transient 4x4x4 lattice of 64 connected curvilinear blocks (slab dims 16-128), "
+                       "3 materials, 302/301/304 walls, RK4 global dt, FP64",
+           "cells": case.ncell, "blocks": case.nb, "steps": k, "seconds": dt, "setup_s": setup_s,
+           "cell_updates_per_s": case.ncell * k / dt, "simulated_time_s": r.time}
+    print(f"[bench] lattice: {case.ncell} cells x {k} steps in {dt:.2f} s", file=sys.stderr, flush=True)
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +311,10 @@ def main():
     ap.add_argument("--no-refine", action="store_true", help="skip the time-to-1e-10 runs")
     ap.add_argument("--plain-conv-n", type=int, default=32, help="cube size of the live plain run to 1e-10 (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (inverse problem) measurement")
+    ap.add_argument("--no-batched", action="store_true", help="skip the sweep and lattice measurements")
+    ap.add_argument("--sweep-cases", type=int, default=256)
+    ap.add_argument("--sweep-steps", type=int, default=5)
+    ap.add_argument("--lattice-steps", type=int, default=1000)
     ap.add_argument("--adj-n", type=int, default=512)
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--adj-lbfgs", type=int, default=20)
@@ -307,6 +388,9 @@ def main():
     s.close()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_check(args)
+    if not args.no_batched:
+        out["sweep"] = sweep_check(args)
+        out["lattice"] = lattice_check(args)
     if args.calib_copy:
         calibration_copy()
     if not args.no_cpu_baseline:

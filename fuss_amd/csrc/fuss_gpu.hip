@@ -1072,6 +1072,23 @@ __global__ __launch_bounds__(256) void k_dt_final(const double* __restrict__ in,
   }
 }
 
+// Per-block sums of per-tile partials (tile lists are ordered by block):
+// work-group b sums partials [rng[b], rng[b+1]) in a fixed order.
+__global__ __launch_bounds__(256) void k_block_sums(const double* __restrict__ part, const int* __restrict__ rng,
+                                                    double* __restrict__ out) {
+  __shared__ double sm[256];
+  const int b = blockIdx.x;
+  double v = 0.0;
+  for (int q = rng[b] + threadIdx.x; q < rng[b + 1]; q += 256) v += part[q];
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[b] = sm[0];
+}
+
 // Level clock from the host (setup, fuss_set_clock, adjoint recompute).
 __global__ void k_set_clock(double* clk, int* cnt, double time, int iter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -1247,6 +1264,9 @@ struct Level {
   std::vector<Tile> htiles;         // one 64x4 cell tile per plane
   std::vector<Tile> hctiles;        // 64x4 columns of KC planes (3-D levels)
   std::vector<Tile> hxytiles;       // 64x16 plane tiles of the x-y IRS pass
+  std::vector<int> hrng[3];         // per tile list (tiles, ctiles, xytiles): first tile of each block
+  DBuf<int> rng[3];
+  int plist = 0;                    // tile list of the last step's residual partials
   int no_ycon = 1;                  // no connection on a j face (fused x-y IRS allowed)
   long npad = 0, ncell = 0, nnodes = 0;
   DBuf<BlockInfo> blk;
@@ -1288,6 +1308,10 @@ struct Refine {
   fuss_refine_params p{};
   std::vector<CLevel> lv;
   DBuf<double> bcd, red;
+  DBuf<double> bres;                // per-block monitor residual (independent cases)
+  double* hbres = nullptr;          // pinned
+  std::vector<int> conv_outer;      // first outer iteration at which each block met the threshold
+  std::vector<double> res;          // last monitor residual per block
 };
 
 // Checkpointed forward and reverse sweep of the time-accurate step.
@@ -1486,6 +1510,12 @@ int build_level_layout(fuss_ctx* c, Level& L) {
   }
   L.npad = off;
   L.nnodes = noff;
+  const std::vector<Tile>* lists[3] = {&L.htiles, &L.hctiles, &L.hxytiles};
+  for (int q = 0; q < 3; q++) {
+    L.hrng[q].assign(L.nb + 1, (int)lists[q]->size());
+    for (int t = (int)lists[q]->size() - 1; t >= 0; t--) L.hrng[q][(*lists[q])[t].b] = t;
+    for (int b = L.nb - 1; b >= 0; b--) L.hrng[q][b] = std::min(L.hrng[q][b], L.hrng[q][b + 1]);
+  }
   (void)c;
   // kernels address a level's arrays with 32-bit byte offsets
   if (L.npad * 8 >= (1L << 32)) return FUSS_EUNSUPPORTED;
@@ -1514,6 +1544,10 @@ int alloc_level(fuss_ctx* c, Level& L) {
   e |= L.pmin.alloc(np);
   e |= L.clk.alloc(2);
   e |= L.cnt.alloc(4);
+  for (int q = 0; q < 3; q++) {
+    e |= L.rng[q].alloc(L.nb + 1);
+    if (!e) e |= upload(L.rng[q].p, L.hrng[q].data(), (L.nb + 1) * sizeof(int));
+  }
   e |= L.red2.alloc(np / RED_CHUNK + 2);
   (void)c;
   return e ? FUSS_EHIP : 0;
@@ -2095,6 +2129,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   const int o = L.cur, a1 = (L.cur + 1) % 3, a2 = (L.cur + 2) % 3;
   double* in = L.Tb[o].p;
   int npart = (int)ntile;  // partial sums written by the kernel that applies the last update
+  L.plist = 0;
   for (int irk = 1; irk <= nrk; irk++) {
     double* out = (irk % 2 == 1) ? L.Tb[a1].p : L.Tb[a2].p;
     int e = launch_ghost(c, L, in);
@@ -2124,7 +2159,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     pid = prof_begin(c, FUSS_K_STAGE);
     if (L.kp == GC) {
       launch_stage3d(c, L, sa, L.tang != 0, prim, irs);
-      if (!irs) npart = (int)L.hctiles.size();
+      if (!irs) { npart = (int)L.hctiles.size(); L.plist = 1; }
     } else {
       launch_stage_n<2>(c, L, sa, L.tang != 0, prim, irs);
     }
@@ -2153,7 +2188,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
         if (c->ndir == 2) {
           prim ? k_irs_xy<2, true, true, double><<<gxy, TILE_T, 0, c->stream>>>(ia)
                : k_irs_xy<2, true, false, double><<<gxy, TILE_T, 0, c->stream>>>(ia);
-          if (sa.last) npart = (int)gxy;
+          if (sa.last) { npart = (int)gxy; L.plist = 2; }
         } else {
           prim ? k_irs_xy<3, false, true, double><<<gxy, TILE_T, 0, c->stream>>>(ia)
                : k_irs_xy<3, false, false, double><<<gxy, TILE_T, 0, c->stream>>>(ia);
@@ -2168,7 +2203,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
                : k_irs_z<false, double><<<(unsigned)L.hctiles.size(), TILE_T, 0, c->stream>>>(ia);
           prof_end(c, pid);
           HIPCHK(hipGetLastError());
-          if (sa.last) npart = (int)L.hctiles.size();
+          if (sa.last) { npart = (int)L.hctiles.size(); L.plist = 1; }
         }
       } else {
         double* rin = L.R.p;
@@ -2192,7 +2227,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
                : k_irs_z<false, double><<<(unsigned)L.hctiles.size(), TILE_T, 0, c->stream>>>(ia);
           prof_end(c, pid);
           HIPCHK(hipGetLastError());
-          if (sa.last) npart = (int)L.hctiles.size();
+          if (sa.last) { npart = (int)L.hctiles.size(); L.plist = 1; }
         }
       }
     }
@@ -2242,6 +2277,9 @@ void refine_free(Refine& R) {
   R.lv.clear();
   R.bcd.free();
   R.red.free();
+  R.bres.free();
+  if (R.hbres) (void)hipHostFree(R.hbres);
+  R.hbres = nullptr;
   R.ready = 0;
 }
 
@@ -2300,7 +2338,9 @@ int refine_build(fuss_ctx* c, const fuss_refine_params* rp) {
     HIPCHK(hipMemcpy(cl.blk.p, cl.hblk.data(), cl.hblk.size() * sizeof(CBlk), hipMemcpyHostToDevice));
     cl.cur = 0;
   }
-  if (R.bcd.alloc(std::max(L.nbound, 1)) || R.red.alloc(2)) return FUSS_EHIP;
+  if (R.bcd.alloc(std::max(L.nbound, 1)) || R.red.alloc(2) || R.bres.alloc(nb) ||
+      hipHostMalloc((void**)&R.hbres, nb * sizeof(double), 0) != hipSuccess)
+    return FUSS_EHIP;
   R.p = *rp;
   R.ready = 1;
   return 0;
@@ -2411,12 +2451,18 @@ int refine_run(fuss_ctx* c, const fuss_step_params* mon, fuss_refine_result* out
   const unsigned ntile = (unsigned)L.htiles.size();
   *out = fuss_refine_result{};
   out->levels = nl;
+  R.conv_outer.assign(L.nb, -1);
+  R.res.assign(L.nb, 0.0);
   bool have_def = false;
   for (int m = 0;; m++) {
     // monitor: one FP64 Explicit_Step; its residual is the reference's
     int e = enqueue_step(c, 1, &pm);
     if (e) return e;
+    const int nb = L.nb;
+    k_block_sums<<<nb, 256, 0, c->stream>>>(L.partials.p, L.rng[L.plist].p, R.bres.p);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->hscal, c->scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(R.hbres, R.bres.p, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (have_def) HIPCHK(hipMemcpyAsync(c->hscal + 4, R.red.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     int bijk[4] = {0, 0, 0, 0};
     const int st = sync_status(c, bijk);
@@ -2426,7 +2472,18 @@ int refine_run(fuss_ctx* c, const fuss_step_params* mon, fuss_refine_result* out
     if (hist) hist[m] = res;
     out->res = res;
     out->outer = m;
-    if (res <= rp.res_threshold) {
+    int all = 1;
+    for (int b = 0; b < nb; b++) {
+      R.res[b] = sqrt(R.hbres[b]);
+      if (R.res[b] <= rp.res_threshold) {
+        if (R.conv_outer[b] < 0) R.conv_outer[b] = m;
+      } else {
+        all = 0;
+      }
+    }
+    // one domain: the reference's residual over all blocks; independent
+    // cases (per_block): every block's own residual
+    if (rp.per_block ? all : res <= rp.res_threshold) {
       out->converged = 1;
       break;
     }
@@ -2737,6 +2794,16 @@ int fuss_adj_backward(fuss_ctx* c, const double* probe_adj, double* grad_qw, dou
 int fuss_refine_setup(fuss_ctx* c, const fuss_refine_params* rp) {
   if (!c || !c->setup_done || !rp) return FUSS_EARG;
   return refine_build(c, rp);
+}
+
+int fuss_refine_blocks(fuss_ctx* c, int* conv_outer, double* res) {
+  if (!c || !c->ref.ready) return FUSS_EARG;
+  const Refine& R = c->ref;
+  for (size_t b = 0; b < R.conv_outer.size(); b++) {
+    if (conv_outer) conv_outer[b] = R.conv_outer[b];
+    if (res) res[b] = R.res[b];
+  }
+  return 0;
 }
 
 int fuss_refine(fuss_ctx* c, const fuss_step_params* monitor, fuss_refine_result* out, double* hist_res) {
@@ -3085,7 +3152,10 @@ void fuss_destroy(fuss_ctx* c) {
   adj_free(c->adj);
   c->stage_i.free(); c->stage_d.free();
   c->ser_t.free(); c->ser_v.free(); c->ser_off.free(); c->ser_n.free(); c->ser_per.free(); c->run_hist.free();
-  for (auto& L : c->lev) { L.clk.free(); L.cnt.free(); }
+  for (auto& L : c->lev) {
+    L.clk.free(); L.cnt.free();
+    for (auto& r : L.rng) r.free();
+  }
   if (c->hrun) (void)hipHostFree(c->hrun);
   if (c->hcnt) (void)hipHostFree(c->hcnt);
   if (c->hscal) (void)hipHostFree(c->hscal);

@@ -48,6 +48,7 @@ module fuss_gpu_m
     integer(c_int) :: ncoarse = 20
     integer(c_int) :: max_outer = 200
     real(c_double) :: res_threshold = 1d-10
+    integer(c_int) :: per_block = 0
   end type
 
   !> outcome of fuss_refine (fuss_refine_result)

@@ -42,7 +42,8 @@ class StepResult(C.Structure):
 class RefineParams(C.Structure):
     _fields_ = [("precision", C.c_int), ("n_stage", C.c_int), ("lam_lo", C.c_double), ("lam_hi", C.c_double),
                 ("irs_beta", C.c_double), ("nu1", C.c_int), ("nu2", C.c_int), ("vcycles", C.c_int),
-                ("max_levels", C.c_int), ("ncoarse", C.c_int), ("max_outer", C.c_int), ("res_threshold", C.c_double)]
+                ("max_levels", C.c_int), ("ncoarse", C.c_int), ("max_outer", C.c_int), ("res_threshold", C.c_double),
+                ("per_block", C.c_int)]
 
 
 class RefineResult(C.Structure):
@@ -51,12 +52,12 @@ class RefineResult(C.Structure):
 
 
 def refine_params(precision=FUSS_FP32, n_stage=4, lam_lo=0.3, lam_hi=2.0, irs_beta=0.0, nu1=1, nu2=1, vcycles=2,
-                  max_levels=0, ncoarse=20, max_outer=200, res_threshold=1e-10):
+                  max_levels=0, ncoarse=20, max_outer=200, res_threshold=1e-10, per_block=False):
     """Defaults: FP32 sweeps, four-stage Chebyshev smoother on [0.3, 2], one
     pre- and one post-smoothing iteration, two V-cycles per outer iteration,
     coarsening down to <= 2 cells per direction."""
     return RefineParams(precision, n_stage, lam_lo, lam_hi, irs_beta, nu1, nu2, vcycles, max_levels, ncoarse,
-                        max_outer, res_threshold)
+                        max_outer, res_threshold, int(per_block))
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -98,6 +99,7 @@ def load():
     L.fuss_graphs.argtypes = [P, C.c_int]
     L.fuss_refine_setup.argtypes = [P, C.POINTER(RefineParams)]
     L.fuss_refine.argtypes = [P, C.POINTER(StepParams), C.POINTER(RefineResult), C.c_void_p]
+    L.fuss_refine_blocks.argtypes = [P, C.c_void_p, C.c_void_p]
     L.fuss_strerror.restype = C.c_char_p
     L.fuss_strerror.argtypes = [C.c_int]
     L.fuss_destroy.argtypes = [P]
@@ -293,3 +295,10 @@ class Solver:
         _chk(self.L.fuss_refine(self.h, C.byref(mon), C.byref(r), hist.ctypes.data_as(C.c_void_p)), "fuss_refine")
         self.level = 1
         return r, hist[: r.outer + 1]
+
+    def refine_blocks(self):
+        """Per block: (first outer iteration meeting the threshold or -1, last residual)."""
+        it = np.zeros(self.case.nb, np.int32)
+        res = np.zeros(self.case.nb)
+        _chk(self.L.fuss_refine_blocks(self.h, it.ctypes.data_as(C.c_void_p), res.ctypes.data_as(C.c_void_p)))
+        return it, res

@@ -153,6 +153,8 @@ typedef struct {
   int ncoarse;         /* smoothing iterations on a coarsest level too large for the direct solve */
   int max_outer;       /* outer iterations (monitor steps - 1) */
   double res_threshold;/* stop when the monitor residual sqrt(res_sq) <= this */
+  int per_block;       /* nonzero: blocks are independent cases, each must meet res_threshold
+                          with its own residual; 0: one residual over all blocks */
 } fuss_refine_params;
 
 typedef struct {
@@ -164,6 +166,9 @@ typedef struct {
   double defect;       /* ||-R + qvol V||_2 (W) before the last correction */
 } fuss_refine_result;
 
+/* Per block: the first outer iteration at which its own monitor residual met
+ * res_threshold (-1: never) and its last monitor residual (nblocks each). */
+int fuss_refine_blocks(fuss_ctx *ctx, int *conv_outer, double *res);
 /* Builds the correction hierarchy (allocates device memory). */
 int fuss_refine_setup(fuss_ctx *ctx, const fuss_refine_params *rp);
 /* Runs the outer iteration; `monitor` are the Explicit_Step numerics of the
