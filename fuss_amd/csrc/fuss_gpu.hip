@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -619,7 +620,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage(StageArgs<R> a) {
 // The residual accumulation keeps the Fluxes_blk order (i, then j, then k
 // faces; Mod_Fluxes.f90:54-192).
 // ------------------------------------------------------------------------
-constexpr int KC = 16;
+constexpr int KC = 8;
 constexpr int PX = TILE_X + 2, PY = TILE_Y + 2;
 
 template <typename R>
@@ -676,13 +677,13 @@ __device__ __forceinline__ const R* kap_row(const StageArgs<R>& a, const BlockIn
 // face_flux (Lib_Diffusive.f90:10-45 regrouped per computational direction).
 template <bool TANG, typename R>
 __device__ __forceinline__ R flux_x(const StageArgs<R>& a, const BlockInfo& B, R (*m)[PX], R (*c)[PX], R (*p)[PX],
-                                    int x, int y, unsigned q) {
+                                    int x, int y, unsigned q, R wn) {
   const R Tl = c[y][x], Tr = c[y][x + 1];
   int bad = 0;
   const R kap = tab_interp2(kap_row(a, B, q), (R)0.5 * (Tl + Tr), a.tab.Ti1, a.tab.Ti2, &bad);
   if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
   const R G0 = Tr - Tl;
-  if (!TANG) return kap * (G0 * ldu(a.W[0], q));
+  if (!TANG) return kap * (G0 * wn);
   const R G1 = (c[y + 1][x] - c[y - 1][x] + c[y + 1][x + 1] - c[y - 1][x + 1]) * (R)0.25;
   const R G2 = (p[y][x] - m[y][x] + p[y][x + 1] - m[y][x + 1]) * (R)0.25;
   return kap * ((G0 * ldu(a.W[0], q) + G1 * ldu(a.W[1], q)) + G2 * ldu(a.W[2], q));
@@ -690,13 +691,13 @@ __device__ __forceinline__ R flux_x(const StageArgs<R>& a, const BlockInfo& B, R
 
 template <bool TANG, typename R>
 __device__ __forceinline__ R flux_y(const StageArgs<R>& a, const BlockInfo& B, R (*m)[PX], R (*c)[PX], R (*p)[PX],
-                                    int x, int y, unsigned q) {
+                                    int x, int y, unsigned q, R wn) {
   const R Tl = c[y][x], Tr = c[y + 1][x];
   int bad = 0;
   const R kap = tab_interp2(kap_row(a, B, q), (R)0.5 * (Tl + Tr), a.tab.Ti1, a.tab.Ti2, &bad);
   if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
   const R G1 = Tr - Tl;
-  if (!TANG) return kap * (G1 * ldu(a.W[4], q));
+  if (!TANG) return kap * (G1 * wn);
   const R G0 = (c[y][x + 1] - c[y][x - 1] + c[y + 1][x + 1] - c[y + 1][x - 1]) * (R)0.25;
   const R G2 = (p[y][x] - m[y][x] + p[y + 1][x] - m[y + 1][x]) * (R)0.25;
   return kap * ((G0 * ldu(a.W[3], q) + G1 * ldu(a.W[4], q)) + G2 * ldu(a.W[5], q));
@@ -704,13 +705,13 @@ __device__ __forceinline__ R flux_y(const StageArgs<R>& a, const BlockInfo& B, R
 
 template <bool TANG, typename R>
 __device__ __forceinline__ R flux_z(const StageArgs<R>& a, const BlockInfo& B, R (*c)[PX], R (*p)[PX], int x, int y,
-                                    unsigned q) {
+                                    unsigned q, R wn) {
   const R Tl = c[y][x], Tr = p[y][x];
   int bad = 0;
   const R kap = tab_interp2(kap_row(a, B, q), (R)0.5 * (Tl + Tr), a.tab.Ti1, a.tab.Ti2, &bad);
   if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
   const R G2 = Tr - Tl;
-  if (!TANG) return kap * (G2 * ldu(a.W[8], q));
+  if (!TANG) return kap * (G2 * wn);
   const R G0 = (c[y][x + 1] - c[y][x - 1] + p[y][x + 1] - p[y][x - 1]) * (R)0.25;
   const R G1 = (c[y + 1][x] - c[y - 1][x] + p[y + 1][x] - p[y - 1][x]) * (R)0.25;
   return kap * ((G0 * ldu(a.W[6], q) + G1 * ldu(a.W[7], q)) + G2 * ldu(a.W[8], q));
@@ -750,24 +751,54 @@ __global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
   R fz_prev = 0;
   if (live && k0 > 1) {
 #pragma clang fp contract(off)
-    fz_prev = flux_z<TANG>(a, B, ring[sm], ring[sc], x, y, p - sz);
+    fz_prev = flux_z<TANG>(a, B, ring[sm], ring[sc], x, y, p - sz, TANG ? (R)0 : ldu(a.W[8], p - sz));
+  }
+  // normal face weights of the thread's cell, fetched one plane ahead
+  // (orthogonal blocks read only these three weights)
+  R w0 = 0, w4 = 0, w8 = 0;
+  if (!TANG && live) {
+    w0 = ldu(a.W[0], p);
+    w4 = ldu(a.W[4], p);
+    w8 = ldu(a.W[8], p);
   }
   double rsq = 0.0;
   for (int k = k0; k <= k1; k++, p += sz, ph += sz) {
     store_plane(ring[sp], pre);
     __syncthreads();
     if (k < k1) fetch_plane(pre, a.T, B, t.i0, t.j0, k + 2);
+    // per-cell streams of this plane, issued before the flux work so that
+    // their latency overlaps it
+    R c_vol = 0, c_qv = 0, c_dt = 0, c_TO = 0;
+    int c_m = B.umat;
+    if (live) {
+      c_vol = ldu(a.vol, p);
+      c_qv = ldu(a.qvol, p);
+      if (!a.use_dtg) c_dt = ldu(a.dt, p);
+      if (!IRSMODE) c_TO = ldu(a.TO, p);
+      if (c_m < 0) c_m = ldu(a.mat, p);
+    }
+    R n0 = 0, n4 = 0, n8 = 0;
+    if (!TANG && live && k < k1) {
+      n0 = ldu(a.W[0], p + sz);
+      n4 = ldu(a.W[4], p + sz);
+      n8 = ldu(a.W[8], p + sz);
+    }
     R fz = 0;
     {
 #pragma clang fp contract(off)
       if (live) {
-        if (i < B.ni) fx[ty][tx + 1] = flux_x<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, y, p);
-        if (j < B.nj) fy[ty + 1][tx] = flux_y<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, y, p);
-        if (k < B.nk) fz = flux_z<TANG>(a, B, ring[sc], ring[sp], x, y, p);
+        if (i < B.ni) fx[ty][tx + 1] = flux_x<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, y, p, w0);
+        if (j < B.nj) fy[ty + 1][tx] = flux_y<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, y, p, w4);
+        if (k < B.nk) fz = flux_z<TANG>(a, B, ring[sc], ring[sp], x, y, p, w8);
       }
-      if (yhalo) fy[0][tx] = flux_y<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, 0, p - sy);
-      if (xhalo) fx[hr][0] = flux_x<TANG>(a, B, ring[sm], ring[sc], ring[sp], 0, hr + 1, ph);
+      if (yhalo)
+        fy[0][tx] = flux_y<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, 0, p - sy, TANG ? (R)0 : ldu(a.W[4], p - sy));
+      if (xhalo)
+        fx[hr][0] = flux_x<TANG>(a, B, ring[sm], ring[sc], ring[sp], 0, hr + 1, ph, TANG ? (R)0 : ldu(a.W[0], ph));
     }
+    w0 = n0;
+    w4 = n4;
+    w8 = n8;
     __syncthreads();
     if (live) {
       R Rv = 0;
@@ -782,14 +813,14 @@ __global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
       }
       Rv = apply_bc_faces(a, B, i, j, k, Rv);
       const R T = ring[sc][y][x];
-      const int m = B.umat >= 0 ? B.umat : ldu(a.mat, p);
-      const R dt = a.use_dtg ? (R)(*a.dtg) : ldu(a.dt, p);
+      const int m = c_m;
+      const R dt = a.use_dtg ? (R)(*a.dtg) : c_dt;
       int bad = 0;
-      const R res = scaled_residual<PRIM>(a.tab, m, T, Rv, ldu(a.vol, p), ldu(a.qvol, p), dt, &bad);
+      const R res = scaled_residual<PRIM>(a.tab, m, T, Rv, c_vol, c_qv, dt, &bad);
       if (IRSMODE) {
         stu(a.Rres, p, res);
       } else {
-        const R TO = ldu(a.TO, p);
+        const R TO = c_TO;
         const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, a.jam, &bad);
         stu(a.Tout, p, Tn);
         if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
@@ -808,6 +839,156 @@ __global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
 }
 
 // ------------------------------------------------------------------------
+// Barrier-free stage kernel for orthogonal 3-D blocks: each wave owns one
+// 64-cell row (i) of the tile and marches its KC planes in k on its own.
+// i-neighbours come from the neighbouring lanes (shuffles; lane 0 and 63
+// read the tile edges), j-neighbours from memory (the rows of the other
+// waves, cache hits), the +z flux is carried to the next plane in a
+// register.  The flux through a j face is evaluated by both rows that share
+// it, from the same operands, so the residual is bit-identical to k_stage3d's.
+// Everything a plane needs is fetched one plane ahead.
+// ------------------------------------------------------------------------
+template <typename R>
+struct RowPlane {
+  R T, Tjm, Tjp, Tih;   // own T, j-1 and j+1 neighbours, tile-edge halo (lanes 0, 63)
+  R w0, w4, w8, w4m;    // normal weights of the cell's +x, +y, +z faces and of the (j-1) cell's +y face
+  R w0h;                // lane 0: +x weight of cell i0-1
+  R vol, qv, dt, TO;
+  int m, mjm, mih;      // material of the cell, of (j-1), of (i0-1)
+};
+
+template <bool PRIM, bool IRSMODE, typename R>
+__device__ __forceinline__ void row_fetch(RowPlane<R>& r, const StageArgs<R>& a, const BlockInfo& B, unsigned p,
+                                          int lane, bool live, bool lo_halo, bool hi_halo, bool has_jm) {
+  const unsigned sy = (unsigned)B.sy;
+  if (live) {
+    r.T = ldu(a.T, p);
+    r.Tjm = ldu(a.T, p - sy);
+    r.Tjp = ldu(a.T, p + sy);
+    r.w0 = ldu(a.W[0], p);
+    r.w4 = ldu(a.W[4], p);
+    r.w8 = ldu(a.W[8], p);
+    r.w4m = has_jm ? ldu(a.W[4], p - sy) : (R)0;
+    r.vol = ldu(a.vol, p);
+    r.qv = ldu(a.qvol, p);
+    if (!a.use_dtg) r.dt = ldu(a.dt, p);
+    if (!IRSMODE) r.TO = ldu(a.TO, p);
+    if (B.umat < 0) {
+      r.m = ldu(a.mat, p);
+      r.mjm = has_jm ? ldu(a.mat, p - sy) : 0;
+    }
+  }
+  if (lo_halo) {
+    r.Tih = ldu(a.T, p - 1);
+    r.w0h = ldu(a.W[0], p - 1);
+    if (B.umat < 0) r.mih = ldu(a.mat, p - 1);
+  } else if (hi_halo) {
+    r.Tih = ldu(a.T, p + 1);
+  }
+  (void)lane;
+}
+
+template <typename R>
+__device__ __forceinline__ R face_kw(const StageArgs<R>& a, int m, R Tl, R Tr, R w) {
+  int bad = 0;
+  const R kap = tab_interp2(a.tab.kap2 + 2 * (long)m * a.tab.nT, (R)0.5 * (Tl + Tr), a.tab.Ti1, a.tab.Ti2, &bad);
+  if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
+  return kap * ((Tr - Tl) * w);
+}
+
+template <bool PRIM, bool IRSMODE, typename R>
+__global__ __launch_bounds__(TILE_T) void k_stage3w(StageArgs<R> a) {
+  if (a.halt && *a.halt) return;
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Tile t = a.ctiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int lane = threadIdx.x % TILE_X, row = threadIdx.x / TILE_X;
+  const int i = t.i0 + lane, j = t.j0 + row;
+  const int k0 = t.k0, k1 = min(B.nk, k0 + KC - 1);
+  const bool rowlive = j <= B.nj;
+  const bool live = rowlive && i <= B.ni;
+  const bool has_jm = j > 1;
+  // lane 0 computes the tile's min-side x face; lane 63 reads the cell past the tile
+  const bool lo_halo = rowlive && lane == 0 && t.i0 > 1;
+  const bool hi_halo = rowlive && lane == TILE_X - 1 && i < B.ni;
+  const unsigned sz = (unsigned)B.sz;
+  unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k0 - 1) * B.sz);
+  RowPlane<R> cur, nxt;
+  cur.m = cur.mjm = cur.mih = B.umat;
+  nxt.m = nxt.mjm = nxt.mih = B.umat;
+  cur.dt = nxt.dt = (R)0;
+  cur.TO = nxt.TO = (R)0;
+  cur.Tih = nxt.Tih = (R)0;
+  cur.w0h = nxt.w0h = (R)0;
+  row_fetch<PRIM, IRSMODE>(cur, a, B, p, lane, live, lo_halo, hi_halo, has_jm);
+  R Tp = live && k0 < B.nk ? ldu(a.T, p + sz) : (R)0;
+  R fz_prev = 0;
+  if (live && k0 > 1) {
+#pragma clang fp contract(off)
+    const R Tm = ldu(a.T, p - sz);
+    const int mm = B.umat >= 0 ? B.umat : ldu(a.mat, p - sz);
+    fz_prev = face_kw(a, mm, Tm, cur.T, ldu(a.W[8], p - sz));
+  }
+  double rsq = 0.0;
+  for (int k = k0; k <= k1; k++, p += sz) {
+    // next plane's data (and T two planes up for the +z face of plane k+1)
+    R Tp2 = 0;
+    if (k < k1) {
+      row_fetch<PRIM, IRSMODE>(nxt, a, B, p + sz, lane, live, lo_halo, hi_halo, has_jm);
+      if (live && k + 1 < B.nk) Tp2 = ldu(a.T, p + 2 * sz);
+    }
+    // i neighbours from the neighbouring lanes
+    R Tl = __shfl_up(cur.T, 1, 64), Tr = __shfl_down(cur.T, 1, 64);
+    if (lane == 0) Tl = cur.Tih;
+    if (lane == TILE_X - 1) Tr = cur.Tih;
+    R fxp = 0, fyp = 0, fym = 0, fz = 0, fxm_own = 0;
+    {
+#pragma clang fp contract(off)
+      if (live) {
+        const int m = cur.m;
+        if (i < B.ni) fxp = face_kw(a, m, cur.T, Tr, cur.w0);
+        if (j < B.nj) fyp = face_kw(a, m, cur.T, cur.Tjp, cur.w4);
+        if (j > 1) fym = face_kw(a, cur.mjm, cur.Tjm, cur.T, cur.w4m);
+        if (k < B.nk) fz = face_kw(a, m, cur.T, Tp, cur.w8);
+      }
+      if (lo_halo) fxm_own = face_kw(a, cur.mih, Tl, cur.T, cur.w0h);
+    }
+    R fxm = __shfl_up(fxp, 1, 64);
+    if (lane == 0) fxm = fxm_own;
+    if (live) {
+      R Rv = 0;
+      {
+#pragma clang fp contract(off)
+        if (i > 1) Rv = Rv + fxm;
+        if (i < B.ni) Rv = Rv - fxp;
+        if (j > 1) Rv = Rv + fym;
+        if (j < B.nj) Rv = Rv - fyp;
+        if (k > 1) Rv = Rv + fz_prev;
+        if (k < B.nk) Rv = Rv - fz;
+      }
+      Rv = apply_bc_faces(a, B, i, j, k, Rv);
+      const R dt = a.use_dtg ? (R)(*a.dtg) : cur.dt;
+      int bad = 0;
+      const R res = scaled_residual<PRIM>(a.tab, cur.m, cur.T, Rv, cur.vol, cur.qv, dt, &bad);
+      if (IRSMODE) {
+        stu(a.Rres, p, res);
+      } else {
+        const R Tn = new_state<PRIM>(a.tab, cur.m, cur.T, cur.TO, res, a.rkc, a.jam, &bad);
+        stu(a.Tout, p, Tn);
+        if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+        const double d = (double)Tn - (double)cur.TO;
+        rsq += d * d;
+      }
+      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+    }
+    fz_prev = fz;
+    cur = nxt;
+    Tp = Tp2;
+  }
+  if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
+}
+
+// ------------------------------------------------------------------------
 // kernels: implicit residual smoothing
 // ------------------------------------------------------------------------
 template <typename R>
@@ -819,6 +1000,7 @@ struct IrsArgs {
   const BCRec* bc;
   const int* fmap;
   R beta, den;         // den = 1 + 2 beta
+  R inv;               // 1/den when den is a power of two (x/den == x*inv exactly), else 0
   // fused update (last direction)
   const R* T;
   const R* TO;
@@ -832,6 +1014,14 @@ struct IrsArgs {
   int* status;
   const int* halt;
 };
+
+// x / (1 + 2 beta): a multiplication by the exact reciprocal when the
+// divisor is a power of two (beta = 0.5: den = 2), which gives the same
+// IEEE result as the division; else the division.
+template <typename R>
+__device__ __forceinline__ R irs_div(R x, R den, R inv) {
+  return inv != (R)0 ? x * inv : x / den;
+}
 
 // Residual_Smoothing_Blk along DIR (Lib_IRS.f90:43-134): two Jacobi sweeps
 // r*_new(s) = (r(s) + beta (r*(s-1) + r*(s+1))) / (1 + 2 beta), r* = r
@@ -871,11 +1061,11 @@ __global__ __launch_bounds__(TILE_T) void k_irs(IrsArgs<R> a) {
       if (q > n) return ghi;
       return Rin[p + d * st];
     };
-    const R beta = a.beta, den = a.den;
+    const R beta = a.beta, den = a.den, inv = a.inv;
     // first sweep at s-1 and s+1 (ghost cells keep r*)
-    const R rm = (s - 1 < 1) ? glo : (r(-1) + beta * (r(-2) + r(0))) / den;
-    const R rp = (s + 1 > n) ? ghi : (r(1) + beta * (r(0) + r(2))) / den;
-    const R rs = (r(0) + beta * (rm + rp)) / den;
+    const R rm = (s - 1 < 1) ? glo : irs_div(r(-1) + beta * (r(-2) + r(0)), den, inv);
+    const R rp = (s + 1 > n) ? ghi : irs_div(r(1) + beta * (r(0) + r(2)), den, inv);
+    const R rs = irs_div(r(0) + beta * (rm + rp), den, inv);
     if (!UPDATE) {
       a.Rout[p] = rs;
     } else {
@@ -896,15 +1086,16 @@ __global__ __launch_bounds__(TILE_T) void k_irs(IrsArgs<R> a) {
 // r(s-2..s+2); positions 0 and n+1 are the ghosts glo/ghi, which the sweeps
 // leave unchanged (Residual_Smoothing_Blk smooths interior cells only).
 template <typename R>
-__device__ __forceinline__ R jacobi2(R rm2, R rm1, R r0, R rp1, R rp2, int s, int n, R glo, R ghi, R beta, R den) {
+__device__ __forceinline__ R jacobi2(R rm2, R rm1, R r0, R rp1, R rp2, int s, int n, R glo, R ghi, R beta, R den,
+                                     R inv) {
 #pragma clang fp contract(off)
   const R am1 = s - 1 >= 1 ? rm1 : glo;
   const R ap1 = s + 1 <= n ? rp1 : ghi;
   const R am2 = s - 2 >= 1 ? rm2 : glo;
   const R ap2 = s + 2 <= n ? rp2 : ghi;
-  const R sm = (s - 1 < 1) ? glo : (am1 + beta * (am2 + r0)) / den;
-  const R sp = (s + 1 > n) ? ghi : (ap1 + beta * (r0 + ap2)) / den;
-  return (r0 + beta * (sm + sp)) / den;
+  const R sm = (s - 1 < 1) ? glo : irs_div(am1 + beta * (am2 + r0), den, inv);
+  const R sp = (s + 1 > n) ? ghi : irs_div(ap1 + beta * (r0 + ap2), den, inv);
+  return irs_div(r0 + beta * (sm + sp), den, inv);
 }
 
 // IRS ghost of the line end on face f (Ghost_Residual_Extrapolation,
@@ -953,7 +1144,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xy(IrsArgs<R> a) {
     if (i <= 2) glo = irs_ghost(a, B, 1, 1, j, k, rin[jj][ii + 2 - (i - 1)]);
     if (i >= B.ni - 1) ghi = irs_ghost(a, B, 2, B.ni, j, k, rin[jj][ii + 2 + (B.ni - i)]);
     xs[jj][ii] = jacobi2(rin[jj][ii], rin[jj][ii + 1], rin[jj][ii + 2], rin[jj][ii + 3], rin[jj][ii + 4], i, B.ni,
-                         glo, ghi, beta, den);
+                         glo, ghi, beta, den, a.inv);
   }
   __syncthreads();
   // y smoothing of the 16 output rows; y ghosts are the x-smoothed rows 1, nj
@@ -966,7 +1157,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xy(IrsArgs<R> a) {
     const R glo = (j <= 2) ? xs[jj - (j - 1)][ii] : (R)0;
     const R ghi = (j >= B.nj - 1) ? xs[jj + (B.nj - j)][ii] : (R)0;
     const R rs = jacobi2(xs[jj - 2][ii], xs[jj - 1][ii], xs[jj][ii], xs[jj + 1][ii], xs[jj + 2][ii], j, B.nj, glo, ghi,
-                         beta, den);
+                         beta, den, a.inv);
     const unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz);
     if (!UPDATE) {
       stu(a.Rout, p, rs);
@@ -1005,7 +1196,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
     R w0 = rd(k0 - 2), w1 = rd(k0 - 1), w2 = rd(k0), w3 = rd(k0 + 1), w4 = rd(k0 + 2);
     const int m = B.umat >= 0 ? B.umat : -1;
     for (int k = k0; k <= k1; k++) {
-      const R rs = jacobi2(w0, w1, w2, w3, w4, k, n, glo, ghi, a.beta, a.den);
+      const R rs = jacobi2(w0, w1, w2, w3, w4, k, n, glo, ghi, a.beta, a.den, a.inv);
       const unsigned p = p1 + (k - 1) * sz;
       const R T = ldu(a.T, p), TO = ldu(a.TO, p);
       int bad = 0;
@@ -1354,6 +1545,7 @@ struct fuss_ctx {
   Adjoint adj;                      // discrete adjoint (fuss_adj_*)
   int adj_store_on = 0;             // enqueue_step copies each stage input to adj.stage[]
   int graphs_off = 0;               // fuss_run without graph capture (fuss_graphs)
+  int stage_lds = 1;                // orthogonal 3-D stages: LDS-ring kernel (1) or row march (0)
   double kscale = 1.0;              // conductivity table scale (fuss_set_kscale)
   DBuf<int> stage_i;                // staging for fuss_set_bc_values
   DBuf<double> stage_d;
@@ -1770,6 +1962,10 @@ int fuss_create(int nlevels, int nblocks, int precision, fuss_ctx** out) {
     fuss_destroy(c);
     return FUSS_EHIP;
   }
+  // orthogonal 3-D stages: the LDS-ring kernel (default) or, with
+  // FUSS_STAGE_ROW=1, the barrier-free row march
+  const char* ev = getenv("FUSS_STAGE_ROW");
+  c->stage_lds = !(ev && ev[0] == '1');
   *out = c;
   return 0;
 }
@@ -2071,9 +2267,12 @@ void launch_stage3d(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang
   if (tang) {
     if (prim) { irs ? k_stage3d<true, true, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, true, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
     else { irs ? k_stage3d<true, false, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, false, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
-  } else {
+  } else if (c->stage_lds) {
     if (prim) { irs ? k_stage3d<false, true, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, true, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
     else { irs ? k_stage3d<false, false, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, false, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
+  } else {
+    if (prim) { irs ? k_stage3w<true, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<true, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
+    else { irs ? k_stage3w<false, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<false, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
   }
 }
 
@@ -2093,6 +2292,14 @@ void launch_irs(fuss_ctx* c, Level& L, int dir, const IrsArgs<double>& a, bool u
 }
 
 const double RKC[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.50, 0.0}, {1.0, 0.25, 2.0 / 3.0}};
+
+// 1/d when d is a (normal) power of two, so that x*(1/d) == x/d for every
+// finite x away from the subnormal range; else 0 (divide).
+double exact_inverse(double d) {
+  int e = 0;
+  const double m = frexp(d, &e);
+  return (m == 0.5 && d > 1e-300 && d < 1e300) ? 1.0 / d : 0.0;
+}
 const double RK4A[4] = {0.25, 1.0 / 3.0, 0.5, 1.0};
 
 // Explicit_Step on one level, enqueued on the stream (no host sync except
@@ -2173,6 +2380,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
       {
 #pragma clang fp contract(off)
         ia.den = 1.0 + 2.0 * p->irs_beta;
+        ia.inv = exact_inverse(ia.den);
       }
       ia.T = in; ia.TO = L.Tb[o].p; ia.Tout = out; ia.mat = L.mat.p; ia.tab = dtab<double>(c);
       ia.rkc = sa.rkc; ia.jam = sa.jam; ia.last = sa.last; ia.partials = L.partials.p; ia.status = c->status.p;
@@ -2373,6 +2581,7 @@ int cmg_smooth(fuss_ctx* c, CLevel& l, int nu, const std::vector<double>& al, do
 #pragma clang fp contract(off)
     den = (S)(1.0 + 2.0 * beta);
   }
+  const S inv = (S)exact_inverse((double)den);
   const int m = (int)al.size();
   for (int it = 0; it < nu; it++) {
     const int base = l.cur, x = (base + 1) % 3, y = (base + 2) % 3;
@@ -2388,12 +2597,12 @@ int cmg_smooth(fuss_ctx* c, CLevel& l, int nu, const std::vector<double>& al, do
         S* s2 = (S*)l.s2;
         k_cmg_stage<S, true><<<gs, 256, 0, c->stream>>>(o, din, nullptr, s1, (S)0);
         if (c->ndir == 3) {
-          k_cmg_irs<0, false, S><<<gs, 256, 0, c->stream>>>(o, s1, s2, bt, den, nullptr, (S)0);
-          k_cmg_irs<1, false, S><<<gs, 256, 0, c->stream>>>(o, s2, s1, bt, den, nullptr, (S)0);
-          k_cmg_irs<2, true, S><<<gs, 256, 0, c->stream>>>(o, s1, out, bt, den, d0, a);
+          k_cmg_irs<0, false, S><<<gs, 256, 0, c->stream>>>(o, s1, s2, bt, den, inv, nullptr, (S)0);
+          k_cmg_irs<1, false, S><<<gs, 256, 0, c->stream>>>(o, s2, s1, bt, den, inv, nullptr, (S)0);
+          k_cmg_irs<2, true, S><<<gs, 256, 0, c->stream>>>(o, s1, out, bt, den, inv, d0, a);
         } else {
-          k_cmg_irs<0, false, S><<<gs, 256, 0, c->stream>>>(o, s1, s2, bt, den, nullptr, (S)0);
-          k_cmg_irs<1, true, S><<<gs, 256, 0, c->stream>>>(o, s2, out, bt, den, d0, a);
+          k_cmg_irs<0, false, S><<<gs, 256, 0, c->stream>>>(o, s1, s2, bt, den, inv, nullptr, (S)0);
+          k_cmg_irs<1, true, S><<<gs, 256, 0, c->stream>>>(o, s2, out, bt, den, inv, d0, a);
         }
       }
       HIPCHK(hipGetLastError());

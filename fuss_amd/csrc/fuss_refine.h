@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void k_cmg_stage(COp<S> o, const S* __restrict
 // zero-gradient line ends; UPDATE applies the stage update.
 template <int DIR, bool UPDATE, typename S>
 __global__ __launch_bounds__(256) void k_cmg_irs(COp<S> o, const S* __restrict__ in, S* __restrict__ out, S beta,
-                                                 S den, const S* __restrict__ d0, S alpha) {
+                                                 S den, S inv, const S* __restrict__ d0, S alpha) {
   const int g = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   if (g >= o.ncell) return;
   const int bi = cmg_find_block(o.blk, o.nb, g);
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void k_cmg_irs(COp<S> o, const S* __restrict__
     return in[g + (t - s) * st];
   };
   const S glo = in[g - s * st], ghi = in[g + (n - 1 - s) * st];
-  const S v = jacobi2(r(-2), r(-1), r(0), r(1), r(2), s + 1, n, glo, ghi, beta, den);
+  const S v = jacobi2(r(-2), r(-1), r(0), r(1), r(2), s + 1, n, glo, ghi, beta, den, inv);
   if (UPDATE) out[g] = d0[g] + alpha * v;
   else out[g] = v;
 }
