@@ -233,6 +233,7 @@ def sweep_check(args):
     s = gpu.Solver(case)
     setup_s = time.perf_counter() - t0
     ncell = case.ncell
+    tuned = s.autotune(1, steps=3)
     for _ in range(2):
         s.explicit_step(1)
     device_sync()
@@ -245,6 +246,7 @@ def sweep_check(args):
     out = {"workload": f"{args.sweep_cases} independent steady cubes, dims 32..96 (multiples of 4), "
                        f"k 5-200, h 10-500, qvol 0-1e6, RK{args.rk} + IRS 0.5, one context",
            "cells": ncell, "setup_s": setup_s, "plain_fp64_ms_per_step": step_s * 1e3,
+           "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring"},
            "plain_fp64_cell_updates_per_s": ncell / step_s}
     finals = {}
     for prec, name in ((gpu.FUSS_FP32, "fp32_refined"), (gpu.FUSS_FP64, "fp64_refined")):
@@ -282,6 +284,7 @@ def lattice_check(args):
     case = cases.lattice(nl=4, lo=16, hi=128, n_rk=4, time_accurate=True)
     s = gpu.Solver(case)
     setup_s = time.perf_counter() - t0
+    tuned = s.autotune(1, steps=3)
     s.run_level(1, 30, 30, -1.0, 1e300, hist=False)   # warm-up (graph capture, module load)
     device_sync()
     t0 = time.perf_counter()
@@ -291,7 +294,8 @@ def lattice_check(args):
     out = {"workload": "transient 4x4x4 lattice of 64 connected curvilinear blocks (slab dims 16-128), "
                        "3 materials, 302/301/304 walls, RK4 global dt, FP64",
            "cells": case.ncell, "blocks": case.nb, "steps": k, "seconds": dt, "setup_s": setup_s,
-           "cell_updates_per_s": case.ncell * k / dt, "simulated_time_s": r.time}
+           "cell_updates_per_s": case.ncell * k / dt, "simulated_time_s": r.time,
+           "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring"}}
     print(f"[bench] lattice: {case.ncell} cells x {k} steps in {dt:.2f} s", file=sys.stderr, flush=True)
     s.close()
     return out
@@ -331,6 +335,9 @@ def main():
     t0 = time.perf_counter()
     s = gpu.Solver(case)
     print(f"[bench] setup {time.perf_counter()-t0:.1f} s", file=sys.stderr, flush=True)
+    tuned = s.autotune(1, steps=4)
+    print(f"[bench] autotune: column height {tuned[0]}, {'row march' if tuned[1] else 'LDS ring'}, "
+          f"{tuned[2]:.3f} ms/step", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         s.explicit_step(1)
     device_sync()
@@ -372,7 +379,9 @@ def main():
         "config": {"workload": f"steady cube {args.n}^3, plain FP64, RK{args.rk} + LTS + IRS(jacobi2, 0.5), "
                                "qvol 1e6, 303 h=50, k(T) 32-knot table, 3 MG levels; step = one "
                                "finest-level Explicit_Step",
-                   "cells": ncell, "parallelism": "single-gpu"},
+                   "cells": ncell, "parallelism": "single-gpu",
+                   "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring",
+                             "chosen_by": "fuss_autotune"}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": STAGE_TRAFFIC_MEASURED["k_stage3d<false, true, true, double>"] if args.n == 256 else None,

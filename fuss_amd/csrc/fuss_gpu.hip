@@ -620,7 +620,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage(StageArgs<R> a) {
 // The residual accumulation keeps the Fluxes_blk order (i, then j, then k
 // faces; Mod_Fluxes.f90:54-192).
 // ------------------------------------------------------------------------
-constexpr int KC = 8;
+constexpr int KC_DEFAULT = 8;   // column height; fuss_autotune may choose 4 or 16
 constexpr int PX = TILE_X + 2, PY = TILE_Y + 2;
 
 template <typename R>
@@ -717,7 +717,7 @@ __device__ __forceinline__ R flux_z(const StageArgs<R>& a, const BlockInfo& B, R
   return kap * ((G0 * ldu(a.W[6], q) + G1 * ldu(a.W[7], q)) + G2 * ldu(a.W[8], q));
 }
 
-template <bool TANG, bool PRIM, bool IRSMODE, typename R>
+template <bool TANG, bool PRIM, bool IRSMODE, typename R, int KCT>
 __global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
   if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
   const BlockInfo B = a.blk[t.b];
   const int tx = threadIdx.x % TILE_X, ty = threadIdx.x / TILE_X;
   const int i = t.i0 + tx, j = t.j0 + ty;
-  const int k0 = t.k0, k1 = min(B.nk, k0 + KC - 1);
+  const int k0 = t.k0, k1 = min(B.nk, k0 + KCT - 1);
   const bool live = i <= B.ni && j <= B.nj;
   const int x = tx + 1, y = ty + 1;  // LDS coordinates of the thread's cell
   // min-side halo faces of the tile: wave 0 computes the y-halo (one face per
@@ -896,7 +896,7 @@ __device__ __forceinline__ R face_kw(const StageArgs<R>& a, int m, R Tl, R Tr, R
   return kap * ((Tr - Tl) * w);
 }
 
-template <bool PRIM, bool IRSMODE, typename R>
+template <bool PRIM, bool IRSMODE, typename R, int KCT>
 __global__ __launch_bounds__(TILE_T) void k_stage3w(StageArgs<R> a) {
   if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -904,7 +904,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3w(StageArgs<R> a) {
   const BlockInfo B = a.blk[t.b];
   const int lane = threadIdx.x % TILE_X, row = threadIdx.x / TILE_X;
   const int i = t.i0 + lane, j = t.j0 + row;
-  const int k0 = t.k0, k1 = min(B.nk, k0 + KC - 1);
+  const int k0 = t.k0, k1 = min(B.nk, k0 + KCT - 1);
   const bool rowlive = j <= B.nj;
   const bool live = rowlive && i <= B.ni;
   const bool has_jm = j > 1;
@@ -1178,14 +1178,14 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xy(IrsArgs<R> a) {
 
 // z smoothing (direction 3) fused with the RK update: a 64x4 column marches
 // through KC planes with r(k-2..k+2) in registers, so R is read once.
-template <bool PRIM, typename R>
+template <bool PRIM, typename R, int KCT>
 __global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
   if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const Tile t = a.tiles[wg];
   const BlockInfo B = a.blk[t.b];
   const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X);
-  const int k0 = t.k0, k1 = min(B.nk, k0 + KC - 1), n = B.nk;
+  const int k0 = t.k0, k1 = min(B.nk, k0 + KCT - 1), n = B.nk;
   double rsq = 0.0;
   if (i <= B.ni && j <= B.nj) {
     const unsigned sz = (unsigned)B.sz;
@@ -1453,7 +1453,9 @@ struct Level {
   std::vector<BlockInfo> hblk;
   std::vector<HostBlock> hb;        // host nodes (every level), fields (level 1)
   std::vector<Tile> htiles;         // one 64x4 cell tile per plane
-  std::vector<Tile> hctiles;        // 64x4 columns of KC planes (3-D levels)
+  std::vector<Tile> hctiles;        // 64x4 columns of kc planes (3-D levels)
+  int kc = KC_DEFAULT;              // planes per column tile (fuss_autotune)
+  int row_march = -1;               // orthogonal 3-D stage kernel: -1 context default, 0 LDS ring, 1 row march
   std::vector<Tile> hxytiles;       // 64x16 plane tiles of the x-y IRS pass
   std::vector<int> hrng[3];         // per tile list (tiles, ctiles, xytiles): first tile of each block
   DBuf<int> rng[3];
@@ -1695,7 +1697,7 @@ int build_level_layout(fuss_ctx* c, Level& L) {
       for (int j = 1; j <= B.nj; j += XY_TJ)
         for (int i = 1; i <= B.ni; i += TILE_X) L.hxytiles.push_back(Tile{b, i, j, k});
     if (L.kp == GC)
-      for (int k = 1; k <= B.nk; k += KC)
+      for (int k = 1; k <= B.nk; k += L.kc)
         for (int j = 1; j <= B.nj; j += TILE_Y)
           for (int i = 1; i <= B.ni; i += TILE_X) L.hctiles.push_back(Tile{b, i, j, k});
     L.ncell += (long)B.ni * B.nj * B.nk;
@@ -2262,18 +2264,33 @@ void launch_stage_n(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang
   }
 }
 
-void launch_stage3d(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang, bool prim, bool irs) {
+template <int KCT>
+void launch_stage3d_k(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang, bool prim, bool irs) {
   const unsigned g = (unsigned)L.hctiles.size();
+  const bool row = L.row_march >= 0 ? L.row_march == 1 : !c->stage_lds;
   if (tang) {
-    if (prim) { irs ? k_stage3d<true, true, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, true, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
-    else { irs ? k_stage3d<true, false, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, false, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
-  } else if (c->stage_lds) {
-    if (prim) { irs ? k_stage3d<false, true, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, true, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
-    else { irs ? k_stage3d<false, false, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, false, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
+    if (prim) { irs ? k_stage3d<true, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+    else { irs ? k_stage3d<true, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+  } else if (!row) {
+    if (prim) { irs ? k_stage3d<false, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+    else { irs ? k_stage3d<false, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
   } else {
-    if (prim) { irs ? k_stage3w<true, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<true, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
-    else { irs ? k_stage3w<false, true, double><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<false, false, double><<<g, TILE_T, 0, c->stream>>>(a); }
+    if (prim) { irs ? k_stage3w<true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+    else { irs ? k_stage3w<false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
   }
+}
+
+void launch_stage3d(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang, bool prim, bool irs) {
+  if (L.kc == 4) launch_stage3d_k<4>(c, L, a, tang, prim, irs);
+  else if (L.kc == 16) launch_stage3d_k<16>(c, L, a, tang, prim, irs);
+  else launch_stage3d_k<8>(c, L, a, tang, prim, irs);
+}
+
+void launch_irs_z(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
+  const unsigned g = (unsigned)L.hctiles.size();
+  if (L.kc == 4) prim ? k_irs_z<true, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
+  else if (L.kc == 16) prim ? k_irs_z<true, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
+  else prim ? k_irs_z<true, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
 }
 
 template <int DIR, int NDIR>
@@ -2407,8 +2424,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
           ia.Rin = L.R2.p;
           ia.tiles = L.ctiles.p;
           pid = prof_begin(c, FUSS_K_IRS_UPDATE);
-          prim ? k_irs_z<true, double><<<(unsigned)L.hctiles.size(), TILE_T, 0, c->stream>>>(ia)
-               : k_irs_z<false, double><<<(unsigned)L.hctiles.size(), TILE_T, 0, c->stream>>>(ia);
+          launch_irs_z(c, L, ia, prim);
           prof_end(c, pid);
           HIPCHK(hipGetLastError());
           if (sa.last) { npart = (int)L.hctiles.size(); L.plist = 1; }
@@ -2431,8 +2447,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
           ia.Rin = rin;
           ia.tiles = L.ctiles.p;
           pid = prof_begin(c, FUSS_K_IRS_UPDATE);
-          prim ? k_irs_z<true, double><<<(unsigned)L.hctiles.size(), TILE_T, 0, c->stream>>>(ia)
-               : k_irs_z<false, double><<<(unsigned)L.hctiles.size(), TILE_T, 0, c->stream>>>(ia);
+          launch_irs_z(c, L, ia, prim);
           prof_end(c, pid);
           HIPCHK(hipGetLastError());
           if (sa.last) { npart = (int)L.hctiles.size(); L.plist = 1; }
@@ -3024,7 +3039,98 @@ int fuss_refine(fuss_ctx* c, const fuss_step_params* monitor, fuss_refine_result
 
 }  // extern "C"
 
+namespace {
+
+// ------------------------------------------------------------------------
+// autotuning of the 3-D column tiles (fuss_autotune)
+// ------------------------------------------------------------------------
+int set_kc(fuss_ctx* c, Level& L, int kc) {
+  if (L.kp != GC) return 0;
+  L.kc = kc;
+  L.hctiles.clear();
+  for (int b = 0; b < L.nb; b++) {
+    const BlockInfo& B = L.hblk[b];
+    for (int k = 1; k <= B.nk; k += kc)
+      for (int j = 1; j <= B.nj; j += TILE_Y)
+        for (int i = 1; i <= B.ni; i += TILE_X) L.hctiles.push_back(Tile{b, i, j, k});
+  }
+  L.hrng[1].assign(L.nb + 1, (int)L.hctiles.size());
+  for (int t = (int)L.hctiles.size() - 1; t >= 0; t--) L.hrng[1][L.hctiles[t].b] = t;
+  for (int b = L.nb - 1; b >= 0; b--) L.hrng[1][b] = std::min(L.hrng[1][b], L.hrng[1][b + 1]);
+  if (L.ctiles.alloc(std::max<size_t>(L.hctiles.size(), 1))) return FUSS_EHIP;
+  HIPCHK(hipMemcpy(L.ctiles.p, L.hctiles.data(), L.hctiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L.rng[1].p, L.hrng[1].data(), (L.nb + 1) * sizeof(int), hipMemcpyHostToDevice));
+  const size_t np = std::max<size_t>(std::max(L.htiles.size(), L.hctiles.size()), 1);
+  if (L.partials.n < np && (L.partials.alloc(np) || L.pmin.alloc(np) || L.red2.alloc(np / RED_CHUNK + 2)))
+    return FUSS_EHIP;
+  return 0;
+}
+
+// Times `steps` Explicit_Steps of each (column height, kernel) candidate on
+// the level's current state, restores the state and clock after each, and
+// keeps the fastest.
+int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* kc_out, int* row_out,
+             double* ms_out) {
+  Level& L = c->lev[level - 1];
+  if (L.kp != GC) {
+    if (kc_out) *kc_out = 0;
+    if (row_out) *row_out = 0;
+    if (ms_out) *ms_out = 0.0;
+    return 0;
+  }
+  if (read_clock(c, L)) return FUSS_EHIP;
+  const double t0 = L.time;
+  const int it0 = L.iter;
+  DBuf<double> save;
+  if (save.alloc(L.npad)) return FUSS_EHIP;
+  HIPCHK(hipMemcpy(save.p, L.Tb[L.cur].p, L.npad * sizeof(double), hipMemcpyDeviceToDevice));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  fuss_step_params q = *p;
+  q.compute_residual = 1;
+  const int kcs[3] = {4, 8, 16};
+  double best = 1e300;
+  int bkc = L.kc, brow = 0, err = 0;
+  for (int row = 0; row <= (L.tang ? 0 : 1) && !err; row++)
+    for (int ki = 0; ki < 3 && !err; ki++) {
+      if ((err = set_kc(c, L, kcs[ki]))) break;
+      L.row_march = row;
+      if ((err = enqueue_step(c, level, &q))) break;  // warm-up
+      HIPCHK(hipEventRecord(e0, c->stream));
+      for (int s2 = 0; s2 < steps && !err; s2++) err = enqueue_step(c, level, &q);
+      HIPCHK(hipEventRecord(e1, c->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      const double per = ms / std::max(steps, 1);
+      if (per < best) { best = per; bkc = kcs[ki]; brow = row; }
+      HIPCHK(hipMemcpy(L.Tb[L.cur].p, save.p, L.npad * sizeof(double), hipMemcpyDeviceToDevice));
+      set_clock(c, L, t0, it0);
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  save.free();
+  if (err) return err;
+  const int st = sync_status(c);
+  if (st) return st;
+  if ((err = set_kc(c, L, bkc))) return err;
+  L.row_march = brow;
+  if (kc_out) *kc_out = bkc;
+  if (row_out) *row_out = brow;
+  if (ms_out) *ms_out = best;
+  return 0;
+}
+
+}  // namespace
+
 extern "C" {
+
+int fuss_autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* kc, int* row_march,
+                  double* ms_per_step) {
+  if (!c || !c->setup_done || !p || level < 1 || level > c->nlev || steps < 1) return FUSS_EARG;
+  return autotune(c, level, p, steps, kc, row_march, ms_per_step);
+}
 
 int fuss_step(fuss_ctx* c, int level, const fuss_step_params* p, fuss_step_result* r) {
   if (!c || !c->setup_done || !p || level < 1 || level > c->nlev) return FUSS_EARG;

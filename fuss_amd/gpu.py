@@ -97,6 +97,8 @@ def load():
     L.fuss_set_stream.argtypes = [P, P]
     L.fuss_memory.argtypes = [P, C.POINTER(C.c_double)]
     L.fuss_graphs.argtypes = [P, C.c_int]
+    L.fuss_autotune.argtypes = [P, C.c_int, C.POINTER(StepParams), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_double)]
     L.fuss_refine_setup.argtypes = [P, C.POINTER(RefineParams)]
     L.fuss_refine.argtypes = [P, C.POINTER(StepParams), C.POINTER(RefineResult), C.c_void_p]
     L.fuss_refine_blocks.argtypes = [P, C.c_void_p, C.c_void_p]
@@ -159,6 +161,15 @@ class Solver:
                  f"fuss_set_bc level {m+1}")
         _chk(self.L.fuss_setup(self.h, 0.0 if self.p.time_accurate else -1.0), "fuss_setup")
         self.level = self.p.mgl
+
+    def autotune(self, level=1, steps=5):
+        """Choose the column height and stage kernel of a 3-D level by timing
+        (fuss_autotune); returns (kc, row_march, ms per step)."""
+        sp = step_params(self.p)
+        kc, row, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        _chk(self.L.fuss_autotune(self.h, level, C.byref(sp), int(steps), C.byref(kc), C.byref(row), C.byref(ms)),
+             "fuss_autotune")
+        return kc.value, row.value, ms.value
 
     def graphs(self, enable):
         """fuss_run on captured HIP graphs (default) or step by step."""

@@ -125,6 +125,15 @@ int fuss_run(fuss_ctx *ctx, int level, const fuss_step_params *p, int max_iters,
  * graphs are switched off here (enable = 0).  Default: on. */
 int fuss_graphs(fuss_ctx *ctx, int enable);
 
+/* Tile autotuner: times `steps` Explicit_Steps with each candidate of the
+ * 3-D stage pass -- column height 4, 8 or 16 planes x LDS-ring or row-march
+ * kernel (row march: orthogonal levels only) -- on the level's current state,
+ * restores the state and clock, and keeps the fastest for this level.  The
+ * per-cell arithmetic is the same for every candidate; only the grouping of
+ * the residual's partial sums changes.  2-D levels: nothing to tune (kc = 0). */
+int fuss_autotune(fuss_ctx *ctx, int level, const fuss_step_params *p, int steps, int *kc,
+                  int *row_march, double *ms_per_step);
+
 /* Restriction (Mod_Multigrid.f90:59-77): coarse T, qvol, matID from level
  * fine_level to fine_level+1. */
 int fuss_restrict(fuss_ctx *ctx, int fine_level);

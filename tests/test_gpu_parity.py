@@ -190,3 +190,20 @@ def test_graph_run_equals_step_loop(gpu, name):
     assert k1 == k2 and c1 == c2
     assert np.array_equal(hr1, hr2) and np.array_equal(ht1, ht2)
     assert np.array_equal(T1, T2)
+
+
+@pytest.mark.parametrize("name", ["cube3d_rk4_irs", "lattice_transient"])
+def test_autotune_keeps_results(gpu, name):
+    """fuss_autotune changes tile shapes and kernels only: the state after
+    the same steps is bit-identical, the residual equal to rounding."""
+    out = []
+    for tune in (False, True):
+        s = gpu.Solver(SYNTH[name]())
+        if tune:
+            kc, row, ms = s.autotune(1, steps=2)
+            assert kc in (0, 4, 8, 16) and ms >= 0.0
+        r = [s.explicit_step(1).res_sq for _ in range(6)]
+        out.append((np.concatenate([T.ravel() for T in s.T()]), np.array(r)))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    np.testing.assert_allclose(out[0][1], out[1][1], rtol=1e-13)
