@@ -175,12 +175,13 @@ __global__ void k_bc_geometry(BCRec* bc, int n, const BlockInfo* blk, const doub
 // ------------------------------------------------------------------------
 // Ghost_Connection / Ghost_Symmetry (Lib_Ghost.f90:132-189), first layer:
 // the explicit step's stencils read no cell beyond it.
+// g[e] = (ghost, source) arena indices of entry e, ghost < 0: none.
 template <typename R>
-__global__ void k_ghost(R* __restrict__ T, const BCRec* __restrict__ bc, int n, const int* __restrict__ halt) {
+__global__ void k_ghost(R* __restrict__ T, const int2* __restrict__ g, int n, const int* __restrict__ halt) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n || (halt && *halt)) return;
-  const long d = bc[e].gdst;
-  if (d >= 0) T[d] = T[bc[e].gsrc];
+  const int2 q = g[e];
+  if (q.x >= 0) stu(T, (unsigned)q.x, ldu((const R*)T, (unsigned)q.y));
 }
 
 // ------------------------------------------------------------------------
@@ -317,6 +318,45 @@ __device__ __forceinline__ R dotrow(const double* M, int r, const double* N) {
   return (R)(M[r - 1] * N[0] + M[r - 1 + 3] * N[1] + M[r - 1 + 6] * N[2]);
 }
 
+// Wall fluxes that need only scalars of their entry; bc_value (full record)
+// and the compact-record pass of k_bcflux both evaluate them here.
+// BC_Wall_HeatTransfer, Lib_BC_Fluxes_Wall_HeatTransfer.f90:12-52 (the
+// reference passes bc%qw, never set for 303: 0 here).
+template <typename R>
+__device__ __forceinline__ R bcf_htc(R kappa, R mn, R A, R hconv, R Tref, R Tc) {
+#pragma clang fp contract(off)
+  const R H = (R)1.0 / ((R)2.0 * kappa * mn) + (R)1.0 / hconv;
+  const R Hrad = hconv / ((R)2.0 * kappa * mn) + (R)1.0;
+  return A * ((R)1.0 / H * (Tref - Tc) + (R)1.0 / Hrad * (R)0.0);
+}
+
+// BC_Wall_Radiation(+Convection), Lib_BC_Fluxes_Wall_Radiation.f90:12-65 and
+// Lib_BC_Fluxes_Wall_RadiationConvection.f90:12-65.  The fixed point stays in
+// FP64: its tolerance (1e-3 W/m^2) is below the FP32 resolution of q.
+// Returns the flux; *fail is set when the iteration does not settle.
+__device__ __forceinline__ double bcf_rad(int type, double kappa, double mn, double A, double Tref, double eps,
+                                          double hconv, double Tcd, int* fail) {
+#pragma clang fp contract(off)
+  const double Tr2 = Tref * Tref, Tr4 = Tr2 * Tr2;
+  double Tw = Tcd, qw = 0.0, err = 1.0e6, Flux = 0.0;
+  int it = 0;
+  while (err > 1.0e-3) {
+    const double Tw2 = Tw * Tw;
+    const double K = SIGMA_SB * eps * (Tw2 * Tw2 - Tr4) / (Tw - Tref);
+    const double H = type == 304 ? 1.0 / (2.0 * kappa * mn) + 1.0 / K : 1.0 / (2.0 * kappa * mn) + 1.0 / (K + hconv);
+    Flux = A * (1.0 / H * (Tref - Tcd));
+    const double q_wall = K * (Tref - Tw);
+    err = fabs(qw - q_wall);
+    qw = q_wall;
+    Tw = Tref - 1.0 / (H * K) * (Tref - Tcd);
+    if (++it > 1000 || err != err) {
+      *fail = 1;
+      break;
+    }
+  }
+  return Flux;
+}
+
 // BC_Fluxes for one entry (Mod_BC_Fluxes.f90:43-79 and the per-type files
 // src/lib/numerics/fluxes/bc/Lib_BC_Fluxes_*.f90), in the reference's
 // arithmetic order.
@@ -421,42 +461,16 @@ __device__ R bc_value(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e,
       Rv = (R)modfm2 * Flux;
     } break;
     case 303: {
-      // BC_Wall_HeatTransfer, Lib_BC_Fluxes_Wall_HeatTransfer.f90:12-52
-      // (the reference passes bc%qw, never set for 303: 0 here)
       const R kappa = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
-      const R mn = dotrow<R>(M, e.dir, Nr);
-      const R H = (R)1.0 / ((R)2.0 * kappa * mn) + (R)1.0 / (R)e.hconv;
-      const R Hrad = (R)e.hconv / ((R)2.0 * kappa * mn) + (R)1.0;
-      const R Flux = (R)e.A * ((R)1.0 / H * ((R)e.Tref - Tc) + (R)1.0 / Hrad * (R)0.0);
-      Rv = Flux;
+      Rv = bcf_htc<R>(kappa, dotrow<R>(M, e.dir, Nr), (R)e.A, (R)e.hconv, (R)e.Tref, Tc);
     } break;
     case 304:
     case 305: {
-      // BC_Wall_Radiation(+Convection), Lib_BC_Fluxes_Wall_Radiation.f90:12-65 and
-      // Lib_BC_Fluxes_Wall_RadiationConvection.f90:12-65.  The fixed point stays in
-      // FP64: its tolerance (1e-3 W/m^2) is below the FP32 resolution of q.
       const double kappa = (double)tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
       const double mn = M[e.dir - 1] * Nr[0] + M[e.dir - 1 + 3] * Nr[1] + M[e.dir - 1 + 6] * Nr[2];
-      const double Tref = e.Tref, Tcd = (double)Tc;
-      const double Tr2 = Tref * Tref, Tr4 = Tr2 * Tr2;
-      double Tw = Tcd, qw = 0.0, err = 1.0e6, Flux = 0.0;
-      int it = 0;
-      while (err > 1.0e-3) {
-        const double Tw2 = Tw * Tw;
-        const double K = SIGMA_SB * e.eps * (Tw2 * Tw2 - Tr4) / (Tw - Tref);
-        const double H = e.type == 304 ? 1.0 / (2.0 * kappa * mn) + 1.0 / K
-                                       : 1.0 / (2.0 * kappa * mn) + 1.0 / (K + e.hconv);
-        Flux = e.A * (1.0 / H * (Tref - Tcd));
-        const double q_wall = K * (Tref - Tw);
-        err = fabs(qw - q_wall);
-        qw = q_wall;
-        Tw = Tref - 1.0 / (H * K) * (Tref - Tcd);
-        if (++it > 1000 || err != err) {
-          flag_error(a.status, FUSS_ESTATE, e.b, e.i, e.j, e.k);
-          break;
-        }
-      }
-      Rv = (R)Flux;
+      int fail = 0;
+      Rv = (R)bcf_rad(e.type, kappa, mn, e.A, e.Tref, e.eps, e.hconv, (double)Tc, &fail);
+      if (fail) flag_error(a.status, FUSS_ESTATE, e.b, e.i, e.j, e.k);
     } break;
     default:
       break;
@@ -490,12 +504,42 @@ __device__ __forceinline__ R apply_bc_faces(const StageArgs<R>& a, const BlockIn
 
 // BC_Fluxes (Mod_BC_Fluxes.f90:10-86) as a surface pass: one thread per
 // entry stores the value its face subtracts from the cell residual.
+// Entries of type 301/303/304/305 are evaluated from their compact record;
+// 101, 300 and 302 (tangential gradients, donor data) from the full record.
 template <int NDIR, typename R>
-__global__ void k_bcflux(StageArgs<R> a, int n) {
+__global__ void k_bcflux(StageArgs<R> a, const BCHot* __restrict__ hot, int n) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n || (a.halt && *a.halt)) return;
+  const int type = hot[e].type;
+  if (!(type == 101 || (type >= 300 && type <= 305))) return;
+  if (type == 301 || type >= 303) {
+#pragma clang fp contract(off)
+    const BCHot h = hot[e];
+    const R Tc = ldu(a.T, h.cell);
+    int bad = 0, fail = 0;
+    R v;
+    if (type == 301) {
+      // BC_Wall_Heat, Lib_BC_Fluxes_Wall_Heat.f90:12-54 (signed along +xi)
+      R qw = (R)h.qw;
+      if (h.series >= 0) {
+        const int q = h.series;
+        qw = (R)series_value(a.ser.per[q], a.ser.n[q], a.ser.t + a.ser.off[q], a.ser.v + a.ser.off[q], *a.clk);
+      }
+      const R Flux = (R)h.A * qw;
+      v = (R)(1 - 2 * (h.f % 2)) * Flux;
+    } else {
+      const R kappa = tab_interp(a.tab.kap + (long)h.m * a.tab.nT, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
+      if (type == 303) v = bcf_htc<R>(kappa, (R)h.mn, (R)h.A, (R)h.hconv, (R)h.Tref, Tc);
+      else v = (R)bcf_rad(type, (double)kappa, h.mn, h.A, h.Tref, h.eps, h.hconv, (double)Tc, &fail);
+    }
+    a.bcv[e] = v;
+    if (bad || fail) {
+      const BCRec& r = a.bc[e];
+      flag_error(a.status, fail ? FUSS_ESTATE : FUSS_ETABLE, r.b, r.i, r.j, r.k);
+    }
+    return;
+  }
   const BCRec& r = a.bc[e];
-  if (!(r.type == 101 || (r.type >= 300 && r.type <= 305))) return;
   const BlockInfo B = a.blk[r.b];
   double sv = 0.0;
   if (r.series >= 0) {
@@ -503,6 +547,25 @@ __global__ void k_bcflux(StageArgs<R> a, int n) {
     sv = series_value(a.ser.per[q], a.ser.n[q], a.ser.t + a.ser.off[q], a.ser.v + a.ser.off[q], *a.clk);
   }
   a.bcv[e] = bc_value<NDIR>(a, B, r, r.cell, sv);
+}
+
+// Compact records of the wall entries (after the geometry of the full
+// records and the cell materials are on the device).
+__global__ void k_bc_hot(const BCRec* __restrict__ bc, int n, const int* __restrict__ mat, BCHot* __restrict__ hot) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const BCRec& r = bc[e];
+  BCHot h;
+  h.type = r.type; h.f = r.f; h.m = mat[r.cell]; h.series = r.series;
+  h.cell = (unsigned)r.cell; h.b = r.b;
+  h.A = r.A;
+  {
+#pragma clang fp contract(off)
+    // the dot product bc_value forms for 303/304/305
+    h.mn = r.M[r.dir - 1] * r.N[0] + r.M[r.dir - 1 + 3] * r.N[1] + r.M[r.dir - 1 + 6] * r.N[2];
+  }
+  h.hconv = r.hconv; h.Tref = r.Tref; h.qw = r.qw; h.eps = r.eps;
+  hot[e] = h;
 }
 
 // RK_stage, Lib_RK.f90:15-26.  jam: the four-stage scheme of Jameson,
@@ -1104,6 +1167,7 @@ __device__ __forceinline__ R jacobi2(R rm2, R rm1, R r0, R rp1, R rp2, int s, in
 template <typename R>
 __device__ __forceinline__ R irs_ghost(const IrsArgs<R>& a, const BlockInfo& B, int f, int i, int j, int k,
                                        R own) {
+  if (!(B.conn & (1 << (f - 1)))) return own;  // no connection on this face
   const int e = a.fmap[face_map_index<R>(B, f, i, j, k)];
   const BCRec& r = a.bc[e];
   return r.type == 101 ? ldu(a.Rin, (unsigned)r.rdonor) : own;
@@ -1209,6 +1273,135 @@ __global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
       w0 = w1; w1 = w2; w2 = w3; w3 = w4;
       w4 = rd(k + 3);
     }
+  }
+  if (a.last) block_sum(rsq, a.partials, wg);
+}
+
+// All three directions of Residual_Smoothing (Lib_IRS.f90:14-40) and the RK
+// update in one pass, for 3-D levels without connections on j or k faces: a
+// 64x4 column marches through its KC planes.  Each R plane enters LDS once
+// with a 2-cell i/j halo, is smoothed along x then y there (the y ghosts are
+// the x-smoothed boundary rows, as in k_irs_xy), and its 64x4 result joins a
+// register window s(k-2..k+2) that the z sweeps read (z ghosts: the
+// xy-smoothed boundary planes).  R is read from HBM once per stage instead of
+// three times (x-y pass in, z pass in) plus once written.
+constexpr int XYZ_RJ = TILE_Y + 4, XYZ_RI = TILE_X + 4;
+constexpr int XYZ_NLD = (XYZ_RJ * XYZ_RI + TILE_T - 1) / TILE_T;  // plane loads per thread
+
+template <typename R>
+struct XyzPlane {
+  R v[XYZ_NLD];
+};
+
+template <typename R>
+__device__ __forceinline__ void xyz_fetch(XyzPlane<R>& r, const R* __restrict__ Rin, const BlockInfo& B, int i0,
+                                          int j0, int q) {
+  const unsigned corner = (unsigned)(B.base + (i0 - 3) + (long)(j0 - 3) * B.sy + (long)(q - 1) * B.sz);
+#pragma unroll
+  for (int u = 0; u < XYZ_NLD; u++) {
+    const int e = threadIdx.x + u * TILE_T;
+    const int ii = e % XYZ_RI, jj = e / XYZ_RI;
+    const int gi = i0 - 2 + ii, gj = j0 - 2 + jj;
+    r.v[u] = (e < XYZ_RJ * XYZ_RI && gi >= 1 && gi <= B.ni && gj >= 1 && gj <= B.nj)
+                 ? ldu(Rin, corner + ii + jj * (unsigned)B.sy)
+                 : (R)0;
+  }
+}
+
+template <bool PRIM, typename R, int KCT>
+__global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
+  if (a.halt && *a.halt) return;
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Tile t = a.tiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int tx = threadIdx.x % TILE_X, ty = threadIdx.x / TILE_X;
+  const int i = t.i0 + tx, j = t.j0 + ty;
+  const int n = B.nk, k0 = t.k0, k1 = min(n, k0 + KCT - 1);
+  const bool live = i <= B.ni && j <= B.nj;
+  __shared__ R rin[XYZ_RJ][XYZ_RI];
+  __shared__ R xs[XYZ_RJ][TILE_X];
+  const R beta = a.beta, den = a.den, inv = a.inv;
+  // xy-smoothed value of this thread's cell on plane q (the staged plane in
+  // `pre`); every thread takes part in the two barriers.  The next plane's
+  // LDS store follows the second barrier (rin is no longer read), and its
+  // x sweeps follow the next first barrier (xs is no longer read).
+  auto smooth_plane = [&](XyzPlane<R>& pre, int q, int qb) -> R {
+#pragma unroll
+    for (int u = 0; u < XYZ_NLD; u++) {
+      const int e = threadIdx.x + u * TILE_T;
+      if (e < XYZ_RJ * XYZ_RI) (&rin[0][0])[e] = pre.v[u];
+    }
+    // the next plane's loads fly while this one is smoothed
+    if (q < qb) xyz_fetch(pre, a.Rin, B, t.i0, t.j0, q + 1);
+    __syncthreads();
+    // x sweeps of rows j0-2 .. j0+5 (two per thread)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int jj = ty + h * TILE_Y, jr = t.j0 - 2 + jj;
+      if (i <= B.ni && jr >= 1 && jr <= B.nj) {
+        R glo = 0, ghi = 0;
+        if (i <= 2) glo = irs_ghost(a, B, 1, 1, jr, q, rin[jj][tx + 2 - (i - 1)]);
+        if (i >= B.ni - 1) ghi = irs_ghost(a, B, 2, B.ni, jr, q, rin[jj][tx + 2 + (B.ni - i)]);
+        xs[jj][tx] = jacobi2(rin[jj][tx], rin[jj][tx + 1], rin[jj][tx + 2], rin[jj][tx + 3], rin[jj][tx + 4], i,
+                             B.ni, glo, ghi, beta, den, inv);
+      }
+    }
+    __syncthreads();
+    R s = 0;
+    if (live) {
+      const int jj = ty + 2;
+      const R glo = (j <= 2) ? xs[jj - (j - 1)][tx] : (R)0;
+      const R ghi = (j >= B.nj - 1) ? xs[jj + (B.nj - j)][tx] : (R)0;
+      s = jacobi2(xs[jj - 2][tx], xs[jj - 1][tx], xs[jj][tx], xs[jj + 1][tx], xs[jj + 2][tx], j, B.nj, glo, ghi, beta,
+                  den, inv);
+    }
+    return s;
+  };
+  // window w[0..4] = s(k-2 .. k+2); planes outside 1..n stay 0 (jacobi2
+  // selects the ghosts there)
+  R w[5] = {0, 0, 0, 0, 0};
+  R zlo = 0, zhi = 0;  // xy-smoothed planes 1 and n (z ghosts of walls)
+  XyzPlane<R> pre;
+  const int qa = max(1, k0 - 2), qb = min(n, k1 + 2);
+  xyz_fetch(pre, a.Rin, B, t.i0, t.j0, qa);
+  for (int q = qa; q <= min(qb, k0 + 1); q++) {
+    const R s = smooth_plane(pre, q, qb);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (q - k0 + 2 == u) w[u] = s;  // static register indices
+    if (q == 1) zlo = s;
+    if (q == n) zhi = s;
+  }
+  const unsigned sz = (unsigned)B.sz;
+  unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k0 - 1) * B.sz);
+  double rsq = 0.0;
+  for (int k = k0; k <= k1; k++, p += sz) {
+    R T = 0, TO = 0;
+    int m = B.umat;
+    if (live) {
+      T = ldu(a.T, p);
+      TO = ldu(a.TO, p);
+      if (m < 0) m = ldu(a.mat, p);
+    }
+    const int q = k + 2;
+    if (q <= qb) {
+      const R s = smooth_plane(pre, q, qb);
+      w[4] = s;
+      if (q == n) zhi = s;
+    } else {
+      w[4] = 0;
+    }
+    if (live) {
+      const R rs = jacobi2(w[0], w[1], w[2], w[3], w[4], k, n, zlo, zhi, beta, den, inv);
+      int bad = 0;
+      const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
+      stu(a.Tout, p, Tn);
+      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+      const double d = (double)Tn - (double)TO;
+      rsq += d * d;
+    }
+    w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
   }
   if (a.last) block_sum(rsq, a.partials, wg);
 }
@@ -1449,7 +1642,7 @@ struct HostBlock {
 
 struct Level {
   int nb = 0;
-  int kp = 2;                       // k ghost layers (0 in 2-D)
+  int kp = 2;                       // k ghost layers (1 in 2-D: k-face data only)
   std::vector<BlockInfo> hblk;
   std::vector<HostBlock> hb;        // host nodes (every level), fields (level 1)
   std::vector<Tile> htiles;         // one 64x4 cell tile per plane
@@ -1461,6 +1654,7 @@ struct Level {
   DBuf<int> rng[3];
   int plist = 0;                    // tile list of the last step's residual partials
   int no_ycon = 1;                  // no connection on a j face (fused x-y IRS allowed)
+  int no_zcon = 1;                  // no connection on a k face (with no_ycon: fused x-y-z IRS)
   long npad = 0, ncell = 0, nnodes = 0;
   DBuf<BlockInfo> blk;
   DBuf<Tile> tiles, ctiles, xytiles;
@@ -1471,6 +1665,8 @@ struct Level {
   DBuf<int> mat;
   DBuf<double> M, F[3], W[9];
   DBuf<BCRec> bc;
+  DBuf<BCHot> bch;                  // compact wall records (k_bcflux)
+  DBuf<int2> gmap;                  // (ghost, source) per entry (k_ghost)
   DBuf<int> fmap;
   DBuf<double> bcv;
   int nbound = 0;
@@ -1678,14 +1874,19 @@ int build_level_layout(fuss_ctx* c, Level& L) {
     BlockInfo& B = L.hblk[b];
     const HostBlock& h = L.hb[b];
     B.ni = h.n[0]; B.nj = h.n[1]; B.nk = h.n[2];
-    B.sy = B.ni + 2 * GC;
+    // rows of whole 128-byte lines, and cell (1,1,1) on a line boundary: every
+    // interior row then starts a line, so a 64-cell tile row is 4 full lines
+    // (unaligned rows straddle 5, the extra line fetched by two work-groups)
+    B.sy = (B.ni + 2 * GC + ROW_ALIGN - 1) / ROW_ALIGN * ROW_ALIGN;
     B.sz = (long)B.sy * (B.nj + 2 * GC);
+    off = (off + GC + ROW_ALIGN - 1) / ROW_ALIGN * ROW_ALIGN - GC;
     B.off = off;
     B.npad = B.sz * (B.nk + 2 * L.kp);
     B.base = off + GC + (long)GC * B.sy + (long)L.kp * B.sz;
     B.node_off = noff;
     for (int f = 0; f < 6; f++) B.fmap_off[f] = -1;
     B.face_active = 0;
+    B.conn = 0;
     B.tang = 0;
     B.umat = -1;
     off += B.npad;
@@ -1750,6 +1951,7 @@ int alloc_level(fuss_ctx* c, Level& L) {
 int build_bc(fuss_ctx* c, Level& L) {
   const int n = L.nbound;
   L.no_ycon = 1;
+  L.no_zcon = 1;
   // grouping check: (block, face) non-decreasing, as BC_Fluxes assumes
   // (Mod_BC_Fluxes.f90:26-33)
   for (int e = 1; e < n; e++) {
@@ -1821,18 +2023,26 @@ int build_bc(fuss_ctx* c, Level& L) {
       // matIDg: the donor cell's material (Ghost_Connection copies it)
     }
     if (type == 101 && (f == 3 || f == 4)) L.no_ycon = 0;
+    if (type == 101 && (f == 5 || f == 6)) L.no_zcon = 0;
     const bool active = type == 101 || (type >= 300 && type <= 305);
     if (active) B.face_active |= 1 << (f - 1);
+    if (type == 101) B.conn |= 1 << (f - 1);
   }
   for (size_t q = 0; q < fmap.size(); q++)
     if (fmap[q] < 0) return FUSS_EBC;
   L.any_active = 0;
   for (int b = 0; b < L.nb; b++) L.any_active |= L.hblk[b].face_active;
+  // (ghost, source) pairs of the ghost fill, 32-bit arena indices
+  std::vector<int2> gm(std::max(n, 1));
+  for (int e = 0; e < n; e++) gm[e] = make_int2(recs[e].gdst >= 0 ? (int)recs[e].gdst : -1, (int)recs[e].gsrc);
   int e = L.bc.alloc(std::max(n, 1));
+  e |= L.bch.alloc(std::max(n, 1));
+  e |= L.gmap.alloc(gm.size());
   e |= L.bcv.alloc(std::max(n, 1));
   e |= L.fmap.alloc(std::max<size_t>(fmap.size(), 1));
   if (e) return FUSS_EHIP;
   if (n) e |= upload(L.bc.p, recs.data(), sizeof(BCRec) * n);
+  if (n) e |= upload(L.gmap.p, gm.data(), sizeof(int2) * n);
   if (!fmap.empty()) e |= upload(L.fmap.p, fmap.data(), sizeof(int) * fmap.size());
   (void)c;
   return e;
@@ -1896,7 +2106,7 @@ Tab<R> dtab(fuss_ctx* c) {
 int launch_ghost(fuss_ctx* c, Level& L, double* T) {
   if (!L.nbound) return 0;
   const int pid = prof_begin(c, FUSS_K_GHOST);
-  k_ghost<double><<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(T, L.bc.p, L.nbound, L.cnt.p + 1);
+  k_ghost<double><<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(T, L.gmap.p, L.nbound, L.cnt.p + 1);
   prof_end(c, pid);
   return hipGetLastError() == hipSuccess ? 0 : FUSS_EHIP;
 }
@@ -2237,6 +2447,13 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     e = launch_ghost(c, L, L.Tb[L.cur].p);
     if (e) return e;
   }
+  // compact wall records, now that every level's materials are set
+  for (int m = 0; m < c->nlev; m++) {
+    Level& L = c->lev[m];
+    if (!L.nbound) continue;
+    k_bc_hot<<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(L.bc.p, L.nbound, L.mat.p, L.bch.p);
+    HIPCHK(hipGetLastError());
+  }
   int bijk[4];
   const int st = sync_status(c, bijk);
   if (st) return st;
@@ -2291,6 +2508,13 @@ void launch_irs_z(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
   if (L.kc == 4) prim ? k_irs_z<true, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
   else if (L.kc == 16) prim ? k_irs_z<true, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
   else prim ? k_irs_z<true, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
+}
+
+void launch_irs_xyz(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
+  const unsigned g = (unsigned)L.hctiles.size();
+  if (L.kc == 4) prim ? k_irs_xyz<true, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
+  else if (L.kc == 16) prim ? k_irs_xyz<true, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
+  else prim ? k_irs_xyz<true, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
 }
 
 template <int DIR, int NDIR>
@@ -2370,8 +2594,8 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = halt;
     if (L.any_active && L.nbound) {
       pid = prof_begin(c, FUSS_K_BC);
-      if (L.kp == GC) k_bcflux<3, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound);
-      else k_bcflux<2, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound);
+      if (L.kp == GC) k_bcflux<3, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
+      else k_bcflux<2, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
       prof_end(c, pid);
       HIPCHK(hipGetLastError());
     }
@@ -2402,7 +2626,16 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
       ia.T = in; ia.TO = L.Tb[o].p; ia.Tout = out; ia.mat = L.mat.p; ia.tab = dtab<double>(c);
       ia.rkc = sa.rkc; ia.jam = sa.jam; ia.last = sa.last; ia.partials = L.partials.p; ia.status = c->status.p;
       ia.halt = halt;
-      if (L.no_ycon && c->ndir >= 2) {
+      if (L.no_ycon && L.no_zcon && c->ndir == 3 && L.kp == GC) {
+        // x, y and z in one k-marching pass with the update
+        ia.Rin = L.R.p;
+        ia.tiles = L.ctiles.p;
+        pid = prof_begin(c, FUSS_K_IRS_UPDATE);
+        launch_irs_xyz(c, L, ia, prim);
+        prof_end(c, pid);
+        HIPCHK(hipGetLastError());
+        if (sa.last) { npart = (int)L.hctiles.size(); L.plist = 1; }
+      } else if (L.no_ycon && c->ndir >= 2) {
         // x and y in one LDS pass; 2-D levels update there, 3-D levels then
         // march the z pass with the update
         ia.Rin = L.R.p;
@@ -2727,8 +2960,8 @@ int refine_run(fuss_ctx* c, const fuss_step_params* mon, fuss_refine_result* out
     const bool d3 = L.kp == GC;
     if (L.nbound) {
       if (L.any_active) {
-        d3 ? k_bcflux<3, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound)
-           : k_bcflux<2, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound);
+        d3 ? k_bcflux<3, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound)
+           : k_bcflux<2, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
       }
       d3 ? k_bcderiv<3><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, R.bcd.p, L.nbound)
          : k_bcderiv<2><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, R.bcd.p, L.nbound);
@@ -2910,7 +3143,7 @@ int adj_backward_t(fuss_ctx* c, const double* g, double* gqw, double* gk, double
         sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.tab = dtab<double>(c); sa.status = c->status.p;
         sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = L.cnt.p + 1;
         if (L.nbound && L.any_active)
-          k_bcflux<NDIR, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound);
+          k_bcflux<NDIR, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
         AdjArgs<S> aa;
         aa.blk = L.blk.p; aa.tiles = L.tiles.p; aa.T = A.stage[sidx - 1].p; aa.qvol = L.qvol.p; aa.vol = L.vol.p;
         aa.mat = L.mat.p;
@@ -2989,6 +3222,8 @@ int fuss_set_bc_values(fuss_ctx* c, int level, int n, const int* entries, int co
   HIPCHK(hipMemcpyAsync(c->stage_i.p, entries, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->stage_d.p, values, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   k_set_bcval<<<(n + 255) / 256, 256, 0, c->stream>>>(L.bc.p, c->stage_i.p, c->stage_d.p, n, column);
+  HIPCHK(hipGetLastError());
+  k_bc_hot<<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(L.bc.p, L.nbound, L.mat.p, L.bch.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;

@@ -6,7 +6,9 @@
 //   (2-D levels keep a single k plane on each side, which only stores the
 //   k-face geometry: the reference zeroes every k term when ndir == 2,
 //   Lib_Diffusive.f90:32).  Cell (i,j,k) of block b lives at
-//   blk[b].base + (i-1) + (j-1)*sy + (k-1)*sz (doubles), i fastest.
+//   blk[b].base + (i-1) + (j-1)*sy + (k-1)*sz (doubles), i fastest; sy is
+//   ni+4 rounded up to ROW_ALIGN and base is a multiple of ROW_ALIGN, so
+//   every interior row starts on a 128-byte line.
 //   Face weights: W[3*D+c][cell] holds, for the face between cell and its
 //   +1 neighbour in direction D, A * sum_x Mbar(c,x) n_x with
 //   Mbar = (M_cell + M_nb)/2 -- the flux is then kappa * sum_c G_c W[3D+c].
@@ -19,6 +21,7 @@ namespace fuss {
 constexpr int TILE_X = 64;
 constexpr int TILE_Y = 4;
 constexpr int TILE_T = TILE_X * TILE_Y;
+constexpr int ROW_ALIGN = 16;        // doubles per 128-byte line
 constexpr int MAX_SERIES = 16;
 constexpr int RUN_G = 30;            // steps per captured graph of fuss_run (multiple of 3)
 
@@ -34,7 +37,7 @@ struct BlockInfo {
   int face_active;   // bit f-1 set when face f has flux-producing entries
   int tang;          // nonzero when any tangential weight is nonzero
   int umat;          // material index when the whole block has one material, else -1
-  int pad_;
+  int conn;          // bit f-1 set when face f has a connection (101) entry
 };
 
 struct Tile {
@@ -65,6 +68,19 @@ struct BCRec {
   double M[9];         // metric tensor of the boundary cell, c(3,3) column-major
   double Mg[9];        // ghost-cell metric Mg(1) (connections)
   double mc[3];        // dot(Mg(r,:), N), r = 1..3
+};
+
+// The per-stage fields of a wall entry of type 301, 303, 304 or 305 in one
+// 72-byte record (the full BCRec is ~380 bytes): the surface flux pass reads
+// these instead of whole records.  Filled on the device from the BCRec
+// array after its geometry (k_bc_hot) and again after fuss_set_bc_values.
+struct BCHot {
+  int type, f, m, series;  // BC type, face, material of the cell, time series (-1: none)
+  unsigned cell;           // arena index of the boundary cell
+  int b;
+  double A;                // face area
+  double mn;               // dot(M(dir,:), N) of the boundary cell
+  double hconv, Tref, qw, eps;
 };
 
 // Property tables, one row per kelvin from Ti1 (Load_Table.f90:61-71).
