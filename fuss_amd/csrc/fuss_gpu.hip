@@ -267,6 +267,7 @@ struct StageArgs {
   const BCRec* bc;
   const int* fmap;
   R* bcv;              // per BC entry: value subtracted from the cell residual
+  R* bcf;              // the same values at the entries' face-map positions
   Tab<R> tab;
   SerDev ser;          // 301/302 time series, evaluated at *clk
   const double* clk;   // level time (device)
@@ -493,12 +494,12 @@ __device__ __forceinline__ R apply_bc_faces(const StageArgs<R>& a, const BlockIn
 #pragma clang fp contract(off)
   const int fa = B.face_active;
   if (!fa) return Rv;
-  if ((fa & 1) && i == 1) Rv = Rv - a.bcv[a.fmap[B.fmap_off[0] + (j - 1) + B.nj * (k - 1)]];
-  if ((fa & 2) && i == B.ni) Rv = Rv - a.bcv[a.fmap[B.fmap_off[1] + (j - 1) + B.nj * (k - 1)]];
-  if ((fa & 4) && j == 1) Rv = Rv - a.bcv[a.fmap[B.fmap_off[2] + (i - 1) + B.ni * (k - 1)]];
-  if ((fa & 8) && j == B.nj) Rv = Rv - a.bcv[a.fmap[B.fmap_off[3] + (i - 1) + B.ni * (k - 1)]];
-  if ((fa & 16) && k == 1) Rv = Rv - a.bcv[a.fmap[B.fmap_off[4] + (i - 1) + B.ni * (j - 1)]];
-  if ((fa & 32) && k == B.nk) Rv = Rv - a.bcv[a.fmap[B.fmap_off[5] + (i - 1) + B.ni * (j - 1)]];
+  if ((fa & 1) && i == 1) Rv = Rv - a.bcf[B.fmap_off[0] + (j - 1) + B.nj * (k - 1)];
+  if ((fa & 2) && i == B.ni) Rv = Rv - a.bcf[B.fmap_off[1] + (j - 1) + B.nj * (k - 1)];
+  if ((fa & 4) && j == 1) Rv = Rv - a.bcf[B.fmap_off[2] + (i - 1) + B.ni * (k - 1)];
+  if ((fa & 8) && j == B.nj) Rv = Rv - a.bcf[B.fmap_off[3] + (i - 1) + B.ni * (k - 1)];
+  if ((fa & 16) && k == 1) Rv = Rv - a.bcf[B.fmap_off[4] + (i - 1) + B.ni * (j - 1)];
+  if ((fa & 32) && k == B.nk) Rv = Rv - a.bcf[B.fmap_off[5] + (i - 1) + B.ni * (j - 1)];
   return Rv;
 }
 
@@ -533,6 +534,7 @@ __global__ void k_bcflux(StageArgs<R> a, const BCHot* __restrict__ hot, int n) {
       else v = (R)bcf_rad(type, (double)kappa, h.mn, h.A, h.Tref, h.eps, h.hconv, (double)Tc, &fail);
     }
     a.bcv[e] = v;
+    a.bcf[h.fpos] = v;
     if (bad || fail) {
       const BCRec& r = a.bc[e];
       flag_error(a.status, fail ? FUSS_ESTATE : FUSS_ETABLE, r.b, r.i, r.j, r.k);
@@ -546,7 +548,9 @@ __global__ void k_bcflux(StageArgs<R> a, const BCHot* __restrict__ hot, int n) {
     const int q = r.series;
     sv = series_value(a.ser.per[q], a.ser.n[q], a.ser.t + a.ser.off[q], a.ser.v + a.ser.off[q], *a.clk);
   }
-  a.bcv[e] = bc_value<NDIR>(a, B, r, r.cell, sv);
+  const R v = bc_value<NDIR>(a, B, r, r.cell, sv);
+  a.bcv[e] = v;
+  a.bcf[r.fpos] = v;
 }
 
 // Compact records of the wall entries (after the geometry of the full
@@ -558,6 +562,7 @@ __global__ void k_bc_hot(const BCRec* __restrict__ bc, int n, const int* __restr
   BCHot h;
   h.type = r.type; h.f = r.f; h.m = mat[r.cell]; h.series = r.series;
   h.cell = (unsigned)r.cell; h.b = r.b;
+  h.fpos = r.fpos; h.pad_ = 0;
   h.A = r.A;
   {
 #pragma clang fp contract(off)
@@ -897,6 +902,260 @@ __global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
     sm = sc;
     sc = sp;
     sp = s0;
+  }
+  if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
+}
+
+// ------------------------------------------------------------------------
+// Software-pipelined 2.5-D stage kernel for orthogonal 3-D blocks (the
+// normal face weights only).  Per plane the table rows of the plane's faces
+// and cell are requested first (their addresses come from T in LDS), then
+// the loads of the next plane (T of plane k+2 and the per-cell streams of
+// k+1), then the flux arithmetic waits for the table rows only: loads retire
+// in issue order, so the next plane's streams stay in flight across the
+// whole plane.  Every load in the loop is unconditional (out-of-block lanes
+// and the last plane read valid cells they then ignore), so the compiler's
+// wait counts can leave the later loads outstanding.  UM: every block of the
+// level has one material (uniform row base, no material stream).
+// Same operations and order as k_stage3d<false, ...>.
+// ------------------------------------------------------------------------
+// Row of T in a 1 K table and the clamped T (co_Kappa etc., Lib_Solid.f90:
+// 19-23: Tm = int(T)); outside [Ti1, Ti2) *bad is set and T clamped as in
+// tab_interp.  With Ti1 >= 0, int(T) >= Ti2 exactly when T >= Ti2.
+template <typename R>
+__device__ __forceinline__ int tab_row(R& T, int Ti1, int Ti2, int* bad) {
+  int Tm = (int)T;
+  if (!(T >= (R)Ti1) || Tm >= Ti2) {
+    *bad = 1;
+    T = T < (R)Ti1 || T != T ? (R)Ti1 : (R)(Ti2 - 1);
+    Tm = (int)T;
+  }
+  return Tm;
+}
+
+template <typename R>
+struct StageStream {
+  R vol, qv, dt, TO;   // per-cell streams
+  R w0, w4, w8;        // normal weights of the cell's +x, +y, +z faces
+  R wh;                // halo lanes: weight of the tile's min-side face
+  R bx, by, bz;        // BC values of the cell's boundary face in x, y, z (if any)
+  int m, mh;           // material of the cell / of the halo face's left cell
+};
+
+// fetch_plane with every load issued (out-of-block halo cells read the
+// tile's corner cell, a ghost or interior cell of the block, and become 0)
+template <typename R>
+__device__ __forceinline__ void fetch_plane_u(PlaneRegs<R>& r, const R* __restrict__ T, const BlockInfo& B, int i0,
+                                              int j0, int kk) {
+  const unsigned corner = (unsigned)(B.base + (i0 - 2) + (long)(j0 - 2) * B.sy + (long)(kk - 1) * B.sz);
+  {
+    const int q = threadIdx.x, ii = q % PX, jj = q / PX;
+    const bool in = i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1;
+    const R v = ldu(T, in ? corner + ii + jj * (unsigned)B.sy : corner);
+    r.v0 = in ? v : (R)0;
+  }
+  {
+    const int q = threadIdx.x + TILE_T, ii = q % PX, jj = q / PX;
+    const bool in = q < PX * PY && i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1;
+    const R v = ldu(T, in ? corner + ii + jj * (unsigned)B.sy : corner);
+    r.v1 = in ? v : (R)0;
+  }
+}
+
+template <bool UM, bool PRIM, bool IRSMODE, typename R, int KCT>
+__global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
+  if (a.halt && *a.halt) return;
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Tile t = a.ctiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int tx = threadIdx.x % TILE_X, ty = threadIdx.x / TILE_X;
+  const int i = t.i0 + tx, j = t.j0 + ty;
+  const int k0 = t.k0, k1 = min(B.nk, k0 + KCT - 1);
+  const bool live = i <= B.ni && j <= B.nj;
+  const int x = tx + 1, y = ty + 1;
+  const bool yhalo = ty == 0 && t.j0 > 1 && i <= B.ni;
+  const int hr = tx;
+  const bool xhalo = ty == 1 && tx < TILE_Y && t.i0 > 1 && t.j0 + hr <= B.nj;
+  const bool hx = i < B.ni, hy = j < B.nj;  // the cell's +x / +y face is interior
+  const int hq = xhalo ? hr + 1 : 1;        // LDS row of the x-halo face (in range for every lane)
+  __shared__ R ring[3][PY][PX];
+  __shared__ R fx[TILE_Y][TILE_X + 1];
+  __shared__ R fy[TILE_Y + 1][TILE_X];
+  using V2 = typename Vec2<R>::T;
+  using V4 = typename Vec4<R>::T;
+  const int Ti1 = a.tab.Ti1, Ti2 = a.tab.Ti2, nT = a.tab.nT;
+  const int mu = B.umat >= 0 ? B.umat : 0;
+  const V2* kap_u = reinterpret_cast<const V2*>(a.tab.kap2) + (long)mu * nT - Ti1;
+  const V4* pr_u = reinterpret_cast<const V4*>(a.tab.pr4) + (long)mu * nT - Ti1;
+  const V2* kap0 = reinterpret_cast<const V2*>(a.tab.kap2) - Ti1;
+  const V4* pr0 = reinterpret_cast<const V4*>(a.tab.pr4) - Ti1;
+  // the halo face's weight array: +y weights for the y-halo row, +x for the x-halo
+  const R* __restrict__ Wh = yhalo ? a.W[4] : a.W[0];
+  int sm = 0, sc = 1, sp = 2;
+  PlaneRegs<R> pre;
+  fetch_plane_u(pre, a.T, B, t.i0, t.j0, k0 - 1);
+  store_plane(ring[sm], pre);
+  fetch_plane_u(pre, a.T, B, t.i0, t.j0, k0);
+  store_plane(ring[sc], pre);
+  fetch_plane_u(pre, a.T, B, t.i0, t.j0, k0 + 1);
+  const unsigned sz = (unsigned)B.sz;
+  // addresses of plane k0: the cell (cell (1,1) for out-of-block lanes) and the
+  // halo face's left cell (the cell itself for other lanes)
+  const int ci = live ? i : 1, cj = live ? j : 1;
+  unsigned p = (unsigned)(B.base + (ci - 1) + (long)(cj - 1) * B.sy + (long)(k0 - 1) * B.sz);
+  unsigned ph = yhalo ? p - (unsigned)B.sy
+                      : (xhalo ? (unsigned)(B.base + (t.i0 - 2) + (long)(t.j0 + hr - 1) * B.sy + (long)(k0 - 1) * B.sz)
+                               : p);
+  // face-map positions of the lane's boundary faces (BC values in face
+  // order; at most one face per axis since every block dimension is >= 2)
+  const int fa = live ? B.face_active : 0;
+  const int xo = (fa & 1) && i == 1 ? B.fmap_off[0] + (j - 1) : ((fa & 2) && i == B.ni ? B.fmap_off[1] + (j - 1) : -1);
+  const int yo = (fa & 4) && j == 1 ? B.fmap_off[2] + (i - 1) : ((fa & 8) && j == B.nj ? B.fmap_off[3] + (i - 1) : -1);
+  const int zlo = (fa & 16) ? B.fmap_off[4] + (i - 1) + B.ni * (j - 1) : -1;
+  const int zhi = (fa & 32) ? B.fmap_off[5] + (i - 1) + B.ni * (j - 1) : -1;
+  auto zo_of = [&](int kk) { return kk == 1 ? zlo : (kk == B.nk ? zhi : -1); };
+  const R dtg = a.use_dtg ? (R)(*a.dtg) : (R)0;
+  auto stream = [&](StageStream<R>& s, unsigned pp, unsigned hh, int kk) {
+    s.bx = ldu(a.bcf, (unsigned)max(xo >= 0 ? xo + B.nj * (kk - 1) : 0, 0));
+    s.by = ldu(a.bcf, (unsigned)max(yo >= 0 ? yo + B.ni * (kk - 1) : 0, 0));
+    s.bz = ldu(a.bcf, (unsigned)max(zo_of(kk), 0));
+    s.vol = ldu(a.vol, pp);
+    s.qv = ldu(a.qvol, pp);
+    s.dt = ldu(a.dt, pp);
+    if (!IRSMODE) s.TO = ldu(a.TO, pp);
+    s.w0 = ldu(a.W[0], pp);
+    s.w4 = ldu(a.W[4], pp);
+    s.w8 = ldu(a.W[8], pp);
+    s.wh = ldu(Wh, hh);
+    if (!UM) {
+      s.m = ldu(a.mat, pp);
+      s.mh = ldu(a.mat, hh);
+    }
+  };
+  StageStream<R> sa_, sb_;
+  sa_.m = sa_.mh = sb_.m = sb_.mh = mu;
+  sa_.TO = sb_.TO = (R)0;
+  stream(sa_, p, ph, k0);
+  __syncthreads();
+  R fz_prev = 0;
+  if (live && k0 > 1) {
+#pragma clang fp contract(off)
+    fz_prev = flux_z<false>(a, B, ring[sm], ring[sc], x, y, p - sz, ldu(a.W[8], p - sz));
+  }
+  double rsq = 0.0;
+  // one plane: cur holds its streams, nxt receives those of the next plane
+  auto plane = [&](StageStream<R>& cur, StageStream<R>& nxt, int k) {
+    store_plane(ring[sp], pre);
+    __syncthreads();
+    // phase A: table rows of the plane's faces and of the cell (loads always
+    // issued, at a clamped row; errors counted for existing faces only)
+    int bad = 0, badc = 0;
+    const R Tc = ring[sc][y][x];
+    const R Txr = ring[sc][y][x + 1], Tyr = ring[sc][y + 1][x], Tzr = ring[sp][y][x];
+    R Tx, Ty, Tz, Th, Tcl = Tc;
+    int rx, ry, rz, rh, rc = Ti1;
+    V2 kx, ky, kz, kh;
+    V4 pc{};
+    {
+#pragma clang fp contract(off)
+      const V2* kr = UM ? kap_u : kap0 + (long)cur.m * nT;
+      int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+      Tx = (R)0.5 * (Tc + Txr);
+      rx = tab_row(Tx, Ti1, Ti2, &b0);
+      kx = kr[rx];
+      Ty = (R)0.5 * (Tc + Tyr);
+      ry = tab_row(Ty, Ti1, Ti2, &b1);
+      ky = kr[ry];
+      Tz = (R)0.5 * (Tc + Tzr);
+      rz = tab_row(Tz, Ti1, Ti2, &b2);
+      kz = kr[rz];
+      if (PRIM) {
+        rc = tab_row(Tcl, Ti1, Ti2, &badc);
+        pc = (UM ? pr_u : pr0 + (long)cur.m * nT)[rc];
+      }
+      const V2* krh = UM ? kap_u : kap0 + (long)cur.mh * nT;
+      Th = yhalo ? (R)0.5 * (ring[sc][0][x] + ring[sc][1][x]) : (R)0.5 * (ring[sc][hq][0] + ring[sc][hq][1]);
+      rh = tab_row(Th, Ti1, Ti2, &b3);
+      kh = krh[rh];
+      bad = (live && ((hx && b0) || (hy && b1) || (k < B.nk && b2))) || ((yhalo || xhalo) && b3);
+      badc = live && badc;
+    }
+    // phase B: the next plane's loads (the last plane re-reads its own cells)
+    {
+      const unsigned step = k < k1 ? sz : 0u;
+      fetch_plane_u(pre, a.T, B, t.i0, t.j0, k < k1 ? k + 2 : k + 1);
+      stream(nxt, p + step, ph + step, k < k1 ? k + 1 : k);
+    }
+    // phase C: fluxes
+    R fz = 0;
+    {
+#pragma clang fp contract(off)
+      if (live) {
+        if (hx) fx[ty][tx + 1] = (kx.x + (Tx - (R)rx) * kx.y) * ((Txr - Tc) * cur.w0);
+        if (hy) fy[ty + 1][tx] = (ky.x + (Ty - (R)ry) * ky.y) * ((Tyr - Tc) * cur.w4);
+        if (k < B.nk) fz = (kz.x + (Tz - (R)rz) * kz.y) * ((Tzr - Tc) * cur.w8);
+      }
+      if (yhalo) fy[0][tx] = (kh.x + (Th - (R)rh) * kh.y) * ((ring[sc][1][x] - ring[sc][0][x]) * cur.wh);
+      if (xhalo) fx[hr][0] = (kh.x + (Th - (R)rh) * kh.y) * ((ring[sc][hq][1] - ring[sc][hq][0]) * cur.wh);
+    }
+    if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
+    __syncthreads();
+    if (live) {
+      R Rv = 0;
+      {
+#pragma clang fp contract(off)
+        if (i > 1) Rv = Rv + fx[ty][tx];
+        if (hx) Rv = Rv - fx[ty][tx + 1];
+        if (j > 1) Rv = Rv + fy[ty][tx];
+        if (hy) Rv = Rv - fy[ty + 1][tx];
+        if (k > 1) Rv = Rv + fz_prev;
+        if (k < B.nk) Rv = Rv - fz;
+      }
+      {
+#pragma clang fp contract(off)
+        // BC_Fluxes contributions in face order (as apply_bc_faces)
+        if (xo >= 0) Rv = Rv - cur.bx;
+        if (yo >= 0) Rv = Rv - cur.by;
+        if (zo_of(k) >= 0) Rv = Rv - cur.bz;
+      }
+      const R dt = a.use_dtg ? dtg : cur.dt;
+      R res;
+      {
+#pragma clang fp contract(off)
+        if (PRIM) {
+          // Compute_Prim_Residual, Lib_Newstate.f90:241-256 (as scaled_residual)
+          const R rho = pc.x + (Tcl - (R)rc) * pc.y;
+          const R cp = pc.z + (Tcl - (R)rc) * pc.w;
+          res = (-Rv / cur.vol + cur.qv) * dt / (Tc * cp * pc.y + rho * cp + Tc * rho * pc.w);
+        } else {
+          res = (-Rv / cur.vol + cur.qv) * dt;
+        }
+      }
+      int bad2 = badc;
+      if (IRSMODE) {
+        stu(a.Rres, p, res);
+      } else {
+        const R Tn = new_state<PRIM>(a.tab, cur.m, Tc, cur.TO, res, a.rkc, a.jam, &bad2);
+        stu(a.Tout, p, Tn);
+        if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+        const double d = (double)Tn - (double)cur.TO;
+        rsq += d * d;
+      }
+      if (bad2) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+    }
+    fz_prev = fz;
+    const int s0 = sm;
+    sm = sc;
+    sc = sp;
+    sp = s0;
+    p += sz;
+    ph += sz;
+  };
+  // two planes per trip with the stream buffers swapping roles (no copy of
+  // in-flight registers)
+  for (int k = k0; k <= k1; k += 2) {
+    plane(sa_, sb_, k);
+    if (k + 1 <= k1) plane(sb_, sa_, k + 1);
   }
   if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
 }
@@ -1655,6 +1914,8 @@ struct Level {
   int plist = 0;                    // tile list of the last step's residual partials
   int no_ycon = 1;                  // no connection on a j face (fused x-y IRS allowed)
   int no_zcon = 1;                  // no connection on a k face (with no_ycon: fused x-y-z IRS)
+  int all_umat = 0;                 // every block has a single material
+  int dims2 = 0;                    // every block has >= 2 cells in each direction
   long npad = 0, ncell = 0, nnodes = 0;
   DBuf<BlockInfo> blk;
   DBuf<Tile> tiles, ctiles, xytiles;
@@ -1668,7 +1929,7 @@ struct Level {
   DBuf<BCHot> bch;                  // compact wall records (k_bcflux)
   DBuf<int2> gmap;                  // (ghost, source) per entry (k_ghost)
   DBuf<int> fmap;
-  DBuf<double> bcv;
+  DBuf<double> bcv, bcf;
   int nbound = 0;
   std::vector<int> hib;             // nbound x 15
   std::vector<double> hrb;          // nbound x 6
@@ -1905,6 +2166,8 @@ int build_level_layout(fuss_ctx* c, Level& L) {
   }
   L.npad = off;
   L.nnodes = noff;
+  L.dims2 = 1;
+  for (int b = 0; b < L.nb; b++) L.dims2 &= L.hblk[b].ni >= 2 && L.hblk[b].nj >= 2 && L.hblk[b].nk >= 2;
   const std::vector<Tile>* lists[3] = {&L.htiles, &L.hctiles, &L.hxytiles};
   for (int q = 0; q < 3; q++) {
     L.hrng[q].assign(L.nb + 1, (int)lists[q]->size());
@@ -1985,10 +2248,12 @@ int build_bc(fuss_ctx* c, Level& L) {
     const int mi = f <= 2 ? (j - 1) + B.nj * (k - 1) : (f <= 4 ? (i - 1) + B.ni * (k - 1) : (i - 1) + B.ni * (j - 1));
     if (fmap[B.fmap_off[f - 1] + mi] != -1) return FUSS_EBC;
     fmap[B.fmap_off[f - 1] + mi] = e;
+    const int fpos = B.fmap_off[f - 1] + mi;
     if (type == 102) return FUSS_EUNSUPPORTED;
     R.type = type; R.f = f; R.dir = (f + 1) / 2; R.series = L.hser.empty() ? -1 : L.hser[e];
     R.b = b; R.i = i; R.j = j; R.k = k;
     R.cell = cidx(B, i, j, k);
+    R.fpos = fpos;
     R.qw = r[0]; R.Tw = r[1]; R.hconv = r[2]; R.Tref = r[4]; R.eps = r[5];
     const int gi = guide(f, 0), gj = guide(f, 1), gk = guide(f, 2);
     const bool kface = f >= 5;
@@ -2039,6 +2304,7 @@ int build_bc(fuss_ctx* c, Level& L) {
   e |= L.bch.alloc(std::max(n, 1));
   e |= L.gmap.alloc(gm.size());
   e |= L.bcv.alloc(std::max(n, 1));
+  e |= L.bcf.alloc(std::max<size_t>(fmap.size(), 1));
   e |= L.fmap.alloc(std::max<size_t>(fmap.size(), 1));
   if (e) return FUSS_EHIP;
   if (n) e |= upload(L.bc.p, recs.data(), sizeof(BCRec) * n);
@@ -2184,7 +2450,9 @@ int fuss_create(int nlevels, int nblocks, int precision, fuss_ctx** out) {
 
 int fuss_set_table(fuss_ctx* c, int nmat, int Ti1, int nT, const double* cp, const double* rho,
                    const double* kappa, const double* h) {
-  if (!c || nmat < 1 || nT < 2 || !cp || !rho || !kappa || !h) return FUSS_EARG;
+  // rows start at a non-negative temperature (Load_Table reads rows from
+  // T = 1 K): the kernels' row test int(T) < Ti2 then equals T < Ti2
+  if (!c || nmat < 1 || nT < 2 || Ti1 < 0 || !cp || !rho || !kappa || !h) return FUSS_EARG;
   const size_t n = (size_t)nmat * nT;
   c->nmat = nmat; c->Ti1 = Ti1; c->nT = nT;
   c->tcp.assign(cp, cp + n); c->trho.assign(rho, rho + n); c->tkap.assign(kappa, kappa + n); c->th.assign(h, h + n);
@@ -2400,6 +2668,11 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
       // a child's material, Lib_Multigrid.f90:113)
       for (int lv = 0; lv < c->nlev; lv++) c->lev[lv].hblk[b].umat = umat;
     }
+    for (int lv = 0; lv < c->nlev; lv++) {
+      Level& M = c->lev[lv];
+      M.all_umat = 1;
+      for (int b = 0; b < M.nb; b++) M.all_umat &= M.hblk[b].umat >= 0;
+    }
     for (int lv = 0; lv < c->nlev; lv++)
       HIPCHK(hipMemcpy(c->lev[lv].blk.p, c->lev[lv].hblk.data(), sizeof(BlockInfo) * c->lev[lv].nb,
                        hipMemcpyHostToDevice));
@@ -2488,9 +2761,17 @@ void launch_stage3d_k(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool ta
   if (tang) {
     if (prim) { irs ? k_stage3d<true, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
     else { irs ? k_stage3d<true, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
-  } else if (!row) {
+  } else if (!row && !L.dims2) {
     if (prim) { irs ? k_stage3d<false, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
     else { irs ? k_stage3d<false, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+  } else if (!row) {
+    if (L.all_umat) {
+      if (prim) { irs ? k_stage3p<true, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3p<true, true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+      else { irs ? k_stage3p<true, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3p<true, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+    } else {
+      if (prim) { irs ? k_stage3p<false, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3p<false, true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+      else { irs ? k_stage3p<false, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3p<false, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+    }
   } else {
     if (prim) { irs ? k_stage3w<true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
     else { irs ? k_stage3w<false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
@@ -2590,7 +2871,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     sa.dt = L.dt.p; sa.dtg = c->scal.p; sa.use_dtg = p->time_accurate; sa.qvol = L.qvol.p; sa.vol = L.vol.p;
     sa.mat = L.mat.p;
     for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
-    sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.tab = dtab<double>(c);
+    sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.bcf = L.bcf.p; sa.tab = dtab<double>(c);
     sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = halt;
     if (L.any_active && L.nbound) {
       pid = prof_begin(c, FUSS_K_BC);
@@ -2954,7 +3235,7 @@ int refine_run(fuss_ctx* c, const fuss_step_params* mon, fuss_refine_result* out
     sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.ctiles = L.ctiles.p; sa.T = T; sa.TO = T;
     sa.qvol = L.qvol.p; sa.vol = L.vol.p; sa.mat = L.mat.p;
     for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
-    sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.tab = dtab<double>(c);
+    sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.bcf = L.bcf.p; sa.tab = dtab<double>(c);
     sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = L.cnt.p + 1;
     sa.partials = L.partials.p; sa.status = c->status.p;
     const bool d3 = L.kp == GC;
@@ -3140,7 +3421,7 @@ int adj_backward_t(fuss_ctx* c, const double* g, double* gqw, double* gk, double
         sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.T = A.stage[sidx - 1].p; sa.qvol = L.qvol.p; sa.vol = L.vol.p;
         sa.mat = L.mat.p;
         for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
-        sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.tab = dtab<double>(c); sa.status = c->status.p;
+        sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.bcf = L.bcf.p; sa.tab = dtab<double>(c); sa.status = c->status.p;
         sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = L.cnt.p + 1;
         if (L.nbound && L.any_active)
           k_bcflux<NDIR, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
@@ -3694,7 +3975,7 @@ void fuss_destroy(fuss_ctx* c) {
     L.M.free();
     for (auto& f : L.F) f.free();
     for (auto& w : L.W) w.free();
-    L.bc.free(); L.bcv.free(); L.fmap.free(); L.xytiles.free(); L.partials.free(); L.pmin.free(); L.red2.free(); L.ctiles.free();
+    L.bc.free(); L.bcv.free(); L.bcf.free(); L.bch.free(); L.gmap.free(); L.fmap.free(); L.xytiles.free(); L.partials.free(); L.pmin.free(); L.red2.free(); L.ctiles.free();
   }
   c->dcp.free(); c->drho.free(); c->dkap.free(); c->dh.free(); c->dkap2.free(); c->dpr4.free();
   c->status.free(); c->scal.free();

@@ -62,7 +62,8 @@ struct BCRec {
   long gdst, gsrc;     // layer-1 ghost of the cell and its source (mirror or donor)
   long dn[4];          // donor cells of Tg(3..6) (connections)
   long rdonor;         // donor cell whose R fills the IRS ghost (connections)
-  int matg, pad0;      // material of the connection ghost (matIDg)
+  int matg;           // material of the connection ghost (matIDg)
+  int fpos;           // position of the entry's face cell in the level's face map
   double qw, Tw, hconv, Tref, eps;
   double A, N[3];
   double M[9];         // metric tensor of the boundary cell, c(3,3) column-major
@@ -71,13 +72,14 @@ struct BCRec {
 };
 
 // The per-stage fields of a wall entry of type 301, 303, 304 or 305 in one
-// 72-byte record (the full BCRec is ~380 bytes): the surface flux pass reads
+// 80-byte record (the full BCRec is ~380 bytes): the surface flux pass reads
 // these instead of whole records.  Filled on the device from the BCRec
 // array after its geometry (k_bc_hot) and again after fuss_set_bc_values.
 struct BCHot {
   int type, f, m, series;  // BC type, face, material of the cell, time series (-1: none)
   unsigned cell;           // arena index of the boundary cell
   int b;
+  int fpos, pad_;          // face-map position of the entry
   double A;                // face area
   double mn;               // dot(M(dir,:), N) of the boundary cell
   double hconv, Tref, qw, eps;

@@ -76,7 +76,7 @@ typedef struct {
 int fuss_create(int nlevels, int nblocks, int precision, fuss_ctx **out);
 
 /* Property tables, Load_Table (src/lib/io/Load_Table.f90:10-75): nmat
- * materials x nT rows of 1 K starting at Ti1; arrays nmat*nT, T fastest. */
+ * materials x nT rows of 1 K starting at Ti1 >= 0; arrays nmat*nT, T fastest. */
 int fuss_set_table(fuss_ctx *ctx, int nmat, int Ti1, int nT, const double *cp,
                    const double *rho, const double *kappa, const double *h);
 
