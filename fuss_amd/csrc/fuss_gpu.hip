@@ -1541,9 +1541,13 @@ __global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
 // 64x4 column marches through its KC planes.  Each R plane enters LDS once
 // with a 2-cell i/j halo, is smoothed along x then y there (the y ghosts are
 // the x-smoothed boundary rows, as in k_irs_xy), and its 64x4 result joins a
-// register window s(k-2..k+2) that the z sweeps read (z ghosts: the
+// register window s(q-4..q) that the z sweeps read (z ghosts: the
 // xy-smoothed boundary planes).  R is read from HBM once per stage instead of
-// three times (x-y pass in, z pass in) plus once written.
+// three times (x-y pass in, z pass in) plus once written.  Planes are
+// fetched two ahead into two register sets that alternate over a 2-plane
+// unrolled loop; every load is unconditional (clamped plane, valid cells for
+// out-of-block lanes) so the waits leave the later planes in flight.
+// XCON: some block has a connection on an i face (x ghosts from the donors).
 constexpr int XYZ_RJ = TILE_Y + 4, XYZ_RI = TILE_X + 4;
 constexpr int XYZ_NLD = (XYZ_RJ * XYZ_RI + TILE_T - 1) / TILE_T;  // plane loads per thread
 
@@ -1552,22 +1556,32 @@ struct XyzPlane {
   R v[XYZ_NLD];
 };
 
+// Element u of a thread's share of the tile's (TILE_Y+4) x (TILE_X+4) halo
+// box lies in the block.
+__device__ __forceinline__ bool xyz_in(const BlockInfo& B, int i0, int j0, int u) {
+  const int e = threadIdx.x + u * TILE_T;
+  const int gi = i0 - 2 + e % XYZ_RI, gj = j0 - 2 + e / XYZ_RI;
+  return e < XYZ_RJ * XYZ_RI && gi >= 1 && gi <= B.ni && gj >= 1 && gj <= B.nj;
+}
+
+// The R values of plane q in the halo box, raw: cells outside the block read
+// the box's first in-block cell, and the LDS store writes 0 for them (the
+// mask is applied there, so no wait for these loads is placed before).
 template <typename R>
 __device__ __forceinline__ void xyz_fetch(XyzPlane<R>& r, const R* __restrict__ Rin, const BlockInfo& B, int i0,
                                           int j0, int q) {
   const unsigned corner = (unsigned)(B.base + (i0 - 3) + (long)(j0 - 3) * B.sy + (long)(q - 1) * B.sz);
+  // first in-block cell of the box: (max(i0-2,1), max(j0-2,1))
+  const unsigned safe = corner + (i0 > 2 ? 0u : (unsigned)(3 - i0)) + (j0 > 2 ? 0u : (unsigned)(3 - j0)) * (unsigned)B.sy;
 #pragma unroll
   for (int u = 0; u < XYZ_NLD; u++) {
     const int e = threadIdx.x + u * TILE_T;
     const int ii = e % XYZ_RI, jj = e / XYZ_RI;
-    const int gi = i0 - 2 + ii, gj = j0 - 2 + jj;
-    r.v[u] = (e < XYZ_RJ * XYZ_RI && gi >= 1 && gi <= B.ni && gj >= 1 && gj <= B.nj)
-                 ? ldu(Rin, corner + ii + jj * (unsigned)B.sy)
-                 : (R)0;
+    r.v[u] = ldu(Rin, xyz_in(B, i0, j0, u) ? corner + ii + jj * (unsigned)B.sy : safe);
   }
 }
 
-template <bool PRIM, typename R, int KCT>
+template <bool PRIM, bool XCON, typename R, int KCT>
 __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
   if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -1580,87 +1594,90 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
   __shared__ R rin[XYZ_RJ][XYZ_RI];
   __shared__ R xs[XYZ_RJ][TILE_X];
   const R beta = a.beta, den = a.den, inv = a.inv;
-  // xy-smoothed value of this thread's cell on plane q (the staged plane in
-  // `pre`); every thread takes part in the two barriers.  The next plane's
-  // LDS store follows the second barrier (rin is no longer read), and its
-  // x sweeps follow the next first barrier (xs is no longer read).
-  auto smooth_plane = [&](XyzPlane<R>& pre, int q, int qb) -> R {
+  // window w[0..4] = s(q-4 .. q); planes outside 1..n are 0 (jacobi2 selects
+  // the ghosts there)
+  R w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0;
+  R zlo = 0, zhi = 0;  // xy-smoothed planes 1 and n (z ghosts of walls)
+  const int qa = max(1, k0 - 2), qb = min(n, k1 + 2), qe = k1 + 2;
+  const int ci = live ? i : 1, cj = live ? j : 1;
+  const unsigned p1 = (unsigned)(B.base + (ci - 1) + (long)(cj - 1) * B.sy);  // plane 1 of the lane's column
+  const unsigned sz = (unsigned)B.sz;
+  double rsq = 0.0;
+  bool inb[XYZ_NLD];
 #pragma unroll
-    for (int u = 0; u < XYZ_NLD; u++) {
-      const int e = threadIdx.x + u * TILE_T;
-      if (e < XYZ_RJ * XYZ_RI) (&rin[0][0])[e] = pre.v[u];
-    }
-    // the next plane's loads fly while this one is smoothed
-    if (q < qb) xyz_fetch(pre, a.Rin, B, t.i0, t.j0, q + 1);
-    __syncthreads();
-    // x sweeps of rows j0-2 .. j0+5 (two per thread)
+  for (int u = 0; u < XYZ_NLD; u++) inb[u] = xyz_in(B, t.i0, t.j0, u);
+  XyzPlane<R> pa, pb;
+  xyz_fetch(pa, a.Rin, B, t.i0, t.j0, qa);
+  xyz_fetch(pb, a.Rin, B, t.i0, t.j0, min(qa + 1, qb));
+  // one step: plane q (staged in `pre`) is smoothed in x-y (if q <= qb), the
+  // window shifts, and plane q-2 is output (if it is in the column)
+  auto step = [&](XyzPlane<R>& pre, int q) {
+    const int ko = q - 2;
+    const bool out = ko >= k0;
+    const unsigned po = p1 + (unsigned)(max(ko, 1) - 1) * sz;
+    const R T = ldu(a.T, po), TO = ldu(a.TO, po);
+    int m = B.umat;
+    if (m < 0) m = ldu(a.mat, po);
+    R s = 0;
+    if (q <= qb) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int jj = ty + h * TILE_Y, jr = t.j0 - 2 + jj;
-      if (i <= B.ni && jr >= 1 && jr <= B.nj) {
-        R glo = 0, ghi = 0;
-        if (i <= 2) glo = irs_ghost(a, B, 1, 1, jr, q, rin[jj][tx + 2 - (i - 1)]);
-        if (i >= B.ni - 1) ghi = irs_ghost(a, B, 2, B.ni, jr, q, rin[jj][tx + 2 + (B.ni - i)]);
-        xs[jj][tx] = jacobi2(rin[jj][tx], rin[jj][tx + 1], rin[jj][tx + 2], rin[jj][tx + 3], rin[jj][tx + 4], i,
-                             B.ni, glo, ghi, beta, den, inv);
+      for (int u = 0; u < XYZ_NLD; u++) {
+        const int e = threadIdx.x + u * TILE_T;
+        if (e < XYZ_RJ * XYZ_RI) (&rin[0][0])[e] = inb[u] ? pre.v[u] : (R)0;
       }
     }
-    __syncthreads();
-    R s = 0;
-    if (live) {
-      const int jj = ty + 2;
-      const R glo = (j <= 2) ? xs[jj - (j - 1)][tx] : (R)0;
-      const R ghi = (j >= B.nj - 1) ? xs[jj + (B.nj - j)][tx] : (R)0;
-      s = jacobi2(xs[jj - 2][tx], xs[jj - 1][tx], xs[jj][tx], xs[jj + 1][tx], xs[jj + 2][tx], j, B.nj, glo, ghi, beta,
-                  den, inv);
-    }
-    return s;
-  };
-  // window w[0..4] = s(k-2 .. k+2); planes outside 1..n stay 0 (jacobi2
-  // selects the ghosts there)
-  R w[5] = {0, 0, 0, 0, 0};
-  R zlo = 0, zhi = 0;  // xy-smoothed planes 1 and n (z ghosts of walls)
-  XyzPlane<R> pre;
-  const int qa = max(1, k0 - 2), qb = min(n, k1 + 2);
-  xyz_fetch(pre, a.Rin, B, t.i0, t.j0, qa);
-  for (int q = qa; q <= min(qb, k0 + 1); q++) {
-    const R s = smooth_plane(pre, q, qb);
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-      if (q - k0 + 2 == u) w[u] = s;  // static register indices
-    if (q == 1) zlo = s;
-    if (q == n) zhi = s;
-  }
-  const unsigned sz = (unsigned)B.sz;
-  unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k0 - 1) * B.sz);
-  double rsq = 0.0;
-  for (int k = k0; k <= k1; k++, p += sz) {
-    R T = 0, TO = 0;
-    int m = B.umat;
-    if (live) {
-      T = ldu(a.T, p);
-      TO = ldu(a.TO, p);
-      if (m < 0) m = ldu(a.mat, p);
-    }
-    const int q = k + 2;
+    // plane q+2 flies while this one is smoothed (issued on every step, so
+    // the wait for T and TO below can leave it outstanding)
+    xyz_fetch(pre, a.Rin, B, t.i0, t.j0, min(q + 2, qb));
     if (q <= qb) {
-      const R s = smooth_plane(pre, q, qb);
-      w[4] = s;
+      __syncthreads();
+      // x sweeps of rows j0-2 .. j0+5 (two per thread)
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int jj = ty + h * TILE_Y, jr = t.j0 - 2 + jj;
+        if (i <= B.ni && jr >= 1 && jr <= B.nj) {
+          R glo = 0, ghi = 0;
+          if (i <= 2) {
+            glo = rin[jj][tx + 2 - (i - 1)];
+            if (XCON) glo = irs_ghost(a, B, 1, 1, jr, q, glo);
+          }
+          if (i >= B.ni - 1) {
+            ghi = rin[jj][tx + 2 + (B.ni - i)];
+            if (XCON) ghi = irs_ghost(a, B, 2, B.ni, jr, q, ghi);
+          }
+          xs[jj][tx] = jacobi2(rin[jj][tx], rin[jj][tx + 1], rin[jj][tx + 2], rin[jj][tx + 3], rin[jj][tx + 4], i,
+                               B.ni, glo, ghi, beta, den, inv);
+        }
+      }
+      __syncthreads();
+      // y sweeps of the lane's cell: the next plane's LDS store follows the
+      // second barrier (rin no longer read), its x sweeps the next first
+      // barrier (xs no longer read)
+      if (live) {
+        const int jj = ty + 2;
+        const R glo = (j <= 2) ? xs[jj - (j - 1)][tx] : (R)0;
+        const R ghi = (j >= B.nj - 1) ? xs[jj + (B.nj - j)][tx] : (R)0;
+        s = jacobi2(xs[jj - 2][tx], xs[jj - 1][tx], xs[jj][tx], xs[jj + 1][tx], xs[jj + 2][tx], j, B.nj, glo, ghi,
+                    beta, den, inv);
+      }
+      if (q == 1) zlo = s;
       if (q == n) zhi = s;
-    } else {
-      w[4] = 0;
     }
-    if (live) {
-      const R rs = jacobi2(w[0], w[1], w[2], w[3], w[4], k, n, zlo, zhi, beta, den, inv);
+    w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = s;
+    if (out && live) {
+      const R rs = jacobi2(w0, w1, w2, w3, w4, ko, n, zlo, zhi, beta, den, inv);
       int bad = 0;
       const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
-      stu(a.Tout, p, Tn);
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
-      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+      stu(a.Tout, po, Tn);
+      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, ko);
+      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, ko);
       const double d = (double)Tn - (double)TO;
       rsq += d * d;
     }
-    w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
+  };
+  for (int q = qa; q <= qe; q += 2) {
+    step(pa, q);
+    if (q + 1 <= qe) step(pb, q + 1);
   }
   if (a.last) block_sum(rsq, a.partials, wg);
 }
@@ -1914,6 +1931,7 @@ struct Level {
   int plist = 0;                    // tile list of the last step's residual partials
   int no_ycon = 1;                  // no connection on a j face (fused x-y IRS allowed)
   int no_zcon = 1;                  // no connection on a k face (with no_ycon: fused x-y-z IRS)
+  int no_xcon = 1;                  // no connection on an i face
   int all_umat = 0;                 // every block has a single material
   int dims2 = 0;                    // every block has >= 2 cells in each direction
   long npad = 0, ncell = 0, nnodes = 0;
@@ -2215,6 +2233,7 @@ int build_bc(fuss_ctx* c, Level& L) {
   const int n = L.nbound;
   L.no_ycon = 1;
   L.no_zcon = 1;
+  L.no_xcon = 1;
   // grouping check: (block, face) non-decreasing, as BC_Fluxes assumes
   // (Mod_BC_Fluxes.f90:26-33)
   for (int e = 1; e < n; e++) {
@@ -2289,6 +2308,7 @@ int build_bc(fuss_ctx* c, Level& L) {
     }
     if (type == 101 && (f == 3 || f == 4)) L.no_ycon = 0;
     if (type == 101 && (f == 5 || f == 6)) L.no_zcon = 0;
+    if (type == 101 && (f == 1 || f == 2)) L.no_xcon = 0;
     const bool active = type == 101 || (type >= 300 && type <= 305);
     if (active) B.face_active |= 1 << (f - 1);
     if (type == 101) B.conn |= 1 << (f - 1);
@@ -2791,11 +2811,17 @@ void launch_irs_z(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
   else prim ? k_irs_z<true, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
 }
 
-void launch_irs_xyz(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
+template <bool XCON>
+void launch_irs_xyz_x(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
   const unsigned g = (unsigned)L.hctiles.size();
-  if (L.kc == 4) prim ? k_irs_xyz<true, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
-  else if (L.kc == 16) prim ? k_irs_xyz<true, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
-  else prim ? k_irs_xyz<true, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
+  if (L.kc == 4) prim ? k_irs_xyz<true, XCON, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
+  else if (L.kc == 16) prim ? k_irs_xyz<true, XCON, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
+  else prim ? k_irs_xyz<true, XCON, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
+}
+
+void launch_irs_xyz(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
+  if (L.no_xcon) launch_irs_xyz_x<false>(c, L, ia, prim);
+  else launch_irs_xyz_x<true>(c, L, ia, prim);
 }
 
 template <int DIR, int NDIR>
