@@ -304,8 +304,9 @@ def lattice_check(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # fuss_run replays captured chunks of 30 steps: keep both multiples of 30
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-iters", type=int, default=2)
@@ -338,17 +339,22 @@ def main():
     tuned = s.autotune(1, steps=4)
     print(f"[bench] autotune: column height {tuned[0]}, {'row march' if tuned[1] else 'LDS ring'}, "
           f"{tuned[2]:.3f} ms/step", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        s.explicit_step(1)
+    # warm-up: module load and the capture of fuss_run's step graph
+    s.run_level(1, max(args.warmup, 1), max(args.warmup, 1), -1.0, 1e300, hist=False)
     device_sync()
-    s.L.fuss_profile(s.h, 1)
-    device_sync()
+    # timed region: `steps` Explicit_Steps through fuss_run (the library's
+    # step loop: captured graph replay, stop tests on the device)
     t0 = time.perf_counter()
-    res = None
-    for _ in range(args.steps):
-        res = s.explicit_step(1)
+    k, _, _, res = s.run_level(1, args.steps, args.steps, -1.0, 1e300, hist=False)
     device_sync()
     wall = time.perf_counter() - t0
+    assert k == args.steps, (k, args.steps)
+    # per-kernel durations: the same steps again, one at a time, with HIP
+    # events around every launch on the library's stream
+    s.L.fuss_profile(s.h, 1)
+    for _ in range(args.steps):
+        s.explicit_step(1)
+    device_sync()
     ncl = gpu.C.c_int * 8
     ms = (ctypes.c_double * 8)()
     cnt = ncl()

@@ -41,6 +41,7 @@ class Params:
     time_accurate: bool = False
     prim: bool = False            # integration-variables: cons (default) / prim
     irs_beta: float = 0.0         # IRS enabled when > 0
+    irs_mode: int = 0             # 0 jacobi2 (reference), 1 tridiag (exact line solve, no reference counterpart)
     mgl: int = 1                  # multigrid levels
     level_iter: list = field(default_factory=lambda: [0])
     res_threshold: float = 1e-10
@@ -98,6 +99,8 @@ def params_from_ini(ini):
     irs = g(ini, "FUSS-Numerics", "irs", ".false.", bool)
     beta = g(ini, "FUSS-Numerics", "irs-beta", "0.0", float)
     p.irs_beta = beta if (irs or beta > 0.0) else 0.0
+    # extension key (absent from the reference's registry): jacobi2 | tridiag
+    p.irs_mode = 1 if g(ini, "FUSS-Numerics", "irs-mode", "jacobi2") == "tridiag" else 0
     p.mgl = max(1, g(ini, "FUSS-Multigrid", "levels", "1", int))
     p.level_iter = [g(ini, "FUSS-Multigrid", f"level{m+1}-iter", "0", int) for m in range(p.mgl)]
     p.res_threshold = g(ini, "FUSS-Parameters", "res-threshold", "1e-10", float)

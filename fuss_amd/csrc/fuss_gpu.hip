@@ -938,7 +938,6 @@ struct StageStream {
   R vol, qv, dt, TO;   // per-cell streams
   R w0, w4, w8;        // normal weights of the cell's +x, +y, +z faces
   R wh;                // halo lanes: weight of the tile's min-side face
-  R bx, by, bz;        // BC values of the cell's boundary face in x, y, z (if any)
   int m, mh;           // material of the cell / of the halo face's left cell
 };
 
@@ -1015,10 +1014,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   const int zhi = (fa & 32) ? B.fmap_off[5] + (i - 1) + B.ni * (j - 1) : -1;
   auto zo_of = [&](int kk) { return kk == 1 ? zlo : (kk == B.nk ? zhi : -1); };
   const R dtg = a.use_dtg ? (R)(*a.dtg) : (R)0;
-  auto stream = [&](StageStream<R>& s, unsigned pp, unsigned hh, int kk) {
-    s.bx = ldu(a.bcf, (unsigned)max(xo >= 0 ? xo + B.nj * (kk - 1) : 0, 0));
-    s.by = ldu(a.bcf, (unsigned)max(yo >= 0 ? yo + B.ni * (kk - 1) : 0, 0));
-    s.bz = ldu(a.bcf, (unsigned)max(zo_of(kk), 0));
+  auto stream = [&](StageStream<R>& s, unsigned pp, unsigned hh) {
     s.vol = ldu(a.vol, pp);
     s.qv = ldu(a.qvol, pp);
     s.dt = ldu(a.dt, pp);
@@ -1035,7 +1031,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   StageStream<R> sa_, sb_;
   sa_.m = sa_.mh = sb_.m = sb_.mh = mu;
   sa_.TO = sb_.TO = (R)0;
-  stream(sa_, p, ph, k0);
+  stream(sa_, p, ph);
   __syncthreads();
   R fz_prev = 0;
   if (live && k0 > 1) {
@@ -1052,10 +1048,15 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
     int bad = 0, badc = 0;
     const R Tc = ring[sc][y][x];
     const R Txr = ring[sc][y][x + 1], Tyr = ring[sc][y + 1][x], Tzr = ring[sp][y][x];
-    R Tx, Ty, Tz, Th, Tcl = Tc;
-    int rx, ry, rz, rh, rc = Ti1;
-    V2 kx, ky, kz, kh;
+    R Tx, Ty, Tz, Th = 0, Tcl = Tc;
+    int rx, ry, rz, rh = Ti1, rc = Ti1;
+    V2 kx, ky, kz, kh{};
     V4 pc{};
+    // BC values of the lane's boundary faces (one load each, issued with the
+    // table rows; lanes without such a face read entry 0)
+    const R bx = ldu(a.bcf, (unsigned)max(xo >= 0 ? xo + B.nj * (k - 1) : 0, 0));
+    const R by = ldu(a.bcf, (unsigned)max(yo >= 0 ? yo + B.ni * (k - 1) : 0, 0));
+    const R bz = ldu(a.bcf, (unsigned)max(zo_of(k), 0));
     {
 #pragma clang fp contract(off)
       const V2* kr = UM ? kap_u : kap0 + (long)cur.m * nT;
@@ -1073,10 +1074,12 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
         rc = tab_row(Tcl, Ti1, Ti2, &badc);
         pc = (UM ? pr_u : pr0 + (long)cur.m * nT)[rc];
       }
-      const V2* krh = UM ? kap_u : kap0 + (long)cur.mh * nT;
-      Th = yhalo ? (R)0.5 * (ring[sc][0][x] + ring[sc][1][x]) : (R)0.5 * (ring[sc][hq][0] + ring[sc][hq][1]);
-      rh = tab_row(Th, Ti1, Ti2, &b3);
-      kh = krh[rh];
+      if (ty < 2) {  // waves 0 and 1 hold the halo lanes (a wave-uniform test)
+        const V2* krh = UM ? kap_u : kap0 + (long)cur.mh * nT;
+        Th = yhalo ? (R)0.5 * (ring[sc][0][x] + ring[sc][1][x]) : (R)0.5 * (ring[sc][hq][0] + ring[sc][hq][1]);
+        rh = tab_row(Th, Ti1, Ti2, &b3);
+        kh = krh[rh];
+      }
       bad = (live && ((hx && b0) || (hy && b1) || (k < B.nk && b2))) || ((yhalo || xhalo) && b3);
       badc = live && badc;
     }
@@ -1084,7 +1087,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
     {
       const unsigned step = k < k1 ? sz : 0u;
       fetch_plane_u(pre, a.T, B, t.i0, t.j0, k < k1 ? k + 2 : k + 1);
-      stream(nxt, p + step, ph + step, k < k1 ? k + 1 : k);
+      stream(nxt, p + step, ph + step);
     }
     // phase C: fluxes
     R fz = 0;
@@ -1114,9 +1117,9 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
       {
 #pragma clang fp contract(off)
         // BC_Fluxes contributions in face order (as apply_bc_faces)
-        if (xo >= 0) Rv = Rv - cur.bx;
-        if (yo >= 0) Rv = Rv - cur.by;
-        if (zo_of(k) >= 0) Rv = Rv - cur.bz;
+        if (xo >= 0) Rv = Rv - bx;
+        if (yo >= 0) Rv = Rv - by;
+        if (zo_of(k) >= 0) Rv = Rv - bz;
       }
       const R dt = a.use_dtg ? dtg : cur.dt;
       R res;
@@ -1499,6 +1502,185 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xy(IrsArgs<R> a) {
   if (UPDATE && a.last) block_sum(rsq, a.partials, wg);
 }
 
+// ------------------------------------------------------------------------
+// Line-tridiagonal IRS (FUSS_IRS_TRIDIAG, no reference counterpart): per
+// direction the exact solution of (1+2b) x_s - b (x_{s-1} + x_{s+1}) = r_s,
+// s = 1..n, with the IRS ghosts x_0, x_{n+1} of Residual_Smoothing held
+// fixed -- the fixed point of the reference's Jacobi sweeps
+// (Lib_IRS.f90:63-89).  Thomas algorithm, one line per work-item; the
+// elimination factors den_s = (1+2b) + b c_{s-1}, c_s = -b / den_s depend
+// only on s and are tabulated per block and direction on the host (tri).
+// Forward: d_1 = (r_1 + b x_0) / den_1, d_s = (r_s + b d_{s-1}) / den_s
+// (r_n + b x_{n+1} on the last); back: x_n = d_n, x_s = d_s - c_s x_{s+1}.
+// The d_s go to Rout and are overwritten by x_s on the way back.
+// ------------------------------------------------------------------------
+template <typename R>
+struct TriArgs {
+  const BlockInfo* blk;
+  const Tile* tiles;
+  const R* Rin;
+  R* Rout;
+  const BCRec* bc;
+  const int* fmap;
+  const double2* tri;  // (den_s, c_s) rows
+  const int* trioff;   // per block and direction: first row of its table
+  R beta;
+  const int* halt;
+};
+
+// Ghost of a line end for Residual_Smoothing: the donor's R on a
+// connection, else the line's own end value (Ghost_Residual_Extrapolation,
+// Lib_IRS.f90:137-187).
+template <typename R>
+__device__ __forceinline__ R tri_ghost(const TriArgs<R>& a, const BlockInfo& B, int f, int i, int j, int k, R own) {
+  if (!(B.conn & (1 << (f - 1)))) return own;
+  const BCRec& r = a.bc[a.fmap[face_map_index<R>(B, f, i, j, k)]];
+  return r.type == 101 ? ldu(a.Rin, (unsigned)r.rdonor) : own;
+}
+
+// y (DIR 1) and z (DIR 2) lines: a 64x4 tile of line origins, i fastest, so
+// every step of the sweep is a coalesced row access.
+template <int DIR, typename R>
+__global__ __launch_bounds__(TILE_T) void k_tri_yz(TriArgs<R> a) {
+  if (a.halt && *a.halt) return;
+  const Tile t = a.tiles[blockIdx.x];
+  const BlockInfo B = a.blk[t.b];
+  const int i = t.i0 + threadIdx.x % TILE_X, o2 = threadIdx.x / TILE_X;
+  // DIR 1: lines (i, k = k0 + o2); DIR 2: lines (i, j = j0 + o2)
+  const int j = DIR == 1 ? 1 : t.j0 + o2, k = DIR == 1 ? t.k0 + o2 : 1;
+  const int n = DIR == 1 ? B.nj : B.nk;
+  if (i > B.ni || (DIR == 1 ? k > B.nk : j > B.nj)) return;
+  const unsigned st = DIR == 1 ? (unsigned)B.sy : (unsigned)B.sz;
+  const unsigned p1 = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz);
+  const double2* tab = a.tri + a.trioff[3 * t.b + DIR];
+  const R beta = a.beta;
+  const R r1 = ldu(a.Rin, p1), rn = ldu(a.Rin, p1 + (unsigned)(n - 1) * st);
+  const R glo = DIR == 1 ? tri_ghost(a, B, 3, i, 1, k, r1) : tri_ghost(a, B, 5, i, j, 1, r1);
+  const R ghi = DIR == 1 ? tri_ghost(a, B, 4, i, B.nj, k, rn) : tri_ghost(a, B, 6, i, j, B.nk, rn);
+  R d = 0;
+  unsigned p = p1;
+  for (int s = 1; s <= n; s++, p += st) {
+#pragma clang fp contract(off)
+    R r = ldu(a.Rin, p);
+    if (s == 1) r = r + beta * glo;
+    if (s == n) r = r + beta * ghi;
+    const R den = (R)tab[s - 1].x;
+    d = s == 1 ? r / den : (r + beta * d) / den;
+    stu(a.Rout, p, d);
+  }
+  R x = d;
+  p = p1 + (unsigned)(n - 1) * st;
+  for (int s = n - 1; s >= 1; s--) {
+#pragma clang fp contract(off)
+    p -= st;
+    x = ldu((const R*)a.Rout, p) - (R)tab[s - 1].y * x;
+    stu(a.Rout, p, x);
+  }
+}
+
+// x lines: one wave per 64 consecutive rows j of a plane; the lines pass
+// through LDS in chunks of 16 cells so that global accesses are whole
+// 128-byte row segments.
+constexpr int TRI_CH = 16;
+
+template <typename R>
+__global__ __launch_bounds__(64) void k_tri_x(TriArgs<R> a) {
+  if (a.halt && *a.halt) return;
+  const Tile t = a.tiles[blockIdx.x];
+  const BlockInfo B = a.blk[t.b];
+  const int l = threadIdx.x;                // this lane's line: row j0 + l
+  const int j = t.j0 + l, k = t.k0, n = B.ni;
+  const bool live = j <= B.nj;
+  __shared__ R ch[TRI_CH][TILE_X + 1];
+  const unsigned row0 = (unsigned)(B.base + (long)(t.j0 - 1) * B.sy + (long)(k - 1) * B.sz);  // (i=1, j0, k)
+  const double2* tab = a.tri + a.trioff[3 * t.b];
+  const R beta = a.beta;
+  R glo = 0, ghi = 0;
+  if (live) {
+    const unsigned pl = row0 + (unsigned)l * (unsigned)B.sy;
+    glo = tri_ghost(a, B, 1, 1, j, k, ldu(a.Rin, pl));
+    ghi = tri_ghost(a, B, 2, B.ni, j, k, ldu(a.Rin, pl + (unsigned)(n - 1)));
+  }
+  const int nch = (n + TRI_CH - 1) / TRI_CH;
+  // chunk c holds cells i = c*16+1 .. c*16+16 of the 64 rows: element e of a
+  // lane's 16 loads is row e/16, cell e%16 (16 lanes per 128-byte segment)
+  auto chunk_io = [&](int c, bool load, const R* src, R* dst) {
+    for (int q = 0; q < TRI_CH; q++) {
+      const int e = l + 64 * q, rr = e / TRI_CH, ii = e % TRI_CH;
+      const int i = c * TRI_CH + ii + 1;
+      if (t.j0 + rr <= B.nj && i <= n) {
+        const unsigned p = row0 + (unsigned)rr * (unsigned)B.sy + (unsigned)(i - 1);
+        if (load) ch[ii][rr] = ldu(src, p);
+        else stu(dst, p, ch[ii][rr]);
+      }
+    }
+  };
+  R d = 0;
+  for (int c = 0; c < nch; c++) {
+    chunk_io(c, true, a.Rin, nullptr);
+    __syncthreads();
+    if (live) {
+      for (int ii = 0; ii < TRI_CH; ii++) {
+#pragma clang fp contract(off)
+        const int s = c * TRI_CH + ii + 1;
+        if (s > n) break;
+        R r = ch[ii][l];
+        if (s == 1) r = r + beta * glo;
+        if (s == n) r = r + beta * ghi;
+        const R den = (R)tab[s - 1].x;
+        d = s == 1 ? r / den : (r + beta * d) / den;
+        ch[ii][l] = d;
+      }
+    }
+    __syncthreads();
+    chunk_io(c, false, nullptr, a.Rout);
+    __syncthreads();
+  }
+  __threadfence_block();
+  R x = d;
+  for (int c = nch - 1; c >= 0; c--) {
+    chunk_io(c, true, a.Rout, nullptr);
+    __syncthreads();
+    if (live) {
+      for (int ii = TRI_CH - 1; ii >= 0; ii--) {
+#pragma clang fp contract(off)
+        const int s = c * TRI_CH + ii + 1;
+        if (s >= n) continue;  // x_n = d_n stays
+        x = ch[ii][l] - (R)tab[s - 1].y * x;
+        ch[ii][l] = x;
+      }
+    }
+    __syncthreads();
+    chunk_io(c, false, nullptr, a.Rout);
+    __syncthreads();
+  }
+}
+
+// The RK update from the smoothed residual (after the last tridiagonal
+// direction), with the residual partials of the step.
+template <bool PRIM, typename R>
+__global__ __launch_bounds__(TILE_T) void k_rk_update(IrsArgs<R> a) {
+  if (a.halt && *a.halt) return;
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Tile t = a.tiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
+  double rsq = 0.0;
+  if (i <= B.ni && j <= B.nj) {
+    const unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz);
+    const R T = ldu(a.T, p), TO = ldu(a.TO, p), rs = ldu(a.Rin, p);
+    int bad = 0;
+    const int m = B.umat >= 0 ? B.umat : ldu(a.mat, p);
+    const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
+    stu(a.Tout, p, Tn);
+    if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+    if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+    const double dd = (double)Tn - (double)TO;
+    rsq = dd * dd;
+  }
+  if (a.last) block_sum(rsq, a.partials, wg);
+}
+
 // z smoothing (direction 3) fused with the RK update: a 64x4 column marches
 // through KC planes with r(k-2..k+2) in registers, so R is read once.
 template <bool PRIM, typename R, int KCT>
@@ -1761,19 +1943,20 @@ __global__ void k_set_clock(double* clk, int* cnt, double time, int iter) {
 // fuss_run on a captured graph: counters cnt[2] (steps of this run), cnt[3]
 // (steps before this chunk); a step that meets a stop test raises the halt
 // flag cnt[1], which turns every later kernel of the graph into a no-op.
-__global__ void k_run_begin(int* cnt) {
+__global__ void k_run_begin(int* cnt, int maxcount) {
   if (threadIdx.x == 0) {
     cnt[1] = 0;
     cnt[2] = 0;
+    cnt[4] = maxcount;
   }
 }
 __global__ void k_chunk_begin(int* cnt) {
   if (threadIdx.x == 0) cnt[3] = cnt[2];
 }
 __global__ void k_step_end(const double* __restrict__ scal, const double* __restrict__ clk, int* cnt,
-                           const int* __restrict__ status, double* hres, double* htime, double thr, double tthr,
-                           int maxcount) {
+                           const int* __restrict__ status, double* hres, double* htime, double thr, double tthr) {
   if (threadIdx.x != 0 || cnt[1]) return;
+  const int maxcount = cnt[4];
   const double res = sqrt(scal[2]);
   const int n = cnt[2];
   hres[n - cnt[3]] = res;
@@ -1932,6 +2115,13 @@ struct Level {
   int no_ycon = 1;                  // no connection on a j face (fused x-y IRS allowed)
   int no_zcon = 1;                  // no connection on a k face (with no_ycon: fused x-y-z IRS)
   int no_xcon = 1;                  // no connection on an i face
+  // line-tridiagonal IRS: line tiles (x: 64 rows of a plane; y, z: 64x4
+  // line origins) and the per-block elimination factors for tri_beta
+  std::vector<Tile> htx, hty, htz;
+  DBuf<Tile> ttx, tty, ttz;
+  DBuf<double2> tri;
+  DBuf<int> trioff;
+  double tri_beta = -1.0;
   int all_umat = 0;                 // every block has a single material
   int dims2 = 0;                    // every block has >= 2 cells in each direction
   long npad = 0, ncell = 0, nnodes = 0;
@@ -1956,7 +2146,14 @@ struct Level {
   double time = 0.0;                // host mirrors of the device clock
   int iter = 0;
   DBuf<double> clk;                 // [0] level time
-  DBuf<int> cnt;                    // [0] iteration, [1] halt, [2] run steps, [3] chunk start
+  DBuf<int> cnt;                    // [0] iteration, [1] halt, [2] run steps, [3] chunk start, [4] run step limit
+  // fuss_run's captured chunk of RUN_G steps, replayed by later calls with the
+  // same step parameters and stop thresholds (dropped when tiles change)
+  hipGraph_t run_graph = nullptr;
+  hipGraphExec_t run_exec = nullptr;
+  fuss_step_params run_p{};
+  double run_thr = 0.0, run_tthr = 0.0;
+  int run_seq[RUN_G + 1] = {};
   DBuf<double> partials, pmin, red2;
 };
 
@@ -2143,6 +2340,13 @@ int upload(void* dst, const void* src, size_t bytes) {
   return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : FUSS_EHIP;
 }
 
+void run_graph_free(Level& L) {
+  if (L.run_exec) (void)hipGraphExecDestroy(L.run_exec);
+  if (L.run_graph) (void)hipGraphDestroy(L.run_graph);
+  L.run_exec = nullptr;
+  L.run_graph = nullptr;
+}
+
 int build_level_layout(fuss_ctx* c, Level& L) {
   long off = 0, noff = 0;
   L.hblk.resize(L.nb);
@@ -2219,7 +2423,7 @@ int alloc_level(fuss_ctx* c, Level& L) {
   e |= L.partials.alloc(np);
   e |= L.pmin.alloc(np);
   e |= L.clk.alloc(2);
-  e |= L.cnt.alloc(4);
+  e |= L.cnt.alloc(5);
   for (int q = 0; q < 3; q++) {
     e |= L.rng[q].alloc(L.nb + 1);
     if (!e) e |= upload(L.rng[q].p, L.hrng[q].data(), (L.nb + 1) * sizeof(int));
@@ -2839,6 +3043,91 @@ void launch_irs(fuss_ctx* c, Level& L, int dir, const IrsArgs<double>& a, bool u
   else launch_irs_d<2, NDIR>(c, L, a, update, prim);
 }
 
+// Line tiles and elimination factors of the tridiagonal IRS for beta (host
+// arithmetic in the oracle's order: den_1 = 1+2b, den_s = (1+2b) + b c_{s-1},
+// c_s = -b / den_s).  Synchronous uploads: call outside stream capture.
+int tri_prepare(fuss_ctx* c, Level& L, double beta) {
+  if (L.tri_beta == beta && L.tri.p) return 0;
+  if (L.htx.empty()) {
+    for (int b = 0; b < L.nb; b++) {
+      const BlockInfo& B = L.hblk[b];
+      for (int k = 1; k <= B.nk; k++)
+        for (int j = 1; j <= B.nj; j += TILE_X) L.htx.push_back(Tile{b, 1, j, k});
+      for (int k = 1; k <= B.nk; k += TILE_Y)
+        for (int i = 1; i <= B.ni; i += TILE_X) L.hty.push_back(Tile{b, i, 1, k});
+      for (int j = 1; j <= B.nj; j += TILE_Y)
+        for (int i = 1; i <= B.ni; i += TILE_X) L.htz.push_back(Tile{b, i, j, 1});
+    }
+    if (L.ttx.alloc(L.htx.size()) || L.tty.alloc(L.hty.size()) || L.ttz.alloc(L.htz.size())) return FUSS_EHIP;
+    int e = upload(L.ttx.p, L.htx.data(), L.htx.size() * sizeof(Tile));
+    e |= upload(L.tty.p, L.hty.data(), L.hty.size() * sizeof(Tile));
+    e |= upload(L.ttz.p, L.htz.data(), L.htz.size() * sizeof(Tile));
+    if (e) return e;
+  }
+  std::vector<double2> tab;
+  std::vector<int> off(3 * L.nb);
+  {
+#pragma clang fp contract(off)
+    const double bb = 1.0 + 2.0 * beta;
+    for (int b = 0; b < L.nb; b++)
+      for (int d = 0; d < 3; d++) {
+        const BlockInfo& B = L.hblk[b];
+        const int n = d == 0 ? B.ni : (d == 1 ? B.nj : B.nk);
+        off[3 * b + d] = (int)tab.size();
+        double cp = 0.0;
+        for (int s = 1; s <= n; s++) {
+          const double den = s == 1 ? bb : bb + beta * cp;
+          cp = -beta / den;
+          tab.push_back(make_double2(den, cp));
+        }
+      }
+  }
+  if (L.tri.alloc(std::max<size_t>(tab.size(), 1)) || L.trioff.alloc(off.size())) return FUSS_EHIP;
+  int e = upload(L.tri.p, tab.data(), tab.size() * sizeof(double2));
+  e |= upload(L.trioff.p, off.data(), off.size() * sizeof(int));
+  if (e) return e;
+  L.tri_beta = beta;
+  (void)c;
+  return 0;
+}
+
+// Residual_Smoothing with the tridiagonal lines: x, y (, z) passes
+// R -> R2 -> R (-> R2), each with the ghosts of the current R, then the RK
+// update.  Returns the tile list count of the update's residual partials.
+int launch_irs_tridiag(fuss_ctx* c, Level& L, IrsArgs<double> ia, bool prim, int* npart) {
+  TriArgs<double> ta;
+  ta.blk = L.blk.p; ta.bc = L.bc.p; ta.fmap = L.fmap.p; ta.tri = L.tri.p; ta.trioff = L.trioff.p;
+  ta.beta = ia.beta; ta.halt = ia.halt;
+  double* rin = L.R.p;
+  double* rout = L.R2.p;
+  for (int d = 0; d < c->ndir; d++) {
+    ta.Rin = rin; ta.Rout = rout;
+    const int pid = prof_begin(c, FUSS_K_IRS);
+    if (d == 0) {
+      ta.tiles = L.ttx.p;
+      k_tri_x<double><<<(unsigned)L.htx.size(), 64, 0, c->stream>>>(ta);
+    } else if (d == 1) {
+      ta.tiles = L.tty.p;
+      k_tri_yz<1, double><<<(unsigned)L.hty.size(), TILE_T, 0, c->stream>>>(ta);
+    } else {
+      ta.tiles = L.ttz.p;
+      k_tri_yz<2, double><<<(unsigned)L.htz.size(), TILE_T, 0, c->stream>>>(ta);
+    }
+    prof_end(c, pid);
+    HIPCHK(hipGetLastError());
+    std::swap(rin, rout);
+  }
+  ia.Rin = rin;
+  ia.tiles = L.tiles.p;
+  const int pid = prof_begin(c, FUSS_K_IRS_UPDATE);
+  prim ? k_rk_update<true, double><<<(unsigned)L.htiles.size(), TILE_T, 0, c->stream>>>(ia)
+       : k_rk_update<false, double><<<(unsigned)L.htiles.size(), TILE_T, 0, c->stream>>>(ia);
+  prof_end(c, pid);
+  HIPCHK(hipGetLastError());
+  *npart = (int)L.htiles.size();
+  return 0;
+}
+
 const double RKC[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.50, 0.0}, {1.0, 0.25, 2.0 / 3.0}};
 
 // 1/d when d is a (normal) power of two, so that x*(1/d) == x/d for every
@@ -2858,7 +3147,12 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   if (nrk < 1 || nrk > 4) return FUSS_EARG;
   const bool prim = p->integration == FUSS_PRIM;
   const bool irs = p->irs_beta > 0.0;
-  if (irs && p->irs_mode != FUSS_IRS_JACOBI2) return FUSS_EUNSUPPORTED;
+  if (irs && p->irs_mode != FUSS_IRS_JACOBI2 && p->irs_mode != FUSS_IRS_TRIDIAG) return FUSS_EARG;
+  const bool tri = irs && p->irs_mode == FUSS_IRS_TRIDIAG;
+  if (tri) {
+    const int e = tri_prepare(c, L, p->irs_beta);  // no-op once prepared (fuss_run prepares before capture)
+    if (e) return e;
+  }
   const unsigned ntile = (unsigned)L.htiles.size();
   const int* halt = L.cnt.p + 1;
   // Compute_dt + min (Mod_dt.f90:10-84); dtglobal starts at 1e5 (Mod_Explicit.f90:40)
@@ -2933,7 +3227,12 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
       ia.T = in; ia.TO = L.Tb[o].p; ia.Tout = out; ia.mat = L.mat.p; ia.tab = dtab<double>(c);
       ia.rkc = sa.rkc; ia.jam = sa.jam; ia.last = sa.last; ia.partials = L.partials.p; ia.status = c->status.p;
       ia.halt = halt;
-      if (L.no_ycon && L.no_zcon && c->ndir == 3 && L.kp == GC) {
+      if (tri) {
+        int np_ = 0;
+        const int e = launch_irs_tridiag(c, L, ia, prim, &np_);
+        if (e) return e;
+        if (sa.last) { npart = np_; L.plist = 0; }
+      } else if (L.no_ycon && L.no_zcon && c->ndir == 3 && L.kp == GC) {
         // x, y and z in one k-marching pass with the update
         ia.Rin = L.R.p;
         ia.tiles = L.ctiles.p;
@@ -3588,6 +3887,7 @@ namespace {
 // ------------------------------------------------------------------------
 int set_kc(fuss_ctx* c, Level& L, int kc) {
   if (L.kp != GC) return 0;
+  run_graph_free(L);  // its kernels address the old tile list
   L.kc = kc;
   L.hctiles.clear();
   for (int b = 0; b < L.nb; b++) {
@@ -3731,35 +4031,48 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
   // back to their starting roles) captured once and replayed; the stop tests
   // run on the device (k_step_end) and later steps of the chunk become no-ops.
   Level& L = c->lev[level - 1];
-  k_run_begin<<<1, 64, 0, c->stream>>>(L.cnt.p);
+  k_run_begin<<<1, 64, 0, c->stream>>>(L.cnt.p, maxcount);
   HIPCHK(hipGetLastError());
-  int cur_seq[RUN_G + 1];
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int e = 0;
-  {
-    k_chunk_begin<<<1, 64, 0, c->stream>>>(L.cnt.p);
-    cur_seq[0] = L.cur;
-    for (int s = 0; s < RUN_G && !e; s++) {
-      e = enqueue_step(c, level, &q);
-      k_step_end<<<1, 64, 0, c->stream>>>(c->scal.p, L.clk.p, L.cnt.p, c->status.p, c->run_hist.p,
-                                          c->run_hist.p + RUN_G, res_threshold, time_threshold, maxcount);
-      cur_seq[s + 1] = L.cur;
+  if (q.irs_beta > 0.0 && q.irs_mode == FUSS_IRS_TRIDIAG) {
+    const int e0 = tri_prepare(c, L, q.irs_beta);  // uploads: not inside the capture
+    if (e0) return e0;
+  }
+  const bool hit = L.run_exec && L.run_seq[0] == L.cur && !memcmp(&L.run_p, &q, sizeof(q)) &&
+                   L.run_thr == res_threshold && L.run_tthr == time_threshold;
+  int* cur_seq = L.run_seq;
+  if (!hit) {
+    run_graph_free(L);
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int e = 0;
+    {
+      k_chunk_begin<<<1, 64, 0, c->stream>>>(L.cnt.p);
+      cur_seq[0] = L.cur;
+      for (int s = 0; s < RUN_G && !e; s++) {
+        e = enqueue_step(c, level, &q);
+        k_step_end<<<1, 64, 0, c->stream>>>(c->scal.p, L.clk.p, L.cnt.p, c->status.p, c->run_hist.p,
+                                            c->run_hist.p + RUN_G, res_threshold, time_threshold);
+        cur_seq[s + 1] = L.cur;
+      }
     }
-  }
-  const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
-  if (e || ce != hipSuccess) {
-    if (graph) (void)hipGraphDestroy(graph);
+    const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
     L.cur = cur_seq[0];
-    return e ? e : FUSS_EHIP;
+    if (e || ce != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return e ? e : FUSS_EHIP;
+    }
+    if (hipGraphInstantiate(&L.run_exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      L.run_exec = nullptr;
+      return FUSS_EHIP;
+    }
+    L.run_graph = graph;
+    L.run_p = q;
+    L.run_thr = res_threshold;
+    L.run_tthr = time_threshold;
   }
-  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-    (void)hipGraphDestroy(graph);
-    L.cur = cur_seq[0];
-    return FUSS_EHIP;
-  }
-  L.cur = cur_seq[0];
+  hipGraphExec_t exec = L.run_exec;
+  int e = 0;
   int done = 0, halted = 0;
   while (!halted) {
     if (hipGraphLaunch(exec, c->stream) != hipSuccess) { e = FUSS_EHIP; break; }
@@ -3775,8 +4088,6 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
     halted = c->hcnt[1];
     L.cur = cur_seq[halted ? k : RUN_G];
   }
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
   if (e) return e;
   HIPCHK(hipMemcpyAsync(c->hscal, c->scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(c->hcnt, L.cnt.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -3784,7 +4095,7 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
   int bijk[4] = {0, 0, 0, 0};
   const int st = sync_status(c, bijk);
   // leave the halt flag down for the next caller
-  k_run_begin<<<1, 64, 0, c->stream>>>(L.cnt.p);
+  k_run_begin<<<1, 64, 0, c->stream>>>(L.cnt.p, maxcount);
   L.time = c->hscal[6];
   L.iter = c->hcnt[0];
   r.res_sq = c->hscal[2];
@@ -3995,6 +4306,8 @@ void fuss_destroy(fuss_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& L : c->lev) {
+    run_graph_free(L);
+    L.ttx.free(); L.tty.free(); L.ttz.free(); L.tri.free(); L.trioff.free();
     L.blk.free(); L.tiles.free(); L.nodes.free();
     for (auto& t : L.Tb) t.free();
     L.R.free(); L.R2.free(); L.dt.free(); L.qvol.free(); L.vol.free(); L.invL2.free(); L.mat.free();

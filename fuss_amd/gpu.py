@@ -130,9 +130,11 @@ def _c(a, dt=np.float64):
     return np.ascontiguousarray(a, dtype=dt)
 
 
-def step_params(p, compute_residual=True, irs_mode=FUSS_IRS_JACOBI2, n_rk=None):
+def step_params(p, compute_residual=True, irs_mode=None, n_rk=None):
+    """StepParams of the case numerics p; irs_mode None: p.irs_mode."""
+    mode = getattr(p, "irs_mode", FUSS_IRS_JACOBI2) if irs_mode is None else irs_mode
     return StepParams(n_rk or p.n_rk, p.vnn, p.rampa_iter, int(p.time_accurate),
-                      FUSS_PRIM if p.prim else FUSS_CONS, p.irs_beta, irs_mode, int(compute_residual))
+                      FUSS_PRIM if p.prim else FUSS_CONS, p.irs_beta, mode, int(compute_residual))
 
 
 class Solver:
@@ -240,7 +242,7 @@ class Solver:
         return s.value
 
     # -- hot path ----------------------------------------------------------
-    def explicit_step(self, level=None, compute_residual=True, irs_mode=FUSS_IRS_JACOBI2):
+    def explicit_step(self, level=None, compute_residual=True, irs_mode=None):
         """One Explicit_Step; returns the StepResult (res_sq, dtglobal, time, iter)."""
         lv = level or self.level
         sp = step_params(self.p, compute_residual, irs_mode)
@@ -257,7 +259,7 @@ class Solver:
         _chk(self.L.fuss_prolong(self.h, fine_level), "fuss_prolong")
 
     def run_level(self, level, max_iters, itermax_left, res_threshold, time_threshold, hist=True,
-                  irs_mode=FUSS_IRS_JACOBI2):
+                  irs_mode=None):
         sp = step_params(self.p, True, irs_mode)
         n = C.c_int(0)
         hr = np.zeros(max(max_iters, 1)) if hist else None
@@ -271,7 +273,7 @@ class Solver:
         k = n.value
         return k, (hr[:k] if hist else None), (ht[:k] if hist else None), r
 
-    def solve(self, max_iters=10**9, irs_mode=FUSS_IRS_JACOBI2):
+    def solve(self, max_iters=10**9, irs_mode=None):
         """FUSS_solve loop to the end of the run: coarse-to-fine sequencing,
         then level 1 until a stop condition.  Returns (n_iter, history) with
         history rows (iter_general, time, residual)."""

@@ -85,6 +85,7 @@ struct ob_ctx {
   double *cp, *rho, *kap, *h;
   int ndir;
   int n_rk, rampa, time_accurate, prim, irs;
+  int irs_mode;   /* 0: two Jacobi sweeps (reference), 1: exact line-tridiagonal solve */
   double vnn, beta;
   int nseries;
   ob_series *series;
@@ -842,11 +843,63 @@ static void irs_blk(ob_blk *b, int dir, double beta) {
       for (int i = 1; i <= n1; i++) b->R[CX(b, i, j, k)] = b->RS1[CX(b, i, j, k)];
 }
 
+/* Line-tridiagonal IRS (no reference counterpart): the fixed point of the
+ * Jacobi iteration of Residual_Smoothing_Blk, i.e. the exact solution of
+ *   (1+2b) x_s - b (x_{s-1} + x_{s+1}) = r_s,  s = 1..n,
+ * with x_0, x_{n+1} the IRS ghosts of Ghost_Residual_Extrapolation held
+ * fixed, by the Thomas algorithm in this order:
+ *   den_1 = 1+2b, c_1 = -b/den_1, d_1 = (r_1 + b x_0 [+ b x_{n+1} if n=1]) / den_1
+ *   den_s = (1+2b) + b c_{s-1}, c_s = -b/den_s,
+ *   d_s = (r_s [+ b x_{n+1} if s=n] + b d_{s-1}) / den_s
+ *   x_n = d_n, x_s = d_s - c_s x_{s+1}. */
+static void irs_blk_tridiag(ob_blk *b, int dir, double beta) {
+  int n1 = b->n[0], n2 = b->n[1], n3 = b->n[2];
+  int di = dir == 1, dj = dir == 2, dk = dir == 3;
+  int n = dir == 1 ? n1 : (dir == 2 ? n2 : n3);
+  const double bb = 1.0 + 2.0 * beta;
+  double *cp = (double *)malloc(sizeof(double) * (n + 1));
+  double *dp = (double *)malloc(sizeof(double) * (n + 1));
+  /* lines: every (i,j,k) with the dir index 1 */
+  for (int k = 1; k <= (dk ? 1 : n3); k++)
+    for (int j = 1; j <= (dj ? 1 : n2); j++)
+      for (int i = 1; i <= (di ? 1 : n1); i++) {
+        const double glo = b->R[CX(b, i - di, j - dj, k - dk)];
+        const double ghi = b->R[CX(b, i + di * n, j + dj * n, k + dk * n)];
+        for (int s = 1; s <= n; s++) {
+          double r = b->R[CX(b, i + di * (s - 1), j + dj * (s - 1), k + dk * (s - 1))];
+          double den;
+          if (s == 1) {
+            den = bb;
+            r = r + beta * glo;
+          } else {
+            den = bb + beta * cp[s - 1];
+          }
+          if (s == n) r = r + beta * ghi;
+          cp[s] = -beta / den;
+          dp[s] = s == 1 ? r / den : (r + beta * dp[s - 1]) / den;
+        }
+        double x = dp[n];
+        b->RS1[CX(b, i + di * (n - 1), j + dj * (n - 1), k + dk * (n - 1))] = x;
+        for (int s = n - 1; s >= 1; s--) {
+          x = dp[s] - cp[s] * x;
+          b->RS1[CX(b, i + di * (s - 1), j + dj * (s - 1), k + dk * (s - 1))] = x;
+        }
+      }
+  for (int k = 1; k <= n3; k++)
+    for (int j = 1; j <= n2; j++)
+      for (int i = 1; i <= n1; i++) b->R[CX(b, i, j, k)] = b->RS1[CX(b, i, j, k)];
+  free(cp);
+  free(dp);
+}
+
 static void residual_smoothing(ob_ctx *c, ob_dom *d) {
   /* Residual_Smoothing, Lib_IRS.f90:14-40 */
   for (int dir = 1; dir <= c->ndir; dir++) {
     irs_ghost(d);
-    for (int bb = 0; bb < d->nb; bb++) irs_blk(&d->blk[bb], dir, c->beta);
+    for (int bb = 0; bb < d->nb; bb++) {
+      if (c->irs_mode == 1) irs_blk_tridiag(&d->blk[bb], dir, c->beta);
+      else irs_blk(&d->blk[bb], dir, c->beta);
+    }
   }
 }
 
@@ -1242,6 +1295,12 @@ int ob_set_params(ob_ctx *c, int n_rk, double vnn, int rampa_iter, int time_accu
   if (!c || n_rk < 1 || n_rk > 4) return OB_ERR_ARG;
   c->n_rk = n_rk; c->vnn = vnn; c->rampa = rampa_iter; c->time_accurate = time_accurate;
   c->prim = prim; c->beta = irs_beta; c->irs = irs_beta > 0.0;
+  return 0;
+}
+
+int ob_set_irs_mode(ob_ctx *c, int mode) {
+  if (!c || mode < 0 || mode > 1) return OB_ERR_ARG;
+  c->irs_mode = mode;
   return 0;
 }
 

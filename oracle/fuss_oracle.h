@@ -64,6 +64,11 @@ int ob_set_bc(ob_ctx *c, int level, int nbound, const int *ib, const double *rb,
 int ob_set_params(ob_ctx *c, int n_rk, double vnn, int rampa_iter,
                   int time_accurate, int prim, double irs_beta);
 
+/* IRS flavour: 0 the reference's two Jacobi sweeps per direction, 1 the
+ * exact line-tridiagonal solve with the same ghosts (no reference
+ * counterpart). */
+int ob_set_irs_mode(ob_ctx *c, int mode);
+
 /* Coarse levels, metrics, matIDg, initial ghost fill and restriction, as
  * FUSS_setup does.  newrun_time: initial time (0 transient, -1 steady). */
 int ob_setup(ob_ctx *c, double initial_time);

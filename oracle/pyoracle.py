@@ -37,6 +37,7 @@ def lib():
         L.ob_add_series.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
         L.ob_set_bc.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _dp, _ip]
         L.ob_set_params.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.ob_set_irs_mode.argtypes = [C.c_void_p, C.c_int]
         L.ob_setup.argtypes = [C.c_void_p, C.c_double]
         L.ob_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.ob_restrict.argtypes = [C.c_void_p, C.c_int]
@@ -91,6 +92,7 @@ class Oracle:
             self._chk(self.L.ob_set_bc(self.h, m + 1, bcl.n, _c(bcl.ib, np.int32), _c(bcl.rb), _c(sid, np.int32)))
         self._chk(self.L.ob_set_params(self.h, p.n_rk, p.vnn, p.rampa_iter, int(p.time_accurate), int(p.prim),
                                        p.irs_beta))
+        self._chk(self.L.ob_set_irs_mode(self.h, int(getattr(p, "irs_mode", 0))))
         self._chk(self.L.ob_setup(self.h, 0.0 if p.time_accurate else -1.0))
 
     def _chk(self, st):

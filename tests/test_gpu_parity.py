@@ -88,6 +88,14 @@ def test_steps_vs_oracle(gpu, oracle_mod, name):
     steps_vs_oracle(gpu, oracle_mod, c)
 
 
+def tridiag(c, beta=None):
+    """The case with the line-tridiagonal IRS (and IRS on, beta given)."""
+    if beta is not None:
+        c.params.irs_beta = beta
+    c.params.irs_mode = 1
+    return c
+
+
 SYNTH = {
     # 3-D orthogonal stretched cube: 303 walls, qvol, k(T) table, IRS, 3 levels
     "cube3d_mg3_irs": lambda: cases.cube3d(24, mgl=3),
@@ -105,6 +113,15 @@ SYNTH = {
     # lattice in conservative variables
     "cube3d_rk4_irs": lambda: cases.cube3d(24, mgl=1, n_rk=4),
     "lattice_rk4_cons": lambda: cases.lattice(nl=2, lo=4, hi=8, n_rk=4, prim=False),
+    # line-tridiagonal IRS extension (FUSS_IRS_TRIDIAG): single block, the
+    # connected lattice (donor ghosts between directions, 2 levels), uneven
+    # line lengths (not multiples of the 16-cell chunks), and 2-D
+    "cube3d_tridiag": lambda: tridiag(cases.cube3d(24, mgl=1, n_rk=4)),
+    "lattice_tridiag_mg2": lambda: tridiag(cases.lattice(nl=2, lo=4, hi=8, prim=False, irs_beta=0.5,
+                                                         time_accurate=False, mgl=2)),
+    "lattice_tridiag_mg3": lambda: tridiag(cases.lattice(nl=2, lo=8, hi=20, irs_beta=0.5, time_accurate=False,
+                                                         mgl=3)),
+    "plate2d_tridiag": lambda: tridiag(cases.plate2d(40), beta=0.6),
 }
 
 
@@ -190,6 +207,20 @@ def test_graph_run_equals_step_loop(gpu, name):
     assert k1 == k2 and c1 == c2
     assert np.array_equal(hr1, hr2) and np.array_equal(ht1, ht2)
     assert np.array_equal(T1, T2)
+
+
+def test_graph_run_tridiag(gpu):
+    """fuss_run with the tridiagonal IRS on captured graphs (its tables are
+    uploaded before the capture) equals the step loop bit for bit."""
+    out = []
+    for graphs in (True, False):
+        s = gpu.Solver(SYNTH["lattice_tridiag_mg3"]())
+        s.graphs(graphs)
+        k, hr, _, _ = s.run_level(3, 60, 60, -1.0, 1e300)
+        out.append((k, hr.copy(), np.concatenate([T.ravel() for T in s.T(3)])))
+        s.close()
+    assert out[0][0] == out[1][0] == 60
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
 
 
 @pytest.mark.parametrize("name", ["cube3d_rk4_irs", "lattice_transient"])

@@ -17,6 +17,9 @@ order) would break:
 * IRS equals two textbook Jacobi iterations of the tridiagonal system
   (1+2b) r*_i - b (r*_{i-1} + r*_{i+1}) = r_i along x then y, with
   zero-gradient wall ghosts (Lib_IRS.f90:43-187);
+* the line-tridiagonal IRS (an extension) solves that system exactly along x
+  then y with the same ghosts: it equals scipy's banded solver, and it is the
+  fixed point that repeated Jacobi sweeps converge to;
 * prolongation reproduces constant and (away from the clamped edges) bilinear
   fields; restriction conserves sum(h V) (Lib_Multigrid.f90:38-220).
 """
@@ -153,6 +156,55 @@ def test_irs_two_jacobi_sweeps(oracle_mod):
     s1x = np.stack([jacobi1_line(res[j], res[j, 0], res[j, -1]) for j in range(m)])
     s1 = np.stack([jacobi1_line(s1x[:, i], s1x[0, i], s1x[-1, i]) for i in range(n)], axis=1)
     assert np.max(np.abs(s1 - sy)) > 1e-3 * np.max(np.abs(sy))
+
+
+def tridiag_line(r, beta, glo, ghi):
+    """(1+2b) x_s - b (x_{s-1} + x_{s+1}) = r_s with x_0 = glo, x_{n+1} = ghi fixed."""
+    from scipy.linalg import solve_banded
+
+    n = r.size
+    ab = np.zeros((3, n))
+    ab[0, 1:] = -beta
+    ab[1, :] = 1 + 2 * beta
+    ab[2, :-1] = -beta
+    rhs = r.astype(np.float64).copy()
+    rhs[0] += beta * glo
+    rhs[-1] += beta * ghi
+    return solve_banded((1, 1), ab, rhs)
+
+
+def test_irs_tridiagonal_lines(oracle_mod):
+    rng = np.random.Generator(np.random.PCG64(5))
+    n, m, beta = 9, 6, 0.5
+    T0 = 400.0 + 20.0 * rng.uniform(-1, 1, size=(1, m, n))
+    base = dict(n=n, m=m, dx=0.1, dy=0.1, T=T0, n_rk=1, vnn=0.3)
+    faces = [(301, cases.R())] * 4 + [(0, cases.R())] * 2
+    o0 = oracle_mod.Oracle(plate(faces=faces, **base))
+    o0.step(1)
+    res = o0.get("T")[0] - T0[0]
+    c = plate(faces=faces, irs_beta=beta, **base)
+    c.params.irs_mode = 1
+    o1 = oracle_mod.Oracle(c)
+    o1.step(1)
+    smoothed = o1.get("T")[0] - T0[0]
+    sx = np.stack([tridiag_line(res[j], beta, res[j, 0], res[j, -1]) for j in range(m)])
+    sy = np.stack([tridiag_line(sx[:, i], beta, sx[0, i], sx[-1, i]) for i in range(n)], axis=1)
+    np.testing.assert_allclose(smoothed, sy, rtol=1e-10, atol=1e-12)
+    # the Jacobi iteration of the reference converges to the same lines
+    def jacobi_inf(r, glo, ghi):
+        x = r.copy()
+        for _ in range(400):
+            xx = np.concatenate([[glo], x, [ghi]])
+            x = (r + beta * (xx[:-2] + xx[2:])) / (1 + 2 * beta)
+        return x
+
+    jx = np.stack([jacobi_inf(res[j], res[j, 0], res[j, -1]) for j in range(m)])
+    jy = np.stack([jacobi_inf(jx[:, i], jx[0, i], jx[-1, i]) for i in range(n)], axis=1)
+    np.testing.assert_allclose(smoothed, jy, rtol=1e-9, atol=1e-11)
+    # and it is not the two-sweep operator
+    sx2 = np.stack([jacobi2_line(res[j], beta, res[j, 0], res[j, -1]) for j in range(m)])
+    sy2 = np.stack([jacobi2_line(sx2[:, i], beta, sx2[0, i], sx2[-1, i]) for i in range(n)], axis=1)
+    assert np.max(np.abs(sy2 - sy)) > 1e-3 * np.max(np.abs(sy))
 
 
 def test_prolongation_and_restriction(oracle_mod):
