@@ -44,7 +44,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the stage kernel measured with rocprofv3 PMC
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB device copy);
 # source: profiles/explicit_step_cube256_fp64.md (256^3 cube, FP64).
-STAGE_TRAFFIC_MEASURED = {"k_stage3d<false, true, true, double>": 1023e6 + 142e6}
+STAGE_KERNEL = "k_stage3p<true, true, true, double, 16>"
+STAGE_TRAFFIC_MEASURED = {STAGE_KERNEL: 482501 * 1024 * 2.0 + 131072 * 1024 * 1.0}
 # Plain FP64 path (four-stage RK + LTS + IRS 0.5, 3-level sequencing) run to
 # residual 1e-10 on the GPU: iterations per cube size (profiles/
 # steady_convergence_cube.md).  Fit: iters = 230218 (n/64)^PLAIN_EXP.
@@ -366,7 +367,8 @@ def main():
           flush=True)
     names = ["dt", "ghost", "stage", "irs", "irs_update", "reduce", "bc"]
     kern = {names[q]: {"ms_avg": ms[q] / cnt[q], "launches": cnt[q]} for q in range(7) if cnt[q]}
-    stage_b = perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True) * ncell
+    # one material: the stage kernel reads no material stream
+    stage_b = perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False) * ncell
     stage_ms = ms[2] / cnt[2]
     achieved = stage_b / (stage_ms * 1e-3) / 1e9
     out = {
@@ -390,12 +392,15 @@ def main():
                              "chosen_by": "fuss_autotune"}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": STAGE_TRAFFIC_MEASURED["k_stage3d<false, true, true, double>"] if args.n == 256 else None,
-                     "kernel": "k_stage (fused interior+BC fluxes, scaled residual)",
-                     "bytes_per_cell": perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True),
+                     "traffic": (STAGE_TRAFFIC_MEASURED[STAGE_KERNEL]
+                                 if args.n == 256 and tuned[0] == 16 and not tuned[1] else None),
+                     "kernel": (STAGE_KERNEL if tuned[0] == 16 and not tuned[1] else "stage kernel")
+                               + " (fused interior + BC fluxes, scaled residual)",
+                     "bytes_per_cell": perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False),
                      "ms_avg": stage_ms},
         "kernels": kern,
-        "iteration_bytes_per_cell": perfmodel.iteration_bytes(args.rk, 3, False, True, True),
+        "iteration_bytes_per_cell": perfmodel.iteration_bytes(args.rk, 3, False, True, True, mat=False,
+                                                             fused_irs=True),
     }
     ms_step = wall / args.steps * 1e3
     if not args.no_refine:

@@ -216,7 +216,7 @@ __global__ __launch_bounds__(TILE_T) void k_dt(DtArgs<R> a) {
   if (i <= B.ni && j <= B.nj) {
     const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
     const R T = a.T[p];
-    const int m = a.mat[p];
+    const int m = B.umat >= 0 ? B.umat : a.mat[p];
     int bad = 0;
     const long row = (long)m * a.tab.nT;
     const R kap = tab_interp(a.tab.kap + row, T, a.tab.Ti1, a.tab.Ti2, &bad);

@@ -5,19 +5,21 @@ precision it is stored in; neighbour re-reads of the stencil and of the
 face weights are excluded (they are served from cache/LDS), as are the
 surface-only BC records.
 
-FP64 storage: T, dt, qvol, vol, weights, R: 8 B; material index: 4 B.
+FP64 storage: T, dt, qvol, vol, weights, R: 8 B; material index: 4 B.  A
+level whose blocks each hold one material reads no material stream
+(mat=False): the index is a per-block constant.
 """
 from __future__ import annotations
 
 
-def stage_bytes(ndir=3, tang=False, irs=True, prim=True, fp=8):
+def stage_bytes(ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True):
     """Fused flux + BC + newstate kernel, per cell.
 
     reads: T, dt, qvol, vol, material, own face weights (ndir normal weights,
     or 3 per face when tangential weights are present); writes the scaled
     residual (IRS on) or reads T^n and writes T (IRS off)."""
     nw = ndir * (3 if tang else 1)
-    b = fp * (1 + 1 + 1 + 1 + nw) + 4
+    b = fp * (1 + 1 + 1 + 1 + nw) + (4 if mat else 0)
     if irs:
         b += fp            # write R
     else:
@@ -25,24 +27,33 @@ def stage_bytes(ndir=3, tang=False, irs=True, prim=True, fp=8):
     return b
 
 
-def irs_bytes(update=False, fp=8):
+def irs_bytes(update=False, fp=8, mat=True):
     """One IRS line pass: read R, write R; the last pass instead applies the
     RK update: read R, T, T^n, material, write T."""
     if update:
-        return fp * 4 + 4
+        return fp * 4 + (4 if mat else 0)
     return 2 * fp
 
 
-def dt_bytes(fp=8):
+def irs_fused_bytes(fp=8, mat=True):
+    """The x-y-z IRS pass with the update (levels without j/k connections):
+    read R, T, T^n (and the material), write T."""
+    return fp * 4 + (4 if mat else 0)
+
+
+def dt_bytes(fp=8, mat=True):
     """Local dt: read T, material, sum 1/dl^2, write dt."""
-    return fp + 4 + 8 + fp
+    return fp + (4 if mat else 0) + 8 + fp
 
 
-def iteration_bytes(n_rk=3, ndir=3, tang=False, irs=True, prim=True, fp=8):
-    b = dt_bytes(fp)
-    per_stage = stage_bytes(ndir, tang, irs, prim, fp)
+def iteration_bytes(n_rk=3, ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True, fused_irs=False):
+    b = dt_bytes(fp, mat)
+    per_stage = stage_bytes(ndir, tang, irs, prim, fp, mat)
     if irs:
-        per_stage += (ndir - 1) * irs_bytes(False, fp) + irs_bytes(True, fp)
+        if fused_irs:
+            per_stage += irs_fused_bytes(fp, mat)
+        else:
+            per_stage += (ndir - 1) * irs_bytes(False, fp) + irs_bytes(True, fp, mat)
     return b + n_rk * per_stage
 
 
