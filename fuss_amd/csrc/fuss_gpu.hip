@@ -2122,6 +2122,7 @@ struct Level {
   DBuf<double2> tri;
   DBuf<int> trioff;
   double tri_beta = -1.0;
+  int ghost_read = 1;               // some BC entry reads layer-1 ghosts (101, 300, 302)
   int all_umat = 0;                 // every block has a single material
   int dims2 = 0;                    // every block has >= 2 cells in each direction
   long npad = 0, ncell = 0, nnodes = 0;
@@ -2438,6 +2439,7 @@ int build_bc(fuss_ctx* c, Level& L) {
   L.no_ycon = 1;
   L.no_zcon = 1;
   L.no_xcon = 1;
+  L.ghost_read = 0;
   // grouping check: (block, face) non-decreasing, as BC_Fluxes assumes
   // (Mod_BC_Fluxes.f90:26-33)
   for (int e = 1; e < n; e++) {
@@ -2513,6 +2515,7 @@ int build_bc(fuss_ctx* c, Level& L) {
     if (type == 101 && (f == 3 || f == 4)) L.no_ycon = 0;
     if (type == 101 && (f == 5 || f == 6)) L.no_zcon = 0;
     if (type == 101 && (f == 1 || f == 2)) L.no_xcon = 0;
+    if (type == 101 || type == 300 || type == 302) L.ghost_read = 1;
     const bool active = type == 101 || (type >= 300 && type <= 305);
     if (active) B.face_active |= 1 << (f - 1);
     if (type == 101) B.conn |= 1 << (f - 1);
@@ -3181,7 +3184,9 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   L.plist = 0;
   for (int irk = 1; irk <= nrk; irk++) {
     double* out = (irk % 2 == 1) ? L.Tb[a1].p : L.Tb[a2].p;
-    int e = launch_ghost(c, L, in);
+    // Fill_Ghost_Cell: only read by tangential gradients (non-orthogonal
+    // blocks; 300/302 walls) and connections; a level with neither skips it
+    int e = (L.tang || L.ghost_read || c->adj_store_on) ? launch_ghost(c, L, in) : 0;
     if (e) return e;
     if (c->adj_store_on)
       HIPCHK(hipMemcpyAsync(c->adj.stage[irk - 1].p, in, L.npad * sizeof(double), hipMemcpyDeviceToDevice,
