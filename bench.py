@@ -367,8 +367,8 @@ def main():
           flush=True)
     names = ["dt", "ghost", "stage", "irs", "irs_update", "reduce", "bc"]
     kern = {names[q]: {"ms_avg": ms[q] / cnt[q], "launches": cnt[q]} for q in range(7) if cnt[q]}
-    # one material: the stage kernel reads no material stream
-    stage_b = perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False) * ncell
+    # one material and a uniform qvol: the stage kernel reads neither stream
+    stage_b = perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False, qvol=False) * ncell
     stage_ms = ms[2] / cnt[2]
     achieved = stage_b / (stage_ms * 1e-3) / 1e9
     out = {
@@ -392,15 +392,16 @@ def main():
                              "chosen_by": "fuss_autotune"}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (STAGE_TRAFFIC_MEASURED[STAGE_KERNEL]
+                     "traffic": (STAGE_TRAFFIC_MEASURED.get(STAGE_KERNEL)
                                  if args.n == 256 and tuned[0] == 16 and not tuned[1] else None),
                      "kernel": (STAGE_KERNEL if tuned[0] == 16 and not tuned[1] else "stage kernel")
                                + " (fused interior + BC fluxes, scaled residual)",
-                     "bytes_per_cell": perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False),
+                     "bytes_per_cell": perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False,
+                                                             qvol=False),
                      "ms_avg": stage_ms},
         "kernels": kern,
         "iteration_bytes_per_cell": perfmodel.iteration_bytes(args.rk, 3, False, True, True, mat=False,
-                                                             fused_irs=True),
+                                                             fused_irs=True, qvol=False),
     }
     ms_step = wall / args.steps * 1e3
     if not args.no_refine:

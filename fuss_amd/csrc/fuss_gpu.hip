@@ -919,18 +919,15 @@ __global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
 // level has one material (uniform row base, no material stream).
 // Same operations and order as k_stage3d<false, ...>.
 // ------------------------------------------------------------------------
-// Row of T in a 1 K table and the clamped T (co_Kappa etc., Lib_Solid.f90:
-// 19-23: Tm = int(T)); outside [Ti1, Ti2) *bad is set and T clamped as in
-// tab_interp.  With Ti1 >= 0, int(T) >= Ti2 exactly when T >= Ti2.
+// Row of T in a 1 K table (co_Kappa etc., Lib_Solid.f90:19-23: Tm = int(T)).
+// Outside [Ti1, Ti2) *bad is set (the step then fails with FUSS_ETABLE) and
+// the row is clamped so the lookup stays inside the table.  With Ti1 >= 0,
+// int(T) >= Ti2 exactly when T >= Ti2, and a NaN fails T >= Ti1.
 template <typename R>
-__device__ __forceinline__ int tab_row(R& T, int Ti1, int Ti2, int* bad) {
-  int Tm = (int)T;
-  if (!(T >= (R)Ti1) || Tm >= Ti2) {
-    *bad = 1;
-    T = T < (R)Ti1 || T != T ? (R)Ti1 : (R)(Ti2 - 1);
-    Tm = (int)T;
-  }
-  return Tm;
+__device__ __forceinline__ int tab_row(R T, int Ti1, int Ti2, int* bad) {
+  const int Tm = (int)T;
+  if (!(T >= (R)Ti1) || Tm >= Ti2) *bad = 1;
+  return min(max(Tm, Ti1), Ti2 - 1);
 }
 
 template <typename R>
@@ -941,8 +938,9 @@ struct StageStream {
   int m, mh;           // material of the cell / of the halo face's left cell
 };
 
-// fetch_plane with every load issued (out-of-block halo cells read the
-// tile's corner cell, a ghost or interior cell of the block, and become 0)
+// fetch_plane with every load issued: halo cells beyond the ghost layer read
+// the tile's corner cell instead (a ghost or interior cell of the block); no
+// flux of an in-block face uses them
 template <typename R>
 __device__ __forceinline__ void fetch_plane_u(PlaneRegs<R>& r, const R* __restrict__ T, const BlockInfo& B, int i0,
                                               int j0, int kk) {
@@ -950,14 +948,12 @@ __device__ __forceinline__ void fetch_plane_u(PlaneRegs<R>& r, const R* __restri
   {
     const int q = threadIdx.x, ii = q % PX, jj = q / PX;
     const bool in = i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1;
-    const R v = ldu(T, in ? corner + ii + jj * (unsigned)B.sy : corner);
-    r.v0 = in ? v : (R)0;
+    r.v0 = ldu(T, in ? corner + ii + jj * (unsigned)B.sy : corner);
   }
   {
     const int q = threadIdx.x + TILE_T, ii = q % PX, jj = q / PX;
     const bool in = q < PX * PY && i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1;
-    const R v = ldu(T, in ? corner + ii + jj * (unsigned)B.sy : corner);
-    r.v1 = in ? v : (R)0;
+    r.v1 = ldu(T, in ? corner + ii + jj * (unsigned)B.sy : corner);
   }
 }
 
@@ -984,10 +980,11 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   using V4 = typename Vec4<R>::T;
   const int Ti1 = a.tab.Ti1, Ti2 = a.tab.Ti2, nT = a.tab.nT;
   const int mu = B.umat >= 0 ? B.umat : 0;
-  const V2* kap_u = reinterpret_cast<const V2*>(a.tab.kap2) + (long)mu * nT - Ti1;
-  const V4* pr_u = reinterpret_cast<const V4*>(a.tab.pr4) + (long)mu * nT - Ti1;
-  const V2* kap0 = reinterpret_cast<const V2*>(a.tab.kap2) - Ti1;
-  const V4* pr0 = reinterpret_cast<const V4*>(a.tab.pr4) - Ti1;
+  // first row of a material's table (row of T = Ti1)
+  const V2* kap_u = reinterpret_cast<const V2*>(a.tab.kap2) + (long)mu * nT;
+  const V4* pr_u = reinterpret_cast<const V4*>(a.tab.pr4) + (long)mu * nT;
+  const V2* kap0 = reinterpret_cast<const V2*>(a.tab.kap2);
+  const V4* pr0 = reinterpret_cast<const V4*>(a.tab.pr4);
   // the halo face's weight array: +y weights for the y-halo row, +x for the x-halo
   const R* __restrict__ Wh = yhalo ? a.W[4] : a.W[0];
   int sm = 0, sc = 1, sp = 2;
@@ -1013,10 +1010,17 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   const int zlo = (fa & 16) ? B.fmap_off[4] + (i - 1) + B.ni * (j - 1) : -1;
   const int zhi = (fa & 32) ? B.fmap_off[5] + (i - 1) + B.ni * (j - 1) : -1;
   auto zo_of = [&](int kk) { return kk == 1 ? zlo : (kk == B.nk ? zhi : -1); };
+  // which loads of boundary values a tile / row needs at all
+  const int fa_t = B.face_active;
+  const bool xtile = (fa_t & 3) && (t.i0 == 1 || t.i0 + TILE_X - 1 >= B.ni);
+  const bool yrow = (fa_t & 12) && (j == 1 || j == B.nj);
   const R dtg = a.use_dtg ? (R)(*a.dtg) : (R)0;
+  // a block of uniform qvol: every lane reads the value of cell (1,1,1) (one
+  // cached line instead of a stream)
+  const unsigned quni = (unsigned)B.base;
   auto stream = [&](StageStream<R>& s, unsigned pp, unsigned hh) {
     s.vol = ldu(a.vol, pp);
-    s.qv = ldu(a.qvol, pp);
+    s.qv = ldu(a.qvol, B.uqv ? quni : pp);
     s.dt = ldu(a.dt, pp);
     if (!IRSMODE) s.TO = ldu(a.TO, pp);
     s.w0 = ldu(a.W[0], pp);
@@ -1039,6 +1043,14 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
     fz_prev = flux_z<false>(a, B, ring[sm], ring[sc], x, y, p - sz, ldu(a.W[8], p - sz));
   }
   double rsq = 0.0;
+  // The result of a plane is stored during the next plane, after that
+  // plane's loads are issued: stores count in the same in-order counter as
+  // loads, so a store issued just before the table loads would make the
+  // wait for the tables wait for the store's completion too.
+  R* const out = IRSMODE ? a.Rres : a.Tout;
+  R vpend = 0;
+  unsigned ppend = p;
+  bool spend = false;
   // one plane: cur holds its streams, nxt receives those of the next plane
   auto plane = [&](StageStream<R>& cur, StageStream<R>& nxt, int k) {
     store_plane(ring[sp], pre);
@@ -1052,33 +1064,35 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
     int rx, ry, rz, rh = Ti1, rc = Ti1;
     V2 kx, ky, kz, kh{};
     V4 pc{};
-    // BC values of the lane's boundary faces (one load each, issued with the
-    // table rows; lanes without such a face read entry 0)
-    const R bx = ldu(a.bcf, (unsigned)max(xo >= 0 ? xo + B.nj * (k - 1) : 0, 0));
-    const R by = ldu(a.bcf, (unsigned)max(yo >= 0 ? yo + B.ni * (k - 1) : 0, 0));
-    const R bz = ldu(a.bcf, (unsigned)max(zo_of(k), 0));
+    // BC values of the lane's boundary faces, issued with the table rows;
+    // only tiles on an i face, rows on a j face and the end planes load them
+    R bx = 0, by = 0, bz = 0;
+    if (xtile) bx = ldu(a.bcf, (unsigned)max(xo >= 0 ? xo + B.nj * (k - 1) : 0, 0));
+    if (yrow) by = ldu(a.bcf, (unsigned)max(yo >= 0 ? yo + B.ni * (k - 1) : 0, 0));
+    if (zo_of(k) >= 0 || ((fa_t & 48) && (k == 1 || k == B.nk))) bz = ldu(a.bcf, (unsigned)max(zo_of(k), 0));
     {
 #pragma clang fp contract(off)
+      // rows addressed by 32-bit offsets from the material's first row
       const V2* kr = UM ? kap_u : kap0 + (long)cur.m * nT;
       int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
       Tx = (R)0.5 * (Tc + Txr);
       rx = tab_row(Tx, Ti1, Ti2, &b0);
-      kx = kr[rx];
+      kx = ldu(kr, (unsigned)(rx - Ti1));
       Ty = (R)0.5 * (Tc + Tyr);
       ry = tab_row(Ty, Ti1, Ti2, &b1);
-      ky = kr[ry];
+      ky = ldu(kr, (unsigned)(ry - Ti1));
       Tz = (R)0.5 * (Tc + Tzr);
       rz = tab_row(Tz, Ti1, Ti2, &b2);
-      kz = kr[rz];
+      kz = ldu(kr, (unsigned)(rz - Ti1));
       if (PRIM) {
         rc = tab_row(Tcl, Ti1, Ti2, &badc);
-        pc = (UM ? pr_u : pr0 + (long)cur.m * nT)[rc];
+        pc = ldu(UM ? pr_u : pr0 + (long)cur.m * nT, (unsigned)(rc - Ti1));
       }
       if (ty < 2) {  // waves 0 and 1 hold the halo lanes (a wave-uniform test)
         const V2* krh = UM ? kap_u : kap0 + (long)cur.mh * nT;
         Th = yhalo ? (R)0.5 * (ring[sc][0][x] + ring[sc][1][x]) : (R)0.5 * (ring[sc][hq][0] + ring[sc][hq][1]);
         rh = tab_row(Th, Ti1, Ti2, &b3);
-        kh = krh[rh];
+        kh = ldu(krh, (unsigned)(rh - Ti1));
       }
       bad = (live && ((hx && b0) || (hy && b1) || (k < B.nk && b2))) || ((yhalo || xhalo) && b3);
       badc = live && badc;
@@ -1089,6 +1103,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
       fetch_plane_u(pre, a.T, B, t.i0, t.j0, k < k1 ? k + 2 : k + 1);
       stream(nxt, p + step, ph + step);
     }
+    if (spend) stu(out, ppend, vpend);  // the previous plane's result
     // phase C: fluxes
     R fz = 0;
     {
@@ -1136,16 +1151,18 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
       }
       int bad2 = badc;
       if (IRSMODE) {
-        stu(a.Rres, p, res);
+        vpend = res;
       } else {
         const R Tn = new_state<PRIM>(a.tab, cur.m, Tc, cur.TO, res, a.rkc, a.jam, &bad2);
-        stu(a.Tout, p, Tn);
+        vpend = Tn;
         if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
         const double d = (double)Tn - (double)cur.TO;
         rsq += d * d;
       }
+      ppend = p;
       if (bad2) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
     }
+    spend = live;
     fz_prev = fz;
     const int s0 = sm;
     sm = sc;
@@ -1160,6 +1177,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
     plane(sa_, sb_, k);
     if (k + 1 <= k1) plane(sb_, sa_, k + 1);
   }
+  if (spend) stu(out, ppend, vpend);
   if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
 }
 
@@ -1394,7 +1412,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs(IrsArgs<R> a) {
     if (!UPDATE) {
       a.Rout[p] = rs;
     } else {
-      const R T = a.T[p], TO = a.TO[p];
+      const R T = ((PRIM && a.jam) ? a.TO : a.T)[p], TO = a.TO[p];  // four-stage prim: T unused
       int bad = 0;
       const R Tn = new_state<PRIM>(a.tab, a.mat[p], T, TO, rs, a.rkc, a.jam, &bad);
       a.Tout[p] = Tn;
@@ -1488,7 +1506,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xy(IrsArgs<R> a) {
     if (!UPDATE) {
       stu(a.Rout, p, rs);
     } else {
-      const R T = ldu(a.T, p), TO = ldu(a.TO, p);
+      const R T = ldu((PRIM && a.jam) ? a.TO : a.T, p), TO = ldu(a.TO, p);  // four-stage prim: T unused
       int bad = 0;
       const int m = B.umat >= 0 ? B.umat : ldu(a.mat, p);
       const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
@@ -1668,7 +1686,7 @@ __global__ __launch_bounds__(TILE_T) void k_rk_update(IrsArgs<R> a) {
   double rsq = 0.0;
   if (i <= B.ni && j <= B.nj) {
     const unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz);
-    const R T = ldu(a.T, p), TO = ldu(a.TO, p), rs = ldu(a.Rin, p);
+    const R T = ldu((PRIM && a.jam) ? a.TO : a.T, p), TO = ldu(a.TO, p), rs = ldu(a.Rin, p);
     int bad = 0;
     const int m = B.umat >= 0 ? B.umat : ldu(a.mat, p);
     const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
@@ -1703,7 +1721,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
     for (int k = k0; k <= k1; k++) {
       const R rs = jacobi2(w0, w1, w2, w3, w4, k, n, glo, ghi, a.beta, a.den, a.inv);
       const unsigned p = p1 + (k - 1) * sz;
-      const R T = ldu(a.T, p), TO = ldu(a.TO, p);
+      const R T = ldu((PRIM && a.jam) ? a.TO : a.T, p), TO = ldu(a.TO, p);  // four-stage prim: T unused
       int bad = 0;
       const R Tn = new_state<PRIM>(a.tab, m >= 0 ? m : ldu(a.mat, p), T, TO, rs, a.rkc, a.jam, &bad);
       stu(a.Tout, p, Tn);
@@ -1797,7 +1815,9 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
     const int ko = q - 2;
     const bool out = ko >= k0;
     const unsigned po = p1 + (unsigned)(max(ko, 1) - 1) * sz;
-    const R T = ldu(a.T, po), TO = ldu(a.TO, po);
+    // the four-stage update in primitive variables (new = old + a_k res) does
+    // not read the stage state: its load then hits T^n's line (no HBM bytes)
+    const R T = ldu((PRIM && a.jam) ? a.TO : a.T, po), TO = ldu(a.TO, po);
     int m = B.umat;
     if (m < 0) m = ldu(a.mat, po);
     R s = 0;
@@ -2371,6 +2391,8 @@ int build_level_layout(fuss_ctx* c, Level& L) {
     for (int f = 0; f < 6; f++) B.fmap_off[f] = -1;
     B.face_active = 0;
     B.conn = 0;
+    B.uqv = 0;
+    B.pad_ = 0;
     B.tang = 0;
     B.umat = -1;
     off += B.npad;
@@ -2946,6 +2968,24 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     }
     e = launch_ghost(c, L, L.Tb[L.cur].p);
     if (e) return e;
+  }
+  // blocks of uniform qvol (after restriction, on every level)
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int m = 0; m < c->nlev; m++) {
+    Level& L = c->lev[m];
+    std::vector<double> q(L.npad);
+    HIPCHK(hipMemcpy(q.data(), L.qvol.p, sizeof(double) * L.npad, hipMemcpyDeviceToHost));
+    for (int b = 0; b < L.nb; b++) {
+      BlockInfo& B = L.hblk[b];
+      const double q0 = q[B.base];
+      int u = 1;
+      for (int k = 1; k <= B.nk && u; k++)
+        for (int j = 1; j <= B.nj && u; j++)
+          for (int i = 1; i <= B.ni; i++)
+            if (q[cidx(B, i, j, k)] != q0) { u = 0; break; }
+      B.uqv = u;
+    }
+    HIPCHK(hipMemcpy(L.blk.p, L.hblk.data(), sizeof(BlockInfo) * L.nb, hipMemcpyHostToDevice));
   }
   // compact wall records, now that every level's materials are set
   for (int m = 0; m < c->nlev; m++) {

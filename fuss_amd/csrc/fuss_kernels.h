@@ -38,6 +38,8 @@ struct BlockInfo {
   int tang;          // nonzero when any tangential weight is nonzero
   int umat;          // material index when the whole block has one material, else -1
   int conn;          // bit f-1 set when face f has a connection (101) entry
+  int uqv;           // nonzero when every cell of the block has the same qvol
+  int pad_;
 };
 
 struct Tile {

@@ -7,19 +7,21 @@ surface-only BC records.
 
 FP64 storage: T, dt, qvol, vol, weights, R: 8 B; material index: 4 B.  A
 level whose blocks each hold one material reads no material stream
-(mat=False): the index is a per-block constant.
+(mat=False): the index is a per-block constant; likewise a block of uniform
+qvol reads no qvol stream (qvol=False).  The four-stage update in primitive
+variables does not read the stage state (stage_state=False).
 """
 from __future__ import annotations
 
 
-def stage_bytes(ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True):
+def stage_bytes(ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True, qvol=True):
     """Fused flux + BC + newstate kernel, per cell.
 
     reads: T, dt, qvol, vol, material, own face weights (ndir normal weights,
     or 3 per face when tangential weights are present); writes the scaled
     residual (IRS on) or reads T^n and writes T (IRS off)."""
     nw = ndir * (3 if tang else 1)
-    b = fp * (1 + 1 + 1 + 1 + nw) + (4 if mat else 0)
+    b = fp * (1 + 1 + (1 if qvol else 0) + 1 + nw) + (4 if mat else 0)
     if irs:
         b += fp            # write R
     else:
@@ -35,10 +37,11 @@ def irs_bytes(update=False, fp=8, mat=True):
     return 2 * fp
 
 
-def irs_fused_bytes(fp=8, mat=True):
+def irs_fused_bytes(fp=8, mat=True, stage_state=True):
     """The x-y-z IRS pass with the update (levels without j/k connections):
-    read R, T, T^n (and the material), write T."""
-    return fp * 4 + (4 if mat else 0)
+    read R, T (the stage state, if the update uses it), T^n (and the
+    material), write T."""
+    return fp * (4 if stage_state else 3) + (4 if mat else 0)
 
 
 def dt_bytes(fp=8, mat=True):
@@ -46,12 +49,12 @@ def dt_bytes(fp=8, mat=True):
     return fp + (4 if mat else 0) + 8 + fp
 
 
-def iteration_bytes(n_rk=3, ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True, fused_irs=False):
+def iteration_bytes(n_rk=3, ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True, fused_irs=False, qvol=True):
     b = dt_bytes(fp, mat)
-    per_stage = stage_bytes(ndir, tang, irs, prim, fp, mat)
+    per_stage = stage_bytes(ndir, tang, irs, prim, fp, mat, qvol)
     if irs:
         if fused_irs:
-            per_stage += irs_fused_bytes(fp, mat)
+            per_stage += irs_fused_bytes(fp, mat, stage_state=not (prim and n_rk == 4))
         else:
             per_stage += (ndir - 1) * irs_bytes(False, fp) + irs_bytes(True, fp, mat)
     return b + n_rk * per_stage
