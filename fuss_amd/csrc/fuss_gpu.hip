@@ -618,6 +618,19 @@ __device__ __forceinline__ R scaled_residual(const Tab<R>& tab, int m, R T, R Rv
   return (-Rv / vol + qvol) * dt / (T * cp * drho + rho * cp + T * rho * dcp);
 }
 
+// Work-group minimum to pmin[wg] (as k_dt's per-tile minimum).
+__device__ __forceinline__ void block_min(double v, double* pmin, int wg) {
+  __shared__ double sm[TILE_T / 64];
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = sm[0];
+    for (int q = 1; q < TILE_T / 64; q++) s = fmin(s, sm[q]);
+    pmin[wg] = s;
+  }
+}
+
 __device__ __forceinline__ void block_sum(double v, double* partials, int wg) {
   __shared__ double sm[TILE_T / 64];
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1356,6 +1369,13 @@ struct IrsArgs {
   double* partials;
   int* status;
   const int* halt;
+  // k_irs_xyz on the last stage: the next step's local dt from the new state
+  // (Compute_dt of the next Explicit_Step, as k_dt computes it)
+  int dtnext;
+  double vnn;
+  const double* invL2;
+  R* dt;
+  double* pmin;
 };
 
 // x / (1 + 2 beta): a multiplication by the exact reciprocal when the
@@ -1802,7 +1822,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
   const int ci = live ? i : 1, cj = live ? j : 1;
   const unsigned p1 = (unsigned)(B.base + (ci - 1) + (long)(cj - 1) * B.sy);  // plane 1 of the lane's column
   const unsigned sz = (unsigned)B.sz;
-  double rsq = 0.0;
+  double rsq = 0.0, dtmin = 1e300;
   bool inb[XYZ_NLD];
 #pragma unroll
   for (int u = 0; u < XYZ_NLD; u++) inb[u] = xyz_in(B, t.i0, t.j0, u);
@@ -1875,6 +1895,30 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
       if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, ko);
       const double d = (double)Tn - (double)TO;
       rsq += d * d;
+      if (a.dtnext) {
+        // Compute_dt (Mod_dt.f90:26-84) at T^{n+1}, without the vnn ramp
+        // (the host fuses only when it cannot apply)
+        int bad3 = 0;
+        using V2 = typename Vec2<R>::T;
+        using V4 = typename Vec4<R>::T;
+        // (v, dv) rows: the interpolation v + (T - Tm) dv of tab_interp
+        const int rw = tab_row(Tn, a.tab.Ti1, a.tab.Ti2, &bad3) - a.tab.Ti1;
+        const unsigned roff = (unsigned)((long)m * a.tab.nT + rw);
+        const V2 kk = ldu(reinterpret_cast<const V2*>(a.tab.kap2), roff);
+        const V4 pp = ldu(reinterpret_cast<const V4*>(a.tab.pr4), roff);
+        double dd;
+        {
+#pragma clang fp contract(off)
+          const R fr = Tn - (R)(rw + a.tab.Ti1);
+          const R kap = kk.x + fr * kk.y, rho = pp.x + fr * pp.y, cp = pp.z + fr * pp.w;
+          const double alpha = (double)kap / (double)rho / (double)cp;
+          dd = 0.5 * a.vnn / (alpha * ldu(a.invL2, po));
+          dd = dd < 1e8 ? dd : 1e8;
+        }
+        stu(a.dt, po, (R)dd);
+        dtmin = fmin(dtmin, dd);
+        if (bad3) flag_error(a.status, FUSS_ETABLE, t.b, i, j, ko);
+      }
     }
   };
   for (int q = qa; q <= qe; q += 2) {
@@ -1882,6 +1926,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
     if (q + 1 <= qe) step(pb, q + 1);
   }
   if (a.last) block_sum(rsq, a.partials, wg);
+  if (a.dtnext) block_min(dtmin, a.pmin, wg);
 }
 
 // ------------------------------------------------------------------------
@@ -2143,6 +2188,8 @@ struct Level {
   DBuf<int> trioff;
   double tri_beta = -1.0;
   int ghost_read = 1;               // some BC entry reads layer-1 ghosts (101, 300, 302)
+  int dt_next = 0;                  // dt and its per-tile minima hold the next step's (fused IRS pass)
+  double dt_vnn = 0.0;              // ... computed with this vnn
   int all_umat = 0;                 // every block has a single material
   int dims2 = 0;                    // every block has >= 2 cells in each direction
   long npad = 0, ncell = 0, nnodes = 0;
@@ -2359,6 +2406,12 @@ int connect_metric(const HostBlock& bm, const HostBlock& bs, const int* q, doubl
 
 int upload(void* dst, const void* src, size_t bytes) {
   return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : FUSS_EHIP;
+}
+
+// The state (or the tables) changed outside the step loop: the next step
+// computes its dt afresh.
+void drop_dt_next(fuss_ctx* c) {
+  for (auto& L : c->lev) L.dt_next = 0;
 }
 
 void run_graph_free(Level& L) {
@@ -2699,6 +2752,7 @@ int fuss_create(int nlevels, int nblocks, int precision, fuss_ctx** out) {
 
 int fuss_set_table(fuss_ctx* c, int nmat, int Ti1, int nT, const double* cp, const double* rho,
                    const double* kappa, const double* h) {
+  if (c) drop_dt_next(c);
   // rows start at a non-negative temperature (Load_Table reads rows from
   // T = 1 K): the kernels' row test int(T) < Ti2 then equals T < Ti2
   if (!c || nmat < 1 || nT < 2 || Ti1 < 0 || !cp || !rho || !kappa || !h) return FUSS_EARG;
@@ -3198,26 +3252,39 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   }
   const unsigned ntile = (unsigned)L.htiles.size();
   const int* halt = L.cnt.p + 1;
-  // Compute_dt + min (Mod_dt.f90:10-84); dtglobal starts at 1e5 (Mod_Explicit.f90:40)
-  DtArgs<double> da;
-  da.blk = L.blk.p; da.tiles = L.tiles.p; da.T = L.Tb[L.cur].p; da.mat = L.mat.p; da.invL2 = L.invL2.p;
-  da.dt = L.dt.p; da.tab = dtab<double>(c); da.vnn = p->vnn; da.cnt = L.cnt.p; da.rampa = p->rampa_iter;
-  da.pmin = L.pmin.p; da.status = c->status.p;
-  int pid = prof_begin(c, FUSS_K_DT);
-  k_dt<double><<<ntile, TILE_T, 0, c->stream>>>(da);
-  prof_end(c, pid);
-  HIPCHK(hipGetLastError());
+  // The fused x-y-z IRS pass of the last stage can leave the next step's
+  // local dt (and its per-tile minima) behind: without a vnn ramp, when the
+  // state is the one that pass wrote and vnn is unchanged.
+  const bool xyz = irs && !tri && L.no_ycon && L.no_zcon && c->ndir == 3 && L.kp == GC;
+  const bool fuse_dt = xyz && p->rampa_iter <= 1 && !c->adj_store_on;
+  const bool have_dt = fuse_dt && L.dt_next && L.dt_vnn == p->vnn;
+  unsigned nmin = ntile;
+  int pid = -1;
+  if (have_dt) {
+    nmin = (unsigned)L.hctiles.size();
+  } else {
+    // Compute_dt + min (Mod_dt.f90:10-84); dtglobal starts at 1e5 (Mod_Explicit.f90:40)
+    DtArgs<double> da;
+    da.blk = L.blk.p; da.tiles = L.tiles.p; da.T = L.Tb[L.cur].p; da.mat = L.mat.p; da.invL2 = L.invL2.p;
+    da.dt = L.dt.p; da.tab = dtab<double>(c); da.vnn = p->vnn; da.cnt = L.cnt.p; da.rampa = p->rampa_iter;
+    da.pmin = L.pmin.p; da.status = c->status.p;
+    pid = prof_begin(c, FUSS_K_DT);
+    k_dt<double><<<ntile, TILE_T, 0, c->stream>>>(da);
+    prof_end(c, pid);
+    HIPCHK(hipGetLastError());
+  }
   // device scalars: [0] dtglobal, [1] time, [2] Sum (T-TO)^2
   pid = prof_begin(c, FUSS_K_REDUCE);
-  if (ntile > (unsigned)RED_CHUNK) {
-    const int nb1 = (int)((ntile + RED_CHUNK - 1) / RED_CHUNK);
-    k_reduce<true><<<nb1, 256, 0, c->stream>>>(L.pmin.p, (int)ntile, L.red2.p, halt);
+  if (nmin > (unsigned)RED_CHUNK) {
+    const int nb1 = (int)((nmin + RED_CHUNK - 1) / RED_CHUNK);
+    k_reduce<true><<<nb1, 256, 0, c->stream>>>(L.pmin.p, (int)nmin, L.red2.p, halt);
     k_dt_final<<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p, L.clk.p, L.cnt.p, p->time_accurate);
   } else {
-    k_dt_final<<<1, 256, 0, c->stream>>>(L.pmin.p, (int)ntile, c->scal.p, L.clk.p, L.cnt.p, p->time_accurate);
+    k_dt_final<<<1, 256, 0, c->stream>>>(L.pmin.p, (int)nmin, c->scal.p, L.clk.p, L.cnt.p, p->time_accurate);
   }
   prof_end(c, pid);
   HIPCHK(hipGetLastError());
+  L.dt_next = 0;
   const int o = L.cur, a1 = (L.cur + 1) % 3, a2 = (L.cur + 2) % 3;
   double* in = L.Tb[o].p;
   int npart = (int)ntile;  // partial sums written by the kernel that applies the last update
@@ -3262,6 +3329,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     if (irs) {
       // Residual_Smoothing (Lib_IRS.f90:14-40): ndir passes R -> R2 -> R ...
       IrsArgs<double> ia;
+      memset(&ia, 0, sizeof(ia));
       ia.blk = L.blk.p; ia.tiles = L.tiles.p; ia.bc = L.bc.p; ia.fmap = L.fmap.p;
       ia.beta = p->irs_beta;
       {
@@ -3277,10 +3345,13 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
         const int e = launch_irs_tridiag(c, L, ia, prim, &np_);
         if (e) return e;
         if (sa.last) { npart = np_; L.plist = 0; }
-      } else if (L.no_ycon && L.no_zcon && c->ndir == 3 && L.kp == GC) {
-        // x, y and z in one k-marching pass with the update
+      } else if (xyz) {
+        // x, y and z in one k-marching pass with the update (and on the last
+        // stage the next step's dt)
         ia.Rin = L.R.p;
         ia.tiles = L.ctiles.p;
+        ia.dtnext = fuse_dt && irk == nrk;
+        ia.vnn = p->vnn; ia.invL2 = L.invL2.p; ia.dt = L.dt.p; ia.pmin = L.pmin.p;
         pid = prof_begin(c, FUSS_K_IRS_UPDATE);
         launch_irs_xyz(c, L, ia, prim);
         prof_end(c, pid);
@@ -3350,6 +3421,8 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   }
   // T^{n+1} becomes the new T^n
   L.cur = (nrk % 2 == 1) ? a1 : a2;
+  L.dt_next = fuse_dt;
+  L.dt_vnn = p->vnn;
   return 0;
 }
 
@@ -3564,6 +3637,7 @@ int refine_run(fuss_ctx* c, const fuss_step_params* mon, fuss_refine_result* out
   bool have_def = false;
   for (int m = 0;; m++) {
     // monitor: one FP64 Explicit_Step; its residual is the reference's
+    drop_dt_next(c);  // T was corrected since the last monitor step
     int e = enqueue_step(c, 1, &pm);
     if (e) return e;
     const int nb = L.nb;
@@ -3845,6 +3919,7 @@ int fuss_graphs(fuss_ctx* c, int enable) {
 }
 
 int fuss_set_kscale(fuss_ctx* c, double scale) {
+  if (c) drop_dt_next(c);
   if (!c || !c->nT || !(scale > 0.0)) return FUSS_EARG;
   const size_t n = (size_t)c->nmat * c->nT;
   std::vector<double> k(n), k2(2 * n, 0.0);
@@ -3886,12 +3961,14 @@ int fuss_adj_setup(fuss_ctx* c, int nsteps, int ckpt_every, int nprobe, const in
 }
 
 int fuss_adj_forward(fuss_ctx* c, const fuss_step_params* p, double* probe_hist) {
+  if (c) drop_dt_next(c);
   if (!c || !c->setup_done || !p || !c->adj.ready) return FUSS_EARG;
   return adj_forward(c, p, probe_hist);
 }
 
 int fuss_adj_backward(fuss_ctx* c, const double* probe_adj, double* grad_qw, double* grad_kscale,
                       double* grad_T0) {
+  if (c) drop_dt_next(c);
   if (!c || !c->adj.ready || !c->adj.forward_done) return FUSS_EARG;
   const bool d3 = c->lev[0].kp == GC;
   if (c->adj.prec == FUSS_FP32)
@@ -3917,10 +3994,13 @@ int fuss_refine_blocks(fuss_ctx* c, int* conv_outer, double* res) {
 }
 
 int fuss_refine(fuss_ctx* c, const fuss_step_params* monitor, fuss_refine_result* out, double* hist_res) {
+  if (c) drop_dt_next(c);
   if (!c || !c->setup_done || !monitor || !out || !c->ref.ready) return FUSS_EARG;
   if (monitor->time_accurate) return FUSS_EARG;
-  return c->ref.p.precision == FUSS_FP32 ? refine_run<float>(c, monitor, out, hist_res)
-                                         : refine_run<double>(c, monitor, out, hist_res);
+  const int e = c->ref.p.precision == FUSS_FP32 ? refine_run<float>(c, monitor, out, hist_res)
+                                               : refine_run<double>(c, monitor, out, hist_res);
+  drop_dt_next(c);
+  return e;
 }
 
 }  // extern "C"
@@ -3983,6 +4063,7 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
     for (int ki = 0; ki < 3 && !err; ki++) {
       if ((err = set_kc(c, L, kcs[ki]))) break;
       L.row_march = row;
+      drop_dt_next(c);  // the state was restored
       if ((err = enqueue_step(c, level, &q))) break;  // warm-up
       HIPCHK(hipEventRecord(e0, c->stream));
       for (int s2 = 0; s2 < steps && !err; s2++) err = enqueue_step(c, level, &q);
@@ -3998,6 +4079,7 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   save.free();
+  drop_dt_next(c);  // the state was restored after the last candidate
   if (err) return err;
   const int st = sync_status(c);
   if (st) return st;
@@ -4015,6 +4097,7 @@ extern "C" {
 
 int fuss_autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* kc, int* row_march,
                   double* ms_per_step) {
+  if (c) drop_dt_next(c);
   if (!c || !c->setup_done || !p || level < 1 || level > c->nlev || steps < 1) return FUSS_EARG;
   return autotune(c, level, p, steps, kc, row_march, ms_per_step);
 }
@@ -4087,6 +4170,9 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
   int* cur_seq = L.run_seq;
   if (!hit) {
     run_graph_free(L);
+    // the chunk's first step computes its dt: a replay may follow any change
+    // of the state
+    L.dt_next = 0;
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int e = 0;
@@ -4155,6 +4241,7 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
 }
 
 int fuss_restrict(fuss_ctx* c, int fine_level) {
+  if (c) drop_dt_next(c);
   if (!c || !c->setup_done || fine_level < 1 || fine_level >= c->nlev) return FUSS_EARG;
   int e = restrict_level(c, fine_level - 1);
   if (e) return e;
@@ -4162,6 +4249,7 @@ int fuss_restrict(fuss_ctx* c, int fine_level) {
 }
 
 int fuss_prolong(fuss_ctx* c, int fine_level) {
+  if (c) drop_dt_next(c);
   if (!c || !c->setup_done || fine_level < 1 || fine_level >= c->nlev) return FUSS_EARG;
   Level& F = c->lev[fine_level - 1];
   Level& C = c->lev[fine_level];
@@ -4207,6 +4295,7 @@ int fuss_get_field(fuss_ctx* c, int level, int b, const char* name, double* out)
 int fuss_get_T(fuss_ctx* c, int level, int b, double* T) { return fuss_get_field(c, level, b, "T", T); }
 
 int fuss_set_T(fuss_ctx* c, int level, int b, const double* T) {
+  if (c) drop_dt_next(c);
   if (!c || !c->setup_done || level < 1 || level > c->nlev || b < 1 || b > c->nb || !T) return FUSS_EARG;
   Level& L = c->lev[level - 1];
   const BlockInfo& B = L.hblk[b - 1];
@@ -4245,6 +4334,7 @@ int fuss_get_clock(fuss_ctx* c, int level, double* time, int* iter) {
 }
 
 int fuss_set_clock(fuss_ctx* c, int level, double time, int iter) {
+  if (c) drop_dt_next(c);
   if (!c || level < 1 || level > c->nlev) return FUSS_EARG;
   c->lev[level - 1].time = time;
   c->lev[level - 1].iter = iter;

@@ -209,6 +209,39 @@ def test_graph_run_equals_step_loop(gpu, name):
     assert np.array_equal(T1, T2)
 
 
+def test_graph_run_fused_dt(gpu):
+    """The cube (fused x-y-z IRS pass, which leaves the next step's dt
+    behind) on captured graphs equals the step loop bit for bit."""
+    out = []
+    for graphs in (True, False):
+        s = gpu.Solver(SYNTH["cube3d_rk4_irs"]())
+        s.graphs(graphs)
+        k, hr, _, _ = s.run_level(1, 60, 60, -1.0, 1e300)
+        out.append((k, hr.copy(), s.get("T")))
+        s.close()
+    assert out[0][0] == out[1][0] == 60
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+
+
+def test_set_T_between_steps(gpu, oracle_mod):
+    """A state set between steps (restart) is stepped with its own dt: the
+    next step's dt left by the fused IRS pass is dropped."""
+    c = SYNTH["cube3d_rk4_irs"]()
+    s = gpu.Solver(c)
+    o = oracle_mod.Oracle(c)
+    for _ in range(3):
+        s.explicit_step(1)
+        o.step(1)
+    T = o.get("T") + 5.0 * np.sin(np.arange(o.get("T").size)).reshape(o.get("T").shape)
+    s.set_T(T)
+    o.set("T", T)
+    for _ in range(3):
+        s.explicit_step(1)
+        o.step(1)
+    assert relerr(s.get("T"), o.get("T")) < RTOL
+    s.close()
+
+
 def test_graph_run_tridiag(gpu):
     """fuss_run with the tridiagonal IRS on captured graphs (its tables are
     uploaded before the capture) equals the step loop bit for bit."""
