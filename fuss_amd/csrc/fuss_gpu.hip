@@ -1032,13 +1032,13 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   // cached line instead of a stream)
   const unsigned quni = (unsigned)B.base;
   auto stream = [&](StageStream<R>& s, unsigned pp, unsigned hh) {
-    s.vol = ldu(a.vol, pp);
+    s.vol = ldn(a.vol, pp);
     s.qv = ldu(a.qvol, B.uqv ? quni : pp);
-    s.dt = ldu(a.dt, pp);
-    if (!IRSMODE) s.TO = ldu(a.TO, pp);
-    s.w0 = ldu(a.W[0], pp);
-    s.w4 = ldu(a.W[4], pp);
-    s.w8 = ldu(a.W[8], pp);
+    s.dt = ldn(a.dt, pp);
+    if (!IRSMODE) s.TO = ldn(a.TO, pp);
+    s.w0 = ldn(a.W[0], pp);
+    s.w4 = ldn(a.W[4], pp);
+    s.w8 = ldn(a.W[8], pp);
     s.wh = ldu(Wh, hh);
     if (!UM) {
       s.m = ldu(a.mat, pp);
@@ -1116,7 +1116,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
       fetch_plane_u(pre, a.T, B, t.i0, t.j0, k < k1 ? k + 2 : k + 1);
       stream(nxt, p + step, ph + step);
     }
-    if (spend) stu(out, ppend, vpend);  // the previous plane's result
+    if (spend) stn(out, ppend, vpend);  // the previous plane's result
     // phase C: fluxes
     R fz = 0;
     {
@@ -1190,7 +1190,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
     plane(sa_, sb_, k);
     if (k + 1 <= k1) plane(sb_, sa_, k + 1);
   }
-  if (spend) stu(out, ppend, vpend);
+  if (spend) stn(out, ppend, vpend);
   if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
 }
 
@@ -1837,7 +1837,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
     const unsigned po = p1 + (unsigned)(max(ko, 1) - 1) * sz;
     // the four-stage update in primitive variables (new = old + a_k res) does
     // not read the stage state: its load then hits T^n's line (no HBM bytes)
-    const R T = ldu((PRIM && a.jam) ? a.TO : a.T, po), TO = ldu(a.TO, po);
+    const R T = ldn((PRIM && a.jam) ? a.TO : a.T, po), TO = ldn(a.TO, po);
     int m = B.umat;
     if (m < 0) m = ldu(a.mat, po);
     R s = 0;
@@ -1890,7 +1890,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
       const R rs = jacobi2(w0, w1, w2, w3, w4, ko, n, zlo, zhi, beta, den, inv);
       int bad = 0;
       const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
-      stu(a.Tout, po, Tn);
+      stn(a.Tout, po, Tn);
       if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, ko);
       if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, ko);
       const double d = (double)Tn - (double)TO;
@@ -1912,10 +1912,10 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
           const R fr = Tn - (R)(rw + a.tab.Ti1);
           const R kap = kk.x + fr * kk.y, rho = pp.x + fr * pp.y, cp = pp.z + fr * pp.w;
           const double alpha = (double)kap / (double)rho / (double)cp;
-          dd = 0.5 * a.vnn / (alpha * ldu(a.invL2, po));
+          dd = 0.5 * a.vnn / (alpha * ldn(a.invL2, po));
           dd = dd < 1e8 ? dd : 1e8;
         }
-        stu(a.dt, po, (R)dd);
+        stn(a.dt, po, (R)dd);
         dtmin = fmin(dtmin, dd);
         if (bad3) flag_error(a.status, FUSS_ETABLE, t.b, i, j, ko);
       }

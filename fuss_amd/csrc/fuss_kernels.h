@@ -117,6 +117,18 @@ template <typename T>
 __device__ __forceinline__ void stu(T* __restrict__ base, unsigned idx, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T)) = v;
 }
+// Streamed (read-once / write-once) elements: non-temporal accesses, so that
+// they do not evict the property-table rows every lookup reuses from the
+// caches.
+template <typename T>
+__device__ __forceinline__ T ldn(const T* __restrict__ base, unsigned idx) {
+  return __builtin_nontemporal_load(
+      reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void stn(T* __restrict__ base, unsigned idx, T v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T)));
+}
 
 // Interpolation from a (v, dv) row: v(Tm) + (T - Tm) * (v(Tm+1) - v(Tm)),
 // the same operations as tab_interp (the difference is formed once at setup).
