@@ -934,13 +934,14 @@ __global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
 // ------------------------------------------------------------------------
 // Row of T in a 1 K table (co_Kappa etc., Lib_Solid.f90:19-23: Tm = int(T)).
 // Outside [Ti1, Ti2) *bad is set (the step then fails with FUSS_ETABLE) and
-// the row is clamped so the lookup stays inside the table.  With Ti1 >= 0,
-// int(T) >= Ti2 exactly when T >= Ti2, and a NaN fails T >= Ti1.
+// T is clamped to [Ti1, Ti2 - 1] before the conversion, so the row stays
+// inside the table and the conversion never sees a NaN or an out-of-range
+// value (fmax of a NaN returns the bound).  With Ti1 >= 0, int(T) >= Ti2
+// exactly when T >= Ti2.
 template <typename R>
 __device__ __forceinline__ int tab_row(R T, int Ti1, int Ti2, int* bad) {
-  const int Tm = (int)T;
-  if (!(T >= (R)Ti1) || Tm >= Ti2) *bad = 1;
-  return min(max(Tm, Ti1), Ti2 - 1);
+  if (!(T >= (R)Ti1 && T < (R)Ti2)) *bad = 1;
+  return (int)fmin(fmax(T, (R)Ti1), (R)(Ti2 - 1));
 }
 
 template <typename R>
@@ -2683,7 +2684,7 @@ int restrict_level(fuss_ctx* c, int fine) {
   Level& F = c->lev[fine];
   Level& C = c->lev[fine + 1];
   for (int b = 0; b < F.nb; b++) {
-    MGArgs a;
+    MGArgs a{};
     a.F = F.hblk[b]; a.Cb = C.hblk[b];
     a.fT = F.Tb[F.cur].p; a.fmat = F.mat.p; a.fvol = F.vol.p; a.fqvol = F.qvol.p;
     a.cT = C.Tb[C.cur].p; a.cmat = C.mat.p; a.cvol = C.vol.p; a.cqvol = C.qvol.p;
@@ -2884,7 +2885,7 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     e |= upload(L.blk.p, L.hblk.data(), sizeof(BlockInfo) * L.nb);
     if (e) return e;
     // cell metrics
-    GeomArgs g;
+    GeomArgs g{};
     g.blk = L.blk.p; g.tiles = L.tiles.p; g.nodes = L.nodes.p; g.M = L.M.p; g.vol = L.vol.p;
     g.invL2 = L.invL2.p; g.ndir = c->ndir; g.status = c->status.p;
     k_cell_metrics<<<(unsigned)L.htiles.size(), TILE_T, 0, c->stream>>>(g);
@@ -2894,7 +2895,7 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     if (tang.alloc(L.nb)) return FUSS_EHIP;
     for (int b = 0; b < L.nb; b++)
       for (int D = 0; D < 3; D++) {
-        FaceArgs f;
+        FaceArgs f{};
         f.B = L.hblk[b]; f.bidx = b; f.D = D; f.sign = norm_sign(L.hb[b], D);
         f.nodes = L.nodes.p; f.M = L.M.p; f.F = L.F[D].p;
         for (int cc = 0; cc < 3; cc++) f.W[cc] = L.W[3 * D + cc].p;
@@ -3192,7 +3193,7 @@ int tri_prepare(fuss_ctx* c, Level& L, double beta) {
 // R -> R2 -> R (-> R2), each with the ghosts of the current R, then the RK
 // update.  Returns the tile list count of the update's residual partials.
 int launch_irs_tridiag(fuss_ctx* c, Level& L, IrsArgs<double> ia, bool prim, int* npart) {
-  TriArgs<double> ta;
+  TriArgs<double> ta{};
   ta.blk = L.blk.p; ta.bc = L.bc.p; ta.fmap = L.fmap.p; ta.tri = L.tri.p; ta.trioff = L.trioff.p;
   ta.beta = ia.beta; ta.halt = ia.halt;
   double* rin = L.R.p;
@@ -3264,7 +3265,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     nmin = (unsigned)L.hctiles.size();
   } else {
     // Compute_dt + min (Mod_dt.f90:10-84); dtglobal starts at 1e5 (Mod_Explicit.f90:40)
-    DtArgs<double> da;
+    DtArgs<double> da{};
     da.blk = L.blk.p; da.tiles = L.tiles.p; da.T = L.Tb[L.cur].p; da.mat = L.mat.p; da.invL2 = L.invL2.p;
     da.dt = L.dt.p; da.tab = dtab<double>(c); da.vnn = p->vnn; da.cnt = L.cnt.p; da.rampa = p->rampa_iter;
     da.pmin = L.pmin.p; da.status = c->status.p;
@@ -3298,13 +3299,14 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     if (c->adj_store_on)
       HIPCHK(hipMemcpyAsync(c->adj.stage[irk - 1].p, in, L.npad * sizeof(double), hipMemcpyDeviceToDevice,
                             c->stream));
-    StageArgs<double> sa;
+    StageArgs<double> sa{};
     sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.T = in; sa.TO = L.Tb[o].p; sa.Tout = out; sa.Rres = L.R.p;
     sa.dt = L.dt.p; sa.dtg = c->scal.p; sa.use_dtg = p->time_accurate; sa.qvol = L.qvol.p; sa.vol = L.vol.p;
     sa.mat = L.mat.p;
     for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
     sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.bcf = L.bcf.p; sa.tab = dtab<double>(c);
     sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = halt;
+    sa.partials = L.partials.p; sa.status = c->status.p;  // k_bcflux flags table errors through status
     if (L.any_active && L.nbound) {
       pid = prof_begin(c, FUSS_K_BC);
       if (L.kp == GC) k_bcflux<3, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
@@ -3315,7 +3317,6 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     sa.rkc = nrk == 4 ? RK4A[irk - 1] : RKC[nrk - 1][irk - 1];
     sa.jam = nrk == 4;
     sa.last = irk == nrk && p->compute_residual;
-    sa.partials = L.partials.p; sa.status = c->status.p;
     sa.ctiles = L.ctiles.p;
     pid = prof_begin(c, FUSS_K_STAGE);
     if (L.kp == GC) {
@@ -3328,7 +3329,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     HIPCHK(hipGetLastError());
     if (irs) {
       // Residual_Smoothing (Lib_IRS.f90:14-40): ndir passes R -> R2 -> R ...
-      IrsArgs<double> ia;
+      IrsArgs<double> ia{};
       memset(&ia, 0, sizeof(ia));
       ia.blk = L.blk.p; ia.tiles = L.tiles.p; ia.bc = L.bc.p; ia.fmap = L.fmap.p;
       ia.beta = p->irs_beta;
@@ -3674,7 +3675,7 @@ int refine_run(fuss_ctx* c, const fuss_step_params* mon, fuss_refine_result* out
     double* T = L.Tb[L.cur].p;
     e = launch_ghost(c, L, T);
     if (e) return e;
-    StageArgs<double> sa;
+    StageArgs<double> sa{};
     memset(&sa, 0, sizeof(sa));
     sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.ctiles = L.ctiles.p; sa.T = T; sa.TO = T;
     sa.qvol = L.qvol.p; sa.vol = L.vol.p; sa.mat = L.mat.p;
@@ -3860,7 +3861,7 @@ int adj_backward_t(fuss_ctx* c, const double* g, double* gqw, double* gk, double
       c->adj_store_on = 0;
       if (err) break;
       for (int sidx = nrk; sidx >= 1; sidx--) {
-        StageArgs<double> sa;
+        StageArgs<double> sa{};
         memset(&sa, 0, sizeof(sa));
         sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.T = A.stage[sidx - 1].p; sa.qvol = L.qvol.p; sa.vol = L.vol.p;
         sa.mat = L.mat.p;
@@ -3869,7 +3870,7 @@ int adj_backward_t(fuss_ctx* c, const double* g, double* gqw, double* gk, double
         sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = L.cnt.p + 1;
         if (L.nbound && L.any_active)
           k_bcflux<NDIR, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
-        AdjArgs<S> aa;
+        AdjArgs<S> aa{};
         aa.blk = L.blk.p; aa.tiles = L.tiles.p; aa.T = A.stage[sidx - 1].p; aa.qvol = L.qvol.p; aa.vol = L.vol.p;
         aa.mat = L.mat.p;
         for (int d = 0; d < 3; d++) aa.W[d] = L.W[4 * d].p;
@@ -4254,7 +4255,7 @@ int fuss_prolong(fuss_ctx* c, int fine_level) {
   Level& F = c->lev[fine_level - 1];
   Level& C = c->lev[fine_level];
   for (int b = 0; b < F.nb; b++) {
-    MGArgs a;
+    MGArgs a{};
     a.F = F.hblk[b]; a.Cb = C.hblk[b]; a.fT = F.Tb[F.cur].p; a.cT = C.Tb[C.cur].p;
     a.tab = dtab<double>(c); a.status = c->status.p; a.bidx = b;
     const long n = (long)a.Cb.ni * a.Cb.nj * a.Cb.nk;

@@ -138,6 +138,13 @@ static void set_err(ob_ctx *c, int code, int b, int i, int j, int k) {
 /* physics: src/lib/physics/Lib_Solid.f90                              */
 /* ------------------------------------------------------------------ */
 static inline int tab_ok(ob_ctx *c, int Tm) { return Tm >= c->Ti1 && Tm + 1 <= c->Ti2; }
+/* int(T) of the reference, taken only where T is a representable int (the
+ * conversion of NaN or |T| >= 2^31 is undefined in C); such T fail the test. */
+static inline int tab_row(ob_ctx *c, double T, int *Tm) {
+  if (!(T > -1.0e9 && T < 1.0e9)) return 0;
+  *Tm = (int)T;
+  return tab_ok(c, *Tm);
+}
 static inline double tab(const ob_ctx *c, const double *a, double matID, int Ti) {
   int m = (int)matID; /* REAL(8) subscript, truncated */
   return a[(long)(m - 1) * c->nT + (Ti - c->Ti1)];
@@ -145,11 +152,12 @@ static inline double tab(const ob_ctx *c, const double *a, double matID, int Ti)
 /* co_Kappa/co_Rho/co_Cp/co_H, Lib_Solid.f90:11-76: linear interpolation at
  * Tm=int(T), v = vm + (T-Tm)/(Tp-Tm)*(vp-vm). */
 static double co_interp(ob_ctx *c, const double *a, double matID, double T) {
-  int Tm = (int)T, Tp = Tm + 1;
-  if (!tab_ok(c, Tm) || !(T == T)) {
+  int Tm = 0;
+  if (!tab_row(c, T, &Tm)) {
     set_err(c, OB_ERR_TABLE, 0, 0, 0, 0);
     return NAN;
   }
+  const int Tp = Tm + 1;
   double vm = tab(c, a, matID, Tm), vp = tab(c, a, matID, Tp);
   return vm + (T - (double)Tm) / (double)(Tp - Tm) * (vp - vm);
 }
@@ -159,11 +167,12 @@ static double co_interp(ob_ctx *c, const double *a, double matID, double T) {
 #define co_H(c, m, T) co_interp(c, (c)->h, m, T)
 /* co_DrhoDT/co_DcpDT, Lib_Solid.f90:101-132: (vp-vm)/(Tp-Tm). */
 static double co_slope(ob_ctx *c, const double *a, double matID, double T) {
-  int Tm = (int)T, Tp = Tm + 1;
-  if (!tab_ok(c, Tm) || !(T == T)) {
+  int Tm = 0;
+  if (!tab_row(c, T, &Tm)) {
     set_err(c, OB_ERR_TABLE, 0, 0, 0, 0);
     return NAN;
   }
+  const int Tp = Tm + 1;
   return (tab(c, a, matID, Tp) - tab(c, a, matID, Tm)) / (double)(Tp - Tm);
 }
 /* co_T, Lib_Solid.f90:79-98: first subscript i=1..size with h(i) > h, then

@@ -242,6 +242,47 @@ def test_set_T_between_steps(gpu, oracle_mod):
     s.close()
 
 
+@pytest.mark.parametrize("where", ["wall", "interior", "below", "all_huge"])
+@pytest.mark.parametrize("graphs", [False, True])
+def test_table_range_error(gpu, oracle_mod, where, graphs):
+    """A temperature outside the property table rows (T in [1, 2000) K here)
+    stops the step with the table status, as the table lookups of the
+    reference stop (Lib_Solid.f90:10-98), in the oracle and on the device:
+    on a wall cell (the wall-flux pass flags it), an interior cell, below the
+    first row, and a field far beyond the int range of the row index."""
+    c = SYNTH["cube3d_rk4_irs"]()
+    s = gpu.Solver(c)
+    s.graphs(graphs)
+    o = oracle_mod.Oracle(c)
+    T = o.get("T").copy()
+    if where == "wall":
+        T[5, 7, 0] = 2500.0
+    elif where == "interior":
+        T[12, 12, 12] = 2500.0
+    elif where == "below":
+        T[12, 12, 12] = 0.5
+    else:
+        T[:] = 1e12
+    s.set_T(T)
+    o.set("T", T)
+    with pytest.raises(oracle_mod.OracleError, match="status 3"):
+        o.step(1)
+    with pytest.raises(gpu.FussError) as ei:
+        if graphs:
+            s.run_level(1, 40, 40, -1.0, 1e300)
+        else:
+            s.explicit_step(1)
+    assert ei.value.code == gpu.FUSS_ETABLE
+    s.close()
+    # the device runs the next context normally
+    s = gpu.Solver(c)
+    o = oracle_mod.Oracle(c)
+    s.explicit_step(1)
+    o.step(1)
+    assert relerr(s.get("T"), o.get("T")) < RTOL
+    s.close()
+
+
 def test_graph_run_tridiag(gpu):
     """fuss_run with the tridiagonal IRS on captured graphs (its tables are
     uploaded before the capture) equals the step loop bit for bit."""
