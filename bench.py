@@ -44,8 +44,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the stage kernel measured with rocprofv3 PMC
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB device copy);
 # source: profiles/explicit_step_cube256_fp64.md (256^3 cube, FP64).
-STAGE_KERNEL = "k_stage3p<true, true, true, double, 16>"
-STAGE_TRAFFIC_MEASURED = {STAGE_KERNEL: 482501 * 1024 * 2.0 + 131072 * 1024 * 1.0}
+# The LDS-ring stage kernel with 16-plane columns, one material, primitive
+# variables, IRS on; RECT = per-axis metric factors (rectilinear level).
+STAGE_KERNEL = {True: "k_stage3p<true, true, true, true, double, 16>",
+                False: "k_stage3p<true, true, true, false, double, 16>"}
+STAGE_TRAFFIC_MEASURED = {}
 # Plain FP64 path (four-stage RK + LTS + IRS 0.5, 3-level sequencing) run to
 # residual 1e-10 on the GPU: iterations per cube size (profiles/
 # steady_convergence_cube.md).  Fit: iters = 230218 (n/64)^PLAIN_EXP.
@@ -314,6 +317,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="also run 1 GiB D2D copies (counter calibration)")
     ap.add_argument("--rk", type=int, default=4)
+    ap.add_argument("--no-rect", action="store_true",
+                    help="stream the stored 3-D metric arrays instead of the per-axis factors")
     ap.add_argument("--no-refine", action="store_true", help="skip the time-to-1e-10 runs")
     ap.add_argument("--plain-conv-n", type=int, default=32, help="cube size of the live plain run to 1e-10 (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (inverse problem) measurement")
@@ -337,6 +342,12 @@ def main():
     t0 = time.perf_counter()
     s = gpu.Solver(case)
     print(f"[bench] setup {time.perf_counter()-t0:.1f} s", file=sys.stderr, flush=True)
+    rect = s.rect_info(1)
+    print(f"[bench] per-axis metric factors: weights {rect[0]}, dt {rect[1]}, "
+          f"max relative deviation {rect[2]:.3e}", file=sys.stderr, flush=True)
+    if args.no_rect:
+        s.rect(False)
+        rect = (False, False, rect[2])
     tuned = s.autotune(1, steps=4)
     print(f"[bench] autotune: column height {tuned[0]}, {'row march' if tuned[1] else 'LDS ring'}, "
           f"{tuned[2]:.3f} ms/step", file=sys.stderr, flush=True)
@@ -367,8 +378,12 @@ def main():
           flush=True)
     names = ["dt", "ghost", "stage", "irs", "irs_update", "reduce", "bc"]
     kern = {names[q]: {"ms_avg": ms[q] / cnt[q], "launches": cnt[q]} for q in range(7) if cnt[q]}
-    # one material and a uniform qvol: the stage kernel reads neither stream
-    stage_b = perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False, qvol=False) * ncell
+    # one material and a uniform qvol: the stage kernel reads neither stream;
+    # on a rectilinear level it reads no per-cell geometry either
+    stage_bpc = perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False, qvol=False, rect=rect[0])
+    stage_b = stage_bpc * ncell
+    skern = STAGE_KERNEL[rect[0]]
+    ldsring16 = args.n == 256 and tuned[0] == 16 and not tuned[1]
     stage_ms = ms[2] / cnt[2]
     achieved = stage_b / (stage_ms * 1e-3) / 1e9
     out = {
@@ -392,16 +407,20 @@ def main():
                              "chosen_by": "fuss_autotune"}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (STAGE_TRAFFIC_MEASURED.get(STAGE_KERNEL)
-                                 if args.n == 256 and tuned[0] == 16 and not tuned[1] else None),
-                     "kernel": (STAGE_KERNEL if tuned[0] == 16 and not tuned[1] else "stage kernel")
+                     "traffic": STAGE_TRAFFIC_MEASURED.get(skern) if ldsring16 else None,
+                     "kernel": (skern if ldsring16 else "stage kernel")
                                + " (fused interior + BC fluxes, scaled residual)",
-                     "bytes_per_cell": perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False,
-                                                             qvol=False),
+                     "bytes_per_cell": stage_bpc,
                      "ms_avg": stage_ms},
         "kernels": kern,
+        "metric_factors": {"per_axis": rect[0], "dt_per_axis": rect[1], "max_rel_dev": rect[2]},
         "iteration_bytes_per_cell": perfmodel.iteration_bytes(args.rk, 3, False, True, True, mat=False,
-                                                             fused_irs=True, qvol=False),
+                                                             fused_irs=True, qvol=False, rect=rect[0],
+                                                             fused_dt=True),
+        # the bytes a straightforward implementation streams per cell-update
+        # (every operand per stage), and the throughput they amount to here
+        "canonical_bytes_per_cell_update": perfmodel.canonical_bytes(args.rk, True, 3),
+        "canonical_GBs": perfmodel.canonical_bytes(args.rk, True, 3) * value / 1e9,
     }
     ms_step = wall / args.steps * 1e3
     if not args.no_refine:

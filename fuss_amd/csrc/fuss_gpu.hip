@@ -144,6 +144,44 @@ __global__ void k_face_metrics(FaceArgs a) {
   }
 }
 
+// Deviation of the per-axis factors (RectAxes) of the rectilinear blocks
+// from the stored metrics, with the products the stage kernel forms: *dev =
+// the largest relative deviation of a volume or interior normal weight (bits
+// of a non-negative double, so the integer maximum is the numeric one),
+// *nbad += cells whose sum 1/dl^2 differs in any bit.
+__global__ __launch_bounds__(TILE_T) void k_rect_check(const BlockInfo* __restrict__ blk, const Tile* __restrict__ tiles,
+                                                       const double* __restrict__ rect, const double* __restrict__ vol,
+                                                       const double* __restrict__ invL2, const double* __restrict__ W0,
+                                                       const double* __restrict__ W4, const double* __restrict__ W8,
+                                                       unsigned long long* dev, int* nbad) {
+  const Tile t = tiles[blockIdx.x];
+  const BlockInfo B = blk[t.b];
+  if (B.roff < 0) return;
+  const int i = t.i0 + threadIdx.x % TILE_X, j = t.j0 + threadIdx.x / TILE_X, k = t.k0;
+  double d = 0.0;
+  int bad = 0;
+  if (i <= B.ni && j <= B.nj) {
+    const RectAxes r = rect_axes(rect, B);
+    const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
+    auto rel = [](double ref, double v) { return v == ref ? 0.0 : fabs(v - ref) / fabs(ref); };
+    const double axy = r.ax[i] * r.ay[j];
+    d = rel(vol[p], axy * r.az[k]);
+    if (i < B.ni) d = fmax(d, rel(W0[p], (r.fx[i] * r.ay[j]) * r.az[k]));
+    if (j < B.nj) d = fmax(d, rel(W4[p], (r.fy[j] * r.ax[i]) * r.az[k]));
+    if (k < B.nk) d = fmax(d, rel(W8[p], r.fz[k] * axy));
+    if (!(d <= 1.0)) d = 1.0;  // NaN or a gross mismatch
+    bad = invL2[p] != (r.qx[i] + r.qy[j]) + r.qz[k];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    d = fmax(d, __shfl_xor(d, o, 64));
+    bad += __shfl_xor(bad, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(dev, (unsigned long long)__double_as_longlong(d));
+    if (bad) atomicAdd(nbad, bad);
+  }
+}
+
 // Fill geometry of BC records from the cell metrics and faces.
 __global__ void k_bc_geometry(BCRec* bc, int n, const BlockInfo* blk, const double* M, const double* F0,
                               const double* F1, const double* F2) {
@@ -277,6 +315,7 @@ struct StageArgs {
   int last;
   double* partials;
   int* status;
+  const double* rect;  // per-axis metric factors (rectilinear levels, k_stage3p<RECT>)
 };
 
 // One interior face flux between cells q and q+e_D (left cell q), in the
@@ -971,7 +1010,7 @@ __device__ __forceinline__ void fetch_plane_u(PlaneRegs<R>& r, const R* __restri
   }
 }
 
-template <bool UM, bool PRIM, bool IRSMODE, typename R, int KCT>
+template <bool UM, bool PRIM, bool IRSMODE, bool RECT, typename R, int KCT>
 __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -1001,6 +1040,25 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   const V4* pr0 = reinterpret_cast<const V4*>(a.tab.pr4);
   // the halo face's weight array: +y weights for the y-halo row, +x for the x-halo
   const R* __restrict__ Wh = yhalo ? a.W[4] : a.W[0];
+  // RECT (every block rectilinear): volume and normal weights are products of
+  // per-axis factors, the lane's (i, j) part formed once, the column's k
+  // parts staged in LDS (a global load inside the plane loop would make the
+  // in-order wait for it also wait for the next plane's prefetch)
+  R cvol = 0, cw0 = 0, cw4 = 0, cwh = 0;
+  __shared__ double zfac[2][KCT + 1];  // a_z, f_z of planes k0-1 .. k0+KCT-1
+  if (RECT) {
+    const RectAxes rx_ = rect_axes(a.rect, B);
+    if (threadIdx.x < 2 * (KCT + 1)) {
+      const int q = threadIdx.x % (KCT + 1), kk = min(k0 - 1 + q, B.nk + 1);
+      zfac[threadIdx.x / (KCT + 1)][q] = threadIdx.x < KCT + 1 ? rx_.az[kk] : rx_.fz[kk];
+    }
+    const int ci_ = live ? i : 1, cj_ = live ? j : 1;
+    cvol = (R)(rx_.ax[ci_] * rx_.ay[cj_]);
+    cw0 = (R)(rx_.fx[ci_] * rx_.ay[cj_]);
+    cw4 = (R)(rx_.fy[cj_] * rx_.ax[ci_]);
+    // the halo face: +y face of (i, j0-1) or +x face of (i0-1, j0+hr)
+    cwh = yhalo ? (R)(rx_.fy[t.j0 - 1] * rx_.ax[i]) : (xhalo ? (R)(rx_.fx[t.i0 - 1] * rx_.ay[t.j0 + hr]) : (R)0);
+  }
   int sm = 0, sc = 1, sp = 2;
   PlaneRegs<R> pre;
   fetch_plane_u(pre, a.T, B, t.i0, t.j0, k0 - 1);
@@ -1033,14 +1091,16 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   // cached line instead of a stream)
   const unsigned quni = (unsigned)B.base;
   auto stream = [&](StageStream<R>& s, unsigned pp, unsigned hh) {
-    s.vol = ldn(a.vol, pp);
     s.qv = ldu(a.qvol, B.uqv ? quni : pp);
     s.dt = ldn(a.dt, pp);
     if (!IRSMODE) s.TO = ldn(a.TO, pp);
-    s.w0 = ldn(a.W[0], pp);
-    s.w4 = ldn(a.W[4], pp);
-    s.w8 = ldn(a.W[8], pp);
-    s.wh = ldu(Wh, hh);
+    if (!RECT) {
+      s.vol = ldn(a.vol, pp);
+      s.w0 = ldn(a.W[0], pp);
+      s.w4 = ldn(a.W[4], pp);
+      s.w8 = ldn(a.W[8], pp);
+      s.wh = ldu(Wh, hh);
+    }
     if (!UM) {
       s.m = ldu(a.mat, pp);
       s.mh = ldu(a.mat, hh);
@@ -1049,12 +1109,14 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   StageStream<R> sa_, sb_;
   sa_.m = sa_.mh = sb_.m = sb_.mh = mu;
   sa_.TO = sb_.TO = (R)0;
+  if (RECT) sa_.vol = sb_.vol = sa_.w0 = sb_.w0 = sa_.w4 = sb_.w4 = sa_.w8 = sb_.w8 = sa_.wh = sb_.wh = (R)0;
   stream(sa_, p, ph);
   __syncthreads();
   R fz_prev = 0;
   if (live && k0 > 1) {
 #pragma clang fp contract(off)
-    fz_prev = flux_z<false>(a, B, ring[sm], ring[sc], x, y, p - sz, ldu(a.W[8], p - sz));
+    fz_prev = flux_z<false>(a, B, ring[sm], ring[sc], x, y, p - sz,
+                            RECT ? (R)(zfac[1][0] * (double)cvol) : ldu(a.W[8], p - sz));
   }
   double rsq = 0.0;
   // The result of a plane is stored during the next plane, after that
@@ -1118,6 +1180,16 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
       stream(nxt, p + step, ph + step);
     }
     if (spend) stn(out, ppend, vpend);  // the previous plane's result
+    if (RECT) {
+      // the plane's geometry from the per-axis factors (same products as the
+      // setup check against the stored metrics)
+      const double azk = zfac[0][k - k0 + 1], fzk = zfac[1][k - k0 + 1];
+      cur.vol = (R)((double)cvol * azk);
+      cur.w0 = (R)((double)cw0 * azk);
+      cur.w4 = (R)((double)cw4 * azk);
+      cur.w8 = (R)(fzk * (double)cvol);
+      cur.wh = (R)((double)cwh * azk);
+    }
     // phase C: fluxes
     R fz = 0;
     {
@@ -1375,6 +1447,7 @@ struct IrsArgs {
   int dtnext;
   double vnn;
   const double* invL2;
+  const double* rect;  // non-null: sum 1/dl^2 from the per-axis factors instead of invL2
   R* dt;
   double* pmin;
 };
@@ -1824,6 +1897,16 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
   const unsigned p1 = (unsigned)(B.base + (ci - 1) + (long)(cj - 1) * B.sy);  // plane 1 of the lane's column
   const unsigned sz = (unsigned)B.sz;
   double rsq = 0.0, dtmin = 1e300;
+  // next-step dt on a rectilinear level: sum 1/dl^2 = (s_x + s_y) + s_z from
+  // the per-axis factors (bit-identical to invL2), the column's s_z in LDS
+  __shared__ double qzs[KCT];
+  const bool rdt = a.dtnext && a.rect;
+  double qxy = 0.0;
+  if (rdt) {
+    const RectAxes rx_ = rect_axes(a.rect, B);
+    if (threadIdx.x < KCT) qzs[threadIdx.x] = rx_.qz[min(k0 + (int)threadIdx.x, n)];
+    qxy = rx_.qx[ci] + rx_.qy[cj];
+  }
   bool inb[XYZ_NLD];
 #pragma unroll
   for (int u = 0; u < XYZ_NLD; u++) inb[u] = xyz_in(B, t.i0, t.j0, u);
@@ -1913,7 +1996,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
           const R fr = Tn - (R)(rw + a.tab.Ti1);
           const R kap = kk.x + fr * kk.y, rho = pp.x + fr * pp.y, cp = pp.z + fr * pp.w;
           const double alpha = (double)kap / (double)rho / (double)cp;
-          dd = 0.5 * a.vnn / (alpha * ldn(a.invL2, po));
+          dd = 0.5 * a.vnn / (alpha * (rdt ? qxy + qzs[ko - k0] : ldn(a.invL2, po)));
           dd = dd < 1e8 ? dd : 1e8;
         }
         stn(a.dt, po, (R)dd);
@@ -2193,6 +2276,12 @@ struct Level {
   double dt_vnn = 0.0;              // ... computed with this vnn
   int all_umat = 0;                 // every block has a single material
   int dims2 = 0;                    // every block has >= 2 cells in each direction
+  // rectilinear levels: per-axis metric factors (RectAxes) that reproduce the
+  // stored volumes and normal weights to rect_dev (relative) and sum 1/dl^2 exactly
+  DBuf<double> rfac;
+  int rect_w = 0;                   // every block rectilinear, factors within RECT_TOL
+  int rect_dt = 0;                  // ... and sum 1/dl^2 bit-identical
+  double rect_dev = -1.0;           // largest relative deviation found (-1: not checked)
   long npad = 0, ncell = 0, nnodes = 0;
   DBuf<BlockInfo> blk;
   DBuf<Tile> tiles, ctiles, xytiles;
@@ -2289,6 +2378,7 @@ struct fuss_ctx {
   int adj_store_on = 0;             // enqueue_step copies each stage input to adj.stage[]
   int graphs_off = 0;               // fuss_run without graph capture (fuss_graphs)
   int stage_lds = 1;                // orthogonal 3-D stages: LDS-ring kernel (1) or row march (0)
+  int rect_off = 0;                 // per-axis metric factors switched off (fuss_rect)
   double kscale = 1.0;              // conductivity table scale (fuss_set_kscale)
   DBuf<int> stage_i;                // staging for fuss_set_bc_values
   DBuf<double> stage_d;
@@ -2339,6 +2429,115 @@ double norm_sign(const HostBlock& h, int dir) {
   for (int x = 0; x < 3; x++) c2[x] = 0.25 * (a[x] + b[x] + c[x] + d[x]);
   const double scal = (c1[0] - c2[0]) * sn[0] + (c1[1] - c2[1]) * sn[1] + (c1[2] - c2[2]) * sn[2];
   return scal >= 0.0 ? 1.0 : -1.0;
+}
+
+// Largest relative deviation of the per-axis factors from the stored
+// volumes and normal weights for which a level uses them (about 4 ulp of
+// rounding-order difference is expected; anything larger keeps the stored
+// arrays).
+constexpr double RECT_TOL = 1e-14;
+
+// Per-axis metric factors (RectAxes) of a rectilinear block, from the same
+// Compute_Metric_Tensor / Compute_Norm_Area arithmetic as the 3-D arrays
+// (cells and faces on the block's first line in each direction).  Returns
+// false when some node is off the block's coordinate planes.
+bool rect_factors(const HostBlock& h, std::vector<double>& out) {
+  const int n[3] = {h.n[0], h.n[1], h.n[2]};
+  auto N = [&](int i, int j, int k) { return &h.nodes[3 * (i + (size_t)(n[0] + 1) * (j + (size_t)(n[1] + 1) * k))]; };
+  for (int k = 0; k <= n[2]; k++)
+    for (int j = 0; j <= n[1]; j++)
+      for (int i = 0; i <= n[0]; i++) {
+        const double* v = N(i, j, k);
+        if (v[0] != N(i, 0, 0)[0] || v[1] != N(0, j, 0)[1] || v[2] != N(0, 0, k)[2]) return false;
+      }
+  for (int d = 0; d < 3; d++) {
+    const int nd = n[d];
+    std::vector<double> a(nd + 2), m(nd + 2), f(nd + 2, 0.0), q(nd + 2);
+    for (int s = 1; s <= nd; s++) {
+      int c[3] = {1, 1, 1};
+      c[d] = s;
+      const int i = c[0], j = c[1], k = c[2];
+      double M[9], dl[3], v;
+      if (metric_tensor(N(i - 1, j - 1, k - 1), N(i - 1, j - 1, k), N(i - 1, j, k - 1), N(i - 1, j, k),
+                        N(i, j - 1, k - 1), N(i, j - 1, k), N(i, j, k - 1), N(i, j, k), M, dl, &v))
+        return false;
+      // edge along d: the node difference the face areas and tetrahedra use
+      // (dl_d sums four such differences in coordinate order, which rounds
+      // at the coordinates' ulp rather than the edge's)
+      a[s] = N(i, j, k)[d] - N(c[0] - (d == 0), c[1] - (d == 1), c[2] - (d == 2))[d];
+      m[s] = M[4 * d];  // M_dd
+      q[s] = 1.0 / (dl[d] * dl[d]);
+    }
+    a[0] = a[1]; a[nd + 1] = a[nd];
+    q[0] = q[1]; q[nd + 1] = q[nd];
+    // the normal's sign on the direction's faces: face_normal of the first
+    // face as k_face_metrics evaluates it (its component d is +-1 exactly)
+    const double* p4[4];
+    if (d == 0) { p4[0] = N(1, 1, 1); p4[1] = N(1, 0, 0); p4[2] = N(1, 0, 1); p4[3] = N(1, 1, 0); }
+    else if (d == 1) { p4[0] = N(0, 1, 1); p4[1] = N(1, 1, 0); p4[2] = N(1, 1, 1); p4[3] = N(0, 1, 0); }
+    else { p4[0] = N(0, 1, 1); p4[1] = N(1, 0, 1); p4[2] = N(0, 0, 1); p4[3] = N(1, 1, 1); }
+    double Nv[3], A;
+    face_normal(p4[0], p4[1], p4[2], p4[3], norm_sign(h, d), Nv, &A);
+    if (!(Nv[d] == 1.0 || Nv[d] == -1.0)) return false;
+    for (int s = 1; s < nd; s++) {
+#pragma clang fp contract(off)
+      f[s] = 0.5 * (m[s] + m[s + 1]) * Nv[d];
+    }
+    out.insert(out.end(), a.begin(), a.end());
+    out.insert(out.end(), f.begin(), f.end());
+    out.insert(out.end(), q.begin(), q.end());
+  }
+  return true;
+}
+
+int upload(void* dst, const void* src, size_t bytes);
+
+// Per-axis factors of a 3-D level whose blocks are all rectilinear, checked
+// on the device against the stored metrics (after k_cell_metrics and
+// k_face_metrics, with L.hblk uploaded); sets L.rect_w / L.rect_dt and the
+// blocks' roff (-1 again when neither check passes).
+int rect_prepare(fuss_ctx* c, Level& L) {
+  L.rect_w = L.rect_dt = 0;
+  L.rect_dev = -1.0;
+  if (c->ndir != 3 || L.kp != GC || L.tang || !L.dims2) return 0;
+  std::vector<double> fac;
+  for (int b = 0; b < L.nb; b++) {
+    const size_t o = fac.size();
+    if (!rect_factors(L.hb[b], fac)) {
+      for (auto& B : L.hblk) B.roff = -1;
+      return 0;
+    }
+    L.hblk[b].roff = (int)o;
+  }
+  if (L.rfac.alloc(fac.size())) return FUSS_EHIP;
+  int e = upload(L.rfac.p, fac.data(), fac.size() * 8);
+  e |= upload(L.blk.p, L.hblk.data(), sizeof(BlockInfo) * L.nb);
+  DBuf<unsigned long long> dev;
+  DBuf<int> nbad;
+  if (e || dev.alloc(1) || nbad.alloc(1)) return FUSS_EHIP;
+  HIPCHK(hipMemsetAsync(dev.p, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(hipMemsetAsync(nbad.p, 0, sizeof(int), c->stream));
+  k_rect_check<<<(unsigned)L.htiles.size(), TILE_T, 0, c->stream>>>(L.blk.p, L.tiles.p, L.rfac.p, L.vol.p, L.invL2.p,
+                                                                    L.W[0].p, L.W[4].p, L.W[8].p, dev.p, nbad.p);
+  HIPCHK(hipGetLastError());
+  unsigned long long hd = 0;
+  int hb = 0;
+  HIPCHK(hipMemcpyAsync(&hd, dev.p, sizeof(hd), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&hb, nbad.p, sizeof(hb), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  dev.free();
+  nbad.free();
+  double dv;
+  memcpy(&dv, &hd, sizeof(dv));
+  L.rect_dev = dv;
+  L.rect_w = dv <= RECT_TOL;
+  L.rect_dt = hb == 0;
+  if (!L.rect_w && !L.rect_dt) {
+    for (auto& B : L.hblk) B.roff = -1;
+    L.rfac.free();
+    if (upload(L.blk.p, L.hblk.data(), sizeof(BlockInfo) * L.nb)) return FUSS_EHIP;
+  }
+  return 0;
 }
 
 // BC_Connect_Metrics, Lib_Metrics.f90:704-936: Mg(1) of a 101 entry from the
@@ -2446,7 +2645,7 @@ int build_level_layout(fuss_ctx* c, Level& L) {
     B.face_active = 0;
     B.conn = 0;
     B.uqv = 0;
-    B.pad_ = 0;
+    B.roff = -1;
     B.tang = 0;
     B.umat = -1;
     off += B.npad;
@@ -2914,6 +3113,8 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
       L.tang |= htang[b];
     }
     HIPCHK(hipMemcpy(L.blk.p, L.hblk.data(), sizeof(BlockInfo) * L.nb, hipMemcpyHostToDevice));
+    e = rect_prepare(c, L);
+    if (e) return e;
     if (L.nbound) {
       k_bc_geometry<<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(L.bc.p, L.nbound, L.blk.p, L.M.p, L.F[0].p,
                                                                   L.F[1].p, L.F[2].p);
@@ -3076,6 +3277,17 @@ void launch_stage_n(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang
   }
 }
 
+template <bool UM, bool RECT, int KCT>
+void launch_stage3p(fuss_ctx* c, unsigned g, const StageArgs<double>& a, bool prim, bool irs) {
+  if (prim) {
+    irs ? k_stage3p<UM, true, true, RECT, double, KCT><<<g, TILE_T, 0, c->stream>>>(a)
+        : k_stage3p<UM, true, false, RECT, double, KCT><<<g, TILE_T, 0, c->stream>>>(a);
+  } else {
+    irs ? k_stage3p<UM, false, true, RECT, double, KCT><<<g, TILE_T, 0, c->stream>>>(a)
+        : k_stage3p<UM, false, false, RECT, double, KCT><<<g, TILE_T, 0, c->stream>>>(a);
+  }
+}
+
 template <int KCT>
 void launch_stage3d_k(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang, bool prim, bool irs) {
   const unsigned g = (unsigned)L.hctiles.size();
@@ -3088,11 +3300,9 @@ void launch_stage3d_k(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool ta
     else { irs ? k_stage3d<false, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
   } else if (!row) {
     if (L.all_umat) {
-      if (prim) { irs ? k_stage3p<true, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3p<true, true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
-      else { irs ? k_stage3p<true, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3p<true, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+      a.rect ? launch_stage3p<true, true, KCT>(c, g, a, prim, irs) : launch_stage3p<true, false, KCT>(c, g, a, prim, irs);
     } else {
-      if (prim) { irs ? k_stage3p<false, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3p<false, true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
-      else { irs ? k_stage3p<false, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3p<false, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+      a.rect ? launch_stage3p<false, true, KCT>(c, g, a, prim, irs) : launch_stage3p<false, false, KCT>(c, g, a, prim, irs);
     }
   } else {
     if (prim) { irs ? k_stage3w<true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
@@ -3307,6 +3517,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.bcf = L.bcf.p; sa.tab = dtab<double>(c);
     sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = halt;
     sa.partials = L.partials.p; sa.status = c->status.p;  // k_bcflux flags table errors through status
+    sa.rect = (L.rect_w && !c->rect_off) ? L.rfac.p : nullptr;
     if (L.any_active && L.nbound) {
       pid = prof_begin(c, FUSS_K_BC);
       if (L.kp == GC) k_bcflux<3, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
@@ -3353,6 +3564,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
         ia.tiles = L.ctiles.p;
         ia.dtnext = fuse_dt && irk == nrk;
         ia.vnn = p->vnn; ia.invL2 = L.invL2.p; ia.dt = L.dt.p; ia.pmin = L.pmin.p;
+        ia.rect = (L.rect_dt && !c->rect_off) ? L.rfac.p : nullptr;
         pid = prof_begin(c, FUSS_K_IRS_UPDATE);
         launch_irs_xyz(c, L, ia, prim);
         prof_end(c, pid);
@@ -3919,6 +4131,22 @@ int fuss_graphs(fuss_ctx* c, int enable) {
   return 0;
 }
 
+int fuss_rect(fuss_ctx* c, int enable) {
+  if (!c) return FUSS_EARG;
+  c->rect_off = !enable;
+  for (auto& L : c->lev) run_graph_free(L);  // captured launches name the kernels
+  return 0;
+}
+
+int fuss_rect_info(fuss_ctx* c, int level, int* weights, int* dt, double* max_rel_dev) {
+  if (!c || !c->setup_done || level < 1 || level > c->nlev) return FUSS_EARG;
+  const Level& L = c->lev[level - 1];
+  if (weights) *weights = L.rect_w && !c->rect_off;
+  if (dt) *dt = L.rect_dt && !c->rect_off;
+  if (max_rel_dev) *max_rel_dev = L.rect_dev;
+  return 0;
+}
+
 int fuss_set_kscale(fuss_ctx* c, double scale) {
   if (c) drop_dt_next(c);
   if (!c || !c->nT || !(scale > 0.0)) return FUSS_EARG;
@@ -4444,7 +4672,7 @@ void fuss_destroy(fuss_ctx* c) {
   for (auto& L : c->lev) {
     run_graph_free(L);
     L.ttx.free(); L.tty.free(); L.ttz.free(); L.tri.free(); L.trioff.free();
-    L.blk.free(); L.tiles.free(); L.nodes.free();
+    L.blk.free(); L.tiles.free(); L.nodes.free(); L.rfac.free();
     for (auto& t : L.Tb) t.free();
     L.R.free(); L.R2.free(); L.dt.free(); L.qvol.free(); L.vol.free(); L.invL2.free(); L.mat.free();
     L.M.free();

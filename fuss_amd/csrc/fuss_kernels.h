@@ -39,12 +39,39 @@ struct BlockInfo {
   int umat;          // material index when the whole block has one material, else -1
   int conn;          // bit f-1 set when face f has a connection (101) entry
   int uqv;           // nonzero when every cell of the block has the same qvol
-  int pad_;
+  int roff;          // rectilinear block: offset of its per-axis metric factors, else -1
 };
 
 struct Tile {
   int b, i0, j0, k0;
 };
+
+// Per-axis metric factors of a rectilinear block (every node lies on the
+// coordinate planes x = x_i, y = y_j, z = z_k).  Nine arrays of n_d + 2
+// values start at BlockInfo::roff: (a, f, s) for d = x, y, z, indexed by the
+// cell index q = 0 .. n_d + 1 (0 and n_d + 1 repeat the end cells):
+//   a_d(q)  edge length of the cells of index q along d (node difference
+//           x_q - x_{q-1}, as the face areas and tetrahedra use it),
+//   f_d(q)  normal coefficient of the face between q and q+1,
+//           0.5 (M_dd(q) + M_dd(q+1)) N_d (0 on the block's faces),
+//   s_d(q)  1 / dl_d^2 (dl_d of Compute_Metric_Tensor depends on q alone).
+// Cell volume (a_x a_y) a_z, normal flux weights (f_x a_y) a_z,
+// (f_y a_x) a_z and f_z (a_x a_y), sum of 1/dl^2 (s_x + s_y) + s_z.
+struct RectAxes {
+  const double *ax, *fx, *qx, *ay, *fy, *qy, *az, *fz, *qz;  // q: 1/dl^2
+};
+
+__host__ __device__ inline RectAxes rect_axes(const double* rect, const BlockInfo& B) {
+  const long nx = B.ni + 2, ny = B.nj + 2, nz = B.nk + 2;
+  const double* p = rect + B.roff;
+  RectAxes r;
+  r.ax = p; r.fx = p + nx; r.qx = p + 2 * nx;
+  p += 3 * nx;
+  r.ay = p; r.fy = p + ny; r.qy = p + 2 * ny;
+  p += 3 * ny;
+  r.az = p; r.fz = p + nz; r.qz = p + 2 * nz;
+  return r;
+}
 
 // Time series of 301/302 values on the device (one concatenated table).
 struct SerDev {

@@ -97,6 +97,8 @@ def load():
     L.fuss_get_stream.argtypes = [P, C.POINTER(P)]
     L.fuss_set_stream.argtypes = [P, P]
     L.fuss_memory.argtypes = [P, C.POINTER(C.c_double)]
+    L.fuss_rect.argtypes = [P, C.c_int]
+    L.fuss_rect_info.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.fuss_graphs.argtypes = [P, C.c_int]
     L.fuss_autotune.argtypes = [P, C.c_int, C.POINTER(StepParams), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                 C.POINTER(C.c_double)]
@@ -177,6 +179,18 @@ class Solver:
     def graphs(self, enable):
         """fuss_run on captured HIP graphs (default) or step by step."""
         _chk(self.L.fuss_graphs(self.h, int(bool(enable))))
+
+    def rect(self, enable):
+        """Per-axis metric factors on rectilinear levels (default) or the
+        stored 3-D metric arrays (fuss_rect)."""
+        _chk(self.L.fuss_rect(self.h, int(bool(enable))), "fuss_rect")
+
+    def rect_info(self, level=1):
+        """(weights from factors, dt from factors, largest relative deviation
+        of the factors from the stored metrics; -1 when not rectilinear)."""
+        w, d, dev = C.c_int(0), C.c_int(0), C.c_double(0)
+        _chk(self.L.fuss_rect_info(self.h, level, C.byref(w), C.byref(d), C.byref(dev)), "fuss_rect_info")
+        return bool(w.value), bool(d.value), dev.value
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:

@@ -14,14 +14,15 @@ variables does not read the stage state (stage_state=False).
 from __future__ import annotations
 
 
-def stage_bytes(ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True, qvol=True):
+def stage_bytes(ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True, qvol=True, rect=False):
     """Fused flux + BC + newstate kernel, per cell.
 
     reads: T, dt, qvol, vol, material, own face weights (ndir normal weights,
     or 3 per face when tangential weights are present); writes the scaled
-    residual (IRS on) or reads T^n and writes T (IRS off)."""
-    nw = ndir * (3 if tang else 1)
-    b = fp * (1 + 1 + (1 if qvol else 0) + 1 + nw) + (4 if mat else 0)
+    residual (IRS on) or reads T^n and writes T (IRS off).  rect: volume and
+    weights are products of per-axis factors (no per-cell geometry read)."""
+    nw = 0 if rect else ndir * (3 if tang else 1)
+    b = fp * (1 + 1 + (1 if qvol else 0) + (0 if rect else 1) + nw) + (4 if mat else 0)
     if irs:
         b += fp            # write R
     else:
@@ -49,9 +50,13 @@ def dt_bytes(fp=8, mat=True):
     return fp + (4 if mat else 0) + 8 + fp
 
 
-def iteration_bytes(n_rk=3, ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True, fused_irs=False, qvol=True):
-    b = dt_bytes(fp, mat)
-    per_stage = stage_bytes(ndir, tang, irs, prim, fp, mat, qvol)
+def iteration_bytes(n_rk=3, ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True, fused_irs=False, qvol=True,
+                    rect=False, fused_dt=False):
+    """Bytes per cell of one Explicit_Step.  fused_dt: the last stage's IRS
+    pass computes the next step's dt from the new state (it writes dt and,
+    unless rect, reads sum 1/dl^2) instead of a separate pass over T."""
+    b = (fp + (0 if rect else 8)) if fused_dt else dt_bytes(fp, mat)
+    per_stage = stage_bytes(ndir, tang, irs, prim, fp, mat, qvol, rect)
     if irs:
         if fused_irs:
             per_stage += irs_fused_bytes(fp, mat, stage_state=not (prim and n_rk == 4))
@@ -64,3 +69,12 @@ def cmg_stage_bytes(fp=4):
     """One stage of the correction smoother (no IRS): read d, d_0, b, the
     cell's three low-face conductances and the BC diagonal, write d."""
     return fp * 8
+
+
+def canonical_bytes(n_rk=4, irs=True, ndir=3):
+    """Per cell-update bytes of a straightforward FP64 implementation that
+    streams every operand per stage: T, T^n, T written, nine face-geometry
+    coefficients (three per owned face), 1/V, dt and qvol = 120 B per stage;
+    T, three edge lengths read and dt written = 40 B per step; and with IRS
+    16 B (R read and written) per direction and stage."""
+    return n_rk * 120 + 40 + (ndir * 16 * n_rk if irs else 0)

@@ -125,6 +125,21 @@ int fuss_run(fuss_ctx *ctx, int level, const fuss_step_params *p, int max_iters,
  * graphs are switched off here (enable = 0).  Default: on. */
 int fuss_graphs(fuss_ctx *ctx, int enable);
 
+/* Rectilinear levels (3-D, every node of every block on its coordinate
+ * planes x = x_i, y = y_j, z = z_k, no tangential weights): the 3-D stage
+ * pass forms cell volumes and normal face weights as products of per-axis
+ * factors instead of streaming the stored arrays, and the fused next-step dt
+ * forms sum 1/dl^2 from them.  No reference counterpart (the reference reads
+ * its 3-D metric arrays, Lib_Diffusive.f90:34-40, Mod_dt.f90:26-84): a storage
+ * choice of this library.  fuss_setup checks the factors against the stored
+ * metrics on the device and uses them only when every volume and weight
+ * agrees to 1e-14 relative (sum 1/dl^2: bit for bit).  Default: on;
+ * enable = 0 streams the stored arrays. */
+int fuss_rect(fuss_ctx *ctx, int enable);
+/* Whether `level` uses the factors for weights / dt, and the largest relative
+ * deviation found by the setup check (-1: level not rectilinear). */
+int fuss_rect_info(fuss_ctx *ctx, int level, int *weights, int *dt, double *max_rel_dev);
+
 /* Tile autotuner: times `steps` Explicit_Steps with each candidate of the
  * 3-D stage pass -- column height 4, 8 or 16 planes x LDS-ring or row-march
  * kernel (row march: orthogonal levels only) -- on the level's current state,

@@ -242,6 +242,50 @@ def test_set_T_between_steps(gpu, oracle_mod):
     s.close()
 
 
+def test_rect_factors_active(gpu):
+    """The setup check accepts the per-axis factors on a stretched orthogonal
+    cube (every level) and rejects them on the perturbed lattice."""
+    c = SYNTH["cube3d_mg3_irs"]()
+    s = gpu.Solver(c)
+    for lv in range(1, c.params.mgl + 1):
+        w, d, dev = s.rect_info(lv)
+        assert w and d and 0.0 <= dev <= 1e-14, (lv, w, d, dev)
+    s.rect(False)
+    assert s.rect_info(1)[:2] == (False, False)
+    s.close()
+    c = SYNTH["lattice_transient"]()
+    s = gpu.Solver(c)
+    w, d, dev = s.rect_info(1)
+    assert not w and not d and dev == -1.0
+    s.close()
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_rect_vs_stored_metrics(gpu, oracle_mod, graphs):
+    """Stage passes with the per-axis factors and with the stored 3-D metric
+    arrays agree to rounding (the factors differ from the stored weights by a
+    few ulp), and both meet the oracle's field to 1e-12 after 40 steps."""
+    c = SYNTH["cube3d_rk4_irs"]()
+    o = oracle_mod.Oracle(c)
+    for _ in range(40):
+        o.step(1)
+    out = []
+    for rect in (True, False):
+        s = gpu.Solver(c)
+        s.rect(rect)
+        s.graphs(graphs)
+        if graphs:
+            s.run_level(1, 40, 40, -1.0, 1e300)
+        else:
+            for _ in range(40):
+                s.explicit_step(1)
+        out.append(s.get("T"))
+        s.close()
+    assert relerr(out[0], out[1]) < 1e-13
+    for T in out:
+        assert relerr(T, o.get("T")) < RTOL
+
+
 @pytest.mark.parametrize("where", ["wall", "interior", "below", "all_huge"])
 @pytest.mark.parametrize("graphs", [False, True])
 def test_table_range_error(gpu, oracle_mod, where, graphs):
