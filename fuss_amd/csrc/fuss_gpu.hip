@@ -1535,6 +1535,36 @@ __device__ __forceinline__ R jacobi2(R rm2, R rm1, R r0, R rp1, R rp2, int s, in
   return irs_div(r0 + beta * (sm + sp), den, inv);
 }
 
+// The same two sweeps with the division chosen at compile time (P2: den is
+// a power of two and x * inv is the IEEE quotient), and jacobi2_in for a
+// cell whose neighbours at distance 1 and 2 lie inside the line
+// (3 <= s <= n-2), where jacobi2 selects no ghost: the same operations.
+template <bool P2, typename R>
+__device__ __forceinline__ R irs_q(R x, R den, R inv) {
+  return P2 ? x * inv : x / den;
+}
+
+template <bool P2, typename R>
+__device__ __forceinline__ R jacobi2_p(R rm2, R rm1, R r0, R rp1, R rp2, int s, int n, R glo, R ghi, R beta, R den,
+                                       R inv) {
+#pragma clang fp contract(off)
+  const R am1 = s - 1 >= 1 ? rm1 : glo;
+  const R ap1 = s + 1 <= n ? rp1 : ghi;
+  const R am2 = s - 2 >= 1 ? rm2 : glo;
+  const R ap2 = s + 2 <= n ? rp2 : ghi;
+  const R sm = (s - 1 < 1) ? glo : irs_q<P2>(am1 + beta * (am2 + r0), den, inv);
+  const R sp = (s + 1 > n) ? ghi : irs_q<P2>(ap1 + beta * (r0 + ap2), den, inv);
+  return irs_q<P2>(r0 + beta * (sm + sp), den, inv);
+}
+
+template <bool P2, typename R>
+__device__ __forceinline__ R jacobi2_in(R rm2, R rm1, R r0, R rp1, R rp2, R beta, R den, R inv) {
+#pragma clang fp contract(off)
+  const R sm = irs_q<P2>(rm1 + beta * (rm2 + r0), den, inv);
+  const R sp = irs_q<P2>(rp1 + beta * (r0 + rp2), den, inv);
+  return irs_q<P2>(r0 + beta * (sm + sp), den, inv);
+}
+
 // IRS ghost of the line end on face f (Ghost_Residual_Extrapolation,
 // Lib_IRS.f90:170-184): the donor's R for a connection, else the boundary
 // cell's own R (zero gradient).
@@ -1875,7 +1905,7 @@ __device__ __forceinline__ void xyz_fetch(XyzPlane<R>& r, const R* __restrict__ 
   }
 }
 
-template <bool PRIM, bool XCON, typename R, int KCT>
+template <bool PRIM, bool XCON, bool P2, typename R, int KCT>
 __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
   if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -1910,6 +1940,11 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
   bool inb[XYZ_NLD];
 #pragma unroll
   for (int u = 0; u < XYZ_NLD; u++) inb[u] = xyz_in(B, t.i0, t.j0, u);
+  // the tile's x lines and the wave's y line (one row j per wave) away from
+  // the block faces: no ghost selection
+  const bool xin = t.i0 >= 3 && t.i0 + TILE_X - 1 <= B.ni - 2;
+  const int ju = t.j0 + __builtin_amdgcn_readfirstlane(ty);
+  const bool yin = ju >= 3 && ju <= B.nj - 2;
   XyzPlane<R> pa, pb;
   xyz_fetch(pa, a.Rin, B, t.i0, t.j0, qa);
   xyz_fetch(pb, a.Rin, B, t.i0, t.j0, min(qa + 1, qb));
@@ -1937,22 +1972,28 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
     xyz_fetch(pre, a.Rin, B, t.i0, t.j0, min(q + 2, qb));
     if (q <= qb) {
       __syncthreads();
-      // x sweeps of rows j0-2 .. j0+5 (two per thread)
+      // x sweeps of rows j0-2 .. j0+5 (two per thread); a tile whose cells
+      // are all >= 2 cells from the i faces selects no ghost (uniform branch)
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         const int jj = ty + h * TILE_Y, jr = t.j0 - 2 + jj;
         if (i <= B.ni && jr >= 1 && jr <= B.nj) {
-          R glo = 0, ghi = 0;
-          if (i <= 2) {
-            glo = rin[jj][tx + 2 - (i - 1)];
-            if (XCON) glo = irs_ghost(a, B, 1, 1, jr, q, glo);
+          if (xin) {
+            xs[jj][tx] = jacobi2_in<P2>(rin[jj][tx], rin[jj][tx + 1], rin[jj][tx + 2], rin[jj][tx + 3],
+                                        rin[jj][tx + 4], beta, den, inv);
+          } else {
+            R glo = 0, ghi = 0;
+            if (i <= 2) {
+              glo = rin[jj][tx + 2 - (i - 1)];
+              if (XCON) glo = irs_ghost(a, B, 1, 1, jr, q, glo);
+            }
+            if (i >= B.ni - 1) {
+              ghi = rin[jj][tx + 2 + (B.ni - i)];
+              if (XCON) ghi = irs_ghost(a, B, 2, B.ni, jr, q, ghi);
+            }
+            xs[jj][tx] = jacobi2_p<P2>(rin[jj][tx], rin[jj][tx + 1], rin[jj][tx + 2], rin[jj][tx + 3],
+                                       rin[jj][tx + 4], i, B.ni, glo, ghi, beta, den, inv);
           }
-          if (i >= B.ni - 1) {
-            ghi = rin[jj][tx + 2 + (B.ni - i)];
-            if (XCON) ghi = irs_ghost(a, B, 2, B.ni, jr, q, ghi);
-          }
-          xs[jj][tx] = jacobi2(rin[jj][tx], rin[jj][tx + 1], rin[jj][tx + 2], rin[jj][tx + 3], rin[jj][tx + 4], i,
-                               B.ni, glo, ghi, beta, den, inv);
         }
       }
       __syncthreads();
@@ -1961,17 +2002,23 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
       // barrier (xs no longer read)
       if (live) {
         const int jj = ty + 2;
-        const R glo = (j <= 2) ? xs[jj - (j - 1)][tx] : (R)0;
-        const R ghi = (j >= B.nj - 1) ? xs[jj + (B.nj - j)][tx] : (R)0;
-        s = jacobi2(xs[jj - 2][tx], xs[jj - 1][tx], xs[jj][tx], xs[jj + 1][tx], xs[jj + 2][tx], j, B.nj, glo, ghi,
-                    beta, den, inv);
+        if (yin) {
+          s = jacobi2_in<P2>(xs[jj - 2][tx], xs[jj - 1][tx], xs[jj][tx], xs[jj + 1][tx], xs[jj + 2][tx], beta, den,
+                             inv);
+        } else {
+          const R glo = (j <= 2) ? xs[jj - (j - 1)][tx] : (R)0;
+          const R ghi = (j >= B.nj - 1) ? xs[jj + (B.nj - j)][tx] : (R)0;
+          s = jacobi2_p<P2>(xs[jj - 2][tx], xs[jj - 1][tx], xs[jj][tx], xs[jj + 1][tx], xs[jj + 2][tx], j, B.nj,
+                            glo, ghi, beta, den, inv);
+        }
       }
       if (q == 1) zlo = s;
       if (q == n) zhi = s;
     }
     w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = s;
     if (out && live) {
-      const R rs = jacobi2(w0, w1, w2, w3, w4, ko, n, zlo, zhi, beta, den, inv);
+      const R rs = (ko >= 3 && ko <= n - 2) ? jacobi2_in<P2>(w0, w1, w2, w3, w4, beta, den, inv)
+                                            : jacobi2_p<P2>(w0, w1, w2, w3, w4, ko, n, zlo, zhi, beta, den, inv);
       int bad = 0;
       const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
       stn(a.Tout, po, Tn);
@@ -3323,17 +3370,20 @@ void launch_irs_z(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
   else prim ? k_irs_z<true, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
 }
 
-template <bool XCON>
+template <bool XCON, bool P2>
 void launch_irs_xyz_x(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
   const unsigned g = (unsigned)L.hctiles.size();
-  if (L.kc == 4) prim ? k_irs_xyz<true, XCON, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
-  else if (L.kc == 16) prim ? k_irs_xyz<true, XCON, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
-  else prim ? k_irs_xyz<true, XCON, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
+  if (L.kc == 4) prim ? k_irs_xyz<true, XCON, P2, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
+  else if (L.kc == 16) prim ? k_irs_xyz<true, XCON, P2, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
+  else prim ? k_irs_xyz<true, XCON, P2, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
 }
 
+// P2: 1 + 2 beta is a power of two (beta = 0.5: 2), the sweeps multiply by
+// its exact reciprocal
 void launch_irs_xyz(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
-  if (L.no_xcon) launch_irs_xyz_x<false>(c, L, ia, prim);
-  else launch_irs_xyz_x<true>(c, L, ia, prim);
+  const bool p2 = ia.inv != 0.0;
+  if (L.no_xcon) p2 ? launch_irs_xyz_x<false, true>(c, L, ia, prim) : launch_irs_xyz_x<false, false>(c, L, ia, prim);
+  else p2 ? launch_irs_xyz_x<true, true>(c, L, ia, prim) : launch_irs_xyz_x<true, false>(c, L, ia, prim);
 }
 
 template <int DIR, int NDIR>
