@@ -1905,7 +1905,7 @@ __device__ __forceinline__ void xyz_fetch(XyzPlane<R>& r, const R* __restrict__ 
   }
 }
 
-template <bool PRIM, bool XCON, bool P2, typename R, int KCT>
+template <bool PRIM, bool XCON, bool P2, bool DT, typename R, int KCT>
 __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
   if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -1930,7 +1930,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
   // next-step dt on a rectilinear level: sum 1/dl^2 = (s_x + s_y) + s_z from
   // the per-axis factors (bit-identical to invL2), the column's s_z in LDS
   __shared__ double qzs[KCT];
-  const bool rdt = a.dtnext && a.rect;
+  const bool rdt = DT && a.rect;
   double qxy = 0.0;
   if (rdt) {
     const RectAxes rx_ = rect_axes(a.rect, B);
@@ -2026,9 +2026,10 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
       if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, ko);
       const double d = (double)Tn - (double)TO;
       rsq += d * d;
-      if (a.dtnext) {
+      if (DT) {
         // Compute_dt (Mod_dt.f90:26-84) at T^{n+1}, without the vnn ramp
-        // (the host fuses only when it cannot apply)
+        // (the host fuses only when it cannot apply); a separate instance of
+        // the kernel, so the other stages keep the registers it needs
         int bad3 = 0;
         using V2 = typename Vec2<R>::T;
         using V4 = typename Vec4<R>::T;
@@ -2057,7 +2058,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
     if (q + 1 <= qe) step(pb, q + 1);
   }
   if (a.last) block_sum(rsq, a.partials, wg);
-  if (a.dtnext) block_min(dtmin, a.pmin, wg);
+  if (DT) block_min(dtmin, a.pmin, wg);
 }
 
 // ------------------------------------------------------------------------
@@ -3370,20 +3371,25 @@ void launch_irs_z(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
   else prim ? k_irs_z<true, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
 }
 
-template <bool XCON, bool P2>
+template <bool XCON, bool P2, bool DT>
 void launch_irs_xyz_x(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
   const unsigned g = (unsigned)L.hctiles.size();
-  if (L.kc == 4) prim ? k_irs_xyz<true, XCON, P2, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
-  else if (L.kc == 16) prim ? k_irs_xyz<true, XCON, P2, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
-  else prim ? k_irs_xyz<true, XCON, P2, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
+  if (L.kc == 4) prim ? k_irs_xyz<true, XCON, P2, DT, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
+  else if (L.kc == 16) prim ? k_irs_xyz<true, XCON, P2, DT, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
+  else prim ? k_irs_xyz<true, XCON, P2, DT, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
+}
+
+template <bool XCON, bool P2>
+void launch_irs_xyz_p(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
+  ia.dtnext ? launch_irs_xyz_x<XCON, P2, true>(c, L, ia, prim) : launch_irs_xyz_x<XCON, P2, false>(c, L, ia, prim);
 }
 
 // P2: 1 + 2 beta is a power of two (beta = 0.5: 2), the sweeps multiply by
-// its exact reciprocal
+// its exact reciprocal; DT: the pass also computes the next step's dt
 void launch_irs_xyz(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
   const bool p2 = ia.inv != 0.0;
-  if (L.no_xcon) p2 ? launch_irs_xyz_x<false, true>(c, L, ia, prim) : launch_irs_xyz_x<false, false>(c, L, ia, prim);
-  else p2 ? launch_irs_xyz_x<true, true>(c, L, ia, prim) : launch_irs_xyz_x<true, false>(c, L, ia, prim);
+  if (L.no_xcon) p2 ? launch_irs_xyz_p<false, true>(c, L, ia, prim) : launch_irs_xyz_p<false, false>(c, L, ia, prim);
+  else p2 ? launch_irs_xyz_p<true, true>(c, L, ia, prim) : launch_irs_xyz_p<true, false>(c, L, ia, prim);
 }
 
 template <int DIR, int NDIR>
