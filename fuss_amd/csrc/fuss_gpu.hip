@@ -1170,7 +1170,10 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
         rh = tab_row(Th, Ti1, Ti2, &b3);
         kh = ldu(krh, (unsigned)(rh - Ti1));
       }
-      bad = (live && ((hx && b0) || (hy && b1) || (k < B.nk && b2))) || ((yhalo || xhalo) && b3);
+      // primitive variables: every cell's own row is checked (badc), and a face
+      // average lies between its two cells' temperatures, so a face outside
+      // the table implies a cell outside it -- the face tests are redundant
+      bad = PRIM ? 0 : (live && ((hx && b0) || (hy && b1) || (k < B.nk && b2))) || ((yhalo || xhalo) && b3);
       badc = live && badc;
     }
     // phase B: the next plane's loads (the last plane re-reads its own cells)
@@ -2082,6 +2085,37 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ in, i
     __syncthreads();
   }
   if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
+}
+
+// Both passes of k_reduce<false> in one work-group, for up to RED_FUSE
+// chunks: each chunk is summed with pass 1's thread partition and tree, the
+// chunk sums then with pass 2's (bit-identical to the two launches).
+constexpr int RED_FUSE = 8;
+__global__ __launch_bounds__(256) void k_reduce_sum1(const double* __restrict__ in, int n, double* __restrict__ out,
+                                                     const int* __restrict__ halt) {
+  if (halt && *halt) return;
+  __shared__ double sm[256];
+  __shared__ double cs[256];
+  cs[threadIdx.x] = 0.0;
+  const int nb = (n + RED_CHUNK - 1) / RED_CHUNK;
+  for (int b = 0; b < nb; b++) {
+    const int lo = b * RED_CHUNK, hi = min(n, lo + RED_CHUNK);
+    double v = 0.0;
+    for (int q = lo + threadIdx.x; q < hi; q += 256) v = v + in[q];
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) sm[threadIdx.x] = sm[threadIdx.x] + sm[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) cs[b] = 0.0 + sm[0];  // pass 2 starts each thread's sum at 0
+    __syncthreads();
+  }
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) cs[threadIdx.x] = cs[threadIdx.x] + cs[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = cs[0];
 }
 
 // Pass 2 of the dt minimum: dtglobal = min(1e5, min dt) (Mod_Explicit.f90:40,
@@ -3542,7 +3576,9 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   }
   // device scalars: [0] dtglobal, [1] time, [2] Sum (T-TO)^2
   pid = prof_begin(c, FUSS_K_REDUCE);
-  if (nmin > (unsigned)RED_CHUNK) {
+  // the minimum does not depend on the order: k_dt_final reads up to
+  // RED_FUSE chunks of per-tile minima itself
+  if (nmin > (unsigned)(RED_FUSE * RED_CHUNK)) {
     const int nb1 = (int)((nmin + RED_CHUNK - 1) / RED_CHUNK);
     k_reduce<true><<<nb1, 256, 0, c->stream>>>(L.pmin.p, (int)nmin, L.red2.p, halt);
     k_dt_final<<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p, L.clk.p, L.cnt.p, p->time_accurate);
@@ -3683,8 +3719,12 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   if (p->compute_residual) {
     pid = prof_begin(c, FUSS_K_REDUCE);
     const int nb1 = (npart + RED_CHUNK - 1) / RED_CHUNK;
-    k_reduce<false><<<nb1, 256, 0, c->stream>>>(L.partials.p, npart, L.red2.p, halt);
-    k_reduce<false><<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p + 2, halt);
+    if (nb1 <= RED_FUSE) {
+      k_reduce_sum1<<<1, 256, 0, c->stream>>>(L.partials.p, npart, c->scal.p + 2, halt);
+    } else {
+      k_reduce<false><<<nb1, 256, 0, c->stream>>>(L.partials.p, npart, L.red2.p, halt);
+      k_reduce<false><<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p + 2, halt);
+    }
     prof_end(c, pid);
     HIPCHK(hipGetLastError());
   }

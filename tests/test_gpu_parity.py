@@ -112,6 +112,9 @@ SYNTH = {
     # four-stage (Jameson) extension: the cube with IRS, and the transient
     # lattice in conservative variables
     "cube3d_rk4_irs": lambda: cases.cube3d(24, mgl=1, n_rk=4),
+    # three independent rectilinear blocks of different sizes and materials
+    # (per-axis metric factors per block), four-stage RK with IRS
+    "sweep3_rk4_irs": lambda: cases.sweep(ncase=3, lo=12, hi=28, n_rk=4, mgl=1, seed=7),
     "lattice_rk4_cons": lambda: cases.lattice(nl=2, lo=4, hi=8, n_rk=4, prim=False),
     # line-tridiagonal IRS extension (FUSS_IRS_TRIDIAG): single block, the
     # connected lattice (donor ghosts between directions, 2 levels), uneven
