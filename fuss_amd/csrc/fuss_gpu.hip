@@ -316,6 +316,11 @@ struct StageArgs {
   double* partials;
   int* status;
   const double* rect;  // per-axis metric factors (rectilinear levels, k_stage3p<RECT>)
+  // k_stage3p<DTC>: the first stage of a steady step computes the local dt
+  // from its input (T^n) and stores it for the later stages
+  double vnn;
+  R* dtw;
+  double* pmin;        // per-tile dt minima
 };
 
 // One interior face flux between cells q and q+e_D (left cell q), in the
@@ -1010,7 +1015,7 @@ __device__ __forceinline__ void fetch_plane_u(PlaneRegs<R>& r, const R* __restri
   }
 }
 
-template <bool UM, bool PRIM, bool IRSMODE, bool RECT, typename R, int KCT>
+template <bool UM, bool PRIM, bool IRSMODE, bool RECT, bool DTC, typename R, int KCT>
 __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
@@ -1045,16 +1050,19 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   // parts staged in LDS (a global load inside the plane loop would make the
   // in-order wait for it also wait for the next plane's prefetch)
   R cvol = 0, cw0 = 0, cw4 = 0, cwh = 0;
-  __shared__ double zfac[2][KCT + 1];  // a_z, f_z of planes k0-1 .. k0+KCT-1
+  // a_z, f_z (and 1/dl_z^2 for DTC) of planes k0-1 .. k0+KCT-1
+  __shared__ double zfac[DTC ? 3 : 2][KCT + 1];
+  double qxy = 0.0;  // DTC: 1/dl_x^2 + 1/dl_y^2 of the lane's column
   if (RECT) {
     const RectAxes rx_ = rect_axes(a.rect, B);
-    if (threadIdx.x < 2 * (KCT + 1)) {
-      const int q = threadIdx.x % (KCT + 1), kk = min(k0 - 1 + q, B.nk + 1);
-      zfac[threadIdx.x / (KCT + 1)][q] = threadIdx.x < KCT + 1 ? rx_.az[kk] : rx_.fz[kk];
+    if (threadIdx.x < (DTC ? 3 : 2) * (KCT + 1)) {
+      const int q = threadIdx.x % (KCT + 1), kk = min(k0 - 1 + q, B.nk + 1), row = threadIdx.x / (KCT + 1);
+      zfac[row][q] = row == 0 ? rx_.az[kk] : (row == 1 ? rx_.fz[kk] : rx_.qz[kk]);
     }
     const int ci_ = live ? i : 1, cj_ = live ? j : 1;
     cvol = (R)(rx_.ax[ci_] * rx_.ay[cj_]);
     cw0 = (R)(rx_.fx[ci_] * rx_.ay[cj_]);
+    if (DTC) qxy = rx_.qx[ci_] + rx_.qy[cj_];
     cw4 = (R)(rx_.fy[cj_] * rx_.ax[ci_]);
     // the halo face: +y face of (i, j0-1) or +x face of (i0-1, j0+hr)
     cwh = yhalo ? (R)(rx_.fy[t.j0 - 1] * rx_.ax[i]) : (xhalo ? (R)(rx_.fx[t.i0 - 1] * rx_.ay[t.j0 + hr]) : (R)0);
@@ -1092,7 +1100,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   const unsigned quni = (unsigned)B.base;
   auto stream = [&](StageStream<R>& s, unsigned pp, unsigned hh) {
     s.qv = ldu(a.qvol, B.uqv ? quni : pp);
-    s.dt = ldn(a.dt, pp);
+    if (!DTC) s.dt = ldn(a.dt, pp);
     if (!IRSMODE) s.TO = ldn(a.TO, pp);
     if (!RECT) {
       s.vol = ldn(a.vol, pp);
@@ -1109,6 +1117,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   StageStream<R> sa_, sb_;
   sa_.m = sa_.mh = sb_.m = sb_.mh = mu;
   sa_.TO = sb_.TO = (R)0;
+  if (DTC) sa_.dt = sb_.dt = (R)0;
   if (RECT) sa_.vol = sb_.vol = sa_.w0 = sb_.w0 = sa_.w4 = sb_.w4 = sa_.w8 = sb_.w8 = sa_.wh = sb_.wh = (R)0;
   stream(sa_, p, ph);
   __syncthreads();
@@ -1124,9 +1133,10 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   // loads, so a store issued just before the table loads would make the
   // wait for the tables wait for the store's completion too.
   R* const out = IRSMODE ? a.Rres : a.Tout;
-  R vpend = 0;
+  R vpend = 0, dtpend = 0;
   unsigned ppend = p;
   bool spend = false;
+  double dtmin = 1e300;
   // one plane: cur holds its streams, nxt receives those of the next plane
   auto plane = [&](StageStream<R>& cur, StageStream<R>& nxt, int k) {
     store_plane(ring[sp], pre);
@@ -1138,7 +1148,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
     const R Txr = ring[sc][y][x + 1], Tyr = ring[sc][y + 1][x], Tzr = ring[sp][y][x];
     R Tx, Ty, Tz, Th = 0, Tcl = Tc;
     int rx, ry, rz, rh = Ti1, rc = Ti1;
-    V2 kx, ky, kz, kh{};
+    V2 kx, ky, kz, kh{}, kc{};
     V4 pc{};
     // BC values of the lane's boundary faces, issued with the table rows;
     // only tiles on an i face, rows on a j face and the end planes load them
@@ -1163,6 +1173,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
       if (PRIM) {
         rc = tab_row(Tcl, Ti1, Ti2, &badc);
         pc = ldu(UM ? pr_u : pr0 + (long)cur.m * nT, (unsigned)(rc - Ti1));
+        if (DTC) kc = ldu(kr, (unsigned)(rc - Ti1));  // kappa at the cell for its dt
       }
       if (ty < 2) {  // waves 0 and 1 hold the halo lanes (a wave-uniform test)
         const V2* krh = UM ? kap_u : kap0 + (long)cur.mh * nT;
@@ -1182,7 +1193,10 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
       fetch_plane_u(pre, a.T, B, t.i0, t.j0, k < k1 ? k + 2 : k + 1);
       stream(nxt, p + step, ph + step);
     }
-    if (spend) stn(out, ppend, vpend);  // the previous plane's result
+    if (spend) {  // the previous plane's result
+      stn(out, ppend, vpend);
+      if (DTC) stn(a.dtw, ppend, dtpend);
+    }
     if (RECT) {
       // the plane's geometry from the per-axis factors (same products as the
       // setup check against the stored metrics)
@@ -1225,7 +1239,21 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
         if (yo >= 0) Rv = Rv - by;
         if (zo_of(k) >= 0) Rv = Rv - bz;
       }
-      const R dt = a.use_dtg ? dtg : cur.dt;
+      R dt = a.use_dtg ? dtg : cur.dt;
+      if (DTC) {
+        // Compute_dt (Mod_dt.f90:26-84) at T^n, the first stage's input: the
+        // kappa, rho, cp interpolations of k_dt at the cell's row (rho and cp
+        // are the residual's below), sum 1/dl^2 from the per-axis factors
+#pragma clang fp contract(off)
+        const R fr = Tcl - (R)rc;
+        const R kap = kc.x + fr * kc.y, rho = pc.x + fr * pc.y, cp = pc.z + fr * pc.w;
+        const double alpha = (double)kap / (double)rho / (double)cp;
+        double dd = 0.5 * a.vnn / (alpha * (qxy + zfac[2][k - k0 + 1]));
+        dd = dd < 1e8 ? dd : 1e8;
+        dt = (R)dd;
+        dtpend = dt;
+        dtmin = fmin(dtmin, dd);
+      }
       R res;
       {
 #pragma clang fp contract(off)
@@ -1266,8 +1294,12 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
     plane(sa_, sb_, k);
     if (k + 1 <= k1) plane(sb_, sa_, k + 1);
   }
-  if (spend) stn(out, ppend, vpend);
+  if (spend) {
+    stn(out, ppend, vpend);
+    if (DTC) stn(a.dtw, ppend, dtpend);
+  }
   if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
+  if (DTC) block_min(dtmin, a.pmin, wg);
 }
 
 // ------------------------------------------------------------------------
@@ -3361,12 +3393,15 @@ void launch_stage_n(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang
 
 template <bool UM, bool RECT, int KCT>
 void launch_stage3p(fuss_ctx* c, unsigned g, const StageArgs<double>& a, bool prim, bool irs) {
-  if (prim) {
-    irs ? k_stage3p<UM, true, true, RECT, double, KCT><<<g, TILE_T, 0, c->stream>>>(a)
-        : k_stage3p<UM, true, false, RECT, double, KCT><<<g, TILE_T, 0, c->stream>>>(a);
+  if (RECT && prim && a.dtw) {  // first stage of a steady step: dt here (rectilinear, primitive)
+    irs ? k_stage3p<UM, true, true, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a)
+        : k_stage3p<UM, true, false, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a);
+  } else if (prim) {
+    irs ? k_stage3p<UM, true, true, RECT, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a)
+        : k_stage3p<UM, true, false, RECT, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a);
   } else {
-    irs ? k_stage3p<UM, false, true, RECT, double, KCT><<<g, TILE_T, 0, c->stream>>>(a)
-        : k_stage3p<UM, false, false, RECT, double, KCT><<<g, TILE_T, 0, c->stream>>>(a);
+    irs ? k_stage3p<UM, false, true, RECT, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a)
+        : k_stage3p<UM, false, false, RECT, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a);
   }
 }
 
@@ -3556,12 +3591,18 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   // The fused x-y-z IRS pass of the last stage can leave the next step's
   // local dt (and its per-tile minima) behind: without a vnn ramp, when the
   // state is the one that pass wrote and vnn is unchanged.
+  // A steady primitive step on a rectilinear 3-D level whose stages run the
+  // LDS-ring kernel computes dt in its first stage instead (sdt): that
+  // stage's input is T^n, and the cell's rho/cp row is loaded there anyway.
   const bool xyz = irs && !tri && L.no_ycon && L.no_zcon && c->ndir == 3 && L.kp == GC;
-  const bool fuse_dt = xyz && p->rampa_iter <= 1 && !c->adj_store_on;
+  const bool ring = L.kp == GC && !L.tang && L.dims2 && !(L.row_march >= 0 ? L.row_march == 1 : !c->stage_lds);
+  const bool sdt = ring && L.rect_w && !c->rect_off && prim && !p->time_accurate && p->rampa_iter <= 1 &&
+                   !c->adj_store_on;
+  const bool fuse_dt = !sdt && xyz && p->rampa_iter <= 1 && !c->adj_store_on;
   const bool have_dt = fuse_dt && L.dt_next && L.dt_vnn == p->vnn;
   unsigned nmin = ntile;
   int pid = -1;
-  if (have_dt) {
+  if (have_dt || sdt) {
     nmin = (unsigned)L.hctiles.size();
   } else {
     // Compute_dt + min (Mod_dt.f90:10-84); dtglobal starts at 1e5 (Mod_Explicit.f90:40)
@@ -3575,18 +3616,26 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     HIPCHK(hipGetLastError());
   }
   // device scalars: [0] dtglobal, [1] time, [2] Sum (T-TO)^2
-  pid = prof_begin(c, FUSS_K_REDUCE);
-  // the minimum does not depend on the order: k_dt_final reads up to
-  // RED_FUSE chunks of per-tile minima itself
-  if (nmin > (unsigned)(RED_FUSE * RED_CHUNK)) {
-    const int nb1 = (int)((nmin + RED_CHUNK - 1) / RED_CHUNK);
-    k_reduce<true><<<nb1, 256, 0, c->stream>>>(L.pmin.p, (int)nmin, L.red2.p, halt);
-    k_dt_final<<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p, L.clk.p, L.cnt.p, p->time_accurate);
-  } else {
-    k_dt_final<<<1, 256, 0, c->stream>>>(L.pmin.p, (int)nmin, c->scal.p, L.clk.p, L.cnt.p, p->time_accurate);
+  auto dt_final = [&]() -> int {
+    pid = prof_begin(c, FUSS_K_REDUCE);
+    // the minimum does not depend on the order: k_dt_final reads up to
+    // RED_FUSE chunks of per-tile minima itself
+    if (nmin > (unsigned)(RED_FUSE * RED_CHUNK)) {
+      const int nb1 = (int)((nmin + RED_CHUNK - 1) / RED_CHUNK);
+      k_reduce<true><<<nb1, 256, 0, c->stream>>>(L.pmin.p, (int)nmin, L.red2.p, halt);
+      k_dt_final<<<1, 256, 0, c->stream>>>(L.red2.p, nb1, c->scal.p, L.clk.p, L.cnt.p, p->time_accurate);
+    } else {
+      k_dt_final<<<1, 256, 0, c->stream>>>(L.pmin.p, (int)nmin, c->scal.p, L.clk.p, L.cnt.p, p->time_accurate);
+    }
+    prof_end(c, pid);
+    HIPCHK(hipGetLastError());
+    return 0;
+  };
+  // (sdt: after the first stage, which reads neither dtglobal nor the clock)
+  if (!sdt) {
+    const int e = dt_final();
+    if (e) return e;
   }
-  prof_end(c, pid);
-  HIPCHK(hipGetLastError());
   L.dt_next = 0;
   const int o = L.cur, a1 = (L.cur + 1) % 3, a2 = (L.cur + 2) % 3;
   double* in = L.Tb[o].p;
@@ -3610,6 +3659,9 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = halt;
     sa.partials = L.partials.p; sa.status = c->status.p;  // k_bcflux flags table errors through status
     sa.rect = (L.rect_w && !c->rect_off) ? L.rfac.p : nullptr;
+    if (sdt && irk == 1) {
+      sa.dtw = L.dt.p; sa.vnn = p->vnn; sa.pmin = L.pmin.p;
+    }
     if (L.any_active && L.nbound) {
       pid = prof_begin(c, FUSS_K_BC);
       if (L.kp == GC) k_bcflux<3, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
@@ -3630,6 +3682,10 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     }
     prof_end(c, pid);
     HIPCHK(hipGetLastError());
+    if (sdt && irk == 1) {
+      const int e2 = dt_final();
+      if (e2) return e2;
+    }
     if (irs) {
       // Residual_Smoothing (Lib_IRS.f90:14-40): ndir passes R -> R2 -> R ...
       IrsArgs<double> ia{};

@@ -212,18 +212,40 @@ def test_graph_run_equals_step_loop(gpu, name):
     assert np.array_equal(T1, T2)
 
 
-def test_graph_run_fused_dt(gpu):
-    """The cube (fused x-y-z IRS pass, which leaves the next step's dt
-    behind) on captured graphs equals the step loop bit for bit."""
+@pytest.mark.parametrize("rect", [False, True])
+def test_graph_run_fused_dt(gpu, rect):
+    """The cube on captured graphs equals the step loop bit for bit: with the
+    stored metrics the fused x-y-z IRS pass leaves the next step's dt behind;
+    with the per-axis factors (steady, primitive) the first stage computes it."""
     out = []
     for graphs in (True, False):
         s = gpu.Solver(SYNTH["cube3d_rk4_irs"]())
+        s.rect(rect)
         s.graphs(graphs)
         k, hr, _, _ = s.run_level(1, 60, 60, -1.0, 1e300)
         out.append((k, hr.copy(), s.get("T")))
         s.close()
     assert out[0][0] == out[1][0] == 60
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("rect", [False, True])
+def test_dtglobal_vs_oracle(gpu, oracle_mod, rect):
+    """dtglobal of each step (k_dt, the fused IRS pass's next-step dt, or the
+    first stage's dt) against the oracle's Compute_dt: bit for bit on the
+    first step (same state), to rounding after."""
+    c = SYNTH["cube3d_rk4_irs"]()
+    s = gpu.Solver(c)
+    s.rect(rect)
+    o = oracle_mod.Oracle(c)
+    for n in range(6):
+        r = s.explicit_step(1)
+        o.step(1)
+        if n == 0:
+            assert r.dtglobal == o.dtglobal(1)
+        else:
+            assert abs(r.dtglobal - o.dtglobal(1)) <= 1e-13 * o.dtglobal(1)
+    s.close()
 
 
 def test_set_T_between_steps(gpu, oracle_mod):
