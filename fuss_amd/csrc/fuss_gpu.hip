@@ -735,6 +735,65 @@ __global__ __launch_bounds__(TILE_T) void k_stage(StageArgs<R> a) {
   if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
 }
 
+// The first half of k_stage for a step with a host hook between BC_Fluxes
+// and RK_Newstate (External_Function, Mod_Explicit.f90:53-57): the raw
+// residual R of Fluxes + BC_Fluxes, in k_stage's order, stored to Rres.
+template <int NDIR, bool TANG, typename R>
+__global__ __launch_bounds__(TILE_T) void k_stage_raw(StageArgs<R> a) {
+  if (a.halt && *a.halt) return;
+  const Tile t = a.tiles[blockIdx.x];
+  const BlockInfo B = a.blk[t.b];
+  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
+  if (i > B.ni || j > B.nj) return;
+  const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
+  R Rv = 0;
+  {
+#pragma clang fp contract(off)
+    if (i > 1) Rv = Rv + face_flux<0, NDIR, TANG>(a, B, p - 1);
+    if (i < B.ni) Rv = Rv - face_flux<0, NDIR, TANG>(a, B, p);
+    if (j > 1) Rv = Rv + face_flux<1, NDIR, TANG>(a, B, p - B.sy);
+    if (j < B.nj) Rv = Rv - face_flux<1, NDIR, TANG>(a, B, p);
+    if (NDIR == 3) {
+      if (k > 1) Rv = Rv + face_flux<2, NDIR, TANG>(a, B, p - B.sz);
+      if (k < B.nk) Rv = Rv - face_flux<2, NDIR, TANG>(a, B, p);
+    }
+  }
+  a.Rres[p] = apply_bc_faces(a, B, i, j, k, Rv);
+}
+
+// The second half: RK_Newstate from the (hook-modified) R in Rres, as k_stage
+// continues (IRSMODE: the scaled residual replaces R for the smoothing passes).
+template <bool PRIM, bool IRSMODE, typename R>
+__global__ __launch_bounds__(TILE_T) void k_newstate(StageArgs<R> a) {
+  if (a.halt && *a.halt) return;
+  const int wg = blockIdx.x;
+  const Tile t = a.tiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
+  double rsq = 0.0;
+  if (i <= B.ni && j <= B.nj) {
+    const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
+    const R Rv = a.Rres[p];
+    const R T = a.T[p];
+    const int m = a.mat[p];
+    const R dt = a.use_dtg ? (R)(*a.dtg) : a.dt[p];
+    int bad = 0;
+    const R res = scaled_residual<PRIM>(a.tab, m, T, Rv, a.vol[p], a.qvol[p], dt, &bad);
+    if (IRSMODE) {
+      a.Rres[p] = res;
+    } else {
+      const R TO = a.TO[p];
+      const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, a.jam, &bad);
+      a.Tout[p] = Tn;
+      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+      const double d = (double)Tn - (double)TO;
+      rsq = d * d;
+    }
+    if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+  }
+  if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
+}
+
 
 // ------------------------------------------------------------------------
 // 2.5-D stage kernel for 3-D levels: a work-group owns a 64x4 column of
@@ -3574,7 +3633,42 @@ const double RK4A[4] = {0.25, 1.0 / 3.0, 0.5, 1.0};
 
 // Explicit_Step on one level, enqueued on the stream (no host sync except
 // the dtglobal readback of time-accurate runs with time-dependent BCs).
-int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
+// The host side of External_Function (fuss_step_hook): the hook and its
+// user pointer, called once per block and stage.
+struct StageHook {
+  fuss_stage_fn fn;
+  void* user;
+};
+
+// Hands each block's raw R (and the stage input T) of the level to the hook
+// in the block's i-fastest order and puts the returned R back.
+int run_hook(fuss_ctx* c, Level& L, int level, int irk, const double* Tin, const StageHook* hk) {
+  for (int b = 0; b < L.nb; b++) {
+    const BlockInfo& B = L.hblk[b];
+    const size_t n = (size_t)B.ni * B.nj * B.nk;
+    std::vector<double> padR(B.npad), padT(B.npad), R(n), T(n);
+    HIPCHK(hipMemcpyAsync(padR.data(), L.R.p + B.off, B.npad * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(padT.data(), Tin + B.off, B.npad * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 1; k <= B.nk; k++)
+      for (int j = 1; j <= B.nj; j++)
+        for (int i = 1; i <= B.ni; i++) {
+          const size_t q = (i - 1) + (size_t)B.ni * ((j - 1) + (size_t)B.nj * (k - 1));
+          R[q] = padR[cidx(B, i, j, k) - B.off];
+          T[q] = padT[cidx(B, i, j, k) - B.off];
+        }
+    hk->fn(hk->user, level, irk, b + 1, B.ni, B.nj, B.nk, T.data(), R.data());
+    for (int k = 1; k <= B.nk; k++)
+      for (int j = 1; j <= B.nj; j++)
+        for (int i = 1; i <= B.ni; i++)
+          padR[cidx(B, i, j, k) - B.off] = R[(i - 1) + (size_t)B.ni * ((j - 1) + (size_t)B.nj * (k - 1))];
+    HIPCHK(hipMemcpyAsync(L.R.p + B.off, padR.data(), B.npad * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageHook* hk = nullptr) {
   Level& L = c->lev[level - 1];
   const int nrk = p->n_rk;
   if (nrk < 1 || nrk > 4) return FUSS_EARG;
@@ -3596,9 +3690,9 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
   // stage's input is T^n, and the cell's rho/cp row is loaded there anyway.
   const bool xyz = irs && !tri && L.no_ycon && L.no_zcon && c->ndir == 3 && L.kp == GC;
   const bool ring = L.kp == GC && !L.tang && L.dims2 && !(L.row_march >= 0 ? L.row_march == 1 : !c->stage_lds);
-  const bool sdt = ring && L.rect_w && !c->rect_off && prim && !p->time_accurate && p->rampa_iter <= 1 &&
+  const bool sdt = !hk && ring && L.rect_w && !c->rect_off && prim && !p->time_accurate && p->rampa_iter <= 1 &&
                    !c->adj_store_on;
-  const bool fuse_dt = !sdt && xyz && p->rampa_iter <= 1 && !c->adj_store_on;
+  const bool fuse_dt = !hk && !sdt && xyz && p->rampa_iter <= 1 && !c->adj_store_on;
   const bool have_dt = fuse_dt && L.dt_next && L.dt_vnn == p->vnn;
   unsigned nmin = ntile;
   int pid = -1;
@@ -3674,7 +3768,28 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p) {
     sa.last = irk == nrk && p->compute_residual;
     sa.ctiles = L.ctiles.p;
     pid = prof_begin(c, FUSS_K_STAGE);
-    if (L.kp == GC) {
+    if (hk) {
+      // Fluxes + BC_Fluxes to R, External_Function on the host, RK_Newstate
+      // (Mod_Explicit.f90:53-59)
+      sa.Rres = L.R.p;
+      const unsigned gt = (unsigned)L.htiles.size();
+      if (c->ndir == 3)
+        L.tang ? k_stage_raw<3, true, double><<<gt, TILE_T, 0, c->stream>>>(sa)
+               : k_stage_raw<3, false, double><<<gt, TILE_T, 0, c->stream>>>(sa);
+      else
+        L.tang ? k_stage_raw<2, true, double><<<gt, TILE_T, 0, c->stream>>>(sa)
+               : k_stage_raw<2, false, double><<<gt, TILE_T, 0, c->stream>>>(sa);
+      HIPCHK(hipGetLastError());
+      const int eh = run_hook(c, L, level, irk, in, hk);
+      if (eh) return eh;
+      if (prim)
+        irs ? k_newstate<true, true, double><<<gt, TILE_T, 0, c->stream>>>(sa)
+            : k_newstate<true, false, double><<<gt, TILE_T, 0, c->stream>>>(sa);
+      else
+        irs ? k_newstate<false, true, double><<<gt, TILE_T, 0, c->stream>>>(sa)
+            : k_newstate<false, false, double><<<gt, TILE_T, 0, c->stream>>>(sa);
+      if (!irs) { npart = (int)gt; L.plist = 0; }
+    } else if (L.kp == GC) {
       launch_stage3d(c, L, sa, L.tang != 0, prim, irs);
       if (!irs) { npart = (int)L.hctiles.size(); L.plist = 1; }
     } else {
@@ -4483,9 +4598,9 @@ int fuss_autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, 
   return autotune(c, level, p, steps, kc, row_march, ms_per_step);
 }
 
-int fuss_step(fuss_ctx* c, int level, const fuss_step_params* p, fuss_step_result* r) {
+static int step_sync(fuss_ctx* c, int level, const fuss_step_params* p, fuss_step_result* r, const StageHook* hk) {
   if (!c || !c->setup_done || !p || level < 1 || level > c->nlev) return FUSS_EARG;
-  int e = enqueue_step(c, level, p);
+  int e = enqueue_step(c, level, p, hk);
   if (e) return e;
   Level& L = c->lev[level - 1];
   HIPCHK(hipMemcpyAsync(c->hscal, c->scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -4504,6 +4619,17 @@ int fuss_step(fuss_ctx* c, int level, const fuss_step_params* p, fuss_step_resul
     for (int q = 0; q < 4; q++) r->bijk[q] = bijk[q];
   }
   return st;
+}
+
+int fuss_step(fuss_ctx* c, int level, const fuss_step_params* p, fuss_step_result* r) {
+  return step_sync(c, level, p, r, nullptr);
+}
+
+int fuss_step_hook(fuss_ctx* c, int level, const fuss_step_params* p, fuss_stage_fn fn, void* user,
+                   fuss_step_result* r) {
+  if (!c || !fn || c->adj_store_on) return FUSS_EARG;
+  const StageHook hk{fn, user};
+  return step_sync(c, level, p, r, &hk);
 }
 
 int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, int itermax_left,

@@ -58,7 +58,7 @@ module fuss_gpu_m
   end type
 
   public :: fuss_create, fuss_set_table, fuss_set_block, fuss_add_series, fuss_set_bc, fuss_setup
-  public :: fuss_step, fuss_run, fuss_restrict, fuss_prolong
+  public :: fuss_step, fuss_step_hook, fuss_run, fuss_restrict, fuss_prolong
   public :: fuss_refine_setup, fuss_refine
   public :: fuss_get_T, fuss_set_T, fuss_probe, fuss_get_clock, fuss_set_clock, fuss_ndir, fuss_destroy
 
@@ -111,6 +111,20 @@ module fuss_gpu_m
       type(c_ptr), value :: ctx
       integer(c_int), value :: level
       type(fuss_step_params), intent(in) :: p
+      type(fuss_step_result), intent(out) :: r
+    end function
+
+    !> fuss_step with the External_Function slot: fn(user, level, irk, block,
+    !> ni, nj, nk, T, R) receives the block's stage input T and raw residual R
+    !> (ni*nj*nk each, i fastest) between BC_Fluxes and RK_Newstate; it may
+    !> change R.  fn is a bind(c) procedure (c_funloc), user is passed through.
+    integer(c_int) function fuss_step_hook(ctx, level, p, fn, user, r) bind(c, name='fuss_step_hook')
+      import :: c_int, c_ptr, c_funptr, fuss_step_params, fuss_step_result
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level
+      type(fuss_step_params), intent(in) :: p
+      type(c_funptr), value :: fn
+      type(c_ptr), value :: user
       type(fuss_step_result), intent(out) :: r
     end function
 

@@ -28,6 +28,10 @@ FUSS_IRS_JACOBI2, FUSS_IRS_TRIDIAG = 0, 1
 FUSS_ESTATE = 2
 FUSS_ETABLE = 3
 
+# fuss_stage_fn: (user, level, irk, block, ni, nj, nk, const double *T, double *R)
+STAGE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                       C.POINTER(C.c_double), C.POINTER(C.c_double))
+
 
 class StepParams(C.Structure):
     _fields_ = [("n_rk", C.c_int), ("vnn", C.c_double), ("rampa_iter", C.c_int), ("time_accurate", C.c_int),
@@ -98,6 +102,7 @@ def load():
     L.fuss_set_stream.argtypes = [P, P]
     L.fuss_memory.argtypes = [P, C.POINTER(C.c_double)]
     L.fuss_rect.argtypes = [P, C.c_int]
+    L.fuss_step_hook.argtypes = [P, C.c_int, C.POINTER(StepParams), STAGE_FN, C.c_void_p, C.POINTER(StepResult)]
     L.fuss_rect_info.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.fuss_graphs.argtypes = [P, C.c_int]
     L.fuss_autotune.argtypes = [P, C.c_int, C.POINTER(StepParams), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -257,12 +262,24 @@ class Solver:
         return s.value
 
     # -- hot path ----------------------------------------------------------
-    def explicit_step(self, level=None, compute_residual=True, irs_mode=None):
-        """One Explicit_Step; returns the StepResult (res_sq, dtglobal, time, iter)."""
+    def explicit_step(self, level=None, compute_residual=True, irs_mode=None, hook=None):
+        """One Explicit_Step; returns the StepResult (res_sq, dtglobal, time, iter).
+
+        hook(level, irk, block, T, R): the External_Function slot, called per
+        stage and block with the stage input T and the raw residual R as
+        (nk, nj, ni) arrays over host memory; modify R in place (fuss_step_hook)."""
         lv = level or self.level
         sp = step_params(self.p, compute_residual, irs_mode)
         r = StepResult()
-        code = self.L.fuss_step(self.h, lv, C.byref(sp), C.byref(r))
+        if hook is None:
+            code = self.L.fuss_step(self.h, lv, C.byref(sp), C.byref(r))
+        else:
+            def cb(user, level, irk, block, ni, nj, nk, Tp, Rp):
+                shape = (nk, nj, ni)
+                hook(level, irk, block, np.ctypeslib.as_array(Tp, shape=shape),
+                     np.ctypeslib.as_array(Rp, shape=shape))
+            fn = STAGE_FN(cb)
+            code = self.L.fuss_step_hook(self.h, lv, C.byref(sp), fn, None, C.byref(r))
         if code:
             raise FussError(code, "fuss_step", list(r.bijk))
         return r

@@ -110,6 +110,20 @@ int fuss_setup(fuss_ctx *ctx, double initial_time);
  * RK_Newstate [with Residual_Smoothing]), Compute_Residual. */
 int fuss_step(fuss_ctx *ctx, int level, const fuss_step_params *p, fuss_step_result *r);
 
+/* The External_Function slot of Explicit_Step (Mod_Explicit.f90:24,53-59): called
+ * once per stage and block between BC_Fluxes and RK_Newstate with the block's
+ * residual R -- the sum of its face fluxes and BC contributions as BC_Fluxes
+ * leaves it (rho cp V dT/dt = -R + qvol V) -- and the stage's input T, both
+ * ni*nj*nk doubles on the host in i-fastest order; R may be modified in place
+ * (source terms).  block is 1-based, irk = 1..n_rk. */
+typedef void (*fuss_stage_fn)(void *user, int level, int irk, int block, int ni, int nj, int nk,
+                              const double *T, double *R);
+/* fuss_step with the hook: the stage runs unfused (raw R to HBM, a host round
+ * trip per stage and block, then the newstate pass), so it is for coupling,
+ * not speed.  Not capturable in a graph. */
+int fuss_step_hook(fuss_ctx *ctx, int level, const fuss_step_params *p, fuss_stage_fn fn,
+                   void *user, fuss_step_result *r);
+
 /* Up to max_iters Explicit_Steps on `level`, stopping after the first step
  * whose residual sqrt(res_sq) <= res_threshold, whose time >= time_threshold,
  * or when iter_from_call reaches itermax (the endsim/endmg tests of

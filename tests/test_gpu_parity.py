@@ -381,3 +381,57 @@ def test_autotune_keeps_results(gpu, name):
         s.close()
     assert np.array_equal(out[0][0], out[1][0])
     np.testing.assert_allclose(out[0][1], out[1][1], rtol=1e-13)
+
+
+@pytest.mark.parametrize("name", ["cube3d_mg3_irs", "lattice_transient", "plate2d_64", "lattice_rk4_cons"])
+def test_stage_hook_noop(gpu, name):
+    """fuss_step_hook with a hook that leaves R alone: the unfused stage
+    (raw R, host round trip, newstate pass) reproduces the step loop, and the
+    hook sees every stage of every block with the stage's input T."""
+    c = SYNTH[name]()
+    lv = c.params.mgl
+    out = []
+    for hooked in (False, True):
+        s = gpu.Solver(c)
+        s.rect(False)
+        calls = []
+
+        # (an exception inside a ctypes callback is printed, not raised:
+        # record, then check outside)
+        def hook(level, irk, block, T, R):
+            calls.append((irk, block, level, T.shape, R.shape))
+
+        for _ in range(4):
+            s.explicit_step(lv, hook=hook if hooked else None)
+        out.append([s.get("T", lv, b + 1) for b in range(c.nb)])
+        if hooked:
+            assert len(calls) == 4 * c.params.n_rk * c.nb
+            for irk, block, level, ts, rs in calls:
+                assert level == lv and ts == rs == tuple(reversed(s.dims(level, block)))
+            assert [q[0] for q in calls[: c.params.n_rk * c.nb]] == [irk for irk in range(1, c.params.n_rk + 1)
+                                                                        for _ in range(c.nb)]
+        s.close()
+    for a, b in zip(out[0], out[1]):
+        assert relerr(b, a) < 1e-14
+
+
+def test_stage_hook_source_vs_oracle(gpu, oracle_mod):
+    """A hook adding a volumetric source q0 (R -= q0 V per cell) is the same
+    step as qvol + q0 in the oracle (Newstate: res = (-R/V + qvol) dt / ...)."""
+    c = SYNTH["cube3d_rk4_irs"]()
+    q0 = 2.5e5
+    s = gpu.Solver(c)
+    _, vol, _ = s.metrics(1, 1)
+
+    def hook(level, irk, block, T, R):
+        R -= q0 * vol
+
+    c2 = SYNTH["cube3d_rk4_irs"]()
+    for b in c2.blocks:
+        b.qvol[...] = b.qvol + q0
+    o = oracle_mod.Oracle(c2)
+    for _ in range(10):
+        s.explicit_step(1, hook=hook)
+        o.step(1)
+    assert relerr(s.get("T"), o.get("T")) < RTOL
+    s.close()
