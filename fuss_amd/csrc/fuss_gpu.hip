@@ -597,6 +597,94 @@ __global__ void k_bcflux(StageArgs<R> a, const BCHot* __restrict__ hot, int n) {
   a.bcf[r.fpos] = v;
 }
 
+// Extract_Properties_From_BC (src/lib/io/IO_Wall.f90:131-239): wall
+// temperature and heat flux (Compute_Wall_Properties, Lib_BC_Fluxes.f90:48-58)
+// of every entry of types 301-305 at the state a.T (0 for other types), with
+// the wall routines' arithmetic: 301 T_wall = T - (G M)(dir)
+// (Lib_BC_Fluxes_Wall_Heat.f90:37-52), 302 q_wall = kappa (G M).n
+// (Lib_BC_Fluxes_Wall_Temperature.f90:37-66), 303 the conductance balance
+// (Lib_BC_Fluxes_Wall_HeatTransfer.f90:44-46, its unset qw as 0), 304/305 the
+// last iterate of the radiative fixed point.
+template <int NDIR>
+__global__ void k_wall(StageArgs<double> a, int n, double* __restrict__ Tw_out, double* __restrict__ qw_out) {
+#pragma clang fp contract(off)
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const BCRec& r = a.bc[e];
+  double Tw = 0.0, qw = 0.0;
+  if (r.type >= 301 && r.type <= 305) {
+    const BlockInfo B = a.blk[r.b];
+    const double* __restrict__ T = a.T;
+    const long p = r.cell, sy = B.sy, sz = B.sz;
+    const int Fm = r.f, dir = r.dir;
+    const int modfm2 = 1 - 2 * (Fm % 2), modfm3 = 2 * (2 * (Fm % 2) - 1);
+    const double Tc = T[p];
+    const double* M = r.M;
+    const double* Nr = r.N;
+    int bad = 0;
+    const double kappa = tab_interp(a.tab.kap + (long)a.mat[p] * a.tab.nT, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
+    const double mn = M[dir - 1] * Nr[0] + M[dir - 1 + 3] * Nr[1] + M[dir - 1 + 6] * Nr[2];
+    double sv = 0.0;
+    if (r.series >= 0)
+      sv = series_value(a.ser.per[r.series], a.ser.n[r.series], a.ser.t + a.ser.off[r.series],
+                        a.ser.v + a.ser.off[r.series], *a.clk);
+    if (r.type == 301) {
+      const double q = r.series >= 0 ? sv : r.qw;
+      double G[3] = {0.0, 0.0, 0.0};
+      G[dir - 1] = -q / (2.0 * kappa * mn) * modfm2;
+      if (NDIR == 2) G[2] = 0.0;
+      const double grd = G[0] * M[3 * (dir - 1)] + G[1] * M[3 * (dir - 1) + 1] + G[2] * M[3 * (dir - 1) + 2];
+      Tw = Tc - grd;
+      qw = q;
+    } else if (r.type == 302) {
+      const double Tv = r.series >= 0 ? sv : r.Tw;
+      double G[3] = {0.0, 0.0, 0.0};
+      G[dir - 1] = (Tc - Tv) * modfm3;
+      if (Fm <= 2) {
+        G[1] = (T[p - sy] - T[p + sy]) / 2.0;
+        if (NDIR == 3) G[2] = (T[p - sz] - T[p + sz]) / 2.0;
+      } else if (Fm <= 4) {
+        G[0] = (T[p - 1] - T[p + 1]) / 2.0;
+        if (NDIR == 3) G[2] = (T[p - sz] - T[p + sz]) / 2.0;
+      } else {
+        G[0] = (T[p - 1] - T[p + 1]) / 2.0;
+        G[1] = (T[p - sy] - T[p + sy]) / 2.0;
+      }
+      if (NDIR == 2) G[2] = 0.0;
+      double gr[3];
+      for (int x = 0; x < 3; x++) gr[x] = G[0] * M[3 * x] + G[1] * M[3 * x + 1] + G[2] * M[3 * x + 2];
+      Tw = Tv;
+      qw = kappa * (gr[0] * Nr[0] + gr[1] * Nr[1] + gr[2] * Nr[2]);
+    } else if (r.type == 303) {
+      const double q0 = 0.0;
+      Tw = (r.hconv * r.Tref + Tc * 2.0 * kappa * mn + q0) / (r.hconv + 2.0 * kappa * mn);
+      qw = r.hconv * (r.Tref - Tw) + q0;
+    } else {
+      double Tr2 = r.Tref * r.Tref, Tr4 = Tr2 * Tr2, w = Tc, qo = 0.0, err = 1.0e6, q = 0.0;
+      int it = 0;
+      while (err > 1.0e-3) {
+        const double w2 = w * w;
+        const double K = SIGMA_SB * r.eps * (w2 * w2 - Tr4) / (w - r.Tref);
+        const double H = r.type == 304 ? 1.0 / (2.0 * kappa * mn) + 1.0 / K
+                                       : 1.0 / (2.0 * kappa * mn) + 1.0 / (K + r.hconv);
+        q = K * (r.Tref - w);
+        err = fabs(qo - q);
+        qo = q;
+        w = r.Tref - 1.0 / (H * K) * (r.Tref - Tc);
+        if (++it > 1000 || err != err) {
+          flag_error(a.status, FUSS_ESTATE, r.b, r.i, r.j, r.k);
+          break;
+        }
+      }
+      Tw = w;
+      qw = q;
+    }
+    if (bad) flag_error(a.status, FUSS_ETABLE, r.b, r.i, r.j, r.k);
+  }
+  Tw_out[e] = Tw;
+  qw_out[e] = qw;
+}
+
 // Compact records of the wall entries (after the geometry of the full
 // records and the cell materials are on the device).
 __global__ void k_bc_hot(const BCRec* __restrict__ bc, int n, const int* __restrict__ mat, BCHot* __restrict__ hot) {
@@ -4396,6 +4484,30 @@ int fuss_graphs(fuss_ctx* c, int enable) {
   if (!c) return FUSS_EARG;
   c->graphs_off = !enable;
   return 0;
+}
+
+int fuss_wall(fuss_ctx* c, int level, double* Tw, double* qw) {
+  if (!c || !c->setup_done || level < 1 || level > c->nlev || !Tw || !qw) return FUSS_EARG;
+  Level& L = c->lev[level - 1];
+  if (!L.nbound) return 0;
+  double* T = L.Tb[L.cur].p;
+  // ghosts mirrored from the current state (the 302 tangential differences
+  // of face-corner cells read them)
+  int e = launch_ghost(c, L, T);
+  if (e) return e;
+  StageArgs<double> sa{};
+  sa.blk = L.blk.p; sa.T = T; sa.mat = L.mat.p; sa.bc = L.bc.p; sa.tab = dtab<double>(c);
+  sa.ser = serdev(c); sa.clk = L.clk.p; sa.status = c->status.p;
+  DBuf<double> out;
+  if (out.alloc(2 * (size_t)L.nbound)) return FUSS_EHIP;
+  if (c->ndir == 3) k_wall<3><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound, out.p, out.p + L.nbound);
+  else k_wall<2><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound, out.p, out.p + L.nbound);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(Tw, out.p, L.nbound * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(qw, out.p + L.nbound, L.nbound * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  const int st = sync_status(c);
+  out.free();
+  return st;
 }
 
 int fuss_rect(fuss_ctx* c, int enable) {

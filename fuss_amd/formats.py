@@ -166,6 +166,54 @@ def write_tecplot(path, zones, varnames, cell_vars, solutiontime=0.0):
                 fh.write(_fmt15(data[v]) + "\n")
 
 
+WALL_TYPES = (301, 302, 303, 304, 305)
+
+
+def wall_zones(bcl, dims, Tw, qw):
+    """Per-entry wall values (fuss_wall / Extract_Properties_From_BC order) to
+    the reference's wall zones (Initialize_Wall_File, IO_Wall.f90:12-95): one
+    zone per (block, face) that holds a wall entry (its viscous_flag), named
+    B<b>F<f>, Tw and qw cell-centred on the face, shaped (K, J, I) with the
+    face's own axis of length 1.  dims[b-1] = (ni, nj, nk).
+
+    Returns {(b, f): {"Tw": array, "qw": array}} in block, face order."""
+    out = {}
+    for n in range(bcl.n):
+        b, i, j, k, f, typ = (int(x) for x in bcl.ib[n, :6])
+        if typ not in WALL_TYPES:
+            continue
+        if (b, f) not in out:
+            ni, nj, nk = dims[b - 1]
+            shape = (nk, nj, 1) if f <= 2 else ((nk, 1, ni) if f <= 4 else (1, nj, ni))
+            out[(b, f)] = {"Tw": np.zeros(shape), "qw": np.zeros(shape)}
+        z = out[(b, f)]
+        at = (k - 1, j - 1, 0) if f <= 2 else ((k - 1, 0, i - 1) if f <= 4 else (0, j - 1, i - 1))
+        z["Tw"][at] = Tw[n]
+        z["qw"][at] = qw[n]
+    return dict(sorted(out.items()))
+
+
+def write_wall_tecplot(path, nodes, dims, zones, solutiontime=0.0):
+    """Write wall zones (wall_zones) as the reference's wall file
+    (Write_Wall_Solution, IO_Wall.f90:98-126): x, y, z of the face's nodes,
+    then Tw, qw cell-centred.  nodes[b-1]: the block's (nk+1, nj+1, ni+1, 3)
+    node array."""
+    out = []
+    for (b, f), z in zones.items():
+        ni, nj, nk = dims[b - 1]
+        X = np.asarray(nodes[b - 1]).reshape(nk + 1, nj + 1, ni + 1, 3)
+        if f <= 2:
+            sl = X[:, :, 0:1] if f == 1 else X[:, :, ni:ni + 1]
+        elif f <= 4:
+            sl = X[:, 0:1, :] if f == 3 else X[:, nj:nj + 1, :]
+        else:
+            sl = X[0:1] if f == 5 else X[nk:nk + 1]
+        K, J, I = sl.shape[:3]
+        out.append((f"B{b}F{f}", I, J, K, {"x": sl[..., 0], "y": sl[..., 1], "z": sl[..., 2],
+                                           "Tw": z["Tw"], "qw": z["qw"]}))
+    write_tecplot(path, out, ["x", "y", "z", "Tw", "qw"], {3, 4}, solutiontime)
+
+
 # --------------------------------------------------------------------------
 # boundary conditions
 # --------------------------------------------------------------------------

@@ -61,6 +61,7 @@ module fuss_gpu_m
   public :: fuss_step, fuss_step_hook, fuss_run, fuss_restrict, fuss_prolong
   public :: fuss_refine_setup, fuss_refine
   public :: fuss_get_T, fuss_set_T, fuss_probe, fuss_get_clock, fuss_set_clock, fuss_ndir, fuss_destroy
+  public :: fuss_wall
 
   interface
     integer(c_int) function fuss_create(nlevels, nblocks, precision, ctx) bind(c, name='fuss_create')
@@ -207,6 +208,14 @@ module fuss_gpu_m
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), intent(out) :: ndir
+    end function
+
+    !> Extract_Properties_From_BC: Tw, qw of each BC entry (nbound each)
+    integer(c_int) function fuss_wall(ctx, level, Tw, qw) bind(c, name='fuss_wall')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level
+      real(c_double), intent(out) :: Tw(*), qw(*)
     end function
 
     subroutine fuss_destroy(ctx) bind(c, name='fuss_destroy')

@@ -102,6 +102,7 @@ def load():
     L.fuss_set_stream.argtypes = [P, P]
     L.fuss_memory.argtypes = [P, C.POINTER(C.c_double)]
     L.fuss_rect.argtypes = [P, C.c_int]
+    L.fuss_wall.argtypes = [P, C.c_int, _dp, _dp]
     L.fuss_step_hook.argtypes = [P, C.c_int, C.POINTER(StepParams), STAGE_FN, C.c_void_p, C.POINTER(StepResult)]
     L.fuss_rect_info.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.fuss_graphs.argtypes = [P, C.c_int]
@@ -184,6 +185,14 @@ class Solver:
     def graphs(self, enable):
         """fuss_run on captured HIP graphs (default) or step by step."""
         _chk(self.L.fuss_graphs(self.h, int(bool(enable))))
+
+    def wall(self, level=1):
+        """(Tw, qw) of every BC entry of the level at the current state, in
+        BC-list order (fuss_wall; 0 for entries that are not walls)."""
+        n = int(self.case.bcs[level - 1].n)
+        Tw, qw = np.zeros(n), np.zeros(n)
+        _chk(self.L.fuss_wall(self.h, level, Tw, qw), "fuss_wall")
+        return Tw, qw
 
     def rect(self, enable):
         """Per-axis metric factors on rectilinear levels (default) or the

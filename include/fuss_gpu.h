@@ -245,6 +245,11 @@ int fuss_adj_backward(fuss_ctx *ctx, const double *probe_adj, double *grad_qw, d
                       double *grad_T0);
 
 /* --- state access (output cadence, restart, probes) ------------------- */
+/* Wall output (Extract_Properties_From_BC, src/lib/io/IO_Wall.f90:131-239):
+ * wall temperature and heat flux of each BC entry of `level` at the current
+ * state, in BC-list order (nbound values each; 0 for entries that are not
+ * walls 301-305).  Mirrors the level's ghosts from the current T first. */
+int fuss_wall(fuss_ctx *ctx, int level, double *Tw, double *qw);
 int fuss_get_T(fuss_ctx *ctx, int level, int b, double *T);
 int fuss_set_T(fuss_ctx *ctx, int level, int b, const double *T);
 /* Probes (src/lib/io/IO_Probes.f90:124-166): T of n level-1 cells bijk[4n]. */

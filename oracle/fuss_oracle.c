@@ -773,6 +773,91 @@ static void bc_fluxes(ob_ctx *c, ob_dom *d) {
   }
 }
 
+/* Extract_Properties_From_BC (src/lib/io/IO_Wall.f90:131-239): the wall
+ * temperature and heat flux (Ovar, Compute_Wall_Properties,
+ * Lib_BC_Fluxes.f90:48-58) that the wall routines 301-305 return at the
+ * current state; entries of other types get 0.  Boundary values of 301/302
+ * are the ones the last BC_Fluxes used (their series at the level time). */
+static void wall_props(ob_ctx *c, ob_dom *d, ob_bc *e, double *Tw_out, double *qw_out) {
+  *Tw_out = 0.0;
+  *qw_out = 0.0;
+  if (e->type < 301 || e->type > 305) return;
+  ob_blk *B = &d->blk[e->b - 1];
+  int Im = e->i, Jm = e->j, Km = e->k, Fm = e->f;
+  int dir, fi, fj, fk;
+  face_index(Fm, Im, Jm, Km, &dir, &fi, &fj, &fk);
+  long f = FX(B, dir - 1, fi, fj, fk);
+  const double *Nrm = B->fN[dir - 1] + 3 * f;
+  const double *M = Mc(B, Im, Jm, Km);
+  const double *TT = B->T;
+  long p = CX(B, Im, Jm, Km);
+  double Tc = TT[p];
+  double mat = B->matID[p];
+  int modfm = Fm % 2, modfm2 = 1 - 2 * modfm, modfm3 = 2 * (2 * modfm - 1);
+  double kappa = co_Kappa(c, mat, Tc);
+  double mn = MEL(M, dir, 1) * Nrm[0] + MEL(M, dir, 2) * Nrm[1] + MEL(M, dir, 3) * Nrm[2];
+  switch (e->type) {
+    case 301: {
+      /* Lib_BC_Fluxes_Wall_Heat.f90:37-52: T_wall = T - (G M)(Dir) */
+      double G[3] = {0.0, 0.0, 0.0}, gr[3];
+      G[dir - 1] = -e->qw / (2.0 * kappa * mn) * modfm2;
+      if (c->ndir == 2) G[2] = 0.0;
+      matvec_GM(G, M, gr);
+      *Tw_out = Tc - gr[dir - 1];
+      *qw_out = e->qw;
+    } break;
+    case 302: {
+      /* Lib_BC_Fluxes_Wall_Temperature.f90:37-66: q_wall = kappa (G M) . n */
+      double G[3] = {0.0, 0.0, 0.0}, gr[3];
+      G[dir - 1] = (Tc - e->Tw) * modfm3;
+      if (Fm <= 2) {
+        G[1] = (TT[CX(B, Im, Jm - 1, Km)] - TT[CX(B, Im, Jm + 1, Km)]) / 2.0;
+        G[2] = (TT[CX(B, Im, Jm, Km - 1)] - TT[CX(B, Im, Jm, Km + 1)]) / 2.0;
+      } else if (Fm <= 4) {
+        G[0] = (TT[CX(B, Im - 1, Jm, Km)] - TT[CX(B, Im + 1, Jm, Km)]) / 2.0;
+        G[2] = (TT[CX(B, Im, Jm, Km - 1)] - TT[CX(B, Im, Jm, Km + 1)]) / 2.0;
+      } else {
+        G[0] = (TT[CX(B, Im - 1, Jm, Km)] - TT[CX(B, Im + 1, Jm, Km)]) / 2.0;
+        G[1] = (TT[CX(B, Im, Jm - 1, Km)] - TT[CX(B, Im, Jm + 1, Km)]) / 2.0;
+      }
+      if (c->ndir == 2) G[2] = 0.0;
+      matvec_GM(G, M, gr);
+      *Tw_out = e->Tw;
+      *qw_out = kappa * dot3(gr, Nrm);
+    } break;
+    case 303: {
+      /* Lib_BC_Fluxes_Wall_HeatTransfer.f90:44-46 (qw unset for 303: 0) */
+      double qw = 0.0;
+      double Tw = (e->hconv * e->Tref + Tc * 2.0 * kappa * mn + qw) / (e->hconv + 2.0 * kappa * mn);
+      *Tw_out = Tw;
+      *qw_out = e->hconv * (e->Tref - Tw) + qw;
+    } break;
+    default: {
+      /* 304 / 305: the fixed point of Lib_BC_Fluxes_Wall_Radiation.f90:41-57
+       * (RadiationConvection: K + hconv in H); its last T_wall and q_wall */
+      double Tw = Tc, qw = 0.0, err = 1.0e6, q_wall = 0.0;
+      double Tr2 = e->Tref * e->Tref, Tr4 = Tr2 * Tr2;
+      int it = 0;
+      while (err > 1.0e-3) {
+        double Tw2 = Tw * Tw;
+        double K = SIGMA_SB * e->eps * (Tw2 * Tw2 - Tr4) / (Tw - e->Tref);
+        double H = (e->type == 304) ? 1.0 / (2.0 * kappa * mn) + 1.0 / K
+                                    : 1.0 / (2.0 * kappa * mn) + 1.0 / (K + e->hconv);
+        q_wall = K * (e->Tref - Tw);
+        err = fabs(qw - q_wall);
+        qw = q_wall;
+        Tw = e->Tref - 1.0 / (H * K) * (e->Tref - Tc);
+        if (++it > 100000 || !(err == err)) {
+          set_err(c, OB_ERR_STATE, e->b, Im, Jm, Km);
+          break;
+        }
+      }
+      *Tw_out = Tw;
+      *qw_out = q_wall;
+    } break;
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* time step: src/lib/numerics/time/Mod_dt.f90                          */
 /* ------------------------------------------------------------------ */
@@ -1501,6 +1586,17 @@ int ob_get_metrics(ob_ctx *c, int level, int b, double *M, double *dl, double *v
     if (fA) memcpy(fA, B->fA[d - 1], sizeof(double) * nfaces(B, d - 1));
   }
   return 0;
+}
+
+int ob_wall(ob_ctx *c, int level, double *Tw, double *qw) {
+  if (!c || level < 1 || level > c->nlev || !Tw || !qw) return OB_ERR_ARG;
+  ob_dom *d = &c->dom[level - 1];
+  /* ghosts mirrored from the current state: the reference reads the ghosts
+   * its last stage left (mirrors of that stage's input), which only the 302
+   * tangential differences at face corners see */
+  ghost_fill(c, d);
+  for (int n = 0; n < d->nbound; n++) wall_props(c, d, &d->bc[n], &Tw[n], &qw[n]);
+  return c->err;
 }
 
 int ob_get_bc_ghost(ob_ctx *c, int level, int n, double *Mg1, double *matIDg, double *Tg) {

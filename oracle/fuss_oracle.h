@@ -107,6 +107,10 @@ int ob_last_error(ob_ctx *c, int *bijk);
 int ob_get_metrics(ob_ctx *c, int level, int b, double *M, double *dl,
                    double *vol, int d, double *fN, double *fA);
 /* Mg(1) and matIDg of BC entry n (1-based) of a level. */
+/* Wall temperature and heat flux of every BC entry of `level` at the
+ * current state (Extract_Properties_From_BC, IO_Wall.f90:131-239; entries
+ * that are not walls 301-305 get 0), in BC-list order. */
+int ob_wall(ob_ctx *c, int level, double *Tw, double *qw);
 int ob_get_bc_ghost(ob_ctx *c, int level, int n, double *Mg1, double *matIDg,
                     double *Tg);
 

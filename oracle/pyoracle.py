@@ -56,6 +56,7 @@ def lib():
         L.ob_get_metrics.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.ob_get_bc_ghost.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.ob_wall.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -170,6 +171,14 @@ class Oracle:
                               vol.ctypes.data_as(C.c_void_p), 0, None, None)
         sh = (nk + 4, nj + 4, ni + 4)
         return M.reshape(sh + (3, 3)).swapaxes(-1, -2), dl.reshape(sh + (3,)), vol.reshape(sh), faces
+
+    def wall(self, level=1):
+        """(Tw, qw) of every BC entry of the level at the current state
+        (Extract_Properties_From_BC; 0 for entries that are not walls)."""
+        n = int(self.case.bcs[level - 1].n)
+        Tw, qw = np.zeros(n), np.zeros(n)
+        self._chk(self.L.ob_wall(self.h, level, Tw, qw))
+        return Tw, qw
 
     def bc_ghost(self, n, level=1):
         Mg = np.zeros(9)

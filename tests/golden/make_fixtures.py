@@ -8,6 +8,7 @@ outputs for it:
 
   temperature_Plate, qvol_Plate, convective_Plate, radiative_Rod
       test/steady-state/<case>/SOLUTION/field.tec -> final T, stop iteration
+      test/steady-state/<case>/SOLUTION/wall.tec  -> wall Tw, qw per face zone
   multimat_Plate
       test/transient/multimat_Plate/SOLUTION/field{1,21,41,61,81,101}.tec
   oscillating_Rod
@@ -78,6 +79,10 @@ def main(ref="/root/reference"):
             arr["stop_iter"] = np.int32(round(sol[0].solutiontime))
             for b, z in enumerate(sol):
                 arr[f"ref_T{b}"] = z.data["T"]
+            # wall output of the same state (Tw, qw per wall face zone)
+            for z in fm.read_tecplot(os.path.join(d, "SOLUTION", "wall.tec")):
+                arr[f"wall_{z.name}_Tw"] = z.data["Tw"]
+                arr[f"wall_{z.name}_qw"] = z.data["qw"]
         elif name == "multimat_Plate":
             snaps = [1, 21, 41, 61, 81, 101]
             arr["snap_iter"] = np.asarray([10 * s for s in snaps], np.int32)

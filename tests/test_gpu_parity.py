@@ -12,8 +12,10 @@ import numpy as np
 import pytest
 
 from fuss_amd import cases
+from fuss_amd import formats as fm
 from tests.conftest import gpu_available
 from tests.golden import fixtures
+from tests.test_oracle_golden import check_wall
 
 pytestmark = pytest.mark.gpu
 RTOL = 1e-12
@@ -145,6 +147,34 @@ def test_steady_to_convergence(gpu, name):
         err = relerr(s.get("T", 1, b + 1), z[f"ref_T{b}"])
         print(f"{name}: stop {n}, block {b+1} max rel err vs reference {err:.3e}", flush=True)
         assert err < RTOL
+    check_wall(c, z, *s.wall(1))
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["box_walls", "lattice_transient", "plate2d_64", "convective"])
+def test_wall_vs_oracle(gpu, oracle_mod, name):
+    """fuss_wall (Extract_Properties_From_BC on the device) against the
+    oracle after 8 steps: walls 301 (with a time series on the lattice),
+    302, 303, 304, 305 in 3-D, curvilinear multi-block and 2-D."""
+    from tests.test_refine import box
+
+    if name == "box_walls":
+        c = box()
+    elif name == "convective":
+        c, _ = fixtures.load("convective_Plate")
+    else:
+        c = SYNTH[name]()
+    s = gpu.Solver(c)
+    o = oracle_mod.Oracle(c)
+    for _ in range(8):
+        s.explicit_step(1)
+        o.step(1)
+    Tw, qw = s.wall(1)
+    Two, qwo = o.wall(1)
+    walls = np.isin(c.bcs[0].ib[:, 5], fm.WALL_TYPES)
+    assert walls.any() and not Tw[~walls].any() and not qw[~walls].any()
+    assert np.max(np.abs(Tw[walls] - Two[walls]) / np.abs(Two[walls])) < RTOL
+    assert np.max(np.abs(qw[walls] - qwo[walls])) <= RTOL * np.max(np.abs(qwo[walls]))
     s.close()
 
 

@@ -4,8 +4,9 @@ Each fixture (tests/golden/*.npz, written by make_fixtures.py from the
 reference's test/ directory) runs through the oracle with the reference's
 inputs and numerics; the oracle must reproduce the stored solution:
 
-* steady cases: the stop iteration (first residual <= res-threshold) and the
-  final T field (Tecplot output, 15 significant digits);
+* steady cases: the stop iteration (first residual <= res-threshold), the
+  final T field (Tecplot output, 15 significant digits) and the wall output
+  (wall.tec: Tw, qw of every wall face cell);
 * multimat_Plate: six transient snapshots (time and T);
 * oscillating_Rod: both probe histories (time, T) of a periodic 302 series;
 * mg_irs: all 80603 residuals of a 2-level multigrid + IRS + connection run.
@@ -13,12 +14,17 @@ inputs and numerics; the oracle must reproduce the stored solution:
 import numpy as np
 import pytest
 
+from fuss_amd import formats as fm
 from tests.golden import fixtures
 
 STEADY = ["temperature_Plate", "qvol_Plate", "convective_Plate", "radiative_Rod"]
 # Tecplot stores 15 significant digits; the oracle follows the reference's
 # operation order, so agreement is at that level.
 FIELD_RTOL = 1e-13
+# wall.tec is printed the same way; qw (a conductance times a difference of
+# nearby temperatures, scaled by the zone's largest |qw|) keeps a few digits
+# fewer: observed <= 6e-14
+WALL_RTOL = 1e-12
 
 
 @pytest.mark.parametrize("name", STEADY)
@@ -33,6 +39,21 @@ def test_steady_field(oracle_mod, name):
         ref = z[f"ref_T{b}"]
         err = np.max(np.abs(o.get("T", 1, b + 1) - ref) / np.abs(ref))
         assert err < FIELD_RTOL, f"block {b+1}: max rel err {err:.3e}"
+    check_wall(c, z, *o.wall(1))
+
+
+def check_wall(c, z, Tw, qw):
+    """Wall output (Extract_Properties_From_BC) of the converged state against
+    the reference's wall.tec: the same wall zones, Tw and qw per face cell.
+    qw is scaled by the zone's largest |qw| (it crosses 0 on some walls)."""
+    zones = fm.wall_zones(c.bcs[0], [b.dims for b in c.blocks], Tw, qw)
+    names = {k[5:-3] for k in z.files if k.startswith("wall_") and k.endswith("_Tw")}
+    assert {f"B{b}F{f}" for b, f in zones} == names
+    for (b, f), v in zones.items():
+        rT, rq = z[f"wall_B{b}F{f}_Tw"], z[f"wall_B{b}F{f}_qw"]
+        assert v["Tw"].shape == rT.shape
+        assert np.max(np.abs(v["Tw"] - rT) / np.abs(rT)) < WALL_RTOL, (b, f)
+        assert np.max(np.abs(v["qw"] - rq)) <= WALL_RTOL * max(np.max(np.abs(rq)), 1e-300), (b, f)
 
 
 def test_multimat_transient(oracle_mod):
