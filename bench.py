@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 # variables, IRS on; RECT = per-axis metric factors (rectilinear level).
 STAGE_KERNEL = {True: "k_stage3p<true, true, true, true, double, 16>",
                 False: "k_stage3p<true, true, true, false, double, 16>"}
-STAGE_TRAFFIC_MEASURED = {STAGE_KERNEL[True]: 149403 * 1024 * 2.0 + 131072 * 1024 * 1.0}
+STAGE_TRAFFIC_MEASURED = {STAGE_KERNEL[True]: 149388 * 1024 * 2.0 + 131072 * 1024 * 1.0}
 # Plain FP64 path (four-stage RK + LTS + IRS 0.5, 3-level sequencing) run to
 # residual 1e-10 on the GPU: iterations per cube size (profiles/
 # steady_convergence_cube.md).  Fit: iters = 230218 (n/64)^PLAIN_EXP.
