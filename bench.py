@@ -44,11 +44,16 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the stage kernel measured with rocprofv3 PMC
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB device copy);
 # source: profiles/explicit_step_cube256_fp64.md (256^3 cube, FP64).
-# The LDS-ring stage kernel with 16-plane columns, one material, primitive
-# variables, IRS on; RECT = per-axis metric factors (rectilinear level).
-STAGE_KERNEL = {True: "k_stage3p<true, true, true, true, double, 16>",
-                False: "k_stage3p<true, true, true, false, double, 16>"}
-STAGE_TRAFFIC_MEASURED = {STAGE_KERNEL[True]: 149388 * 1024 * 2.0 + 131072 * 1024 * 1.0}
+# The LDS-ring stage kernel (stages 2-4 of a steady step: no dt) with kc-plane
+# columns, one material, primitive variables, IRS on; rect = per-axis metric
+# factors (rectilinear level).
+
+
+def stage_kernel(rect, kc):
+    return f"k_stage3p<true, true, true, {'true' if rect else 'false'}, false, double, {kc}>"
+
+
+STAGE_TRAFFIC_MEASURED = {stage_kernel(True, 16): 149388 * 1024 * 2.0 + 131072 * 1024 * 1.0}
 # Plain FP64 path (four-stage RK + LTS + IRS 0.5, 3-level sequencing) run to
 # residual 1e-10 on the GPU: iterations per cube size (profiles/
 # steady_convergence_cube.md).  Fit: iters = 230218 (n/64)^PLAIN_EXP.
@@ -382,8 +387,8 @@ def main():
     # on a rectilinear level it reads no per-cell geometry either
     stage_bpc = perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False, qvol=False, rect=rect[0])
     stage_b = stage_bpc * ncell
-    skern = STAGE_KERNEL[rect[0]]
-    ldsring16 = args.n == 256 and tuned[0] == 16 and not tuned[1]
+    skern = stage_kernel(rect[0], tuned[0])
+    ldsring = args.n == 256 and not tuned[1]
     stage_ms = ms[2] / cnt[2]
     achieved = stage_b / (stage_ms * 1e-3) / 1e9
     out = {
@@ -407,8 +412,8 @@ def main():
                              "chosen_by": "fuss_autotune"}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": STAGE_TRAFFIC_MEASURED.get(skern) if ldsring16 else None,
-                     "kernel": (skern if ldsring16 else "stage kernel")
+                     "traffic": STAGE_TRAFFIC_MEASURED.get(skern) if ldsring else None,
+                     "kernel": (skern if ldsring else "stage kernel")
                                + " (fused interior + BC fluxes, scaled residual)",
                      "bytes_per_cell": stage_bpc,
                      "ms_avg": stage_ms},

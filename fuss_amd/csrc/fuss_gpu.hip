@@ -892,7 +892,7 @@ __global__ __launch_bounds__(TILE_T) void k_newstate(StageArgs<R> a) {
 // The residual accumulation keeps the Fluxes_blk order (i, then j, then k
 // faces; Mod_Fluxes.f90:54-192).
 // ------------------------------------------------------------------------
-constexpr int KC_DEFAULT = 8;   // column height; fuss_autotune may choose 4 or 16
+constexpr int KC_DEFAULT = 8;   // column height; fuss_autotune may choose 4, 16, 32 or 64
 constexpr int PX = TILE_X + 2, PY = TILE_Y + 2;
 
 template <typename R>
@@ -3577,6 +3577,8 @@ void launch_stage3d_k(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool ta
 void launch_stage3d(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang, bool prim, bool irs) {
   if (L.kc == 4) launch_stage3d_k<4>(c, L, a, tang, prim, irs);
   else if (L.kc == 16) launch_stage3d_k<16>(c, L, a, tang, prim, irs);
+  else if (L.kc == 32) launch_stage3d_k<32>(c, L, a, tang, prim, irs);
+  else if (L.kc == 64) launch_stage3d_k<64>(c, L, a, tang, prim, irs);
   else launch_stage3d_k<8>(c, L, a, tang, prim, irs);
 }
 
@@ -3584,6 +3586,8 @@ void launch_irs_z(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
   const unsigned g = (unsigned)L.hctiles.size();
   if (L.kc == 4) prim ? k_irs_z<true, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
   else if (L.kc == 16) prim ? k_irs_z<true, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
+  else if (L.kc == 32) prim ? k_irs_z<true, double, 32><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 32><<<g, TILE_T, 0, c->stream>>>(ia);
+  else if (L.kc == 64) prim ? k_irs_z<true, double, 64><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 64><<<g, TILE_T, 0, c->stream>>>(ia);
   else prim ? k_irs_z<true, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
 }
 
@@ -3592,6 +3596,8 @@ void launch_irs_xyz_x(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool pri
   const unsigned g = (unsigned)L.hctiles.size();
   if (L.kc == 4) prim ? k_irs_xyz<true, XCON, P2, DT, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
   else if (L.kc == 16) prim ? k_irs_xyz<true, XCON, P2, DT, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
+  else if (L.kc == 32) prim ? k_irs_xyz<true, XCON, P2, DT, double, 32><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 32><<<g, TILE_T, 0, c->stream>>>(ia);
+  else if (L.kc == 64) prim ? k_irs_xyz<true, XCON, P2, DT, double, 64><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 64><<<g, TILE_T, 0, c->stream>>>(ia);
   else prim ? k_irs_xyz<true, XCON, P2, DT, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
 }
 
@@ -4664,11 +4670,11 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
   HIPCHK(hipEventCreate(&e1));
   fuss_step_params q = *p;
   q.compute_residual = 1;
-  const int kcs[3] = {4, 8, 16};
+  const int kcs[5] = {4, 8, 16, 32, 64};
   double best = 1e300;
   int bkc = L.kc, brow = 0, err = 0;
   for (int row = 0; row <= (L.tang ? 0 : 1) && !err; row++)
-    for (int ki = 0; ki < 3 && !err; ki++) {
+    for (int ki = 0; ki < 5 && !err; ki++) {
       if ((err = set_kc(c, L, kcs[ki]))) break;
       L.row_march = row;
       drop_dt_next(c);  // the state was restored

@@ -155,7 +155,7 @@ int fuss_rect(fuss_ctx *ctx, int enable);
 int fuss_rect_info(fuss_ctx *ctx, int level, int *weights, int *dt, double *max_rel_dev);
 
 /* Tile autotuner: times `steps` Explicit_Steps with each candidate of the
- * 3-D stage pass -- column height 4, 8 or 16 planes x LDS-ring or row-march
+ * 3-D stage pass -- column height 4, 8, 16, 32 or 64 planes x LDS-ring or row-march
  * kernel (row march: orthogonal levels only) -- on the level's current state,
  * restores the state and clock, and keeps the fastest for this level.  The
  * per-cell arithmetic is the same for every candidate; only the grouping of
