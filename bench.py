@@ -53,7 +53,8 @@ def stage_kernel(rect, kc):
     return f"k_stage3p<true, true, true, {'true' if rect else 'false'}, false, double, {kc}>"
 
 
-STAGE_TRAFFIC_MEASURED = {stage_kernel(True, 16): 149388 * 1024 * 2.0 + 131072 * 1024 * 1.0}
+STAGE_TRAFFIC_MEASURED = {stage_kernel(True, 16): 149388 * 1024 * 2.0 + 131072 * 1024 * 1.0,
+                          stage_kernel(True, 64): 140750 * 1024 * 2.0 + 131072 * 1024 * 1.0}
 # Plain FP64 path (four-stage RK + LTS + IRS 0.5, 3-level sequencing) run to
 # residual 1e-10 on the GPU: iterations per cube size (profiles/
 # steady_convergence_cube.md).  Fit: iters = 230218 (n/64)^PLAIN_EXP.
@@ -255,7 +256,8 @@ def sweep_check(args):
     out = {"workload": f"{args.sweep_cases} independent steady cubes, dims 32..96 (multiples of 4), "
                        f"k 5-200, h 10-500, qvol 0-1e6, RK{args.rk} + IRS 0.5, one context",
            "cells": ncell, "setup_s": setup_s, "plain_fp64_ms_per_step": step_s * 1e3,
-           "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring"},
+           "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring",
+                     "irs_rows": s.tiles(1)[2]},
            "plain_fp64_cell_updates_per_s": ncell / step_s}
     finals = {}
     for prec, name in ((gpu.FUSS_FP32, "fp32_refined"), (gpu.FUSS_FP64, "fp64_refined")):
@@ -322,6 +324,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="also run 1 GiB D2D copies (counter calibration)")
     ap.add_argument("--rk", type=int, default=4)
+    ap.add_argument("--kc", type=int, default=0, help="column height (4..64); with --irs-rows: tiles without autotune")
+    ap.add_argument("--irs-rows", type=int, default=4, help="rows of the fused IRS pass's columns (4 or 8)")
+    ap.add_argument("--row-march", action="store_true", help="with --kc: the row-march stage kernel")
     ap.add_argument("--no-rect", action="store_true",
                     help="stream the stored 3-D metric arrays instead of the per-axis factors")
     ap.add_argument("--no-refine", action="store_true", help="skip the time-to-1e-10 runs")
@@ -353,9 +358,16 @@ def main():
     if args.no_rect:
         s.rect(False)
         rect = (False, False, rect[2])
-    tuned = s.autotune(1, steps=4)
-    print(f"[bench] autotune: column height {tuned[0]}, {'row march' if tuned[1] else 'LDS ring'}, "
-          f"{tuned[2]:.3f} ms/step", file=sys.stderr, flush=True)
+    if args.kc:
+        s.set_tiles(1, kc=args.kc, row_march=int(args.row_march), irs_rows=args.irs_rows)
+        tuned = (args.kc, int(args.row_march), 0.0)
+        chosen = "fixed (--kc)"
+    else:
+        tuned = s.autotune(1, steps=4)
+        chosen = "fuss_autotune"
+    irs_rows = s.tiles(1)[2]
+    print(f"[bench] tiles ({chosen}): column height {tuned[0]}, {'row march' if tuned[1] else 'LDS ring'}, "
+          f"IRS columns of {irs_rows} rows, {tuned[2]:.3f} ms/step", file=sys.stderr, flush=True)
     # warm-up: module load and the capture of fuss_run's step graph
     s.run_level(1, max(args.warmup, 1), max(args.warmup, 1), -1.0, 1e300, hist=False)
     device_sync()
@@ -381,8 +393,10 @@ def main():
     value = ncell * args.steps / wall
     print(f"[bench] {args.steps} steps in {wall:.3f} s, residual {np.sqrt(res.res_sq):.4e}", file=sys.stderr,
           flush=True)
-    names = ["dt", "ghost", "stage", "irs", "irs_update", "reduce", "bc"]
-    kern = {names[q]: {"ms_avg": ms[q] / cnt[q], "launches": cnt[q]} for q in range(7) if cnt[q]}
+    # "stage": the stage kernel without dt (stages 2-4 of a steady step on a
+    # rectilinear level, every stage otherwise); "stage_dt": stage 1 with dt
+    names = ["dt", "ghost", "stage", "irs", "irs_update", "reduce", "bc", "stage_dt"]
+    kern = {names[q]: {"ms_avg": ms[q] / cnt[q], "launches": cnt[q]} for q in range(8) if cnt[q]}
     # one material and a uniform qvol: the stage kernel reads neither stream;
     # on a rectilinear level it reads no per-cell geometry either
     stage_bpc = perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False, qvol=False, rect=rect[0])
@@ -409,7 +423,7 @@ def main():
                                "finest-level Explicit_Step",
                    "cells": ncell, "parallelism": "single-gpu",
                    "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring",
-                             "chosen_by": "fuss_autotune"}},
+                             "irs_rows": irs_rows, "chosen_by": chosen}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": STAGE_TRAFFIC_MEASURED.get(skern) if ldsring else None,

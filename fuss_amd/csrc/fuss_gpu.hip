@@ -751,26 +751,29 @@ __device__ __forceinline__ R scaled_residual(const Tab<R>& tab, int m, R T, R Rv
 }
 
 // Work-group minimum to pmin[wg] (as k_dt's per-tile minimum).
+// NT: threads of the work-group (a multiple of 64)
+template <int NT = TILE_T>
 __device__ __forceinline__ void block_min(double v, double* pmin, int wg) {
-  __shared__ double sm[TILE_T / 64];
+  __shared__ double sm[NT / 64];
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = sm[0];
-    for (int q = 1; q < TILE_T / 64; q++) s = fmin(s, sm[q]);
+    for (int q = 1; q < NT / 64; q++) s = fmin(s, sm[q]);
     pmin[wg] = s;
   }
 }
 
+template <int NT = TILE_T>
 __device__ __forceinline__ void block_sum(double v, double* partials, int wg) {
-  __shared__ double sm[TILE_T / 64];
+  __shared__ double sm[NT / 64];
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = sm[0];
-    for (int q = 1; q < TILE_T / 64; q++) s += sm[q];
+    for (int q = 1; q < NT / 64; q++) s += sm[q];
     partials[wg] = s;
   }
 }
@@ -2044,9 +2047,9 @@ __global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
 
 // All three directions of Residual_Smoothing (Lib_IRS.f90:14-40) and the RK
 // update in one pass, for 3-D levels without connections on j or k faces: a
-// 64x4 column marches through its KC planes.  Each R plane enters LDS once
+// 64xTY column marches through its KC planes.  Each R plane enters LDS once
 // with a 2-cell i/j halo, is smoothed along x then y there (the y ghosts are
-// the x-smoothed boundary rows, as in k_irs_xy), and its 64x4 result joins a
+// the x-smoothed boundary rows, as in k_irs_xy), and its 64xTY result joins a
 // register window s(q-4..q) that the z sweeps read (z ghosts: the
 // xy-smoothed boundary planes).  R is read from HBM once per stage instead of
 // three times (x-y pass in, z pass in) plus once written.  Planes are
@@ -2054,41 +2057,51 @@ __global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
 // unrolled loop; every load is unconditional (clamped plane, valid cells for
 // out-of-block lanes) so the waits leave the later planes in flight.
 // XCON: some block has a connection on an i face (x ghosts from the donors).
-constexpr int XYZ_RJ = TILE_Y + 4, XYZ_RI = TILE_X + 4;
-constexpr int XYZ_NLD = (XYZ_RJ * XYZ_RI + TILE_T - 1) / TILE_T;  // plane loads per thread
-
-template <typename R>
-struct XyzPlane {
-  R v[XYZ_NLD];
+// TY: rows of the column (4 or 8; one wave per row): the 8-row column reads
+// 12 instead of 16 halo rows per 8 rows of output and sweeps 12 x lines.
+template <int TY>
+struct Xyz {
+  static constexpr int TT = TILE_X * TY;                // threads
+  static constexpr int RJ = TY + 4, RI = TILE_X + 4;    // halo box
+  static constexpr int NLD = (RJ * RI + TT - 1) / TT;   // plane loads per thread
 };
 
-// Element u of a thread's share of the tile's (TILE_Y+4) x (TILE_X+4) halo
-// box lies in the block.
+template <typename R, int NLD>
+struct XyzPlane {
+  R v[NLD];
+};
+
+// Element u of a thread's share of the tile's (TY+4) x (TILE_X+4) halo box
+// lies in the block.
+template <int TY>
 __device__ __forceinline__ bool xyz_in(const BlockInfo& B, int i0, int j0, int u) {
-  const int e = threadIdx.x + u * TILE_T;
-  const int gi = i0 - 2 + e % XYZ_RI, gj = j0 - 2 + e / XYZ_RI;
-  return e < XYZ_RJ * XYZ_RI && gi >= 1 && gi <= B.ni && gj >= 1 && gj <= B.nj;
+  using G = Xyz<TY>;
+  const int e = threadIdx.x + u * G::TT;
+  const int gi = i0 - 2 + e % G::RI, gj = j0 - 2 + e / G::RI;
+  return e < G::RJ * G::RI && gi >= 1 && gi <= B.ni && gj >= 1 && gj <= B.nj;
 }
 
 // The R values of plane q in the halo box, raw: cells outside the block read
 // the box's first in-block cell, and the LDS store writes 0 for them (the
 // mask is applied there, so no wait for these loads is placed before).
-template <typename R>
-__device__ __forceinline__ void xyz_fetch(XyzPlane<R>& r, const R* __restrict__ Rin, const BlockInfo& B, int i0,
-                                          int j0, int q) {
+template <int TY, typename R>
+__device__ __forceinline__ void xyz_fetch(XyzPlane<R, Xyz<TY>::NLD>& r, const R* __restrict__ Rin,
+                                          const BlockInfo& B, int i0, int j0, int q) {
+  using G = Xyz<TY>;
   const unsigned corner = (unsigned)(B.base + (i0 - 3) + (long)(j0 - 3) * B.sy + (long)(q - 1) * B.sz);
   // first in-block cell of the box: (max(i0-2,1), max(j0-2,1))
   const unsigned safe = corner + (i0 > 2 ? 0u : (unsigned)(3 - i0)) + (j0 > 2 ? 0u : (unsigned)(3 - j0)) * (unsigned)B.sy;
 #pragma unroll
-  for (int u = 0; u < XYZ_NLD; u++) {
-    const int e = threadIdx.x + u * TILE_T;
-    const int ii = e % XYZ_RI, jj = e / XYZ_RI;
-    r.v[u] = ldu(Rin, xyz_in(B, i0, j0, u) ? corner + ii + jj * (unsigned)B.sy : safe);
+  for (int u = 0; u < G::NLD; u++) {
+    const int e = threadIdx.x + u * G::TT;
+    const int ii = e % G::RI, jj = e / G::RI;
+    r.v[u] = ldu(Rin, xyz_in<TY>(B, i0, j0, u) ? corner + ii + jj * (unsigned)B.sy : safe);
   }
 }
 
-template <bool PRIM, bool XCON, bool P2, bool DT, typename R, int KCT>
-__global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
+template <bool PRIM, bool XCON, bool P2, bool DT, typename R, int KCT, int TY>
+__global__ __launch_bounds__(Xyz<TY>::TT) void k_irs_xyz(IrsArgs<R> a) {
+  using G = Xyz<TY>;
   if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const Tile t = a.tiles[wg];
@@ -2097,8 +2110,8 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
   const int i = t.i0 + tx, j = t.j0 + ty;
   const int n = B.nk, k0 = t.k0, k1 = min(n, k0 + KCT - 1);
   const bool live = i <= B.ni && j <= B.nj;
-  __shared__ R rin[XYZ_RJ][XYZ_RI];
-  __shared__ R xs[XYZ_RJ][TILE_X];
+  __shared__ R rin[G::RJ][G::RI];
+  __shared__ R xs[G::RJ][TILE_X];
   const R beta = a.beta, den = a.den, inv = a.inv;
   // window w[0..4] = s(q-4 .. q); planes outside 1..n are 0 (jacobi2 selects
   // the ghosts there)
@@ -2119,20 +2132,20 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
     if (threadIdx.x < KCT) qzs[threadIdx.x] = rx_.qz[min(k0 + (int)threadIdx.x, n)];
     qxy = rx_.qx[ci] + rx_.qy[cj];
   }
-  bool inb[XYZ_NLD];
+  bool inb[G::NLD];
 #pragma unroll
-  for (int u = 0; u < XYZ_NLD; u++) inb[u] = xyz_in(B, t.i0, t.j0, u);
+  for (int u = 0; u < G::NLD; u++) inb[u] = xyz_in<TY>(B, t.i0, t.j0, u);
   // the tile's x lines and the wave's y line (one row j per wave) away from
   // the block faces: no ghost selection
   const bool xin = t.i0 >= 3 && t.i0 + TILE_X - 1 <= B.ni - 2;
   const int ju = t.j0 + __builtin_amdgcn_readfirstlane(ty);
   const bool yin = ju >= 3 && ju <= B.nj - 2;
-  XyzPlane<R> pa, pb;
-  xyz_fetch(pa, a.Rin, B, t.i0, t.j0, qa);
-  xyz_fetch(pb, a.Rin, B, t.i0, t.j0, min(qa + 1, qb));
+  XyzPlane<R, G::NLD> pa, pb;
+  xyz_fetch<TY>(pa, a.Rin, B, t.i0, t.j0, qa);
+  xyz_fetch<TY>(pb, a.Rin, B, t.i0, t.j0, min(qa + 1, qb));
   // one step: plane q (staged in `pre`) is smoothed in x-y (if q <= qb), the
   // window shifts, and plane q-2 is output (if it is in the column)
-  auto step = [&](XyzPlane<R>& pre, int q) {
+  auto step = [&](XyzPlane<R, G::NLD>& pre, int q) {
     const int ko = q - 2;
     const bool out = ko >= k0;
     const unsigned po = p1 + (unsigned)(max(ko, 1) - 1) * sz;
@@ -2144,22 +2157,23 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
     R s = 0;
     if (q <= qb) {
 #pragma unroll
-      for (int u = 0; u < XYZ_NLD; u++) {
-        const int e = threadIdx.x + u * TILE_T;
-        if (e < XYZ_RJ * XYZ_RI) (&rin[0][0])[e] = inb[u] ? pre.v[u] : (R)0;
+      for (int u = 0; u < G::NLD; u++) {
+        const int e = threadIdx.x + u * G::TT;
+        if (e < G::RJ * G::RI) (&rin[0][0])[e] = inb[u] ? pre.v[u] : (R)0;
       }
     }
     // plane q+2 flies while this one is smoothed (issued on every step, so
     // the wait for T and TO below can leave it outstanding)
-    xyz_fetch(pre, a.Rin, B, t.i0, t.j0, min(q + 2, qb));
+    xyz_fetch<TY>(pre, a.Rin, B, t.i0, t.j0, min(q + 2, qb));
     if (q <= qb) {
       __syncthreads();
-      // x sweeps of rows j0-2 .. j0+5 (two per thread); a tile whose cells
-      // are all >= 2 cells from the i faces selects no ghost (uniform branch)
+      // x sweeps of rows j0-2 .. j0+TY+1 (wave ty: rows ty, ty+TY, ...); a
+      // tile whose cells are all >= 2 cells from the i faces selects no ghost
+      // (uniform branch)
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int jj = ty + h * TILE_Y, jr = t.j0 - 2 + jj;
-        if (i <= B.ni && jr >= 1 && jr <= B.nj) {
+      for (int h = 0; h < (G::RJ + TY - 1) / TY; h++) {
+        const int jj = ty + h * TY, jr = t.j0 - 2 + jj;
+        if (jj < G::RJ && i <= B.ni && jr >= 1 && jr <= B.nj) {
           if (xin) {
             xs[jj][tx] = jacobi2_in<P2>(rin[jj][tx], rin[jj][tx + 1], rin[jj][tx + 2], rin[jj][tx + 3],
                                         rin[jj][tx + 4], beta, den, inv);
@@ -2239,8 +2253,8 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xyz(IrsArgs<R> a) {
     step(pa, q);
     if (q + 1 <= qe) step(pb, q + 1);
   }
-  if (a.last) block_sum(rsq, a.partials, wg);
-  if (DT) block_min(dtmin, a.pmin, wg);
+  if (a.last) block_sum<G::TT>(rsq, a.partials, wg);
+  if (DT) block_min<G::TT>(dtmin, a.pmin, wg);
 }
 
 // ------------------------------------------------------------------------
@@ -2516,11 +2530,14 @@ struct Level {
   std::vector<HostBlock> hb;        // host nodes (every level), fields (level 1)
   std::vector<Tile> htiles;         // one 64x4 cell tile per plane
   std::vector<Tile> hctiles;        // 64x4 columns of kc planes (3-D levels)
+  std::vector<Tile> hc8tiles;       // 64x8 columns of kc planes (fused x-y-z IRS pass)
   int kc = KC_DEFAULT;              // planes per column tile (fuss_autotune)
+  int xyz_ty = 4;                   // rows of the fused IRS pass's columns, 4 or 8 (fuss_autotune)
+  int dt_ty = 4;                    // ... of the pass that wrote the dt minima (dt_next)
   int row_march = -1;               // orthogonal 3-D stage kernel: -1 context default, 0 LDS ring, 1 row march
   std::vector<Tile> hxytiles;       // 64x16 plane tiles of the x-y IRS pass
-  std::vector<int> hrng[3];         // per tile list (tiles, ctiles, xytiles): first tile of each block
-  DBuf<int> rng[3];
+  std::vector<int> hrng[4];         // per tile list (tiles, ctiles, xytiles, c8tiles): first tile of each block
+  DBuf<int> rng[4];
   int plist = 0;                    // tile list of the last step's residual partials
   int no_ycon = 1;                  // no connection on a j face (fused x-y IRS allowed)
   int no_zcon = 1;                  // no connection on a k face (with no_ycon: fused x-y-z IRS)
@@ -2545,7 +2562,7 @@ struct Level {
   double rect_dev = -1.0;           // largest relative deviation found (-1: not checked)
   long npad = 0, ncell = 0, nnodes = 0;
   DBuf<BlockInfo> blk;
-  DBuf<Tile> tiles, ctiles, xytiles;
+  DBuf<Tile> tiles, ctiles, xytiles, c8tiles;
   DBuf<double> nodes;
   DBuf<double> Tb[3];               // T^n and two stage buffers
   int cur = 0;                      // index of T^n in Tb
@@ -2882,11 +2899,36 @@ void run_graph_free(Level& L) {
   L.run_graph = nullptr;
 }
 
+// hrng[q]: index of the first tile of each block in a block-ordered list
+void block_ranges(Level& L, int q, const std::vector<Tile>& list) {
+  L.hrng[q].assign(L.nb + 1, (int)list.size());
+  for (int t = (int)list.size() - 1; t >= 0; t--) L.hrng[q][list[t].b] = t;
+  for (int b = L.nb - 1; b >= 0; b--) L.hrng[q][b] = std::min(L.hrng[q][b], L.hrng[q][b + 1]);
+}
+
+// Column tiles of kc planes on 3-D levels: 64x4 (stage passes, z pass) and
+// 64x8 (fused x-y-z IRS pass), with their block ranges.
+void column_tiles(Level& L) {
+  L.hctiles.clear();
+  L.hc8tiles.clear();
+  if (L.kp == GC)
+    for (int b = 0; b < L.nb; b++) {
+      const BlockInfo& B = L.hblk[b];
+      for (int k = 1; k <= B.nk; k += L.kc) {
+        for (int j = 1; j <= B.nj; j += TILE_Y)
+          for (int i = 1; i <= B.ni; i += TILE_X) L.hctiles.push_back(Tile{b, i, j, k});
+        for (int j = 1; j <= B.nj; j += 2 * TILE_Y)
+          for (int i = 1; i <= B.ni; i += TILE_X) L.hc8tiles.push_back(Tile{b, i, j, k});
+      }
+    }
+  block_ranges(L, 1, L.hctiles);
+  block_ranges(L, 3, L.hc8tiles);
+}
+
 int build_level_layout(fuss_ctx* c, Level& L) {
   long off = 0, noff = 0;
   L.hblk.resize(L.nb);
   L.htiles.clear();
-  L.hctiles.clear();
   L.hxytiles.clear();
   for (int b = 0; b < L.nb; b++) {
     BlockInfo& B = L.hblk[b];
@@ -2917,22 +2959,15 @@ int build_level_layout(fuss_ctx* c, Level& L) {
     for (int k = 1; k <= B.nk; k++)
       for (int j = 1; j <= B.nj; j += XY_TJ)
         for (int i = 1; i <= B.ni; i += TILE_X) L.hxytiles.push_back(Tile{b, i, j, k});
-    if (L.kp == GC)
-      for (int k = 1; k <= B.nk; k += L.kc)
-        for (int j = 1; j <= B.nj; j += TILE_Y)
-          for (int i = 1; i <= B.ni; i += TILE_X) L.hctiles.push_back(Tile{b, i, j, k});
     L.ncell += (long)B.ni * B.nj * B.nk;
   }
   L.npad = off;
   L.nnodes = noff;
   L.dims2 = 1;
   for (int b = 0; b < L.nb; b++) L.dims2 &= L.hblk[b].ni >= 2 && L.hblk[b].nj >= 2 && L.hblk[b].nk >= 2;
-  const std::vector<Tile>* lists[3] = {&L.htiles, &L.hctiles, &L.hxytiles};
-  for (int q = 0; q < 3; q++) {
-    L.hrng[q].assign(L.nb + 1, (int)lists[q]->size());
-    for (int t = (int)lists[q]->size() - 1; t >= 0; t--) L.hrng[q][(*lists[q])[t].b] = t;
-    for (int b = L.nb - 1; b >= 0; b--) L.hrng[q][b] = std::min(L.hrng[q][b], L.hrng[q][b + 1]);
-  }
+  block_ranges(L, 0, L.htiles);
+  block_ranges(L, 2, L.hxytiles);
+  column_tiles(L);
   (void)c;
   // kernels address a level's arrays with 32-bit byte offsets
   if (L.npad * 8 >= (1L << 32)) return FUSS_EUNSUPPORTED;
@@ -2956,12 +2991,13 @@ int alloc_level(fuss_ctx* c, Level& L) {
   e |= L.tiles.alloc(L.htiles.size());
   e |= L.ctiles.alloc(std::max<size_t>(L.hctiles.size(), 1));
   e |= L.xytiles.alloc(std::max<size_t>(L.hxytiles.size(), 1));
+  e |= L.c8tiles.alloc(std::max<size_t>(L.hc8tiles.size(), 1));
   const size_t np = std::max<size_t>(std::max(L.htiles.size(), L.hctiles.size()), 1);
   e |= L.partials.alloc(np);
   e |= L.pmin.alloc(np);
   e |= L.clk.alloc(2);
   e |= L.cnt.alloc(5);
-  for (int q = 0; q < 3; q++) {
+  for (int q = 0; q < 4; q++) {
     e |= L.rng[q].alloc(L.nb + 1);
     if (!e) e |= upload(L.rng[q].p, L.hrng[q].data(), (L.nb + 1) * sizeof(int));
   }
@@ -3336,6 +3372,7 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     if (e) return e;
     e = upload(L.tiles.p, L.htiles.data(), sizeof(Tile) * L.htiles.size());
     if (!L.hctiles.empty()) e |= upload(L.ctiles.p, L.hctiles.data(), sizeof(Tile) * L.hctiles.size());
+    if (!L.hc8tiles.empty()) e |= upload(L.c8tiles.p, L.hc8tiles.data(), sizeof(Tile) * L.hc8tiles.size());
     e |= upload(L.xytiles.p, L.hxytiles.data(), sizeof(Tile) * L.hxytiles.size());
     // nodes
     std::vector<double> nodes(3 * L.nnodes);
@@ -3591,27 +3628,38 @@ void launch_irs_z(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
   else prim ? k_irs_z<true, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
 }
 
-template <bool XCON, bool P2, bool DT>
+template <bool XCON, bool P2, bool DT, int TY>
 void launch_irs_xyz_x(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
-  const unsigned g = (unsigned)L.hctiles.size();
-  if (L.kc == 4) prim ? k_irs_xyz<true, XCON, P2, DT, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
-  else if (L.kc == 16) prim ? k_irs_xyz<true, XCON, P2, DT, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
-  else if (L.kc == 32) prim ? k_irs_xyz<true, XCON, P2, DT, double, 32><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 32><<<g, TILE_T, 0, c->stream>>>(ia);
-  else if (L.kc == 64) prim ? k_irs_xyz<true, XCON, P2, DT, double, 64><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 64><<<g, TILE_T, 0, c->stream>>>(ia);
-  else prim ? k_irs_xyz<true, XCON, P2, DT, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
+  const unsigned g = (unsigned)(TY == 8 ? L.hc8tiles.size() : L.hctiles.size());
+  constexpr int T = Xyz<TY>::TT;
+  if (L.kc == 4) prim ? k_irs_xyz<true, XCON, P2, DT, double, 4, TY><<<g, T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 4, TY><<<g, T, 0, c->stream>>>(ia);
+  else if (L.kc == 16) prim ? k_irs_xyz<true, XCON, P2, DT, double, 16, TY><<<g, T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 16, TY><<<g, T, 0, c->stream>>>(ia);
+  else if (L.kc == 32) prim ? k_irs_xyz<true, XCON, P2, DT, double, 32, TY><<<g, T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 32, TY><<<g, T, 0, c->stream>>>(ia);
+  else if (L.kc == 64) prim ? k_irs_xyz<true, XCON, P2, DT, double, 64, TY><<<g, T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 64, TY><<<g, T, 0, c->stream>>>(ia);
+  else prim ? k_irs_xyz<true, XCON, P2, DT, double, 8, TY><<<g, T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 8, TY><<<g, T, 0, c->stream>>>(ia);
 }
 
-template <bool XCON, bool P2>
+template <bool XCON, bool P2, int TY>
 void launch_irs_xyz_p(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
-  ia.dtnext ? launch_irs_xyz_x<XCON, P2, true>(c, L, ia, prim) : launch_irs_xyz_x<XCON, P2, false>(c, L, ia, prim);
+  ia.dtnext ? launch_irs_xyz_x<XCON, P2, true, TY>(c, L, ia, prim)
+            : launch_irs_xyz_x<XCON, P2, false, TY>(c, L, ia, prim);
+}
+
+template <int TY>
+void launch_irs_xyz_t(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
+  const bool p2 = ia.inv != 0.0;
+  if (L.no_xcon)
+    p2 ? launch_irs_xyz_p<false, true, TY>(c, L, ia, prim) : launch_irs_xyz_p<false, false, TY>(c, L, ia, prim);
+  else
+    p2 ? launch_irs_xyz_p<true, true, TY>(c, L, ia, prim) : launch_irs_xyz_p<true, false, TY>(c, L, ia, prim);
 }
 
 // P2: 1 + 2 beta is a power of two (beta = 0.5: 2), the sweeps multiply by
-// its exact reciprocal; DT: the pass also computes the next step's dt
+// its exact reciprocal; DT: the pass also computes the next step's dt.  The
+// caller passes the tile list of L.xyz_ty (ctiles for 4 rows, c8tiles for 8).
 void launch_irs_xyz(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
-  const bool p2 = ia.inv != 0.0;
-  if (L.no_xcon) p2 ? launch_irs_xyz_p<false, true>(c, L, ia, prim) : launch_irs_xyz_p<false, false>(c, L, ia, prim);
-  else p2 ? launch_irs_xyz_p<true, true>(c, L, ia, prim) : launch_irs_xyz_p<true, false>(c, L, ia, prim);
+  if (L.xyz_ty == 8) launch_irs_xyz_t<8>(c, L, ia, prim);
+  else launch_irs_xyz_t<4>(c, L, ia, prim);
 }
 
 template <int DIR, int NDIR>
@@ -3790,8 +3838,11 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
   const bool have_dt = fuse_dt && L.dt_next && L.dt_vnn == p->vnn;
   unsigned nmin = ntile;
   int pid = -1;
-  if (have_dt || sdt) {
+  if (sdt) {
     nmin = (unsigned)L.hctiles.size();
+  } else if (have_dt) {
+    // minima of the fused IRS pass's tiles (its list when dt was written)
+    nmin = (unsigned)(L.dt_ty == 8 ? L.hc8tiles.size() : L.hctiles.size());
   } else {
     // Compute_dt + min (Mod_dt.f90:10-84); dtglobal starts at 1e5 (Mod_Explicit.f90:40)
     DtArgs<double> da{};
@@ -3861,7 +3912,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
     sa.jam = nrk == 4;
     sa.last = irk == nrk && p->compute_residual;
     sa.ctiles = L.ctiles.p;
-    pid = prof_begin(c, FUSS_K_STAGE);
+    pid = prof_begin(c, (sdt && irk == 1) ? FUSS_K_STAGE_DT : FUSS_K_STAGE);
     if (hk) {
       // Fluxes + BC_Fluxes to R, External_Function on the host, RK_Newstate
       // (Mod_Explicit.f90:53-59)
@@ -3917,8 +3968,9 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
       } else if (xyz) {
         // x, y and z in one k-marching pass with the update (and on the last
         // stage the next step's dt)
+        const bool c8 = L.xyz_ty == 8;
         ia.Rin = L.R.p;
-        ia.tiles = L.ctiles.p;
+        ia.tiles = c8 ? L.c8tiles.p : L.ctiles.p;
         ia.dtnext = fuse_dt && irk == nrk;
         ia.vnn = p->vnn; ia.invL2 = L.invL2.p; ia.dt = L.dt.p; ia.pmin = L.pmin.p;
         ia.rect = (L.rect_dt && !c->rect_off) ? L.rfac.p : nullptr;
@@ -3926,7 +3978,11 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
         launch_irs_xyz(c, L, ia, prim);
         prof_end(c, pid);
         HIPCHK(hipGetLastError());
-        if (sa.last) { npart = (int)L.hctiles.size(); L.plist = 1; }
+        if (ia.dtnext) L.dt_ty = L.xyz_ty;
+        if (sa.last) {
+          npart = (int)(c8 ? L.hc8tiles.size() : L.hctiles.size());
+          L.plist = c8 ? 3 : 1;
+        }
       } else if (L.no_ycon && c->ndir >= 2) {
         // x and y in one LDS pass; 2-D levels update there, 3-D levels then
         // march the z pass with the update
@@ -4628,19 +4684,13 @@ int set_kc(fuss_ctx* c, Level& L, int kc) {
   if (L.kp != GC) return 0;
   run_graph_free(L);  // its kernels address the old tile list
   L.kc = kc;
-  L.hctiles.clear();
-  for (int b = 0; b < L.nb; b++) {
-    const BlockInfo& B = L.hblk[b];
-    for (int k = 1; k <= B.nk; k += kc)
-      for (int j = 1; j <= B.nj; j += TILE_Y)
-        for (int i = 1; i <= B.ni; i += TILE_X) L.hctiles.push_back(Tile{b, i, j, k});
-  }
-  L.hrng[1].assign(L.nb + 1, (int)L.hctiles.size());
-  for (int t = (int)L.hctiles.size() - 1; t >= 0; t--) L.hrng[1][L.hctiles[t].b] = t;
-  for (int b = L.nb - 1; b >= 0; b--) L.hrng[1][b] = std::min(L.hrng[1][b], L.hrng[1][b + 1]);
+  column_tiles(L);
   if (L.ctiles.alloc(std::max<size_t>(L.hctiles.size(), 1))) return FUSS_EHIP;
+  if (L.c8tiles.alloc(std::max<size_t>(L.hc8tiles.size(), 1))) return FUSS_EHIP;
   HIPCHK(hipMemcpy(L.ctiles.p, L.hctiles.data(), L.hctiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L.c8tiles.p, L.hc8tiles.data(), L.hc8tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(L.rng[1].p, L.hrng[1].data(), (L.nb + 1) * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(L.rng[3].p, L.hrng[3].data(), (L.nb + 1) * sizeof(int), hipMemcpyHostToDevice));
   const size_t np = std::max<size_t>(std::max(L.htiles.size(), L.hctiles.size()), 1);
   if (L.partials.n < np && (L.partials.alloc(np) || L.pmin.alloc(np) || L.red2.alloc(np / RED_CHUNK + 2)))
     return FUSS_EHIP;
@@ -4672,24 +4722,34 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
   q.compute_residual = 1;
   const int kcs[5] = {4, 8, 16, 32, 64};
   double best = 1e300;
-  int bkc = L.kc, brow = 0, err = 0;
+  int bkc = L.kc, brow = 0, bty = 4, err = 0;
+  auto trial = [&](int row, int kc, int ty) -> int {
+    int e = set_kc(c, L, kc);
+    if (e) return e;
+    L.row_march = row;
+    L.xyz_ty = ty;
+    drop_dt_next(c);  // the state was restored
+    if ((e = enqueue_step(c, level, &q))) return e;  // warm-up
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int s2 = 0; s2 < steps && !e; s2++) e = enqueue_step(c, level, &q);
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    const double per = ms / std::max(steps, 1);
+    if (!e && per < best) { best = per; bkc = kc; brow = row; bty = ty; }
+    HIPCHK(hipMemcpy(L.Tb[L.cur].p, save.p, L.npad * sizeof(double), hipMemcpyDeviceToDevice));
+    set_clock(c, L, t0, it0);
+    return e;
+  };
+  // stage kernel and column height with 4-row IRS columns, then the column
+  // heights with 8-row IRS columns at the chosen stage kernel (the fused
+  // x-y-z IRS pass is the only user of the 8-row columns)
   for (int row = 0; row <= (L.tang ? 0 : 1) && !err; row++)
-    for (int ki = 0; ki < 5 && !err; ki++) {
-      if ((err = set_kc(c, L, kcs[ki]))) break;
-      L.row_march = row;
-      drop_dt_next(c);  // the state was restored
-      if ((err = enqueue_step(c, level, &q))) break;  // warm-up
-      HIPCHK(hipEventRecord(e0, c->stream));
-      for (int s2 = 0; s2 < steps && !err; s2++) err = enqueue_step(c, level, &q);
-      HIPCHK(hipEventRecord(e1, c->stream));
-      HIPCHK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-      const double per = ms / std::max(steps, 1);
-      if (per < best) { best = per; bkc = kcs[ki]; brow = row; }
-      HIPCHK(hipMemcpy(L.Tb[L.cur].p, save.p, L.npad * sizeof(double), hipMemcpyDeviceToDevice));
-      set_clock(c, L, t0, it0);
-    }
+    for (int ki = 0; ki < 5 && !err; ki++) err = trial(row, kcs[ki], 4);
+  const int row4 = brow;
+  const bool xyz = p->irs_beta > 0.0 && p->irs_mode == FUSS_IRS_JACOBI2 && L.no_ycon && L.no_zcon && c->ndir == 3;
+  for (int ki = 0; ki < 5 && !err && xyz; ki++) err = trial(row4, kcs[ki], 8);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   save.free();
@@ -4699,6 +4759,7 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
   if (st) return st;
   if ((err = set_kc(c, L, bkc))) return err;
   L.row_march = brow;
+  L.xyz_ty = bty;
   if (kc_out) *kc_out = bkc;
   if (row_out) *row_out = brow;
   if (ms_out) *ms_out = best;
@@ -4714,6 +4775,31 @@ int fuss_autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, 
   if (c) drop_dt_next(c);
   if (!c || !c->setup_done || !p || level < 1 || level > c->nlev || steps < 1) return FUSS_EARG;
   return autotune(c, level, p, steps, kc, row_march, ms_per_step);
+}
+
+int fuss_set_tiles(fuss_ctx* c, int level, int kc, int row_march, int irs_rows) {
+  if (c) drop_dt_next(c);
+  if (!c || !c->setup_done || level < 1 || level > c->nlev) return FUSS_EARG;
+  if ((kc != 4 && kc != 8 && kc != 16 && kc != 32 && kc != 64) || row_march < -1 || row_march > 1 ||
+      (irs_rows != 4 && irs_rows != 8))
+    return FUSS_EARG;
+  Level& L = c->lev[level - 1];
+  if (L.kp != GC) return 0;  // 2-D levels have no column tiles
+  const int e = set_kc(c, L, kc);
+  if (e) return e;
+  L.row_march = row_march;
+  L.xyz_ty = irs_rows;
+  return 0;
+}
+
+int fuss_tiles(fuss_ctx* c, int level, int* kc, int* row_march, int* irs_rows) {
+  if (!c || !c->setup_done || level < 1 || level > c->nlev) return FUSS_EARG;
+  const Level& L = c->lev[level - 1];
+  const bool col = L.kp == GC;
+  if (kc) *kc = col ? L.kc : 0;
+  if (row_march) *row_march = col ? L.row_march : 0;
+  if (irs_rows) *irs_rows = col ? L.xyz_ty : 0;
+  return 0;
 }
 
 static int step_sync(fuss_ctx* c, int level, const fuss_step_params* p, fuss_step_result* r, const StageHook* hk) {
@@ -5075,6 +5161,8 @@ void fuss_destroy(fuss_ctx* c) {
     for (auto& f : L.F) f.free();
     for (auto& w : L.W) w.free();
     L.bc.free(); L.bcv.free(); L.bcf.free(); L.bch.free(); L.gmap.free(); L.fmap.free(); L.xytiles.free(); L.partials.free(); L.pmin.free(); L.red2.free(); L.ctiles.free();
+    L.c8tiles.free();
+    for (auto& r : L.rng) r.free();
   }
   c->dcp.free(); c->drho.free(); c->dkap.free(); c->dh.free(); c->dkap2.free(); c->dpr4.free();
   c->status.free(); c->scal.free();

@@ -156,12 +156,22 @@ int fuss_rect_info(fuss_ctx *ctx, int level, int *weights, int *dt, double *max_
 
 /* Tile autotuner: times `steps` Explicit_Steps with each candidate of the
  * 3-D stage pass -- column height 4, 8, 16, 32 or 64 planes x LDS-ring or row-march
- * kernel (row march: orthogonal levels only) -- on the level's current state,
- * restores the state and clock, and keeps the fastest for this level.  The
+ * kernel (row march: orthogonal levels only), then, where the fused x-y-z IRS
+ * pass runs, the column heights with its 8-row columns -- on the level's
+ * current state, restores the state and clock, and keeps the fastest for this level.  The
  * per-cell arithmetic is the same for every candidate; only the grouping of
  * the residual's partial sums changes.  2-D levels: nothing to tune (kc = 0). */
 int fuss_autotune(fuss_ctx *ctx, int level, const fuss_step_params *p, int steps, int *kc,
                   int *row_march, double *ms_per_step);
+/* Column tiles of a 3-D level without timing (no reference counterpart: the
+ * reference has no tiles): kc = 4, 8, 16, 32 or 64 planes; row_march = -1
+ * (context default), 0 (LDS ring) or 1 (row march); irs_rows = 4 or 8 rows
+ * per column of the fused x-y-z IRS pass (levels without j/k connections,
+ * Jacobi IRS).  fuss_autotune also chooses irs_rows; fuss_tiles reports the
+ * level's current choice (0s on 2-D levels).  Results do not depend on the
+ * tiles except for the grouping of the residual's partial sums. */
+int fuss_set_tiles(fuss_ctx *ctx, int level, int kc, int row_march, int irs_rows);
+int fuss_tiles(fuss_ctx *ctx, int level, int *kc, int *row_march, int *irs_rows);
 
 /* Restriction (Mod_Multigrid.f90:59-77): coarse T, qvol, matID from level
  * fine_level to fine_level+1. */
@@ -284,6 +294,7 @@ int fuss_set_stream(fuss_ctx *ctx, void *stream);
 #define FUSS_K_IRS_UPDATE 4  /* last IRS line pass fused with the RK update */
 #define FUSS_K_REDUCE 5      /* dt minimum and residual sum */
 #define FUSS_K_BC 6          /* boundary-face fluxes (surface pass) */
+#define FUSS_K_STAGE_DT 7    /* first stage computing the step's local dt (steady, rectilinear) */
 #define FUSS_K_NCLASS 8
 int fuss_profile(fuss_ctx *ctx, int enable);
 int fuss_profile_read(fuss_ctx *ctx, int nclass, double *ms, int *count);

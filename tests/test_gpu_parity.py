@@ -405,12 +405,43 @@ def test_autotune_keeps_results(gpu, name):
         s = gpu.Solver(SYNTH[name]())
         if tune:
             kc, row, ms = s.autotune(1, steps=2)
-            assert kc in (0, 4, 8, 16) and ms >= 0.0
+            assert kc in (0, 4, 8, 16, 32, 64) and ms >= 0.0
+            assert s.tiles(1)[0] == kc and s.tiles(1)[2] in ((0,) if kc == 0 else (4, 8))
         r = [s.explicit_step(1).res_sq for _ in range(6)]
         out.append((np.concatenate([T.ravel() for T in s.T()]), np.array(r)))
         s.close()
     assert np.array_equal(out[0][0], out[1][0])
     np.testing.assert_allclose(out[0][1], out[1][1], rtol=1e-13)
+
+
+@pytest.mark.parametrize("rect", [False, True])
+@pytest.mark.parametrize("name", ["cube3d_rk4_irs", "sweep3_rk4_irs", "cube3d_mg3_irs"])
+def test_irs_rows_keep_results(gpu, name, rect):
+    """The fused x-y-z IRS pass on 8-row columns (fuss_set_tiles) gives the
+    4-row columns' state bit for bit at each column height (ragged blocks:
+    rows past nj masked), dtglobal bit for bit (with the stored metrics the
+    pass leaves the next step's dt and one minimum per 8-row column), and
+    the residual to rounding (its partial sums group by column)."""
+    c = SYNTH[name]()
+    out = []
+    for kc, rows in ((8, 4), (4, 8), (16, 8), (64, 8)):
+        s = gpu.Solver(c)
+        s.rect(rect)
+        s.set_tiles(1, kc=kc, irs_rows=rows)
+        assert s.tiles(1) == (kc, -1, rows)
+        rs = [s.explicit_step(1) for _ in range(5)]
+        out.append((np.concatenate([T.ravel() for T in s.T()]), np.array([r.res_sq for r in rs]),
+                    np.array([r.dtglobal for r in rs])))
+        if rows == 8:
+            with pytest.raises(gpu.FussError):
+                s.set_tiles(1, kc=12)
+            with pytest.raises(gpu.FussError):
+                s.set_tiles(1, kc=8, irs_rows=6)
+        s.close()
+    for T, res, dt in out[1:]:
+        assert np.array_equal(T, out[0][0])
+        np.testing.assert_allclose(res, out[0][1], rtol=1e-13)
+        assert np.array_equal(dt, out[0][2])
 
 
 @pytest.mark.parametrize("name", ["cube3d_mg3_irs", "lattice_transient", "plate2d_64", "lattice_rk4_cons"])
