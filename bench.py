@@ -49,12 +49,12 @@ HBM_PEAK_GBS = 8000.0
 # factors (rectilinear level).
 
 
-def stage_kernel(rect, kc):
-    return f"k_stage3p<true, true, true, {'true' if rect else 'false'}, false, double, {kc}>"
+def stage_kernel(rect, kc, rows):
+    return f"k_stage3p<true, true, true, {'true' if rect else 'false'}, false, double, {kc}, {rows}>"
 
 
-STAGE_TRAFFIC_MEASURED = {stage_kernel(True, 16): 149388 * 1024 * 2.0 + 131072 * 1024 * 1.0,
-                          stage_kernel(True, 64): 140750 * 1024 * 2.0 + 131072 * 1024 * 1.0}
+STAGE_TRAFFIC_MEASURED = {stage_kernel(True, 16, 4): 149388 * 1024 * 2.0 + 131072 * 1024 * 1.0,
+                          stage_kernel(True, 64, 4): 140750 * 1024 * 2.0 + 131072 * 1024 * 1.0}
 # Plain FP64 path (four-stage RK + LTS + IRS 0.5, 3-level sequencing) run to
 # residual 1e-10 on the GPU: iterations per cube size (profiles/
 # steady_convergence_cube.md).  Fit: iters = 230218 (n/64)^PLAIN_EXP.
@@ -257,7 +257,7 @@ def sweep_check(args):
                        f"k 5-200, h 10-500, qvol 0-1e6, RK{args.rk} + IRS 0.5, one context",
            "cells": ncell, "setup_s": setup_s, "plain_fp64_ms_per_step": step_s * 1e3,
            "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring",
-                     "irs_rows": s.tiles(1)[2]},
+                     "stage_rows": s.tiles(1)[2], "irs_rows": s.tiles(1)[3]},
            "plain_fp64_cell_updates_per_s": ncell / step_s}
     finals = {}
     for prec, name in ((gpu.FUSS_FP32, "fp32_refined"), (gpu.FUSS_FP64, "fp64_refined")):
@@ -325,6 +325,7 @@ def main():
     ap.add_argument("--calib-copy", action="store_true", help="also run 1 GiB D2D copies (counter calibration)")
     ap.add_argument("--rk", type=int, default=4)
     ap.add_argument("--kc", type=int, default=0, help="column height (4..64); with --irs-rows: tiles without autotune")
+    ap.add_argument("--stage-rows", type=int, default=4, help="rows of the stage kernel's columns (4 or 8)")
     ap.add_argument("--irs-rows", type=int, default=4, help="rows of the fused IRS pass's columns (4 or 8)")
     ap.add_argument("--row-march", action="store_true", help="with --kc: the row-march stage kernel")
     ap.add_argument("--no-rect", action="store_true",
@@ -359,15 +360,17 @@ def main():
         s.rect(False)
         rect = (False, False, rect[2])
     if args.kc:
-        s.set_tiles(1, kc=args.kc, row_march=int(args.row_march), irs_rows=args.irs_rows)
+        s.set_tiles(1, kc=args.kc, row_march=int(args.row_march), stage_rows=args.stage_rows,
+                    irs_rows=args.irs_rows)
         tuned = (args.kc, int(args.row_march), 0.0)
         chosen = "fixed (--kc)"
     else:
         tuned = s.autotune(1, steps=4)
         chosen = "fuss_autotune"
-    irs_rows = s.tiles(1)[2]
+    _, _, stage_rows, irs_rows = s.tiles(1)
     print(f"[bench] tiles ({chosen}): column height {tuned[0]}, {'row march' if tuned[1] else 'LDS ring'}, "
-          f"IRS columns of {irs_rows} rows, {tuned[2]:.3f} ms/step", file=sys.stderr, flush=True)
+          f"stage columns of {stage_rows} rows, IRS columns of {irs_rows} rows, {tuned[2]:.3f} ms/step",
+          file=sys.stderr, flush=True)
     # warm-up: module load and the capture of fuss_run's step graph
     s.run_level(1, max(args.warmup, 1), max(args.warmup, 1), -1.0, 1e300, hist=False)
     device_sync()
@@ -401,7 +404,7 @@ def main():
     # on a rectilinear level it reads no per-cell geometry either
     stage_bpc = perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False, qvol=False, rect=rect[0])
     stage_b = stage_bpc * ncell
-    skern = stage_kernel(rect[0], tuned[0])
+    skern = stage_kernel(rect[0], tuned[0], stage_rows)
     ldsring = args.n == 256 and not tuned[1]
     stage_ms = ms[2] / cnt[2]
     achieved = stage_b / (stage_ms * 1e-3) / 1e9
@@ -423,7 +426,7 @@ def main():
                                "finest-level Explicit_Step",
                    "cells": ncell, "parallelism": "single-gpu",
                    "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring",
-                             "irs_rows": irs_rows, "chosen_by": chosen}},
+                             "stage_rows": stage_rows, "irs_rows": irs_rows, "chosen_by": chosen}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": STAGE_TRAFFIC_MEASURED.get(skern) if ldsring else None,

@@ -933,10 +933,13 @@ __device__ __forceinline__ void fetch_plane(PlaneRegs<R>& r, const R* __restrict
                                                                                 : (R)0;
   }
 }
-template <typename R>
+// TY: rows of the tile (planes of PX x (TY+2) values; 64*TY threads, two
+// values each)
+template <int TY = TILE_Y, typename R>
 __device__ __forceinline__ void store_plane(R (*dst)[PX], const PlaneRegs<R>& r) {
+  constexpr int TT = TILE_X * TY;
   (&dst[0][0])[threadIdx.x] = r.v0;
-  if (threadIdx.x + TILE_T < PX * PY) (&dst[0][0])[threadIdx.x + TILE_T] = r.v1;
+  if (threadIdx.x + TT < PX * (TY + 2)) (&dst[0][0])[threadIdx.x + TT] = r.v1;
 }
 
 // kappa row of the left cell of a face: blocks of one material skip the
@@ -1149,9 +1152,11 @@ struct StageStream {
 // fetch_plane with every load issued: halo cells beyond the ghost layer read
 // the tile's corner cell instead (a ghost or interior cell of the block); no
 // flux of an in-block face uses them
-template <typename R>
+template <int TY, typename R>
 __device__ __forceinline__ void fetch_plane_u(PlaneRegs<R>& r, const R* __restrict__ T, const BlockInfo& B, int i0,
                                               int j0, int kk) {
+  constexpr int TT = TILE_X * TY;
+  static_assert(PX * (TY + 2) <= 2 * TT, "two values per thread");
   const unsigned corner = (unsigned)(B.base + (i0 - 2) + (long)(j0 - 2) * B.sy + (long)(kk - 1) * B.sz);
   {
     const int q = threadIdx.x, ii = q % PX, jj = q / PX;
@@ -1159,14 +1164,18 @@ __device__ __forceinline__ void fetch_plane_u(PlaneRegs<R>& r, const R* __restri
     r.v0 = ldu(T, in ? corner + ii + jj * (unsigned)B.sy : corner);
   }
   {
-    const int q = threadIdx.x + TILE_T, ii = q % PX, jj = q / PX;
-    const bool in = q < PX * PY && i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1;
+    const int q = threadIdx.x + TT, ii = q % PX, jj = q / PX;
+    const bool in = q < PX * (TY + 2) && i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1;
     r.v1 = ldu(T, in ? corner + ii + jj * (unsigned)B.sy : corner);
   }
 }
 
-template <bool UM, bool PRIM, bool IRSMODE, bool RECT, bool DTC, typename R, int KCT>
-__global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
+// TY: rows of the column (4 or 8, one wave per row; the 8-row column
+// evaluates 9 instead of 10 y faces and loads 10 instead of 12 T rows per 8
+// rows of cells)
+template <bool UM, bool PRIM, bool IRSMODE, bool RECT, bool DTC, typename R, int KCT, int TY>
+__global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
+  constexpr int TT = TILE_X * TY;
   if (a.halt && *a.halt) return;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const Tile t = a.ctiles[wg];
@@ -1178,12 +1187,12 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   const int x = tx + 1, y = ty + 1;
   const bool yhalo = ty == 0 && t.j0 > 1 && i <= B.ni;
   const int hr = tx;
-  const bool xhalo = ty == 1 && tx < TILE_Y && t.i0 > 1 && t.j0 + hr <= B.nj;
+  const bool xhalo = ty == 1 && tx < TY && t.i0 > 1 && t.j0 + hr <= B.nj;
   const bool hx = i < B.ni, hy = j < B.nj;  // the cell's +x / +y face is interior
   const int hq = xhalo ? hr + 1 : 1;        // LDS row of the x-halo face (in range for every lane)
-  __shared__ R ring[3][PY][PX];
-  __shared__ R fx[TILE_Y][TILE_X + 1];
-  __shared__ R fy[TILE_Y + 1][TILE_X];
+  __shared__ R ring[3][TY + 2][PX];
+  __shared__ R fx[TY][TILE_X + 1];
+  __shared__ R fy[TY + 1][TILE_X];
   using V2 = typename Vec2<R>::T;
   using V4 = typename Vec4<R>::T;
   const int Ti1 = a.tab.Ti1, Ti2 = a.tab.Ti2, nT = a.tab.nT;
@@ -1219,11 +1228,11 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   }
   int sm = 0, sc = 1, sp = 2;
   PlaneRegs<R> pre;
-  fetch_plane_u(pre, a.T, B, t.i0, t.j0, k0 - 1);
-  store_plane(ring[sm], pre);
-  fetch_plane_u(pre, a.T, B, t.i0, t.j0, k0);
-  store_plane(ring[sc], pre);
-  fetch_plane_u(pre, a.T, B, t.i0, t.j0, k0 + 1);
+  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0 - 1);
+  store_plane<TY>(ring[sm], pre);
+  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0);
+  store_plane<TY>(ring[sc], pre);
+  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0 + 1);
   const unsigned sz = (unsigned)B.sz;
   // addresses of plane k0: the cell (cell (1,1) for out-of-block lanes) and the
   // halo face's left cell (the cell itself for other lanes)
@@ -1289,7 +1298,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
   double dtmin = 1e300;
   // one plane: cur holds its streams, nxt receives those of the next plane
   auto plane = [&](StageStream<R>& cur, StageStream<R>& nxt, int k) {
-    store_plane(ring[sp], pre);
+    store_plane<TY>(ring[sp], pre);
     __syncthreads();
     // phase A: table rows of the plane's faces and of the cell (loads always
     // issued, at a clamped row; errors counted for existing faces only)
@@ -1340,7 +1349,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
     // phase B: the next plane's loads (the last plane re-reads its own cells)
     {
       const unsigned step = k < k1 ? sz : 0u;
-      fetch_plane_u(pre, a.T, B, t.i0, t.j0, k < k1 ? k + 2 : k + 1);
+      fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k < k1 ? k + 2 : k + 1);
       stream(nxt, p + step, ph + step);
     }
     if (spend) {  // the previous plane's result
@@ -1448,8 +1457,8 @@ __global__ __launch_bounds__(TILE_T) void k_stage3p(StageArgs<R> a) {
     stn(out, ppend, vpend);
     if (DTC) stn(a.dtw, ppend, dtpend);
   }
-  if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
-  if (DTC) block_min(dtmin, a.pmin, wg);
+  if (!IRSMODE && a.last) block_sum<TT>(rsq, a.partials, wg);
+  if (DTC) block_min<TT>(dtmin, a.pmin, wg);
 }
 
 // ------------------------------------------------------------------------
@@ -2533,6 +2542,7 @@ struct Level {
   std::vector<Tile> hc8tiles;       // 64x8 columns of kc planes (fused x-y-z IRS pass)
   int kc = KC_DEFAULT;              // planes per column tile (fuss_autotune)
   int xyz_ty = 4;                   // rows of the fused IRS pass's columns, 4 or 8 (fuss_autotune)
+  int stage_ty = 4;                 // rows of the LDS-ring stage kernel's columns, 4 or 8 (fuss_autotune)
   int dt_ty = 4;                    // ... of the pass that wrote the dt minima (dt_next)
   int row_march = -1;               // orthogonal 3-D stage kernel: -1 context default, 0 LDS ring, 1 row march
   std::vector<Tile> hxytiles;       // 64x16 plane tiles of the x-y IRS pass
@@ -3575,18 +3585,33 @@ void launch_stage_n(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang
   }
 }
 
-template <bool UM, bool RECT, int KCT>
+template <bool UM, bool RECT, int KCT, int TY>
 void launch_stage3p(fuss_ctx* c, unsigned g, const StageArgs<double>& a, bool prim, bool irs) {
+  constexpr int T = TILE_X * TY;
   if (RECT && prim && a.dtw) {  // first stage of a steady step: dt here (rectilinear, primitive)
-    irs ? k_stage3p<UM, true, true, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a)
-        : k_stage3p<UM, true, false, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a);
+    irs ? k_stage3p<UM, true, true, true, true, double, KCT, TY><<<g, T, 0, c->stream>>>(a)
+        : k_stage3p<UM, true, false, true, true, double, KCT, TY><<<g, T, 0, c->stream>>>(a);
   } else if (prim) {
-    irs ? k_stage3p<UM, true, true, RECT, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a)
-        : k_stage3p<UM, true, false, RECT, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a);
+    irs ? k_stage3p<UM, true, true, RECT, false, double, KCT, TY><<<g, T, 0, c->stream>>>(a)
+        : k_stage3p<UM, true, false, RECT, false, double, KCT, TY><<<g, T, 0, c->stream>>>(a);
   } else {
-    irs ? k_stage3p<UM, false, true, RECT, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a)
-        : k_stage3p<UM, false, false, RECT, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a);
+    irs ? k_stage3p<UM, false, true, RECT, false, double, KCT, TY><<<g, T, 0, c->stream>>>(a)
+        : k_stage3p<UM, false, false, RECT, false, double, KCT, TY><<<g, T, 0, c->stream>>>(a);
   }
+}
+
+// the LDS-ring kernel on the level's 4-row (ctiles) or 8-row (c8tiles) columns
+template <int KCT, int TY>
+void launch_stage3p_t(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool prim, bool irs) {
+  StageArgs<double> at = a;
+  at.ctiles = TY == 8 ? L.c8tiles.p : L.ctiles.p;
+  const unsigned g = (unsigned)(TY == 8 ? L.hc8tiles.size() : L.hctiles.size());
+  if (L.all_umat)
+    a.rect ? launch_stage3p<true, true, KCT, TY>(c, g, at, prim, irs)
+           : launch_stage3p<true, false, KCT, TY>(c, g, at, prim, irs);
+  else
+    a.rect ? launch_stage3p<false, true, KCT, TY>(c, g, at, prim, irs)
+           : launch_stage3p<false, false, KCT, TY>(c, g, at, prim, irs);
 }
 
 template <int KCT>
@@ -3600,11 +3625,7 @@ void launch_stage3d_k(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool ta
     if (prim) { irs ? k_stage3d<false, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
     else { irs ? k_stage3d<false, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
   } else if (!row) {
-    if (L.all_umat) {
-      a.rect ? launch_stage3p<true, true, KCT>(c, g, a, prim, irs) : launch_stage3p<true, false, KCT>(c, g, a, prim, irs);
-    } else {
-      a.rect ? launch_stage3p<false, true, KCT>(c, g, a, prim, irs) : launch_stage3p<false, false, KCT>(c, g, a, prim, irs);
-    }
+    L.stage_ty == 8 ? launch_stage3p_t<KCT, 8>(c, L, a, prim, irs) : launch_stage3p_t<KCT, 4>(c, L, a, prim, irs);
   } else {
     if (prim) { irs ? k_stage3w<true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
     else { irs ? k_stage3w<false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
@@ -3832,6 +3853,9 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
   // stage's input is T^n, and the cell's rho/cp row is loaded there anyway.
   const bool xyz = irs && !tri && L.no_ycon && L.no_zcon && c->ndir == 3 && L.kp == GC;
   const bool ring = L.kp == GC && !L.tang && L.dims2 && !(L.row_march >= 0 ? L.row_march == 1 : !c->stage_lds);
+  // the stage kernel's columns (partials and dt minima per column)
+  const unsigned nstcol = (unsigned)(ring && L.stage_ty == 8 ? L.hc8tiles.size() : L.hctiles.size());
+  const int stlist = ring && L.stage_ty == 8 ? 3 : 1;
   const bool sdt = !hk && ring && L.rect_w && !c->rect_off && prim && !p->time_accurate && p->rampa_iter <= 1 &&
                    !c->adj_store_on;
   const bool fuse_dt = !hk && !sdt && xyz && p->rampa_iter <= 1 && !c->adj_store_on;
@@ -3839,7 +3863,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
   unsigned nmin = ntile;
   int pid = -1;
   if (sdt) {
-    nmin = (unsigned)L.hctiles.size();
+    nmin = nstcol;
   } else if (have_dt) {
     // minima of the fused IRS pass's tiles (its list when dt was written)
     nmin = (unsigned)(L.dt_ty == 8 ? L.hc8tiles.size() : L.hctiles.size());
@@ -3936,7 +3960,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
       if (!irs) { npart = (int)gt; L.plist = 0; }
     } else if (L.kp == GC) {
       launch_stage3d(c, L, sa, L.tang != 0, prim, irs);
-      if (!irs) { npart = (int)L.hctiles.size(); L.plist = 1; }
+      if (!irs) { npart = (int)nstcol; L.plist = stlist; }
     } else {
       launch_stage_n<2>(c, L, sa, L.tang != 0, prim, irs);
     }
@@ -4722,11 +4746,12 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
   q.compute_residual = 1;
   const int kcs[5] = {4, 8, 16, 32, 64};
   double best = 1e300;
-  int bkc = L.kc, brow = 0, bty = 4, err = 0;
-  auto trial = [&](int row, int kc, int ty) -> int {
+  int bkc = L.kc, brow = 0, bst = 4, bty = 4, err = 0;
+  auto trial = [&](int row, int kc, int st, int ty) -> int {
     int e = set_kc(c, L, kc);
     if (e) return e;
     L.row_march = row;
+    L.stage_ty = st;
     L.xyz_ty = ty;
     drop_dt_next(c);  // the state was restored
     if ((e = enqueue_step(c, level, &q))) return e;  // warm-up
@@ -4737,19 +4762,22 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     const double per = ms / std::max(steps, 1);
-    if (!e && per < best) { best = per; bkc = kc; brow = row; bty = ty; }
+    if (!e && per < best) { best = per; bkc = kc; brow = row; bst = st; bty = ty; }
     HIPCHK(hipMemcpy(L.Tb[L.cur].p, save.p, L.npad * sizeof(double), hipMemcpyDeviceToDevice));
     set_clock(c, L, t0, it0);
     return e;
   };
-  // stage kernel and column height with 4-row IRS columns, then the column
-  // heights with 8-row IRS columns at the chosen stage kernel (the fused
-  // x-y-z IRS pass is the only user of the 8-row columns)
+  // stage kernel and column height on 4-row columns, then the column heights
+  // with 8-row columns for the LDS-ring stage kernel and / or the fused x-y-z
+  // IRS pass (their only users) at the chosen stage kernel
   for (int row = 0; row <= (L.tang ? 0 : 1) && !err; row++)
-    for (int ki = 0; ki < 5 && !err; ki++) err = trial(row, kcs[ki], 4);
+    for (int ki = 0; ki < 5 && !err; ki++) err = trial(row, kcs[ki], 4, 4);
   const int row4 = brow;
   const bool xyz = p->irs_beta > 0.0 && p->irs_mode == FUSS_IRS_JACOBI2 && L.no_ycon && L.no_zcon && c->ndir == 3;
-  for (int ki = 0; ki < 5 && !err && xyz; ki++) err = trial(row4, kcs[ki], 8);
+  const bool ring = !L.tang && L.dims2 && row4 == 0;
+  for (int st = 4; st <= (ring ? 8 : 4); st += 4)
+    for (int ty = 4; ty <= (xyz ? 8 : 4); ty += 4)
+      for (int ki = 0; ki < 5 && !err && (st == 8 || ty == 8); ki++) err = trial(row4, kcs[ki], st, ty);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   save.free();
@@ -4759,6 +4787,7 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
   if (st) return st;
   if ((err = set_kc(c, L, bkc))) return err;
   L.row_march = brow;
+  L.stage_ty = bst;
   L.xyz_ty = bty;
   if (kc_out) *kc_out = bkc;
   if (row_out) *row_out = brow;
@@ -4777,27 +4806,29 @@ int fuss_autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, 
   return autotune(c, level, p, steps, kc, row_march, ms_per_step);
 }
 
-int fuss_set_tiles(fuss_ctx* c, int level, int kc, int row_march, int irs_rows) {
+int fuss_set_tiles(fuss_ctx* c, int level, int kc, int row_march, int stage_rows, int irs_rows) {
   if (c) drop_dt_next(c);
   if (!c || !c->setup_done || level < 1 || level > c->nlev) return FUSS_EARG;
   if ((kc != 4 && kc != 8 && kc != 16 && kc != 32 && kc != 64) || row_march < -1 || row_march > 1 ||
-      (irs_rows != 4 && irs_rows != 8))
+      (stage_rows != 4 && stage_rows != 8) || (irs_rows != 4 && irs_rows != 8))
     return FUSS_EARG;
   Level& L = c->lev[level - 1];
   if (L.kp != GC) return 0;  // 2-D levels have no column tiles
   const int e = set_kc(c, L, kc);
   if (e) return e;
   L.row_march = row_march;
+  L.stage_ty = stage_rows;
   L.xyz_ty = irs_rows;
   return 0;
 }
 
-int fuss_tiles(fuss_ctx* c, int level, int* kc, int* row_march, int* irs_rows) {
+int fuss_tiles(fuss_ctx* c, int level, int* kc, int* row_march, int* stage_rows, int* irs_rows) {
   if (!c || !c->setup_done || level < 1 || level > c->nlev) return FUSS_EARG;
   const Level& L = c->lev[level - 1];
   const bool col = L.kp == GC;
   if (kc) *kc = col ? L.kc : 0;
   if (row_march) *row_march = col ? L.row_march : 0;
+  if (stage_rows) *stage_rows = col ? L.stage_ty : 0;
   if (irs_rows) *irs_rows = col ? L.xyz_ty : 0;
   return 0;
 }

@@ -108,8 +108,9 @@ def load():
     L.fuss_graphs.argtypes = [P, C.c_int]
     L.fuss_autotune.argtypes = [P, C.c_int, C.POINTER(StepParams), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                 C.POINTER(C.c_double)]
-    L.fuss_set_tiles.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
-    L.fuss_tiles.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.fuss_set_tiles.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.fuss_tiles.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                             C.POINTER(C.c_int)]
     L.fuss_refine_setup.argtypes = [P, C.POINTER(RefineParams)]
     L.fuss_refine.argtypes = [P, C.POINTER(StepParams), C.POINTER(RefineResult), C.c_void_p]
     L.fuss_refine_blocks.argtypes = [P, C.c_void_p, C.c_void_p]
@@ -184,15 +185,16 @@ class Solver:
              "fuss_autotune")
         return kc.value, row.value, ms.value
 
-    def set_tiles(self, level=1, kc=8, row_march=-1, irs_rows=4):
+    def set_tiles(self, level=1, kc=8, row_march=-1, stage_rows=4, irs_rows=4):
         """Column tiles of a 3-D level without timing (fuss_set_tiles)."""
-        _chk(self.L.fuss_set_tiles(self.h, level, int(kc), int(row_march), int(irs_rows)), "fuss_set_tiles")
+        _chk(self.L.fuss_set_tiles(self.h, level, int(kc), int(row_march), int(stage_rows), int(irs_rows)),
+             "fuss_set_tiles")
 
     def tiles(self, level=1):
-        """(kc, row_march, irs_rows) of a level (fuss_tiles; 0s on 2-D levels)."""
-        kc, row, ty = C.c_int(0), C.c_int(0), C.c_int(0)
-        _chk(self.L.fuss_tiles(self.h, level, C.byref(kc), C.byref(row), C.byref(ty)), "fuss_tiles")
-        return kc.value, row.value, ty.value
+        """(kc, row_march, stage_rows, irs_rows) of a level (fuss_tiles; 0s on 2-D levels)."""
+        kc, row, st, ty = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+        _chk(self.L.fuss_tiles(self.h, level, C.byref(kc), C.byref(row), C.byref(st), C.byref(ty)), "fuss_tiles")
+        return kc.value, row.value, st.value, ty.value
 
     def graphs(self, enable):
         """fuss_run on captured HIP graphs (default) or step by step."""

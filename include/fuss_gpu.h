@@ -156,8 +156,9 @@ int fuss_rect_info(fuss_ctx *ctx, int level, int *weights, int *dt, double *max_
 
 /* Tile autotuner: times `steps` Explicit_Steps with each candidate of the
  * 3-D stage pass -- column height 4, 8, 16, 32 or 64 planes x LDS-ring or row-march
- * kernel (row march: orthogonal levels only), then, where the fused x-y-z IRS
- * pass runs, the column heights with its 8-row columns -- on the level's
+ * kernel (row march: orthogonal levels only), then the column heights with
+ * 8-row columns for the LDS-ring stage kernel and the fused x-y-z IRS pass
+ * (where they run) -- on the level's
  * current state, restores the state and clock, and keeps the fastest for this level.  The
  * per-cell arithmetic is the same for every candidate; only the grouping of
  * the residual's partial sums changes.  2-D levels: nothing to tune (kc = 0). */
@@ -165,13 +166,14 @@ int fuss_autotune(fuss_ctx *ctx, int level, const fuss_step_params *p, int steps
                   int *row_march, double *ms_per_step);
 /* Column tiles of a 3-D level without timing (no reference counterpart: the
  * reference has no tiles): kc = 4, 8, 16, 32 or 64 planes; row_march = -1
- * (context default), 0 (LDS ring) or 1 (row march); irs_rows = 4 or 8 rows
- * per column of the fused x-y-z IRS pass (levels without j/k connections,
- * Jacobi IRS).  fuss_autotune also chooses irs_rows; fuss_tiles reports the
- * level's current choice (0s on 2-D levels).  Results do not depend on the
- * tiles except for the grouping of the residual's partial sums. */
-int fuss_set_tiles(fuss_ctx *ctx, int level, int kc, int row_march, int irs_rows);
-int fuss_tiles(fuss_ctx *ctx, int level, int *kc, int *row_march, int *irs_rows);
+ * (context default), 0 (LDS ring) or 1 (row march); stage_rows = 4 or 8 rows
+ * per column of the LDS-ring stage kernel (orthogonal blocks); irs_rows = 4
+ * or 8 rows per column of the fused x-y-z IRS pass (levels without j/k
+ * connections, Jacobi IRS).  fuss_autotune also chooses the rows; fuss_tiles
+ * reports the level's current choice (0s on 2-D levels).  Results do not
+ * depend on the tiles except for the grouping of the residual's partial sums. */
+int fuss_set_tiles(fuss_ctx *ctx, int level, int kc, int row_march, int stage_rows, int irs_rows);
+int fuss_tiles(fuss_ctx *ctx, int level, int *kc, int *row_march, int *stage_rows, int *irs_rows);
 
 /* Restriction (Mod_Multigrid.f90:59-77): coarse T, qvol, matID from level
  * fine_level to fine_level+1. */

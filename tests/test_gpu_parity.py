@@ -406,7 +406,8 @@ def test_autotune_keeps_results(gpu, name):
         if tune:
             kc, row, ms = s.autotune(1, steps=2)
             assert kc in (0, 4, 8, 16, 32, 64) and ms >= 0.0
-            assert s.tiles(1)[0] == kc and s.tiles(1)[2] in ((0,) if kc == 0 else (4, 8))
+            tl = s.tiles(1)
+            assert tl[0] == kc and all(r in ((0,) if kc == 0 else (4, 8)) for r in tl[2:])
         r = [s.explicit_step(1).res_sq for _ in range(6)]
         out.append((np.concatenate([T.ravel() for T in s.T()]), np.array(r)))
         s.close()
@@ -415,20 +416,22 @@ def test_autotune_keeps_results(gpu, name):
 
 
 @pytest.mark.parametrize("rect", [False, True])
-@pytest.mark.parametrize("name", ["cube3d_rk4_irs", "sweep3_rk4_irs", "cube3d_mg3_irs"])
-def test_irs_rows_keep_results(gpu, name, rect):
-    """The fused x-y-z IRS pass on 8-row columns (fuss_set_tiles) gives the
-    4-row columns' state bit for bit at each column height (ragged blocks:
-    rows past nj masked), dtglobal bit for bit (with the stored metrics the
-    pass leaves the next step's dt and one minimum per 8-row column), and
-    the residual to rounding (its partial sums group by column)."""
-    c = SYNTH[name]()
+@pytest.mark.parametrize("name", ["cube3d_rk4_irs", "sweep3_rk4_irs", "cube3d_mg3_irs", "cube3d_cons"])
+def test_column_rows_keep_results(gpu, name, rect):
+    """The LDS-ring stage kernel and the fused x-y-z IRS pass on 8-row
+    columns (fuss_set_tiles) give the 4-row columns' state bit for bit at
+    each column height (ragged blocks: rows past nj masked), dtglobal bit for
+    bit (the first stage's dt or the IRS pass's next-step dt, one minimum per
+    8-row column), and the residual to rounding (its partial sums group by
+    column).  Without IRS (conservative, 20^3: ragged columns) the stage
+    kernel applies the update and writes the residual partials."""
+    c = cases.cube3d(20, mgl=1, irs_beta=0.0, prim=False, vnn=0.8) if name == "cube3d_cons" else SYNTH[name]()
     out = []
-    for kc, rows in ((8, 4), (4, 8), (16, 8), (64, 8)):
+    for kc, st, rows in ((8, 4, 4), (4, 8, 8), (16, 8, 4), (32, 4, 8), (64, 8, 8)):
         s = gpu.Solver(c)
         s.rect(rect)
-        s.set_tiles(1, kc=kc, irs_rows=rows)
-        assert s.tiles(1) == (kc, -1, rows)
+        s.set_tiles(1, kc=kc, row_march=0, stage_rows=st, irs_rows=rows)
+        assert s.tiles(1) == (kc, 0, st, rows)
         rs = [s.explicit_step(1) for _ in range(5)]
         out.append((np.concatenate([T.ravel() for T in s.T()]), np.array([r.res_sq for r in rs]),
                     np.array([r.dtglobal for r in rs])))
@@ -437,6 +440,8 @@ def test_irs_rows_keep_results(gpu, name, rect):
                 s.set_tiles(1, kc=12)
             with pytest.raises(gpu.FussError):
                 s.set_tiles(1, kc=8, irs_rows=6)
+            with pytest.raises(gpu.FussError):
+                s.set_tiles(1, kc=8, stage_rows=2)
         s.close()
     for T, res, dt in out[1:]:
         assert np.array_equal(T, out[0][0])
