@@ -44,17 +44,17 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the stage kernel measured with rocprofv3 PMC
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB device copy);
 # source: profiles/explicit_step_cube256_fp64.md (256^3 cube, FP64).
-# The LDS-ring stage kernel (stages 2-4 of a steady step: no dt) with kc-plane
-# columns, one material, primitive variables, IRS on; rect = per-axis metric
-# factors (rectilinear level).
+# The LDS-ring stage kernel (stages 2-4 of a steady step: no dt) on 64x4
+# columns of kc planes, one material, primitive variables, IRS on; rect =
+# per-axis metric factors (rectilinear level).
 
 
 def stage_kernel(rect, kc, rows):
     return f"k_stage3p<true, true, true, {'true' if rect else 'false'}, false, double, {kc}, {rows}>"
 
 
-STAGE_TRAFFIC_MEASURED = {stage_kernel(True, 16, 4): 149388 * 1024 * 2.0 + 131072 * 1024 * 1.0,
-                          stage_kernel(True, 64, 4): 140750 * 1024 * 2.0 + 131072 * 1024 * 1.0}
+STAGE_TRAFFIC_MEASURED = {stage_kernel(True, kc, 4): fetch * 1024 * 2.0 + 131072 * 1024 * 1.0
+                          for kc, fetch in ((4, 153140), (8, 154981), (16, 149399), (32, 143788), (64, 140748))}
 # Plain FP64 path (four-stage RK + LTS + IRS 0.5, 3-level sequencing) run to
 # residual 1e-10 on the GPU: iterations per cube size (profiles/
 # steady_convergence_cube.md).  Fit: iters = 230218 (n/64)^PLAIN_EXP.

@@ -246,16 +246,17 @@ def flux_plate(n=512, nseg=64, L=1.0, seed=11, nprobe=32, probe_rows=8, n_rk=3, 
 
 def lattice(nl=4, lo=16, hi=128, seed=7, h=1e-3, amp=0.05, nmat=3, T0=500.0, n_rk=3, vnn=0.8,
             prim=True, irs_beta=0.0, time_accurate=True, mgl=1, walls=None, res_threshold=1e-10):
-    """nl^3 blocks in a lattice with per-slab dimensions drawn from U{lo..hi}
-    (so neighbouring blocks share face dimensions), joined by 1-to-1 101
-    connections.  Nodes are a smooth global curvilinear map: base spacing h,
+    """nl^3 blocks (nl an int) or nx*ny*nz blocks (nl a 3-tuple) in a lattice
+    with per-slab dimensions drawn from U{lo..hi} (so neighbouring blocks
+    share face dimensions), joined by 1-to-1 101 connections.  Nodes are a smooth global curvilinear map: base spacing h,
     each coordinate perturbed by amp*h*sin(2 pi (k . X/Lg) + phi) (seeded), so
     the faces of neighbouring blocks coincide.  Exterior faces by global side:
     x-min 302, x-max 304, y-min 301, y-max 304, z-min 302, z-max 301 unless
     `walls` overrides them (6 (type, rrec) pairs)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     step = 2 ** (mgl - 1)
-    dims = [rng.integers(lo // step, hi // step + 1, size=nl) * step for _ in range(3)]
+    nls = (int(nl),) * 3 if np.isscalar(nl) else tuple(int(x) for x in nl)
+    dims = [rng.integers(lo // step, hi // step + 1, size=nls[d]) * step for d in range(3)]
     off = [np.concatenate([[0], np.cumsum(d)]) for d in dims]
     Lg = [off[d][-1] * h for d in range(3)]
     kv = rng.integers(1, 3, size=(3, 3))
@@ -272,10 +273,10 @@ def lattice(nl=4, lo=16, hi=128, seed=7, h=1e-3, amp=0.05, nmat=3, T0=500.0, n_r
     walls = walls or [(302, R(Tw=450.0)), (304, R(eps=0.8, Tref=300.0)), (301, R(qw=-5e3)),
                       (304, R(eps=0.8, Tref=300.0)), (302, R(Tw=550.0)), (301, R(qw=0.0))]
     blocks, bid = [], {}
-    mats = rng.integers(1, nmat + 1, size=nl ** 3)
-    for c in range(nl):
-        for b in range(nl):
-            for a in range(nl):
+    mats = rng.integers(1, nmat + 1, size=nls[0] * nls[1] * nls[2])
+    for c in range(nls[2]):
+        for b in range(nls[1]):
+            for a in range(nls[0]):
                 n = len(blocks)
                 bid[(a, b, c)] = n + 1
                 ni, nj, nk = int(dims[0][a]), int(dims[1][b]), int(dims[2][c])
@@ -289,16 +290,16 @@ def lattice(nl=4, lo=16, hi=128, seed=7, h=1e-3, amp=0.05, nmat=3, T0=500.0, n_r
 
     def lattice_bc(level):
         ib, rb = [], []
-        for c in range(nl):
-            for b in range(nl):
-                for a in range(nl):
+        for c in range(nls[2]):
+            for b in range(nls[1]):
+                for a in range(nls[0]):
                     me = bid[(a, b, c)]
                     d0 = cs.coarse_dims(blocks[me - 1].dims, level + 1)[level]
                     for f in range(1, 7):
                         ax, hi_side = (f - 1) // 2, (f - 1) % 2
                         pos = [a, b, c]
                         pos[ax] += 1 if hi_side else -1
-                        if 0 <= pos[ax] < nl:
+                        if 0 <= pos[ax] < nls[ax]:
                             nb_ = bid[tuple(pos)]
                             dn = cs.coarse_dims(blocks[nb_ - 1].dims, level + 1)[level]
                             fs = f + 1 if f % 2 == 1 else f - 1
@@ -322,7 +323,7 @@ def lattice(nl=4, lo=16, hi=128, seed=7, h=1e-3, amp=0.05, nmat=3, T0=500.0, n_r
     p = cs.Params(n_rk=n_rk, vnn=vnn, prim=prim, irs_beta=irs_beta, time_accurate=time_accurate, mgl=mgl,
                   level_iter=[10**9] * mgl, res_threshold=res_threshold)
     return cs.Case(blocks, bcs, material_tables(nmat, seed + 1), p, [], [np.full(x.n, -1, np.int32) for x in bcs],
-                   name=f"lattice{nl}")
+                   name=f"lattice{nl}" if np.isscalar(nl) else f"lattice{'x'.join(map(str, nls))}")
 
 
 def write_case_dir(case, path, ini_extra=""):

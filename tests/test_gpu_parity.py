@@ -118,6 +118,9 @@ SYNTH = {
     # (per-axis metric factors per block), four-stage RK with IRS
     "sweep3_rk4_irs": lambda: cases.sweep(ncase=3, lo=12, hi=28, n_rk=4, mgl=1, seed=7),
     "lattice_rk4_cons": lambda: cases.lattice(nl=2, lo=4, hi=8, n_rk=4, prim=False),
+    # a row of three curvilinear blocks joined on i faces only: the fused
+    # x-y-z IRS pass with donor x ghosts (steady, local dt)
+    "xrow_irs": lambda: cases.lattice(nl=(3, 1, 1), lo=6, hi=20, irs_beta=0.5, time_accurate=False),
     # line-tridiagonal IRS extension (FUSS_IRS_TRIDIAG): single block, the
     # connected lattice (donor ghosts between directions, 2 levels), uneven
     # line lengths (not multiples of the 16-cell chunks), and 2-D
@@ -416,7 +419,7 @@ def test_autotune_keeps_results(gpu, name):
 
 
 @pytest.mark.parametrize("rect", [False, True])
-@pytest.mark.parametrize("name", ["cube3d_rk4_irs", "sweep3_rk4_irs", "cube3d_mg3_irs", "cube3d_cons"])
+@pytest.mark.parametrize("name", ["cube3d_rk4_irs", "sweep3_rk4_irs", "cube3d_mg3_irs", "cube3d_cons", "xrow_irs"])
 def test_column_rows_keep_results(gpu, name, rect):
     """The LDS-ring stage kernel and the fused x-y-z IRS pass on 8-row
     columns (fuss_set_tiles) give the 4-row columns' state bit for bit at
@@ -447,6 +450,23 @@ def test_column_rows_keep_results(gpu, name, rect):
         assert np.array_equal(T, out[0][0])
         np.testing.assert_allclose(res, out[0][1], rtol=1e-13)
         assert np.array_equal(dt, out[0][2])
+
+
+def test_xrow_takes_fused_irs(gpu):
+    """Blocks joined on i faces only run the fused x-y-z IRS pass (one
+    IRS launch per stage, the update in it), so xrow_irs exercises its
+    donor x ghosts."""
+    import ctypes
+
+    c = SYNTH["xrow_irs"]()
+    s = gpu.Solver(c)
+    s.L.fuss_profile(s.h, 1)
+    s.explicit_step(1)
+    ms, cnt = (ctypes.c_double * 8)(), (ctypes.c_int * 8)()
+    s.L.fuss_profile_read(s.h, 8, ms, cnt)
+    s.L.fuss_profile(s.h, 0)
+    assert cnt[3] == 0 and cnt[4] == c.params.n_rk  # FUSS_K_IRS, FUSS_K_IRS_UPDATE
+    s.close()
 
 
 @pytest.mark.parametrize("name", ["cube3d_mg3_irs", "lattice_transient", "plate2d_64", "lattice_rk4_cons"])
