@@ -2517,7 +2517,11 @@ struct DBuf {
     n = count;
     if (!count) return 0;
     if (hipMalloc(&p, sizeof(T) * count) != hipSuccess) { p = nullptr; return FUSS_EHIP; }
+    // the zero fill runs on the null stream, which the library's non-blocking
+    // stream does not wait for: finish it before any kernel on that stream
+    // writes the buffer (else the fill can land after, over the results)
     if (hipMemset(p, 0, sizeof(T) * count) != hipSuccess) return FUSS_EHIP;
+    if (hipStreamSynchronize(nullptr) != hipSuccess) return FUSS_EHIP;
     return 0;
   }
   void free() {
@@ -4738,7 +4742,7 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
   const int it0 = L.iter;
   DBuf<double> save;
   if (save.alloc(L.npad)) return FUSS_EHIP;
-  HIPCHK(hipMemcpy(save.p, L.Tb[L.cur].p, L.npad * sizeof(double), hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpyAsync(save.p, L.Tb[L.cur].p, L.npad * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -4763,7 +4767,8 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     const double per = ms / std::max(steps, 1);
     if (!e && per < best) { best = per; bkc = kc; brow = row; bst = st; bty = ty; }
-    HIPCHK(hipMemcpy(L.Tb[L.cur].p, save.p, L.npad * sizeof(double), hipMemcpyDeviceToDevice));
+    // on the library's stream: the next candidate's steps follow the restore
+    HIPCHK(hipMemcpyAsync(L.Tb[L.cur].p, save.p, L.npad * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     set_clock(c, L, t0, it0);
     return e;
   };
@@ -4780,6 +4785,7 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
       for (int ki = 0; ki < 5 && !err && (st == 8 || ty == 8); ki++) err = trial(row4, kcs[ki], st, ty);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  HIPCHK(hipStreamSynchronize(c->stream));  // the last restore has read `save`
   save.free();
   drop_dt_next(c);  // the state was restored after the last candidate
   if (err) return err;
