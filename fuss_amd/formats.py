@@ -222,10 +222,25 @@ class BCList:
     ib: np.ndarray          # (n, 15) int32
     rb: np.ndarray          # (n, 6) float64
     series_file: list       # per entry: None or the time-series file name
+    # chimera (102) entries: BC index -> (n1, donors (n1+n2, 4) int32 as
+    # (b, i, j, k), volume fractions (n1+n2,)); rows 1..n1 make the first
+    # ghost layer, the rest the second (IO_BC.f90:218-229)
+    chim: dict = field(default_factory=dict)
 
     @property
     def n(self):
         return self.ib.shape[0]
+
+    def chimera_arrays(self):
+        """Flattened chimera donor lists in BC order: (entry, n1, n2, donors
+        (m, 4), vf (m,)) -- the layout of fuss_set_chimera."""
+        ent = sorted(self.chim)
+        n1 = np.array([self.chim[e][0] for e in ent], np.int32)
+        n2 = np.array([len(self.chim[e][2]) - self.chim[e][0] for e in ent], np.int32)
+        don = (np.concatenate([self.chim[e][1] for e in ent]).astype(np.int32) if ent
+               else np.zeros((0, 4), np.int32))
+        vf = np.concatenate([self.chim[e][2] for e in ent]) if ent else np.zeros(0)
+        return np.array(ent, np.int32), n1, n2, np.ascontiguousarray(don), np.ascontiguousarray(vf, np.float64)
 
 
 def _tokens(line):
@@ -241,6 +256,7 @@ def read_bc(path):
     with open(path, "r") as fh:
         lines = [ln for ln in fh.read().splitlines() if ln.strip()]
     ib, rb, sf = [], [], []
+    chim = {}
     p = 0
     while p < len(lines):
         t = _tokens(lines[p])
@@ -277,14 +293,24 @@ def read_bc(path):
             p += 1
             rrec[2], rrec[5], rrec[4] = u
         elif typ == 102:
-            raise NotImplementedError("chimera (102) boundary entries are not supported")
+            # receiver counts of the two ghost layers, then one line per donor:
+            # b i j k volume_fraction
+            n1, n2 = (int(x) for x in _tokens(lines[p])[:2])
+            p += 1
+            don, vf = [], []
+            for _ in range(n1 + n2):
+                u = _tokens(lines[p])
+                p += 1
+                don.append([int(x) for x in u[:4]])
+                vf.append(_fnum(u[4]))
+            chim[len(ib)] = (n1, np.asarray(don, np.int32).reshape(-1, 4), np.asarray(vf, np.float64))
         elif typ not in (0, 300):
             raise ValueError(f"{path}: unknown BC type {typ}")
         ib.append(irec)
         rb.append(rrec)
         sf.append(series)
     return BCList(np.asarray(ib, dtype=np.int32).reshape(-1, BC_NI),
-                  np.asarray(rb, dtype=np.float64).reshape(-1, BC_NR), sf)
+                  np.asarray(rb, dtype=np.float64).reshape(-1, BC_NR), sf, chim)
 
 
 def write_bc(path, bcl: BCList):
@@ -307,6 +333,11 @@ def write_bc(path, bcl: BCList):
             out.append(f"  {r[5]: .15E},  {r[4]: .15E},")
         elif typ == 305:
             out.append(f"  {r[2]: .15E},  {r[5]: .15E},  {r[4]: .15E},")
+        elif typ == 102:
+            n1, don, vf = bcl.chim[n]
+            out.append(f"{n1:8d}{len(vf) - n1:8d}")
+            for d, v in zip(don, vf):
+                out.append("".join(f"{int(x):8d}" for x in d) + f"  {v: .15E}")
     with open(path, "w") as fh:
         fh.write("\n".join(out) + "\n")
 

@@ -61,6 +61,11 @@ typedef struct {
   double Tg[6], matIDg;
   double Mg[2][9];
   int series;
+  /* chimera (102): nch donors (b,i,j,k; 1-based) with volume fractions, the
+   * first nch1 for ghost layer 1 (IO_BC.f90:218-229) */
+  int nch1, nch;
+  int *cdon;
+  double *cvf;
 } ob_bc;
 
 typedef struct {
@@ -445,6 +450,58 @@ static int connect_metrics(ob_ctx *c, ob_bc *e, ob_blk *bm, ob_blk *bs) {
   return 0;
 }
 
+/* BC_Extrapolate_Metrics, Lib_Metrics.f90:379-600, first ghost layer: the
+ * ghost cell's nodes extrapolated along the face normal from the block's
+ * node layers (cubic, 3a - 3b + c; linear, 2a - b, across j faces of blocks
+ * with 2 or fewer j cells and across k faces in 2-D), its metric tensor
+ * Mg(1).  Only chimera entries read it (BC_Connection's M_con). */
+static void xnode(double *out, const double *a, const double *b, const double *cc, int cubic) {
+  for (int x = 0; x < 3; x++) out[x] = cubic ? 3.0 * a[x] - 3.0 * b[x] + cc[x] : 2.0 * a[x] - b[x];
+}
+static int extrap_metric(ob_ctx *c, ob_bc *e, ob_blk *bm) {
+  int Im = e->i, Jm = e->j, Km = e->k, Fm = e->f;
+  /* per face: the 4 node columns (i, j, k offsets of layer 0), the step
+   * into the block (-1: the ghost lies at +1), whether the ghost is on the
+   * min side, cubic or linear */
+  int cubic = 1;
+  if ((Fm == 3 || Fm == 4) && bm->n[1] <= 2) cubic = 0;
+  if ((Fm == 5 || Fm == 6) && c->ndir != 3) cubic = 0;
+  int base[4][3], di = 0, dj = 0, dk = 0;
+  switch (Fm) {
+    case 1: { int q[4][3] = {{Im - 1, Jm - 1, Km - 1}, {Im - 1, Jm - 1, Km}, {Im - 1, Jm, Km - 1}, {Im - 1, Jm, Km}}; memcpy(base, q, sizeof(base)); di = 1; } break;
+    case 2: { int q[4][3] = {{Im, Jm - 1, Km - 1}, {Im, Jm - 1, Km}, {Im, Jm, Km - 1}, {Im, Jm, Km}}; memcpy(base, q, sizeof(base)); di = -1; } break;
+    case 3: { int q[4][3] = {{Im - 1, Jm - 1, Km - 1}, {Im - 1, Jm - 1, Km}, {Im, Jm - 1, Km - 1}, {Im, Jm - 1, Km}}; memcpy(base, q, sizeof(base)); dj = 1; } break;
+    case 4: { int q[4][3] = {{Im - 1, Jm, Km - 1}, {Im - 1, Jm, Km}, {Im, Jm, Km - 1}, {Im, Jm, Km}}; memcpy(base, q, sizeof(base)); dj = -1; } break;
+    case 5: { int q[4][3] = {{Im - 1, Jm - 1, Km - 1}, {Im - 1, Jm, Km - 1}, {Im, Jm - 1, Km - 1}, {Im, Jm, Km - 1}}; memcpy(base, q, sizeof(base)); dk = 1; } break;
+    default: { int q[4][3] = {{Im - 1, Jm - 1, Km}, {Im - 1, Jm, Km}, {Im, Jm - 1, Km}, {Im, Jm, Km}}; memcpy(base, q, sizeof(base)); dk = -1; } break;
+  }
+  double L0[4][3], G[4][3];
+  for (int q = 0; q < 4; q++) {
+    int i0 = base[q][0], j0 = base[q][1], k0 = base[q][2];
+    for (int s2 = 0; s2 <= 1 + cubic; s2++) {
+      int ii = i0 + di * s2, jj = j0 + dj * s2, kk = k0 + dk * s2;
+      if (ii < 0 || ii > bm->n[0] || jj < 0 || jj > bm->n[1] || kk < 0 || kk > bm->n[2]) return OB_ERR_ARG;
+    }
+    const double *a = ND(bm, i0, j0, k0), *b = ND(bm, i0 + di, j0 + dj, k0 + dk),
+                 *cc = cubic ? ND(bm, i0 + 2 * di, j0 + 2 * dj, k0 + 2 * dk) : a;
+    memcpy(L0[q], a, 3 * sizeof(double));
+    xnode(G[q], a, b, cc, cubic);
+  }
+  double dl[3], vol;
+  double *Mg = e->Mg[0];
+  int err = 0;
+  switch (Fm) {
+    case 1: err = metric_tensor(G[0], G[1], G[2], G[3], L0[0], L0[1], L0[2], L0[3], Mg, dl, &vol); break;
+    case 2: err = metric_tensor(L0[0], L0[1], L0[2], L0[3], G[0], G[1], G[2], G[3], Mg, dl, &vol); break;
+    case 3: err = metric_tensor(G[0], G[1], L0[0], L0[1], G[2], G[3], L0[2], L0[3], Mg, dl, &vol); break;
+    case 4: err = metric_tensor(L0[0], L0[1], G[0], G[1], L0[2], L0[3], G[2], G[3], Mg, dl, &vol); break;
+    case 5: err = metric_tensor(G[0], L0[0], G[1], L0[1], G[2], L0[2], G[3], L0[3], Mg, dl, &vol); break;
+    default: err = metric_tensor(L0[0], G[0], L0[1], G[1], L0[2], G[2], L0[3], G[3], Mg, dl, &vol); break;
+  }
+  if (err) set_err(c, OB_ERR_METRIC, e->b, e->i, e->j, e->k);
+  return err;
+}
+
 /* ------------------------------------------------------------------ */
 /* ghost fill: src/lib/numerics/space/Lib_Ghost.f90                     */
 /* ------------------------------------------------------------------ */
@@ -468,7 +525,25 @@ static void ghost_fill(ob_ctx *c, ob_dom *d) {
         if (g == 1) e->matIDg = bm->matID[CX(bm, Ig, Jg, Kg)];
       }
     } else if (e->type == 102) {
-      set_err(c, OB_ERR_UNSUPPORTED, e->b, e->i, e->j, e->k);
+      /* Ghost_Chimera, Lib_Ghost.f90:192-245: the first ghost layer from the
+       * volume-fraction-weighted enthalpy of its donors, inverted with the
+       * material of the last donor.  The second layer (donors nch1+1..nch)
+       * is not formed: no term of the step reads it, and the reference
+       * inverts it with a ghost material that nothing sets. */
+      int Ig = e->i - GUIDE[Fm - 1][0], Jg = e->j - GUIDE[Fm - 1][1], Kg = e->k - GUIDE[Fm - 1][2];
+      long pg = CX(bm, Ig, Jg, Kg);
+      double consg = 0.0;
+      for (int q = 0; q < e->nch1; q++) {
+        const int *dn = e->cdon + 4 * q;
+        ob_blk *bs = &d->blk[dn[0] - 1];
+        long ps = CX(bs, dn[1], dn[2], dn[3]);
+        double consi = co_H(c, bs->matID[ps], bs->T[ps]);
+        consg = consg + consi * e->cvf[q];
+        bm->matID[pg] = bs->matID[ps];
+      }
+      bm->T[pg] = co_T(c, bm->matID[pg], consg);
+      e->Tg[0] = bm->T[pg];
+      e->matIDg = bm->matID[pg];
     } else {
       /* Ghost_Symmetry (also the default for walls/null), Lib_Ghost.f90:159-189 */
       for (int g = 1; g <= GC; g++) {
@@ -501,6 +576,42 @@ static void ghost_fill(ob_ctx *c, ob_dom *d) {
       memcpy(p, q, sizeof(p));
     }
     for (int q = 0; q < 4; q++) e->Tg[2 + q] = bs->T[CX(bs, p[q][0], p[q][1], p[q][2])];
+  }
+  /* Tg(3:6) of chimera entries, Lib_Ghost.f90:88-101 with
+   * Fill_BC_Ghost_Chimera :354-433: the ghost's tangential neighbours at the
+   * block's edges take its value, then Tg(3:6) are the boundary cell's own
+   * tangential neighbours */
+  for (int n = 0; n < d->nbound; n++) {
+    ob_bc *e = &d->bc[n];
+    if (e->type != 102) continue;
+    ob_blk *bm = &d->blk[e->b - 1];
+    int Im = e->i, Jm = e->j, Km = e->k, Fm = e->f;
+    int Ig = Im - GUIDE[Fm - 1][0], Jg = Jm - GUIDE[Fm - 1][1], Kg = Km - GUIDE[Fm - 1][2];
+    int *n3 = bm->n;
+    int q[4][3];
+    if (Fm <= 2) {
+      if (Jm == 1) bm->T[CX(bm, Ig, Jm - 1, Km)] = bm->T[CX(bm, Ig, Jm, Km)];
+      if (Jm == n3[1]) bm->T[CX(bm, Ig, Jm + 1, Km)] = bm->T[CX(bm, Ig, Jm, Km)];
+      if (Km == 1) bm->T[CX(bm, Ig, Jm, Km - 1)] = bm->T[CX(bm, Ig, Jm, Km)];
+      if (Km == n3[2]) bm->T[CX(bm, Ig, Jm, Km + 1)] = bm->T[CX(bm, Ig, Jm, Km)];
+      int r[4][3] = {{Im, Jm - 1, Km}, {Im, Jm + 1, Km}, {Im, Jm, Km - 1}, {Im, Jm, Km + 1}};
+      memcpy(q, r, sizeof(q));
+    } else if (Fm <= 4) {
+      if (Im == 1) bm->T[CX(bm, Im - 1, Jg, Km)] = bm->T[CX(bm, Im, Jg, Km)];
+      if (Im == n3[0]) bm->T[CX(bm, Im + 1, Jg, Km)] = bm->T[CX(bm, Im, Jg, Km)];
+      if (Km == 1) bm->T[CX(bm, Im, Jg, Km - 1)] = bm->T[CX(bm, Im, Jg, Km)];
+      if (Km == n3[2]) bm->T[CX(bm, Im, Jg, Km + 1)] = bm->T[CX(bm, Im, Jg, Km)];
+      int r[4][3] = {{Im - 1, Jm, Km}, {Im + 1, Jm, Km}, {Im, Jm, Km - 1}, {Im, Jm, Km + 1}};
+      memcpy(q, r, sizeof(q));
+    } else {
+      if (Im == 1) bm->T[CX(bm, Im - 1, Jm, Kg)] = bm->T[CX(bm, Im, Jm, Kg)];
+      if (Im == n3[0]) bm->T[CX(bm, Im + 1, Jm, Kg)] = bm->T[CX(bm, Im, Jm, Kg)];
+      if (Jm == 1) bm->T[CX(bm, Im, Jm - 1, Kg)] = bm->T[CX(bm, Im, Jm, Kg)];
+      if (Jm == n3[1]) bm->T[CX(bm, Im, Jm + 1, Kg)] = bm->T[CX(bm, Im, Jm, Kg)];
+      int r[4][3] = {{Im - 1, Jm, Km}, {Im + 1, Jm, Km}, {Im, Jm - 1, Km}, {Im, Jm + 1, Km}};
+      memcpy(q, r, sizeof(q));
+    }
+    for (int t = 0; t < 4; t++) e->Tg[2 + t] = bm->T[CX(bm, q[t][0], q[t][1], q[t][2])];
   }
 }
 
@@ -640,8 +751,10 @@ static void bc_fluxes(ob_ctx *c, ob_dom *d) {
     double Tc = TT[p];
     double mat = B->matID[p];
     switch (e->type) {
-      case 101: {
-        /* BC_Connection, Lib_BC_Fluxes_Connection.f90:12-113 */
+      case 101:
+      case 102: {
+        /* BC_Connection, Lib_BC_Fluxes_Connection.f90:12-113 (chimera:
+         * Mod_BC_Fluxes.f90:45, with its ghost, Tg and extrapolated Mg) */
         const double *Mcn = e->Mg[0];
         double Tip = TT[CX(B, Im + 1, Jm, Km)], Tim = TT[CX(B, Im - 1, Jm, Km)];
         double Tjp = TT[CX(B, Im, Jm + 1, Km)], Tjm = TT[CX(B, Im, Jm - 1, Km)];
@@ -1306,6 +1419,7 @@ void ob_destroy(ob_ctx *c) {
     for (int b = 0; b < c->nb; b++)
       if (c->dom[m].blk[b].T) free_blk(&c->dom[m].blk[b]);
     free(c->dom[m].blk);
+    for (int n = 0; n < c->dom[m].nbound; n++) { free(c->dom[m].bc[n].cdon); free(c->dom[m].bc[n].cvf); }
     free(c->dom[m].bc);
     free(c->dom[m].n_bf);
   }
@@ -1363,6 +1477,7 @@ int ob_set_bc(ob_ctx *c, int level, int nbound, const int *ib, const double *rb,
               const int *series) {
   if (!c || level < 1 || level > c->nlev || nbound < 0) return OB_ERR_ARG;
   ob_dom *d = &c->dom[level - 1];
+  for (int n = 0; d->bc && n < d->nbound; n++) { free(d->bc[n].cdon); free(d->bc[n].cvf); }
   free(d->bc);
   free(d->n_bf);
   d->nbound = nbound;
@@ -1380,6 +1495,34 @@ int ob_set_bc(ob_ctx *c, int level, int nbound, const int *ib, const double *rb,
     if (e->b < 1 || e->b > d->nb || e->f < 1 || e->f > 6) return OB_ERR_ARG;
     if (e->type == 101 && (e->bs < 1 || e->bs > d->nb || e->fs < 1 || e->fs > 6)) return OB_ERR_ARG;
     d->n_bf[(e->b - 1) * 6 + (e->f - 1)]++;
+  }
+  return 0;
+}
+
+int ob_set_chimera(ob_ctx *c, int level, int n, const int *entry, const int *n1, const int *n2,
+                   const int *donors, const double *vf) {
+  if (!c || level < 1 || level > c->nlev || n < 0) return OB_ERR_ARG;
+  ob_dom *d = &c->dom[level - 1];
+  long off = 0;
+  for (int q = 0; q < n; q++) {
+    if (entry[q] < 0 || entry[q] >= d->nbound || n1[q] < 1 || n2[q] < 0) return OB_ERR_ARG;
+    ob_bc *e = &d->bc[entry[q]];
+    if (e->type != 102) return OB_ERR_ARG;
+    int m = n1[q] + n2[q];
+    free(e->cdon); free(e->cvf);
+    e->cdon = malloc(4 * (size_t)m * sizeof(int));
+    e->cvf = malloc((size_t)m * sizeof(double));
+    memcpy(e->cdon, donors + 4 * off, 4 * (size_t)m * sizeof(int));
+    memcpy(e->cvf, vf + off, (size_t)m * sizeof(double));
+    e->nch1 = n1[q]; e->nch = m;
+    for (int t = 0; t < m; t++) {
+      const int *dn = e->cdon + 4 * t;
+      if (dn[0] < 1 || dn[0] > d->nb) return OB_ERR_ARG;
+      ob_blk *bs = &d->blk[dn[0] - 1];
+      if (dn[1] < 1 || dn[1] > bs->n[0] || dn[2] < 1 || dn[2] > bs->n[1] || dn[3] < 1 || dn[3] > bs->n[2])
+        return OB_ERR_ARG;
+    }
+    off += m;
   }
   return 0;
 }
@@ -1445,6 +1588,14 @@ int ob_setup(ob_ctx *c, double initial_time) {
       ob_bc *e = &d->bc[n];
       if (e->type == 101) {
         int r = connect_metrics(c, e, &d->blk[e->b - 1], &d->blk[e->bs - 1]);
+        if (r) return r;
+      } else if (e->type == 102) {
+        /* donor lists are given for the finest level only */
+        if (m > 0 || e->nch1 < 1) {
+          set_err(c, OB_ERR_UNSUPPORTED, e->b, e->i, e->j, e->k);
+          return OB_ERR_UNSUPPORTED;
+        }
+        int r = extrap_metric(c, e, &d->blk[e->b - 1]);
         if (r) return r;
       }
     }

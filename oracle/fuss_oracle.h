@@ -61,6 +61,11 @@ int ob_set_bc(ob_ctx *c, int level, int nbound, const int *ib, const double *rb,
 
 /* Numerics: n_rk in {1,2,3} (reference RK), 4 = Jameson four-stage; prim=1 primitive, 0 conservative; irs_beta>0
  * enables implicit residual smoothing. */
+/* chimera donor lists of the level's 102 entries: entry[q] (0-based BC index),
+ * n1[q] + n2[q] donors (b, i, j, k; 1-based) with volume fractions, in entry
+ * order (IO_BC.f90:218-229) */
+int ob_set_chimera(ob_ctx *c, int level, int n, const int *entry, const int *n1, const int *n2,
+                   const int *donors, const double *vf);
 int ob_set_params(ob_ctx *c, int n_rk, double vnn, int rampa_iter,
                   int time_accurate, int prim, double irs_beta);
 

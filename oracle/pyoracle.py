@@ -57,6 +57,7 @@ def lib():
                                      C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.ob_get_bc_ghost.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         L.ob_wall.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.ob_set_chimera.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _dp]
         _lib = L
     return _lib
 
@@ -91,6 +92,10 @@ class Oracle:
             bcl = case.bcs[m]
             sid = case.series_id[m] if m < len(case.series_id) else np.full(bcl.n, -1, np.int32)
             self._chk(self.L.ob_set_bc(self.h, m + 1, bcl.n, _c(bcl.ib, np.int32), _c(bcl.rb), _c(sid, np.int32)))
+            if getattr(bcl, "chim", None):
+                ent, n1, n2, don, vf = bcl.chimera_arrays()
+                self._chk(self.L.ob_set_chimera(self.h, m + 1, len(ent), ent, n1, n2, _c(don.ravel(), np.int32),
+                                                _c(vf)))
         self._chk(self.L.ob_set_params(self.h, p.n_rk, p.vnn, p.rampa_iter, int(p.time_accurate), int(p.prim),
                                        p.irs_beta))
         self._chk(self.L.ob_set_irs_mode(self.h, int(getattr(p, "irs_mode", 0))))

@@ -11,7 +11,7 @@ from fuss_amd import formats as fm
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 NAMES = ("temperature_Plate", "qvol_Plate", "convective_Plate", "radiative_Rod", "multimat_Plate",
-         "oscillating_Rod", "mg_irs")
+         "oscillating_Rod", "mg_irs", "chimera")
 
 
 def load(name):
@@ -24,7 +24,14 @@ def load(name):
     params = cs.Params(**json.loads(bytes(z["params"]).decode()))
     bcs, sids = [], []
     for m in range(params.mgl):
-        bcs.append(fm.BCList(z[f"bc{m}_ib"], z[f"bc{m}_rb"], [None] * z[f"bc{m}_ib"].shape[0]))
+        chim = {}
+        if f"bc{m}_chent" in z:
+            off = 0
+            for e, n1, n2 in zip(z[f"bc{m}_chent"], z[f"bc{m}_chn1"], z[f"bc{m}_chn2"]):
+                sl = slice(off, off + int(n1) + int(n2))
+                chim[int(e)] = (int(n1), z[f"bc{m}_chdon"][sl], z[f"bc{m}_chvf"][sl])
+                off = sl.stop
+        bcs.append(fm.BCList(z[f"bc{m}_ib"], z[f"bc{m}_rb"], [None] * z[f"bc{m}_ib"].shape[0], chim))
         sids.append(z[f"bc{m}_sid"])
     series = [(int(z[f"s{s}_periodic"]), z[f"s{s}_t"], z[f"s{s}_v"]) for s in range(int(z["nseries"]))]
     tab = fm.PropertyTable(int(z["tab_Ti1"]), z["tab_cp"], z["tab_rho"], z["tab_kap"], z["tab_h"], [])

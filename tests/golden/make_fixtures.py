@@ -16,6 +16,10 @@ outputs for it:
   mg_irs
       test/numerics-features/residual_convergence/SOLUTION/mg_irs.dat
       (residual of all 80603 iterations, 10 significant digits)
+  chimera
+      test/numerics-features/chimera/SOLUTION/field.tec -> final T, stop
+      iteration; SOLUTION/wall.tec -> wall Tw, qw (two 2-D blocks of
+      different j resolution joined by chimera (102) entries)
 
 Usage: python tests/golden/make_fixtures.py [/root/reference]
 """
@@ -41,6 +45,7 @@ CASES = {
     "multimat_Plate": "test/transient/multimat_Plate",
     "oscillating_Rod": "test/transient/oscillating_Rod",
     "mg_irs": "test/numerics-features/residual_convergence/mg_irs",
+    "chimera": "test/numerics-features/chimera",
 }
 
 
@@ -57,6 +62,10 @@ def case_arrays(c):
         out[f"bc{m}_ib"] = bcl.ib
         out[f"bc{m}_rb"] = bcl.rb
         out[f"bc{m}_sid"] = c.series_id[m]
+        if bcl.chim:
+            ent, n1, n2, don, vf = bcl.chimera_arrays()
+            out[f"bc{m}_chent"], out[f"bc{m}_chn1"], out[f"bc{m}_chn2"] = ent, n1, n2
+            out[f"bc{m}_chdon"], out[f"bc{m}_chvf"] = don, vf
     out["nseries"] = np.int32(len(c.series))
     for s, (per, t, v) in enumerate(c.series):
         out[f"s{s}_periodic"] = np.int32(per)
@@ -74,7 +83,7 @@ def main(ref="/root/reference"):
         d = os.path.join(ref, rel)
         c = cs.load_case_dir(d)
         arr = case_arrays(c)
-        if name in ("temperature_Plate", "qvol_Plate", "convective_Plate", "radiative_Rod"):
+        if name in ("temperature_Plate", "qvol_Plate", "convective_Plate", "radiative_Rod", "chimera"):
             sol = fm.read_tecplot(os.path.join(d, "SOLUTION", "field.tec"))
             arr["stop_iter"] = np.int32(round(sol[0].solutiontime))
             for b, z in enumerate(sol):

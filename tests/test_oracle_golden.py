@@ -6,7 +6,9 @@ inputs and numerics; the oracle must reproduce the stored solution:
 
 * steady cases: the stop iteration (first residual <= res-threshold), the
   final T field (Tecplot output, 15 significant digits) and the wall output
-  (wall.tec: Tw, qw of every wall face cell);
+  (wall.tec: Tw, qw of every wall face cell) -- among them the chimera case
+  (two blocks of different j resolution joined by 102 entries: ghost
+  enthalpy from volume-fraction-weighted donors, 90 204 iterations);
 * multimat_Plate: six transient snapshots (time and T);
 * oscillating_Rod: both probe histories (time, T) of a periodic 302 series;
 * mg_irs: all 80603 residuals of a 2-level multigrid + IRS + connection run.
@@ -17,7 +19,7 @@ import pytest
 from fuss_amd import formats as fm
 from tests.golden import fixtures
 
-STEADY = ["temperature_Plate", "qvol_Plate", "convective_Plate", "radiative_Rod"]
+STEADY = ["temperature_Plate", "qvol_Plate", "convective_Plate", "radiative_Rod", "chimera"]
 # Tecplot stores 15 significant digits; the oracle follows the reference's
 # operation order, so agreement is at that level.
 FIELD_RTOL = 1e-13
