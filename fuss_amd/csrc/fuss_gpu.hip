@@ -222,6 +222,29 @@ __global__ void k_ghost(R* __restrict__ T, const int2* __restrict__ g, int n, co
   if (q.x >= 0) stu(T, (unsigned)q.x, ldu((const R*)T, (unsigned)q.y));
 }
 
+// Ghost_Chimera (Lib_Ghost.f90:192-245), first layer: the ghost's enthalpy is
+// the volume-fraction-weighted sum of its donors' (co_H at each donor's
+// material and T, summed in donor order), its T the inversion (co_T) with the
+// last donor's material.  r = (ghost, first donor, donors, material); the
+// second layer is read by no term of the step and is not formed.
+__global__ void k_chimera(double* __restrict__ T, const int4* __restrict__ r, const int* __restrict__ don,
+                          const double* __restrict__ vf, int n, const int* __restrict__ mat, Tab<double> tab,
+                          int* status, const int* __restrict__ halt) {
+#pragma clang fp contract(off)
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n || (halt && *halt)) return;
+  const int4 q = r[e];
+  double consg = 0.0;
+  int bad = 0;
+  for (int t = q.y; t < q.y + q.z; t++) {
+    const int p = don[t];
+    const double consi = tab_interp(tab.h + (long)mat[p] * tab.nT, T[p], tab.Ti1, tab.Ti2, &bad);
+    consg = consg + consi * vf[t];
+  }
+  T[q.x] = tab_inverse_search(tab.h + (long)q.w * tab.nT, consg, tab.Ti1, tab.Ti2, &bad);
+  if (bad) flag_error(status, FUSS_ETABLE, -1, 0, 0, 0);
+}
+
 // ------------------------------------------------------------------------
 // kernels: local dt (per iteration)
 // ------------------------------------------------------------------------
@@ -423,8 +446,10 @@ __device__ R bc_value(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e,
   int bad = 0;
   const R* krow = a.tab.kap + (long)a.mat[p] * a.tab.nT;
   switch (e.type) {
-    case 101: {
-      // BC_Connection, Lib_BC_Fluxes_Connection.f90:12-113
+    case 101:
+    case 102: {
+      // BC_Connection, Lib_BC_Fluxes_Connection.f90:12-113 (chimera:
+      // Mod_BC_Fluxes.f90:45, with its ghost as Tg(1) and the extrapolated Mg)
       const R Tip = T[p + 1], Tim = T[p - 1], Tjp = T[p + sy], Tjm = T[p - sy];
       R Tkp = 0, Tkm = 0;
       if (NDIR == 3) { Tkp = T[p + sz]; Tkm = T[p - sz]; }
@@ -556,7 +581,7 @@ __global__ void k_bcflux(StageArgs<R> a, const BCHot* __restrict__ hot, int n) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n || (a.halt && *a.halt)) return;
   const int type = hot[e].type;
-  if (!(type == 101 || (type >= 300 && type <= 305))) return;
+  if (!(type == 101 || type == 102 || (type >= 300 && type <= 305))) return;
   if (type == 301 || type >= 303) {
 #pragma clang fp contract(off)
     const BCHot h = hot[e];
@@ -2590,6 +2615,17 @@ struct Level {
   DBuf<double> bcv, bcf;
   int nbound = 0;
   std::vector<int> hib;             // nbound x 15
+  // chimera (102) donor lists (fuss_set_chimera, host): per listed entry
+  // its BC index and donor counts, then 4 ints (b, i, j, k) and a volume
+  // fraction per donor, entries in list order
+  std::vector<int> hch_ent, hch_n1, hch_n2, hch_don;
+  std::vector<double> hch_vf;
+  // device: per chimera entry (ghost cell, first donor, layer-1 donors,
+  // material index), the donors' arena indices and fractions
+  DBuf<int4> chr;
+  DBuf<int> chd;
+  DBuf<double> chv;
+  int nchim = 0;
   std::vector<double> hrb;          // nbound x 6
   std::vector<int> hser;
   int tang = 0, any_active = 0;
@@ -2896,6 +2932,52 @@ int connect_metric(const HostBlock& bm, const HostBlock& bs, const int* q, doubl
   return e ? FUSS_EMETRIC : 0;
 }
 
+// BC_Extrapolate_Metrics, Lib_Metrics.f90:379-600, first ghost layer: the
+// ghost cell's nodes extrapolated along the face normal from the block's node
+// layers -- cubic (3a - 3b + c), linear (2a - b) across j faces of blocks with
+// 2 or fewer j cells and across k faces of 2-D levels -- and their metric
+// tensor Mg(1) (host side, chimera entries).
+int extrap_metric(const HostBlock& h, int Im, int Jm, int Km, int Fm, int ndir, double* Mg1) {
+  const bool cubic = !((Fm == 3 || Fm == 4) && h.n[1] <= 2) && !((Fm == 5 || Fm == 6) && ndir != 3);
+  int base[4][3], di = 0, dj = 0, dk = 0;
+  switch (Fm) {
+    case 1: { int q[4][3] = {{Im - 1, Jm - 1, Km - 1}, {Im - 1, Jm - 1, Km}, {Im - 1, Jm, Km - 1}, {Im - 1, Jm, Km}}; memcpy(base, q, sizeof(base)); di = 1; } break;
+    case 2: { int q[4][3] = {{Im, Jm - 1, Km - 1}, {Im, Jm - 1, Km}, {Im, Jm, Km - 1}, {Im, Jm, Km}}; memcpy(base, q, sizeof(base)); di = -1; } break;
+    case 3: { int q[4][3] = {{Im - 1, Jm - 1, Km - 1}, {Im - 1, Jm - 1, Km}, {Im, Jm - 1, Km - 1}, {Im, Jm - 1, Km}}; memcpy(base, q, sizeof(base)); dj = 1; } break;
+    case 4: { int q[4][3] = {{Im - 1, Jm, Km - 1}, {Im - 1, Jm, Km}, {Im, Jm, Km - 1}, {Im, Jm, Km}}; memcpy(base, q, sizeof(base)); dj = -1; } break;
+    case 5: { int q[4][3] = {{Im - 1, Jm - 1, Km - 1}, {Im - 1, Jm, Km - 1}, {Im, Jm - 1, Km - 1}, {Im, Jm, Km - 1}}; memcpy(base, q, sizeof(base)); dk = 1; } break;
+    case 6: { int q[4][3] = {{Im - 1, Jm - 1, Km}, {Im - 1, Jm, Km}, {Im, Jm - 1, Km}, {Im, Jm, Km}}; memcpy(base, q, sizeof(base)); dk = -1; } break;
+    default: return FUSS_EBC;
+  }
+  double L0[4][3], G[4][3];
+  for (int q = 0; q < 4; q++) {
+    const double* p[3];
+    for (int s2 = 0; s2 <= (cubic ? 2 : 1); s2++) {
+      const int ii = base[q][0] + di * s2, jj = base[q][1] + dj * s2, kk = base[q][2] + dk * s2;
+      if (ii < 0 || ii > h.n[0] || jj < 0 || jj > h.n[1] || kk < 0 || kk > h.n[2]) return FUSS_EBC;
+      p[s2] = &h.nodes[3 * (ii + (long)(h.n[0] + 1) * (jj + (long)(h.n[1] + 1) * kk))];
+    }
+    {
+#pragma clang fp contract(off)
+      for (int x = 0; x < 3; x++) {
+        L0[q][x] = p[0][x];
+        G[q][x] = cubic ? 3.0 * p[0][x] - 3.0 * p[1][x] + p[2][x] : 2.0 * p[0][x] - p[1][x];
+      }
+    }
+  }
+  double dl[3], v;
+  int e = 0;
+  switch (Fm) {
+    case 1: e = metric_tensor(G[0], G[1], G[2], G[3], L0[0], L0[1], L0[2], L0[3], Mg1, dl, &v); break;
+    case 2: e = metric_tensor(L0[0], L0[1], L0[2], L0[3], G[0], G[1], G[2], G[3], Mg1, dl, &v); break;
+    case 3: e = metric_tensor(G[0], G[1], L0[0], L0[1], G[2], G[3], L0[2], L0[3], Mg1, dl, &v); break;
+    case 4: e = metric_tensor(L0[0], L0[1], G[0], G[1], L0[2], L0[3], G[2], G[3], Mg1, dl, &v); break;
+    case 5: e = metric_tensor(G[0], L0[0], G[1], L0[1], G[2], L0[2], G[3], L0[3], Mg1, dl, &v); break;
+    default: e = metric_tensor(L0[0], G[0], L0[1], G[1], L0[2], G[2], L0[3], G[3], Mg1, dl, &v); break;
+  }
+  return e ? FUSS_EMETRIC : 0;
+}
+
 int upload(void* dst, const void* src, size_t bytes) {
   return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : FUSS_EHIP;
 }
@@ -3060,7 +3142,7 @@ int build_bc(fuss_ctx* c, Level& L) {
     if (fmap[B.fmap_off[f - 1] + mi] != -1) return FUSS_EBC;
     fmap[B.fmap_off[f - 1] + mi] = e;
     const int fpos = B.fmap_off[f - 1] + mi;
-    if (type == 102) return FUSS_EUNSUPPORTED;
+    if (type == 102 && &L != &c->lev[0]) return FUSS_EUNSUPPORTED;  // donor lists: finest level only
     R.type = type; R.f = f; R.dir = (f + 1) / 2; R.series = L.hser.empty() ? -1 : L.hser[e];
     R.b = b; R.i = i; R.j = j; R.k = k;
     R.cell = cidx(B, i, j, k);
@@ -3098,11 +3180,25 @@ int build_bc(fuss_ctx* c, Level& L) {
       if (e2) return e2;
       // matIDg: the donor cell's material (Ghost_Connection copies it)
     }
+    if (type == 102) {
+      // chimera: the flux reads its ghost (filled by k_chimera) as Tg(1),
+      // the boundary cell's own tangential neighbours as Tg(3:6)
+      // (Fill_BC_Ghost_Chimera, Lib_Ghost.f90:354-433) and the extrapolated
+      // ghost metric (BC_Extrapolate_Metrics, Lib_Metrics.f90:379-600)
+      R.gsrc = R.gdst;
+      int p[4][3];
+      if (f <= 2) { int t[4][3] = {{i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}}; memcpy(p, t, sizeof(p)); }
+      else if (f <= 4) { int t[4][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j, k - 1}, {i, j, k + 1}}; memcpy(p, t, sizeof(p)); }
+      else { int t[4][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}}; memcpy(p, t, sizeof(p)); }
+      for (int t = 0; t < 4; t++) R.dn[t] = cidx(B, p[t][0], p[t][1], p[t][2]);
+      const int e2 = extrap_metric(L.hb[b], i, j, k, f, c->ndir, R.Mg);
+      if (e2) return e2;
+    }
     if (type == 101 && (f == 3 || f == 4)) L.no_ycon = 0;
     if (type == 101 && (f == 5 || f == 6)) L.no_zcon = 0;
     if (type == 101 && (f == 1 || f == 2)) L.no_xcon = 0;
-    if (type == 101 || type == 300 || type == 302) L.ghost_read = 1;
-    const bool active = type == 101 || (type >= 300 && type <= 305);
+    if (type == 101 || type == 102 || type == 300 || type == 302) L.ghost_read = 1;
+    const bool active = type == 101 || type == 102 || (type >= 300 && type <= 305);
     if (active) B.face_active |= 1 << (f - 1);
     if (type == 101) B.conn |= 1 << (f - 1);
   }
@@ -3112,7 +3208,46 @@ int build_bc(fuss_ctx* c, Level& L) {
   for (int b = 0; b < L.nb; b++) L.any_active |= L.hblk[b].face_active;
   // (ghost, source) pairs of the ghost fill, 32-bit arena indices
   std::vector<int2> gm(std::max(n, 1));
-  for (int e = 0; e < n; e++) gm[e] = make_int2(recs[e].gdst >= 0 ? (int)recs[e].gdst : -1, (int)recs[e].gsrc);
+  for (int e = 0; e < n; e++)
+    gm[e] = make_int2(recs[e].gdst >= 0 && recs[e].type != 102 ? (int)recs[e].gdst : -1, (int)recs[e].gsrc);
+  // chimera entries: every 102 entry needs its donor list (interior cells)
+  std::vector<int4> chr;
+  std::vector<int> chd;
+  std::vector<double> chv;
+  {
+    std::vector<int> slot(n, -1);
+    size_t off = 0;
+    std::vector<size_t> offs;
+    for (size_t q = 0; q < L.hch_ent.size(); q++) {
+      const int ent = L.hch_ent[q];
+      if (ent < 0 || ent >= n || recs[ent].type != 102 || slot[ent] >= 0) return FUSS_EBC;
+      slot[ent] = (int)q;
+      offs.push_back(off);
+      off += (size_t)(L.hch_n1[q] + L.hch_n2[q]);
+    }
+    for (int e = 0; e < n; e++) {
+      if (recs[e].type != 102) continue;
+      const int q = slot[e];
+      if (q < 0 || L.hch_n1[q] < 1) return FUSS_EBC;
+      const int first = (int)chd.size();
+      int mat = 0;
+      for (int t = 0; t < L.hch_n1[q]; t++) {  // the first ghost layer's donors
+        const int* dn = &L.hch_don[4 * (offs[q] + t)];
+        const int bs = dn[0] - 1;
+        if (bs < 0 || bs >= L.nb) return FUSS_EBC;
+        const BlockInfo& S = L.hblk[bs];
+        if (dn[1] < 1 || dn[1] > S.ni || dn[2] < 1 || dn[2] > S.nj || dn[3] < 1 || dn[3] > S.nk) return FUSS_EBC;
+        chd.push_back((int)cidx(S, dn[1], dn[2], dn[3]));
+        chv.push_back(L.hch_vf[offs[q] + t]);
+        const HostBlock& H = L.hb[bs];
+        mat = (int)H.matID[(dn[1] - 1) + (size_t)H.n[0] * ((dn[2] - 1) + (size_t)H.n[1] * (dn[3] - 1))] - 1;
+      }
+      // Ghost_Chimera leaves the last donor's material in the ghost (matIDg)
+      recs[e].matg = mat;
+      chr.push_back(make_int4((int)recs[e].gdst, first, L.hch_n1[q], mat));
+    }
+  }
+  L.nchim = (int)chr.size();
   int e = L.bc.alloc(std::max(n, 1));
   e |= L.bch.alloc(std::max(n, 1));
   e |= L.gmap.alloc(gm.size());
@@ -3122,6 +3257,12 @@ int build_bc(fuss_ctx* c, Level& L) {
   if (e) return FUSS_EHIP;
   if (n) e |= upload(L.bc.p, recs.data(), sizeof(BCRec) * n);
   if (n) e |= upload(L.gmap.p, gm.data(), sizeof(int2) * n);
+  if (L.nchim) {
+    if (L.chr.alloc(chr.size()) || L.chd.alloc(chd.size()) || L.chv.alloc(chv.size())) return FUSS_EHIP;
+    e |= upload(L.chr.p, chr.data(), sizeof(int4) * chr.size());
+    e |= upload(L.chd.p, chd.data(), sizeof(int) * chd.size());
+    e |= upload(L.chv.p, chv.data(), sizeof(double) * chv.size());
+  }
   if (!fmap.empty()) e |= upload(L.fmap.p, fmap.data(), sizeof(int) * fmap.size());
   (void)c;
   return e;
@@ -3186,6 +3327,9 @@ int launch_ghost(fuss_ctx* c, Level& L, double* T) {
   if (!L.nbound) return 0;
   const int pid = prof_begin(c, FUSS_K_GHOST);
   k_ghost<double><<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(T, L.gmap.p, L.nbound, L.cnt.p + 1);
+  if (L.nchim)
+    k_chimera<<<(L.nchim + 127) / 128, 128, 0, c->stream>>>(T, L.chr.p, L.chd.p, L.chv.p, L.nchim, L.mat.p,
+                                                            dtab<double>(c), c->status.p, L.cnt.p + 1);
   prof_end(c, pid);
   return hipGetLastError() == hipSuccess ? 0 : FUSS_EHIP;
 }
@@ -3220,7 +3364,7 @@ const char* fuss_strerror(int code) {
     case FUSS_ESTATE: return "NaN or T<0 detected";
     case FUSS_ETABLE: return "temperature or enthalpy outside the property table";
     case FUSS_EMETRIC: return "metric tensor det=0 or negative volume";
-    case FUSS_EUNSUPPORTED: return "unsupported feature (chimera 102)";
+    case FUSS_EUNSUPPORTED: return "unsupported feature for this case";
     case FUSS_EHIP: return "HIP runtime error";
     case FUSS_EBC: return "boundary list inconsistent with the blocks";
   }
@@ -3333,6 +3477,25 @@ int fuss_set_bc(fuss_ctx* c, int level, int nbound, const int* ib, const double*
   else L.hser.clear();
   for (int e = 0; e < nbound && series; e++)
     if (series[e] >= (int)c->series.size()) return FUSS_EARG;
+  return 0;
+}
+
+int fuss_set_chimera(fuss_ctx* c, int level, int n, const int* entry, const int* n1, const int* n2,
+                     const int* donors, const double* vf) {
+  if (!c || c->setup_done || level < 1 || level > c->nlev || n < 0 || (n && (!entry || !n1 || !n2))) return FUSS_EARG;
+  Level& L = c->lev[level - 1];
+  size_t m = 0;
+  for (int q = 0; q < n; q++) {
+    if (entry[q] < 0 || entry[q] >= L.nbound || L.hib[15 * (size_t)entry[q] + 5] != 102 || n1[q] < 1 || n2[q] < 0)
+      return FUSS_EARG;
+    m += (size_t)(n1[q] + n2[q]);
+  }
+  if (m && (!donors || !vf)) return FUSS_EARG;
+  L.hch_ent.assign(entry, entry + n);
+  L.hch_n1.assign(n1, n1 + n);
+  L.hch_n2.assign(n2, n2 + n);
+  L.hch_don.assign(donors, donors + 4 * m);
+  L.hch_vf.assign(vf, vf + m);
   return 0;
 }
 
@@ -5200,6 +5363,7 @@ void fuss_destroy(fuss_ctx* c) {
     L.bc.free(); L.bcv.free(); L.bcf.free(); L.bch.free(); L.gmap.free(); L.fmap.free(); L.xytiles.free(); L.partials.free(); L.pmin.free(); L.red2.free(); L.ctiles.free();
     L.c8tiles.free();
     for (auto& r : L.rng) r.free();
+    L.chr.free(); L.chd.free(); L.chv.free();
   }
   c->dcp.free(); c->drho.free(); c->dkap.free(); c->dh.free(); c->dkap2.free(); c->dpr4.free();
   c->status.free(); c->scal.free();

@@ -58,6 +58,7 @@ module fuss_gpu_m
   end type
 
   public :: fuss_create, fuss_set_table, fuss_set_block, fuss_add_series, fuss_set_bc, fuss_setup
+  public :: fuss_set_chimera
   public :: fuss_step, fuss_step_hook, fuss_run, fuss_restrict, fuss_prolong
   public :: fuss_refine_setup, fuss_refine
   public :: fuss_get_T, fuss_set_T, fuss_probe, fuss_get_clock, fuss_set_clock, fuss_ndir, fuss_destroy
@@ -99,6 +100,16 @@ module fuss_gpu_m
       integer(c_int), intent(in) :: ib(*)
       real(c_double), intent(in) :: rb(*)
       integer(c_int), intent(in) :: series(*)
+    end function
+
+    ! chimera (102) donor lists: bc(i)%ni(1:2), %donorID(:,1:4), %volume_fraction
+    integer(c_int) function fuss_set_chimera(ctx, level, n, entry, n1, n2, donors, vf) &
+        bind(c, name='fuss_set_chimera')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level, n
+      integer(c_int), intent(in) :: entry(*), n1(*), n2(*), donors(*)
+      real(c_double), intent(in) :: vf(*)
     end function
 
     integer(c_int) function fuss_setup(ctx, initial_time) bind(c, name='fuss_setup')

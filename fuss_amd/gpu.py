@@ -84,6 +84,7 @@ def load():
     L.fuss_set_block.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
     L.fuss_add_series.argtypes = [P, C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
     L.fuss_set_bc.argtypes = [P, C.c_int, C.c_int, _ip, _dp, _ip]
+    L.fuss_set_chimera.argtypes = [P, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _dp]
     L.fuss_setup.argtypes = [P, C.c_double]
     L.fuss_step.argtypes = [P, C.c_int, C.POINTER(StepParams), C.POINTER(StepResult)]
     L.fuss_run.argtypes = [P, C.c_int, C.POINTER(StepParams), C.c_int, C.c_int, C.c_double, C.c_double,
@@ -173,6 +174,10 @@ class Solver:
             sid = case.series_id[m] if m < len(case.series_id) else np.full(bcl.n, -1, np.int32)
             _chk(self.L.fuss_set_bc(self.h, m + 1, bcl.n, _c(bcl.ib, np.int32), _c(bcl.rb), _c(sid, np.int32)),
                  f"fuss_set_bc level {m+1}")
+            if getattr(bcl, "chim", None):
+                ent, n1, n2, don, vf = bcl.chimera_arrays()
+                _chk(self.L.fuss_set_chimera(self.h, m + 1, len(ent), ent, n1, n2, _c(don.ravel(), np.int32), _c(vf)),
+                     f"fuss_set_chimera level {m+1}")
         _chk(self.L.fuss_setup(self.h, 0.0 if self.p.time_accurate else -1.0), "fuss_setup")
         self.level = self.p.mgl
 

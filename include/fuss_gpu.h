@@ -98,6 +98,18 @@ int fuss_add_series(fuss_ctx *ctx, int periodic, int n, const double *t, const d
 int fuss_set_bc(fuss_ctx *ctx, int level, int nbound, const int *ib, const double *rb,
                 const int *series);
 
+/* Chimera (102) donor lists of a level's BC entries (IO_BC.f90:218-229,
+ * read into bc(i)%ni, %donorID, %volume_fraction): n entries, entry[q] the
+ * 0-based BC index of a 102 entry, n1[q] donors for its first ghost layer and
+ * n2[q] for the second, then per donor 4 ints (block, i, j, k; 1-based) in
+ * `donors` and its volume fraction in `vf`, entries in list order.  Call after
+ * fuss_set_bc and before fuss_setup; finest level only (FUSS_EUNSUPPORTED for
+ * 102 entries on coarser levels).  The ghost fill forms the first layer from
+ * the donors' enthalpies (Ghost_Chimera, Lib_Ghost.f90:192-245); the
+ * boundary flux is the connection flux (Mod_BC_Fluxes.f90:45). */
+int fuss_set_chimera(fuss_ctx *ctx, int level, int n, const int *entry, const int *n1, const int *n2,
+                     const int *donors, const double *vf);
+
 /* Coarse grids, metrics, ghost metrics, initial ghost fill and restriction
  * of the initial state to coarse levels.  initial_time: 0 for time-accurate
  * runs, -1 for steady runs (Wrap_Setup.f90:123-144). */

@@ -19,8 +19,8 @@ from tests.test_oracle_golden import check_wall
 
 pytestmark = pytest.mark.gpu
 RTOL = 1e-12
-ALL = [n for n in fixtures.NAMES if n != "chimera"]  # chimera: GPU path pending
-STEADY = ["temperature_Plate", "qvol_Plate", "convective_Plate", "radiative_Rod"]
+ALL = list(fixtures.NAMES)
+STEADY = ["temperature_Plate", "qvol_Plate", "convective_Plate", "radiative_Rod", "chimera"]
 
 
 @pytest.fixture(scope="module")
