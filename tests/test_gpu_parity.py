@@ -98,6 +98,13 @@ def tridiag(c, beta=None):
     return c
 
 
+def with_type(c, old, new):
+    """The case with every BC entry of type `old` changed to `new`."""
+    for bcl in c.bcs:
+        bcl.ib[bcl.ib[:, 5] == old, 5] = new
+    return c
+
+
 SYNTH = {
     # 3-D orthogonal stretched cube: 303 walls, qvol, k(T) table, IRS, 3 levels
     "cube3d_mg3_irs": lambda: cases.cube3d(24, mgl=3),
@@ -130,6 +137,9 @@ SYNTH = {
     "lattice_tridiag_mg3": lambda: tridiag(cases.lattice(nl=2, lo=8, hi=20, irs_beta=0.5, time_accurate=False,
                                                          mgl=3)),
     "plate2d_tridiag": lambda: tridiag(cases.plate2d(40), beta=0.6),
+    # the 2-D plate with its 301 walls turned into 103 coupling entries
+    # (ext_flux 0: no flux, mirrored ghost)
+    "plate2d_103": lambda: with_type(cases.plate2d(32), 301, 103),
 }
 
 

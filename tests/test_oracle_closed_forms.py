@@ -231,3 +231,22 @@ def test_prolongation_and_restriction(oracle_mod):
     o.set("T", np.full((1, m // 2, n // 2), 321.0), level=2)
     o.prolong(1)
     np.testing.assert_allclose(o.get("T", 1), 321.0, rtol=1e-15)
+
+
+def test_coupling_entry_is_null(oracle_mod):
+    """A 103 entry adds its ext_flux to R (Mod_BC_Fluxes.f90:76-77), which
+    the reference's reader sets to 0 (IO_BC.f90:235), and mirrors its ghost
+    like a null entry: 20 steps with the plate's 301 walls as 103 entries
+    equal the same steps with them as null (type 0) entries bit for bit."""
+    from fuss_amd import cases
+
+    out = []
+    for t in (103, 0):
+        c = cases.plate2d(24)
+        for bcl in c.bcs:
+            bcl.ib[bcl.ib[:, 5] == 301, 5] = t
+        o = oracle_mod.Oracle(c)
+        for _ in range(20):
+            o.step(1)
+        out.append(o.get("T", 1, 1))
+    assert np.array_equal(out[0], out[1])
