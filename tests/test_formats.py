@@ -47,6 +47,34 @@ def test_bc_types_round_trip(tmp_path):
             assert r.rb[n, q] == expect
 
 
+def test_chimera_bc_round_trip(tmp_path):
+    """102 entries carry their donor lists (IO_BC.f90:218-229): the chimera
+    fixture's BC list (the reference's own bc.txt as read) writes and reads
+    back with the same entries, layer counts, donors and volume fractions,
+    and in the layout fuss_set_chimera takes."""
+    from tests.golden import fixtures
+
+    c, _ = fixtures.load("chimera")
+    bcl = c.bcs[0]
+    assert len(bcl.chim) == 17 and all(int(t) == 102 for t in bcl.ib[sorted(bcl.chim), 5])
+    p = os.path.join(str(tmp_path), "bc.txt")
+    fm.write_bc(p, bcl)
+    r = fm.read_bc(p)
+    np.testing.assert_array_equal(r.ib, bcl.ib)
+    assert sorted(r.chim) == sorted(bcl.chim)
+    for e, (n1, don, vf) in bcl.chim.items():
+        assert r.chim[e][0] == n1
+        np.testing.assert_array_equal(r.chim[e][1], don)
+        np.testing.assert_array_equal(r.chim[e][2], vf)
+    ent, n1, n2, don, vf = r.chimera_arrays()
+    assert len(ent) == 17 and don.shape == (int(n1.sum() + n2.sum()), 4) and vf.shape == (don.shape[0],)
+    # the layer-1 fractions of each ghost sum to one (a partition of its volume)
+    off = 0
+    for a, b in zip(n1, n2):
+        assert abs(vf[off:off + a].sum() - 1.0) < 1e-8
+        off += a + b
+
+
 def test_bc_count_check():
     c = cases.plate2d(n=8)
     fm.check_bc_count(c.bcs[0], [b.dims for b in c.blocks])
