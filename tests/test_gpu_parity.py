@@ -468,6 +468,32 @@ def test_column_rows_keep_results(gpu, name, rect):
         assert np.array_equal(dt, out[0][2])
 
 
+def test_chimera_setup_errors(gpu):
+    """A 102 entry without its donor list is an inconsistent BC list
+    (FUSS_EBC at fuss_setup), and a donor list naming a cell outside its
+    block or a non-102 entry is rejected when it is set (FUSS_EARG)."""
+    c, _ = fixtures.load("chimera")
+    bcl = c.bcs[0]
+    first = min(bcl.chim)
+    for bad in ("missing", "donor", "entry"):
+        chim = dict(bcl.chim)
+        if bad == "missing":
+            del chim[first]
+        elif bad == "donor":
+            n1, don, vf = chim[first]
+            don = don.copy()
+            don[0, 1] = 10**6
+            chim[first] = (n1, don, vf)
+        else:
+            chim[0] = chim.pop(first)  # entry 0 is a 302 wall
+        c2 = fixtures.load("chimera")[0]
+        c2.bcs[0].chim = chim
+        with pytest.raises(gpu.FussError) as ei:
+            gpu.Solver(c2)
+        msg = str(ei.value)
+        assert ("fuss_setup" in msg) if bad in ("missing", "donor") else ("fuss_set_chimera" in msg), msg
+
+
 def test_xrow_takes_fused_irs(gpu):
     """Blocks joined on i faces only run the fused x-y-z IRS pass (one
     IRS launch per stage, the update in it), so xrow_irs exercises its
