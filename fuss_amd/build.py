@@ -54,11 +54,31 @@ def build_fortran(verbose=False):
     return DEMO
 
 
-def build(verbose=False):
-    """Compile fuss_amd/csrc/*.hip into fuss_amd/libfuss_gpu.so (always rebuilds)."""
+def build(verbose=False, jobs=None):
+    """Compile each fuss_amd/csrc/*.hip unit (in parallel) and link them into
+    fuss_amd/libfuss_gpu.so (always rebuilds)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     srcs = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")]
+    odir = os.path.join(CSRC, "_build")
+    os.makedirs(odir, exist_ok=True)
+    inc = ["-I", os.path.join(HERE, "..", "include")]
+
+    def compile_one(src):
+        obj = os.path.join(odir, os.path.basename(src)[:-4] + ".o")
+        cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"] + inc + ["-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {os.path.basename(src)}:\n{r.stderr}")
+        return obj
+
+    jobs = jobs or max(1, min(len(srcs), os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(HERE, "..", "include"), "-o", tmp] + srcs
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -26,19 +26,7 @@
 #include <vector>
 
 #include "../../include/fuss_gpu.h"
-#include "fuss_kernels.h"
-
-using namespace fuss;
-
-#define HIPCHK(x)                                                     \
-  do {                                                                \
-    hipError_t e_ = (x);                                              \
-    if (e_ != hipSuccess) {                                           \
-      fprintf(stderr, "fuss_gpu: HIP error %s at %s:%d\n",            \
-              hipGetErrorString(e_), __FILE__, __LINE__);             \
-      return FUSS_EHIP;                                               \
-    }                                                                 \
-  } while (0)
+#include "fuss_host.h"
 
 // ------------------------------------------------------------------------
 // kernels: geometry (setup)
@@ -245,25 +233,6 @@ __global__ void k_chimera(double* __restrict__ T, const int4* __restrict__ r, co
   if (bad) flag_error(status, FUSS_ETABLE, -1, 0, 0, 0);
 }
 
-// ------------------------------------------------------------------------
-// kernels: local dt (per iteration)
-// ------------------------------------------------------------------------
-template <typename R>
-struct DtArgs {
-  const BlockInfo* blk;
-  const Tile* tiles;
-  const R* T;
-  const int* mat;
-  const double* invL2;
-  R* dt;
-  Tab<R> tab;
-  double vnn;
-  const int* cnt;      // level counters: [0] iteration before this step, [1] halt
-  int rampa;
-  double* pmin;        // per-work-group minimum
-  int* status;
-};
-
 // Compute_dt, Mod_dt.f90:26-84
 template <typename R>
 __global__ __launch_bounds__(TILE_T) void k_dt(DtArgs<R> a) {
@@ -306,276 +275,6 @@ __global__ __launch_bounds__(TILE_T) void k_dt(DtArgs<R> a) {
   }
 }
 
-// ------------------------------------------------------------------------
-// kernels: stage (fluxes + BC fluxes + newstate)
-// ------------------------------------------------------------------------
-template <typename R>
-struct StageArgs {
-  const BlockInfo* blk;
-  const Tile* tiles;
-  const Tile* ctiles;  // column tiles (k0 = first plane of a KC-plane chunk)
-  const R* T;          // stage input (ghosts filled)
-  const R* TO;         // T^n
-  R* Tout;             // stage output (update mode)
-  R* Rres;             // scaled residual (IRS mode)
-  const R* dt;
-  const double* dtg;   // device dtglobal (time-accurate)
-  int use_dtg;
-  const R* qvol;
-  const R* vol;
-  const int* mat;
-  const R* W[9];
-  const BCRec* bc;
-  const int* fmap;
-  R* bcv;              // per BC entry: value subtracted from the cell residual
-  R* bcf;              // the same values at the entries' face-map positions
-  Tab<R> tab;
-  SerDev ser;          // 301/302 time series, evaluated at *clk
-  const double* clk;   // level time (device)
-  const int* halt;     // nonzero: the step is skipped (graph replay past a stop)
-  R rkc;
-  int jam;             // four-stage Jameson update (n_rk = 4)
-  int last;
-  double* partials;
-  int* status;
-  const double* rect;  // per-axis metric factors (rectilinear levels, k_stage3p<RECT>)
-  // k_stage3p<DTC>: the first stage of a steady step computes the local dt
-  // from its input (T^n) and stores it for the later stages
-  double vnn;
-  R* dtw;
-  double* pmin;        // per-tile dt minima
-};
-
-// One interior face flux between cells q and q+e_D (left cell q), in the
-// gather form of Diffusive_Flux (Lib_Diffusive.f90:10-45).
-template <int D, int NDIR, bool TANG, typename R>
-__device__ __forceinline__ R face_flux(const StageArgs<R>& a, const BlockInfo& B, long q) {
-  const R* __restrict__ T = a.T;
-  const long sy = B.sy, sz = B.sz;
-  const long s = D == 0 ? 1 : (D == 1 ? sy : sz);
-  const R Tl = T[q], Tr = T[q + s];
-  R G0, G1 = 0, G2 = 0;
-  G0 = Tr - Tl;
-  if (TANG) {
-    // tangential differences, Tangential_Gradient (Lib_Diffusive.f90:48-55)
-    const long t1 = D == 0 ? sy : 1;
-    const long t2 = D == 2 ? sy : sz;
-    G1 = (T[q + t1] - T[q - t1] + T[q + s + t1] - T[q + s - t1]) * (R)0.25;
-    if (NDIR == 3 || D == 2) G2 = (T[q + t2] - T[q - t2] + T[q + s + t2] - T[q + s - t2]) * (R)0.25;
-  }
-  int bad = 0;
-  const R kap = tab_interp(a.tab.kap + (long)a.mat[q] * a.tab.nT, (R)0.5 * (Tl + Tr), a.tab.Ti1,
-                           a.tab.Ti2, &bad);
-  if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
-  // weights in computational-direction order; the normal direction is D
-  const R wn = a.W[3 * D + D][q];
-  if (!TANG) return kap * (G0 * wn);
-  const int c1 = D == 0 ? 1 : 0, c2 = D == 2 ? 1 : 2;
-  const R w1 = a.W[3 * D + c1][q], w2 = a.W[3 * D + c2][q];
-  // sum over c = 0,1,2 in order of the computational index
-  R g[3], w[3];
-  g[D] = G0; w[D] = wn;
-  g[c1] = G1; w[c1] = w1;
-  g[c2] = G2; w[c2] = w2;
-  return kap * ((g[0] * w[0] + g[1] * w[1]) + g[2] * w[2]);
-}
-
-template <typename R>
-__device__ __forceinline__ R dotrow(const double* M, int r, const double* N) {
-  // dot_product(M(r,:), N), r = 1..3
-  return (R)(M[r - 1] * N[0] + M[r - 1 + 3] * N[1] + M[r - 1 + 6] * N[2]);
-}
-
-// Wall fluxes that need only scalars of their entry; bc_value (full record)
-// and the compact-record pass of k_bcflux both evaluate them here.
-// BC_Wall_HeatTransfer, Lib_BC_Fluxes_Wall_HeatTransfer.f90:12-52 (the
-// reference passes bc%qw, never set for 303: 0 here).
-template <typename R>
-__device__ __forceinline__ R bcf_htc(R kappa, R mn, R A, R hconv, R Tref, R Tc) {
-#pragma clang fp contract(off)
-  const R H = (R)1.0 / ((R)2.0 * kappa * mn) + (R)1.0 / hconv;
-  const R Hrad = hconv / ((R)2.0 * kappa * mn) + (R)1.0;
-  return A * ((R)1.0 / H * (Tref - Tc) + (R)1.0 / Hrad * (R)0.0);
-}
-
-// BC_Wall_Radiation(+Convection), Lib_BC_Fluxes_Wall_Radiation.f90:12-65 and
-// Lib_BC_Fluxes_Wall_RadiationConvection.f90:12-65.  The fixed point stays in
-// FP64: its tolerance (1e-3 W/m^2) is below the FP32 resolution of q.
-// Returns the flux; *fail is set when the iteration does not settle.
-__device__ __forceinline__ double bcf_rad(int type, double kappa, double mn, double A, double Tref, double eps,
-                                          double hconv, double Tcd, int* fail) {
-#pragma clang fp contract(off)
-  const double Tr2 = Tref * Tref, Tr4 = Tr2 * Tr2;
-  double Tw = Tcd, qw = 0.0, err = 1.0e6, Flux = 0.0;
-  int it = 0;
-  while (err > 1.0e-3) {
-    const double Tw2 = Tw * Tw;
-    const double K = SIGMA_SB * eps * (Tw2 * Tw2 - Tr4) / (Tw - Tref);
-    const double H = type == 304 ? 1.0 / (2.0 * kappa * mn) + 1.0 / K : 1.0 / (2.0 * kappa * mn) + 1.0 / (K + hconv);
-    Flux = A * (1.0 / H * (Tref - Tcd));
-    const double q_wall = K * (Tref - Tw);
-    err = fabs(qw - q_wall);
-    qw = q_wall;
-    Tw = Tref - 1.0 / (H * K) * (Tref - Tcd);
-    if (++it > 1000 || err != err) {
-      *fail = 1;
-      break;
-    }
-  }
-  return Flux;
-}
-
-// BC_Fluxes for one entry (Mod_BC_Fluxes.f90:43-79 and the per-type files
-// src/lib/numerics/fluxes/bc/Lib_BC_Fluxes_*.f90), in the reference's
-// arithmetic order.
-// Returns v such that the reference's residual update is R = R - v; the stage
-// kernels apply the v of a cell's faces in face order, which reproduces the
-// reference's accumulation exactly.
-template <int NDIR, typename R>
-__device__ R bc_value(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e, long p, double sv) {
-#pragma clang fp contract(off)
-  R Rv = 0;
-  const R* __restrict__ T = a.T;
-  const int Fm = e.f;
-  const int modfm2 = 1 - 2 * (Fm % 2);
-  const int modfm3 = 2 * (2 * (Fm % 2) - 1);
-  const R Tc = T[p];
-  const long sy = B.sy, sz = B.sz;
-  const double* M = e.M;
-  const double* Nr = e.N;
-  int bad = 0;
-  const R* krow = a.tab.kap + (long)a.mat[p] * a.tab.nT;
-  switch (e.type) {
-    case 101:
-    case 102: {
-      // BC_Connection, Lib_BC_Fluxes_Connection.f90:12-113 (chimera:
-      // Mod_BC_Fluxes.f90:45, with its ghost as Tg(1) and the extrapolated Mg)
-      const R Tip = T[p + 1], Tim = T[p - 1], Tjp = T[p + sy], Tjm = T[p - sy];
-      R Tkp = 0, Tkm = 0;
-      if (NDIR == 3) { Tkp = T[p + sz]; Tkm = T[p - sz]; }
-      const R Tcon = T[e.gsrc];
-      const R kl = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
-      const R kc = tab_interp(a.tab.kap + (long)e.matg * a.tab.nT, Tcon, a.tab.Ti1, a.tab.Ti2, &bad);
-      const R g3 = T[e.dn[0]], g4 = T[e.dn[1]], g5 = T[e.dn[2]], g6 = T[e.dn[3]];
-      const R ml0 = dotrow<R>(M, 1, Nr), ml1 = dotrow<R>(M, 2, Nr), ml2 = dotrow<R>(M, 3, Nr);
-      const R mc0 = (R)e.mc[0], mc1 = (R)e.mc[1], mc2 = (R)e.mc[2];
-      R Dl, Nl, Dc, Nc;
-      if (Fm <= 2) {
-        Dl = (R)2.0 * ml0 * modfm2;
-        Nl = (R)2.0 * Tc * ml0 * modfm2 - (R)0.5 * (Tjp - Tjm) * ml1 - (R)0.5 * (Tkp - Tkm) * ml2;
-        Dc = (R)2.0 * mc0 * modfm2;
-        Nc = (R)2.0 * Tcon * mc0 * modfm2 + (R)0.5 * (g4 - g3) * mc1 + (R)0.5 * (g6 - g5) * mc2;
-      } else if (Fm <= 4) {
-        Dl = (R)2.0 * ml1 * modfm2;
-        Nl = -(R)0.5 * (Tip - Tim) * ml0 + (R)2.0 * Tc * ml1 * modfm2 - (R)0.5 * (Tkp - Tkm) * ml2;
-        Dc = (R)2.0 * mc1 * modfm2;
-        Nc = (R)0.5 * (g4 - g3) * mc0 + (R)2.0 * Tcon * mc1 * modfm2 + (R)0.5 * (g6 - g5) * mc2;
-      } else {
-        Dl = (R)2.0 * ml2 * modfm2;
-        Nl = -(R)0.5 * (Tip - Tim) * ml0 - (R)0.5 * (Tjp - Tjm) * ml1 + (R)2.0 * Tc * ml2 * modfm2;
-        Dc = (R)2.0 * mc2 * modfm2;
-        Nc = (R)0.5 * (g4 - g3) * mc0 + (R)0.5 * (g6 - g5) * mc1 + (R)2.0 * Tcon * mc2 * modfm2;
-      }
-      const R Tint = (kl * Nl + kc * Nc) / (kl * Dl + kc * Dc);
-      R G[3];
-      if (Fm <= 2) { G[0] = (R)2.0 * (Tint - Tc) * modfm2; G[1] = (Tjp - Tjm) * (R)0.5; G[2] = (Tkp - Tkm) * (R)0.5; }
-      else if (Fm <= 4) { G[0] = (Tip - Tim) * (R)0.5; G[1] = (R)2.0 * (Tint - Tc) * modfm2; G[2] = (Tkp - Tkm) * (R)0.5; }
-      else { G[0] = (Tip - Tim) * (R)0.5; G[1] = (Tjp - Tjm) * (R)0.5; G[2] = (R)2.0 * (Tint - Tc) * modfm2; }
-      if (NDIR == 2) G[2] = 0;
-      R gr[3];
-      for (int x = 0; x < 3; x++) gr[x] = G[0] * (R)M[3 * x] + G[1] * (R)M[3 * x + 1] + G[2] * (R)M[3 * x + 2];
-      const R Flux = (R)e.A * kl * (gr[0] * (R)Nr[0] + gr[1] * (R)Nr[1] + gr[2] * (R)Nr[2]);
-      Rv = (R)modfm2 * Flux;
-    } break;
-    case 300: {
-      // BC_Symmetry, Lib_BC_Fluxes_Symmetry.f90:12-67
-      R G[3] = {0, 0, 0};
-      if (Fm > 2) G[0] = (R)0.5 * (T[p + 1] - T[p - 1]) * modfm2;
-      if (Fm <= 2 || Fm > 4) G[1] = (R)0.5 * (T[p + sy] - T[p - sy]) * modfm2;
-      if (NDIR == 3 && Fm <= 4) G[2] = (R)0.5 * (T[p + sz] - T[p - sz]) * modfm2;
-      R gr[3];
-      for (int x = 0; x < 3; x++) gr[x] = G[0] * (R)M[3 * x] + G[1] * (R)M[3 * x + 1] + G[2] * (R)M[3 * x + 2];
-      const R kappa = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
-      const R Flux = (R)e.A * kappa * (gr[0] * (R)Nr[0] + gr[1] * (R)Nr[1] + gr[2] * (R)Nr[2]);
-      Rv = (R)modfm2 * Flux;
-    } break;
-    case 301: {
-      // BC_Wall_Heat, Lib_BC_Fluxes_Wall_Heat.f90:12-54
-      const R qw = e.series >= 0 ? (R)sv : (R)e.qw;
-      const R Flux = (R)e.A * qw;
-      Rv = (R)modfm2 * Flux;
-    } break;
-    case 302: {
-      // BC_Wall_Temperature, Lib_BC_Fluxes_Wall_Temperature.f90:12-68
-      const R Tw = e.series >= 0 ? (R)sv : (R)e.Tw;
-      const R kappa = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
-      R G[3] = {0, 0, 0};
-      const R gn = (Tc - Tw) * (R)modfm3;
-      if (e.dir == 1) G[0] = gn;
-      else if (e.dir == 2) G[1] = gn;
-      else G[2] = gn;
-      if (Fm <= 2) {
-        G[1] = (T[p - sy] - T[p + sy]) / (R)2.0;
-        if (NDIR == 3) G[2] = (T[p - sz] - T[p + sz]) / (R)2.0;
-      } else if (Fm <= 4) {
-        G[0] = (T[p - 1] - T[p + 1]) / (R)2.0;
-        if (NDIR == 3) G[2] = (T[p - sz] - T[p + sz]) / (R)2.0;
-      } else {
-        G[0] = (T[p - 1] - T[p + 1]) / (R)2.0;
-        G[1] = (T[p - sy] - T[p + sy]) / (R)2.0;
-      }
-      if (NDIR == 2) G[2] = 0;
-      R gr[3];
-      for (int x = 0; x < 3; x++) gr[x] = G[0] * (R)M[3 * x] + G[1] * (R)M[3 * x + 1] + G[2] * (R)M[3 * x + 2];
-      const R Flux = (R)e.A * kappa * (gr[0] * (R)Nr[0] + gr[1] * (R)Nr[1] + gr[2] * (R)Nr[2]);
-      Rv = (R)modfm2 * Flux;
-    } break;
-    case 303: {
-      const R kappa = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
-      Rv = bcf_htc<R>(kappa, dotrow<R>(M, e.dir, Nr), (R)e.A, (R)e.hconv, (R)e.Tref, Tc);
-    } break;
-    case 304:
-    case 305: {
-      const double kappa = (double)tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
-      const double mn = M[e.dir - 1] * Nr[0] + M[e.dir - 1 + 3] * Nr[1] + M[e.dir - 1 + 6] * Nr[2];
-      int fail = 0;
-      Rv = (R)bcf_rad(e.type, kappa, mn, e.A, e.Tref, e.eps, e.hconv, (double)Tc, &fail);
-      if (fail) flag_error(a.status, FUSS_ESTATE, e.b, e.i, e.j, e.k);
-    } break;
-    default:
-      break;
-  }
-  if (bad) flag_error(a.status, FUSS_ETABLE, e.b, e.i, e.j, e.k);
-  return Rv;
-}
-
-template <typename R>
-__device__ __forceinline__ int face_map_index(const BlockInfo& B, int f, int i, int j, int k) {
-  if (f <= 2) return B.fmap_off[f - 1] + (j - 1) + B.nj * (k - 1);
-  if (f <= 4) return B.fmap_off[f - 1] + (i - 1) + B.ni * (k - 1);
-  return B.fmap_off[f - 1] + (i - 1) + B.ni * (j - 1);
-}
-
-// BC entries of a boundary cell in the (block, face) order of the BC list
-// (Mod_BC_Fluxes.f90:29-33): faces 1..6 in turn.
-template <typename R>
-__device__ __forceinline__ R apply_bc_faces(const StageArgs<R>& a, const BlockInfo& B, int i, int j, int k, R Rv) {
-#pragma clang fp contract(off)
-  const int fa = B.face_active;
-  if (!fa) return Rv;
-  if ((fa & 1) && i == 1) Rv = Rv - a.bcf[B.fmap_off[0] + (j - 1) + B.nj * (k - 1)];
-  if ((fa & 2) && i == B.ni) Rv = Rv - a.bcf[B.fmap_off[1] + (j - 1) + B.nj * (k - 1)];
-  if ((fa & 4) && j == 1) Rv = Rv - a.bcf[B.fmap_off[2] + (i - 1) + B.ni * (k - 1)];
-  if ((fa & 8) && j == B.nj) Rv = Rv - a.bcf[B.fmap_off[3] + (i - 1) + B.ni * (k - 1)];
-  if ((fa & 16) && k == 1) Rv = Rv - a.bcf[B.fmap_off[4] + (i - 1) + B.ni * (j - 1)];
-  if ((fa & 32) && k == B.nk) Rv = Rv - a.bcf[B.fmap_off[5] + (i - 1) + B.ni * (j - 1)];
-  return Rv;
-}
-
-// BC_Fluxes (Mod_BC_Fluxes.f90:10-86) as a surface pass: one thread per
-// entry stores the value its face subtracts from the cell residual.
-// Entries of type 301/303/304/305 are evaluated from their compact record;
-// 101, 300 and 302 (tangential gradients, donor data) from the full record.
 template <int NDIR, typename R>
 __global__ void k_bcflux(StageArgs<R> a, const BCHot* __restrict__ hot, int n) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -730,1566 +429,6 @@ __global__ void k_bc_hot(const BCRec* __restrict__ bc, int n, const int* __restr
   hot[e] = h;
 }
 
-// RK_stage, Lib_RK.f90:15-26.  jam: the four-stage scheme of Jameson,
-// Schmidt and Turkel (new = old + a_k res, a = 1/4, 1/3, 1/2, 1), which has
-// no counterpart in the reference (its RKcoeff stops at three stages).
-template <typename R>
-__device__ __forceinline__ R rk_stage(R state, R old, R res, R c, int jam) {
-#pragma clang fp contract(off)
-  if (jam) return old + res * c;
-  const R r_ = (state - old + res) * c;
-  return old + r_;
-}
-
-// Newstate for one cell from its scaled residual.  PRIM: Newstate_Primitive
-// (Lib_Newstate.f90:241-270); else Newstate_Conservative (:99-146).
-template <bool PRIM, typename R>
-__device__ __forceinline__ R new_state(const Tab<R>& tab, int m, R T, R TO, R res, R c, int jam, int* bad) {
-#pragma clang fp contract(off)
-  if (PRIM) return rk_stage(T, TO, res, c, jam);
-  const long row = (long)m * tab.nT;
-  const R hO = tab_interp(tab.h + row, TO, tab.Ti1, tab.Ti2, bad);
-  R h = tab_interp(tab.h + row, T, tab.Ti1, tab.Ti2, bad);
-  h = rk_stage(h, hO, res, c, jam);
-  return tab_inverse(tab.h + row, h, T, tab.Ti1, tab.Ti2, bad);
-}
-
-template <bool PRIM, typename R>
-__device__ __forceinline__ R scaled_residual(const Tab<R>& tab, int m, R T, R Rv, R vol, R qvol, R dt,
-                                             int* bad) {
-#pragma clang fp contract(off)
-  if (!PRIM) return (-Rv / vol + qvol) * dt;  // Scale_Residual_Cons, Lib_Newstate.f90:99-104
-  // co_Rho / co_Cp / co_DrhoDT / co_DcpDT from one (rho, drho, cp, dcp) row
-  R Tc = T;
-  if (!(Tc >= (R)tab.Ti1 && Tc < (R)tab.Ti2)) {
-    *bad = 1;
-    Tc = Tc < (R)tab.Ti1 || Tc != Tc ? (R)tab.Ti1 : (R)(tab.Ti2 - 1);
-  }
-  const int Tm = (int)Tc;
-  const typename Vec4<R>::T v =
-      ldu(reinterpret_cast<const typename Vec4<R>::T*>(tab.pr4), (unsigned)((long)m * tab.nT + (Tm - tab.Ti1)));
-  const R rho = v.x + (Tc - (R)Tm) * v.y;
-  const R cp = v.z + (Tc - (R)Tm) * v.w;
-  const R drho = v.y, dcp = v.w;
-  // Compute_Prim_Residual, Lib_Newstate.f90:241-256
-  return (-Rv / vol + qvol) * dt / (T * cp * drho + rho * cp + T * rho * dcp);
-}
-
-// Work-group minimum to pmin[wg] (as k_dt's per-tile minimum).
-// NT: threads of the work-group (a multiple of 64)
-template <int NT = TILE_T>
-__device__ __forceinline__ void block_min(double v, double* pmin, int wg) {
-  __shared__ double sm[NT / 64];
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = sm[0];
-    for (int q = 1; q < NT / 64; q++) s = fmin(s, sm[q]);
-    pmin[wg] = s;
-  }
-}
-
-template <int NT = TILE_T>
-__device__ __forceinline__ void block_sum(double v, double* partials, int wg) {
-  __shared__ double sm[NT / 64];
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = sm[0];
-    for (int q = 1; q < NT / 64; q++) s += sm[q];
-    partials[wg] = s;
-  }
-}
-
-// IRSMODE: write the scaled residual to Rres (smoothing follows);
-// otherwise apply the RK update to Tout.
-template <int NDIR, bool TANG, bool PRIM, bool IRSMODE, typename R>
-__global__ __launch_bounds__(TILE_T) void k_stage(StageArgs<R> a) {
-  if (a.halt && *a.halt) return;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Tile t = a.tiles[wg];
-  const BlockInfo B = a.blk[t.b];
-  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
-  double rsq = 0.0;
-  if (i <= B.ni && j <= B.nj) {
-    const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
-    R Rv = 0;
-    {
-#pragma clang fp contract(off)
-      // Fluxes_blk ordering (Mod_Fluxes.f90:54-192): i faces, then j, then k;
-      // Res2 += F for the face on the min side, Res1 -= F on the max side.
-      if (i > 1) Rv = Rv + face_flux<0, NDIR, TANG>(a, B, p - 1);
-      if (i < B.ni) Rv = Rv - face_flux<0, NDIR, TANG>(a, B, p);
-      if (j > 1) Rv = Rv + face_flux<1, NDIR, TANG>(a, B, p - B.sy);
-      if (j < B.nj) Rv = Rv - face_flux<1, NDIR, TANG>(a, B, p);
-      if (NDIR == 3) {
-        if (k > 1) Rv = Rv + face_flux<2, NDIR, TANG>(a, B, p - B.sz);
-        if (k < B.nk) Rv = Rv - face_flux<2, NDIR, TANG>(a, B, p);
-      }
-    }
-    Rv = apply_bc_faces(a, B, i, j, k, Rv);
-    const R T = a.T[p];
-    const int m = a.mat[p];
-    const R dt = a.use_dtg ? (R)(*a.dtg) : a.dt[p];
-    int bad = 0;
-    const R res = scaled_residual<PRIM>(a.tab, m, T, Rv, a.vol[p], a.qvol[p], dt, &bad);
-    if (IRSMODE) {
-      a.Rres[p] = res;
-    } else {
-      const R TO = a.TO[p];
-      const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, a.jam, &bad);
-      a.Tout[p] = Tn;
-      // Check_And_Fix_State, Lib_Newstate.f90:278-290
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
-      const double d = (double)Tn - (double)TO;
-      rsq = d * d;
-    }
-    if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
-  }
-  if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
-}
-
-// The first half of k_stage for a step with a host hook between BC_Fluxes
-// and RK_Newstate (External_Function, Mod_Explicit.f90:53-57): the raw
-// residual R of Fluxes + BC_Fluxes, in k_stage's order, stored to Rres.
-template <int NDIR, bool TANG, typename R>
-__global__ __launch_bounds__(TILE_T) void k_stage_raw(StageArgs<R> a) {
-  if (a.halt && *a.halt) return;
-  const Tile t = a.tiles[blockIdx.x];
-  const BlockInfo B = a.blk[t.b];
-  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
-  if (i > B.ni || j > B.nj) return;
-  const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
-  R Rv = 0;
-  {
-#pragma clang fp contract(off)
-    if (i > 1) Rv = Rv + face_flux<0, NDIR, TANG>(a, B, p - 1);
-    if (i < B.ni) Rv = Rv - face_flux<0, NDIR, TANG>(a, B, p);
-    if (j > 1) Rv = Rv + face_flux<1, NDIR, TANG>(a, B, p - B.sy);
-    if (j < B.nj) Rv = Rv - face_flux<1, NDIR, TANG>(a, B, p);
-    if (NDIR == 3) {
-      if (k > 1) Rv = Rv + face_flux<2, NDIR, TANG>(a, B, p - B.sz);
-      if (k < B.nk) Rv = Rv - face_flux<2, NDIR, TANG>(a, B, p);
-    }
-  }
-  a.Rres[p] = apply_bc_faces(a, B, i, j, k, Rv);
-}
-
-// The second half: RK_Newstate from the (hook-modified) R in Rres, as k_stage
-// continues (IRSMODE: the scaled residual replaces R for the smoothing passes).
-template <bool PRIM, bool IRSMODE, typename R>
-__global__ __launch_bounds__(TILE_T) void k_newstate(StageArgs<R> a) {
-  if (a.halt && *a.halt) return;
-  const int wg = blockIdx.x;
-  const Tile t = a.tiles[wg];
-  const BlockInfo B = a.blk[t.b];
-  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
-  double rsq = 0.0;
-  if (i <= B.ni && j <= B.nj) {
-    const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
-    const R Rv = a.Rres[p];
-    const R T = a.T[p];
-    const int m = a.mat[p];
-    const R dt = a.use_dtg ? (R)(*a.dtg) : a.dt[p];
-    int bad = 0;
-    const R res = scaled_residual<PRIM>(a.tab, m, T, Rv, a.vol[p], a.qvol[p], dt, &bad);
-    if (IRSMODE) {
-      a.Rres[p] = res;
-    } else {
-      const R TO = a.TO[p];
-      const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, a.jam, &bad);
-      a.Tout[p] = Tn;
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
-      const double d = (double)Tn - (double)TO;
-      rsq = d * d;
-    }
-    if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
-  }
-  if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
-}
-
-
-// ------------------------------------------------------------------------
-// 2.5-D stage kernel for 3-D levels: a work-group owns a 64x4 column of
-// cells over KC consecutive k-planes and marches up in k.  Three planes of T
-// (k-1, k, k+1, each with a one-cell halo in i and j) rotate through LDS;
-// every face flux is evaluated once: +x / +y fluxes go through LDS to the
-// neighbouring cell, the +z flux is carried in a register to the next plane.
-// The residual accumulation keeps the Fluxes_blk order (i, then j, then k
-// faces; Mod_Fluxes.f90:54-192).
-// ------------------------------------------------------------------------
-constexpr int KC_DEFAULT = 8;   // column height; fuss_autotune may choose 4, 16, 32 or 64
-constexpr int PX = TILE_X + 2, PY = TILE_Y + 2;
-
-template <typename R>
-__device__ __forceinline__ void load_plane(R (*dst)[PX], const R* __restrict__ T, const BlockInfo& B, int i0,
-                                           int j0, int kk) {
-  const long base = B.base + (long)(kk - 1) * B.sz;
-  for (int q = threadIdx.x; q < PX * PY; q += TILE_T) {
-    const int ii = q % PX, jj = q / PX;
-    const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
-    R v = 0;
-    if (gi <= B.ni + 1 && gj <= B.nj + 1) v = T[base + (gi - 1) + (long)(gj - 1) * B.sy];
-    dst[jj][ii] = v;
-  }
-}
-
-// A plane with halo has PX*PY = 396 values: thread t holds elements t and
-// t + TILE_T (when < 396) in registers between the global load and the LDS
-// store, so the load of plane k+2 overlaps the flux work of plane k.
-template <typename R>
-struct PlaneRegs {
-  R v0, v1;
-};
-template <typename R>
-__device__ __forceinline__ void fetch_plane(PlaneRegs<R>& r, const R* __restrict__ T, const BlockInfo& B, int i0,
-                                            int j0, int kk) {
-  // arena index of the halo corner (i0-1, j0-1, kk); all offsets stay 32-bit
-  const unsigned corner = (unsigned)(B.base + (i0 - 2) + (long)(j0 - 2) * B.sy + (long)(kk - 1) * B.sz);
-  {
-    const int q = threadIdx.x, ii = q % PX, jj = q / PX;
-    r.v0 = (i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1) ? ldu(T, corner + ii + jj * B.sy) : (R)0;
-  }
-  {
-    const int q = threadIdx.x + TILE_T, ii = q % PX, jj = q / PX;
-    r.v1 = (q < PX * PY && i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1) ? ldu(T, corner + ii + jj * B.sy)
-                                                                                : (R)0;
-  }
-}
-// TY: rows of the tile (planes of PX x (TY+2) values; 64*TY threads, two
-// values each)
-template <int TY = TILE_Y, typename R>
-__device__ __forceinline__ void store_plane(R (*dst)[PX], const PlaneRegs<R>& r) {
-  constexpr int TT = TILE_X * TY;
-  (&dst[0][0])[threadIdx.x] = r.v0;
-  if (threadIdx.x + TT < PX * (TY + 2)) (&dst[0][0])[threadIdx.x + TT] = r.v1;
-}
-
-// kappa row of the left cell of a face: blocks of one material skip the
-// per-cell material load
-template <typename R>
-__device__ __forceinline__ const R* kap_row(const StageArgs<R>& a, const BlockInfo& B, unsigned q) {
-  const int m = B.umat >= 0 ? B.umat : ldu(a.mat, q);
-  return a.tab.kap2 + 2 * (long)m * a.tab.nT;
-}
-
-// Face fluxes from the LDS planes (m = k-1, c = k, p = k+1); (x, y) are the
-// LDS coordinates of the left cell, q its arena index.  Same algebra as
-// face_flux (Lib_Diffusive.f90:10-45 regrouped per computational direction).
-template <bool TANG, typename R>
-__device__ __forceinline__ R flux_x(const StageArgs<R>& a, const BlockInfo& B, R (*m)[PX], R (*c)[PX], R (*p)[PX],
-                                    int x, int y, unsigned q, R wn) {
-  const R Tl = c[y][x], Tr = c[y][x + 1];
-  int bad = 0;
-  const R kap = tab_interp2(kap_row(a, B, q), (R)0.5 * (Tl + Tr), a.tab.Ti1, a.tab.Ti2, &bad);
-  if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
-  const R G0 = Tr - Tl;
-  if (!TANG) return kap * (G0 * wn);
-  const R G1 = (c[y + 1][x] - c[y - 1][x] + c[y + 1][x + 1] - c[y - 1][x + 1]) * (R)0.25;
-  const R G2 = (p[y][x] - m[y][x] + p[y][x + 1] - m[y][x + 1]) * (R)0.25;
-  return kap * ((G0 * ldu(a.W[0], q) + G1 * ldu(a.W[1], q)) + G2 * ldu(a.W[2], q));
-}
-
-template <bool TANG, typename R>
-__device__ __forceinline__ R flux_y(const StageArgs<R>& a, const BlockInfo& B, R (*m)[PX], R (*c)[PX], R (*p)[PX],
-                                    int x, int y, unsigned q, R wn) {
-  const R Tl = c[y][x], Tr = c[y + 1][x];
-  int bad = 0;
-  const R kap = tab_interp2(kap_row(a, B, q), (R)0.5 * (Tl + Tr), a.tab.Ti1, a.tab.Ti2, &bad);
-  if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
-  const R G1 = Tr - Tl;
-  if (!TANG) return kap * (G1 * wn);
-  const R G0 = (c[y][x + 1] - c[y][x - 1] + c[y + 1][x + 1] - c[y + 1][x - 1]) * (R)0.25;
-  const R G2 = (p[y][x] - m[y][x] + p[y + 1][x] - m[y + 1][x]) * (R)0.25;
-  return kap * ((G0 * ldu(a.W[3], q) + G1 * ldu(a.W[4], q)) + G2 * ldu(a.W[5], q));
-}
-
-template <bool TANG, typename R>
-__device__ __forceinline__ R flux_z(const StageArgs<R>& a, const BlockInfo& B, R (*c)[PX], R (*p)[PX], int x, int y,
-                                    unsigned q, R wn) {
-  const R Tl = c[y][x], Tr = p[y][x];
-  int bad = 0;
-  const R kap = tab_interp2(kap_row(a, B, q), (R)0.5 * (Tl + Tr), a.tab.Ti1, a.tab.Ti2, &bad);
-  if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
-  const R G2 = Tr - Tl;
-  if (!TANG) return kap * (G2 * wn);
-  const R G0 = (c[y][x + 1] - c[y][x - 1] + p[y][x + 1] - p[y][x - 1]) * (R)0.25;
-  const R G1 = (c[y + 1][x] - c[y - 1][x] + p[y + 1][x] - p[y - 1][x]) * (R)0.25;
-  return kap * ((G0 * ldu(a.W[6], q) + G1 * ldu(a.W[7], q)) + G2 * ldu(a.W[8], q));
-}
-
-template <bool TANG, bool PRIM, bool IRSMODE, typename R, int KCT>
-__global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
-  if (a.halt && *a.halt) return;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Tile t = a.ctiles[wg];
-  const BlockInfo B = a.blk[t.b];
-  const int tx = threadIdx.x % TILE_X, ty = threadIdx.x / TILE_X;
-  const int i = t.i0 + tx, j = t.j0 + ty;
-  const int k0 = t.k0, k1 = min(B.nk, k0 + KCT - 1);
-  const bool live = i <= B.ni && j <= B.nj;
-  const int x = tx + 1, y = ty + 1;  // LDS coordinates of the thread's cell
-  // min-side halo faces of the tile: wave 0 computes the y-halo (one face per
-  // lane), wave 1 lanes 0..TILE_Y-1 the x-halo of rows 0..TILE_Y-1
-  const bool yhalo = ty == 0 && t.j0 > 1 && i <= B.ni;
-  const int hr = tx;  // x-halo row handled by this lane of wave 1
-  const bool xhalo = ty == 1 && tx < TILE_Y && t.i0 > 1 && t.j0 + hr <= B.nj;
-  __shared__ R ring[3][PY][PX];
-  __shared__ R fx[TILE_Y][TILE_X + 1];
-  __shared__ R fy[TILE_Y + 1][TILE_X];
-  int sm = 0, sc = 1, sp = 2;
-  PlaneRegs<R> pre;
-  fetch_plane(pre, a.T, B, t.i0, t.j0, k0 - 1);
-  store_plane(ring[sm], pre);
-  fetch_plane(pre, a.T, B, t.i0, t.j0, k0);
-  store_plane(ring[sc], pre);
-  fetch_plane(pre, a.T, B, t.i0, t.j0, k0 + 1);
-  __syncthreads();
-  const unsigned sy = (unsigned)B.sy, sz = (unsigned)B.sz;
-  unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k0 - 1) * B.sz);
-  // arena index of the x-halo cell (i0-1, j0+hr, k) of wave 1
-  unsigned ph = (unsigned)(B.base + (t.i0 - 2) + (long)(t.j0 + hr - 1) * B.sy + (long)(k0 - 1) * B.sz);
-  R fz_prev = 0;
-  if (live && k0 > 1) {
-#pragma clang fp contract(off)
-    fz_prev = flux_z<TANG>(a, B, ring[sm], ring[sc], x, y, p - sz, TANG ? (R)0 : ldu(a.W[8], p - sz));
-  }
-  // normal face weights of the thread's cell, fetched one plane ahead
-  // (orthogonal blocks read only these three weights)
-  R w0 = 0, w4 = 0, w8 = 0;
-  if (!TANG && live) {
-    w0 = ldu(a.W[0], p);
-    w4 = ldu(a.W[4], p);
-    w8 = ldu(a.W[8], p);
-  }
-  double rsq = 0.0;
-  for (int k = k0; k <= k1; k++, p += sz, ph += sz) {
-    store_plane(ring[sp], pre);
-    __syncthreads();
-    if (k < k1) fetch_plane(pre, a.T, B, t.i0, t.j0, k + 2);
-    // per-cell streams of this plane, issued before the flux work so that
-    // their latency overlaps it
-    R c_vol = 0, c_qv = 0, c_dt = 0, c_TO = 0;
-    int c_m = B.umat;
-    if (live) {
-      c_vol = ldu(a.vol, p);
-      c_qv = ldu(a.qvol, p);
-      if (!a.use_dtg) c_dt = ldu(a.dt, p);
-      if (!IRSMODE) c_TO = ldu(a.TO, p);
-      if (c_m < 0) c_m = ldu(a.mat, p);
-    }
-    R n0 = 0, n4 = 0, n8 = 0;
-    if (!TANG && live && k < k1) {
-      n0 = ldu(a.W[0], p + sz);
-      n4 = ldu(a.W[4], p + sz);
-      n8 = ldu(a.W[8], p + sz);
-    }
-    R fz = 0;
-    {
-#pragma clang fp contract(off)
-      if (live) {
-        if (i < B.ni) fx[ty][tx + 1] = flux_x<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, y, p, w0);
-        if (j < B.nj) fy[ty + 1][tx] = flux_y<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, y, p, w4);
-        if (k < B.nk) fz = flux_z<TANG>(a, B, ring[sc], ring[sp], x, y, p, w8);
-      }
-      if (yhalo)
-        fy[0][tx] = flux_y<TANG>(a, B, ring[sm], ring[sc], ring[sp], x, 0, p - sy, TANG ? (R)0 : ldu(a.W[4], p - sy));
-      if (xhalo)
-        fx[hr][0] = flux_x<TANG>(a, B, ring[sm], ring[sc], ring[sp], 0, hr + 1, ph, TANG ? (R)0 : ldu(a.W[0], ph));
-    }
-    w0 = n0;
-    w4 = n4;
-    w8 = n8;
-    __syncthreads();
-    if (live) {
-      R Rv = 0;
-      {
-#pragma clang fp contract(off)
-        if (i > 1) Rv = Rv + fx[ty][tx];
-        if (i < B.ni) Rv = Rv - fx[ty][tx + 1];
-        if (j > 1) Rv = Rv + fy[ty][tx];
-        if (j < B.nj) Rv = Rv - fy[ty + 1][tx];
-        if (k > 1) Rv = Rv + fz_prev;
-        if (k < B.nk) Rv = Rv - fz;
-      }
-      Rv = apply_bc_faces(a, B, i, j, k, Rv);
-      const R T = ring[sc][y][x];
-      const int m = c_m;
-      const R dt = a.use_dtg ? (R)(*a.dtg) : c_dt;
-      int bad = 0;
-      const R res = scaled_residual<PRIM>(a.tab, m, T, Rv, c_vol, c_qv, dt, &bad);
-      if (IRSMODE) {
-        stu(a.Rres, p, res);
-      } else {
-        const R TO = c_TO;
-        const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, a.jam, &bad);
-        stu(a.Tout, p, Tn);
-        if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
-        const double d = (double)Tn - (double)TO;
-        rsq += d * d;
-      }
-      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
-    }
-    fz_prev = fz;
-    const int s0 = sm;
-    sm = sc;
-    sc = sp;
-    sp = s0;
-  }
-  if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
-}
-
-// ------------------------------------------------------------------------
-// Software-pipelined 2.5-D stage kernel for orthogonal 3-D blocks (the
-// normal face weights only).  Per plane the table rows of the plane's faces
-// and cell are requested first (their addresses come from T in LDS), then
-// the loads of the next plane (T of plane k+2 and the per-cell streams of
-// k+1), then the flux arithmetic waits for the table rows only: loads retire
-// in issue order, so the next plane's streams stay in flight across the
-// whole plane.  Every load in the loop is unconditional (out-of-block lanes
-// and the last plane read valid cells they then ignore), so the compiler's
-// wait counts can leave the later loads outstanding.  UM: every block of the
-// level has one material (uniform row base, no material stream).
-// Same operations and order as k_stage3d<false, ...>.
-// ------------------------------------------------------------------------
-// Row of T in a 1 K table (co_Kappa etc., Lib_Solid.f90:19-23: Tm = int(T)).
-// Outside [Ti1, Ti2) *bad is set (the step then fails with FUSS_ETABLE) and
-// T is clamped to [Ti1, Ti2 - 1] before the conversion, so the row stays
-// inside the table and the conversion never sees a NaN or an out-of-range
-// value (fmax of a NaN returns the bound).  With Ti1 >= 0, int(T) >= Ti2
-// exactly when T >= Ti2.
-template <typename R>
-__device__ __forceinline__ int tab_row(R T, int Ti1, int Ti2, int* bad) {
-  if (!(T >= (R)Ti1 && T < (R)Ti2)) *bad = 1;
-  return (int)fmin(fmax(T, (R)Ti1), (R)(Ti2 - 1));
-}
-
-template <typename R>
-struct StageStream {
-  R vol, qv, dt, TO;   // per-cell streams
-  R w0, w4, w8;        // normal weights of the cell's +x, +y, +z faces
-  R wh;                // halo lanes: weight of the tile's min-side face
-  int m, mh;           // material of the cell / of the halo face's left cell
-};
-
-// fetch_plane with every load issued: halo cells beyond the ghost layer read
-// the tile's corner cell instead (a ghost or interior cell of the block); no
-// flux of an in-block face uses them
-template <int TY, typename R>
-__device__ __forceinline__ void fetch_plane_u(PlaneRegs<R>& r, const R* __restrict__ T, const BlockInfo& B, int i0,
-                                              int j0, int kk) {
-  constexpr int TT = TILE_X * TY;
-  static_assert(PX * (TY + 2) <= 2 * TT, "two values per thread");
-  const unsigned corner = (unsigned)(B.base + (i0 - 2) + (long)(j0 - 2) * B.sy + (long)(kk - 1) * B.sz);
-  {
-    const int q = threadIdx.x, ii = q % PX, jj = q / PX;
-    const bool in = i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1;
-    r.v0 = ldu(T, in ? corner + ii + jj * (unsigned)B.sy : corner);
-  }
-  {
-    const int q = threadIdx.x + TT, ii = q % PX, jj = q / PX;
-    const bool in = q < PX * (TY + 2) && i0 - 1 + ii <= B.ni + 1 && j0 - 1 + jj <= B.nj + 1;
-    r.v1 = ldu(T, in ? corner + ii + jj * (unsigned)B.sy : corner);
-  }
-}
-
-// TY: rows of the column (4 or 8, one wave per row; the 8-row column
-// evaluates 9 instead of 10 y faces and loads 10 instead of 12 T rows per 8
-// rows of cells)
-template <bool UM, bool PRIM, bool IRSMODE, bool RECT, bool DTC, typename R, int KCT, int TY>
-__global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
-  constexpr int TT = TILE_X * TY;
-  if (a.halt && *a.halt) return;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Tile t = a.ctiles[wg];
-  const BlockInfo B = a.blk[t.b];
-  const int tx = threadIdx.x % TILE_X, ty = threadIdx.x / TILE_X;
-  const int i = t.i0 + tx, j = t.j0 + ty;
-  const int k0 = t.k0, k1 = min(B.nk, k0 + KCT - 1);
-  const bool live = i <= B.ni && j <= B.nj;
-  const int x = tx + 1, y = ty + 1;
-  const bool yhalo = ty == 0 && t.j0 > 1 && i <= B.ni;
-  const int hr = tx;
-  const bool xhalo = ty == 1 && tx < TY && t.i0 > 1 && t.j0 + hr <= B.nj;
-  const bool hx = i < B.ni, hy = j < B.nj;  // the cell's +x / +y face is interior
-  const int hq = xhalo ? hr + 1 : 1;        // LDS row of the x-halo face (in range for every lane)
-  __shared__ R ring[3][TY + 2][PX];
-  __shared__ R fx[TY][TILE_X + 1];
-  __shared__ R fy[TY + 1][TILE_X];
-  using V2 = typename Vec2<R>::T;
-  using V4 = typename Vec4<R>::T;
-  const int Ti1 = a.tab.Ti1, Ti2 = a.tab.Ti2, nT = a.tab.nT;
-  const int mu = B.umat >= 0 ? B.umat : 0;
-  // first row of a material's table (row of T = Ti1)
-  const V2* kap_u = reinterpret_cast<const V2*>(a.tab.kap2) + (long)mu * nT;
-  const V4* pr_u = reinterpret_cast<const V4*>(a.tab.pr4) + (long)mu * nT;
-  const V2* kap0 = reinterpret_cast<const V2*>(a.tab.kap2);
-  const V4* pr0 = reinterpret_cast<const V4*>(a.tab.pr4);
-  // the halo face's weight array: +y weights for the y-halo row, +x for the x-halo
-  const R* __restrict__ Wh = yhalo ? a.W[4] : a.W[0];
-  // RECT (every block rectilinear): volume and normal weights are products of
-  // per-axis factors, the lane's (i, j) part formed once, the column's k
-  // parts staged in LDS (a global load inside the plane loop would make the
-  // in-order wait for it also wait for the next plane's prefetch)
-  R cvol = 0, cw0 = 0, cw4 = 0, cwh = 0;
-  // a_z, f_z (and 1/dl_z^2 for DTC) of planes k0-1 .. k0+KCT-1
-  __shared__ double zfac[DTC ? 3 : 2][KCT + 1];
-  double qxy = 0.0;  // DTC: 1/dl_x^2 + 1/dl_y^2 of the lane's column
-  if (RECT) {
-    const RectAxes rx_ = rect_axes(a.rect, B);
-    if (threadIdx.x < (DTC ? 3 : 2) * (KCT + 1)) {
-      const int q = threadIdx.x % (KCT + 1), kk = min(k0 - 1 + q, B.nk + 1), row = threadIdx.x / (KCT + 1);
-      zfac[row][q] = row == 0 ? rx_.az[kk] : (row == 1 ? rx_.fz[kk] : rx_.qz[kk]);
-    }
-    const int ci_ = live ? i : 1, cj_ = live ? j : 1;
-    cvol = (R)(rx_.ax[ci_] * rx_.ay[cj_]);
-    cw0 = (R)(rx_.fx[ci_] * rx_.ay[cj_]);
-    if (DTC) qxy = rx_.qx[ci_] + rx_.qy[cj_];
-    cw4 = (R)(rx_.fy[cj_] * rx_.ax[ci_]);
-    // the halo face: +y face of (i, j0-1) or +x face of (i0-1, j0+hr)
-    cwh = yhalo ? (R)(rx_.fy[t.j0 - 1] * rx_.ax[i]) : (xhalo ? (R)(rx_.fx[t.i0 - 1] * rx_.ay[t.j0 + hr]) : (R)0);
-  }
-  int sm = 0, sc = 1, sp = 2;
-  PlaneRegs<R> pre;
-  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0 - 1);
-  store_plane<TY>(ring[sm], pre);
-  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0);
-  store_plane<TY>(ring[sc], pre);
-  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0 + 1);
-  const unsigned sz = (unsigned)B.sz;
-  // addresses of plane k0: the cell (cell (1,1) for out-of-block lanes) and the
-  // halo face's left cell (the cell itself for other lanes)
-  const int ci = live ? i : 1, cj = live ? j : 1;
-  unsigned p = (unsigned)(B.base + (ci - 1) + (long)(cj - 1) * B.sy + (long)(k0 - 1) * B.sz);
-  unsigned ph = yhalo ? p - (unsigned)B.sy
-                      : (xhalo ? (unsigned)(B.base + (t.i0 - 2) + (long)(t.j0 + hr - 1) * B.sy + (long)(k0 - 1) * B.sz)
-                               : p);
-  // face-map positions of the lane's boundary faces (BC values in face
-  // order; at most one face per axis since every block dimension is >= 2)
-  const int fa = live ? B.face_active : 0;
-  const int xo = (fa & 1) && i == 1 ? B.fmap_off[0] + (j - 1) : ((fa & 2) && i == B.ni ? B.fmap_off[1] + (j - 1) : -1);
-  const int yo = (fa & 4) && j == 1 ? B.fmap_off[2] + (i - 1) : ((fa & 8) && j == B.nj ? B.fmap_off[3] + (i - 1) : -1);
-  const int zlo = (fa & 16) ? B.fmap_off[4] + (i - 1) + B.ni * (j - 1) : -1;
-  const int zhi = (fa & 32) ? B.fmap_off[5] + (i - 1) + B.ni * (j - 1) : -1;
-  auto zo_of = [&](int kk) { return kk == 1 ? zlo : (kk == B.nk ? zhi : -1); };
-  // which loads of boundary values a tile / row needs at all
-  const int fa_t = B.face_active;
-  const bool xtile = (fa_t & 3) && (t.i0 == 1 || t.i0 + TILE_X - 1 >= B.ni);
-  const bool yrow = (fa_t & 12) && (j == 1 || j == B.nj);
-  const R dtg = a.use_dtg ? (R)(*a.dtg) : (R)0;
-  // a block of uniform qvol: every lane reads the value of cell (1,1,1) (one
-  // cached line instead of a stream)
-  const unsigned quni = (unsigned)B.base;
-  auto stream = [&](StageStream<R>& s, unsigned pp, unsigned hh) {
-    s.qv = ldu(a.qvol, B.uqv ? quni : pp);
-    if (!DTC) s.dt = ldn(a.dt, pp);
-    if (!IRSMODE) s.TO = ldn(a.TO, pp);
-    if (!RECT) {
-      s.vol = ldn(a.vol, pp);
-      s.w0 = ldn(a.W[0], pp);
-      s.w4 = ldn(a.W[4], pp);
-      s.w8 = ldn(a.W[8], pp);
-      s.wh = ldu(Wh, hh);
-    }
-    if (!UM) {
-      s.m = ldu(a.mat, pp);
-      s.mh = ldu(a.mat, hh);
-    }
-  };
-  StageStream<R> sa_, sb_;
-  sa_.m = sa_.mh = sb_.m = sb_.mh = mu;
-  sa_.TO = sb_.TO = (R)0;
-  if (DTC) sa_.dt = sb_.dt = (R)0;
-  if (RECT) sa_.vol = sb_.vol = sa_.w0 = sb_.w0 = sa_.w4 = sb_.w4 = sa_.w8 = sb_.w8 = sa_.wh = sb_.wh = (R)0;
-  stream(sa_, p, ph);
-  __syncthreads();
-  R fz_prev = 0;
-  if (live && k0 > 1) {
-#pragma clang fp contract(off)
-    fz_prev = flux_z<false>(a, B, ring[sm], ring[sc], x, y, p - sz,
-                            RECT ? (R)(zfac[1][0] * (double)cvol) : ldu(a.W[8], p - sz));
-  }
-  double rsq = 0.0;
-  // The result of a plane is stored during the next plane, after that
-  // plane's loads are issued: stores count in the same in-order counter as
-  // loads, so a store issued just before the table loads would make the
-  // wait for the tables wait for the store's completion too.
-  R* const out = IRSMODE ? a.Rres : a.Tout;
-  R vpend = 0, dtpend = 0;
-  unsigned ppend = p;
-  bool spend = false;
-  double dtmin = 1e300;
-  // one plane: cur holds its streams, nxt receives those of the next plane
-  auto plane = [&](StageStream<R>& cur, StageStream<R>& nxt, int k) {
-    store_plane<TY>(ring[sp], pre);
-    __syncthreads();
-    // phase A: table rows of the plane's faces and of the cell (loads always
-    // issued, at a clamped row; errors counted for existing faces only)
-    int bad = 0, badc = 0;
-    const R Tc = ring[sc][y][x];
-    const R Txr = ring[sc][y][x + 1], Tyr = ring[sc][y + 1][x], Tzr = ring[sp][y][x];
-    R Tx, Ty, Tz, Th = 0, Tcl = Tc;
-    int rx, ry, rz, rh = Ti1, rc = Ti1;
-    V2 kx, ky, kz, kh{}, kc{};
-    V4 pc{};
-    // BC values of the lane's boundary faces, issued with the table rows;
-    // only tiles on an i face, rows on a j face and the end planes load them
-    R bx = 0, by = 0, bz = 0;
-    if (xtile) bx = ldu(a.bcf, (unsigned)max(xo >= 0 ? xo + B.nj * (k - 1) : 0, 0));
-    if (yrow) by = ldu(a.bcf, (unsigned)max(yo >= 0 ? yo + B.ni * (k - 1) : 0, 0));
-    if (zo_of(k) >= 0 || ((fa_t & 48) && (k == 1 || k == B.nk))) bz = ldu(a.bcf, (unsigned)max(zo_of(k), 0));
-    {
-#pragma clang fp contract(off)
-      // rows addressed by 32-bit offsets from the material's first row
-      const V2* kr = UM ? kap_u : kap0 + (long)cur.m * nT;
-      int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
-      Tx = (R)0.5 * (Tc + Txr);
-      rx = tab_row(Tx, Ti1, Ti2, &b0);
-      kx = ldu(kr, (unsigned)(rx - Ti1));
-      Ty = (R)0.5 * (Tc + Tyr);
-      ry = tab_row(Ty, Ti1, Ti2, &b1);
-      ky = ldu(kr, (unsigned)(ry - Ti1));
-      Tz = (R)0.5 * (Tc + Tzr);
-      rz = tab_row(Tz, Ti1, Ti2, &b2);
-      kz = ldu(kr, (unsigned)(rz - Ti1));
-      if (PRIM) {
-        rc = tab_row(Tcl, Ti1, Ti2, &badc);
-        pc = ldu(UM ? pr_u : pr0 + (long)cur.m * nT, (unsigned)(rc - Ti1));
-        if (DTC) kc = ldu(kr, (unsigned)(rc - Ti1));  // kappa at the cell for its dt
-      }
-      if (ty < 2) {  // waves 0 and 1 hold the halo lanes (a wave-uniform test)
-        const V2* krh = UM ? kap_u : kap0 + (long)cur.mh * nT;
-        Th = yhalo ? (R)0.5 * (ring[sc][0][x] + ring[sc][1][x]) : (R)0.5 * (ring[sc][hq][0] + ring[sc][hq][1]);
-        rh = tab_row(Th, Ti1, Ti2, &b3);
-        kh = ldu(krh, (unsigned)(rh - Ti1));
-      }
-      // primitive variables: every cell's own row is checked (badc), and a face
-      // average lies between its two cells' temperatures, so a face outside
-      // the table implies a cell outside it -- the face tests are redundant
-      bad = PRIM ? 0 : (live && ((hx && b0) || (hy && b1) || (k < B.nk && b2))) || ((yhalo || xhalo) && b3);
-      badc = live && badc;
-    }
-    // phase B: the next plane's loads (the last plane re-reads its own cells)
-    {
-      const unsigned step = k < k1 ? sz : 0u;
-      fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k < k1 ? k + 2 : k + 1);
-      stream(nxt, p + step, ph + step);
-    }
-    if (spend) {  // the previous plane's result
-      stn(out, ppend, vpend);
-      if (DTC) stn(a.dtw, ppend, dtpend);
-    }
-    if (RECT) {
-      // the plane's geometry from the per-axis factors (same products as the
-      // setup check against the stored metrics)
-      const double azk = zfac[0][k - k0 + 1], fzk = zfac[1][k - k0 + 1];
-      cur.vol = (R)((double)cvol * azk);
-      cur.w0 = (R)((double)cw0 * azk);
-      cur.w4 = (R)((double)cw4 * azk);
-      cur.w8 = (R)(fzk * (double)cvol);
-      cur.wh = (R)((double)cwh * azk);
-    }
-    // phase C: fluxes
-    R fz = 0;
-    {
-#pragma clang fp contract(off)
-      if (live) {
-        if (hx) fx[ty][tx + 1] = (kx.x + (Tx - (R)rx) * kx.y) * ((Txr - Tc) * cur.w0);
-        if (hy) fy[ty + 1][tx] = (ky.x + (Ty - (R)ry) * ky.y) * ((Tyr - Tc) * cur.w4);
-        if (k < B.nk) fz = (kz.x + (Tz - (R)rz) * kz.y) * ((Tzr - Tc) * cur.w8);
-      }
-      if (yhalo) fy[0][tx] = (kh.x + (Th - (R)rh) * kh.y) * ((ring[sc][1][x] - ring[sc][0][x]) * cur.wh);
-      if (xhalo) fx[hr][0] = (kh.x + (Th - (R)rh) * kh.y) * ((ring[sc][hq][1] - ring[sc][hq][0]) * cur.wh);
-    }
-    if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
-    __syncthreads();
-    if (live) {
-      R Rv = 0;
-      {
-#pragma clang fp contract(off)
-        if (i > 1) Rv = Rv + fx[ty][tx];
-        if (hx) Rv = Rv - fx[ty][tx + 1];
-        if (j > 1) Rv = Rv + fy[ty][tx];
-        if (hy) Rv = Rv - fy[ty + 1][tx];
-        if (k > 1) Rv = Rv + fz_prev;
-        if (k < B.nk) Rv = Rv - fz;
-      }
-      {
-#pragma clang fp contract(off)
-        // BC_Fluxes contributions in face order (as apply_bc_faces)
-        if (xo >= 0) Rv = Rv - bx;
-        if (yo >= 0) Rv = Rv - by;
-        if (zo_of(k) >= 0) Rv = Rv - bz;
-      }
-      R dt = a.use_dtg ? dtg : cur.dt;
-      if (DTC) {
-        // Compute_dt (Mod_dt.f90:26-84) at T^n, the first stage's input: the
-        // kappa, rho, cp interpolations of k_dt at the cell's row (rho and cp
-        // are the residual's below), sum 1/dl^2 from the per-axis factors
-#pragma clang fp contract(off)
-        const R fr = Tcl - (R)rc;
-        const R kap = kc.x + fr * kc.y, rho = pc.x + fr * pc.y, cp = pc.z + fr * pc.w;
-        const double alpha = (double)kap / (double)rho / (double)cp;
-        double dd = 0.5 * a.vnn / (alpha * (qxy + zfac[2][k - k0 + 1]));
-        dd = dd < 1e8 ? dd : 1e8;
-        dt = (R)dd;
-        dtpend = dt;
-        dtmin = fmin(dtmin, dd);
-      }
-      R res;
-      {
-#pragma clang fp contract(off)
-        if (PRIM) {
-          // Compute_Prim_Residual, Lib_Newstate.f90:241-256 (as scaled_residual)
-          const R rho = pc.x + (Tcl - (R)rc) * pc.y;
-          const R cp = pc.z + (Tcl - (R)rc) * pc.w;
-          res = (-Rv / cur.vol + cur.qv) * dt / (Tc * cp * pc.y + rho * cp + Tc * rho * pc.w);
-        } else {
-          res = (-Rv / cur.vol + cur.qv) * dt;
-        }
-      }
-      int bad2 = badc;
-      if (IRSMODE) {
-        vpend = res;
-      } else {
-        const R Tn = new_state<PRIM>(a.tab, cur.m, Tc, cur.TO, res, a.rkc, a.jam, &bad2);
-        vpend = Tn;
-        if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
-        const double d = (double)Tn - (double)cur.TO;
-        rsq += d * d;
-      }
-      ppend = p;
-      if (bad2) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
-    }
-    spend = live;
-    fz_prev = fz;
-    const int s0 = sm;
-    sm = sc;
-    sc = sp;
-    sp = s0;
-    p += sz;
-    ph += sz;
-  };
-  // two planes per trip with the stream buffers swapping roles (no copy of
-  // in-flight registers)
-  for (int k = k0; k <= k1; k += 2) {
-    plane(sa_, sb_, k);
-    if (k + 1 <= k1) plane(sb_, sa_, k + 1);
-  }
-  if (spend) {
-    stn(out, ppend, vpend);
-    if (DTC) stn(a.dtw, ppend, dtpend);
-  }
-  if (!IRSMODE && a.last) block_sum<TT>(rsq, a.partials, wg);
-  if (DTC) block_min<TT>(dtmin, a.pmin, wg);
-}
-
-// ------------------------------------------------------------------------
-// Barrier-free stage kernel for orthogonal 3-D blocks: each wave owns one
-// 64-cell row (i) of the tile and marches its KC planes in k on its own.
-// i-neighbours come from the neighbouring lanes (shuffles; lane 0 and 63
-// read the tile edges), j-neighbours from memory (the rows of the other
-// waves, cache hits), the +z flux is carried to the next plane in a
-// register.  The flux through a j face is evaluated by both rows that share
-// it, from the same operands, so the residual is bit-identical to k_stage3d's.
-// Everything a plane needs is fetched one plane ahead.
-// ------------------------------------------------------------------------
-template <typename R>
-struct RowPlane {
-  R T, Tjm, Tjp, Tih;   // own T, j-1 and j+1 neighbours, tile-edge halo (lanes 0, 63)
-  R w0, w4, w8, w4m;    // normal weights of the cell's +x, +y, +z faces and of the (j-1) cell's +y face
-  R w0h;                // lane 0: +x weight of cell i0-1
-  R vol, qv, dt, TO;
-  int m, mjm, mih;      // material of the cell, of (j-1), of (i0-1)
-};
-
-template <bool PRIM, bool IRSMODE, typename R>
-__device__ __forceinline__ void row_fetch(RowPlane<R>& r, const StageArgs<R>& a, const BlockInfo& B, unsigned p,
-                                          int lane, bool live, bool lo_halo, bool hi_halo, bool has_jm) {
-  const unsigned sy = (unsigned)B.sy;
-  if (live) {
-    r.T = ldu(a.T, p);
-    r.Tjm = ldu(a.T, p - sy);
-    r.Tjp = ldu(a.T, p + sy);
-    r.w0 = ldu(a.W[0], p);
-    r.w4 = ldu(a.W[4], p);
-    r.w8 = ldu(a.W[8], p);
-    r.w4m = has_jm ? ldu(a.W[4], p - sy) : (R)0;
-    r.vol = ldu(a.vol, p);
-    r.qv = ldu(a.qvol, p);
-    if (!a.use_dtg) r.dt = ldu(a.dt, p);
-    if (!IRSMODE) r.TO = ldu(a.TO, p);
-    if (B.umat < 0) {
-      r.m = ldu(a.mat, p);
-      r.mjm = has_jm ? ldu(a.mat, p - sy) : 0;
-    }
-  }
-  if (lo_halo) {
-    r.Tih = ldu(a.T, p - 1);
-    r.w0h = ldu(a.W[0], p - 1);
-    if (B.umat < 0) r.mih = ldu(a.mat, p - 1);
-  } else if (hi_halo) {
-    r.Tih = ldu(a.T, p + 1);
-  }
-  (void)lane;
-}
-
-template <typename R>
-__device__ __forceinline__ R face_kw(const StageArgs<R>& a, int m, R Tl, R Tr, R w) {
-  int bad = 0;
-  const R kap = tab_interp2(a.tab.kap2 + 2 * (long)m * a.tab.nT, (R)0.5 * (Tl + Tr), a.tab.Ti1, a.tab.Ti2, &bad);
-  if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
-  return kap * ((Tr - Tl) * w);
-}
-
-template <bool PRIM, bool IRSMODE, typename R, int KCT>
-__global__ __launch_bounds__(TILE_T) void k_stage3w(StageArgs<R> a) {
-  if (a.halt && *a.halt) return;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Tile t = a.ctiles[wg];
-  const BlockInfo B = a.blk[t.b];
-  const int lane = threadIdx.x % TILE_X, row = threadIdx.x / TILE_X;
-  const int i = t.i0 + lane, j = t.j0 + row;
-  const int k0 = t.k0, k1 = min(B.nk, k0 + KCT - 1);
-  const bool rowlive = j <= B.nj;
-  const bool live = rowlive && i <= B.ni;
-  const bool has_jm = j > 1;
-  // lane 0 computes the tile's min-side x face; lane 63 reads the cell past the tile
-  const bool lo_halo = rowlive && lane == 0 && t.i0 > 1;
-  const bool hi_halo = rowlive && lane == TILE_X - 1 && i < B.ni;
-  const unsigned sz = (unsigned)B.sz;
-  unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k0 - 1) * B.sz);
-  RowPlane<R> cur, nxt;
-  cur.m = cur.mjm = cur.mih = B.umat;
-  nxt.m = nxt.mjm = nxt.mih = B.umat;
-  cur.dt = nxt.dt = (R)0;
-  cur.TO = nxt.TO = (R)0;
-  cur.Tih = nxt.Tih = (R)0;
-  cur.w0h = nxt.w0h = (R)0;
-  row_fetch<PRIM, IRSMODE>(cur, a, B, p, lane, live, lo_halo, hi_halo, has_jm);
-  R Tp = live && k0 < B.nk ? ldu(a.T, p + sz) : (R)0;
-  R fz_prev = 0;
-  if (live && k0 > 1) {
-#pragma clang fp contract(off)
-    const R Tm = ldu(a.T, p - sz);
-    const int mm = B.umat >= 0 ? B.umat : ldu(a.mat, p - sz);
-    fz_prev = face_kw(a, mm, Tm, cur.T, ldu(a.W[8], p - sz));
-  }
-  double rsq = 0.0;
-  for (int k = k0; k <= k1; k++, p += sz) {
-    // next plane's data (and T two planes up for the +z face of plane k+1)
-    R Tp2 = 0;
-    if (k < k1) {
-      row_fetch<PRIM, IRSMODE>(nxt, a, B, p + sz, lane, live, lo_halo, hi_halo, has_jm);
-      if (live && k + 1 < B.nk) Tp2 = ldu(a.T, p + 2 * sz);
-    }
-    // i neighbours from the neighbouring lanes
-    R Tl = __shfl_up(cur.T, 1, 64), Tr = __shfl_down(cur.T, 1, 64);
-    if (lane == 0) Tl = cur.Tih;
-    if (lane == TILE_X - 1) Tr = cur.Tih;
-    R fxp = 0, fyp = 0, fym = 0, fz = 0, fxm_own = 0;
-    {
-#pragma clang fp contract(off)
-      if (live) {
-        const int m = cur.m;
-        if (i < B.ni) fxp = face_kw(a, m, cur.T, Tr, cur.w0);
-        if (j < B.nj) fyp = face_kw(a, m, cur.T, cur.Tjp, cur.w4);
-        if (j > 1) fym = face_kw(a, cur.mjm, cur.Tjm, cur.T, cur.w4m);
-        if (k < B.nk) fz = face_kw(a, m, cur.T, Tp, cur.w8);
-      }
-      if (lo_halo) fxm_own = face_kw(a, cur.mih, Tl, cur.T, cur.w0h);
-    }
-    R fxm = __shfl_up(fxp, 1, 64);
-    if (lane == 0) fxm = fxm_own;
-    if (live) {
-      R Rv = 0;
-      {
-#pragma clang fp contract(off)
-        if (i > 1) Rv = Rv + fxm;
-        if (i < B.ni) Rv = Rv - fxp;
-        if (j > 1) Rv = Rv + fym;
-        if (j < B.nj) Rv = Rv - fyp;
-        if (k > 1) Rv = Rv + fz_prev;
-        if (k < B.nk) Rv = Rv - fz;
-      }
-      Rv = apply_bc_faces(a, B, i, j, k, Rv);
-      const R dt = a.use_dtg ? (R)(*a.dtg) : cur.dt;
-      int bad = 0;
-      const R res = scaled_residual<PRIM>(a.tab, cur.m, cur.T, Rv, cur.vol, cur.qv, dt, &bad);
-      if (IRSMODE) {
-        stu(a.Rres, p, res);
-      } else {
-        const R Tn = new_state<PRIM>(a.tab, cur.m, cur.T, cur.TO, res, a.rkc, a.jam, &bad);
-        stu(a.Tout, p, Tn);
-        if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
-        const double d = (double)Tn - (double)cur.TO;
-        rsq += d * d;
-      }
-      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
-    }
-    fz_prev = fz;
-    cur = nxt;
-    Tp = Tp2;
-  }
-  if (!IRSMODE && a.last) block_sum(rsq, a.partials, wg);
-}
-
-// ------------------------------------------------------------------------
-// kernels: implicit residual smoothing
-// ------------------------------------------------------------------------
-template <typename R>
-struct IrsArgs {
-  const BlockInfo* blk;
-  const Tile* tiles;
-  const R* Rin;
-  R* Rout;
-  const BCRec* bc;
-  const int* fmap;
-  R beta, den;         // den = 1 + 2 beta
-  R inv;               // 1/den when den is a power of two (x/den == x*inv exactly), else 0
-  // fused update (last direction)
-  const R* T;
-  const R* TO;
-  R* Tout;
-  const int* mat;
-  Tab<R> tab;
-  R rkc;
-  int jam;
-  int last;
-  double* partials;
-  int* status;
-  const int* halt;
-  // k_irs_xyz on the last stage: the next step's local dt from the new state
-  // (Compute_dt of the next Explicit_Step, as k_dt computes it)
-  int dtnext;
-  double vnn;
-  const double* invL2;
-  const double* rect;  // non-null: sum 1/dl^2 from the per-axis factors instead of invL2
-  R* dt;
-  double* pmin;
-};
-
-// x / (1 + 2 beta): a multiplication by the exact reciprocal when the
-// divisor is a power of two (beta = 0.5: den = 2), which gives the same
-// IEEE result as the division; else the division.
-template <typename R>
-__device__ __forceinline__ R irs_div(R x, R den, R inv) {
-  return inv != (R)0 ? x * inv : x / den;
-}
-
-// Residual_Smoothing_Blk along DIR (Lib_IRS.f90:43-134): two Jacobi sweeps
-// r*_new(s) = (r(s) + beta (r*(s-1) + r*(s+1))) / (1 + 2 beta), r* = r
-// initially, ghosts from Ghost_Residual_Extrapolation (:137-187).  The
-// second sweep needs r(s-2..s+2) only, so each cell is computed directly.
-template <int DIR, int NDIR, bool UPDATE, bool PRIM, typename R>
-__global__ __launch_bounds__(TILE_T) void k_irs(IrsArgs<R> a) {
-  if (a.halt && *a.halt) return;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Tile t = a.tiles[wg];
-  const BlockInfo B = a.blk[t.b];
-  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
-  double rsq = 0.0;
-  if (i <= B.ni && j <= B.nj) {
-#pragma clang fp contract(off)
-    const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
-    const long st = DIR == 0 ? 1 : (DIR == 1 ? (long)B.sy : B.sz);
-    const int s = DIR == 0 ? i : (DIR == 1 ? j : k);
-    const int n = DIR == 0 ? B.ni : (DIR == 1 ? B.nj : B.nk);
-    const R* __restrict__ Rin = a.Rin;
-    R glo = 0, ghi = 0;
-    if (s <= 2) {
-      const int f = 2 * DIR + 1;
-      const int i1 = DIR == 0 ? 1 : i, j1 = DIR == 1 ? 1 : j, k1 = DIR == 2 ? 1 : k;
-      const BCRec& e = a.bc[a.fmap[face_map_index<R>(B, f, i1, j1, k1)]];
-      glo = e.type == 101 ? Rin[e.rdonor] : Rin[p - (long)(s - 1) * st];
-    }
-    if (s >= n - 1) {
-      const int f = 2 * DIR + 2;
-      const int i1 = DIR == 0 ? B.ni : i, j1 = DIR == 1 ? B.nj : j, k1 = DIR == 2 ? B.nk : k;
-      const BCRec& e = a.bc[a.fmap[face_map_index<R>(B, f, i1, j1, k1)]];
-      ghi = e.type == 101 ? Rin[e.rdonor] : Rin[p + (long)(n - s) * st];
-    }
-    auto r = [&](int d) -> R {
-      const int q = s + d;
-      if (q < 1) return glo;
-      if (q > n) return ghi;
-      return Rin[p + d * st];
-    };
-    const R beta = a.beta, den = a.den, inv = a.inv;
-    // first sweep at s-1 and s+1 (ghost cells keep r*)
-    const R rm = (s - 1 < 1) ? glo : irs_div(r(-1) + beta * (r(-2) + r(0)), den, inv);
-    const R rp = (s + 1 > n) ? ghi : irs_div(r(1) + beta * (r(0) + r(2)), den, inv);
-    const R rs = irs_div(r(0) + beta * (rm + rp), den, inv);
-    if (!UPDATE) {
-      a.Rout[p] = rs;
-    } else {
-      const R T = ((PRIM && a.jam) ? a.TO : a.T)[p], TO = a.TO[p];  // four-stage prim: T unused
-      int bad = 0;
-      const R Tn = new_state<PRIM>(a.tab, a.mat[p], T, TO, rs, a.rkc, a.jam, &bad);
-      a.Tout[p] = Tn;
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
-      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
-      const double d = (double)Tn - (double)TO;
-      rsq = d * d;
-    }
-  }
-  if (UPDATE && a.last) block_sum(rsq, a.partials, wg);
-}
-
-// Two Jacobi sweeps at line position s of n (Lib_IRS.f90:63-89) from
-// r(s-2..s+2); positions 0 and n+1 are the ghosts glo/ghi, which the sweeps
-// leave unchanged (Residual_Smoothing_Blk smooths interior cells only).
-template <typename R>
-__device__ __forceinline__ R jacobi2(R rm2, R rm1, R r0, R rp1, R rp2, int s, int n, R glo, R ghi, R beta, R den,
-                                     R inv) {
-#pragma clang fp contract(off)
-  const R am1 = s - 1 >= 1 ? rm1 : glo;
-  const R ap1 = s + 1 <= n ? rp1 : ghi;
-  const R am2 = s - 2 >= 1 ? rm2 : glo;
-  const R ap2 = s + 2 <= n ? rp2 : ghi;
-  const R sm = (s - 1 < 1) ? glo : irs_div(am1 + beta * (am2 + r0), den, inv);
-  const R sp = (s + 1 > n) ? ghi : irs_div(ap1 + beta * (r0 + ap2), den, inv);
-  return irs_div(r0 + beta * (sm + sp), den, inv);
-}
-
-// The same two sweeps with the division chosen at compile time (P2: den is
-// a power of two and x * inv is the IEEE quotient), and jacobi2_in for a
-// cell whose neighbours at distance 1 and 2 lie inside the line
-// (3 <= s <= n-2), where jacobi2 selects no ghost: the same operations.
-template <bool P2, typename R>
-__device__ __forceinline__ R irs_q(R x, R den, R inv) {
-  return P2 ? x * inv : x / den;
-}
-
-template <bool P2, typename R>
-__device__ __forceinline__ R jacobi2_p(R rm2, R rm1, R r0, R rp1, R rp2, int s, int n, R glo, R ghi, R beta, R den,
-                                       R inv) {
-#pragma clang fp contract(off)
-  const R am1 = s - 1 >= 1 ? rm1 : glo;
-  const R ap1 = s + 1 <= n ? rp1 : ghi;
-  const R am2 = s - 2 >= 1 ? rm2 : glo;
-  const R ap2 = s + 2 <= n ? rp2 : ghi;
-  const R sm = (s - 1 < 1) ? glo : irs_q<P2>(am1 + beta * (am2 + r0), den, inv);
-  const R sp = (s + 1 > n) ? ghi : irs_q<P2>(ap1 + beta * (r0 + ap2), den, inv);
-  return irs_q<P2>(r0 + beta * (sm + sp), den, inv);
-}
-
-template <bool P2, typename R>
-__device__ __forceinline__ R jacobi2_in(R rm2, R rm1, R r0, R rp1, R rp2, R beta, R den, R inv) {
-#pragma clang fp contract(off)
-  const R sm = irs_q<P2>(rm1 + beta * (rm2 + r0), den, inv);
-  const R sp = irs_q<P2>(rp1 + beta * (r0 + rp2), den, inv);
-  return irs_q<P2>(r0 + beta * (sm + sp), den, inv);
-}
-
-// IRS ghost of the line end on face f (Ghost_Residual_Extrapolation,
-// Lib_IRS.f90:170-184): the donor's R for a connection, else the boundary
-// cell's own R (zero gradient).
-template <typename R>
-__device__ __forceinline__ R irs_ghost(const IrsArgs<R>& a, const BlockInfo& B, int f, int i, int j, int k,
-                                       R own) {
-  if (!(B.conn & (1 << (f - 1)))) return own;  // no connection on this face
-  const int e = a.fmap[face_map_index<R>(B, f, i, j, k)];
-  const BCRec& r = a.bc[e];
-  return r.type == 101 ? ldu(a.Rin, (unsigned)r.rdonor) : own;
-}
-
-constexpr int XY_TJ = 16;                 // rows of an x-y IRS tile
-constexpr int XY_RJ = XY_TJ + 4, XY_RI = TILE_X + 4;
-
-// x then y smoothing of one plane tile of 64 x 16 cells (Residual_Smoothing
-// directions 1 and 2, Lib_IRS.f90:25-38) through LDS: R is read once.  Used
-// when no block has a connection on a j face (the y ghosts are then the
-// x-smoothed boundary rows).  2-D levels finish the stage here (UPDATE).
-template <int NDIR, bool UPDATE, bool PRIM, typename R>
-__global__ __launch_bounds__(TILE_T) void k_irs_xy(IrsArgs<R> a) {
-  if (a.halt && *a.halt) return;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Tile t = a.tiles[wg];
-  const BlockInfo B = a.blk[t.b];
-  const int k = t.k0;
-  __shared__ R rin[XY_RJ][XY_RI];
-  __shared__ R xs[XY_RJ][TILE_X];
-  const unsigned corner = (unsigned)(B.base + (t.i0 - 3) + (long)(t.j0 - 3) * B.sy + (long)(k - 1) * B.sz);
-  for (int q = threadIdx.x; q < XY_RJ * XY_RI; q += TILE_T) {
-    const int ii = q % XY_RI, jj = q / XY_RI;
-    const int gi = t.i0 - 2 + ii, gj = t.j0 - 2 + jj;
-    R v = 0;
-    if (gi >= 1 && gi <= B.ni && gj >= 1 && gj <= B.nj) v = ldu(a.Rin, corner + ii + jj * B.sy);
-    rin[jj][ii] = v;
-  }
-  __syncthreads();
-  const R beta = a.beta, den = a.den;
-  // x smoothing of rows j0-2 .. j0+17 inside the block
-  for (int q = threadIdx.x; q < XY_RJ * TILE_X; q += TILE_T) {
-    const int ii = q % TILE_X, jj = q / TILE_X;
-    const int i = t.i0 + ii, j = t.j0 - 2 + jj;
-    if (i > B.ni || j < 1 || j > B.nj) continue;
-    R glo = 0, ghi = 0;
-    if (i <= 2) glo = irs_ghost(a, B, 1, 1, j, k, rin[jj][ii + 2 - (i - 1)]);
-    if (i >= B.ni - 1) ghi = irs_ghost(a, B, 2, B.ni, j, k, rin[jj][ii + 2 + (B.ni - i)]);
-    xs[jj][ii] = jacobi2(rin[jj][ii], rin[jj][ii + 1], rin[jj][ii + 2], rin[jj][ii + 3], rin[jj][ii + 4], i, B.ni,
-                         glo, ghi, beta, den, a.inv);
-  }
-  __syncthreads();
-  // y smoothing of the 16 output rows; y ghosts are the x-smoothed rows 1, nj
-  double rsq = 0.0;
-  for (int q = threadIdx.x; q < XY_TJ * TILE_X; q += TILE_T) {
-    const int ii = q % TILE_X, jr = q / TILE_X;
-    const int i = t.i0 + ii, j = t.j0 + jr;
-    if (i > B.ni || j > B.nj) continue;
-    const int jj = jr + 2;  // LDS row of j
-    const R glo = (j <= 2) ? xs[jj - (j - 1)][ii] : (R)0;
-    const R ghi = (j >= B.nj - 1) ? xs[jj + (B.nj - j)][ii] : (R)0;
-    const R rs = jacobi2(xs[jj - 2][ii], xs[jj - 1][ii], xs[jj][ii], xs[jj + 1][ii], xs[jj + 2][ii], j, B.nj, glo, ghi,
-                         beta, den, a.inv);
-    const unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz);
-    if (!UPDATE) {
-      stu(a.Rout, p, rs);
-    } else {
-      const R T = ldu((PRIM && a.jam) ? a.TO : a.T, p), TO = ldu(a.TO, p);  // four-stage prim: T unused
-      int bad = 0;
-      const int m = B.umat >= 0 ? B.umat : ldu(a.mat, p);
-      const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
-      stu(a.Tout, p, Tn);
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
-      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
-      const double d = (double)Tn - (double)TO;
-      rsq += d * d;
-    }
-  }
-  if (UPDATE && a.last) block_sum(rsq, a.partials, wg);
-}
-
-// ------------------------------------------------------------------------
-// Line-tridiagonal IRS (FUSS_IRS_TRIDIAG, no reference counterpart): per
-// direction the exact solution of (1+2b) x_s - b (x_{s-1} + x_{s+1}) = r_s,
-// s = 1..n, with the IRS ghosts x_0, x_{n+1} of Residual_Smoothing held
-// fixed -- the fixed point of the reference's Jacobi sweeps
-// (Lib_IRS.f90:63-89).  Thomas algorithm, one line per work-item; the
-// elimination factors den_s = (1+2b) + b c_{s-1}, c_s = -b / den_s depend
-// only on s and are tabulated per block and direction on the host (tri).
-// Forward: d_1 = (r_1 + b x_0) / den_1, d_s = (r_s + b d_{s-1}) / den_s
-// (r_n + b x_{n+1} on the last); back: x_n = d_n, x_s = d_s - c_s x_{s+1}.
-// The d_s go to Rout and are overwritten by x_s on the way back.
-// ------------------------------------------------------------------------
-template <typename R>
-struct TriArgs {
-  const BlockInfo* blk;
-  const Tile* tiles;
-  const R* Rin;
-  R* Rout;
-  const BCRec* bc;
-  const int* fmap;
-  const double2* tri;  // (den_s, c_s) rows
-  const int* trioff;   // per block and direction: first row of its table
-  R beta;
-  const int* halt;
-};
-
-// Ghost of a line end for Residual_Smoothing: the donor's R on a
-// connection, else the line's own end value (Ghost_Residual_Extrapolation,
-// Lib_IRS.f90:137-187).
-template <typename R>
-__device__ __forceinline__ R tri_ghost(const TriArgs<R>& a, const BlockInfo& B, int f, int i, int j, int k, R own) {
-  if (!(B.conn & (1 << (f - 1)))) return own;
-  const BCRec& r = a.bc[a.fmap[face_map_index<R>(B, f, i, j, k)]];
-  return r.type == 101 ? ldu(a.Rin, (unsigned)r.rdonor) : own;
-}
-
-// y (DIR 1) and z (DIR 2) lines: a 64x4 tile of line origins, i fastest, so
-// every step of the sweep is a coalesced row access.
-template <int DIR, typename R>
-__global__ __launch_bounds__(TILE_T) void k_tri_yz(TriArgs<R> a) {
-  if (a.halt && *a.halt) return;
-  const Tile t = a.tiles[blockIdx.x];
-  const BlockInfo B = a.blk[t.b];
-  const int i = t.i0 + threadIdx.x % TILE_X, o2 = threadIdx.x / TILE_X;
-  // DIR 1: lines (i, k = k0 + o2); DIR 2: lines (i, j = j0 + o2)
-  const int j = DIR == 1 ? 1 : t.j0 + o2, k = DIR == 1 ? t.k0 + o2 : 1;
-  const int n = DIR == 1 ? B.nj : B.nk;
-  if (i > B.ni || (DIR == 1 ? k > B.nk : j > B.nj)) return;
-  const unsigned st = DIR == 1 ? (unsigned)B.sy : (unsigned)B.sz;
-  const unsigned p1 = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz);
-  const double2* tab = a.tri + a.trioff[3 * t.b + DIR];
-  const R beta = a.beta;
-  const R r1 = ldu(a.Rin, p1), rn = ldu(a.Rin, p1 + (unsigned)(n - 1) * st);
-  const R glo = DIR == 1 ? tri_ghost(a, B, 3, i, 1, k, r1) : tri_ghost(a, B, 5, i, j, 1, r1);
-  const R ghi = DIR == 1 ? tri_ghost(a, B, 4, i, B.nj, k, rn) : tri_ghost(a, B, 6, i, j, B.nk, rn);
-  R d = 0;
-  unsigned p = p1;
-  for (int s = 1; s <= n; s++, p += st) {
-#pragma clang fp contract(off)
-    R r = ldu(a.Rin, p);
-    if (s == 1) r = r + beta * glo;
-    if (s == n) r = r + beta * ghi;
-    const R den = (R)tab[s - 1].x;
-    d = s == 1 ? r / den : (r + beta * d) / den;
-    stu(a.Rout, p, d);
-  }
-  R x = d;
-  p = p1 + (unsigned)(n - 1) * st;
-  for (int s = n - 1; s >= 1; s--) {
-#pragma clang fp contract(off)
-    p -= st;
-    x = ldu((const R*)a.Rout, p) - (R)tab[s - 1].y * x;
-    stu(a.Rout, p, x);
-  }
-}
-
-// x lines: one wave per 64 consecutive rows j of a plane; the lines pass
-// through LDS in chunks of 16 cells so that global accesses are whole
-// 128-byte row segments.
-constexpr int TRI_CH = 16;
-
-template <typename R>
-__global__ __launch_bounds__(64) void k_tri_x(TriArgs<R> a) {
-  if (a.halt && *a.halt) return;
-  const Tile t = a.tiles[blockIdx.x];
-  const BlockInfo B = a.blk[t.b];
-  const int l = threadIdx.x;                // this lane's line: row j0 + l
-  const int j = t.j0 + l, k = t.k0, n = B.ni;
-  const bool live = j <= B.nj;
-  __shared__ R ch[TRI_CH][TILE_X + 1];
-  const unsigned row0 = (unsigned)(B.base + (long)(t.j0 - 1) * B.sy + (long)(k - 1) * B.sz);  // (i=1, j0, k)
-  const double2* tab = a.tri + a.trioff[3 * t.b];
-  const R beta = a.beta;
-  R glo = 0, ghi = 0;
-  if (live) {
-    const unsigned pl = row0 + (unsigned)l * (unsigned)B.sy;
-    glo = tri_ghost(a, B, 1, 1, j, k, ldu(a.Rin, pl));
-    ghi = tri_ghost(a, B, 2, B.ni, j, k, ldu(a.Rin, pl + (unsigned)(n - 1)));
-  }
-  const int nch = (n + TRI_CH - 1) / TRI_CH;
-  // chunk c holds cells i = c*16+1 .. c*16+16 of the 64 rows: element e of a
-  // lane's 16 loads is row e/16, cell e%16 (16 lanes per 128-byte segment)
-  auto chunk_io = [&](int c, bool load, const R* src, R* dst) {
-    for (int q = 0; q < TRI_CH; q++) {
-      const int e = l + 64 * q, rr = e / TRI_CH, ii = e % TRI_CH;
-      const int i = c * TRI_CH + ii + 1;
-      if (t.j0 + rr <= B.nj && i <= n) {
-        const unsigned p = row0 + (unsigned)rr * (unsigned)B.sy + (unsigned)(i - 1);
-        if (load) ch[ii][rr] = ldu(src, p);
-        else stu(dst, p, ch[ii][rr]);
-      }
-    }
-  };
-  R d = 0;
-  for (int c = 0; c < nch; c++) {
-    chunk_io(c, true, a.Rin, nullptr);
-    __syncthreads();
-    if (live) {
-      for (int ii = 0; ii < TRI_CH; ii++) {
-#pragma clang fp contract(off)
-        const int s = c * TRI_CH + ii + 1;
-        if (s > n) break;
-        R r = ch[ii][l];
-        if (s == 1) r = r + beta * glo;
-        if (s == n) r = r + beta * ghi;
-        const R den = (R)tab[s - 1].x;
-        d = s == 1 ? r / den : (r + beta * d) / den;
-        ch[ii][l] = d;
-      }
-    }
-    __syncthreads();
-    chunk_io(c, false, nullptr, a.Rout);
-    __syncthreads();
-  }
-  __threadfence_block();
-  R x = d;
-  for (int c = nch - 1; c >= 0; c--) {
-    chunk_io(c, true, a.Rout, nullptr);
-    __syncthreads();
-    if (live) {
-      for (int ii = TRI_CH - 1; ii >= 0; ii--) {
-#pragma clang fp contract(off)
-        const int s = c * TRI_CH + ii + 1;
-        if (s >= n) continue;  // x_n = d_n stays
-        x = ch[ii][l] - (R)tab[s - 1].y * x;
-        ch[ii][l] = x;
-      }
-    }
-    __syncthreads();
-    chunk_io(c, false, nullptr, a.Rout);
-    __syncthreads();
-  }
-}
-
-// The RK update from the smoothed residual (after the last tridiagonal
-// direction), with the residual partials of the step.
-template <bool PRIM, typename R>
-__global__ __launch_bounds__(TILE_T) void k_rk_update(IrsArgs<R> a) {
-  if (a.halt && *a.halt) return;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Tile t = a.tiles[wg];
-  const BlockInfo B = a.blk[t.b];
-  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
-  double rsq = 0.0;
-  if (i <= B.ni && j <= B.nj) {
-    const unsigned p = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz);
-    const R T = ldu((PRIM && a.jam) ? a.TO : a.T, p), TO = ldu(a.TO, p), rs = ldu(a.Rin, p);
-    int bad = 0;
-    const int m = B.umat >= 0 ? B.umat : ldu(a.mat, p);
-    const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
-    stu(a.Tout, p, Tn);
-    if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
-    if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
-    const double dd = (double)Tn - (double)TO;
-    rsq = dd * dd;
-  }
-  if (a.last) block_sum(rsq, a.partials, wg);
-}
-
-// z smoothing (direction 3) fused with the RK update: a 64x4 column marches
-// through KC planes with r(k-2..k+2) in registers, so R is read once.
-template <bool PRIM, typename R, int KCT>
-__global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
-  if (a.halt && *a.halt) return;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Tile t = a.tiles[wg];
-  const BlockInfo B = a.blk[t.b];
-  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X);
-  const int k0 = t.k0, k1 = min(B.nk, k0 + KCT - 1), n = B.nk;
-  double rsq = 0.0;
-  if (i <= B.ni && j <= B.nj) {
-    const unsigned sz = (unsigned)B.sz;
-    const unsigned p1 = (unsigned)(B.base + (i - 1) + (long)(j - 1) * B.sy);  // plane k = 1
-    const R glo = (k0 <= 2) ? irs_ghost(a, B, 5, i, j, 1, ldu(a.Rin, p1)) : (R)0;
-    const R ghi = (k1 >= n - 1) ? irs_ghost(a, B, 6, i, j, n, ldu(a.Rin, p1 + (n - 1) * sz)) : (R)0;
-    auto rd = [&](int kk) -> R { return (kk >= 1 && kk <= n) ? ldu(a.Rin, p1 + (kk - 1) * sz) : (R)0; };
-    R w0 = rd(k0 - 2), w1 = rd(k0 - 1), w2 = rd(k0), w3 = rd(k0 + 1), w4 = rd(k0 + 2);
-    const int m = B.umat >= 0 ? B.umat : -1;
-    for (int k = k0; k <= k1; k++) {
-      const R rs = jacobi2(w0, w1, w2, w3, w4, k, n, glo, ghi, a.beta, a.den, a.inv);
-      const unsigned p = p1 + (k - 1) * sz;
-      const R T = ldu((PRIM && a.jam) ? a.TO : a.T, p), TO = ldu(a.TO, p);  // four-stage prim: T unused
-      int bad = 0;
-      const R Tn = new_state<PRIM>(a.tab, m >= 0 ? m : ldu(a.mat, p), T, TO, rs, a.rkc, a.jam, &bad);
-      stu(a.Tout, p, Tn);
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
-      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
-      const double d = (double)Tn - (double)TO;
-      rsq += d * d;
-      w0 = w1; w1 = w2; w2 = w3; w3 = w4;
-      w4 = rd(k + 3);
-    }
-  }
-  if (a.last) block_sum(rsq, a.partials, wg);
-}
-
-// All three directions of Residual_Smoothing (Lib_IRS.f90:14-40) and the RK
-// update in one pass, for 3-D levels without connections on j or k faces: a
-// 64xTY column marches through its KC planes.  Each R plane enters LDS once
-// with a 2-cell i/j halo, is smoothed along x then y there (the y ghosts are
-// the x-smoothed boundary rows, as in k_irs_xy), and its 64xTY result joins a
-// register window s(q-4..q) that the z sweeps read (z ghosts: the
-// xy-smoothed boundary planes).  R is read from HBM once per stage instead of
-// three times (x-y pass in, z pass in) plus once written.  Planes are
-// fetched two ahead into two register sets that alternate over a 2-plane
-// unrolled loop; every load is unconditional (clamped plane, valid cells for
-// out-of-block lanes) so the waits leave the later planes in flight.
-// XCON: some block has a connection on an i face (x ghosts from the donors).
-// TY: rows of the column (4 or 8; one wave per row): the 8-row column reads
-// 12 instead of 16 halo rows per 8 rows of output and sweeps 12 x lines.
-template <int TY>
-struct Xyz {
-  static constexpr int TT = TILE_X * TY;                // threads
-  static constexpr int RJ = TY + 4, RI = TILE_X + 4;    // halo box
-  static constexpr int NLD = (RJ * RI + TT - 1) / TT;   // plane loads per thread
-};
-
-template <typename R, int NLD>
-struct XyzPlane {
-  R v[NLD];
-};
-
-// Element u of a thread's share of the tile's (TY+4) x (TILE_X+4) halo box
-// lies in the block.
-template <int TY>
-__device__ __forceinline__ bool xyz_in(const BlockInfo& B, int i0, int j0, int u) {
-  using G = Xyz<TY>;
-  const int e = threadIdx.x + u * G::TT;
-  const int gi = i0 - 2 + e % G::RI, gj = j0 - 2 + e / G::RI;
-  return e < G::RJ * G::RI && gi >= 1 && gi <= B.ni && gj >= 1 && gj <= B.nj;
-}
-
-// The R values of plane q in the halo box, raw: cells outside the block read
-// the box's first in-block cell, and the LDS store writes 0 for them (the
-// mask is applied there, so no wait for these loads is placed before).
-template <int TY, typename R>
-__device__ __forceinline__ void xyz_fetch(XyzPlane<R, Xyz<TY>::NLD>& r, const R* __restrict__ Rin,
-                                          const BlockInfo& B, int i0, int j0, int q) {
-  using G = Xyz<TY>;
-  const unsigned corner = (unsigned)(B.base + (i0 - 3) + (long)(j0 - 3) * B.sy + (long)(q - 1) * B.sz);
-  // first in-block cell of the box: (max(i0-2,1), max(j0-2,1))
-  const unsigned safe = corner + (i0 > 2 ? 0u : (unsigned)(3 - i0)) + (j0 > 2 ? 0u : (unsigned)(3 - j0)) * (unsigned)B.sy;
-#pragma unroll
-  for (int u = 0; u < G::NLD; u++) {
-    const int e = threadIdx.x + u * G::TT;
-    const int ii = e % G::RI, jj = e / G::RI;
-    r.v[u] = ldu(Rin, xyz_in<TY>(B, i0, j0, u) ? corner + ii + jj * (unsigned)B.sy : safe);
-  }
-}
-
-template <bool PRIM, bool XCON, bool P2, bool DT, typename R, int KCT, int TY>
-__global__ __launch_bounds__(Xyz<TY>::TT) void k_irs_xyz(IrsArgs<R> a) {
-  using G = Xyz<TY>;
-  if (a.halt && *a.halt) return;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Tile t = a.tiles[wg];
-  const BlockInfo B = a.blk[t.b];
-  const int tx = threadIdx.x % TILE_X, ty = threadIdx.x / TILE_X;
-  const int i = t.i0 + tx, j = t.j0 + ty;
-  const int n = B.nk, k0 = t.k0, k1 = min(n, k0 + KCT - 1);
-  const bool live = i <= B.ni && j <= B.nj;
-  __shared__ R rin[G::RJ][G::RI];
-  __shared__ R xs[G::RJ][TILE_X];
-  const R beta = a.beta, den = a.den, inv = a.inv;
-  // window w[0..4] = s(q-4 .. q); planes outside 1..n are 0 (jacobi2 selects
-  // the ghosts there)
-  R w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0;
-  R zlo = 0, zhi = 0;  // xy-smoothed planes 1 and n (z ghosts of walls)
-  const int qa = max(1, k0 - 2), qb = min(n, k1 + 2), qe = k1 + 2;
-  const int ci = live ? i : 1, cj = live ? j : 1;
-  const unsigned p1 = (unsigned)(B.base + (ci - 1) + (long)(cj - 1) * B.sy);  // plane 1 of the lane's column
-  const unsigned sz = (unsigned)B.sz;
-  double rsq = 0.0, dtmin = 1e300;
-  // next-step dt on a rectilinear level: sum 1/dl^2 = (s_x + s_y) + s_z from
-  // the per-axis factors (bit-identical to invL2), the column's s_z in LDS
-  __shared__ double qzs[KCT];
-  const bool rdt = DT && a.rect;
-  double qxy = 0.0;
-  if (rdt) {
-    const RectAxes rx_ = rect_axes(a.rect, B);
-    if (threadIdx.x < KCT) qzs[threadIdx.x] = rx_.qz[min(k0 + (int)threadIdx.x, n)];
-    qxy = rx_.qx[ci] + rx_.qy[cj];
-  }
-  bool inb[G::NLD];
-#pragma unroll
-  for (int u = 0; u < G::NLD; u++) inb[u] = xyz_in<TY>(B, t.i0, t.j0, u);
-  // the tile's x lines and the wave's y line (one row j per wave) away from
-  // the block faces: no ghost selection
-  const bool xin = t.i0 >= 3 && t.i0 + TILE_X - 1 <= B.ni - 2;
-  const int ju = t.j0 + __builtin_amdgcn_readfirstlane(ty);
-  const bool yin = ju >= 3 && ju <= B.nj - 2;
-  XyzPlane<R, G::NLD> pa, pb;
-  xyz_fetch<TY>(pa, a.Rin, B, t.i0, t.j0, qa);
-  xyz_fetch<TY>(pb, a.Rin, B, t.i0, t.j0, min(qa + 1, qb));
-  // one step: plane q (staged in `pre`) is smoothed in x-y (if q <= qb), the
-  // window shifts, and plane q-2 is output (if it is in the column)
-  auto step = [&](XyzPlane<R, G::NLD>& pre, int q) {
-    const int ko = q - 2;
-    const bool out = ko >= k0;
-    const unsigned po = p1 + (unsigned)(max(ko, 1) - 1) * sz;
-    // the four-stage update in primitive variables (new = old + a_k res) does
-    // not read the stage state: its load then hits T^n's line (no HBM bytes)
-    const R T = ldn((PRIM && a.jam) ? a.TO : a.T, po), TO = ldn(a.TO, po);
-    int m = B.umat;
-    if (m < 0) m = ldu(a.mat, po);
-    R s = 0;
-    if (q <= qb) {
-#pragma unroll
-      for (int u = 0; u < G::NLD; u++) {
-        const int e = threadIdx.x + u * G::TT;
-        if (e < G::RJ * G::RI) (&rin[0][0])[e] = inb[u] ? pre.v[u] : (R)0;
-      }
-    }
-    // plane q+2 flies while this one is smoothed (issued on every step, so
-    // the wait for T and TO below can leave it outstanding)
-    xyz_fetch<TY>(pre, a.Rin, B, t.i0, t.j0, min(q + 2, qb));
-    if (q <= qb) {
-      __syncthreads();
-      // x sweeps of rows j0-2 .. j0+TY+1 (wave ty: rows ty, ty+TY, ...); a
-      // tile whose cells are all >= 2 cells from the i faces selects no ghost
-      // (uniform branch)
-#pragma unroll
-      for (int h = 0; h < (G::RJ + TY - 1) / TY; h++) {
-        const int jj = ty + h * TY, jr = t.j0 - 2 + jj;
-        if (jj < G::RJ && i <= B.ni && jr >= 1 && jr <= B.nj) {
-          if (xin) {
-            xs[jj][tx] = jacobi2_in<P2>(rin[jj][tx], rin[jj][tx + 1], rin[jj][tx + 2], rin[jj][tx + 3],
-                                        rin[jj][tx + 4], beta, den, inv);
-          } else {
-            R glo = 0, ghi = 0;
-            if (i <= 2) {
-              glo = rin[jj][tx + 2 - (i - 1)];
-              if (XCON) glo = irs_ghost(a, B, 1, 1, jr, q, glo);
-            }
-            if (i >= B.ni - 1) {
-              ghi = rin[jj][tx + 2 + (B.ni - i)];
-              if (XCON) ghi = irs_ghost(a, B, 2, B.ni, jr, q, ghi);
-            }
-            xs[jj][tx] = jacobi2_p<P2>(rin[jj][tx], rin[jj][tx + 1], rin[jj][tx + 2], rin[jj][tx + 3],
-                                       rin[jj][tx + 4], i, B.ni, glo, ghi, beta, den, inv);
-          }
-        }
-      }
-      __syncthreads();
-      // y sweeps of the lane's cell: the next plane's LDS store follows the
-      // second barrier (rin no longer read), its x sweeps the next first
-      // barrier (xs no longer read)
-      if (live) {
-        const int jj = ty + 2;
-        if (yin) {
-          s = jacobi2_in<P2>(xs[jj - 2][tx], xs[jj - 1][tx], xs[jj][tx], xs[jj + 1][tx], xs[jj + 2][tx], beta, den,
-                             inv);
-        } else {
-          const R glo = (j <= 2) ? xs[jj - (j - 1)][tx] : (R)0;
-          const R ghi = (j >= B.nj - 1) ? xs[jj + (B.nj - j)][tx] : (R)0;
-          s = jacobi2_p<P2>(xs[jj - 2][tx], xs[jj - 1][tx], xs[jj][tx], xs[jj + 1][tx], xs[jj + 2][tx], j, B.nj,
-                            glo, ghi, beta, den, inv);
-        }
-      }
-      if (q == 1) zlo = s;
-      if (q == n) zhi = s;
-    }
-    w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = s;
-    if (out && live) {
-      const R rs = (ko >= 3 && ko <= n - 2) ? jacobi2_in<P2>(w0, w1, w2, w3, w4, beta, den, inv)
-                                            : jacobi2_p<P2>(w0, w1, w2, w3, w4, ko, n, zlo, zhi, beta, den, inv);
-      int bad = 0;
-      const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
-      stn(a.Tout, po, Tn);
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, ko);
-      if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, ko);
-      const double d = (double)Tn - (double)TO;
-      rsq += d * d;
-      if (DT) {
-        // Compute_dt (Mod_dt.f90:26-84) at T^{n+1}, without the vnn ramp
-        // (the host fuses only when it cannot apply); a separate instance of
-        // the kernel, so the other stages keep the registers it needs
-        int bad3 = 0;
-        using V2 = typename Vec2<R>::T;
-        using V4 = typename Vec4<R>::T;
-        // (v, dv) rows: the interpolation v + (T - Tm) dv of tab_interp
-        const int rw = tab_row(Tn, a.tab.Ti1, a.tab.Ti2, &bad3) - a.tab.Ti1;
-        const unsigned roff = (unsigned)((long)m * a.tab.nT + rw);
-        const V2 kk = ldu(reinterpret_cast<const V2*>(a.tab.kap2), roff);
-        const V4 pp = ldu(reinterpret_cast<const V4*>(a.tab.pr4), roff);
-        double dd;
-        {
-#pragma clang fp contract(off)
-          const R fr = Tn - (R)(rw + a.tab.Ti1);
-          const R kap = kk.x + fr * kk.y, rho = pp.x + fr * pp.y, cp = pp.z + fr * pp.w;
-          const double alpha = (double)kap / (double)rho / (double)cp;
-          dd = 0.5 * a.vnn / (alpha * (rdt ? qxy + qzs[ko - k0] : ldn(a.invL2, po)));
-          dd = dd < 1e8 ? dd : 1e8;
-        }
-        stn(a.dt, po, (R)dd);
-        dtmin = fmin(dtmin, dd);
-        if (bad3) flag_error(a.status, FUSS_ETABLE, t.b, i, j, ko);
-      }
-    }
-  };
-  for (int q = qa; q <= qe; q += 2) {
-    step(pa, q);
-    if (q + 1 <= qe) step(pb, q + 1);
-  }
-  if (a.last) block_sum<G::TT>(rsq, a.partials, wg);
-  if (DT) block_min<G::TT>(dtmin, a.pmin, wg);
-}
 
 // ------------------------------------------------------------------------
 // kernels: reductions (deterministic: fixed partition and order)
@@ -2531,198 +670,6 @@ __global__ void k_prolong(MGArgs a) {
 // ------------------------------------------------------------------------
 // host side
 // ------------------------------------------------------------------------
-namespace {
-
-template <typename T>
-struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  int alloc(size_t count) {
-    free();
-    n = count;
-    if (!count) return 0;
-    if (hipMalloc(&p, sizeof(T) * count) != hipSuccess) { p = nullptr; return FUSS_EHIP; }
-    // the zero fill runs on the null stream, which the library's non-blocking
-    // stream does not wait for: finish it before any kernel on that stream
-    // writes the buffer (else the fill can land after, over the results)
-    if (hipMemset(p, 0, sizeof(T) * count) != hipSuccess) return FUSS_EHIP;
-    if (hipStreamSynchronize(nullptr) != hipSuccess) return FUSS_EHIP;
-    return 0;
-  }
-  void free() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-};
-
-struct HostBlock {
-  int n[3] = {0, 0, 0};
-  std::vector<double> nodes, T, matID, qvol;
-};
-
-struct Level {
-  int nb = 0;
-  int kp = 2;                       // k ghost layers (1 in 2-D: k-face data only)
-  std::vector<BlockInfo> hblk;
-  std::vector<HostBlock> hb;        // host nodes (every level), fields (level 1)
-  std::vector<Tile> htiles;         // one 64x4 cell tile per plane
-  std::vector<Tile> hctiles;        // 64x4 columns of kc planes (3-D levels)
-  std::vector<Tile> hc8tiles;       // 64x8 columns of kc planes (fused x-y-z IRS pass)
-  int kc = KC_DEFAULT;              // planes per column tile (fuss_autotune)
-  int xyz_ty = 4;                   // rows of the fused IRS pass's columns, 4 or 8 (fuss_autotune)
-  int stage_ty = 4;                 // rows of the LDS-ring stage kernel's columns, 4 or 8 (fuss_autotune)
-  int dt_ty = 4;                    // ... of the pass that wrote the dt minima (dt_next)
-  int row_march = -1;               // orthogonal 3-D stage kernel: -1 context default, 0 LDS ring, 1 row march
-  std::vector<Tile> hxytiles;       // 64x16 plane tiles of the x-y IRS pass
-  std::vector<int> hrng[4];         // per tile list (tiles, ctiles, xytiles, c8tiles): first tile of each block
-  DBuf<int> rng[4];
-  int plist = 0;                    // tile list of the last step's residual partials
-  int no_ycon = 1;                  // no connection on a j face (fused x-y IRS allowed)
-  int no_zcon = 1;                  // no connection on a k face (with no_ycon: fused x-y-z IRS)
-  int no_xcon = 1;                  // no connection on an i face
-  // line-tridiagonal IRS: line tiles (x: 64 rows of a plane; y, z: 64x4
-  // line origins) and the per-block elimination factors for tri_beta
-  std::vector<Tile> htx, hty, htz;
-  DBuf<Tile> ttx, tty, ttz;
-  DBuf<double2> tri;
-  DBuf<int> trioff;
-  double tri_beta = -1.0;
-  int ghost_read = 1;               // some BC entry reads layer-1 ghosts (101, 300, 302)
-  int dt_next = 0;                  // dt and its per-tile minima hold the next step's (fused IRS pass)
-  double dt_vnn = 0.0;              // ... computed with this vnn
-  int all_umat = 0;                 // every block has a single material
-  int dims2 = 0;                    // every block has >= 2 cells in each direction
-  // rectilinear levels: per-axis metric factors (RectAxes) that reproduce the
-  // stored volumes and normal weights to rect_dev (relative) and sum 1/dl^2 exactly
-  DBuf<double> rfac;
-  int rect_w = 0;                   // every block rectilinear, factors within RECT_TOL
-  int rect_dt = 0;                  // ... and sum 1/dl^2 bit-identical
-  double rect_dev = -1.0;           // largest relative deviation found (-1: not checked)
-  long npad = 0, ncell = 0, nnodes = 0;
-  DBuf<BlockInfo> blk;
-  DBuf<Tile> tiles, ctiles, xytiles, c8tiles;
-  DBuf<double> nodes;
-  DBuf<double> Tb[3];               // T^n and two stage buffers
-  int cur = 0;                      // index of T^n in Tb
-  DBuf<double> R, R2, dt, qvol, vol, invL2;
-  DBuf<int> mat;
-  DBuf<double> M, F[3], W[9];
-  DBuf<BCRec> bc;
-  DBuf<BCHot> bch;                  // compact wall records (k_bcflux)
-  DBuf<int2> gmap;                  // (ghost, source) per entry (k_ghost)
-  DBuf<int> fmap;
-  DBuf<double> bcv, bcf;
-  int nbound = 0;
-  std::vector<int> hib;             // nbound x 15
-  // chimera (102) donor lists (fuss_set_chimera, host): per listed entry
-  // its BC index and donor counts, then 4 ints (b, i, j, k) and a volume
-  // fraction per donor, entries in list order
-  std::vector<int> hch_ent, hch_n1, hch_n2, hch_don;
-  std::vector<double> hch_vf;
-  // device: per chimera entry (ghost cell, first donor, layer-1 donors,
-  // material index), the donors' arena indices and fractions
-  DBuf<int4> chr;
-  DBuf<int> chd;
-  DBuf<double> chv;
-  int nchim = 0;
-  std::vector<double> hrb;          // nbound x 6
-  std::vector<int> hser;
-  int tang = 0, any_active = 0;
-  double time = 0.0;                // host mirrors of the device clock
-  int iter = 0;
-  DBuf<double> clk;                 // [0] level time
-  DBuf<int> cnt;                    // [0] iteration, [1] halt, [2] run steps, [3] chunk start, [4] run step limit
-  // fuss_run's captured chunk of RUN_G steps, replayed by later calls with the
-  // same step parameters and stop thresholds (dropped when tiles change)
-  hipGraph_t run_graph = nullptr;
-  hipGraphExec_t run_exec = nullptr;
-  fuss_step_params run_p{};
-  double run_thr = 0.0, run_tthr = 0.0;
-  int run_seq[RUN_G + 1] = {};
-  DBuf<double> partials, pmin, red2;
-};
-
-// One level of the correction multigrid (arrays in the inner precision).
-struct CLevel {
-  int ncell = 0, maxcells = 0;
-  std::vector<CBlk> hblk;
-  DBuf<CBlk> blk;
-  DBuf<unsigned char> mem;
-  void* d[3] = {nullptr, nullptr, nullptr};
-  void *b = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *D = nullptr, *s = nullptr, *s2 = nullptr;
-  int cur = 0;
-};
-
-struct Refine {
-  int ready = 0;
-  fuss_refine_params p{};
-  std::vector<CLevel> lv;
-  DBuf<double> bcd, red;
-  DBuf<double> bres;                // per-block monitor residual (independent cases)
-  double* hbres = nullptr;          // pinned
-  std::vector<int> conv_outer;      // first outer iteration at which each block met the threshold
-  std::vector<double> res;          // last monitor residual per block
-};
-
-// Checkpointed forward and reverse sweep of the time-accurate step.
-struct Adjoint {
-  int ready = 0, prec = FUSS_FP64;
-  int nsteps = 0, K = 0, nprobe = 0, forward_done = 0;
-  std::vector<long> hprobe;
-  DBuf<long> probe, coff;
-  DBuf<double> ckpt;                // (nsteps / K + 1) states
-  DBuf<double> seg;                 // the K states of one segment
-  DBuf<double> stage[4];            // stage inputs of the recomputed step
-  DBuf<unsigned char> lam;          // lam, lin, lTO in the adjoint precision
-  DBuf<double> part, red, acc;      // partial sums; acc[0] dL/d(dt), acc[1] dL/dc
-  DBuf<int> amin;
-  std::vector<double> times;        // level time before each step
-  std::vector<int> iters;
-  DBuf<double> dtimes;              // the same, recorded on the device during the forward
-  DBuf<int> diters;
-  fuss_step_params p{};
-};
-
-}  // namespace
-
-struct fuss_ctx {
-  int nlev = 0, nb = 0, precision = FUSS_FP64, ndir = 0;
-  std::vector<Level> lev;
-  hipStream_t stream = nullptr;
-  bool own_stream = false;
-  int nmat = 0, Ti1 = 0, nT = 0;
-  std::vector<double> tcp, trho, tkap, th;
-  DBuf<double> dcp, drho, dkap, dh, dkap2, dpr4;
-  struct Series { int periodic; std::vector<double> t, v; };
-  std::vector<Series> series;
-  DBuf<int> status;                 // code, b, i, j, k
-  DBuf<double> scal;                // dtglobal, time, res_sq
-  double* hscal = nullptr;          // pinned readback
-  int setup_done = 0;
-  int* hstatus = nullptr;
-  Refine ref;                       // steady defect correction (fuss_refine)
-  Adjoint adj;                      // discrete adjoint (fuss_adj_*)
-  int adj_store_on = 0;             // enqueue_step copies each stage input to adj.stage[]
-  int graphs_off = 0;               // fuss_run without graph capture (fuss_graphs)
-  int stage_lds = 1;                // orthogonal 3-D stages: LDS-ring kernel (1) or row march (0)
-  int rect_off = 0;                 // per-axis metric factors switched off (fuss_rect)
-  double kscale = 1.0;              // conductivity table scale (fuss_set_kscale)
-  DBuf<int> stage_i;                // staging for fuss_set_bc_values
-  DBuf<double> stage_d;
-  DBuf<double> ser_t, ser_v;        // time series on the device
-  DBuf<int> ser_off, ser_n, ser_per;
-  DBuf<double> run_hist;            // per-chunk residual / time history of fuss_run
-  double* hrun = nullptr;           // pinned: run history readback
-  int* hcnt = nullptr;              // pinned: level counters readback
-  // per-kernel-class timing with HIP events on the context stream
-  struct Prof {
-    int enabled = 0;
-    std::vector<hipEvent_t> pool;
-    std::vector<int> cls;       // class of each recorded (start, end) pair
-    size_t used = 0;
-  } prof;
-};
 
 namespace {
 
@@ -2993,6 +940,22 @@ void run_graph_free(Level& L) {
   if (L.run_graph) (void)hipGraphDestroy(L.run_graph);
   L.run_exec = nullptr;
   L.run_graph = nullptr;
+  L.run_bufs.clear();
+}
+
+// Every device buffer a captured step of level L may address (kernel
+// arguments are fixed at capture): a replay is allowed only while all of them
+// are the allocations the graph was captured with.
+std::vector<const void*> run_buffers(const fuss_ctx* c, const Level& L) {
+  std::vector<const void*> v = {
+      L.blk.p, L.tiles.p, L.ctiles.p, L.c8tiles.p, L.xytiles.p, L.Tb[0].p, L.Tb[1].p, L.Tb[2].p, L.R.p, L.R2.p,
+      L.dt.p, L.qvol.p, L.vol.p, L.invL2.p, L.mat.p, L.bc.p, L.bch.p, L.gmap.p, L.fmap.p, L.bcv.p, L.bcf.p,
+      L.rfac.p, L.ttx.p, L.tty.p, L.ttz.p, L.tri.p, L.trioff.p, L.chr.p, L.chd.p, L.chv.p, L.clk.p, L.cnt.p,
+      L.partials.p, L.pmin.p, L.red2.p, c->dcp.p, c->drho.p, c->dkap.p, c->dh.p, c->dkap2.p, c->dpr4.p,
+      c->status.p, c->scal.p, c->run_hist.p, c->ser_t.p, c->ser_v.p, c->ser_off.p, c->ser_n.p, c->ser_per.p};
+  for (const auto& w : L.W) v.push_back(w.p);
+  for (const auto& r : L.rng) v.push_back(r.p);
+  return v;
 }
 
 // hrng[q]: index of the first tile of each block in a block-ordered list
@@ -3149,9 +1112,9 @@ int build_bc(fuss_ctx* c, Level& L) {
     R.fpos = fpos;
     R.qw = r[0]; R.Tw = r[1]; R.hconv = r[2]; R.Tref = r[4]; R.eps = r[5];
     const int gi = guide(f, 0), gj = guide(f, 1), gk = guide(f, 2);
-    const bool kface = f >= 5;
-    const bool two_d = L.kp == 0;
-    R.gdst = (two_d && kface) ? -1 : cidx(B, i - gi, j - gj, k - gk);
+    // layer-1 ghost (2-D levels: k faces address the single k plane kept on
+    // each side of the block)
+    R.gdst = cidx(B, i - gi, j - gj, k - gk);
     R.gsrc = R.cell;  // Ghost_Symmetry layer 1: T(ghost) = T(cell)
     R.rdonor = -1;
     for (int t = 0; t < 4; t++) R.dn[t] = R.cell;
@@ -3168,11 +1131,11 @@ int build_bc(fuss_ctx* c, Level& L) {
       else if (Fs <= 4) { int t[4][3] = {{Is - d11, Js, Ks - d12}, {Is + d11, Js, Ks + d12}, {Is - d21, Js, Ks - d22}, {Is + d21, Js, Ks + d22}}; memcpy(p, t, sizeof(p)); }
       else { int t[4][3] = {{Is - d11, Js - d12, Ks}, {Is + d11, Js + d12, Ks}, {Is - d21, Js - d22, Ks}, {Is + d21, Js + d22, Ks}}; memcpy(p, t, sizeof(p)); }
       for (int t = 0; t < 4; t++) {
-        // 2-D: the k offsets of Tg(5:6) address mirrored planes equal to the
-        // donor plane; they enter multiplied by a vanishing M_con(3,:).N
-        if (two_d) p[t][2] = 1;
-        if (p[t][0] < 0 || p[t][0] > S.ni + 1 || p[t][1] < 0 || p[t][1] > S.nj + 1 || p[t][2] < (two_d ? 1 : 0) ||
-            p[t][2] > (two_d ? 1 : S.nk + 1))
+        // (2-D levels keep one k plane on each side, so k offsets of +-1 stay
+        // inside the block's arena box; those Tg(5:6) terms enter multiplied
+        // by a vanishing M_con(3,:).N)
+        if (p[t][0] < 0 || p[t][0] > S.ni + 1 || p[t][1] < 0 || p[t][1] > S.nj + 1 || p[t][2] < 0 ||
+            p[t][2] > S.nk + 1)
           return FUSS_EBC;
         R.dn[t] = cidx(S, p[t][0], p[t][1], p[t][2]);
       }
@@ -3293,6 +1256,10 @@ int set_clock(fuss_ctx* c, Level& L, double time, int iter) {
   return 0;
 }
 
+}  // namespace
+
+namespace fuss {
+
 // Event pair around one launch of a kernel class (profiling mode only).
 int prof_begin(fuss_ctx* c, int cls) {
   auto& P = c->prof;
@@ -3313,6 +1280,11 @@ int prof_begin(fuss_ctx* c, int cls) {
 void prof_end(fuss_ctx* c, int id) {
   if (id >= 0) (void)hipEventRecord(c->prof.pool[2 * id + 1], c->stream);
 }
+
+}  // namespace fuss
+
+namespace {
+
 
 template <typename R>
 Tab<R> dtab(fuss_ctx* c) {
@@ -3736,134 +1708,8 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
 
 namespace {
 
-template <int NDIR, bool TANG, bool PRIM, bool IRSMODE>
-void launch_stage_t(fuss_ctx* c, Level& L, const StageArgs<double>& a) {
-  k_stage<NDIR, TANG, PRIM, IRSMODE, double><<<(unsigned)L.htiles.size(), TILE_T, 0, c->stream>>>(a);
-}
+// (stage and IRS launchers: stage_*.hip, irs_*.hip)
 
-template <int NDIR>
-void launch_stage_n(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang, bool prim, bool irs) {
-  if (tang) {
-    if (prim) { irs ? launch_stage_t<NDIR, true, true, true>(c, L, a) : launch_stage_t<NDIR, true, true, false>(c, L, a); }
-    else { irs ? launch_stage_t<NDIR, true, false, true>(c, L, a) : launch_stage_t<NDIR, true, false, false>(c, L, a); }
-  } else {
-    if (prim) { irs ? launch_stage_t<NDIR, false, true, true>(c, L, a) : launch_stage_t<NDIR, false, true, false>(c, L, a); }
-    else { irs ? launch_stage_t<NDIR, false, false, true>(c, L, a) : launch_stage_t<NDIR, false, false, false>(c, L, a); }
-  }
-}
-
-template <bool UM, bool RECT, int KCT, int TY>
-void launch_stage3p(fuss_ctx* c, unsigned g, const StageArgs<double>& a, bool prim, bool irs) {
-  constexpr int T = TILE_X * TY;
-  if (RECT && prim && a.dtw) {  // first stage of a steady step: dt here (rectilinear, primitive)
-    irs ? k_stage3p<UM, true, true, true, true, double, KCT, TY><<<g, T, 0, c->stream>>>(a)
-        : k_stage3p<UM, true, false, true, true, double, KCT, TY><<<g, T, 0, c->stream>>>(a);
-  } else if (prim) {
-    irs ? k_stage3p<UM, true, true, RECT, false, double, KCT, TY><<<g, T, 0, c->stream>>>(a)
-        : k_stage3p<UM, true, false, RECT, false, double, KCT, TY><<<g, T, 0, c->stream>>>(a);
-  } else {
-    irs ? k_stage3p<UM, false, true, RECT, false, double, KCT, TY><<<g, T, 0, c->stream>>>(a)
-        : k_stage3p<UM, false, false, RECT, false, double, KCT, TY><<<g, T, 0, c->stream>>>(a);
-  }
-}
-
-// the LDS-ring kernel on the level's 4-row (ctiles) or 8-row (c8tiles) columns
-template <int KCT, int TY>
-void launch_stage3p_t(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool prim, bool irs) {
-  StageArgs<double> at = a;
-  at.ctiles = TY == 8 ? L.c8tiles.p : L.ctiles.p;
-  const unsigned g = (unsigned)(TY == 8 ? L.hc8tiles.size() : L.hctiles.size());
-  if (L.all_umat)
-    a.rect ? launch_stage3p<true, true, KCT, TY>(c, g, at, prim, irs)
-           : launch_stage3p<true, false, KCT, TY>(c, g, at, prim, irs);
-  else
-    a.rect ? launch_stage3p<false, true, KCT, TY>(c, g, at, prim, irs)
-           : launch_stage3p<false, false, KCT, TY>(c, g, at, prim, irs);
-}
-
-template <int KCT>
-void launch_stage3d_k(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang, bool prim, bool irs) {
-  const unsigned g = (unsigned)L.hctiles.size();
-  const bool row = L.row_march >= 0 ? L.row_march == 1 : !c->stage_lds;
-  if (tang) {
-    if (prim) { irs ? k_stage3d<true, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
-    else { irs ? k_stage3d<true, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
-  } else if (!row && !L.dims2) {
-    if (prim) { irs ? k_stage3d<false, true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
-    else { irs ? k_stage3d<false, false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
-  } else if (!row) {
-    L.stage_ty == 8 ? launch_stage3p_t<KCT, 8>(c, L, a, prim, irs) : launch_stage3p_t<KCT, 4>(c, L, a, prim, irs);
-  } else {
-    if (prim) { irs ? k_stage3w<true, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<true, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
-    else { irs ? k_stage3w<false, true, double, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3w<false, false, double, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
-  }
-}
-
-void launch_stage3d(fuss_ctx* c, Level& L, const StageArgs<double>& a, bool tang, bool prim, bool irs) {
-  if (L.kc == 4) launch_stage3d_k<4>(c, L, a, tang, prim, irs);
-  else if (L.kc == 16) launch_stage3d_k<16>(c, L, a, tang, prim, irs);
-  else if (L.kc == 32) launch_stage3d_k<32>(c, L, a, tang, prim, irs);
-  else if (L.kc == 64) launch_stage3d_k<64>(c, L, a, tang, prim, irs);
-  else launch_stage3d_k<8>(c, L, a, tang, prim, irs);
-}
-
-void launch_irs_z(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
-  const unsigned g = (unsigned)L.hctiles.size();
-  if (L.kc == 4) prim ? k_irs_z<true, double, 4><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 4><<<g, TILE_T, 0, c->stream>>>(ia);
-  else if (L.kc == 16) prim ? k_irs_z<true, double, 16><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 16><<<g, TILE_T, 0, c->stream>>>(ia);
-  else if (L.kc == 32) prim ? k_irs_z<true, double, 32><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 32><<<g, TILE_T, 0, c->stream>>>(ia);
-  else if (L.kc == 64) prim ? k_irs_z<true, double, 64><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 64><<<g, TILE_T, 0, c->stream>>>(ia);
-  else prim ? k_irs_z<true, double, 8><<<g, TILE_T, 0, c->stream>>>(ia) : k_irs_z<false, double, 8><<<g, TILE_T, 0, c->stream>>>(ia);
-}
-
-template <bool XCON, bool P2, bool DT, int TY>
-void launch_irs_xyz_x(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
-  const unsigned g = (unsigned)(TY == 8 ? L.hc8tiles.size() : L.hctiles.size());
-  constexpr int T = Xyz<TY>::TT;
-  if (L.kc == 4) prim ? k_irs_xyz<true, XCON, P2, DT, double, 4, TY><<<g, T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 4, TY><<<g, T, 0, c->stream>>>(ia);
-  else if (L.kc == 16) prim ? k_irs_xyz<true, XCON, P2, DT, double, 16, TY><<<g, T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 16, TY><<<g, T, 0, c->stream>>>(ia);
-  else if (L.kc == 32) prim ? k_irs_xyz<true, XCON, P2, DT, double, 32, TY><<<g, T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 32, TY><<<g, T, 0, c->stream>>>(ia);
-  else if (L.kc == 64) prim ? k_irs_xyz<true, XCON, P2, DT, double, 64, TY><<<g, T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 64, TY><<<g, T, 0, c->stream>>>(ia);
-  else prim ? k_irs_xyz<true, XCON, P2, DT, double, 8, TY><<<g, T, 0, c->stream>>>(ia) : k_irs_xyz<false, XCON, P2, DT, double, 8, TY><<<g, T, 0, c->stream>>>(ia);
-}
-
-template <bool XCON, bool P2, int TY>
-void launch_irs_xyz_p(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
-  ia.dtnext ? launch_irs_xyz_x<XCON, P2, true, TY>(c, L, ia, prim)
-            : launch_irs_xyz_x<XCON, P2, false, TY>(c, L, ia, prim);
-}
-
-template <int TY>
-void launch_irs_xyz_t(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
-  const bool p2 = ia.inv != 0.0;
-  if (L.no_xcon)
-    p2 ? launch_irs_xyz_p<false, true, TY>(c, L, ia, prim) : launch_irs_xyz_p<false, false, TY>(c, L, ia, prim);
-  else
-    p2 ? launch_irs_xyz_p<true, true, TY>(c, L, ia, prim) : launch_irs_xyz_p<true, false, TY>(c, L, ia, prim);
-}
-
-// P2: 1 + 2 beta is a power of two (beta = 0.5: 2), the sweeps multiply by
-// its exact reciprocal; DT: the pass also computes the next step's dt.  The
-// caller passes the tile list of L.xyz_ty (ctiles for 4 rows, c8tiles for 8).
-void launch_irs_xyz(fuss_ctx* c, Level& L, const IrsArgs<double>& ia, bool prim) {
-  if (L.xyz_ty == 8) launch_irs_xyz_t<8>(c, L, ia, prim);
-  else launch_irs_xyz_t<4>(c, L, ia, prim);
-}
-
-template <int DIR, int NDIR>
-void launch_irs_d(fuss_ctx* c, Level& L, const IrsArgs<double>& a, bool update, bool prim) {
-  const unsigned g = (unsigned)L.htiles.size();
-  if (!update) k_irs<DIR, NDIR, false, true, double><<<g, TILE_T, 0, c->stream>>>(a);
-  else if (prim) k_irs<DIR, NDIR, true, true, double><<<g, TILE_T, 0, c->stream>>>(a);
-  else k_irs<DIR, NDIR, true, false, double><<<g, TILE_T, 0, c->stream>>>(a);
-}
-
-template <int NDIR>
-void launch_irs(fuss_ctx* c, Level& L, int dir, const IrsArgs<double>& a, bool update, bool prim) {
-  if (dir == 0) launch_irs_d<0, NDIR>(c, L, a, update, prim);
-  else if (dir == 1) launch_irs_d<1, NDIR>(c, L, a, update, prim);
-  else launch_irs_d<2, NDIR>(c, L, a, update, prim);
-}
 
 // Line tiles and elimination factors of the tridiagonal IRS for beta (host
 // arithmetic in the oracle's order: den_1 = 1+2b, den_s = (1+2b) + b c_{s-1},
@@ -3913,42 +1759,6 @@ int tri_prepare(fuss_ctx* c, Level& L, double beta) {
   return 0;
 }
 
-// Residual_Smoothing with the tridiagonal lines: x, y (, z) passes
-// R -> R2 -> R (-> R2), each with the ghosts of the current R, then the RK
-// update.  Returns the tile list count of the update's residual partials.
-int launch_irs_tridiag(fuss_ctx* c, Level& L, IrsArgs<double> ia, bool prim, int* npart) {
-  TriArgs<double> ta{};
-  ta.blk = L.blk.p; ta.bc = L.bc.p; ta.fmap = L.fmap.p; ta.tri = L.tri.p; ta.trioff = L.trioff.p;
-  ta.beta = ia.beta; ta.halt = ia.halt;
-  double* rin = L.R.p;
-  double* rout = L.R2.p;
-  for (int d = 0; d < c->ndir; d++) {
-    ta.Rin = rin; ta.Rout = rout;
-    const int pid = prof_begin(c, FUSS_K_IRS);
-    if (d == 0) {
-      ta.tiles = L.ttx.p;
-      k_tri_x<double><<<(unsigned)L.htx.size(), 64, 0, c->stream>>>(ta);
-    } else if (d == 1) {
-      ta.tiles = L.tty.p;
-      k_tri_yz<1, double><<<(unsigned)L.hty.size(), TILE_T, 0, c->stream>>>(ta);
-    } else {
-      ta.tiles = L.ttz.p;
-      k_tri_yz<2, double><<<(unsigned)L.htz.size(), TILE_T, 0, c->stream>>>(ta);
-    }
-    prof_end(c, pid);
-    HIPCHK(hipGetLastError());
-    std::swap(rin, rout);
-  }
-  ia.Rin = rin;
-  ia.tiles = L.tiles.p;
-  const int pid = prof_begin(c, FUSS_K_IRS_UPDATE);
-  prim ? k_rk_update<true, double><<<(unsigned)L.htiles.size(), TILE_T, 0, c->stream>>>(ia)
-       : k_rk_update<false, double><<<(unsigned)L.htiles.size(), TILE_T, 0, c->stream>>>(ia);
-  prof_end(c, pid);
-  HIPCHK(hipGetLastError());
-  *npart = (int)L.htiles.size();
-  return 0;
-}
 
 const double RKC[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.50, 0.0}, {1.0, 0.25, 2.0 / 3.0}};
 
@@ -4109,27 +1919,17 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
       // (Mod_Explicit.f90:53-59)
       sa.Rres = L.R.p;
       const unsigned gt = (unsigned)L.htiles.size();
-      if (c->ndir == 3)
-        L.tang ? k_stage_raw<3, true, double><<<gt, TILE_T, 0, c->stream>>>(sa)
-               : k_stage_raw<3, false, double><<<gt, TILE_T, 0, c->stream>>>(sa);
-      else
-        L.tang ? k_stage_raw<2, true, double><<<gt, TILE_T, 0, c->stream>>>(sa)
-               : k_stage_raw<2, false, double><<<gt, TILE_T, 0, c->stream>>>(sa);
+      launch_stage_raw(c, L, sa, c->ndir, L.tang != 0);
       HIPCHK(hipGetLastError());
       const int eh = run_hook(c, L, level, irk, in, hk);
       if (eh) return eh;
-      if (prim)
-        irs ? k_newstate<true, true, double><<<gt, TILE_T, 0, c->stream>>>(sa)
-            : k_newstate<true, false, double><<<gt, TILE_T, 0, c->stream>>>(sa);
-      else
-        irs ? k_newstate<false, true, double><<<gt, TILE_T, 0, c->stream>>>(sa)
-            : k_newstate<false, false, double><<<gt, TILE_T, 0, c->stream>>>(sa);
+      launch_newstate(c, L, sa, prim, irs);
       if (!irs) { npart = (int)gt; L.plist = 0; }
     } else if (L.kp == GC) {
       launch_stage3d(c, L, sa, L.tang != 0, prim, irs);
       if (!irs) { npart = (int)nstcol; L.plist = stlist; }
     } else {
-      launch_stage_n<2>(c, L, sa, L.tang != 0, prim, irs);
+      launch_stage2d(c, L, sa, L.tang != 0, prim, irs);
     }
     prof_end(c, pid);
     HIPCHK(hipGetLastError());
@@ -4153,7 +1953,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
       ia.halt = halt;
       if (tri) {
         int np_ = 0;
-        const int e = launch_irs_tridiag(c, L, ia, prim, &np_);
+        const int e = launch_irs_tridiag(c, L, ia, L.R.p, L.R2.p, prim, &np_);
         if (e) return e;
         if (sa.last) { npart = np_; L.plist = 0; }
       } else if (xyz) {
@@ -4182,14 +1982,8 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
         ia.tiles = L.xytiles.p;
         const unsigned gxy = (unsigned)L.hxytiles.size();
         pid = prof_begin(c, c->ndir == 2 ? FUSS_K_IRS_UPDATE : FUSS_K_IRS);
-        if (c->ndir == 2) {
-          prim ? k_irs_xy<2, true, true, double><<<gxy, TILE_T, 0, c->stream>>>(ia)
-               : k_irs_xy<2, true, false, double><<<gxy, TILE_T, 0, c->stream>>>(ia);
-          if (sa.last) { npart = (int)gxy; L.plist = 2; }
-        } else {
-          prim ? k_irs_xy<3, false, true, double><<<gxy, TILE_T, 0, c->stream>>>(ia)
-               : k_irs_xy<3, false, false, double><<<gxy, TILE_T, 0, c->stream>>>(ia);
-        }
+        launch_irs_xy(c, L, c->ndir, ia, prim);
+        if (c->ndir == 2 && sa.last) { npart = (int)gxy; L.plist = 2; }
         prof_end(c, pid);
         HIPCHK(hipGetLastError());
         if (c->ndir == 3) {
@@ -4209,8 +2003,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
           ia.Rin = rin; ia.Rout = rout;
           const bool upd = d == c->ndir - 1;
           pid = prof_begin(c, upd ? FUSS_K_IRS_UPDATE : FUSS_K_IRS);
-          if (c->ndir == 2) launch_irs<2>(c, L, d, ia, upd, prim);
-          else launch_irs<3>(c, L, d, ia, upd, prim);
+          launch_irs_line(c, L, c->ndir, d, ia, upd, prim);
           prof_end(c, pid);
           HIPCHK(hipGetLastError());
           std::swap(rin, rout);
@@ -5077,7 +2870,7 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
     if (e0) return e0;
   }
   const bool hit = L.run_exec && L.run_seq[0] == L.cur && !memcmp(&L.run_p, &q, sizeof(q)) &&
-                   L.run_thr == res_threshold && L.run_tthr == time_threshold;
+                   L.run_thr == res_threshold && L.run_tthr == time_threshold && L.run_bufs == run_buffers(c, L);
   int* cur_seq = L.run_seq;
   if (!hit) {
     run_graph_free(L);
@@ -5109,6 +2902,7 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
       return FUSS_EHIP;
     }
     L.run_graph = graph;
+    L.run_bufs = run_buffers(c, L);
     L.run_p = q;
     L.run_thr = res_threshold;
     L.run_tthr = time_threshold;

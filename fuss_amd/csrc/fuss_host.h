@@ -1,0 +1,233 @@
+// fuss_host.h -- host-side context, level and buffer types of the FUSS
+// explicit step library, and the launchers each kernel unit provides.
+#pragma once
+
+#include <vector>
+
+#include "fuss_dev.h"
+#include "fuss_refine_types.h"
+
+namespace fuss {
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { free(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { free(); }
+  int alloc(size_t count) {
+    free();
+    n = count;
+    if (!count) return 0;
+    if (hipMalloc(&p, sizeof(T) * count) != hipSuccess) { p = nullptr; n = 0; return FUSS_EHIP; }
+    // the zero fill runs on the null stream, which the library's non-blocking
+    // stream does not wait for: finish it before any kernel on that stream
+    // writes the buffer (else the fill can land after, over the results)
+    if (hipMemset(p, 0, sizeof(T) * count) != hipSuccess) return FUSS_EHIP;
+    if (hipStreamSynchronize(nullptr) != hipSuccess) return FUSS_EHIP;
+    return 0;
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct HostBlock {
+  int n[3] = {0, 0, 0};
+  std::vector<double> nodes, T, matID, qvol;
+};
+
+struct Level {
+  int nb = 0;
+  int kp = 2;                       // k ghost layers (1 in 2-D: k-face data only)
+  std::vector<BlockInfo> hblk;
+  std::vector<HostBlock> hb;        // host nodes (every level), fields (level 1)
+  std::vector<Tile> htiles;         // one 64x4 cell tile per plane
+  std::vector<Tile> hctiles;        // 64x4 columns of kc planes (3-D levels)
+  std::vector<Tile> hc8tiles;       // 64x8 columns of kc planes (fused x-y-z IRS pass)
+  int kc = KC_DEFAULT;              // planes per column tile (fuss_autotune)
+  int xyz_ty = 4;                   // rows of the fused IRS pass's columns, 4 or 8 (fuss_autotune)
+  int stage_ty = 4;                 // rows of the LDS-ring stage kernel's columns, 4 or 8 (fuss_autotune)
+  int dt_ty = 4;                    // ... of the pass that wrote the dt minima (dt_next)
+  int row_march = -1;               // orthogonal 3-D stage kernel: -1 context default, 0 LDS ring, 1 row march
+  std::vector<Tile> hxytiles;       // 64x16 plane tiles of the x-y IRS pass
+  std::vector<int> hrng[4];         // per tile list (tiles, ctiles, xytiles, c8tiles): first tile of each block
+  DBuf<int> rng[4];
+  int plist = 0;                    // tile list of the last step's residual partials
+  int no_ycon = 1;                  // no connection on a j face (fused x-y IRS allowed)
+  int no_zcon = 1;                  // no connection on a k face (with no_ycon: fused x-y-z IRS)
+  int no_xcon = 1;                  // no connection on an i face
+  // line-tridiagonal IRS: line tiles (x: 64 rows of a plane; y, z: 64x4
+  // line origins) and the per-block elimination factors for tri_beta
+  std::vector<Tile> htx, hty, htz;
+  DBuf<Tile> ttx, tty, ttz;
+  DBuf<double2> tri;
+  DBuf<int> trioff;
+  double tri_beta = -1.0;
+  int ghost_read = 1;               // some BC entry reads layer-1 ghosts (101, 300, 302)
+  int dt_next = 0;                  // dt and its per-tile minima hold the next step's (fused IRS pass)
+  double dt_vnn = 0.0;              // ... computed with this vnn
+  int all_umat = 0;                 // every block has a single material
+  int dims2 = 0;                    // every block has >= 2 cells in each direction
+  // rectilinear levels: per-axis metric factors (RectAxes) that reproduce the
+  // stored volumes and normal weights to rect_dev (relative) and sum 1/dl^2 exactly
+  DBuf<double> rfac;
+  int rect_w = 0;                   // every block rectilinear, factors within RECT_TOL
+  int rect_dt = 0;                  // ... and sum 1/dl^2 bit-identical
+  double rect_dev = -1.0;           // largest relative deviation found (-1: not checked)
+  long npad = 0, ncell = 0, nnodes = 0;
+  DBuf<BlockInfo> blk;
+  DBuf<Tile> tiles, ctiles, xytiles, c8tiles;
+  DBuf<double> nodes;
+  DBuf<double> Tb[3];               // T^n and two stage buffers
+  int cur = 0;                      // index of T^n in Tb
+  DBuf<double> R, R2, dt, qvol, vol, invL2;
+  DBuf<int> mat;
+  DBuf<double> M, F[3], W[9];
+  DBuf<BCRec> bc;
+  DBuf<BCHot> bch;                  // compact wall records (k_bcflux)
+  DBuf<int2> gmap;                  // (ghost, source) per entry (k_ghost)
+  DBuf<int> fmap;
+  DBuf<double> bcv, bcf;
+  int nbound = 0;
+  std::vector<int> hib;             // nbound x 15
+  // chimera (102) donor lists (fuss_set_chimera, host): per listed entry
+  // its BC index and donor counts, then 4 ints (b, i, j, k) and a volume
+  // fraction per donor, entries in list order
+  std::vector<int> hch_ent, hch_n1, hch_n2, hch_don;
+  std::vector<double> hch_vf;
+  // device: per chimera entry (ghost cell, first donor, layer-1 donors,
+  // material index), the donors' arena indices and fractions
+  DBuf<int4> chr;
+  DBuf<int> chd;
+  DBuf<double> chv;
+  int nchim = 0;
+  std::vector<double> hrb;          // nbound x 6
+  std::vector<int> hser;
+  int tang = 0, any_active = 0;
+  double time = 0.0;                // host mirrors of the device clock
+  int iter = 0;
+  DBuf<double> clk;                 // [0] level time
+  DBuf<int> cnt;                    // [0] iteration, [1] halt, [2] run steps, [3] chunk start, [4] run step limit
+  // fuss_run's captured chunk of RUN_G steps, replayed by later calls with the
+  // same step parameters and stop thresholds (dropped when tiles change)
+  hipGraph_t run_graph = nullptr;
+  hipGraphExec_t run_exec = nullptr;
+  fuss_step_params run_p{};
+  double run_thr = 0.0, run_tthr = 0.0;
+  int run_seq[RUN_G + 1] = {};
+  std::vector<const void*> run_bufs;  // device buffers the captured steps address (checked before a replay)
+  DBuf<double> partials, pmin, red2;
+};
+
+// One level of the correction multigrid (arrays in the inner precision).
+struct CLevel {
+  int ncell = 0, maxcells = 0;
+  std::vector<CBlk> hblk;
+  DBuf<CBlk> blk;
+  DBuf<unsigned char> mem;
+  void* d[3] = {nullptr, nullptr, nullptr};
+  void *b = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *D = nullptr, *s = nullptr, *s2 = nullptr;
+  int cur = 0;
+};
+
+struct Refine {
+  int ready = 0;
+  fuss_refine_params p{};
+  std::vector<CLevel> lv;
+  DBuf<double> bcd, red;
+  DBuf<double> bres;                // per-block monitor residual (independent cases)
+  double* hbres = nullptr;          // pinned
+  std::vector<int> conv_outer;      // first outer iteration at which each block met the threshold
+  std::vector<double> res;          // last monitor residual per block
+};
+
+// Checkpointed forward and reverse sweep of the time-accurate step.
+struct Adjoint {
+  int ready = 0, prec = FUSS_FP64;
+  int nsteps = 0, K = 0, nprobe = 0, forward_done = 0;
+  std::vector<long> hprobe;
+  DBuf<long> probe, coff;
+  DBuf<double> ckpt;                // (nsteps / K + 1) states
+  DBuf<double> seg;                 // the K states of one segment
+  DBuf<double> stage[4];            // stage inputs of the recomputed step
+  DBuf<unsigned char> lam;          // lam, lin, lTO in the adjoint precision
+  DBuf<double> part, red, acc;      // partial sums; acc[0] dL/d(dt), acc[1] dL/dc
+  DBuf<int> amin;
+  std::vector<double> times;        // level time before each step
+  std::vector<int> iters;
+  DBuf<double> dtimes;              // the same, recorded on the device during the forward
+  DBuf<int> diters;
+  fuss_step_params p{};
+};
+
+}  // namespace fuss
+
+struct fuss_ctx {
+  int nlev = 0, nb = 0, precision = FUSS_FP64, ndir = 0;
+  std::vector<Level> lev;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int nmat = 0, Ti1 = 0, nT = 0;
+  std::vector<double> tcp, trho, tkap, th;
+  DBuf<double> dcp, drho, dkap, dh, dkap2, dpr4;
+  struct Series { int periodic; std::vector<double> t, v; };
+  std::vector<Series> series;
+  DBuf<int> status;                 // code, b, i, j, k
+  DBuf<double> scal;                // dtglobal, time, res_sq
+  double* hscal = nullptr;          // pinned readback
+  int setup_done = 0;
+  int* hstatus = nullptr;
+  Refine ref;                       // steady defect correction (fuss_refine)
+  Adjoint adj;                      // discrete adjoint (fuss_adj_*)
+  int adj_store_on = 0;             // enqueue_step copies each stage input to adj.stage[]
+  int graphs_off = 0;               // fuss_run without graph capture (fuss_graphs)
+  int stage_lds = 1;                // orthogonal 3-D stages: LDS-ring kernel (1) or row march (0)
+  int rect_off = 0;                 // per-axis metric factors switched off (fuss_rect)
+  double kscale = 1.0;              // conductivity table scale (fuss_set_kscale)
+  DBuf<int> stage_i;                // staging for fuss_set_bc_values
+  DBuf<double> stage_d;
+  DBuf<double> ser_t, ser_v;        // time series on the device
+  DBuf<int> ser_off, ser_n, ser_per;
+  DBuf<double> run_hist;            // per-chunk residual / time history of fuss_run
+  double* hrun = nullptr;           // pinned: run history readback
+  int* hcnt = nullptr;              // pinned: level counters readback
+  // per-kernel-class timing with HIP events on the context stream
+  struct Prof {
+    int enabled = 0;
+    std::vector<hipEvent_t> pool;
+    std::vector<int> cls;       // class of each recorded (start, end) pair
+    size_t used = 0;
+  } prof;
+};
+
+namespace fuss {
+
+// Event pair around one launch of a kernel class (profiling mode only).
+int prof_begin(fuss_ctx* c, int cls);
+void prof_end(fuss_ctx* c, int id);
+
+// Launchers (one per kernel family; defined in the stage_*.hip and irs_*.hip units).
+template <typename R> void launch_stage2d(fuss_ctx* c, Level& L, const StageArgs<R>& a, bool tang, bool prim, bool irs);
+template <typename R> void launch_stage3d(fuss_ctx* c, Level& L, const StageArgs<R>& a, bool tang, bool prim, bool irs);
+template <typename R, int TY> void launch_stage3p_ring(fuss_ctx* c, Level& L, const StageArgs<R>& a, bool prim, bool irs);
+template <typename R> void launch_stage_raw(fuss_ctx* c, Level& L, const StageArgs<R>& a, int ndir, bool tang);
+template <typename R> void launch_newstate(fuss_ctx* c, Level& L, const StageArgs<R>& a, bool prim, bool irs);
+template <typename R> void launch_irs_xyz(fuss_ctx* c, Level& L, const IrsArgs<R>& ia, bool prim);
+template <typename R, int TY> void launch_irs_xyz_ty(fuss_ctx* c, Level& L, const IrsArgs<R>& ia, bool prim);
+template <typename R> void launch_irs_z(fuss_ctx* c, Level& L, const IrsArgs<R>& ia, bool prim);
+template <typename R> void launch_irs_line(fuss_ctx* c, Level& L, int ndir, int dir, const IrsArgs<R>& a, bool update, bool prim);
+template <typename R> void launch_irs_xy(fuss_ctx* c, Level& L, int ndir, const IrsArgs<R>& ia, bool prim);
+template <typename R> int launch_irs_tridiag(fuss_ctx* c, Level& L, IrsArgs<R> ia, R* rin, R* rout, bool prim, int* npart);
+
+}  // namespace fuss
+

@@ -29,15 +29,6 @@
 
 namespace fuss {
 
-struct CBlk {
-  int n[3];     // cells of the block at this level
-  int r[3];     // aggregation ratio from the finer level per direction (1 or 2)
-  int nf[3];    // cells of the block at the finer level
-  int off;      // first cell of the block in this level's arrays
-  int foff;     // first cell of the block at the finer level
-  int pad_;
-};
-
 template <typename S>
 struct COp {
   const CBlk* blk;
