@@ -73,6 +73,7 @@ struct StageArgs {
   R rkc;
   int jam;             // four-stage Jameson update (n_rk = 4)
   int last;
+  int irk;             // stage number (1-based) of the step, for the state-check location
   double* partials;
   int* status;
   const double* rect;  // per-axis metric factors (rectilinear levels, k_stage3p<RECT>)
@@ -412,6 +413,7 @@ struct IrsArgs {
   R rkc;
   int jam;
   int last;
+  int irk;             // stage number of the step (state-check location)
   double* partials;
   int* status;
   const int* halt;

@@ -1232,9 +1232,21 @@ int build_bc(fuss_ctx* c, Level& L) {
 }
 
 int sync_status(fuss_ctx* c, int* bijk = nullptr) {
-  HIPCHK(hipMemcpyAsync(c->hstatus, c->status.p, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(c->hstatus, c->status.p, 8 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (bijk) for (int q = 0; q < 4; q++) bijk[q] = c->hstatus[1 + q];
+  unsigned long long key;
+  memcpy(&key, c->hstatus + 6, sizeof(key));
+  if (bijk) {
+    if (c->hstatus[0] == FUSS_ESTATE && key != ~0ULL) {
+      // first failing cell in the reference's loop order (flag_state)
+      bijk[0] = (int)((key >> 48) & 0x1fff);
+      bijk[1] = (int)(key & 0xffff);
+      bijk[2] = (int)((key >> 16) & 0xffff);
+      bijk[3] = (int)((key >> 32) & 0xffff);
+    } else {
+      for (int q = 0; q < 4; q++) bijk[q] = c->hstatus[1 + q];
+    }
+  }
   return c->hstatus[0];
 }
 
@@ -1505,7 +1517,8 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
             d[0] = s[0]; d[1] = s[1]; d[2] = s[2];
           }
     }
-  HIPCHK(hipMemsetAsync(c->status.p, 0, 8 * sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->status.p, 0, 6 * sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->status.p + 6, 0xff, 2 * sizeof(int), c->stream));  // no failing cell yet
   for (int m = 0; m < c->nlev; m++) {
     Level& L = c->lev[m];
     // 2-D levels keep one plane on each side: it stores the k-face data
@@ -1912,6 +1925,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
     sa.rkc = nrk == 4 ? RK4A[irk - 1] : RKC[nrk - 1][irk - 1];
     sa.jam = nrk == 4;
     sa.last = irk == nrk && p->compute_residual;
+    sa.irk = irk;
     sa.ctiles = L.ctiles.p;
     pid = prof_begin(c, (sdt && irk == 1) ? FUSS_K_STAGE_DT : FUSS_K_STAGE);
     if (hk) {
@@ -1949,7 +1963,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
         ia.inv = exact_inverse(ia.den);
       }
       ia.T = in; ia.TO = L.Tb[o].p; ia.Tout = out; ia.mat = L.mat.p; ia.tab = dtab<double>(c);
-      ia.rkc = sa.rkc; ia.jam = sa.jam; ia.last = sa.last; ia.partials = L.partials.p; ia.status = c->status.p;
+      ia.rkc = sa.rkc; ia.jam = sa.jam; ia.last = sa.last; ia.irk = irk; ia.partials = L.partials.p; ia.status = c->status.p;
       ia.halt = halt;
       if (tri) {
         int np_ = 0;

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs(IrsArgs<R> a) {
       int bad = 0;
       const R Tn = new_state<PRIM>(a.tab, a.mat[p], T, TO, rs, a.rkc, a.jam, &bad);
       a.Tout[p] = Tn;
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+      if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
       if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
       const double d = (double)Tn - (double)TO;
       rsq = d * d;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_xy(IrsArgs<R> a) {
       const int m = B.umat >= 0 ? B.umat : ldu(a.mat, p);
       const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
       stu(a.Tout, p, Tn);
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+      if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
       if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
       const double d = (double)Tn - (double)TO;
       rsq += d * d;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(TILE_T) void k_rk_update(IrsArgs<R> a) {
     const int m = B.umat >= 0 ? B.umat : ldu(a.mat, p);
     const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
     stu(a.Tout, p, Tn);
-    if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+    if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
     if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
     const double dd = (double)Tn - (double)TO;
     rsq = dd * dd;
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(TILE_T) void k_irs_z(IrsArgs<R> a) {
       int bad = 0;
       const R Tn = new_state<PRIM>(a.tab, m >= 0 ? m : ldu(a.mat, p), T, TO, rs, a.rkc, a.jam, &bad);
       stu(a.Tout, p, Tn);
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+      if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
       if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
       const double d = (double)Tn - (double)TO;
       rsq += d * d;
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(Xyz<TY>::TT) void k_irs_xyz(IrsArgs<R> a) {
       int bad = 0;
       const R Tn = new_state<PRIM>(a.tab, m, T, TO, rs, a.rkc, a.jam, &bad);
       stn(a.Tout, po, Tn);
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, ko);
+      if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, ko);
       if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, ko);
       const double d = (double)Tn - (double)TO;
       rsq += d * d;

@@ -14,6 +14,7 @@
 //   Mbar = (M_cell + M_nb)/2 -- the flux is then kappa * sum_c G_c W[3D+c].
 #pragma once
 
+#include "../../include/fuss_gpu.h"
 #include "fuss_common.h"
 
 namespace fuss {
@@ -177,6 +178,19 @@ __device__ __forceinline__ void flag_error(int* st, int code, int b, int i, int 
     st[3] = j;
     st[4] = k;
   }
+}
+
+// Check_And_Fix_State failure (Lib_Newstate.f90:278-290): the reference
+// stops at the first failing cell of its loops (stage, then block, k, j, i).
+// Every failing cell offers the packed key (stage, b, k, j, i) to an atomic
+// minimum at status[6..7] (initialised to all ones), so the reported cell is
+// that first one whatever order the work-groups run in.
+__device__ __forceinline__ void flag_state(int* st, int irk, int b, int i, int j, int k) {
+  atomicCAS(st, 0, FUSS_ESTATE);
+  const unsigned long long key = ((unsigned long long)(irk & 7) << 61) | ((unsigned long long)((b + 1) & 0x1fff) << 48) |
+                                 ((unsigned long long)(k & 0xffff) << 32) | ((unsigned long long)(j & 0xffff) << 16) |
+                                 (unsigned long long)(i & 0xffff);
+  atomicMin(reinterpret_cast<unsigned long long*>(st + 6), key);
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage(StageArgs<R> a) {
       const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, a.jam, &bad);
       a.Tout[p] = Tn;
       // Check_And_Fix_State, Lib_Newstate.f90:278-290
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+      if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
       const double d = (double)Tn - (double)TO;
       rsq = d * d;
     }
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(TILE_T) void k_newstate(StageArgs<R> a) {
       const R TO = a.TO[p];
       const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, a.jam, &bad);
       a.Tout[p] = Tn;
-      if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+      if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
       const double d = (double)Tn - (double)TO;
       rsq = d * d;
     }
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3d(StageArgs<R> a) {
         const R TO = c_TO;
         const R Tn = new_state<PRIM>(a.tab, m, T, TO, res, a.rkc, a.jam, &bad);
         stu(a.Tout, p, Tn);
-        if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+        if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
         const double d = (double)Tn - (double)TO;
         rsq += d * d;
       }
@@ -643,7 +643,7 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
       } else {
         const R Tn = new_state<PRIM>(a.tab, cur.m, Tc, cur.TO, res, a.rkc, a.jam, &bad2);
         vpend = Tn;
-        if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+        if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
         const double d = (double)Tn - (double)cur.TO;
         rsq += d * d;
       }
@@ -810,7 +810,7 @@ __global__ __launch_bounds__(TILE_T) void k_stage3w(StageArgs<R> a) {
       } else {
         const R Tn = new_state<PRIM>(a.tab, cur.m, cur.T, cur.TO, res, a.rkc, a.jam, &bad);
         stu(a.Tout, p, Tn);
-        if (Tn != Tn || Tn < (R)0) flag_error(a.status, FUSS_ESTATE, t.b, i, j, k);
+        if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
         const double d = (double)Tn - (double)cur.TO;
         rsq += d * d;
       }
