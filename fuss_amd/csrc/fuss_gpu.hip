@@ -26,6 +26,8 @@
 #include <vector>
 
 #include "../../include/fuss_gpu.h"
+#include <type_traits>
+
 #include "fuss_host.h"
 
 // ------------------------------------------------------------------------
@@ -231,6 +233,16 @@ __global__ void k_chimera(double* __restrict__ T, const int4* __restrict__ r, co
   }
   T[q.x] = tab_inverse_search(tab.h + (long)q.w * tab.nT, consg, tab.Ti1, tab.Ti2, &bad);
   if (bad) flag_error(status, FUSS_ETABLE, -1, 0, 0, 0);
+}
+
+// Precision conversion of a padded field (FP32 explicit steps).
+__global__ __launch_bounds__(256) void k_d2f(const double* __restrict__ src, float* __restrict__ dst, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+__global__ __launch_bounds__(256) void k_f2d(const float* __restrict__ src, double* __restrict__ dst, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dst[i] = (double)src[i];
 }
 
 // Compute_dt, Mod_dt.f90:26-84
@@ -955,6 +967,11 @@ std::vector<const void*> run_buffers(const fuss_ctx* c, const Level& L) {
       c->status.p, c->scal.p, c->run_hist.p, c->ser_t.p, c->ser_v.p, c->ser_off.p, c->ser_n.p, c->ser_per.p};
   for (const auto& w : L.W) v.push_back(w.p);
   for (const auto& r : L.rng) v.push_back(r.p);
+  const auto& F = L.f;
+  for (const auto* q : {F.Tb[0].p, F.Tb[1].p, F.Tb[2].p, F.R.p, F.R2.p, F.dt.p, F.qvol.p, F.vol.p, F.bcv.p, F.bcf.p, F.W[0].p, F.W[1].p, F.W[2].p, F.W[3].p, F.W[4].p, F.W[5].p, F.W[6].p,
+                        F.W[7].p, F.W[8].p})
+    v.push_back(q);
+  for (const auto* q : {c->fcp.p, c->frho.p, c->fkap.p, c->fh.p, c->fkap2.p, c->fpr4.p}) v.push_back(q);
   return v;
 }
 
@@ -1299,21 +1316,126 @@ namespace {
 
 
 template <typename R>
-Tab<R> dtab(fuss_ctx* c) {
-  Tab<R> t;
+Tab<R> dtab(fuss_ctx* c);
+template <>
+Tab<double> dtab<double>(fuss_ctx* c) {
+  Tab<double> t;
   t.cp = c->dcp.p; t.rho = c->drho.p; t.kap = c->dkap.p; t.h = c->dh.p;
   t.kap2 = c->dkap2.p; t.pr4 = c->dpr4.p;
   t.nT = c->nT; t.Ti1 = c->Ti1; t.Ti2 = c->Ti1 + c->nT - 1;
   return t;
 }
+// the FP32 copies of the tables (converted by f32_prepare)
+template <>
+Tab<float> dtab<float>(fuss_ctx* c) {
+  Tab<float> t;
+  t.cp = c->fcp.p; t.rho = c->frho.p; t.kap = c->fkap.p; t.h = c->fh.p;
+  t.kap2 = c->fkap2.p; t.pr4 = c->fpr4.p;
+  t.nT = c->nT; t.Ti1 = c->Ti1; t.Ti2 = c->Ti1 + c->nT - 1;
+  return t;
+}
 
-int launch_ghost(fuss_ctx* c, Level& L, double* T) {
+template <typename R>
+FieldView<R> fields(fuss_ctx* c, Level& L);
+template <>
+FieldView<double> fields<double>(fuss_ctx* c, Level& L) {
+  FieldView<double> v;
+  for (int q = 0; q < 3; q++) v.Tb[q] = L.Tb[q].p;
+  v.res = L.R.p; v.res2 = L.R2.p; v.dt = L.dt.p; v.qvol = L.qvol.p; v.vol = L.vol.p; v.invL2 = L.invL2.p;
+  for (int w = 0; w < 9; w++) v.W[w] = L.W[w].p;
+  v.bcv = L.bcv.p; v.bcf = L.bcf.p;
+  (void)c;
+  return v;
+}
+template <>
+FieldView<float> fields<float>(fuss_ctx* c, Level& L) {
+  FieldView<float> v;
+  for (int q = 0; q < 3; q++) v.Tb[q] = L.f.Tb[q].p;
+  v.res = L.f.R.p; v.res2 = L.f.R2.p; v.dt = L.f.dt.p; v.qvol = L.f.qvol.p; v.vol = L.f.vol.p;
+  v.invL2 = L.invL2.p;
+  for (int w = 0; w < 9; w++) v.W[w] = L.f.W[w].p;
+  v.bcv = L.f.bcv.p; v.bcf = L.f.bcf.p;
+  (void)c;
+  return v;
+}
+
+int convert_d2f(fuss_ctx* c, const double* src, float* dst, size_t n) {
+  if (!n) return 0;
+  k_d2f<<<(unsigned)std::min<size_t>((n + 255) / 256, 8192), 256, 0, c->stream>>>(src, dst, (long)n);
+  return hipGetLastError() == hipSuccess ? 0 : FUSS_EHIP;
+}
+int convert_f2d(fuss_ctx* c, const float* src, double* dst, size_t n) {
+  if (!n) return 0;
+  k_f2d<<<(unsigned)std::min<size_t>((n + 255) / 256, 8192), 256, 0, c->stream>>>(src, dst, (long)n);
+  return hipGetLastError() == hipSuccess ? 0 : FUSS_EHIP;
+}
+
+// FP32 mirrors of a level's static fields and of the tables (allocated and
+// converted on the first FP32 step, the tables again after a change).
+// Outside stream capture.
+int f32_prepare(fuss_ctx* c, Level& L) {
+  if (!c->ftab_ok) {
+    const size_t n = (size_t)c->nmat * c->nT;
+    if (c->fcp.n != n && (c->fcp.alloc(n) || c->frho.alloc(n) || c->fkap.alloc(n) || c->fh.alloc(n) ||
+                          c->fkap2.alloc(2 * n) || c->fpr4.alloc(4 * n)))
+      return FUSS_EHIP;
+    int e = convert_d2f(c, c->dcp.p, c->fcp.p, n) | convert_d2f(c, c->drho.p, c->frho.p, n) |
+            convert_d2f(c, c->dkap.p, c->fkap.p, n) | convert_d2f(c, c->dh.p, c->fh.p, n) |
+            convert_d2f(c, c->dkap2.p, c->fkap2.p, 2 * n) | convert_d2f(c, c->dpr4.p, c->fpr4.p, 4 * n);
+    if (e) return FUSS_EHIP;
+    c->ftab_ok = 1;
+  }
+  if (L.f_ready) return 0;
+  const size_t n = (size_t)L.npad;
+  auto& F = L.f;
+  for (auto& t : F.Tb)
+    if (t.alloc(n)) return FUSS_EHIP;
+  if (F.R.alloc(n) || F.R2.alloc(n) || F.dt.alloc(n) || F.qvol.alloc(n) || F.vol.alloc(n)) return FUSS_EHIP;
+  for (int w = 0; w < 9; w++)
+    if (F.W[w].alloc(L.W[w].n)) return FUSS_EHIP;
+  if (F.bcv.alloc(L.bcv.n) || F.bcf.alloc(L.bcf.n)) return FUSS_EHIP;
+  int e = convert_d2f(c, L.qvol.p, F.qvol.p, n) | convert_d2f(c, L.vol.p, F.vol.p, n);
+  for (int w = 0; w < 9; w++) e |= convert_d2f(c, L.W[w].p, F.W[w].p, L.W[w].n);
+  if (e) return FUSS_EHIP;
+  L.f_ready = 1;
+  return 0;
+}
+
+// The level's current state into the FP32 buffers (ghosts included), or
+// back into the FP64 ones; the next step then computes its dt afresh.
+int state_to_f32(fuss_ctx* c, Level& L) {
+  int e = f32_prepare(c, L);
+  if (e || L.state_f) return e;
+  e = convert_d2f(c, L.Tb[L.cur].p, L.f.Tb[L.cur].p, L.npad);
+  L.state_f = 1;
+  L.dt_next = 0;
+  return e;
+}
+int state_to_f64(fuss_ctx* c, Level& L) {
+  if (!L.state_f) return 0;
+  const int e = convert_f2d(c, L.f.Tb[L.cur].p, L.Tb[L.cur].p, L.npad);
+  L.state_f = 0;
+  L.dt_next = 0;
+  return e;
+}
+int all_to_f64(fuss_ctx* c) {
+  for (auto& L : c->lev) {
+    const int e = state_to_f64(c, L);
+    if (e) return e;
+  }
+  return 0;
+}
+
+template <typename R>
+int launch_ghost(fuss_ctx* c, Level& L, R* T) {
   if (!L.nbound) return 0;
   const int pid = prof_begin(c, FUSS_K_GHOST);
-  k_ghost<double><<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(T, L.gmap.p, L.nbound, L.cnt.p + 1);
-  if (L.nchim)
-    k_chimera<<<(L.nchim + 127) / 128, 128, 0, c->stream>>>(T, L.chr.p, L.chd.p, L.chv.p, L.nchim, L.mat.p,
-                                                            dtab<double>(c), c->status.p, L.cnt.p + 1);
+  k_ghost<R><<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(T, L.gmap.p, L.nbound, L.cnt.p + 1);
+  if constexpr (std::is_same<R, double>::value) {
+    if (L.nchim)
+      k_chimera<<<(L.nchim + 127) / 128, 128, 0, c->stream>>>(T, L.chr.p, L.chd.p, L.chv.p, L.nchim, L.mat.p,
+                                                              dtab<double>(c), c->status.p, L.cnt.p + 1);
+  }
   prof_end(c, pid);
   return hipGetLastError() == hipSuccess ? 0 : FUSS_EHIP;
 }
@@ -1821,8 +1943,10 @@ int run_hook(fuss_ctx* c, Level& L, int level, int irk, const double* Tin, const
   return 0;
 }
 
-int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageHook* hk = nullptr) {
+template <typename R>
+int enqueue_step_t(fuss_ctx* c, int level, const fuss_step_params* p, const StageHook* hk) {
   Level& L = c->lev[level - 1];
+  const FieldView<R> V = fields<R>(c, L);
   const int nrk = p->n_rk;
   if (nrk < 1 || nrk > 4) return FUSS_EARG;
   const bool prim = p->integration == FUSS_PRIM;
@@ -1859,12 +1983,12 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
     nmin = (unsigned)(L.dt_ty == 8 ? L.hc8tiles.size() : L.hctiles.size());
   } else {
     // Compute_dt + min (Mod_dt.f90:10-84); dtglobal starts at 1e5 (Mod_Explicit.f90:40)
-    DtArgs<double> da{};
-    da.blk = L.blk.p; da.tiles = L.tiles.p; da.T = L.Tb[L.cur].p; da.mat = L.mat.p; da.invL2 = L.invL2.p;
-    da.dt = L.dt.p; da.tab = dtab<double>(c); da.vnn = p->vnn; da.cnt = L.cnt.p; da.rampa = p->rampa_iter;
+    DtArgs<R> da{};
+    da.blk = L.blk.p; da.tiles = L.tiles.p; da.T = V.Tb[L.cur]; da.mat = L.mat.p; da.invL2 = V.invL2;
+    da.dt = V.dt; da.tab = dtab<R>(c); da.vnn = p->vnn; da.cnt = L.cnt.p; da.rampa = p->rampa_iter;
     da.pmin = L.pmin.p; da.status = c->status.p;
     pid = prof_begin(c, FUSS_K_DT);
-    k_dt<double><<<ntile, TILE_T, 0, c->stream>>>(da);
+    k_dt<R><<<ntile, TILE_T, 0, c->stream>>>(da);
     prof_end(c, pid);
     HIPCHK(hipGetLastError());
   }
@@ -1891,34 +2015,36 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
   }
   L.dt_next = 0;
   const int o = L.cur, a1 = (L.cur + 1) % 3, a2 = (L.cur + 2) % 3;
-  double* in = L.Tb[o].p;
+  R* in = V.Tb[o];
   int npart = (int)ntile;  // partial sums written by the kernel that applies the last update
   L.plist = 0;
   for (int irk = 1; irk <= nrk; irk++) {
-    double* out = (irk % 2 == 1) ? L.Tb[a1].p : L.Tb[a2].p;
+    R* out = (irk % 2 == 1) ? V.Tb[a1] : V.Tb[a2];
     // Fill_Ghost_Cell: only read by tangential gradients (non-orthogonal
     // blocks; 300/302 walls) and connections; a level with neither skips it
     int e = (L.tang || L.ghost_read || c->adj_store_on) ? launch_ghost(c, L, in) : 0;
     if (e) return e;
-    if (c->adj_store_on)
-      HIPCHK(hipMemcpyAsync(c->adj.stage[irk - 1].p, in, L.npad * sizeof(double), hipMemcpyDeviceToDevice,
-                            c->stream));
-    StageArgs<double> sa{};
-    sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.T = in; sa.TO = L.Tb[o].p; sa.Tout = out; sa.Rres = L.R.p;
-    sa.dt = L.dt.p; sa.dtg = c->scal.p; sa.use_dtg = p->time_accurate; sa.qvol = L.qvol.p; sa.vol = L.vol.p;
+    if constexpr (std::is_same<R, double>::value) {
+      if (c->adj_store_on)
+        HIPCHK(hipMemcpyAsync(c->adj.stage[irk - 1].p, in, L.npad * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
+    }
+    StageArgs<R> sa{};
+    sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.T = in; sa.TO = V.Tb[o]; sa.Tout = out; sa.Rres = V.res;
+    sa.dt = V.dt; sa.dtg = c->scal.p; sa.use_dtg = p->time_accurate; sa.qvol = V.qvol; sa.vol = V.vol;
     sa.mat = L.mat.p;
-    for (int w = 0; w < 9; w++) sa.W[w] = L.W[w].p;
-    sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = L.bcv.p; sa.bcf = L.bcf.p; sa.tab = dtab<double>(c);
+    for (int w = 0; w < 9; w++) sa.W[w] = V.W[w];
+    sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = V.bcv; sa.bcf = V.bcf; sa.tab = dtab<R>(c);
     sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = halt;
     sa.partials = L.partials.p; sa.status = c->status.p;  // k_bcflux flags table errors through status
     sa.rect = (L.rect_w && !c->rect_off) ? L.rfac.p : nullptr;
     if (sdt && irk == 1) {
-      sa.dtw = L.dt.p; sa.vnn = p->vnn; sa.pmin = L.pmin.p;
+      sa.dtw = V.dt; sa.vnn = p->vnn; sa.pmin = L.pmin.p;
     }
     if (L.any_active && L.nbound) {
       pid = prof_begin(c, FUSS_K_BC);
-      if (L.kp == GC) k_bcflux<3, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
-      else k_bcflux<2, double><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
+      if (L.kp == GC) k_bcflux<3, R><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
+      else k_bcflux<2, R><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
       prof_end(c, pid);
       HIPCHK(hipGetLastError());
     }
@@ -1931,11 +2057,12 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
     if (hk) {
       // Fluxes + BC_Fluxes to R, External_Function on the host, RK_Newstate
       // (Mod_Explicit.f90:53-59)
-      sa.Rres = L.R.p;
+      sa.Rres = V.res;
       const unsigned gt = (unsigned)L.htiles.size();
       launch_stage_raw(c, L, sa, c->ndir, L.tang != 0);
       HIPCHK(hipGetLastError());
-      const int eh = run_hook(c, L, level, irk, in, hk);
+      int eh = FUSS_EUNSUPPORTED;  // (FP64 steps only)
+      if constexpr (std::is_same<R, double>::value) eh = run_hook(c, L, level, irk, in, hk);
       if (eh) return eh;
       launch_newstate(c, L, sa, prim, irs);
       if (!irs) { npart = (int)gt; L.plist = 0; }
@@ -1953,7 +2080,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
     }
     if (irs) {
       // Residual_Smoothing (Lib_IRS.f90:14-40): ndir passes R -> R2 -> R ...
-      IrsArgs<double> ia{};
+      IrsArgs<R> ia{};
       memset(&ia, 0, sizeof(ia));
       ia.blk = L.blk.p; ia.tiles = L.tiles.p; ia.bc = L.bc.p; ia.fmap = L.fmap.p;
       ia.beta = p->irs_beta;
@@ -1962,22 +2089,22 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
         ia.den = 1.0 + 2.0 * p->irs_beta;
         ia.inv = exact_inverse(ia.den);
       }
-      ia.T = in; ia.TO = L.Tb[o].p; ia.Tout = out; ia.mat = L.mat.p; ia.tab = dtab<double>(c);
+      ia.T = in; ia.TO = V.Tb[o]; ia.Tout = out; ia.mat = L.mat.p; ia.tab = dtab<R>(c);
       ia.rkc = sa.rkc; ia.jam = sa.jam; ia.last = sa.last; ia.irk = irk; ia.partials = L.partials.p; ia.status = c->status.p;
       ia.halt = halt;
       if (tri) {
         int np_ = 0;
-        const int e = launch_irs_tridiag(c, L, ia, L.R.p, L.R2.p, prim, &np_);
+        const int e = launch_irs_tridiag(c, L, ia, V.res, V.res2, prim, &np_);
         if (e) return e;
         if (sa.last) { npart = np_; L.plist = 0; }
       } else if (xyz) {
         // x, y and z in one k-marching pass with the update (and on the last
         // stage the next step's dt)
         const bool c8 = L.xyz_ty == 8;
-        ia.Rin = L.R.p;
+        ia.Rin = V.res;
         ia.tiles = c8 ? L.c8tiles.p : L.ctiles.p;
         ia.dtnext = fuse_dt && irk == nrk;
-        ia.vnn = p->vnn; ia.invL2 = L.invL2.p; ia.dt = L.dt.p; ia.pmin = L.pmin.p;
+        ia.vnn = p->vnn; ia.invL2 = V.invL2; ia.dt = V.dt; ia.pmin = L.pmin.p;
         ia.rect = (L.rect_dt && !c->rect_off) ? L.rfac.p : nullptr;
         pid = prof_begin(c, FUSS_K_IRS_UPDATE);
         launch_irs_xyz(c, L, ia, prim);
@@ -1991,8 +2118,8 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
       } else if (L.no_ycon && c->ndir >= 2) {
         // x and y in one LDS pass; 2-D levels update there, 3-D levels then
         // march the z pass with the update
-        ia.Rin = L.R.p;
-        ia.Rout = L.R2.p;
+        ia.Rin = V.res;
+        ia.Rout = V.res2;
         ia.tiles = L.xytiles.p;
         const unsigned gxy = (unsigned)L.hxytiles.size();
         pid = prof_begin(c, c->ndir == 2 ? FUSS_K_IRS_UPDATE : FUSS_K_IRS);
@@ -2001,7 +2128,7 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
         prof_end(c, pid);
         HIPCHK(hipGetLastError());
         if (c->ndir == 3) {
-          ia.Rin = L.R2.p;
+          ia.Rin = V.res2;
           ia.tiles = L.ctiles.p;
           pid = prof_begin(c, FUSS_K_IRS_UPDATE);
           launch_irs_z(c, L, ia, prim);
@@ -2010,8 +2137,8 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
           if (sa.last) { npart = (int)L.hctiles.size(); L.plist = 1; }
         }
       } else {
-        double* rin = L.R.p;
-        double* rout = L.R2.p;
+        R* rin = V.res;
+        R* rout = V.res2;
         const int nline = c->ndir == 3 ? 2 : c->ndir;  // 3-D: z pass marched below
         for (int d = 0; d < nline; d++) {
           ia.Rin = rin; ia.Rout = rout;
@@ -2052,6 +2179,30 @@ int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageH
   L.dt_next = fuse_dt;
   L.dt_vnn = p->vnn;
   return 0;
+}
+
+// Explicit_Step in the context's precision: FP32 contexts step their FP32
+// mirrors (the state is converted once, before the first FP32 step, outside
+// any capture); hooks, adjoint recording and chimera ghosts are FP64 only.
+int enqueue_step(fuss_ctx* c, int level, const fuss_step_params* p, const StageHook* hk = nullptr) {
+  Level& L = c->lev[level - 1];
+  if (c->precision == FUSS_FP32) {
+    if (hk || c->adj_store_on || L.nchim) return FUSS_EUNSUPPORTED;
+    const int e = state_to_f32(c, L);
+    if (e) return e;
+    return enqueue_step_t<float>(c, level, p, hk);
+  }
+  const int e = state_to_f64(c, L);
+  if (e) return e;
+  return enqueue_step_t<double>(c, level, p, hk);
+}
+
+// FP64 step whatever the context's precision (the defect correction's
+// monitor, the adjoint's recorded forward steps)
+int enqueue_step_f64(fuss_ctx* c, int level, const fuss_step_params* p) {
+  const int e = state_to_f64(c, c->lev[level - 1]);
+  if (e) return e;
+  return enqueue_step_t<double>(c, level, p, nullptr);
 }
 
 // ------------------------------------------------------------------------
@@ -2266,7 +2417,7 @@ int refine_run(fuss_ctx* c, const fuss_step_params* mon, fuss_refine_result* out
   for (int m = 0;; m++) {
     // monitor: one FP64 Explicit_Step; its residual is the reference's
     drop_dt_next(c);  // T was corrected since the last monitor step
-    int e = enqueue_step(c, 1, &pm);
+    int e = enqueue_step_f64(c, 1, &pm);
     if (e) return e;
     const int nb = L.nb;
     k_block_sums<<<nb, 256, 0, c->stream>>>(L.partials.p, L.rng[L.plist].p, R.bres.p);
@@ -2416,7 +2567,7 @@ int adj_forward(fuss_ctx* c, const fuss_step_params* p, double* hist) {
       HIPCHK(hipMemcpyAsync(A.ckpt.p + (size_t)(n / A.K) * np, L.Tb[L.cur].p, np * sizeof(double),
                             hipMemcpyDeviceToDevice, c->stream));
     k_clock_rec<<<1, 64, 0, c->stream>>>(L.clk.p, L.cnt.p, A.dtimes.p + n, A.diters.p + n);
-    const int e = enqueue_step(c, 1, &A.p);
+    const int e = enqueue_step_f64(c, 1, &A.p);
     if (e) return e;
     if (A.nprobe && hist) {
       k_probe_rec<<<(A.nprobe + 63) / 64, 64, 0, c->stream>>>(L.Tb[L.cur].p, A.probe.p, hist + (size_t)n * A.nprobe,
@@ -2476,7 +2627,7 @@ int adj_backward_t(fuss_ctx* c, const double* g, double* gqw, double* gk, double
                             hipMemcpyDeviceToDevice, c->stream));
       if (n < n1 - 1) {
         set_clock(c, L, A.times[n], A.iters[n]);
-        if ((err = enqueue_step(c, 1, &A.p))) break;
+        if ((err = enqueue_step_f64(c, 1, &A.p))) break;
       }
     }
     for (int n = n1 - 1; n >= n0 && !err; n--) {
@@ -2484,7 +2635,7 @@ int adj_backward_t(fuss_ctx* c, const double* g, double* gqw, double* gk, double
       HIPCHK(hipMemcpyAsync(L.Tb[L.cur].p, Tn, np * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
       set_clock(c, L, A.times[n], A.iters[n]);
       c->adj_store_on = 1;
-      err = enqueue_step(c, 1, &A.p);
+      err = enqueue_step_f64(c, 1, &A.p);
       c->adj_store_on = 0;
       if (err) break;
       for (int sidx = nrk; sidx >= 1; sidx--) {
@@ -2550,6 +2701,7 @@ int fuss_wall(fuss_ctx* c, int level, double* Tw, double* qw) {
   if (!c || !c->setup_done || level < 1 || level > c->nlev || !Tw || !qw) return FUSS_EARG;
   Level& L = c->lev[level - 1];
   if (!L.nbound) return 0;
+  if (state_to_f64(c, L)) return FUSS_EHIP;
   double* T = L.Tb[L.cur].p;
   // ghosts mirrored from the current state (the 302 tangential differences
   // of face-corner cells read them)
@@ -2601,6 +2753,8 @@ int fuss_set_kscale(fuss_ctx* c, double scale) {
   int e = upload(c->dkap.p, k.data(), n * 8) | upload(c->dkap2.p, k2.data(), 2 * n * 8);
   if (e) return FUSS_EHIP;
   c->kscale = scale;
+  c->ftab_ok = 0;  // the FP32 tables follow on the next FP32 step
+  for (auto& L : c->lev) run_graph_free(L);
   return 0;
 }
 
@@ -2631,6 +2785,7 @@ int fuss_adj_setup(fuss_ctx* c, int nsteps, int ckpt_every, int nprobe, const in
 int fuss_adj_forward(fuss_ctx* c, const fuss_step_params* p, double* probe_hist) {
   if (c) drop_dt_next(c);
   if (!c || !c->setup_done || !p || !c->adj.ready) return FUSS_EARG;
+  if (all_to_f64(c)) return FUSS_EHIP;
   return adj_forward(c, p, probe_hist);
 }
 
@@ -2638,6 +2793,7 @@ int fuss_adj_backward(fuss_ctx* c, const double* probe_adj, double* grad_qw, dou
                       double* grad_T0) {
   if (c) drop_dt_next(c);
   if (!c || !c->adj.ready || !c->adj.forward_done) return FUSS_EARG;
+  if (all_to_f64(c)) return FUSS_EHIP;
   const bool d3 = c->lev[0].kp == GC;
   if (c->adj.prec == FUSS_FP32)
     return d3 ? adj_backward_t<3, float>(c, probe_adj, grad_qw, grad_kscale, grad_T0)
@@ -2648,6 +2804,7 @@ int fuss_adj_backward(fuss_ctx* c, const double* probe_adj, double* grad_qw, dou
 
 int fuss_refine_setup(fuss_ctx* c, const fuss_refine_params* rp) {
   if (!c || !c->setup_done || !rp) return FUSS_EARG;
+  if (all_to_f64(c)) return FUSS_EHIP;
   return refine_build(c, rp);
 }
 
@@ -2665,6 +2822,7 @@ int fuss_refine(fuss_ctx* c, const fuss_step_params* monitor, fuss_refine_result
   if (c) drop_dt_next(c);
   if (!c || !c->setup_done || !monitor || !out || !c->ref.ready) return FUSS_EARG;
   if (monitor->time_accurate) return FUSS_EARG;
+  if (all_to_f64(c)) return FUSS_EHIP;
   const int e = c->ref.p.precision == FUSS_FP32 ? refine_run<float>(c, monitor, out, hist_res)
                                                : refine_run<double>(c, monitor, out, hist_res);
   drop_dt_next(c);
@@ -2710,6 +2868,9 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
   if (read_clock(c, L)) return FUSS_EHIP;
   const double t0 = L.time;
   const int it0 = L.iter;
+  // the FP64 state is saved and restored after each candidate (FP32
+  // contexts convert it again in each candidate's untimed first step)
+  if (state_to_f64(c, L)) return FUSS_EHIP;
   DBuf<double> save;
   if (save.alloc(L.npad)) return FUSS_EHIP;
   HIPCHK(hipMemcpyAsync(save.p, L.Tb[L.cur].p, L.npad * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -2739,6 +2900,7 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
     if (!e && per < best) { best = per; bkc = kc; brow = row; bst = st; bty = ty; }
     // on the library's stream: the next candidate's steps follow the restore
     HIPCHK(hipMemcpyAsync(L.Tb[L.cur].p, save.p, L.npad * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    L.state_f = 0;
     set_clock(c, L, t0, it0);
     return e;
   };
@@ -2877,6 +3039,14 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
   // back to their starting roles) captured once and replayed; the stop tests
   // run on the device (k_step_end) and later steps of the chunk become no-ops.
   Level& L = c->lev[level - 1];
+  // the state in the precision of the captured steps (conversions stay
+  // outside the capture)
+  if (c->precision == FUSS_FP32) {
+    if (L.nchim) return FUSS_EUNSUPPORTED;
+    if (state_to_f32(c, L)) return FUSS_EHIP;
+  } else if (state_to_f64(c, L)) {
+    return FUSS_EHIP;
+  }
   k_run_begin<<<1, 64, 0, c->stream>>>(L.cnt.p, maxcount);
   HIPCHK(hipGetLastError());
   if (q.irs_beta > 0.0 && q.irs_mode == FUSS_IRS_TRIDIAG) {
@@ -2962,6 +3132,8 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
 int fuss_restrict(fuss_ctx* c, int fine_level) {
   if (c) drop_dt_next(c);
   if (!c || !c->setup_done || fine_level < 1 || fine_level >= c->nlev) return FUSS_EARG;
+  if (state_to_f64(c, c->lev[fine_level - 1])) return FUSS_EHIP;
+  c->lev[fine_level].state_f = 0;  // overwritten in FP64
   int e = restrict_level(c, fine_level - 1);
   if (e) return e;
   return sync_status(c);
@@ -2972,6 +3144,8 @@ int fuss_prolong(fuss_ctx* c, int fine_level) {
   if (!c || !c->setup_done || fine_level < 1 || fine_level >= c->nlev) return FUSS_EARG;
   Level& F = c->lev[fine_level - 1];
   Level& C = c->lev[fine_level];
+  if (state_to_f64(c, C)) return FUSS_EHIP;
+  F.state_f = 0;  // overwritten in FP64
   for (int b = 0; b < F.nb; b++) {
     MGArgs a{};
     a.F = F.hblk[b]; a.Cb = C.hblk[b]; a.fT = F.Tb[F.cur].p; a.cT = C.Tb[C.cur].p;
@@ -2985,13 +3159,21 @@ int fuss_prolong(fuss_ctx* c, int fine_level) {
 }
 
 static int field_ptr(fuss_ctx* c, Level& L, const char* name, double** p) {
-  if (!strcmp(name, "T")) *p = L.Tb[L.cur].p;
-  else if (!strcmp(name, "R")) *p = L.R.p;
-  else if (!strcmp(name, "dt")) *p = L.dt.p;
-  else if (!strcmp(name, "vol")) *p = L.vol.p;
+  // after FP32 steps the FP64 arrays of R and dt are refreshed from the
+  // FP32 ones (the FP64 arrays are otherwise unused scratch then)
+  const bool f = L.state_f != 0;
+  if (!strcmp(name, "T")) {
+    if (state_to_f64(c, L)) return FUSS_EHIP;
+    *p = L.Tb[L.cur].p;
+  } else if (!strcmp(name, "R")) {
+    if (f && convert_f2d(c, L.f.R.p, L.R.p, L.npad)) return FUSS_EHIP;
+    *p = L.R.p;
+  } else if (!strcmp(name, "dt")) {
+    if (f && convert_f2d(c, L.f.dt.p, L.dt.p, L.npad)) return FUSS_EHIP;
+    *p = L.dt.p;
+  } else if (!strcmp(name, "vol")) *p = L.vol.p;
   else if (!strcmp(name, "qvol")) *p = L.qvol.p;
   else return FUSS_EARG;
-  (void)c;
   return 0;
 }
 
@@ -2999,7 +3181,8 @@ int fuss_get_field(fuss_ctx* c, int level, int b, const char* name, double* out)
   if (!c || !c->setup_done || level < 1 || level > c->nlev || b < 1 || b > c->nb || !out || !name) return FUSS_EARG;
   Level& L = c->lev[level - 1];
   double* src = nullptr;
-  if (field_ptr(c, L, name, &src)) return FUSS_EARG;
+  const int ef = field_ptr(c, L, name, &src);
+  if (ef) return ef;
   const BlockInfo& B = L.hblk[b - 1];
   std::vector<double> pad(B.npad);
   HIPCHK(hipMemcpyAsync(pad.data(), src + B.off, B.npad * 8, hipMemcpyDeviceToHost, c->stream));
@@ -3017,6 +3200,7 @@ int fuss_set_T(fuss_ctx* c, int level, int b, const double* T) {
   if (c) drop_dt_next(c);
   if (!c || !c->setup_done || level < 1 || level > c->nlev || b < 1 || b > c->nb || !T) return FUSS_EARG;
   Level& L = c->lev[level - 1];
+  if (state_to_f64(c, L)) return FUSS_EHIP;
   const BlockInfo& B = L.hblk[b - 1];
   std::vector<double> pad(B.npad);
   HIPCHK(hipMemcpyAsync(pad.data(), L.Tb[L.cur].p + B.off, B.npad * 8, hipMemcpyDeviceToHost, c->stream));
@@ -3032,6 +3216,7 @@ int fuss_set_T(fuss_ctx* c, int level, int b, const double* T) {
 int fuss_probe(fuss_ctx* c, int n, const int* bijk, double* out) {
   if (!c || !c->setup_done || n < 0 || (n && (!bijk || !out))) return FUSS_EARG;
   Level& L = c->lev[0];
+  if (state_to_f64(c, L)) return FUSS_EHIP;
   for (int q = 0; q < n; q++) {
     const int b = bijk[4 * q] - 1;
     if (b < 0 || b >= L.nb) return FUSS_EARG;

@@ -127,6 +127,22 @@ struct Level {
   int run_seq[RUN_G + 1] = {};
   std::vector<const void*> run_bufs;  // device buffers the captured steps address (checked before a replay)
   DBuf<double> partials, pmin, red2;
+  // FP32 explicit steps (FUSS_FP32 contexts): the per-cell fields in single
+  // precision.  state_f: the float T buffers hold the level's current state
+  // (the double ones are stale until state_to_f64); f_ready: the static
+  // fields (volumes, weights, qvol) are converted.
+  struct F32 {
+    DBuf<float> Tb[3], R, R2, dt, qvol, vol, W[9], bcv, bcf;
+  } f;
+  int state_f = 0, f_ready = 0;
+};
+
+// The per-cell fields one precision's step reads and writes.
+template <typename R>
+struct FieldView {
+  R* Tb[3];
+  R *res, *res2, *dt, *qvol, *vol, *W[9], *bcv, *bcf;
+  double* invL2;  // sum 1/dl^2 stays FP64 (the dt is formed in FP64 in either precision)
 };
 
 // One level of the correction multigrid (arrays in the inner precision).
@@ -180,6 +196,8 @@ struct fuss_ctx {
   int nmat = 0, Ti1 = 0, nT = 0;
   std::vector<double> tcp, trho, tkap, th;
   DBuf<double> dcp, drho, dkap, dh, dkap2, dpr4;
+  DBuf<float> fcp, frho, fkap, fh, fkap2, fpr4;  // the tables in FP32 (FUSS_FP32 steps)
+  int ftab_ok = 0;                  // ... converted from the current double tables
   struct Series { int periodic; std::vector<double> t, v; };
   std::vector<Series> series;
   DBuf<int> status;                 // code, b, i, j, k

@@ -27,6 +27,7 @@ FUSS_CONS, FUSS_PRIM = 0, 1
 FUSS_IRS_JACOBI2, FUSS_IRS_TRIDIAG = 0, 1
 FUSS_ESTATE = 2
 FUSS_ETABLE = 3
+FUSS_EUNSUPPORTED = 5
 
 # fuss_stage_fn: (user, level, irk, block, ni, nj, nk, const double *T, double *R)
 STAGE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

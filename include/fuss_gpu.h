@@ -35,7 +35,12 @@ typedef struct fuss_ctx fuss_ctx;
 
 /* precision modes of a context */
 #define FUSS_FP64 0          /* plain FP64 path: parity with the reference */
-#define FUSS_FP32 1          /* FP32 sweeps (flux, RK, IRS) + FP64 state/reductions */
+#define FUSS_FP32 1          /* FP32 explicit steps: T, fluxes, RK and IRS in single
+                                precision; the local dt is formed in FP64 and both
+                                reductions (dt min, Sum (T-TO)^2) and the clock stay
+                                FP64.  fuss_refine's monitor and the adjoint's
+                                forward run FP64 steps; hooks and chimera entries
+                                are FP64 only (FUSS_EUNSUPPORTED). */
 
 /* integration variables (Mod_Newstate.f90:191-205) */
 #define FUSS_CONS 0

@@ -57,8 +57,8 @@ def check_close(Tr, res_ref, plain):
         assert relerr(a, b) < 1e-10
 
 
-def refined(gpu, c, precision, **kw):
-    s = gpu.Solver(c)
+def refined(gpu, c, precision, ctx_precision=None, **kw):
+    s = gpu.Solver(c) if ctx_precision is None else gpu.Solver(c, precision=ctx_precision)
     rp = gpu.refine_params(precision=precision, res_threshold=c.params.res_threshold, **kw)
     r, hist = s.refine(rp)
     T = s.T()
