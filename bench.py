@@ -11,18 +11,21 @@ flux/BC/newstate stage, 3 IRS line passes), residual sum -- inputs resident
 in HBM.  value = interior cells x steps / wall time (cell-updates/s).
 
 Time to residual 1e-10 (||T^{n+1}-T^n||_2 of an FP64 Explicit_Step, the
-reference's stopping test): measured for the FP64 defect correction with
-FP32 inner sweeps (fuss_refine) and with FP64 inner sweeps, from the initial
-state T = 300 K.  The plain FP64 path needs ~3e6 iterations at 256^3 (hours),
-so its time is the measured time per iteration x an iteration count
-extrapolated from plain runs to 1e-10 at 32^3, 48^3, 64^3 (table below; the
-32^3 count is re-measured live unless --plain-conv-n 0).
+reference's stopping test), from T = 300 K: measured for the FP64 defect
+correction with FP32 inner sweeps (fuss_refine, 3-level correction V-cycle as
+the configuration states, and full depth) and with FP64 inner sweeps at
+256^3; measured for both the plain FP64 march and the FP32-refined solve at
+64^3 (--plain-conv-n), whose ratio is the measured speedup.  At 256^3 the
+plain march needs ~3e6 iterations (about an hour): its time there is
+reported as an estimate (measured time per iteration x an iteration count
+extrapolated from measured plain runs at 32^3, 48^3, 64^3).
 
-Also reported: the fused stage kernel's achieved bandwidth (algorithmic
-bytes per launch / average launch time from HIP events on the library's
-stream) against the MI355X peak of 8 TB/s, and the oracle (scalar C port of
-the reference, FP64) timed on the host on a bounded sample of the same
-workload.
+Also reported: per-kernel achieved bandwidth of the stage kernels and the
+fused IRS pass (algorithmic bytes per launch / average launch time from HIP
+events on the library's stream) against the MI355X peak of 8 TB/s; the
+batched sweep and the transient lattice in FP64 and FP32 bulk; the adjoint;
+and the oracle's OpenMP build on the host cores on a bounded sample of the
+same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -84,75 +87,106 @@ def calibration_copy(nbytes=1 << 30, reps=3):
 
 
 def cpu_baseline(n, iters):
-    """Oracle (scalar FP64 C port) on the host: `iters` finest-level
+    """The oracle's OpenMP build (the C restatement of the reference, FP64,
+    its per-cell and per-line loops shared over the host's cores as the
+    reference's OpenMP loops are) on the host: `iters` finest-level
     iterations of the same cube at n^3 (single level, same numerics)."""
     from fuss_amd import cases
     from oracle import pyoracle
 
     c = cases.cube3d(n=n, mgl=1, n_rk=4)
-    o = pyoracle.Oracle(c)
+    o = pyoracle.Oracle(c, omp=True)
+    threads = int(o.L.ob_threads())
     t0 = time.perf_counter()
     for _ in range(iters):
         o.step(1)
     dt = time.perf_counter() - t0
-    return {"value": c.ncell * iters / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (scalar C, FP64), {iters} iteration(s) of the {n}^3 cube, RK4+IRS, "
-                      f"{dt:.1f} s"}
+    return {"value": c.ncell * iters / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle, OpenMP build ({threads} threads; {os.cpu_count()} host CPUs visible), FP64, "
+                      f"{iters} iteration(s) of the {n}^3 cube, RK4 + LTS + IRS 0.5, {dt:.1f} s wall"}
+
+
+def refined_run(s, T0, prec, max_levels, label, max_outer=300):
+    """FP64 defect correction with `prec` inner sweeps from T0; wall time of
+    the second of two identical runs (the first loads the modules)."""
+    from fuss_amd import gpu
+
+    rp = gpu.refine_params(precision=prec, max_outer=max_outer, res_threshold=1e-10, max_levels=max_levels)
+    gpu._chk(s.L.fuss_refine_setup(s.h, gpu.C.byref(rp)), "fuss_refine_setup")
+    s.set_T(T0)
+    s.refine(rp, setup=False)
+    s.set_T(T0)
+    device_sync()
+    t0 = time.perf_counter()
+    r, hist = s.refine(rp, setup=False)
+    device_sync()
+    dt = time.perf_counter() - t0
+    print(f"[bench] {label}: {dt*1e3:.1f} ms, {r.outer} outer, {r.levels} levels, residual {r.res:.3e}",
+          file=sys.stderr, flush=True)
+    return dt, r, hist, s.get("T")
 
 
 def time_to_converge(s, case, args, ms_step):
-    """Wall time to the reference's residual 1e-10 from T = 300 K: defect
-    correction with FP32 and with FP64 inner sweeps (measured), plain FP64
-    (measured time per iteration x extrapolated iteration count)."""
+    """Wall time to the reference's residual 1e-10 (||T^{n+1} - T^n||_2 of an
+    FP64 Explicit_Step) from T = 300 K.
+
+    Measured: the FP64 defect correction with FP32 and with FP64 inner
+    sweeps at the benchmark size (correction V-cycle to full depth; a
+    hierarchy cut at 3 levels, the depth of the configuration's plain
+    multigrid, is reported as well: its coarsest 64^3 level, only smoothed,
+    cannot carry the global heat-up mode, and 30 outer iterations leave the
+    residual far above the threshold), and at a smaller cube both the plain FP64 march (the reference's method, 3-level
+    coarse-to-fine sequencing) and the FP32-refined solve, so the speedup
+    there is a ratio of two measurements.  The plain march at the benchmark
+    size takes ~3e6 iterations; its time there is an estimate (measured time
+    per iteration x an iteration count extrapolated from measured runs)."""
     from fuss_amd import cases, gpu
 
     T0 = case.blocks[0].T
     out = {"threshold": 1e-10, "from": "initial state T = 300 K"}
     finals = {}
-    for prec, name in ((gpu.FUSS_FP32, "fp32_refined"), (gpu.FUSS_FP64, "fp64_refined")):
-        rp = gpu.refine_params(precision=prec, max_outer=60, res_threshold=1e-10)
-        gpu._chk(s.L.fuss_refine_setup(s.h, gpu.C.byref(rp)), "fuss_refine_setup")
-        s.set_T(T0)
-        s.refine(rp, setup=False)        # warm-up (module load)
-        s.set_T(T0)
-        device_sync()
-        t0 = time.perf_counter()
-        r, hist = s.refine(rp, setup=False)
-        device_sync()
-        dt = time.perf_counter() - t0
-        finals[name] = s.get("T")
+    for prec, name, lv, mo in ((gpu.FUSS_FP32, "fp32_refined", 0, 300), (gpu.FUSS_FP64, "fp64_refined", 0, 300),
+                               (gpu.FUSS_FP32, "fp32_refined_3_levels", 3, 30)):
+        dt, r, hist, T = refined_run(s, T0, prec, lv, name, mo)
+        finals[name] = T
         out[name] = {"seconds": dt, "converged": bool(r.converged), "outer": r.outer, "vcycles": r.vcycles,
-                     "levels": r.levels, "final_residual": r.res, "defect_W": r.defect,
+                     "levels": r.levels, "final_residual": r.res, "defect_W": r.defect, "measured": True,
                      "residual_history": [float(x) for x in hist]}
-        print(f"[bench] {name}: {dt*1e3:.1f} ms, {r.outer} outer, residual {r.res:.3e}", file=sys.stderr,
-              flush=True)
     a, b = finals["fp32_refined"], finals["fp64_refined"]
     out["fp32_vs_fp64_refined_max_rel"] = float(np.max(np.abs(a - b) / np.abs(b)))
-    iters = PLAIN_ITERS_MEASURED[64] * (args.n / 64.0) ** PLAIN_EXP
-    plain_s = iters * ms_step * 1e-3
-    out["fp64_plain"] = {"seconds_est": plain_s, "iterations_est": iters, "ms_per_iteration": ms_step,
-                         "iterations_measured": PLAIN_ITERS_MEASURED,
-                         "fit": f"230218 (n/64)^{PLAIN_EXP}"}
-    out["speedup_fp32_refined_vs_plain"] = plain_s / out["fp32_refined"]["seconds"]
     out["speedup_fp32_vs_fp64_refined"] = out["fp64_refined"]["seconds"] / out["fp32_refined"]["seconds"]
-    if args.plain_conv_n:
-        # live check at a small size: plain FP64 to 1e-10 vs the refined solution
-        m = args.plain_conv_n
+    m = args.plain_conv_n
+    if m:
+        # measured at m^3: the plain FP64 march to 1e-10 and the FP32-refined
+        # solve of the same cube
         c = cases.cube3d(n=m, mgl=3, n_rk=args.rk)
         sp = gpu.Solver(c)
+        sp.autotune(1, steps=3)
+        device_sync()
+        t0 = time.perf_counter()
         nit, hist = sp.solve()
+        device_sync()
+        t_plain = time.perf_counter() - t0
         Tp = sp.get("T")
         sp.close()
-        c1 = cases.cube3d(n=m, mgl=1, n_rk=args.rk)
-        sr = gpu.Solver(c1)
-        r, _ = sr.refine(gpu.refine_params(precision=gpu.FUSS_FP32, res_threshold=1e-10))
-        Tr = sr.get("T")
+        sr = gpu.Solver(cases.cube3d(n=m, mgl=1, n_rk=args.rk))
+        t_ref, r, _, Tr = refined_run(sr, cases.cube3d(n=m, mgl=1, n_rk=args.rk).blocks[0].T, gpu.FUSS_FP32, 0,
+                                      f"fp32_refined {m}^3")
         sr.close()
-        out["live_check"] = {"n": m, "plain_iterations": int(nit), "plain_final_residual": float(hist[-1, 2]),
-                             "fp32_refined_outer": r.outer,
-                             "fp32_refined_vs_plain_max_rel": float(np.max(np.abs(Tr - Tp) / np.abs(Tp)))}
-        print(f"[bench] live check {m}^3: plain {nit} iterations; refined vs plain max rel "
-              f"{out['live_check']['fp32_refined_vs_plain_max_rel']:.2e}", file=sys.stderr, flush=True)
+        out["measured_at"] = {"n": m, "plain_fp64": {"seconds": t_plain, "iterations": int(nit),
+                                                      "final_residual": float(hist[-1, 2])},
+                              "fp32_refined": {"seconds": t_ref, "outer": r.outer, "final_residual": r.res},
+                              "speedup_fp32_refined_vs_plain": t_plain / t_ref,
+                              "fp32_refined_vs_plain_max_rel": float(np.max(np.abs(Tr - Tp) / np.abs(Tp)))}
+        print(f"[bench] plain FP64 {m}^3 to 1e-10: {nit} iterations in {t_plain:.1f} s; FP32-refined "
+              f"{t_ref*1e3:.1f} ms; speedup {t_plain / t_ref:.0f}", file=sys.stderr, flush=True)
+    iters = PLAIN_ITERS_MEASURED[64] * (args.n / 64.0) ** PLAIN_EXP
+    plain_s = iters * ms_step * 1e-3
+    out["fp64_plain"] = {"measured": False, "seconds_est": plain_s, "iterations_est": iters,
+                         "ms_per_iteration": ms_step, "iterations_measured": PLAIN_ITERS_MEASURED,
+                         "fit": f"230218 (n/64)^{PLAIN_EXP}",
+                         "note": "estimate: measured ms per iteration x extrapolated iteration count"}
+    out["speedup_fp32_refined_vs_plain_est"] = plain_s / out["fp32_refined"]["seconds"]
     return out
 
 
@@ -288,39 +322,52 @@ def sweep_check(args):
 def lattice_check(args):
     """Transient 4x4x4 block lattice (uneven per-slab dims 16-128, 1-to-1
     connections, curvilinear, 3 tabulated materials, 302/301/304 walls),
-    global-dt RK4 for `lattice_steps` steps through fuss_run (captured graphs)."""
+    global-dt RK4 for `lattice_steps` steps through fuss_run (captured
+    graphs): plain FP64, and FP32 bulk (FUSS_FP32: state, fluxes and update in
+    FP32; dt and the residual reductions in FP64) with its error against the
+    FP64 state after the same steps."""
     from fuss_amd import cases, gpu
 
-    t0 = time.perf_counter()
-    case = cases.lattice(nl=4, lo=16, hi=128, n_rk=4, time_accurate=True)
-    s = gpu.Solver(case)
-    setup_s = time.perf_counter() - t0
-    tuned = s.autotune(1, steps=3)
-    s.run_level(1, 30, 30, -1.0, 1e300, hist=False)   # warm-up (graph capture, module load)
-    device_sync()
-    t0 = time.perf_counter()
-    k, _, _, r = s.run_level(1, args.lattice_steps, args.lattice_steps, -1.0, 1e300, hist=False)
-    device_sync()
-    dt = time.perf_counter() - t0
     out = {"workload": "transient 4x4x4 lattice of 64 connected curvilinear blocks (slab dims 16-128), "
-                       "3 materials, 302/301/304 walls, RK4 global dt, FP64",
-           "cells": case.ncell, "blocks": case.nb, "steps": k, "seconds": dt, "setup_s": setup_s,
-           "cell_updates_per_s": case.ncell * k / dt, "simulated_time_s": r.time,
-           "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring"}}
-    print(f"[bench] lattice: {case.ncell} cells x {k} steps in {dt:.2f} s", file=sys.stderr, flush=True)
-    s.close()
+                       "3 materials, 302/301/304 walls, RK4 global dt"}
+    states = {}
+    for prec, name in ((gpu.FUSS_FP64, "fp64"), (gpu.FUSS_FP32, "fp32_bulk")):
+        t0 = time.perf_counter()
+        case = cases.lattice(nl=4, lo=16, hi=128, n_rk=4, time_accurate=True)
+        s = gpu.Solver(case, precision=prec)
+        setup_s = time.perf_counter() - t0
+        tuned = s.autotune(1, steps=3)
+        s.run_level(1, 30, 30, -1.0, 1e300, hist=False)   # warm-up (graph capture, module load)
+        s.set_T_all([b.T for b in case.blocks])
+        s.set_clock(1, 0.0, 0)
+        device_sync()
+        t0 = time.perf_counter()
+        k, _, _, r = s.run_level(1, args.lattice_steps, args.lattice_steps, -1.0, 1e300, hist=False)
+        device_sync()
+        dt = time.perf_counter() - t0
+        states[name] = [s.get("T", 1, b + 1) for b in range(case.nb)]
+        out["cells"], out["blocks"], out["steps"] = case.ncell, case.nb, k
+        out[name] = {"seconds": dt, "setup_s": setup_s, "cell_updates_per_s": case.ncell * k / dt,
+                     "simulated_time_s": r.time,
+                     "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring"}}
+        print(f"[bench] lattice {name}: {case.ncell} cells x {k} steps in {dt:.2f} s", file=sys.stderr, flush=True)
+        s.close()
+    err = max(float(np.max(np.abs(a - b) / np.abs(b))) for a, b in zip(states["fp32_bulk"], states["fp64"]))
+    out["fp32_bulk"]["max_rel_err_vs_fp64"] = err
+    out["fp32_bulk"]["speedup_vs_fp64"] = out["fp64"]["seconds"] / out["fp32_bulk"]["seconds"]
+    print(f"[bench] lattice FP32 bulk vs FP64 after {out['steps']} steps: max rel err {err:.2e}", file=sys.stderr,
+          flush=True)
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # fuss_run replays captured chunks of 30 steps: keep both multiples of 30
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-n", type=int, default=256)
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true", help="also run 1 GiB D2D copies (counter calibration)")
     ap.add_argument("--rk", type=int, default=4)
@@ -331,7 +378,8 @@ def main():
     ap.add_argument("--no-rect", action="store_true",
                     help="stream the stored 3-D metric arrays instead of the per-axis factors")
     ap.add_argument("--no-refine", action="store_true", help="skip the time-to-1e-10 runs")
-    ap.add_argument("--plain-conv-n", type=int, default=32, help="cube size of the live plain run to 1e-10 (0: skip)")
+    ap.add_argument("--plain-conv-n", type=int, default=64,
+                    help="cube size of the measured plain FP64 and FP32-refined runs to 1e-10 (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (inverse problem) measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the sweep and lattice measurements")
     ap.add_argument("--sweep-cases", type=int, default=256)
@@ -371,8 +419,11 @@ def main():
     print(f"[bench] tiles ({chosen}): column height {tuned[0]}, {'row march' if tuned[1] else 'LDS ring'}, "
           f"stage columns of {stage_rows} rows, IRS columns of {irs_rows} rows, {tuned[2]:.3f} ms/step",
           file=sys.stderr, flush=True)
-    # warm-up: module load and the capture of fuss_run's step graph
+    # warm-up: module load, then a call of the timed size, which captures the
+    # graph of fuss_run's steps (chunks of min(steps, 30) rounded up to a
+    # multiple of 3) that the timed call replays
     s.run_level(1, max(args.warmup, 1), max(args.warmup, 1), -1.0, 1e300, hist=False)
+    s.run_level(1, args.steps, args.steps, -1.0, 1e300, hist=False)
     device_sync()
     # timed region: `steps` Explicit_Steps through fuss_run (the library's
     # step loop: captured graph replay, stop tests on the device)
@@ -400,6 +451,20 @@ def main():
     # rectilinear level, every stage otherwise); "stage_dt": stage 1 with dt
     names = ["dt", "ghost", "stage", "irs", "irs_update", "reduce", "bc", "stage_dt"]
     kern = {names[q]: {"ms_avg": ms[q] / cnt[q], "launches": cnt[q]} for q in range(8) if cnt[q]}
+    # algorithmic bytes per cell of the streamed kernels on this level (one
+    # material, uniform qvol): the stage kernel reads T and dt and writes R
+    # (the first stage writes dt instead of reading it); the fused x-y-z IRS
+    # pass reads R and T^n and writes T (the four-stage primitive update needs
+    # no stage state), the last stage also writing the next step's dt
+    kb = {"stage": perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False, qvol=False,
+                                         rect=rect[0]),
+          "stage_dt": perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False, qvol=False,
+                                            rect=rect[0]),
+          "irs_update": perfmodel.irs_fused_bytes(8, mat=False, stage_state=False)}
+    for kname, bpc in kb.items():
+        if kname in kern:
+            gbs = bpc * case.ncell / (kern[kname]["ms_avg"] * 1e-3) / 1e9
+            kern[kname].update({"bytes_per_cell": bpc, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS})
     # one material and a uniform qvol: the stage kernel reads neither stream;
     # on a rectilinear level it reads no per-cell geometry either
     stage_bpc = perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, mat=False, qvol=False, rect=rect[0])

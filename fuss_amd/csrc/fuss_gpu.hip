@@ -3017,7 +3017,11 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
   // steps this call may take: the loop below stops after the first step with
   // n >= itermax_left, so at least one step runs when max_iters > 0
   const int maxcount = max_iters <= 0 ? 0 : std::min(max_iters, std::max(1, itermax_left));
-  if (c->prof.enabled || maxcount < RUN_G || c->graphs_off) {
+  // captured chunk: RUN_G steps, or the request rounded up to a multiple of 3
+  // (the steps past the limit are no-ops on the device); a few steps run
+  // the loop
+  const int G = maxcount >= RUN_G ? RUN_G : 3 * ((maxcount + 2) / 3);
+  if (c->prof.enabled || maxcount < 6 || c->graphs_off) {
     while (n < max_iters) {
       const int e = fuss_step(c, level, &q, &r);
       if (e) {
@@ -3035,7 +3039,7 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
     if (last) *last = r;
     return 0;
   }
-  // Graph mode: RUN_G steps (a multiple of 3, so the rotating T buffers come
+  // Graph mode: G steps (a multiple of 3, so the rotating T buffers come
   // back to their starting roles) captured once and replayed; the stop tests
   // run on the device (k_step_end) and later steps of the chunk become no-ops.
   Level& L = c->lev[level - 1];
@@ -3053,8 +3057,17 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
     const int e0 = tri_prepare(c, L, q.irs_beta);  // uploads: not inside the capture
     if (e0) return e0;
   }
-  const bool hit = L.run_exec && L.run_seq[0] == L.cur && !memcmp(&L.run_p, &q, sizeof(q)) &&
-                   L.run_thr == res_threshold && L.run_tthr == time_threshold && L.run_bufs == run_buffers(c, L);
+  const bool hit = L.run_exec && !memcmp(&L.run_p, &q, sizeof(q)) && L.run_thr == res_threshold &&
+                   L.run_tthr == time_threshold && L.run_g == G && L.run_bufs == run_buffers(c, L);
+  if (hit && L.run_seq[0] != L.cur) {
+    // the kept graph starts from another of the three rotating buffers: move
+    // the state (T^n with its ghosts; the stage buffers are scratch) there
+    const size_t bytes = L.npad * (c->precision == FUSS_FP32 ? sizeof(float) : sizeof(double));
+    void* dst = c->precision == FUSS_FP32 ? (void*)L.f.Tb[L.run_seq[0]].p : (void*)L.Tb[L.run_seq[0]].p;
+    const void* src = c->precision == FUSS_FP32 ? (const void*)L.f.Tb[L.cur].p : (const void*)L.Tb[L.cur].p;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    L.cur = L.run_seq[0];
+  }
   int* cur_seq = L.run_seq;
   if (!hit) {
     run_graph_free(L);
@@ -3067,7 +3080,7 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
     {
       k_chunk_begin<<<1, 64, 0, c->stream>>>(L.cnt.p);
       cur_seq[0] = L.cur;
-      for (int s = 0; s < RUN_G && !e; s++) {
+      for (int s = 0; s < G && !e; s++) {
         e = enqueue_step(c, level, &q);
         k_step_end<<<1, 64, 0, c->stream>>>(c->scal.p, L.clk.p, L.cnt.p, c->status.p, c->run_hist.p,
                                             c->run_hist.p + RUN_G, res_threshold, time_threshold);
@@ -3087,6 +3100,7 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
     }
     L.run_graph = graph;
     L.run_bufs = run_buffers(c, L);
+    L.run_g = G;
     L.run_p = q;
     L.run_thr = res_threshold;
     L.run_tthr = time_threshold;
@@ -3106,7 +3120,7 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
     }
     done += k;
     halted = c->hcnt[1];
-    L.cur = cur_seq[halted ? k : RUN_G];
+    L.cur = cur_seq[halted ? k : G];
   }
   if (e) return e;
   HIPCHK(hipMemcpyAsync(c->hscal, c->scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));

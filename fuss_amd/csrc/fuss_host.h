@@ -125,6 +125,7 @@ struct Level {
   fuss_step_params run_p{};
   double run_thr = 0.0, run_tthr = 0.0;
   int run_seq[RUN_G + 1] = {};
+  int run_g = 0;                    // steps in the captured chunk (<= RUN_G, multiple of 3)
   std::vector<const void*> run_bufs;  // device buffers the captured steps address (checked before a replay)
   DBuf<double> partials, pmin, red2;
   // FP32 explicit steps (FUSS_FP32 contexts): the per-cell fields in single

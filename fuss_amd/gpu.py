@@ -256,6 +256,15 @@ class Solver:
     def set_T(self, T, level=1, b=1):
         _chk(self.L.fuss_set_T(self.h, level, b, _c(T)), "fuss_set_T")
 
+    def set_T_all(self, Ts, level=1):
+        """T of every block of the level (a list in block order)."""
+        for b, T in enumerate(Ts):
+            self.set_T(T, level, b + 1)
+
+    def set_clock(self, level, time, iteration):
+        """Level time and iteration counter (restart; fuss_set_clock)."""
+        _chk(self.L.fuss_set_clock(self.h, level, float(time), int(iteration)), "fuss_set_clock")
+
     def probe(self, bijk):
         q = _c(np.asarray(bijk).reshape(-1, 4), np.int32)
         out = np.zeros(q.shape[0])

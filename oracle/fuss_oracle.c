@@ -18,6 +18,9 @@
  * (uniform-grid metrics, table interpolation, RK coefficients, IRS operator,
  * prolongation weights, restriction volume weighting).
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "fuss_oracle.h"
 
 #include <math.h>
@@ -130,6 +133,9 @@ static inline double *Mc(const ob_blk *b, int i, int j, int k) {
 #define MEL(m, r, c) ((m)[((r)-1) + 3 * ((c)-1)])
 
 static void set_err(ob_ctx *c, int code, int b, int i, int j, int k) {
+#ifdef _OPENMP
+#pragma omp critical(ob_err)
+#endif
   if (!c->err) {
     c->err = code;
     c->eb = b;
@@ -663,10 +669,18 @@ static void diffusive_flux(ob_ctx *c, double matID, const double *normal, double
 static void fluxes_blk(ob_ctx *c, ob_blk *b) {
   int n1 = b->n[0], n2 = b->n[1], n3 = b->n[2];
   double *T = b->T, *R = b->R;
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) R[CX(b, i, j, k)] = 0.0;
 #define TT(i, j, k) T[CX(b, i, j, k)]
+  /* (OpenMP build: rows, columns and pillars of one sweep touch disjoint
+   * cells, and each cell keeps its serial accumulation order) */
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1 - 1; i++) {
@@ -677,6 +691,9 @@ static void fluxes_blk(ob_ctx *c, ob_blk *b) {
                        TT(i + 1, j, k + 1), Mc(b, i, j, k), Mc(b, i + 1, j, k),
                        &R[CX(b, i, j, k)], &R[CX(b, i + 1, j, k)], 1, 2, 3);
       }
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
   for (int k = 1; k <= n3; k++)
     for (int i = 1; i <= n1; i++)
       for (int j = 1; j <= n2 - 1; j++) {
@@ -687,6 +704,9 @@ static void fluxes_blk(ob_ctx *c, ob_blk *b) {
                        TT(i, j + 1, k + 1), Mc(b, i, j, k), Mc(b, i, j + 1, k),
                        &R[CX(b, i, j, k)], &R[CX(b, i, j + 1, k)], 2, 1, 3);
       }
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
   for (int j = 1; j <= n2; j++)
     for (int i = 1; i <= n1; i++)
       for (int k = 1; k <= n3 - 1; k++) {
@@ -978,6 +998,9 @@ static void compute_dt(ob_ctx *c, ob_dom *d) {
   double dtglobal = d->dtglobal;
   for (int bb = 0; bb < d->nb; bb++) {
     ob_blk *b = &d->blk[bb];
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) reduction(min : dtglobal)
+#endif
     for (int k = 1; k <= b->n[2]; k++)
       for (int j = 1; j <= b->n[1]; j++)
         for (int i = 1; i <= b->n[0]; i++) {
@@ -1028,11 +1051,17 @@ static void irs_ghost(ob_dom *d) {
 static void irs_blk(ob_blk *b, int dir, double beta) {
   /* Residual_Smoothing_Blk, Lib_IRS.f90:43-134 (njacobi = 2) */
   int n1 = b->n[0], n2 = b->n[1], n3 = b->n[2];
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
   for (int k = 0; k <= n3 + 1; k++)
     for (int j = 0; j <= n2 + 1; j++)
       for (int i = 0; i <= n1 + 1; i++) b->RS1[CX(b, i, j, k)] = b->R[CX(b, i, j, k)];
   int di = dir == 1, dj = dir == 2, dk = dir == 3;
   for (int jac = 0; jac < 2; jac++) {
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
     for (int k = 1; k <= n3; k++)
       for (int j = 1; j <= n2; j++)
         for (int i = 1; i <= n1; i++) {
@@ -1041,10 +1070,16 @@ static void irs_blk(ob_blk *b, int dir, double beta) {
           double rp = b->RS1[CX(b, i + di, j + dj, k + dk)];
           b->RS2[CX(b, i, j, k)] = (r + beta * (rm + rp)) / (1.0 + 2.0 * beta);
         }
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
     for (int k = 1; k <= n3; k++)
       for (int j = 1; j <= n2; j++)
         for (int i = 1; i <= n1; i++) b->RS1[CX(b, i, j, k)] = b->RS2[CX(b, i, j, k)];
   }
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
   for (int k = 1; k <= n3; k++)
     for (int j = 1; j <= n2; j++)
       for (int i = 1; i <= n1; i++) b->R[CX(b, i, j, k)] = b->RS1[CX(b, i, j, k)];
@@ -1133,6 +1168,9 @@ static void newstate(ob_ctx *c, ob_dom *d, int irk) {
     /* Newstate_Primitive, Lib_Newstate.f90:154-272 */
     for (int bb = 0; bb < d->nb; bb++) {
       ob_blk *b = &d->blk[bb];
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
       for (int k = 1; k <= b->n[2]; k++)
         for (int j = 1; j <= b->n[1]; j++)
           for (int i = 1; i <= b->n[0]; i++) {
@@ -1153,6 +1191,9 @@ static void newstate(ob_ctx *c, ob_dom *d, int irk) {
       residual_smoothing(c, d);
       for (int bb = 0; bb < d->nb; bb++) {
         ob_blk *b = &d->blk[bb];
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
         for (int k = 1; k <= b->n[2]; k++)
           for (int j = 1; j <= b->n[1]; j++)
             for (int i = 1; i <= b->n[0]; i++) {
@@ -1166,6 +1207,9 @@ static void newstate(ob_ctx *c, ob_dom *d, int irk) {
     /* Newstate_Conservative, Lib_Newstate.f90:13-148 */
     for (int bb = 0; bb < d->nb; bb++) {
       ob_blk *b = &d->blk[bb];
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
       for (int k = 1; k <= b->n[2]; k++)
         for (int j = 1; j <= b->n[1]; j++)
           for (int i = 1; i <= b->n[0]; i++) {
@@ -1184,6 +1228,9 @@ static void newstate(ob_ctx *c, ob_dom *d, int irk) {
       residual_smoothing(c, d);
       for (int bb = 0; bb < d->nb; bb++) {
         ob_blk *b = &d->blk[bb];
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
         for (int k = 1; k <= b->n[2]; k++)
           for (int j = 1; j <= b->n[1]; j++)
             for (int i = 1; i <= b->n[0]; i++) {
@@ -1205,6 +1252,9 @@ static double compute_residual(ob_dom *d) {
   for (int bb = 0; bb < d->nb; bb++) {
     ob_blk *b = &d->blk[bb];
     double res = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) reduction(+ : res)
+#endif
     for (int k = 1; k <= b->n[2]; k++)
       for (int j = 1; j <= b->n[1]; j++)
         for (int i = 1; i <= b->n[0]; i++) {
@@ -1231,6 +1281,9 @@ int ob_step(ob_ctx *c, int level, double *res_sq) {
   /* Copy_State, Mod_Explicit.f90:114-135 */
   for (int bb = 0; bb < d->nb; bb++) {
     ob_blk *b = &d->blk[bb];
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2)
+#endif
     for (int k = 1; k <= b->n[2]; k++)
       for (int j = 1; j <= b->n[1]; j++)
         for (int i = 1; i <= b->n[0]; i++) b->TO[CX(b, i, j, k)] = b->T[CX(b, i, j, k)];
@@ -1718,6 +1771,14 @@ int ob_dims(ob_ctx *c, int level, int b, int *dims) {
   return 0;
 }
 int ob_ndir(ob_ctx *c) { return c ? c->ndir : 0; }
+/* threads of the OpenMP build (1 for the scalar checker) */
+int ob_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 double ob_time(ob_ctx *c, int level) { return c->dom[level - 1].time; }
 double ob_dtglobal(ob_ctx *c, int level) { return c->dom[level - 1].dtglobal; }
 int ob_last_error(ob_ctx *c, int *bijk) {

@@ -102,6 +102,7 @@ int ob_get(ob_ctx *c, int level, int b, const char *name, double *out);
 int ob_set(ob_ctx *c, int level, int b, const char *name, const double *in);
 int ob_dims(ob_ctx *c, int level, int b, int *dims);
 int ob_ndir(ob_ctx *c);
+int ob_threads(void);
 double ob_time(ob_ctx *c, int level);
 double ob_dtglobal(ob_ctx *c, int level);
 int ob_last_error(ob_ctx *c, int *bijk);

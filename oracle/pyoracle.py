@@ -13,7 +13,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "_build", "libfuss_oracle.so")
+_LIB_OMP = os.path.join(_HERE, "_build", "libfuss_oracle_omp.so")
 _lib = None
+_lib_omp = None
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -23,12 +25,15 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.isfile(_LIB):
+def lib(omp=False):
+    """The scalar checker, or (omp=True) the OpenMP build of the same source
+    used as the host-core CPU baseline."""
+    global _lib, _lib_omp
+    if (_lib_omp if omp else _lib) is None:
+        path = _LIB_OMP if omp else _LIB
+        if not os.path.isfile(path):
             build()
-        L = C.CDLL(_LIB)
+        L = C.CDLL(path)
         L.ob_create.restype = C.c_void_p
         L.ob_create.argtypes = [C.c_int, C.c_int]
         L.ob_destroy.argtypes = [C.c_void_p]
@@ -58,8 +63,13 @@ def lib():
         L.ob_get_bc_ghost.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
         L.ob_wall.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.ob_set_chimera.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _dp]
-        _lib = L
-    return _lib
+        L.ob_threads.restype = C.c_int
+        L.ob_threads.argtypes = []
+        if omp:
+            _lib_omp = L
+        else:
+            _lib = L
+    return _lib_omp if omp else _lib
 
 
 class OracleError(RuntimeError):
@@ -73,8 +83,8 @@ def _c(a, dt=np.float64):
 class Oracle:
     """One oracle context holding every multigrid level of a Case."""
 
-    def __init__(self, case, params=None):
-        self.L = lib()
+    def __init__(self, case, params=None, omp=False):
+        self.L = lib(omp)
         self.case = case
         p = params or case.params
         self.p = p

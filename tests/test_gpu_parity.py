@@ -278,6 +278,30 @@ def test_graph_run_fused_dt(gpu, rect):
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
 
 
+@pytest.mark.parametrize("steps", [20, 7, 44, 6])
+def test_graph_chunk_sizes(gpu, steps):
+    """Requests below one 30-step chunk capture a chunk of the request
+    rounded up to a multiple of 3 (the extra steps are device no-ops), and a
+    request that is not a multiple of 30 ends inside a chunk: the state,
+    clock and residual history equal the step loop's bit for bit, and a
+    second call with the same size replays the kept graph."""
+    out = []
+    for graphs in (True, False):
+        s = gpu.Solver(SYNTH["lattice_rk4_cons"]())
+        s.graphs(graphs)
+        rows = []
+        for _ in range(2):
+            k, hr, ht, _ = s.run_level(1, steps, steps, -1.0, 1e300)
+            rows.append((k, hr.copy(), ht.copy()))
+        out.append((rows, np.concatenate([T.ravel() for T in s.T()]), s.clock()))
+        s.close()
+    (r1, T1, c1), (r2, T2, c2) = out
+    assert [x[0] for x in r1] == [x[0] for x in r2] == [steps, steps]
+    for a, b in zip(r1, r2):
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert np.array_equal(T1, T2) and c1 == c2
+
+
 @pytest.mark.parametrize("rect", [False, True])
 def test_dtglobal_vs_oracle(gpu, oracle_mod, rect):
     """dtglobal of each step (k_dt, the fused IRS pass's next-step dt, or the
