@@ -360,6 +360,46 @@ def lattice_check(args):
     return out
 
 
+def c1_check(args):
+    """The 2-D parity plate (128 x 128, RK3, local dt, vnn 0.8, no IRS) from
+    its seeded initial state to residual 1e-10: fuss_run on captured graphs
+    (one launch per kernel) and as persistent launches (one cooperative
+    launch per 32768 steps), stop iteration and wall time of each, and the
+    field against the oracle's run to the same stop (a committed fixture)."""
+    from fuss_amd import cases, gpu
+
+    out = {"workload": "2-D plate 128^2, k=50, rho=7800, cp=460, 302 500/300 K, 301 q=0, RK3 LTS vnn 0.8, "
+                       "to residual 1e-10"}
+    fields = {}
+    for persistent, name in ((False, "graphs"), (True, "persistent")):
+        c = cases.plate2d(128)
+        s = gpu.Solver(c)
+        s.persistent(persistent)
+        s.run_level(1, 60, 60, -1.0, 1e300, hist=False)   # warm-up: modules, graph capture
+        s.set_T(c.blocks[0].T)
+        s.set_clock(1, -1.0, 0)
+        device_sync()
+        t0 = time.perf_counter()
+        n, hist = s.solve()
+        device_sync()
+        dt = time.perf_counter() - t0
+        fields[name] = s.get("T")
+        out[name] = {"iterations": int(n), "seconds": dt, "us_per_step": dt / n * 1e6,
+                     "cell_updates_per_s": c.ncell * n / dt, "final_residual": float(hist[-1, 2])}
+        print(f"[bench] C1 {name}: {n} iterations in {dt:.2f} s ({dt / n * 1e6:.1f} us/step)", file=sys.stderr,
+              flush=True)
+        s.close()
+    out["persistent_vs_graphs_max_rel"] = float(np.max(np.abs(fields["persistent"] - fields["graphs"]) /
+                                                      np.abs(fields["graphs"])))
+    out["speedup_persistent_vs_graphs"] = out["graphs"]["seconds"] / out["persistent"]["seconds"]
+    fx = os.path.join(ROOT, "tests", "golden", "plate2d_128_oracle.npz")
+    if os.path.isfile(fx):
+        z = np.load(fx, allow_pickle=False)
+        out["oracle_stop_iteration"] = int(z["stop_iter"])
+        out["max_rel_err_vs_oracle"] = float(np.max(np.abs(fields["persistent"] - z["T"]) / np.abs(z["T"])))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,6 +422,7 @@ def main():
                     help="cube size of the measured plain FP64 and FP32-refined runs to 1e-10 (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (inverse problem) measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the sweep and lattice measurements")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 2-D plate run to 1e-10")
     ap.add_argument("--sweep-cases", type=int, default=256)
     ap.add_argument("--sweep-steps", type=int, default=5)
     ap.add_argument("--lattice-steps", type=int, default=1000)
@@ -515,6 +556,8 @@ def main():
     s.close()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_check(args)
+    if not args.no_c1:
+        out["c1_plate128"] = c1_check(args)
     if not args.no_batched:
         out["sweep"] = sweep_check(args)
         out["lattice"] = lattice_check(args)

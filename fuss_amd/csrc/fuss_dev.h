@@ -84,6 +84,47 @@ struct StageArgs {
   double* pmin;        // per-tile dt minima
 };
 
+// Compute_dt (Mod_dt.f90:26-84) for the cells of tile `wg` at iteration
+// `iter` (vnn ramp): local dt stored, the tile's minimum to pmin[wg].
+template <typename R>
+__device__ __forceinline__ void dt_tile(const DtArgs<R>& a, int wg, int iter) {
+  const Tile t = a.tiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
+  double dtc = 1e300;
+  if (i <= B.ni && j <= B.nj) {
+    const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
+    const R T = a.T[p];
+    const int m = B.umat >= 0 ? B.umat : a.mat[p];
+    int bad = 0;
+    const long row = (long)m * a.tab.nT;
+    const R kap = tab_interp(a.tab.kap + row, T, a.tab.Ti1, a.tab.Ti2, &bad);
+    const R rho = tab_interp(a.tab.rho + row, T, a.tab.Ti1, a.tab.Ti2, &bad);
+    const R cp = tab_interp(a.tab.cp + row, T, a.tab.Ti1, a.tab.Ti2, &bad);
+    {
+#pragma clang fp contract(off)
+      const double alpha = (double)kap / (double)rho / (double)cp;
+      double d = 0.5 * a.vnn / (alpha * a.invL2[p]);
+      d = d < 1e8 ? d : 1e8;
+      if (iter < a.rampa) d = d * (double)iter / (double)a.rampa;
+      dtc = d;
+    }
+    a.dt[p] = (R)dtc;
+    if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+  }
+  // work-group minimum
+  __shared__ double sm[TILE_T / 64];
+  for (int o = 32; o > 0; o >>= 1) dtc = fmin(dtc, __shfl_xor(dtc, o, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = dtc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = sm[0];
+    for (int q = 1; q < TILE_T / 64; q++) v = fmin(v, sm[q]);
+    a.pmin[wg] = v;
+  }
+  __syncthreads();  // sm is reused by the next call in a persistent loop
+}
+
 // One interior face flux between cells q and q+e_D (left cell q), in the
 // gather form of Diffusive_Flux (Lib_Diffusive.f90:10-45).
 template <int D, int NDIR, bool TANG, typename R>
@@ -308,6 +349,51 @@ __device__ __forceinline__ R apply_bc_faces(const StageArgs<R>& a, const BlockIn
   if ((fa & 16) && k == 1) Rv = Rv - a.bcf[B.fmap_off[4] + (i - 1) + B.ni * (j - 1)];
   if ((fa & 32) && k == B.nk) Rv = Rv - a.bcf[B.fmap_off[5] + (i - 1) + B.ni * (j - 1)];
   return Rv;
+}
+
+// The value BC entry e subtracts from its cell's residual (BC_Fluxes,
+// Mod_BC_Fluxes.f90:43-79, and the per-type files) at level time `time`:
+// 301/303/304/305 from the compact record, 101/102/300/302 (tangential
+// gradients, donor data) from the full record; 0 for entries without a flux.
+template <int NDIR, typename R>
+__device__ __forceinline__ R bc_entry_value(const StageArgs<R>& a, const BCHot* __restrict__ hot, int e,
+                                            double time) {
+  const int type = hot[e].type;
+  if (!(type == 101 || type == 102 || (type >= 300 && type <= 305))) return (R)0;
+  if (type == 301 || type >= 303) {
+#pragma clang fp contract(off)
+    const BCHot h = hot[e];
+    const R Tc = ldu(a.T, h.cell);
+    int bad = 0, fail = 0;
+    R v;
+    if (type == 301) {
+      // BC_Wall_Heat, Lib_BC_Fluxes_Wall_Heat.f90:12-54 (signed along +xi)
+      R qw = (R)h.qw;
+      if (h.series >= 0) {
+        const int q = h.series;
+        qw = (R)series_value(a.ser.per[q], a.ser.n[q], a.ser.t + a.ser.off[q], a.ser.v + a.ser.off[q], time);
+      }
+      const R Flux = (R)h.A * qw;
+      v = (R)(1 - 2 * (h.f % 2)) * Flux;
+    } else {
+      const R kappa = tab_interp(a.tab.kap + (long)h.m * a.tab.nT, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
+      if (type == 303) v = bcf_htc<R>(kappa, (R)h.mn, (R)h.A, (R)h.hconv, (R)h.Tref, Tc);
+      else v = (R)bcf_rad(type, (double)kappa, h.mn, h.A, h.Tref, h.eps, h.hconv, (double)Tc, &fail);
+    }
+    if (bad || fail) {
+      const BCRec& r = a.bc[e];
+      flag_error(a.status, fail ? FUSS_ESTATE : FUSS_ETABLE, r.b, r.i, r.j, r.k);
+    }
+    return v;
+  }
+  const BCRec& r = a.bc[e];
+  const BlockInfo B = a.blk[r.b];
+  double sv = 0.0;
+  if (r.series >= 0) {
+    const int q = r.series;
+    sv = series_value(a.ser.per[q], a.ser.n[q], a.ser.t + a.ser.off[q], a.ser.v + a.ser.off[q], time);
+  }
+  return bc_value<NDIR>(a, B, r, r.cell, sv);
 }
 
 // BC_Fluxes (Mod_BC_Fluxes.f90:10-86) as a surface pass: one thread per

@@ -249,42 +249,8 @@ __global__ __launch_bounds__(256) void k_f2d(const float* __restrict__ src, doub
 template <typename R>
 __global__ __launch_bounds__(TILE_T) void k_dt(DtArgs<R> a) {
   if (a.cnt[1]) return;
-  const int iter = a.cnt[0] + 1;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const Tile t = a.tiles[wg];
-  const BlockInfo B = a.blk[t.b];
-  const int i = t.i0 + (threadIdx.x % TILE_X), j = t.j0 + (threadIdx.x / TILE_X), k = t.k0;
-  double dtc = 1e300;
-  if (i <= B.ni && j <= B.nj) {
-    const long p = B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
-    const R T = a.T[p];
-    const int m = B.umat >= 0 ? B.umat : a.mat[p];
-    int bad = 0;
-    const long row = (long)m * a.tab.nT;
-    const R kap = tab_interp(a.tab.kap + row, T, a.tab.Ti1, a.tab.Ti2, &bad);
-    const R rho = tab_interp(a.tab.rho + row, T, a.tab.Ti1, a.tab.Ti2, &bad);
-    const R cp = tab_interp(a.tab.cp + row, T, a.tab.Ti1, a.tab.Ti2, &bad);
-    {
-#pragma clang fp contract(off)
-      const double alpha = (double)kap / (double)rho / (double)cp;
-      double d = 0.5 * a.vnn / (alpha * a.invL2[p]);
-      d = d < 1e8 ? d : 1e8;
-      if (iter < a.rampa) d = d * (double)iter / (double)a.rampa;
-      dtc = d;
-    }
-    a.dt[p] = (R)dtc;
-    if (bad) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
-  }
-  // workgroup min, then one atomic per workgroup
-  __shared__ double sm[TILE_T / 64];
-  for (int o = 32; o > 0; o >>= 1) dtc = fmin(dtc, __shfl_xor(dtc, o, 64));
-  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = dtc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double v = sm[0];
-    for (int q = 1; q < TILE_T / 64; q++) v = fmin(v, sm[q]);
-    a.pmin[wg] = v;
-  }
+  dt_tile(a, wg, a.cnt[0] + 1);
 }
 
 template <int NDIR, typename R>
@@ -293,44 +259,9 @@ __global__ void k_bcflux(StageArgs<R> a, const BCHot* __restrict__ hot, int n) {
   if (e >= n || (a.halt && *a.halt)) return;
   const int type = hot[e].type;
   if (!(type == 101 || type == 102 || (type >= 300 && type <= 305))) return;
-  if (type == 301 || type >= 303) {
-#pragma clang fp contract(off)
-    const BCHot h = hot[e];
-    const R Tc = ldu(a.T, h.cell);
-    int bad = 0, fail = 0;
-    R v;
-    if (type == 301) {
-      // BC_Wall_Heat, Lib_BC_Fluxes_Wall_Heat.f90:12-54 (signed along +xi)
-      R qw = (R)h.qw;
-      if (h.series >= 0) {
-        const int q = h.series;
-        qw = (R)series_value(a.ser.per[q], a.ser.n[q], a.ser.t + a.ser.off[q], a.ser.v + a.ser.off[q], *a.clk);
-      }
-      const R Flux = (R)h.A * qw;
-      v = (R)(1 - 2 * (h.f % 2)) * Flux;
-    } else {
-      const R kappa = tab_interp(a.tab.kap + (long)h.m * a.tab.nT, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
-      if (type == 303) v = bcf_htc<R>(kappa, (R)h.mn, (R)h.A, (R)h.hconv, (R)h.Tref, Tc);
-      else v = (R)bcf_rad(type, (double)kappa, h.mn, h.A, h.Tref, h.eps, h.hconv, (double)Tc, &fail);
-    }
-    a.bcv[e] = v;
-    a.bcf[h.fpos] = v;
-    if (bad || fail) {
-      const BCRec& r = a.bc[e];
-      flag_error(a.status, fail ? FUSS_ESTATE : FUSS_ETABLE, r.b, r.i, r.j, r.k);
-    }
-    return;
-  }
-  const BCRec& r = a.bc[e];
-  const BlockInfo B = a.blk[r.b];
-  double sv = 0.0;
-  if (r.series >= 0) {
-    const int q = r.series;
-    sv = series_value(a.ser.per[q], a.ser.n[q], a.ser.t + a.ser.off[q], a.ser.v + a.ser.off[q], *a.clk);
-  }
-  const R v = bc_value<NDIR>(a, B, r, r.cell, sv);
+  const R v = bc_entry_value<NDIR>(a, hot, e, *a.clk);
   a.bcv[e] = v;
-  a.bcf[r.fpos] = v;
+  a.bcf[hot[e].fpos] = v;
 }
 
 // Extract_Properties_From_BC (src/lib/io/IO_Wall.f90:131-239): wall
@@ -1578,6 +1509,8 @@ int fuss_set_bc(fuss_ctx* c, int level, int nbound, const int* ib, const double*
   Level& L = c->lev[level - 1];
   L.nbound = nbound;
   L.hib.assign(ib, ib + 15 * (size_t)nbound);
+  L.has_conn = 0;
+  for (int e = 0; e < nbound; e++) L.has_conn |= ib[15 * (size_t)e + 5] == 101 || ib[15 * (size_t)e + 5] == 102;
   L.hrb.assign(rb, rb + 6 * (size_t)nbound);
   if (series) L.hser.assign(series, series + nbound);
   else L.hser.clear();
@@ -1974,10 +1907,18 @@ int enqueue_step_t(fuss_ctx* c, int level, const fuss_step_params* p, const Stag
                    !c->adj_store_on;
   const bool fuse_dt = !hk && !sdt && xyz && p->rampa_iter <= 1 && !c->adj_store_on;
   const bool have_dt = fuse_dt && L.dt_next && L.dt_vnn == p->vnn;
+  // latency-bound 2-D levels without IRS or connections: fused stages (BC
+  // values in place, mirror ghosts written with each new state, the local
+  // dt in the first stage of steady steps) and one kernel closing the step
+  const bool fused = c->fuse_small && L.kp != GC && !irs && !hk && !c->adj_store_on && !L.has_conn && !L.nchim &&
+                     ntile <= 16384;
+  const bool fdt = fused && !p->time_accurate;
   unsigned nmin = ntile;
   int pid = -1;
   if (sdt) {
     nmin = nstcol;
+  } else if (fdt) {
+    // (the first stage computes it)
   } else if (have_dt) {
     // minima of the fused IRS pass's tiles (its list when dt was written)
     nmin = (unsigned)(L.dt_ty == 8 ? L.hc8tiles.size() : L.hctiles.size());
@@ -2009,7 +1950,7 @@ int enqueue_step_t(fuss_ctx* c, int level, const fuss_step_params* p, const Stag
     return 0;
   };
   // (sdt: after the first stage, which reads neither dtglobal nor the clock)
-  if (!sdt) {
+  if (!sdt && !fdt) {
     const int e = dt_final();
     if (e) return e;
   }
@@ -2022,7 +1963,10 @@ int enqueue_step_t(fuss_ctx* c, int level, const fuss_step_params* p, const Stag
     R* out = (irk % 2 == 1) ? V.Tb[a1] : V.Tb[a2];
     // Fill_Ghost_Cell: only read by tangential gradients (non-orthogonal
     // blocks; 300/302 walls) and connections; a level with neither skips it
-    int e = (L.tang || L.ghost_read || c->adj_store_on) ? launch_ghost(c, L, in) : 0;
+    // (fused stages write the ghosts of their outputs; the step's input gets
+    // its ghosts here)
+    const bool gh = L.tang || L.ghost_read || c->adj_store_on;
+    int e = gh && (!fused || irk == 1) ? launch_ghost(c, L, in) : 0;
     if (e) return e;
     if constexpr (std::is_same<R, double>::value) {
       if (c->adj_store_on)
@@ -2041,7 +1985,7 @@ int enqueue_step_t(fuss_ctx* c, int level, const fuss_step_params* p, const Stag
     if (sdt && irk == 1) {
       sa.dtw = V.dt; sa.vnn = p->vnn; sa.pmin = L.pmin.p;
     }
-    if (L.any_active && L.nbound) {
+    if (L.any_active && L.nbound && !fused) {
       pid = prof_begin(c, FUSS_K_BC);
       if (L.kp == GC) k_bcflux<3, R><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
       else k_bcflux<2, R><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
@@ -2066,6 +2010,14 @@ int enqueue_step_t(fuss_ctx* c, int level, const fuss_step_params* p, const Stag
       if (eh) return eh;
       launch_newstate(c, L, sa, prim, irs);
       if (!irs) { npart = (int)gt; L.plist = 0; }
+    } else if (fused) {
+      DtArgs<R> da{};
+      da.blk = L.blk.p; da.tiles = L.tiles.p; da.T = in; da.mat = L.mat.p; da.invL2 = V.invL2; da.dt = V.dt;
+      da.tab = dtab<R>(c); da.vnn = p->vnn; da.cnt = L.cnt.p; da.rampa = p->rampa_iter; da.pmin = L.pmin.p;
+      da.status = c->status.p;
+      launch_stage_fused(c, L, sa, da, c->ndir, L.tang != 0, prim, fdt && irk == 1, gh);
+      npart = (int)ntile;
+      L.plist = 0;
     } else if (L.kp == GC) {
       launch_stage3d(c, L, sa, L.tang != 0, prim, irs);
       if (!irs) { npart = (int)nstcol; L.plist = stlist; }
@@ -2162,7 +2114,14 @@ int enqueue_step_t(fuss_ctx* c, int level, const fuss_step_params* p, const Stag
     }
     in = out;
   }
-  if (p->compute_residual) {
+  if (fused && (p->compute_residual || fdt)) {
+    // the residual sum (and, when the first stage formed dt, its minimum and
+    // the clock) in one work-group
+    pid = prof_begin(c, FUSS_K_REDUCE);
+    launch_fused_end(c, L, (int)ntile, fdt, p->time_accurate);
+    prof_end(c, pid);
+    HIPCHK(hipGetLastError());
+  } else if (p->compute_residual) {
     pid = prof_begin(c, FUSS_K_REDUCE);
     const int nb1 = (npart + RED_CHUNK - 1) / RED_CHUNK;
     if (nb1 <= RED_FUSE) {
@@ -2697,6 +2656,19 @@ int fuss_graphs(fuss_ctx* c, int enable) {
   return 0;
 }
 
+int fuss_persistent(fuss_ctx* c, int enable) {
+  if (!c) return FUSS_EARG;
+  c->persist = enable != 0;
+  return 0;
+}
+
+int fuss_fused(fuss_ctx* c, int enable) {
+  if (!c) return FUSS_EARG;
+  c->fuse_small = enable != 0;
+  for (auto& L : c->lev) run_graph_free(L);  // captured launches name the kernels
+  return 0;
+}
+
 int fuss_wall(fuss_ctx* c, int level, double* Tw, double* qw) {
   if (!c || !c->setup_done || level < 1 || level > c->nlev || !Tw || !qw) return FUSS_EARG;
   Level& L = c->lev[level - 1];
@@ -2971,6 +2943,131 @@ int fuss_tiles(fuss_ctx* c, int level, int* kc, int* row_march, int* stage_rows,
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Levels the persistent path runs: no IRS, no connection or chimera
+// entries, no profiling, hooks or adjoint recording, at most 16384 tiles.
+bool persistent_eligible(fuss_ctx* c, Level& L, const fuss_step_params* q) {
+  if (q->irs_beta > 0.0 || c->prof.enabled || c->adj_store_on || L.nchim) return false;
+  return L.htiles.size() <= 16384 && !L.has_conn;
+}
+
+// fuss_run through cooperative launches of up to PK_CHUNK steps each
+// (persist.hip): the same steps, stop tests, history and final state as the
+// step loop.
+constexpr int PK_CHUNK = 1 << 15;
+
+template <typename R>
+int run_persistent(fuss_ctx* c, int level, const fuss_step_params* q, int maxcount, double thr, double tthr,
+                   int* n_done, double* hist_res, double* hist_time, fuss_step_result* out) {
+  Level& L = c->lev[level - 1];
+  const int nrk = q->n_rk;
+  if (nrk < 1 || nrk > 4) return FUSS_EARG;
+  const int ntile = (int)L.htiles.size();
+  const bool prim = q->integration == FUSS_PRIM;
+  int grid = 0;
+  int e = persistent_grid<R>(c->ndir, L.tang != 0, prim, ntile, &grid);
+  if (e) return e;
+  // control words (bar[2], ctl[8]), 2 x ntile minima and partials, the
+  // history of one launch
+  const bool hist = hist_res || hist_time;
+  const size_t hsteps = hist ? (size_t)std::min(maxcount, PK_CHUNK) : 0;
+  const size_t need = 64 + 4 * (size_t)ntile * sizeof(double) + 2 * hsteps * sizeof(double);
+  if (L.pk.n < need && L.pk.alloc(need)) return FUSS_EHIP;
+  unsigned char* base = L.pk.p;
+  double* pmin2 = reinterpret_cast<double*>(base + 64);
+  double* part2 = pmin2 + 2 * (size_t)ntile;
+  double* hbuf = part2 + 2 * (size_t)ntile;
+  const FieldView<R> V = fields<R>(c, L);
+  // the first step's stage-1 input needs its ghosts (later states get
+  // theirs from the stage that writes them)
+  const bool ghosts = L.tang || L.ghost_read;
+  if (ghosts && (e = launch_ghost(c, L, V.Tb[L.cur]))) return e;
+  if (read_clock(c, L)) return FUSS_EHIP;
+  PArgs<R> a{};
+  StageArgs<R>& sa = a.sa;
+  sa.blk = L.blk.p; sa.tiles = L.tiles.p; sa.dt = V.dt; sa.dtg = c->scal.p; sa.use_dtg = q->time_accurate;
+  sa.qvol = V.qvol; sa.vol = V.vol; sa.mat = L.mat.p;
+  for (int w = 0; w < 9; w++) sa.W[w] = V.W[w];
+  sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = V.bcv; sa.bcf = V.bcf; sa.tab = dtab<R>(c);
+  sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = nullptr; sa.status = c->status.p;
+  DtArgs<R>& da = a.da;
+  da.blk = L.blk.p; da.tiles = L.tiles.p; da.mat = L.mat.p; da.invL2 = V.invL2; da.dt = V.dt;
+  da.tab = dtab<R>(c); da.vnn = q->vnn; da.cnt = L.cnt.p; da.rampa = q->rampa_iter; da.status = c->status.p;
+  a.hot = L.bch.p;
+  a.gmap = L.gmap.p;
+  for (int b = 0; b < 3; b++) a.Tb[b] = V.Tb[b];
+  a.nrk = nrk;
+  a.jam = nrk == 4;
+  for (int s = 0; s < nrk; s++) a.rkc[s] = nrk == 4 ? RK4A[s] : RKC[nrk - 1][s];
+  a.ntile = ntile;
+  a.time_accurate = q->time_accurate;
+  a.ghosts = ghosts;
+  a.compute_residual = 1;
+  a.thr = thr;
+  a.tthr = tthr;
+  a.clk = L.clk.p;
+  a.cnt = L.cnt.p;
+  a.scal = c->scal.p;
+  a.hres = hist ? hbuf : nullptr;
+  a.htime = hist ? hbuf + hsteps : nullptr;
+  a.pmin2 = pmin2;
+  a.part2 = part2;
+  a.bar = reinterpret_cast<unsigned*>(base);
+  a.ctl = reinterpret_cast<int*>(base + 16);
+  int total = 0;
+  std::vector<double> hb(2 * hsteps);
+  while (total < maxcount) {
+    a.maxcount = std::min(maxcount - total, PK_CHUNK);
+    a.cur0 = L.cur;
+    a.time0 = L.time;
+    a.iter0 = L.iter;
+    HIPCHK(hipMemsetAsync(base, 0, 64, c->stream));
+    e = launch_persistent<R>(c, a, c->ndir, L.tang != 0, prim, grid);
+    if (e) return e;
+    int ctl[8];
+    HIPCHK(hipMemcpyAsync(ctl, a.ctl, sizeof(ctl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (ctl[0]) {
+      fprintf(stderr, "fuss_gpu: a grid barrier of the persistent launch timed out\n");
+      return FUSS_EHIP;
+    }
+    const int done = ctl[2];
+    if (hist && done > 0) {
+      HIPCHK(hipMemcpy(hb.data(), hbuf, 2 * hsteps * sizeof(double), hipMemcpyDeviceToHost));
+      for (int s = 0; s < done; s++) {
+        if (hist_res) hist_res[total + s] = hb[s];
+        if (hist_time) hist_time[total + s] = hb[hsteps + s];
+      }
+    }
+    L.cur = ctl[3];
+    total += done;
+    if (read_clock(c, L)) return FUSS_EHIP;
+    if (ctl[4] || done < a.maxcount) break;
+  }
+  L.dt_next = 0;
+  int bijk[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(c->hscal, c->scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  const int st = sync_status(c, bijk);
+  if (out) {
+    memset(out, 0, sizeof(*out));
+    out->res_sq = c->hscal[2];
+    out->dtglobal = c->hscal[0];
+    out->time = L.time;
+    out->iter = L.iter;
+    out->status = st;
+    for (int k = 0; k < 4; k++) out->bijk[k] = bijk[k];
+  }
+  if (n_done) *n_done = total;
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
 static int step_sync(fuss_ctx* c, int level, const fuss_step_params* p, fuss_step_result* r, const StageHook* hk) {
   if (!c || !c->setup_done || !p || level < 1 || level > c->nlev) return FUSS_EARG;
   int e = enqueue_step(c, level, p, hk);
@@ -3021,6 +3118,18 @@ int fuss_run(fuss_ctx* c, int level, const fuss_step_params* p, int max_iters, i
   // (the steps past the limit are no-ops on the device); a few steps run
   // the loop
   const int G = maxcount >= RUN_G ? RUN_G : 3 * ((maxcount + 2) / 3);
+  if (c->persist && maxcount >= 1 && persistent_eligible(c, c->lev[level - 1], &q)) {
+    // one cooperative launch for all the steps (persist.hip)
+    Level& L = c->lev[level - 1];
+    if (c->precision == FUSS_FP32) {
+      if (state_to_f32(c, L)) return FUSS_EHIP;
+      return run_persistent<float>(c, level, &q, maxcount, res_threshold, time_threshold, n_done, hist_res,
+                                   hist_time, last);
+    }
+    if (state_to_f64(c, L)) return FUSS_EHIP;
+    return run_persistent<double>(c, level, &q, maxcount, res_threshold, time_threshold, n_done, hist_res,
+                                  hist_time, last);
+  }
   if (c->prof.enabled || maxcount < 6 || c->graphs_off) {
     while (n < max_iters) {
       const int e = fuss_step(c, level, &q, &r);

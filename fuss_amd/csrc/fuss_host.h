@@ -136,6 +136,11 @@ struct Level {
     DBuf<float> Tb[3], R, R2, dt, qvol, vol, W[9], bcv, bcf;
   } f;
   int state_f = 0, f_ready = 0;
+  int has_conn = 0;                 // some BC entry is a connection (101) or chimera (102) entry
+  // persistent steps (fuss_persistent): barrier and control words, parity
+  // halves of the tile minima and residual partials, per-step history
+  DBuf<unsigned char> pk;
+  size_t pk_hist = 0;               // steps of history the buffer holds
 };
 
 // The per-cell fields one precision's step reads and writes.
@@ -210,6 +215,8 @@ struct fuss_ctx {
   Adjoint adj;                      // discrete adjoint (fuss_adj_*)
   int adj_store_on = 0;             // enqueue_step copies each stage input to adj.stage[]
   int graphs_off = 0;               // fuss_run without graph capture (fuss_graphs)
+  int persist = 0;                  // fuss_run as persistent launches where eligible (fuss_persistent)
+  int fuse_small = 1;               // fused stages on 2-D levels without IRS and connections (fuss_fused)
   int stage_lds = 1;                // orthogonal 3-D stages: LDS-ring kernel (1) or row march (0)
   int rect_off = 0;                 // per-axis metric factors switched off (fuss_rect)
   double kscale = 1.0;              // conductivity table scale (fuss_set_kscale)
@@ -246,6 +253,42 @@ template <typename R, int TY> void launch_irs_xyz_ty(fuss_ctx* c, Level& L, cons
 template <typename R> void launch_irs_z(fuss_ctx* c, Level& L, const IrsArgs<R>& ia, bool prim);
 template <typename R> void launch_irs_line(fuss_ctx* c, Level& L, int ndir, int dir, const IrsArgs<R>& a, bool update, bool prim);
 template <typename R> void launch_irs_xy(fuss_ctx* c, Level& L, int ndir, const IrsArgs<R>& ia, bool prim);
+// Persistent steps (persist.hip): kernel arguments, resident grid size and
+// the cooperative launch.
+template <typename R>
+struct PArgs {
+  StageArgs<R> sa;       // level arrays; T/TO/Tout and the stage fields are set per stage
+  DtArgs<R> da;
+  const BCHot* hot;
+  const int2* gmap;      // (ghost, source) per BC entry
+  R* Tb[3];
+  int cur0;              // index of T^n at the launch
+  int nrk, jam;
+  double rkc[4];
+  int ntile;
+  int time_accurate;
+  int ghosts;            // write the layer-1 ghosts of each new state
+  int compute_residual;
+  int maxcount;
+  double thr, tthr;
+  double time0;
+  int iter0;
+  double* clk;           // level clock [0] time
+  int* cnt;              // level counters [0] iteration
+  double* scal;          // [0] dtglobal, [1] time, [2] Sum (T-TO)^2
+  double* hres;          // per-step residual and time (or null)
+  double* htime;
+  double* pmin2;         // 2 x ntile: tile dt minima by step parity
+  double* part2;         // 2 x ntile: residual partials by step parity
+  unsigned* bar;         // [0] arrivals, [1] releases
+  int* ctl;              // [0] abort, [1] status snapshot, [2] steps done, [3] final buffer index, [4] stopped
+};
+template <typename R>
+void launch_stage_fused(fuss_ctx* c, Level& L, const StageArgs<R>& a, const DtArgs<R>& da, int ndir, bool tang,
+                        bool prim, bool dt_here, bool ghosts);
+void launch_fused_end(fuss_ctx* c, Level& L, int n, bool dt_here, int time_accurate);
+template <typename R> int persistent_grid(int ndir, bool tang, bool prim, int ntile, int* grid);
+template <typename R> int launch_persistent(fuss_ctx* c, const PArgs<R>& a, int ndir, bool tang, bool prim, int grid);
 template <typename R> int launch_irs_tridiag(fuss_ctx* c, Level& L, IrsArgs<R> ia, R* rin, R* rout, bool prim, int* npart);
 
 }  // namespace fuss

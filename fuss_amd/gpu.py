@@ -108,6 +108,8 @@ def load():
     L.fuss_step_hook.argtypes = [P, C.c_int, C.POINTER(StepParams), STAGE_FN, C.c_void_p, C.POINTER(StepResult)]
     L.fuss_rect_info.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.fuss_graphs.argtypes = [P, C.c_int]
+    L.fuss_persistent.argtypes = [P, C.c_int]
+    L.fuss_fused.argtypes = [P, C.c_int]
     L.fuss_autotune.argtypes = [P, C.c_int, C.POINTER(StepParams), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                 C.POINTER(C.c_double)]
     L.fuss_set_tiles.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -201,6 +203,14 @@ class Solver:
         kc, row, st, ty = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
         _chk(self.L.fuss_tiles(self.h, level, C.byref(kc), C.byref(row), C.byref(st), C.byref(ty)), "fuss_tiles")
         return kc.value, row.value, st.value, ty.value
+
+    def persistent(self, enable):
+        """fuss_run as one persistent launch on eligible levels (fuss_persistent)."""
+        _chk(self.L.fuss_persistent(self.h, int(bool(enable))), "fuss_persistent")
+
+    def fused(self, enable):
+        """Fused stages on 2-D levels without IRS and connections (fuss_fused)."""
+        _chk(self.L.fuss_fused(self.h, int(bool(enable))), "fuss_fused")
 
     def graphs(self, enable):
         """fuss_run on captured HIP graphs (default) or step by step."""

@@ -150,11 +150,34 @@ int fuss_run(fuss_ctx *ctx, int level, const fuss_step_params *p, int max_iters,
              int itermax_left, double res_threshold, double time_threshold, int *n_done,
              double *hist_res, double *hist_time, fuss_step_result *last);
 
-/* fuss_run replays its steps from a captured HIP graph (30 steps per graph,
- * stop tests evaluated on the device, so it stops at the same iteration as a
- * step-by-step loop) unless profiling is on, fewer than 30 steps remain, or
- * graphs are switched off here (enable = 0).  Default: on. */
+/* fuss_run replays its steps from a captured HIP graph (chunks of 30 steps,
+ * or of the request rounded up to a multiple of 3; stop tests evaluated on
+ * the device, so it stops at the same iteration as a step-by-step loop)
+ * unless profiling is on, fewer than 6 steps are asked for, or graphs are
+ * switched off here (enable = 0).  Default: on.  No reference counterpart
+ * (the reference's step loop is Mod_Explicit.f90:10-111 called from
+ * Wrap_Solve.f90:24). */
 int fuss_graphs(fuss_ctx *ctx, int enable);
+
+/* With enable = 1, fuss_run runs all its steps in one persistent cooperative
+ * launch on levels without IRS and without connection (101/102) entries:
+ * the work-groups step through dt, the RK stages (fused fluxes, BC values,
+ * update and mirror ghosts) and the stop tests with grid-wide barriers, for
+ * latency-bound levels where per-step launches dominate.  Same results,
+ * stop iteration and history as the other fuss_run modes.  Other levels
+ * keep the graph/loop path.  Default: off.  No reference counterpart
+ * (replaces the per-step OpenMP region of Mod_Explicit.f90:48-62). */
+int fuss_persistent(fuss_ctx *ctx, int enable);
+
+/* On 2-D levels without IRS and without connection entries, each RK stage
+ * is one fused launch: the stage evaluates its cells' BC values in place
+ * (no surface pass), writes the layer-1 mirror ghosts of the new state (no
+ * ghost pass before the next stage), and on steady steps the first stage
+ * computes the local dt; one kernel closes the step (residual sum, dt
+ * minimum, clock).  Same arithmetic and order as the split passes (which
+ * enable = 0 restores).  Default: on.  No reference counterpart (the
+ * passes are Mod_Explicit.f90:48-62's). */
+int fuss_fused(fuss_ctx *ctx, int enable);
 
 /* Rectilinear levels (3-D, every node of every block on its coordinate
  * planes x = x_i, y = y_j, z = z_k, no tangential weights): the 3-D stage
