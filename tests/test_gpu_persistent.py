@@ -212,3 +212,28 @@ def test_fused_stages_equal_split_passes(gpu, name, graphs):
         out.append((k, hr, ht, np.concatenate([x.ravel() for x in s.T()]), s.clock(), r))
         s.close()
     same(out[0], out[1])
+
+
+@pytest.mark.parametrize("mode", ["fused", "split", "persistent"])
+def test_c1_plate128_to_convergence(gpu, mode):
+    """The 2-D parity configuration (128 x 128 plate, RK3, local dt, vnn 0.8,
+    no IRS) from its seeded state to residual 1e-10 on the GPU: the oracle's
+    stop iteration (fixture tests/golden/plate2d_128_oracle.npz, made by
+    tests/golden/make_plate128.py), its field to 1e-12 and its residual
+    history (every 500th iteration) to 1e-9."""
+    import os
+
+    from tests.test_gpu_parity import relerr
+
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "plate2d_128_oracle.npz"), allow_pickle=False)
+    c = cases.plate2d(128)
+    s = gpu.Solver(c)
+    s.fused(mode != "split")
+    s.persistent(mode == "persistent")
+    n, hist = s.solve()
+    T = s.get("T")
+    s.close()
+    assert n == int(z["stop_iter"]), (n, int(z["stop_iter"]))
+    assert relerr(T, z["T"]) < 1e-12
+    np.testing.assert_allclose(hist[z["res_iters"] - 1, 2], z["res"], rtol=1e-9)
+    assert abs(hist[-1, 2] - float(z["res_last"])) <= 1e-9 * float(z["res_last"])
