@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/fuss_gpu.h"
+#include <mutex>
 #include <type_traits>
 
 #include "fuss_host.h"
@@ -1389,6 +1390,30 @@ int restrict_level(fuss_ctx* c, int fine) {
 
 }  // namespace
 
+// Staging of fuss_wall_noalloc: Tw and qw of up to FUSS_WALL_NOALLOC_MAX
+// entries (4 MiB on every GPU the library runs on, held until the process
+// ends).  Host code uses the per-device address from hipGetSymbolAddress,
+// never the array's name.
+static __device__ __align__(256) double scratch[2 * FUSS_WALL_NOALLOC_MAX];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (fuss_create: the library's set-up call).
+int scratch_setup(int dev) {
+  if (dev < 0 || dev >= MAX_DEVICES) return FUSS_EHIP;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[dev]) return 0;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return FUSS_EHIP;
+  g_scratch[dev] = static_cast<double*>(ptr);
+  return 0;
+}
+}  // namespace
+
 // ------------------------------------------------------------------------
 // C-ABI
 // ------------------------------------------------------------------------
@@ -1410,9 +1435,11 @@ const char* fuss_strerror(int code) {
 
 int fuss_create(int nlevels, int nblocks, int precision, fuss_ctx** out) {
   if (!out || nlevels < 1 || nblocks < 1 || (precision != FUSS_FP64 && precision != FUSS_FP32)) return FUSS_EARG;
-  int ndev = 0;
+  int ndev = 0, dev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return FUSS_EHIP;
+  if (hipGetDevice(&dev) != hipSuccess || scratch_setup(dev)) return FUSS_EHIP;
   fuss_ctx* c = new fuss_ctx();
+  c->dev = dev;
   c->nlev = nlevels;
   c->nb = nblocks;
   c->precision = precision;
@@ -2692,6 +2719,36 @@ int fuss_wall(fuss_ctx* c, int level, double* Tw, double* qw) {
   const int st = sync_status(c);
   out.free();
   return st;
+}
+
+// fuss_wall with the per-GPU staging array instead of an allocation: no
+// hipMalloc / hipFree or allocating call.  Every call on a GPU shares the
+// array (calls that overlap on one GPU race); k_wall writes every entry, so
+// nothing needs clearing first.
+int fuss_wall_noalloc(fuss_ctx* c, int level, double* Tw, double* qw) {
+  if (!c || !c->setup_done || level < 1 || level > c->nlev || !Tw || !qw) return FUSS_EARG;
+  Level& L = c->lev[level - 1];
+  if (!L.nbound) return 0;
+  if (L.nbound > FUSS_WALL_NOALLOC_MAX) {
+    fprintf(stderr, "fuss_wall_noalloc: %d BC entries exceed the staging capacity of %d\n", L.nbound,
+            FUSS_WALL_NOALLOC_MAX);
+    return FUSS_EARG;
+  }
+  double* const stage = (c->dev >= 0 && c->dev < MAX_DEVICES) ? g_scratch[c->dev] : nullptr;
+  if (!stage) return FUSS_EHIP;
+  if (state_to_f64(c, L)) return FUSS_EHIP;
+  double* T = L.Tb[L.cur].p;
+  int e = launch_ghost(c, L, T);
+  if (e) return e;
+  StageArgs<double> sa{};
+  sa.blk = L.blk.p; sa.T = T; sa.mat = L.mat.p; sa.bc = L.bc.p; sa.tab = dtab<double>(c);
+  sa.ser = serdev(c); sa.clk = L.clk.p; sa.status = c->status.p;
+  if (c->ndir == 3) k_wall<3><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound, stage, stage + L.nbound);
+  else k_wall<2><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.nbound, stage, stage + L.nbound);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(Tw, stage, L.nbound * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(qw, stage + L.nbound, L.nbound * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return sync_status(c);
 }
 
 int fuss_rect(fuss_ctx* c, int enable) {

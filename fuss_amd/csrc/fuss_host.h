@@ -196,6 +196,7 @@ struct Adjoint {
 
 struct fuss_ctx {
   int nlev = 0, nb = 0, precision = FUSS_FP64, ndir = 0;
+  int dev = 0;                      // HIP device the context was created on
   std::vector<Level> lev;
   hipStream_t stream = nullptr;
   bool own_stream = false;

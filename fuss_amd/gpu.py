@@ -26,6 +26,7 @@ FUSS_FP64, FUSS_FP32 = 0, 1
 FUSS_CONS, FUSS_PRIM = 0, 1
 FUSS_IRS_JACOBI2, FUSS_IRS_TRIDIAG = 0, 1
 FUSS_ESTATE = 2
+FUSS_EARG = 1
 FUSS_ETABLE = 3
 FUSS_EUNSUPPORTED = 5
 

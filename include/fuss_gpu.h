@@ -302,6 +302,19 @@ int fuss_adj_backward(fuss_ctx *ctx, const double *probe_adj, double *grad_qw, d
  * state, in BC-list order (nbound values each; 0 for entries that are not
  * walls 301-305).  Mirrors the level's ghosts from the current T first. */
 int fuss_wall(fuss_ctx *ctx, int level, double *Tw, double *qw);
+
+/* Largest BC list fuss_wall_noalloc stages (entries per level). */
+#define FUSS_WALL_NOALLOC_MAX 262144
+/* fuss_wall without any allocation: the device staging of Tw and qw is a
+ * static per-GPU array of 2 x FUSS_WALL_NOALLOC_MAX doubles (4 MiB, held by
+ * every GPU the library runs on until the process ends) instead of a buffer
+ * allocated per call, so no hipMalloc / hipFree or allocating call is made.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid it).  Levels with more than FUSS_WALL_NOALLOC_MAX entries are
+ * rejected (FUSS_EARG).  Every context on a GPU shares that GPU's array:
+ * calls that overlap on one GPU race. */
+int fuss_wall_noalloc(fuss_ctx *ctx, int level, double *Tw, double *qw);
 int fuss_get_T(fuss_ctx *ctx, int level, int b, double *T);
 int fuss_set_T(fuss_ctx *ctx, int level, int b, const double *T);
 /* Probes (src/lib/io/IO_Probes.f90:124-166): T of n level-1 cells bijk[4n]. */
