@@ -336,6 +336,8 @@ def lattice_check(args):
         case = cases.lattice(nl=4, lo=16, hi=128, n_rk=4, time_accurate=True)
         s = gpu.Solver(case, precision=prec)
         setup_s = time.perf_counter() - t0
+        if args.no_graphs:
+            s.graphs(False)
         tuned = s.autotune(1, steps=3)
         s.run_level(1, 30, 30, -1.0, 1e300, hist=False)   # warm-up (graph capture, module load)
         s.set_T_all([b.T for b in case.blocks])
@@ -357,6 +359,56 @@ def lattice_check(args):
     out["fp32_bulk"]["speedup_vs_fp64"] = out["fp64"]["seconds"] / out["fp32_bulk"]["seconds"]
     print(f"[bench] lattice FP32 bulk vs FP64 after {out['steps']} steps: max rel err {err:.2e}", file=sys.stderr,
           flush=True)
+    return out
+
+
+def fp32_step_check(args, case, kc, stage_rows, irs_rows, s64):
+    """The same cube steps in an FP32 context (FUSS_FP32: state, fluxes,
+    update and IRS in FP32; dt formed in FP64; FP64 reductions), on the FP64
+    run's tiles: ms per step, per-kernel bandwidth at FP32 bytes, and the
+    state's difference from the FP64 steps after the same count."""
+    from fuss_amd import gpu, perfmodel
+
+    sd = gpu.Solver(case)
+    sf = gpu.Solver(case, precision=gpu.FUSS_FP32)
+    for s in (sd, sf):
+        s.set_tiles(1, kc=kc, row_march=0, stage_rows=stage_rows, irs_rows=irs_rows)
+        s.run_level(1, args.steps, args.steps, -1.0, 1e300, hist=False)
+    device_sync()
+    t0 = time.perf_counter()
+    sf.run_level(1, args.steps, args.steps, -1.0, 1e300, hist=False)
+    device_sync()
+    wall = time.perf_counter() - t0
+    sd.run_level(1, args.steps, args.steps, -1.0, 1e300, hist=False)
+    Td, Tf = sd.get("T"), sf.get("T")
+    sf.L.fuss_profile(sf.h, 1)
+    for _ in range(args.steps):
+        sf.explicit_step(1)
+    device_sync()
+    ms = (ctypes.c_double * 8)()
+    cnt = (ctypes.c_int * 8)()
+    sf.L.fuss_profile_read(sf.h, 8, ms, cnt)
+    sf.L.fuss_profile(sf.h, 0)
+    names = ["dt", "ghost", "stage", "irs", "irs_update", "reduce", "bc", "stage_dt"]
+    kern = {names[q]: {"ms_avg": ms[q] / cnt[q], "launches": cnt[q]} for q in range(8) if cnt[q]}
+    rect = sf.rect_info(1)[0]
+    kb = {"stage": perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, fp=4, mat=False, qvol=False,
+                                         rect=rect),
+          "stage_dt": perfmodel.stage_bytes(ndir=3, tang=False, irs=True, prim=True, fp=4, mat=False, qvol=False,
+                                            rect=rect),
+          "irs_update": perfmodel.irs_fused_bytes(4, mat=False, stage_state=False)}
+    for kname, bpc in kb.items():
+        if kname in kern:
+            gbs = bpc * case.ncell / (kern[kname]["ms_avg"] * 1e-3) / 1e9
+            kern[kname].update({"bytes_per_cell": bpc, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS})
+    sd.close()
+    sf.close()
+    out = {"ms_per_step": wall / args.steps * 1e3, "cell_updates_per_s": case.ncell * args.steps / wall,
+           "speedup_vs_fp64_step": s64 / (wall / args.steps), "kernels": kern,
+           "max_rel_diff_vs_fp64_after_steps": float(np.max(np.abs(Tf - Td) / np.abs(Td))),
+           "steps_compared": 2 * args.steps}
+    print(f"[bench] FP32 step: {out['ms_per_step']:.3f} ms ({out['speedup_vs_fp64_step']:.2f}x FP64), "
+          f"max rel diff {out['max_rel_diff_vs_fp64_after_steps']:.2e}", file=sys.stderr, flush=True)
     return out
 
 
@@ -400,6 +452,38 @@ def c1_check(args):
     return out
 
 
+def summarize_profile(db_path, top=16):
+    """Markdown summary of a rocprofv3 results database (kernel-trace
+    statistics, or per-kernel PMC counter values when the run collected
+    counters): calls, mean / min / max duration and share per kernel; for
+    counters the median and max value per dispatch."""
+    import sqlite3
+
+    db = sqlite3.connect(db_path)
+    cur = db.cursor()
+    lines = []
+    npmc = cur.execute("select count(*) from pmc_events").fetchone()[0]
+    if npmc:
+        lines.append("| kernel | counter | dispatches | median | max |")
+        lines.append("|---|---|---:|---:|---:|")
+        rows = cur.execute("select name, counter_name, counter_value from pmc_events").fetchall()
+        by = {}
+        for name, ctr, v in rows:
+            by.setdefault((name, ctr), []).append(v)
+        for (name, ctr), vs in sorted(by.items(), key=lambda kv: -max(kv[1]))[:top]:
+            lines.append(f"| `{name[:100]}` | {ctr} | {len(vs)} | {float(np.median(vs)):.1f} | {max(vs):.1f} |")
+    else:
+        tot = cur.execute("select sum(duration) from kernels").fetchone()[0] or 1
+        lines.append("| kernel | calls | avg us | min us | max us | % |")
+        lines.append("|---|---:|---:|---:|---:|---:|")
+        q = ("select name, count(*), avg(duration), min(duration), max(duration), sum(duration) from kernels "
+             "group by name order by sum(duration) desc limit ?")
+        for name, n, avg, mn, mx, sm in cur.execute(q, (top,)):
+            lines.append(f"| `{name[:100]}` | {n} | {avg / 1e3:.1f} | {mn / 1e3:.1f} | {mx / 1e3:.1f} | "
+                         f"{100.0 * sm / tot:.1f} |")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -423,13 +507,21 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (inverse problem) measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the sweep and lattice measurements")
     ap.add_argument("--no-c1", action="store_true", help="skip the 2-D plate run to 1e-10")
+    ap.add_argument("--no-fp32-step", action="store_true", help="skip the FP32 explicit step on the cube")
+    ap.add_argument("--only", choices=["lattice", "sweep", "c1", "adjoint"],
+                    help="run this measurement alone (profiling) and print its JSON")
+    ap.add_argument("--no-graphs", action="store_true", help="lattice: fuss_run step by step (no captured graphs)")
     ap.add_argument("--sweep-cases", type=int, default=256)
     ap.add_argument("--sweep-steps", type=int, default=5)
     ap.add_argument("--lattice-steps", type=int, default=1000)
     ap.add_argument("--adj-n", type=int, default=512)
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--adj-lbfgs", type=int, default=20)
+    ap.add_argument("--summarize-profile", metavar="DB", help="print a markdown summary of a rocprofv3 database")
     args = ap.parse_args()
+    if args.summarize_profile:
+        summarize_profile(args.summarize_profile)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding of the path is design-only", file=sys.stderr)
@@ -437,6 +529,10 @@ def main():
 
     from fuss_amd import cases, gpu, perfmodel
 
+    if args.only:
+        fn = {"lattice": lattice_check, "sweep": sweep_check, "c1": c1_check, "adjoint": adjoint_check}[args.only]
+        print(json.dumps({args.only: fn(args)}), flush=True)
+        return
     print(f"[bench] building the {args.n}^3 cube", file=sys.stderr, flush=True)
     case = cases.cube3d(n=args.n, mgl=3, n_rk=args.rk)
     t0 = time.perf_counter()
@@ -551,6 +647,8 @@ def main():
         "canonical_GBs": perfmodel.canonical_bytes(args.rk, True, 3) * value / 1e9,
     }
     ms_step = wall / args.steps * 1e3
+    if not args.no_fp32_step:
+        out["fp32_step"] = fp32_step_check(args, case, tuned[0], stage_rows, irs_rows, wall / args.steps)
     if not args.no_refine:
         out["time_to_1e-10"] = time_to_converge(s, case, args, ms_step)
     s.close()
