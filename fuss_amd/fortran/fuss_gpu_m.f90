@@ -229,6 +229,52 @@ module fuss_gpu_m
       real(c_double), intent(out) :: Tw(*), qw(*)
     end function
 
+    !> the same without device allocation (levels of at most FUSS_WALL_NOALLOC_MAX entries)
+    integer(c_int) function fuss_wall_noalloc(ctx, level, Tw, qw) bind(c, name='fuss_wall_noalloc')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level
+      real(c_double), intent(out) :: Tw(*), qw(*)
+    end function
+
+    !> how fuss_run issues its steps: captured graphs, fused 2-D stages,
+    !> persistent cooperative launches (enable = 0 or 1)
+    integer(c_int) function fuss_graphs(ctx, enable) bind(c, name='fuss_graphs')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function
+
+    integer(c_int) function fuss_fused(ctx, enable) bind(c, name='fuss_fused')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function
+
+    integer(c_int) function fuss_persistent(ctx, enable) bind(c, name='fuss_persistent')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function
+
+    !> column tiles of a 3-D level (kc planes, row march, stage / IRS rows)
+    integer(c_int) function fuss_set_tiles(ctx, level, kc, row_march, stage_rows, irs_rows) &
+        bind(c, name='fuss_set_tiles')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level, kc, row_march, stage_rows, irs_rows
+    end function
+
+    integer(c_int) function fuss_autotune(ctx, level, p, steps, kc, row_march, ms_per_step) &
+        bind(c, name='fuss_autotune')
+      import :: c_int, c_ptr, c_double, fuss_step_params
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: level, steps
+      type(fuss_step_params), intent(in) :: p
+      integer(c_int), intent(out) :: kc, row_march
+      real(c_double), intent(out) :: ms_per_step
+    end function
+
     subroutine fuss_destroy(ctx) bind(c, name='fuss_destroy')
       import :: c_ptr
       type(c_ptr), value :: ctx
