@@ -47,8 +47,12 @@ def knot_table(knots_T, knots_v, nT=NT_DEFAULT):
 
 def nodes_from_axes(x, y, z):
     """Structured node array (nk+1, nj+1, ni+1, 3) from 1-D axis coordinates."""
-    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
-    return np.ascontiguousarray(np.stack([X, Y, Z], axis=-1))
+    x, y, z = (np.asarray(a, np.float64) for a in (x, y, z))
+    out = np.empty((z.size, y.size, x.size, 3))
+    out[..., 0] = x[None, None, :]
+    out[..., 1] = y[None, :, None]
+    out[..., 2] = z[:, None, None]
+    return out
 
 
 def stretch(n, L, a, phi):

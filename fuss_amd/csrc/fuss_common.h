@@ -112,4 +112,70 @@ FUSS_HD void face_normal(const double* p, const double* q, const double* r, cons
   *A = a;
 }
 
+// BC_Connect_Metrics, Lib_Metrics.f90:704-936, first ghost layer of a 101
+// entry: the ghost cell's eight nodes -- the four of the block's own face and
+// four of the donor block's node layer beyond its connected face, at the
+// donor stencil's offsets -- and their metric tensor Mg(1).  nm / ns: node
+// (0,0,0) of the block / of the donor block ((n+1)^3 nodes, i fastest, three
+// doubles each); q: the entry's 15 integers (b,i,j,k,f,type,bs,is,js,ks,fs,
+// d11,d12,d21,d22).  Returns 0, 1 when a node lies outside its block (or the
+// face is invalid), 2 when the metric is degenerate.
+FUSS_HD int connect_metric_nodes(const double* nm, int nim, int njm, int nkm, const double* ns, int nis, int njs,
+                                 int nks, const int* q, double* Mg1) {
+  auto node = [](const double* base, int ni, int nj, int nk, int i, int j, int k) -> const double* {
+    if (i < 0 || i > ni || j < 0 || j > nj || k < 0 || k > nk) return nullptr;
+    return base + 3 * (i + (long)(ni + 1) * (j + (long)(nj + 1) * k));
+  };
+  auto Nm = [&](int i, int j, int k) { return node(nm, nim, njm, nkm, i, j, k); };
+  auto Ns = [&](int i, int j, int k) { return node(ns, nis, njs, nks, i, j, k); };
+  const int Im = q[1], Jm = q[2], Km = q[3], Fm = q[4];
+  const int Is = q[7], Js = q[8], Ks = q[9], Fs = q[10], d11 = q[11], d12 = q[12], d21 = q[13], d22 = q[14];
+  if (Fs < 1 || Fs > 6) return 1;
+  const double* P0[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  const double* P1[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  switch (Fm) {
+    case 1: P0[0] = Nm(Im - 1, Jm - 1, Km - 1); P0[1] = Nm(Im - 1, Jm - 1, Km); P0[2] = Nm(Im - 1, Jm, Km - 1); P0[3] = Nm(Im - 1, Jm, Km); break;
+    case 2: P0[4] = Nm(Im, Jm - 1, Km - 1); P0[5] = Nm(Im, Jm - 1, Km); P0[6] = Nm(Im, Jm, Km - 1); P0[7] = Nm(Im, Jm, Km); break;
+    case 3: P0[0] = Nm(Im - 1, Jm - 1, Km - 1); P0[1] = Nm(Im - 1, Jm - 1, Km); P0[4] = Nm(Im, Jm - 1, Km - 1); P0[5] = Nm(Im, Jm - 1, Km); break;
+    case 4: P0[2] = Nm(Im - 1, Jm, Km - 1); P0[3] = Nm(Im - 1, Jm, Km); P0[6] = Nm(Im, Jm, Km - 1); P0[7] = Nm(Im, Jm, Km); break;
+    case 5: P0[0] = Nm(Im - 1, Jm - 1, Km - 1); P0[2] = Nm(Im - 1, Jm, Km - 1); P0[4] = Nm(Im, Jm - 1, Km - 1); P0[6] = Nm(Im, Jm, Km - 1); break;
+    case 6: P0[1] = Nm(Im - 1, Jm - 1, Km); P0[3] = Nm(Im - 1, Jm, Km); P0[5] = Nm(Im, Jm - 1, Km); P0[7] = Nm(Im, Jm, Km); break;
+    default: return 1;
+  }
+  // g = 1: donor node index Is1 = Is + (gc-1) guides(Fs), the node layer on
+  // the min side of a min face
+  const int gsd = (Fs & 1) ? 0 : -1, gax = (Fs - 1) >> 1;
+  const int Is1 = Is + (gax == 0 ? gsd : 0), Js1 = Js + (gax == 1 ? gsd : 0), Ks1 = Ks + (gax == 2 ? gsd : 0);
+  int II, JJ, KK;
+  if (Fs <= 2) { II = Is1; JJ = (2 * Js1 + d11 + d21) / 2; KK = (2 * Ks1 + d12 + d22) / 2; }
+  else if (Fs <= 4) { II = (2 * Is1 + d11 + d21) / 2; JJ = Js1; KK = (2 * Ks1 + d12 + d22) / 2; }
+  else { II = (2 * Is1 + d11 + d21) / 2; JJ = (2 * Js1 + d12 + d22) / 2; KK = Ks1; }
+  // donor node shifted by -a1 along the first tangent and -a2 along the second
+  auto donor = [&](int a1, int a2) -> const double* {
+    if (Fs <= 2) return Ns(II, JJ - a1 * d11 - a2 * d21, KK - a1 * d12 - a2 * d22);
+    if (Fs <= 4) return Ns(II - a1 * d11 - a2 * d21, JJ, KK - a1 * d12 - a2 * d22);
+    return Ns(II - a1 * d11 - a2 * d21, JJ - a1 * d12 - a2 * d22, KK);
+  };
+  // slots of the four donor nodes per local face
+  const int q1[6] = {4, 8, 6, 8, 7, 8}, q2[6] = {2, 6, 2, 4, 3, 4}, q3[6] = {3, 7, 5, 7, 5, 6}, q4[6] = {1, 5, 1, 3, 1, 2};
+  P1[q1[Fm - 1] - 1] = donor(0, 0);
+  P1[q2[Fm - 1] - 1] = donor(1, 0);
+  P1[q3[Fm - 1] - 1] = donor(0, 1);
+  P1[q4[Fm - 1] - 1] = donor(1, 1);
+  // argument lists per face (Lib_Metrics.f90:192-229): g = 1 nodes in P1, g - 1 = 0 in P0
+  const double* n1[8];
+  switch (Fm) {
+    case 1: n1[0] = P1[0]; n1[1] = P1[1]; n1[2] = P1[2]; n1[3] = P1[3]; n1[4] = P0[0]; n1[5] = P0[1]; n1[6] = P0[2]; n1[7] = P0[3]; break;
+    case 2: n1[0] = P0[4]; n1[1] = P0[5]; n1[2] = P0[6]; n1[3] = P0[7]; n1[4] = P1[4]; n1[5] = P1[5]; n1[6] = P1[6]; n1[7] = P1[7]; break;
+    case 3: n1[0] = P1[0]; n1[1] = P1[1]; n1[2] = P0[0]; n1[3] = P0[1]; n1[4] = P1[4]; n1[5] = P1[5]; n1[6] = P0[4]; n1[7] = P0[5]; break;
+    case 4: n1[0] = P0[2]; n1[1] = P0[3]; n1[2] = P1[2]; n1[3] = P1[3]; n1[4] = P0[6]; n1[5] = P0[7]; n1[6] = P1[6]; n1[7] = P1[7]; break;
+    case 5: n1[0] = P1[0]; n1[1] = P0[0]; n1[2] = P1[2]; n1[3] = P0[2]; n1[4] = P1[4]; n1[5] = P0[4]; n1[6] = P1[6]; n1[7] = P0[6]; break;
+    default: n1[0] = P0[1]; n1[1] = P1[1]; n1[2] = P0[3]; n1[3] = P1[3]; n1[4] = P0[5]; n1[5] = P1[5]; n1[6] = P0[7]; n1[7] = P1[7]; break;
+  }
+  for (int s = 0; s < 8; s++)
+    if (!n1[s]) return 1;
+  double dl[3], v;
+  return metric_tensor(n1[0], n1[1], n1[2], n1[3], n1[4], n1[5], n1[6], n1[7], Mg1, dl, &v) ? 2 : 0;
+}
+
 }  // namespace fuss

@@ -199,6 +199,150 @@ __global__ void k_bc_geometry(BCRec* bc, int n, const BlockInfo* blk, const doub
   }
 }
 
+// Setup_BC per entry on the device (the set-up of levels without chimera
+// entries): the entry's record (cell, layer-1 ghost and its source, donor
+// cells of Tg(3:6) and of the IRS ghost, the BC values, the connection's
+// ghost metric Mg(1) from the device nodes), its (ghost, source) pair and its
+// face-map slot, as build_bc forms them on the host.  Checks of build_bc
+// become error keys (entry << 8 | code, the smallest one wins); the face
+// activity and connection bits go to bbits[2 b], bbits[2 b + 1], the level's
+// x / y / z connection and ghost-read flags to lflags[0..3].
+struct BcBuildArgs {
+  const int* ib;           // 15 integers per entry
+  const double* rb;        // 6 values per entry
+  const int* ser;          // time series per entry, or null
+  int n, nb;
+  const BlockInfo* blk;
+  const double* nodes;
+  BCRec* rec;
+  int2* gm;
+  int* fmap;               // -1 before the build
+  int* bbits;
+  int* lflags;
+  unsigned long long* err;
+};
+
+__device__ __forceinline__ long cidx_d(const BlockInfo& B, int i, int j, int k) {
+  return B.base + (i - 1) + (long)(j - 1) * B.sy + (long)(k - 1) * B.sz;
+}
+
+__global__ void k_build_bc(BcBuildArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.n) return;
+  auto fail = [&](int code) { atomicMin(a.err, ((unsigned long long)e << 8) | (unsigned)code); };
+  const int* q = a.ib + 15 * (long)e;
+  const double* r = a.rb + 6 * (long)e;
+  // grouping: (block, face) non-decreasing (Mod_BC_Fluxes.f90:26-33)
+  if (e > 0) {
+    const int* p = q - 15;
+    if (q[0] < p[0] || (q[0] == p[0] && q[4] < p[4])) atomicMin(a.err, (unsigned long long)FUSS_EBC);
+  }
+  const int b = q[0] - 1, i = q[1], j = q[2], k = q[3], f = q[4], type = q[5];
+  if (b < 0 || b >= a.nb || f < 1 || f > 6) return fail(FUSS_EBC);
+  const BlockInfo B = a.blk[b];
+  if (i < 1 || i > B.ni || j < 1 || j > B.nj || k < 1 || k > B.nk) return fail(FUSS_EBC);
+  const bool on = (f == 1 && i == 1) || (f == 2 && i == B.ni) || (f == 3 && j == 1) || (f == 4 && j == B.nj) ||
+                  (f == 5 && k == 1) || (f == 6 && k == B.nk);
+  if (!on) return fail(FUSS_EBC);
+  const int mi = f <= 2 ? (j - 1) + B.nj * (k - 1) : (f <= 4 ? (i - 1) + B.ni * (k - 1) : (i - 1) + B.ni * (j - 1));
+  const int fpos = B.fmap_off[f - 1] + mi;
+  if (atomicCAS(a.fmap + fpos, -1, e) != -1) return fail(FUSS_EBC);
+  if (type == 102) return fail(FUSS_EUNSUPPORTED);  // chimera levels are built on the host
+  BCRec R;
+  memset(&R, 0, sizeof(R));
+  R.type = type; R.f = f; R.dir = (f + 1) / 2; R.series = a.ser ? a.ser[e] : -1;
+  R.b = b; R.i = i; R.j = j; R.k = k;
+  R.cell = cidx_d(B, i, j, k);
+  R.fpos = fpos;
+  R.qw = r[0]; R.Tw = r[1]; R.hconv = r[2]; R.Tref = r[4]; R.eps = r[5];
+  R.gdst = cidx_d(B, i - guide(f, 0), j - guide(f, 1), k - guide(f, 2));
+  R.gsrc = R.cell;
+  R.rdonor = -1;
+  for (int t = 0; t < 4; t++) R.dn[t] = R.cell;
+  if (type == 101) {
+    const int bs = q[6] - 1, Is = q[7], Js = q[8], Ks = q[9], Fs = q[10];
+    if (bs < 0 || bs >= a.nb || Fs < 1 || Fs > 6) return fail(FUSS_EBC);
+    const BlockInfo S = a.blk[bs];
+    if (Is < 1 || Is > S.ni || Js < 1 || Js > S.nj || Ks < 1 || Ks > S.nk) return fail(FUSS_EBC);
+    R.gsrc = cidx_d(S, Is, Js, Ks);
+    R.rdonor = R.gsrc;
+    const int d11 = q[11], d12 = q[12], d21 = q[13], d22 = q[14];
+    for (int t = 0; t < 4; t++) {
+      // Tg(3:6): the donor's neighbours along its face's two tangents
+      const int sg = (t & 1) ? 1 : -1, da = t < 2 ? d11 : d21, db = t < 2 ? d12 : d22;
+      int pi = Is, pj = Js, pk = Ks;
+      if (Fs <= 2) { pj += sg * da; pk += sg * db; }
+      else if (Fs <= 4) { pi += sg * da; pk += sg * db; }
+      else { pi += sg * da; pj += sg * db; }
+      if (pi < 0 || pi > S.ni + 1 || pj < 0 || pj > S.nj + 1 || pk < 0 || pk > S.nk + 1) return fail(FUSS_EBC);
+      R.dn[t] = cidx_d(S, pi, pj, pk);
+    }
+    const int me = connect_metric_nodes(a.nodes + 3 * B.node_off, B.ni, B.nj, B.nk, a.nodes + 3 * S.node_off, S.ni,
+                                        S.nj, S.nk, q, R.Mg);
+    if (me) return fail(me == 1 ? FUSS_EBC : FUSS_EMETRIC);
+  }
+  a.rec[e] = R;
+  a.gm[e] = make_int2((int)R.gdst, (int)R.gsrc);
+  const bool active = type == 101 || (type >= 300 && type <= 305);
+  if (active) atomicOr(a.bbits + 2 * b, 1 << (f - 1));
+  if (type == 101) {
+    atomicOr(a.bbits + 2 * b + 1, 1 << (f - 1));
+    atomicOr(a.lflags + (f - 1) / 2, 1);
+  }
+  if (type == 101 || type == 300 || type == 302) atomicOr(a.lflags + 3, 1);
+}
+
+// every face cell of the level has its entry
+__global__ void k_fmap_check(const int* fmap, long n, int nent, unsigned long long* err) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+    if (fmap[q] < 0) atomicMin(err, ((unsigned long long)nent << 8) | (unsigned)FUSS_EBC);
+}
+
+// Level-1 fields of block B from the caller's arrays (i fastest, nc cells):
+// T, qvol and the 1-based material index into the padded arena; the material
+// range is checked (err = 1) and its min / max kept in mm[0], mm[1].
+__global__ void k_scatter_fields(BlockInfo B, const double* T, const double* qv, const double* mid, double* Td,
+                                 double* qd, int* md, int nmat, int* mm, int* err) {
+  const long nc = (long)B.ni * B.nj * B.nk;
+  int lo = 0x7fffffff, hi = -0x7fffffff;
+  for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < nc; s += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(s % B.ni), j = (int)((s / B.ni) % B.nj), k = (int)(s / ((long)B.ni * B.nj));
+    const long p = B.base + i + (long)j * B.sy + (long)k * B.sz;
+    Td[p] = T[s];
+    qd[p] = qv[s];
+    const int m = (int)mid[s] - 1;
+    if (m < 0 || m >= nmat) *err = 1;
+    md[p] = m;
+    lo = min(lo, m);
+    hi = max(hi, m);
+  }
+  atomicMin(mm, lo);
+  atomicMax(mm + 1, hi);
+}
+
+// matIDg of the connection entries: the material of the donor cell
+// (Ghost_Connection copies it, Fill_matIDg, Lib_Ghost.f90:149-153, :436-470)
+__global__ void k_set_matg(BCRec* bc, int n, const int* mat) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n && bc[e].type == 101) bc[e].matg = mat[bc[e].gsrc];
+}
+
+// blocks whose qvol differs from the value of their cell (1,1,1): nonu[b] = 1
+__global__ void k_qvol_nonuniform(const BlockInfo* blk, const double* q, int* nonu) {
+  const BlockInfo B = blk[blockIdx.x];
+  const double q0 = q[B.base];
+  int bad = 0;
+  for (int k = blockIdx.y; k < B.nk && !bad; k += gridDim.y)
+    for (int j = 0; j < B.nj && !bad; j++)
+      for (int i = threadIdx.x; i < B.ni; i += blockDim.x)
+        if (q[B.base + i + (long)j * B.sy + (long)k * B.sz] != q0) bad = 1;
+  if (bad) nonu[blockIdx.x] = 1;
+}
+
+__global__ void k_fill_i32(int* p, long n, int v) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) p[q] = v;
+}
+
 // ------------------------------------------------------------------------
 // kernels: ghost fill (per stage)
 // ------------------------------------------------------------------------
@@ -762,65 +906,9 @@ int rect_prepare(fuss_ctx* c, Level& L) {
 // BC_Connect_Metrics, Lib_Metrics.f90:704-936: Mg(1) of a 101 entry from the
 // local face nodes and the donor's nodes (host side, surface only).
 int connect_metric(const HostBlock& bm, const HostBlock& bs, const int* q, double* Mg1) {
-  auto N = [](const HostBlock& h, int i, int j, int k) -> const double* {
-    if (i < 0 || i > h.n[0] || j < 0 || j > h.n[1] || k < 0 || k > h.n[2]) return nullptr;
-    return &h.nodes[3 * (i + (long)(h.n[0] + 1) * (j + (long)(h.n[1] + 1) * k))];
-  };
-  const int Im = q[1], Jm = q[2], Km = q[3], Fm = q[4];
-  const int Is = q[7], Js = q[8], Ks = q[9], Fs = q[10], d11 = q[11], d12 = q[12], d21 = q[13], d22 = q[14];
-  static const int gs[6][3] = {{0, 0, 0}, {-1, 0, 0}, {0, 0, 0}, {0, -1, 0}, {0, 0, 0}, {0, 0, -1}};
-  const double* P0[8] = {nullptr};
-  const double* P1[8] = {nullptr};
-  switch (Fm) {
-    case 1: P0[0] = N(bm, Im - 1, Jm - 1, Km - 1); P0[1] = N(bm, Im - 1, Jm - 1, Km); P0[2] = N(bm, Im - 1, Jm, Km - 1); P0[3] = N(bm, Im - 1, Jm, Km); break;
-    case 2: P0[4] = N(bm, Im, Jm - 1, Km - 1); P0[5] = N(bm, Im, Jm - 1, Km); P0[6] = N(bm, Im, Jm, Km - 1); P0[7] = N(bm, Im, Jm, Km); break;
-    case 3: P0[0] = N(bm, Im - 1, Jm - 1, Km - 1); P0[1] = N(bm, Im - 1, Jm - 1, Km); P0[4] = N(bm, Im, Jm - 1, Km - 1); P0[5] = N(bm, Im, Jm - 1, Km); break;
-    case 4: P0[2] = N(bm, Im - 1, Jm, Km - 1); P0[3] = N(bm, Im - 1, Jm, Km); P0[6] = N(bm, Im, Jm, Km - 1); P0[7] = N(bm, Im, Jm, Km); break;
-    case 5: P0[0] = N(bm, Im - 1, Jm - 1, Km - 1); P0[2] = N(bm, Im - 1, Jm, Km - 1); P0[4] = N(bm, Im, Jm - 1, Km - 1); P0[6] = N(bm, Im, Jm, Km - 1); break;
-    case 6: P0[1] = N(bm, Im - 1, Jm - 1, Km); P0[3] = N(bm, Im - 1, Jm, Km); P0[5] = N(bm, Im, Jm - 1, Km); P0[7] = N(bm, Im, Jm, Km); break;
-    default: return FUSS_EBC;
-  }
-  // g = 1: donor node index Is1 = Is + (gc-1)*guides(Fs)
-  const int Is1 = Is + gs[Fs - 1][0], Js1 = Js + gs[Fs - 1][1], Ks1 = Ks + gs[Fs - 1][2];
-  int II, JJ, KK;
-  if (Fs <= 2) { II = Is1; JJ = (2 * Js1 + d11 + d21) / 2; KK = (2 * Ks1 + d12 + d22) / 2; }
-  else if (Fs <= 4) { II = (2 * Is1 + d11 + d21) / 2; JJ = Js1; KK = (2 * Ks1 + d12 + d22) / 2; }
-  else { II = (2 * Is1 + d11 + d21) / 2; JJ = (2 * Js1 + d12 + d22) / 2; KK = Ks1; }
-  auto off = [&](int a1, int a2, int* o) {  // donor index shifted by -a1 (first tangent) -a2 (second)
-    if (Fs <= 2) { o[0] = II; o[1] = JJ - a1 * d11 - a2 * d21; o[2] = KK - a1 * d12 - a2 * d22; }
-    else if (Fs <= 4) { o[0] = II - a1 * d11 - a2 * d21; o[1] = JJ; o[2] = KK - a1 * d12 - a2 * d22; }
-    else { o[0] = II - a1 * d11 - a2 * d21; o[1] = JJ - a1 * d12 - a2 * d22; o[2] = KK; }
-  };
-  static const int q1[6] = {4, 8, 6, 8, 7, 8}, q2[6] = {2, 6, 2, 4, 3, 4}, q3[6] = {3, 7, 5, 7, 5, 6},
-                   q4[6] = {1, 5, 1, 3, 1, 2};
-  int o[3];
-  off(0, 0, o); P1[q1[Fm - 1] - 1] = N(bs, o[0], o[1], o[2]);
-  off(1, 0, o); P1[q2[Fm - 1] - 1] = N(bs, o[0], o[1], o[2]);
-  off(0, 1, o); P1[q3[Fm - 1] - 1] = N(bs, o[0], o[1], o[2]);
-  off(1, 1, o); P1[q4[Fm - 1] - 1] = N(bs, o[0], o[1], o[2]);
-  for (int qq = 0; qq < 8; qq++) {
-    const bool need0 = P0[qq] != nullptr;
-    if (!need0 && !P1[qq]) continue;
-  }
-  const double* n1[8];
-  // argument lists per face (Lib_Metrics.f90:192-229), g=1 nodes in P1, g-1=0 in P0
-#define A1(x) P1[(x)-1]
-#define A0(x) P0[(x)-1]
-  switch (Fm) {
-    case 1: { const double* t[8] = {A1(1), A1(2), A1(3), A1(4), A0(1), A0(2), A0(3), A0(4)}; memcpy(n1, t, sizeof(t)); } break;
-    case 2: { const double* t[8] = {A0(5), A0(6), A0(7), A0(8), A1(5), A1(6), A1(7), A1(8)}; memcpy(n1, t, sizeof(t)); } break;
-    case 3: { const double* t[8] = {A1(1), A1(2), A0(1), A0(2), A1(5), A1(6), A0(5), A0(6)}; memcpy(n1, t, sizeof(t)); } break;
-    case 4: { const double* t[8] = {A0(3), A0(4), A1(3), A1(4), A0(7), A0(8), A1(7), A1(8)}; memcpy(n1, t, sizeof(t)); } break;
-    case 5: { const double* t[8] = {A1(1), A0(1), A1(3), A0(3), A1(5), A0(5), A1(7), A0(7)}; memcpy(n1, t, sizeof(t)); } break;
-    default: { const double* t[8] = {A0(2), A1(2), A0(4), A1(4), A0(6), A1(6), A0(8), A1(8)}; memcpy(n1, t, sizeof(t)); } break;
-  }
-#undef A1
-#undef A0
-  for (int qq = 0; qq < 8; qq++)
-    if (!n1[qq]) return FUSS_EBC;
-  double dl[3], v;
-  const int e = metric_tensor(n1[0], n1[1], n1[2], n1[3], n1[4], n1[5], n1[6], n1[7], Mg1, dl, &v);
-  return e ? FUSS_EMETRIC : 0;
+  const int e = connect_metric_nodes(bm.nodes.data(), bm.n[0], bm.n[1], bm.n[2], bs.nodes.data(), bs.n[0], bs.n[1],
+                                     bs.n[2], q, Mg1);
+  return e == 1 ? FUSS_EBC : (e ? FUSS_EMETRIC : 0);
 }
 
 // BC_Extrapolate_Metrics, Lib_Metrics.f90:379-600, first ghost layer: the
@@ -1178,6 +1266,89 @@ int build_bc(fuss_ctx* c, Level& L) {
   if (!fmap.empty()) e |= upload(L.fmap.p, fmap.data(), sizeof(int) * fmap.size());
   (void)c;
   return e;
+}
+
+// Face-map offsets of every block: faces 1..6 of block 1, then block 2, ...
+// (one slot per boundary face cell); returns the face map's length.
+size_t face_map_offsets(Level& L) {
+  size_t nf = 0;
+  for (auto& B : L.hblk)
+    for (int f = 1; f <= 6; f++) {
+      B.fmap_off[f - 1] = (int)nf;
+      nf += (size_t)(f <= 2 ? B.nj * B.nk : (f <= 4 ? B.ni * B.nk : B.ni * B.nj));
+    }
+  return nf;
+}
+
+bool level_has_chimera(const Level& L) {
+  for (int e = 0; e < L.nbound; e++)
+    if (L.hib[15 * (size_t)e + 5] == 102) return true;
+  return !L.hch_ent.empty();
+}
+
+// build_bc's records, ghost pairs and face map formed on the device
+// (k_build_bc) for a level without chimera entries; needs the level's nodes
+// and block infos with the face-map offsets of face_map_offsets on the device.
+// Merges the face activity and connection bits into L.hblk.
+int build_bc_dev(fuss_ctx* c, Level& L, size_t nf) {
+  const int n = L.nbound;
+  L.no_ycon = L.no_zcon = L.no_xcon = 1;
+  L.ghost_read = 0;
+  L.nchim = 0;
+  int e = L.bc.alloc(std::max(n, 1));
+  e |= L.bch.alloc(std::max(n, 1));
+  e |= L.gmap.alloc(std::max(n, 1));
+  e |= L.bcv.alloc(std::max(n, 1));
+  e |= L.bcf.alloc(std::max<size_t>(nf, 1));
+  e |= L.fmap.alloc(std::max<size_t>(nf, 1));
+  if (e) return FUSS_EHIP;
+  DBuf<int> dib, dser, bits, flags;
+  DBuf<double> drb;
+  DBuf<unsigned long long> err;
+  if (bits.alloc(2 * (size_t)L.nb) || flags.alloc(4) || err.alloc(1)) return FUSS_EHIP;
+  HIPCHK(hipMemsetAsync(err.p, 0xff, sizeof(unsigned long long), c->stream));
+  if (nf) {
+    k_fill_i32<<<(unsigned)std::min<size_t>((nf + 255) / 256, 4096), 256, 0, c->stream>>>(L.fmap.p, (long)nf, -1);
+    HIPCHK(hipGetLastError());
+  }
+  if (n) {
+    if (dib.alloc(15 * (size_t)n) || drb.alloc(6 * (size_t)n) || (!L.hser.empty() && dser.alloc(n)))
+      return FUSS_EHIP;
+    e = upload(dib.p, L.hib.data(), sizeof(int) * 15 * (size_t)n);
+    e |= upload(drb.p, L.hrb.data(), sizeof(double) * 6 * (size_t)n);
+    if (!L.hser.empty()) e |= upload(dser.p, L.hser.data(), sizeof(int) * (size_t)n);
+    if (e) return e;
+    BcBuildArgs a{};
+    a.ib = dib.p; a.rb = drb.p; a.ser = L.hser.empty() ? nullptr : dser.p;
+    a.n = n; a.nb = L.nb; a.blk = L.blk.p; a.nodes = L.nodes.p;
+    a.rec = L.bc.p; a.gm = L.gmap.p; a.fmap = L.fmap.p; a.bbits = bits.p; a.lflags = flags.p; a.err = err.p;
+    k_build_bc<<<(n + 127) / 128, 128, 0, c->stream>>>(a);
+    HIPCHK(hipGetLastError());
+  }
+  if (nf) {
+    k_fmap_check<<<(unsigned)std::min<size_t>((nf + 255) / 256, 4096), 256, 0, c->stream>>>(L.fmap.p, (long)nf, n,
+                                                                                           err.p);
+    HIPCHK(hipGetLastError());
+  }
+  unsigned long long herr = 0;
+  std::vector<int> hbits(2 * (size_t)L.nb);
+  int hflags[4];
+  HIPCHK(hipMemcpyAsync(&herr, err.p, sizeof(herr), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(hbits.data(), bits.p, sizeof(int) * hbits.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(hflags, flags.p, sizeof(hflags), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (herr != ~0ULL) return (int)(herr & 0xff);
+  L.any_active = 0;
+  for (int b = 0; b < L.nb; b++) {
+    L.hblk[b].face_active |= hbits[2 * b];
+    L.hblk[b].conn |= hbits[2 * b + 1];
+    L.any_active |= L.hblk[b].face_active;
+  }
+  L.no_xcon = !hflags[0];
+  L.no_ycon = !hflags[1];
+  L.no_zcon = !hflags[2];
+  L.ghost_read = hflags[3];
+  return 0;
 }
 
 int sync_status(fuss_ctx* c, int* bijk = nullptr) {
@@ -1607,22 +1778,32 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     L.kp = c->ndir == 2 ? 1 : GC;
     int e = build_level_layout(c, L);
     if (e) return e;
-    e = build_bc(c, L);  // fills fmap offsets and face_active into hblk
-    if (e) return e;
+    // BC records: on the device, or on the host for a level with chimera
+    // entries (donor lists, extrapolated ghost metrics); both fill the face
+    // map offsets and the face activity into hblk
+    const bool host_bc = level_has_chimera(L);
+    if (host_bc) {
+      e = build_bc(c, L);
+      if (e) return e;
+    }
     e = alloc_level(c, L);
     if (e) return e;
-    // re-upload block infos after build_bc (face maps, activity)
     e = L.blk.alloc(L.nb);
     if (e) return e;
     e = upload(L.tiles.p, L.htiles.data(), sizeof(Tile) * L.htiles.size());
     if (!L.hctiles.empty()) e |= upload(L.ctiles.p, L.hctiles.data(), sizeof(Tile) * L.hctiles.size());
     if (!L.hc8tiles.empty()) e |= upload(L.c8tiles.p, L.hc8tiles.data(), sizeof(Tile) * L.hc8tiles.size());
     e |= upload(L.xytiles.p, L.hxytiles.data(), sizeof(Tile) * L.hxytiles.size());
-    // nodes
-    std::vector<double> nodes(3 * L.nnodes);
-    for (int b = 0; b < L.nb; b++)
-      std::copy(L.hb[b].nodes.begin(), L.hb[b].nodes.end(), nodes.begin() + 3 * L.hblk[b].node_off);
-    e |= upload(L.nodes.p, nodes.data(), nodes.size() * 8);
+    // nodes, each block's straight into its place in the arena
+    for (int b = 0; b < L.nb && !e; b++)
+      e |= upload(L.nodes.p + 3 * L.hblk[b].node_off, L.hb[b].nodes.data(), L.hb[b].nodes.size() * 8);
+    if (!host_bc) {
+      const size_t nf = face_map_offsets(L);
+      e |= upload(L.blk.p, L.hblk.data(), sizeof(BlockInfo) * L.nb);
+      if (e) return e;
+      e = build_bc_dev(c, L, nf);
+      if (e) return e;
+    }
     e |= upload(L.blk.p, L.hblk.data(), sizeof(BlockInfo) * L.nb);
     if (e) return e;
     // cell metrics
@@ -1690,29 +1871,46 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
       HIPCHK(hipMemcpy(c->ser_per.p, sp.data(), sp.size() * 4, hipMemcpyHostToDevice));
     }
   }
-  // level-1 fields: T, material index, qvol
+  // level-1 fields: T, material index, qvol, scattered on the device from
+  // each block's arrays
   {
     Level& L = c->lev[0];
-    std::vector<double> T(L.npad, 0.0), q(L.npad, 0.0);
-    std::vector<int> mat(L.npad, 0);
+    size_t ncmax = 1;
+    for (int b = 0; b < L.nb; b++) ncmax = std::max(ncmax, L.hb[b].T.size());
+    DBuf<double> sT, sq, sm;
+    DBuf<int> mm, ferr;
+    if (sT.alloc(ncmax) || sq.alloc(ncmax) || sm.alloc(ncmax) || mm.alloc(2 * (size_t)L.nb) || ferr.alloc(1))
+      return FUSS_EHIP;
+    std::vector<int> hmm(2 * (size_t)L.nb);
     for (int b = 0; b < L.nb; b++) {
-      const BlockInfo& B = L.hblk[b];
+      hmm[2 * b] = 0x7fffffff;
+      hmm[2 * b + 1] = -0x7fffffff;
+    }
+    int e = upload(mm.p, hmm.data(), sizeof(int) * hmm.size());
+    for (int b = 0; b < L.nb && !e; b++) {
       const HostBlock& h = L.hb[b];
-      int umat = (int)h.matID[0] - 1;
-      for (int k = 1; k <= B.nk; k++)
-        for (int j = 1; j <= B.nj; j++)
-          for (int i = 1; i <= B.ni; i++) {
-            const size_t s = (i - 1) + (size_t)B.ni * ((j - 1) + (size_t)B.nj * (k - 1));
-            const long p = cidx(B, i, j, k);
-            T[p] = h.T[s];
-            q[p] = h.qvol[s];
-            const int m = (int)h.matID[s] - 1;
-            if (m < 0 || m >= c->nmat) return FUSS_EARG;
-            mat[p] = m;
-            if (m != umat) umat = -1;
-          }
+      const size_t nc = h.T.size();
+      // the staging buffers are reused block after block: each upload waits
+      // for the previous block's scatter
+      HIPCHK(hipStreamSynchronize(c->stream));
+      e |= upload(sT.p, h.T.data(), nc * 8);
+      e |= upload(sq.p, h.qvol.data(), nc * 8);
+      e |= upload(sm.p, h.matID.data(), nc * 8);
+      if (e) break;
+      k_scatter_fields<<<(unsigned)std::min<size_t>((nc + 255) / 256, 8192), 256, 0, c->stream>>>(
+          L.hblk[b], sT.p, sq.p, sm.p, L.Tb[L.cur].p, L.qvol.p, L.mat.p, c->nmat, mm.p + 2 * b, ferr.p);
+      HIPCHK(hipGetLastError());
+    }
+    if (e) return e;
+    int herr = 0;
+    HIPCHK(hipMemcpyAsync(hmm.data(), mm.p, sizeof(int) * hmm.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&herr, ferr.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (herr) return FUSS_EARG;
+    for (int b = 0; b < L.nb; b++) {
       // a block of one material keeps it on every level (restriction copies
       // a child's material, Lib_Multigrid.f90:113)
+      const int umat = hmm[2 * b] == hmm[2 * b + 1] ? hmm[2 * b] : -1;
       for (int lv = 0; lv < c->nlev; lv++) c->lev[lv].hblk[b].umat = umat;
     }
     for (int lv = 0; lv < c->nlev; lv++) {
@@ -1723,28 +1921,15 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     for (int lv = 0; lv < c->nlev; lv++)
       HIPCHK(hipMemcpy(c->lev[lv].blk.p, c->lev[lv].hblk.data(), sizeof(BlockInfo) * c->lev[lv].nb,
                        hipMemcpyHostToDevice));
-    int e = upload(L.Tb[L.cur].p, T.data(), T.size() * 8);
-    e |= upload(L.qvol.p, q.data(), q.size() * 8);
-    e |= upload(L.mat.p, mat.data(), mat.size() * sizeof(int));
-    if (e) return e;
   }
   // matIDg of connections: material of the donor cell (Fill_matIDg after the
-  // first Ghost_Connection copy, Lib_Ghost.f90:149-153, :436-470)
-  HIPCHK(hipStreamSynchronize(c->stream));
-  for (int m = 0; m < c->nlev; m++) {
-    Level& L = c->lev[m];
-    if (!L.nbound) continue;
-    std::vector<BCRec> recs(L.nbound);
-    HIPCHK(hipMemcpy(recs.data(), L.bc.p, sizeof(BCRec) * L.nbound, hipMemcpyDeviceToHost));
-    if (m == 0) {
-      for (auto& r : recs)
-        if (r.type == 101) {
-          const int* q = &L.hib[15 * (&r - &recs[0])];
-          const HostBlock& S = L.hb[q[6] - 1];
-          const size_t s = (q[7] - 1) + (size_t)S.n[0] * ((q[8] - 1) + (size_t)S.n[1] * (q[9] - 1));
-          r.matg = (int)S.matID[s] - 1;
-        }
-      HIPCHK(hipMemcpy(L.bc.p, recs.data(), sizeof(BCRec) * L.nbound, hipMemcpyHostToDevice));
+  // first Ghost_Connection copy, Lib_Ghost.f90:149-153, :436-470); chimera
+  // ghosts got theirs from the donor list
+  {
+    Level& L = c->lev[0];
+    if (L.nbound) {
+      k_set_matg<<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(L.bc.p, L.nbound, L.mat.p);
+      HIPCHK(hipGetLastError());
     }
   }
   int e = launch_ghost(c, c->lev[0], c->lev[0].Tb[c->lev[0].cur].p);
@@ -1754,35 +1939,24 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     if (e) return e;
     // coarse matIDg: donor material from the restricted material field
     Level& L = c->lev[m];
-    HIPCHK(hipStreamSynchronize(c->stream));
     if (L.nbound) {
-      std::vector<int> mat(L.npad);
-      HIPCHK(hipMemcpy(mat.data(), L.mat.p, sizeof(int) * L.npad, hipMemcpyDeviceToHost));
-      std::vector<BCRec> recs(L.nbound);
-      HIPCHK(hipMemcpy(recs.data(), L.bc.p, sizeof(BCRec) * L.nbound, hipMemcpyDeviceToHost));
-      for (auto& r : recs)
-        if (r.type == 101) r.matg = mat[r.gsrc];
-      HIPCHK(hipMemcpy(L.bc.p, recs.data(), sizeof(BCRec) * L.nbound, hipMemcpyHostToDevice));
+      k_set_matg<<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(L.bc.p, L.nbound, L.mat.p);
+      HIPCHK(hipGetLastError());
     }
     e = launch_ghost(c, L, L.Tb[L.cur].p);
     if (e) return e;
   }
   // blocks of uniform qvol (after restriction, on every level)
-  HIPCHK(hipStreamSynchronize(c->stream));
   for (int m = 0; m < c->nlev; m++) {
     Level& L = c->lev[m];
-    std::vector<double> q(L.npad);
-    HIPCHK(hipMemcpy(q.data(), L.qvol.p, sizeof(double) * L.npad, hipMemcpyDeviceToHost));
-    for (int b = 0; b < L.nb; b++) {
-      BlockInfo& B = L.hblk[b];
-      const double q0 = q[B.base];
-      int u = 1;
-      for (int k = 1; k <= B.nk && u; k++)
-        for (int j = 1; j <= B.nj && u; j++)
-          for (int i = 1; i <= B.ni; i++)
-            if (q[cidx(B, i, j, k)] != q0) { u = 0; break; }
-      B.uqv = u;
-    }
+    DBuf<int> nonu;
+    if (nonu.alloc(L.nb)) return FUSS_EHIP;
+    k_qvol_nonuniform<<<dim3(L.nb, 16), 64, 0, c->stream>>>(L.blk.p, L.qvol.p, nonu.p);
+    HIPCHK(hipGetLastError());
+    std::vector<int> hn(L.nb);
+    HIPCHK(hipMemcpyAsync(hn.data(), nonu.p, sizeof(int) * L.nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < L.nb; b++) L.hblk[b].uqv = !hn[b];
     HIPCHK(hipMemcpy(L.blk.p, L.hblk.data(), sizeof(BlockInfo) * L.nb, hipMemcpyHostToDevice));
   }
   // compact wall records, now that every level's materials are set

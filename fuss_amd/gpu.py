@@ -28,7 +28,9 @@ FUSS_IRS_JACOBI2, FUSS_IRS_TRIDIAG = 0, 1
 FUSS_ESTATE = 2
 FUSS_EARG = 1
 FUSS_ETABLE = 3
+FUSS_EMETRIC = 4
 FUSS_EUNSUPPORTED = 5
+FUSS_EBC = 7
 
 # fuss_stage_fn: (user, level, irk, block, ni, nj, nk, const double *T, double *R)
 STAGE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

@@ -518,6 +518,45 @@ def test_chimera_setup_errors(gpu):
         assert ("fuss_setup" in msg) if bad in ("missing", "donor") else ("fuss_set_chimera" in msg), msg
 
 
+def test_bc_list_errors(gpu):
+    """BC lists inconsistent with the blocks are rejected at fuss_setup with
+    FUSS_EBC (records built on the device for levels without chimera
+    entries): entries out of (block, face) order, an entry off its face, a
+    face cell named twice or not at all, a block or a donor block out of
+    range, a donor cell outside its block; the unmodified list sets up."""
+    def fresh():
+        return cases.lattice(nl=2, lo=4, hi=8, mgl=1)
+
+    gpu.Solver(fresh()).close()
+    c0 = fresh()
+    ib0 = c0.bcs[0].ib
+    conn = np.nonzero(ib0[:, 5] == 101)[0][0]
+    wall = np.nonzero((ib0[:, 5] != 101) & (ib0[:, 4] == 1))[0][0]
+    last = ib0.shape[0] - 1
+    edits = {
+        "order": lambda ib, rb: (ib[[last, *range(1, last), 0]], rb[[last, *range(1, last), 0]]),
+        "off_face": lambda ib, rb: (_set(ib, wall, 1, 2), rb),
+        "twice": lambda ib, rb: (_set(ib, wall + 1, slice(1, 4), ib[wall, 1:4]), rb),
+        "missing": lambda ib, rb: (np.delete(ib, wall, axis=0), np.delete(rb, wall, axis=0)),
+        "block": lambda ib, rb: (_set(ib, wall, 0, 99), rb),
+        "donor_block": lambda ib, rb: (_set(ib, conn, 6, 99), rb),
+        "donor_cell": lambda ib, rb: (_set(ib, conn, 7, 10**6), rb),
+    }
+    for name, edit in edits.items():
+        c = fresh()
+        ib, rb = edit(c.bcs[0].ib.copy(), c.bcs[0].rb.copy())
+        c.bcs[0] = cases.bclist(ib, rb)
+        c.series_id = [np.full(ib.shape[0], -1, np.int32)]
+        with pytest.raises(gpu.FussError) as ei:
+            gpu.Solver(c)
+        assert ei.value.code == gpu.FUSS_EBC, (name, ei.value.code)
+
+
+def _set(a, row, col, v):
+    a[row, col] = v
+    return a
+
+
 def test_xrow_takes_fused_irs(gpu):
     """Blocks joined on i faces only run the fused x-y-z IRS pass (one
     IRS launch per stage, the update in it), so xrow_irs exercises its
