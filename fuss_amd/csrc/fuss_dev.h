@@ -82,6 +82,7 @@ struct StageArgs {
   double vnn;
   R* dtw;
   double* pmin;        // per-tile dt minima
+  int conn_split;      // connection (101) entries are evaluated by k_connflux, not k_bcflux
 };
 
 // Compute_dt (Mod_dt.f90:26-84) for the cells of tile `wg` at iteration
@@ -204,6 +205,57 @@ __device__ __forceinline__ double bcf_rad(int type, double kappa, double mn, dou
   return Flux;
 }
 
+// BC_Connection (Lib_BC_Fluxes_Connection.f90:12-113) for one entry: the
+// interface temperature from both sides' conductances, then the flux of the
+// gradient through it, in the reference's order.  p: the boundary cell;
+// gsrc: its layer-1 ghost's donor; dn: the donor cells of Tg(3:6); M, Nr: the
+// cell's metric and the face normal; mc = Mg(1) . N; krow / kcon: the kappa
+// rows of the cell's and the ghost's materials.  Returns v with R = R - v.
+template <int NDIR, typename R>
+__device__ __forceinline__ R conn_value(const R* __restrict__ T, long p, long sy, long sz, int Fm, const double* M,
+                                        const double* Nr, const double* mc3, double A, long gsrc, const long* dn,
+                                        const R* krow, const R* kcon, int Ti1, int Ti2, int* bad) {
+#pragma clang fp contract(off)
+  const int modfm2 = 1 - 2 * (Fm % 2);
+  const R Tc = T[p];
+  const R Tip = T[p + 1], Tim = T[p - 1], Tjp = T[p + sy], Tjm = T[p - sy];
+  R Tkp = 0, Tkm = 0;
+  if (NDIR == 3) { Tkp = T[p + sz]; Tkm = T[p - sz]; }
+  const R Tcon = T[gsrc];
+  const R kl = tab_interp(krow, Tc, Ti1, Ti2, bad);
+  const R kc = tab_interp(kcon, Tcon, Ti1, Ti2, bad);
+  const R g3 = T[dn[0]], g4 = T[dn[1]], g5 = T[dn[2]], g6 = T[dn[3]];
+  const R ml0 = dotrow<R>(M, 1, Nr), ml1 = dotrow<R>(M, 2, Nr), ml2 = dotrow<R>(M, 3, Nr);
+  const R mc0 = (R)mc3[0], mc1 = (R)mc3[1], mc2 = (R)mc3[2];
+  R Dl, Nl, Dc, Nc;
+  if (Fm <= 2) {
+    Dl = (R)2.0 * ml0 * modfm2;
+    Nl = (R)2.0 * Tc * ml0 * modfm2 - (R)0.5 * (Tjp - Tjm) * ml1 - (R)0.5 * (Tkp - Tkm) * ml2;
+    Dc = (R)2.0 * mc0 * modfm2;
+    Nc = (R)2.0 * Tcon * mc0 * modfm2 + (R)0.5 * (g4 - g3) * mc1 + (R)0.5 * (g6 - g5) * mc2;
+  } else if (Fm <= 4) {
+    Dl = (R)2.0 * ml1 * modfm2;
+    Nl = -(R)0.5 * (Tip - Tim) * ml0 + (R)2.0 * Tc * ml1 * modfm2 - (R)0.5 * (Tkp - Tkm) * ml2;
+    Dc = (R)2.0 * mc1 * modfm2;
+    Nc = (R)0.5 * (g4 - g3) * mc0 + (R)2.0 * Tcon * mc1 * modfm2 + (R)0.5 * (g6 - g5) * mc2;
+  } else {
+    Dl = (R)2.0 * ml2 * modfm2;
+    Nl = -(R)0.5 * (Tip - Tim) * ml0 - (R)0.5 * (Tjp - Tjm) * ml1 + (R)2.0 * Tc * ml2 * modfm2;
+    Dc = (R)2.0 * mc2 * modfm2;
+    Nc = (R)0.5 * (g4 - g3) * mc0 + (R)0.5 * (g6 - g5) * mc1 + (R)2.0 * Tcon * mc2 * modfm2;
+  }
+  const R Tint = (kl * Nl + kc * Nc) / (kl * Dl + kc * Dc);
+  R G[3];
+  if (Fm <= 2) { G[0] = (R)2.0 * (Tint - Tc) * modfm2; G[1] = (Tjp - Tjm) * (R)0.5; G[2] = (Tkp - Tkm) * (R)0.5; }
+  else if (Fm <= 4) { G[0] = (Tip - Tim) * (R)0.5; G[1] = (R)2.0 * (Tint - Tc) * modfm2; G[2] = (Tkp - Tkm) * (R)0.5; }
+  else { G[0] = (Tip - Tim) * (R)0.5; G[1] = (Tjp - Tjm) * (R)0.5; G[2] = (R)2.0 * (Tint - Tc) * modfm2; }
+  if (NDIR == 2) G[2] = 0;
+  R gr[3];
+  for (int x = 0; x < 3; x++) gr[x] = G[0] * (R)M[3 * x] + G[1] * (R)M[3 * x + 1] + G[2] * (R)M[3 * x + 2];
+  const R Flux = (R)A * kl * (gr[0] * (R)Nr[0] + gr[1] * (R)Nr[1] + gr[2] * (R)Nr[2]);
+  return (R)modfm2 * Flux;
+}
+
 // BC_Fluxes for one entry (Mod_BC_Fluxes.f90:43-79 and the per-type files
 // src/lib/numerics/fluxes/bc/Lib_BC_Fluxes_*.f90), in the reference's
 // arithmetic order.
@@ -226,46 +278,12 @@ __device__ R bc_value(const StageArgs<R>& a, const BlockInfo& B, const BCRec& e,
   const R* krow = a.tab.kap + (long)a.mat[p] * a.tab.nT;
   switch (e.type) {
     case 101:
-    case 102: {
+    case 102:
       // BC_Connection, Lib_BC_Fluxes_Connection.f90:12-113 (chimera:
       // Mod_BC_Fluxes.f90:45, with its ghost as Tg(1) and the extrapolated Mg)
-      const R Tip = T[p + 1], Tim = T[p - 1], Tjp = T[p + sy], Tjm = T[p - sy];
-      R Tkp = 0, Tkm = 0;
-      if (NDIR == 3) { Tkp = T[p + sz]; Tkm = T[p - sz]; }
-      const R Tcon = T[e.gsrc];
-      const R kl = tab_interp(krow, Tc, a.tab.Ti1, a.tab.Ti2, &bad);
-      const R kc = tab_interp(a.tab.kap + (long)e.matg * a.tab.nT, Tcon, a.tab.Ti1, a.tab.Ti2, &bad);
-      const R g3 = T[e.dn[0]], g4 = T[e.dn[1]], g5 = T[e.dn[2]], g6 = T[e.dn[3]];
-      const R ml0 = dotrow<R>(M, 1, Nr), ml1 = dotrow<R>(M, 2, Nr), ml2 = dotrow<R>(M, 3, Nr);
-      const R mc0 = (R)e.mc[0], mc1 = (R)e.mc[1], mc2 = (R)e.mc[2];
-      R Dl, Nl, Dc, Nc;
-      if (Fm <= 2) {
-        Dl = (R)2.0 * ml0 * modfm2;
-        Nl = (R)2.0 * Tc * ml0 * modfm2 - (R)0.5 * (Tjp - Tjm) * ml1 - (R)0.5 * (Tkp - Tkm) * ml2;
-        Dc = (R)2.0 * mc0 * modfm2;
-        Nc = (R)2.0 * Tcon * mc0 * modfm2 + (R)0.5 * (g4 - g3) * mc1 + (R)0.5 * (g6 - g5) * mc2;
-      } else if (Fm <= 4) {
-        Dl = (R)2.0 * ml1 * modfm2;
-        Nl = -(R)0.5 * (Tip - Tim) * ml0 + (R)2.0 * Tc * ml1 * modfm2 - (R)0.5 * (Tkp - Tkm) * ml2;
-        Dc = (R)2.0 * mc1 * modfm2;
-        Nc = (R)0.5 * (g4 - g3) * mc0 + (R)2.0 * Tcon * mc1 * modfm2 + (R)0.5 * (g6 - g5) * mc2;
-      } else {
-        Dl = (R)2.0 * ml2 * modfm2;
-        Nl = -(R)0.5 * (Tip - Tim) * ml0 - (R)0.5 * (Tjp - Tjm) * ml1 + (R)2.0 * Tc * ml2 * modfm2;
-        Dc = (R)2.0 * mc2 * modfm2;
-        Nc = (R)0.5 * (g4 - g3) * mc0 + (R)0.5 * (g6 - g5) * mc1 + (R)2.0 * Tcon * mc2 * modfm2;
-      }
-      const R Tint = (kl * Nl + kc * Nc) / (kl * Dl + kc * Dc);
-      R G[3];
-      if (Fm <= 2) { G[0] = (R)2.0 * (Tint - Tc) * modfm2; G[1] = (Tjp - Tjm) * (R)0.5; G[2] = (Tkp - Tkm) * (R)0.5; }
-      else if (Fm <= 4) { G[0] = (Tip - Tim) * (R)0.5; G[1] = (R)2.0 * (Tint - Tc) * modfm2; G[2] = (Tkp - Tkm) * (R)0.5; }
-      else { G[0] = (Tip - Tim) * (R)0.5; G[1] = (Tjp - Tjm) * (R)0.5; G[2] = (R)2.0 * (Tint - Tc) * modfm2; }
-      if (NDIR == 2) G[2] = 0;
-      R gr[3];
-      for (int x = 0; x < 3; x++) gr[x] = G[0] * (R)M[3 * x] + G[1] * (R)M[3 * x + 1] + G[2] * (R)M[3 * x + 2];
-      const R Flux = (R)e.A * kl * (gr[0] * (R)Nr[0] + gr[1] * (R)Nr[1] + gr[2] * (R)Nr[2]);
-      Rv = (R)modfm2 * Flux;
-    } break;
+      Rv = conn_value<NDIR, R>(T, p, sy, sz, Fm, M, Nr, e.mc, e.A, e.gsrc, e.dn, krow,
+                               a.tab.kap + (long)e.matg * a.tab.nT, a.tab.Ti1, a.tab.Ti2, &bad);
+      break;
     case 300: {
       // BC_Symmetry, Lib_BC_Fluxes_Symmetry.f90:12-67
       R G[3] = {0, 0, 0};
@@ -610,6 +628,14 @@ constexpr int XY_RJ = XY_TJ + 4, XY_RI = TILE_X + 4;
 // inside the table and the conversion never sees a NaN or an out-of-range
 // value (fmax of a NaN returns the bound).  With Ti1 >= 0, int(T) >= Ti2
 // exactly when T >= Ti2.
+// tab_row's row as a floating value (the integer part of the clamped T):
+// T - row is the interpolation's fraction without a conversion back from the
+// integer, and (int) of it addresses the row
+template <typename R>
+__device__ __forceinline__ R tab_rowf(R T, int Ti1, int Ti2, int* bad) {
+  if (!(T >= (R)Ti1 && T < (R)Ti2)) *bad = 1;
+  return trunc(fmin(fmax(T, (R)Ti1), (R)(Ti2 - 1)));
+}
 template <typename R>
 __device__ __forceinline__ int tab_row(R T, int Ti1, int Ti2, int* bad) {
   if (!(T >= (R)Ti1 && T < (R)Ti2)) *bad = 1;

@@ -403,10 +403,61 @@ __global__ void k_bcflux(StageArgs<R> a, const BCHot* __restrict__ hot, int n) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n || (a.halt && *a.halt)) return;
   const int type = hot[e].type;
-  if (!(type == 101 || type == 102 || (type >= 300 && type <= 305))) return;
+  if (!(type == 101 || type == 102 || (type >= 300 && type <= 305)) || (type == 101 && a.conn_split)) return;
   const R v = bc_entry_value<NDIR>(a, hot, e, *a.clk);
   a.bcv[e] = v;
   a.bcf[hot[e].fpos] = v;
+}
+
+// Connection entries (101) of the compact records: doubles M(9), N(3),
+// Mg(1).N (3), A; ints cell, ghost donor, Tg(3:6) donors (4), face | matIDg
+// << 3, sy, sz, entry, face-map position, block, i | j << 16, k.
+constexpr int CONN_ND = 16, CONN_NI = 15;
+
+// BC_Connection of every 101 entry (the surface pass's connection part, one
+// thread per entry, records read field by field: coalesced): the same
+// arithmetic as bc_value's, its value to bcv and bcf.
+template <int NDIR, typename R>
+__global__ void k_connflux(StageArgs<R> a, const double* __restrict__ cd, const int* __restrict__ ci, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n || (a.halt && *a.halt)) return;
+  double M[9], Nr[3], mc[3];
+  for (int f = 0; f < 9; f++) M[f] = cd[(long)f * n + q];
+  for (int f = 0; f < 3; f++) Nr[f] = cd[(long)(9 + f) * n + q];
+  for (int f = 0; f < 3; f++) mc[f] = cd[(long)(12 + f) * n + q];
+  const double A = cd[15L * n + q];
+  const long p = ci[q], gsrc = ci[n + q];
+  const long dn[4] = {ci[2L * n + q], ci[3L * n + q], ci[4L * n + q], ci[5L * n + q]};
+  const int fm = ci[6L * n + q];
+  const long sy = ci[7L * n + q], sz = ci[8L * n + q];
+  const int e = ci[9L * n + q], fpos = ci[10L * n + q];
+  int bad = 0;
+  const R v = conn_value<NDIR, R>(a.T, p, sy, sz, fm & 7, M, Nr, mc, A, gsrc, dn,
+                                  a.tab.kap + (long)a.mat[p] * a.tab.nT, a.tab.kap + (long)(fm >> 3) * a.tab.nT,
+                                  a.tab.Ti1, a.tab.Ti2, &bad);
+  if (bad) {
+    const int ij = ci[12L * n + q];
+    flag_error(a.status, FUSS_ETABLE, ci[11L * n + q], ij & 0xffff, ij >> 16, ci[13L * n + q]);
+  }
+  a.bcv[e] = v;
+  a.bcf[fpos] = v;
+}
+
+// the compact records of the connection entries listed in ent
+__global__ void k_conn_build(const BCRec* __restrict__ bc, const int* __restrict__ ent, int n,
+                             const BlockInfo* __restrict__ blk, double* cd, int* ci) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int e = ent[q];
+  const BCRec& r = bc[e];
+  const BlockInfo& B = blk[r.b];
+  for (int f = 0; f < 9; f++) cd[(long)f * n + q] = r.M[f];
+  for (int f = 0; f < 3; f++) cd[(long)(9 + f) * n + q] = r.N[f];
+  for (int f = 0; f < 3; f++) cd[(long)(12 + f) * n + q] = r.mc[f];
+  cd[15L * n + q] = r.A;
+  const int iv[CONN_NI] = {(int)r.cell, (int)r.gsrc, (int)r.dn[0], (int)r.dn[1], (int)r.dn[2], (int)r.dn[3],
+                           r.f | (r.matg << 3), B.sy, (int)B.sz, e, r.fpos, r.b, r.i | (r.j << 16), r.k, 0};
+  for (int f = 0; f < CONN_NI; f++) ci[(long)f * n + q] = iv[f];
 }
 
 // Extract_Properties_From_BC (src/lib/io/IO_Wall.f90:131-239): wall
@@ -984,7 +1035,8 @@ std::vector<const void*> run_buffers(const fuss_ctx* c, const Level& L) {
       L.dt.p, L.qvol.p, L.vol.p, L.invL2.p, L.mat.p, L.bc.p, L.bch.p, L.gmap.p, L.fmap.p, L.bcv.p, L.bcf.p,
       L.rfac.p, L.ttx.p, L.tty.p, L.ttz.p, L.tri.p, L.trioff.p, L.chr.p, L.chd.p, L.chv.p, L.clk.p, L.cnt.p,
       L.partials.p, L.pmin.p, L.red2.p, c->dcp.p, c->drho.p, c->dkap.p, c->dh.p, c->dkap2.p, c->dpr4.p,
-      c->status.p, c->scal.p, c->run_hist.p, c->ser_t.p, c->ser_v.p, c->ser_off.p, c->ser_n.p, c->ser_per.p};
+      c->status.p, c->scal.p, c->run_hist.p, c->ser_t.p, c->ser_v.p, c->ser_off.p, c->ser_n.p, c->ser_per.p,
+      L.cd.p, L.ci.p};
   for (const auto& w : L.W) v.push_back(w.p);
   for (const auto& r : L.rng) v.push_back(r.p);
   const auto& F = L.f;
@@ -1965,6 +2017,21 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     if (!L.nbound) continue;
     k_bc_hot<<<(L.nbound + 255) / 256, 256, 0, c->stream>>>(L.bc.p, L.nbound, L.mat.p, L.bch.p);
     HIPCHK(hipGetLastError());
+    // compact records of the connection entries (their geometry and ghost
+    // materials are final now)
+    std::vector<int> ent;
+    for (int e = 0; e < L.nbound; e++)
+      if (L.hib[15 * (size_t)e + 5] == 101) ent.push_back(e);
+    L.nconn = (int)ent.size();
+    if (L.nconn) {
+      DBuf<int> dent;
+      if (dent.alloc(ent.size()) || L.cd.alloc((size_t)CONN_ND * L.nconn) || L.ci.alloc((size_t)CONN_NI * L.nconn))
+        return FUSS_EHIP;
+      if (upload(dent.p, ent.data(), sizeof(int) * ent.size())) return FUSS_EHIP;
+      k_conn_build<<<(L.nconn + 255) / 256, 256, 0, c->stream>>>(L.bc.p, dent.p, L.nconn, L.blk.p, L.cd.p, L.ci.p);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
   }
   int bijk[4];
   const int st = sync_status(c, bijk);
@@ -2188,8 +2255,14 @@ int enqueue_step_t(fuss_ctx* c, int level, const fuss_step_params* p, const Stag
     }
     if (L.any_active && L.nbound && !fused) {
       pid = prof_begin(c, FUSS_K_BC);
+      sa.conn_split = L.nconn > 0;
       if (L.kp == GC) k_bcflux<3, R><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
       else k_bcflux<2, R><<<(L.nbound + 127) / 128, 128, 0, c->stream>>>(sa, L.bch.p, L.nbound);
+      if (L.nconn) {
+        if (L.kp == GC) k_connflux<3, R><<<(L.nconn + 127) / 128, 128, 0, c->stream>>>(sa, L.cd.p, L.ci.p, L.nconn);
+        else k_connflux<2, R><<<(L.nconn + 127) / 128, 128, 0, c->stream>>>(sa, L.cd.p, L.ci.p, L.nconn);
+      }
+      sa.conn_split = 0;
       prof_end(c, pid);
       HIPCHK(hipGetLastError());
     }

@@ -95,6 +95,12 @@ struct Level {
   DBuf<double> M, F[3], W[9];
   DBuf<BCRec> bc;
   DBuf<BCHot> bch;                  // compact wall records (k_bcflux)
+  // connection (101) entries as structure-of-arrays records for k_connflux:
+  // CONN_ND doubles and CONN_NI ints per entry, field-major (field f of
+  // entry q at f * nconn + q)
+  DBuf<double> cd;
+  DBuf<int> ci;
+  int nconn = 0;
   DBuf<int2> gmap;                  // (ghost, source) per entry (k_ghost)
   DBuf<int> fmap;
   DBuf<double> bcv, bcf;

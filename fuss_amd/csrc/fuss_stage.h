@@ -392,7 +392,9 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const Tile t = a.ctiles[wg];
   const BlockInfo B = a.blk[t.b];
-  const int tx = threadIdx.x % TILE_X, ty = threadIdx.x / TILE_X;
+  static_assert(TILE_X == 64, "one wave per tile row");
+  // the row of the lane's wave: uniform, so tests on it are scalar branches
+  const int tx = threadIdx.x % TILE_X, ty = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE_X);
   const int i = t.i0 + tx, j = t.j0 + ty;
   const int k0 = t.k0, k1 = min(B.nk, k0 + KCT - 1);
   const bool live = i <= B.ni && j <= B.nj;
@@ -517,8 +519,9 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
     int bad = 0, badc = 0;
     const R Tc = ring[sc][y][x];
     const R Txr = ring[sc][y][x + 1], Tyr = ring[sc][y + 1][x], Tzr = ring[sp][y][x];
-    R Tx, Ty, Tz, Th = 0, Tcl = Tc;
-    int rx, ry, rz, rh = Ti1, rc = Ti1;
+    // face averages, then the fractions of their table rows (T - row, the
+    // row taken as a floating value: no conversion back from the integer)
+    R Tx, Ty, Tz, Th = 0, Tcl = Tc, fc = 0;
     V2 kx, ky, kz, kh{}, kc{};
     V4 pc{};
     // BC values of the lane's boundary faces, issued with the table rows;
@@ -526,31 +529,37 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
     R bx = 0, by = 0, bz = 0;
     if (xtile) bx = ldu(a.bcf, (unsigned)max(xo >= 0 ? xo + B.nj * (k - 1) : 0, 0));
     if (yrow) by = ldu(a.bcf, (unsigned)max(yo >= 0 ? yo + B.ni * (k - 1) : 0, 0));
-    if (zo_of(k) >= 0 || ((fa_t & 48) && (k == 1 || k == B.nk))) bz = ldu(a.bcf, (unsigned)max(zo_of(k), 0));
+    // (a lane's z face has an entry only on an end plane of a block with z entries)
+    if ((fa_t & 48) && (k == 1 || k == B.nk)) bz = ldu(a.bcf, (unsigned)max(zo_of(k), 0));
     {
 #pragma clang fp contract(off)
       // rows addressed by 32-bit offsets from the material's first row
       const V2* kr = UM ? kap_u : kap0 + (long)cur.m * nT;
       int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
       Tx = (R)0.5 * (Tc + Txr);
-      rx = tab_row(Tx, Ti1, Ti2, &b0);
-      kx = ldu(kr, (unsigned)(rx - Ti1));
+      const R rxf = tab_rowf(Tx, Ti1, Ti2, &b0);
+      kx = ldu(kr, (unsigned)((int)rxf - Ti1));
+      Tx = Tx - rxf;
       Ty = (R)0.5 * (Tc + Tyr);
-      ry = tab_row(Ty, Ti1, Ti2, &b1);
-      ky = ldu(kr, (unsigned)(ry - Ti1));
+      const R ryf = tab_rowf(Ty, Ti1, Ti2, &b1);
+      ky = ldu(kr, (unsigned)((int)ryf - Ti1));
+      Ty = Ty - ryf;
       Tz = (R)0.5 * (Tc + Tzr);
-      rz = tab_row(Tz, Ti1, Ti2, &b2);
-      kz = ldu(kr, (unsigned)(rz - Ti1));
+      const R rzf = tab_rowf(Tz, Ti1, Ti2, &b2);
+      kz = ldu(kr, (unsigned)((int)rzf - Ti1));
+      Tz = Tz - rzf;
       if (PRIM) {
-        rc = tab_row(Tcl, Ti1, Ti2, &badc);
-        pc = ldu(UM ? pr_u : pr0 + (long)cur.m * nT, (unsigned)(rc - Ti1));
-        if (DTC) kc = ldu(kr, (unsigned)(rc - Ti1));  // kappa at the cell for its dt
+        const R rcf = tab_rowf(Tcl, Ti1, Ti2, &badc);
+        pc = ldu(UM ? pr_u : pr0 + (long)cur.m * nT, (unsigned)((int)rcf - Ti1));
+        if (DTC) kc = ldu(kr, (unsigned)((int)rcf - Ti1));  // kappa at the cell for its dt
+        fc = Tcl - rcf;
       }
       if (ty < 2) {  // waves 0 and 1 hold the halo lanes (a wave-uniform test)
         const V2* krh = UM ? kap_u : kap0 + (long)cur.mh * nT;
         Th = yhalo ? (R)0.5 * (ring[sc][0][x] + ring[sc][1][x]) : (R)0.5 * (ring[sc][hq][0] + ring[sc][hq][1]);
-        rh = tab_row(Th, Ti1, Ti2, &b3);
-        kh = ldu(krh, (unsigned)(rh - Ti1));
+        const R rhf = tab_rowf(Th, Ti1, Ti2, &b3);
+        kh = ldu(krh, (unsigned)((int)rhf - Ti1));
+        Th = Th - rhf;
       }
       // primitive variables: every cell's own row is checked (badc), and a face
       // average lies between its two cells' temperatures, so a face outside
@@ -583,12 +592,12 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
     {
 #pragma clang fp contract(off)
       if (live) {
-        if (hx) fx[ty][tx + 1] = (kx.x + (Tx - (R)rx) * kx.y) * ((Txr - Tc) * cur.w0);
-        if (hy) fy[ty + 1][tx] = (ky.x + (Ty - (R)ry) * ky.y) * ((Tyr - Tc) * cur.w4);
-        if (k < B.nk) fz = (kz.x + (Tz - (R)rz) * kz.y) * ((Tzr - Tc) * cur.w8);
+        if (hx) fx[ty][tx + 1] = (kx.x + Tx * kx.y) * ((Txr - Tc) * cur.w0);
+        if (hy) fy[ty + 1][tx] = (ky.x + Ty * ky.y) * ((Tyr - Tc) * cur.w4);
+        if (k < B.nk) fz = (kz.x + Tz * kz.y) * ((Tzr - Tc) * cur.w8);
       }
-      if (yhalo) fy[0][tx] = (kh.x + (Th - (R)rh) * kh.y) * ((ring[sc][1][x] - ring[sc][0][x]) * cur.wh);
-      if (xhalo) fx[hr][0] = (kh.x + (Th - (R)rh) * kh.y) * ((ring[sc][hq][1] - ring[sc][hq][0]) * cur.wh);
+      if (yhalo) fy[0][tx] = (kh.x + Th * kh.y) * ((ring[sc][1][x] - ring[sc][0][x]) * cur.wh);
+      if (xhalo) fx[hr][0] = (kh.x + Th * kh.y) * ((ring[sc][hq][1] - ring[sc][hq][0]) * cur.wh);
     }
     if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
     __syncthreads();
@@ -616,7 +625,7 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
         // kappa, rho, cp interpolations of k_dt at the cell's row (rho and cp
         // are the residual's below), sum 1/dl^2 from the per-axis factors
 #pragma clang fp contract(off)
-        const R fr = Tcl - (R)rc;
+        const R fr = fc;
         const R kap = kc.x + fr * kc.y, rho = pc.x + fr * pc.y, cp = pc.z + fr * pc.w;
         const double alpha = (double)kap / (double)rho / (double)cp;
         double dd = 0.5 * a.vnn / (alpha * (qxy + zfac[2][k - k0 + 1]));
@@ -630,8 +639,8 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
 #pragma clang fp contract(off)
         if (PRIM) {
           // Compute_Prim_Residual, Lib_Newstate.f90:241-256 (as scaled_residual)
-          const R rho = pc.x + (Tcl - (R)rc) * pc.y;
-          const R cp = pc.z + (Tcl - (R)rc) * pc.w;
+          const R rho = pc.x + fc * pc.y;
+          const R cp = pc.z + fc * pc.w;
           res = (-Rv / cur.vol + cur.qv) * dt / (Tc * cp * pc.y + rho * cp + Tc * rho * pc.w);
         } else {
           res = (-Rv / cur.vol + cur.qv) * dt;
