@@ -683,6 +683,279 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
 }
 
 // ------------------------------------------------------------------------
+// Software-pipelined stage kernel for 3-D blocks with tangential weights
+// (curvilinear grids): k_stage3d's arithmetic on k_stage3p's schedule.  Per
+// plane the table rows of the plane's faces and cell are requested first,
+// then T of plane k+2 and the per-cell streams of plane k+1 (1/V, qvol, dt,
+// T^n and the nine face weights, plus the three weights of the lane's halo
+// face), so the streams stay in flight across a whole plane; every load in
+// the loop is unconditional.  The tangential gradients read the planes k-1,
+// k, k+1 of the LDS ring before the plane's second barrier, so the slot of
+// k-1 is rewritten (with plane k+2) only after every wave is done with it.
+// ------------------------------------------------------------------------
+template <typename R>
+struct StageStreamT {
+  R vol, qv, dt, TO;
+  int m, mh;  // material of the cell / of the halo face's left cell
+};
+
+template <bool UM, bool PRIM, bool IRSMODE, typename R, int KCT>
+// (three waves per SIMD: the FP64 instantiations then fit 168 VGPRs with one
+// value spilled to scratch per two planes; the compiler's own choice, 175
+// VGPRs and two waves, runs the lattice's stage 1.3 % slower)
+__global__ __launch_bounds__(TILE_T) __attribute__((amdgpu_waves_per_eu(3))) void k_stage3t(StageArgs<R> a) {
+  constexpr int TY = TILE_Y, TT = TILE_T;
+  static_assert(TILE_X == 64, "one wave per tile row");
+  if (a.halt && *a.halt) return;
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const Tile t = a.ctiles[wg];
+  const BlockInfo B = a.blk[t.b];
+  const int tx = threadIdx.x % TILE_X, ty = __builtin_amdgcn_readfirstlane(threadIdx.x / TILE_X);
+  const int i = t.i0 + tx, j = t.j0 + ty;
+  const int k0 = t.k0, k1 = min(B.nk, k0 + KCT - 1);
+  const bool live = i <= B.ni && j <= B.nj;
+  const int x = tx + 1, y = ty + 1;
+  const bool yhalo = ty == 0 && t.j0 > 1 && i <= B.ni;
+  const int hr = tx;
+  const bool xhalo = ty == 1 && tx < TY && t.i0 > 1 && t.j0 + hr <= B.nj;
+  const bool hx = i < B.ni, hy = j < B.nj;
+  const int hq = xhalo ? hr + 1 : 1;  // LDS row of the x-halo face (in range for every lane)
+  __shared__ R ring[3][TY + 2][PX];
+  __shared__ R fx[TY][TILE_X + 1];
+  __shared__ R fy[TY + 1][TILE_X];
+  using V2 = typename Vec2<R>::T;
+  using V4 = typename Vec4<R>::T;
+  const int Ti1 = a.tab.Ti1, Ti2 = a.tab.Ti2, nT = a.tab.nT;
+  const int mu = B.umat >= 0 ? B.umat : 0;
+  const V2* kap_u = reinterpret_cast<const V2*>(a.tab.kap2) + (long)mu * nT;
+  const V4* pr_u = reinterpret_cast<const V4*>(a.tab.pr4) + (long)mu * nT;
+  const V2* kap0 = reinterpret_cast<const V2*>(a.tab.kap2);
+  const V4* pr0 = reinterpret_cast<const V4*>(a.tab.pr4);
+  // the halo face's weights: +y face (W3, W4, W5) for the y-halo row, +x
+  // (W0, W1, W2) for the x-halo lanes
+  const int wbase = yhalo ? 3 : 0;
+  int sm = 0, sc = 1, sp = 2;
+  PlaneRegs<R> pre;
+  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0 - 1);
+  store_plane<TY>(ring[sm], pre);
+  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0);
+  store_plane<TY>(ring[sc], pre);
+  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0 + 1);
+  const unsigned sz = (unsigned)B.sz;
+  const int ci = live ? i : 1, cj = live ? j : 1;
+  unsigned p = (unsigned)(B.base + (ci - 1) + (long)(cj - 1) * B.sy + (long)(k0 - 1) * B.sz);
+  unsigned ph = yhalo ? p - (unsigned)B.sy
+                      : (xhalo ? (unsigned)(B.base + (t.i0 - 2) + (long)(t.j0 + hr - 1) * B.sy + (long)(k0 - 1) * B.sz)
+                               : p);
+  const int fa = live ? B.face_active : 0;
+  const int xo = (fa & 1) && i == 1 ? B.fmap_off[0] + (j - 1) : ((fa & 2) && i == B.ni ? B.fmap_off[1] + (j - 1) : -1);
+  const int yo = (fa & 4) && j == 1 ? B.fmap_off[2] + (i - 1) : ((fa & 8) && j == B.nj ? B.fmap_off[3] + (i - 1) : -1);
+  const int zlo = (fa & 16) ? B.fmap_off[4] + (i - 1) + B.ni * (j - 1) : -1;
+  const int zhi = (fa & 32) ? B.fmap_off[5] + (i - 1) + B.ni * (j - 1) : -1;
+  auto zo_of = [&](int kk) { return kk == 1 ? zlo : (kk == B.nk ? zhi : -1); };
+  const int fa_t = B.face_active;
+  const bool xtile = (fa_t & 3) && (t.i0 == 1 || t.i0 + TILE_X - 1 >= B.ni);
+  const bool yrow = (fa_t & 12) && (j == 1 || j == B.nj);
+  const R dtg = a.use_dtg ? (R)(*a.dtg) : (R)0;
+  const unsigned quni = (unsigned)B.base;
+  // with a global dt every lane reads the dt of cell (1,1,1): no stream
+  const bool dtu = a.use_dtg != 0;
+  // the face weights (nine per cell, three for the halo face) are held for
+  // one plane only: a plane's are requested after the previous plane's
+  // fluxes, so they arrive while that plane finishes (registers for a second
+  // set would halve the waves per SIMD)
+  R w[9], wh[3];
+  auto weights = [&](unsigned pp, unsigned hh) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) w[q] = ldn(a.W[q], pp);
+#pragma unroll
+    for (int q = 0; q < 3; q++) wh[q] = ldu(a.W[wbase + q], hh);
+  };
+  auto stream = [&](StageStreamT<R>& s, unsigned pp, unsigned hh) {
+    s.qv = ldu(a.qvol, B.uqv ? quni : pp);
+    s.dt = ldn(a.dt, dtu ? quni : pp);
+    if (!IRSMODE) s.TO = ldn(a.TO, pp);
+    s.vol = ldn(a.vol, pp);
+    if (!UM) {
+      s.m = ldu(a.mat, pp);
+      s.mh = ldu(a.mat, hh);
+    }
+  };
+  StageStreamT<R> sa_, sb_;
+  sa_.m = sa_.mh = sb_.m = sb_.mh = mu;
+  sa_.TO = sb_.TO = (R)0;
+  stream(sa_, p, ph);
+  weights(p, ph);
+  __syncthreads();
+  R fz_prev = 0;
+  if (live && k0 > 1) {
+#pragma clang fp contract(off)
+    fz_prev = flux_z<true>(a, B, ring[sm], ring[sc], x, y, p - sz, (R)0);
+  }
+  double rsq = 0.0;
+  R* const out = IRSMODE ? a.Rres : a.Tout;
+  R vpend = 0;
+  unsigned ppend = p;
+  bool spend = false;
+  auto plane = [&](StageStreamT<R>& cur, StageStreamT<R>& nxt, int k) {
+    store_plane<TY>(ring[sp], pre);
+    __syncthreads();
+    // phase A: table rows of the plane's faces and of the cell
+    int bad = 0, badc = 0;
+    const R Tc = ring[sc][y][x];
+    const R Txr = ring[sc][y][x + 1], Tyr = ring[sc][y + 1][x], Tzr = ring[sp][y][x];
+    R Tx, Ty, Tz, Th = 0, Tcl = Tc, fc = 0;
+    V2 kx, ky, kz, kh{};
+    V4 pc{};
+    R bx = 0, by = 0, bz = 0;
+    if (xtile) bx = ldu(a.bcf, (unsigned)max(xo >= 0 ? xo + B.nj * (k - 1) : 0, 0));
+    if (yrow) by = ldu(a.bcf, (unsigned)max(yo >= 0 ? yo + B.ni * (k - 1) : 0, 0));
+    if ((fa_t & 48) && (k == 1 || k == B.nk)) bz = ldu(a.bcf, (unsigned)max(zo_of(k), 0));
+    {
+#pragma clang fp contract(off)
+      const V2* kr = UM ? kap_u : kap0 + (long)cur.m * nT;
+      int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+      Tx = (R)0.5 * (Tc + Txr);
+      const R rxf = tab_rowf(Tx, Ti1, Ti2, &b0);
+      kx = ldu(kr, (unsigned)((int)rxf - Ti1));
+      Tx = Tx - rxf;
+      Ty = (R)0.5 * (Tc + Tyr);
+      const R ryf = tab_rowf(Ty, Ti1, Ti2, &b1);
+      ky = ldu(kr, (unsigned)((int)ryf - Ti1));
+      Ty = Ty - ryf;
+      Tz = (R)0.5 * (Tc + Tzr);
+      const R rzf = tab_rowf(Tz, Ti1, Ti2, &b2);
+      kz = ldu(kr, (unsigned)((int)rzf - Ti1));
+      Tz = Tz - rzf;
+      if (PRIM) {
+        const R rcf = tab_rowf(Tcl, Ti1, Ti2, &badc);
+        pc = ldu(UM ? pr_u : pr0 + (long)cur.m * nT, (unsigned)((int)rcf - Ti1));
+        fc = Tcl - rcf;
+      }
+      if (ty < 2) {  // waves 0 and 1 hold the halo lanes
+        const V2* krh = UM ? kap_u : kap0 + (long)cur.mh * nT;
+        Th = yhalo ? (R)0.5 * (ring[sc][0][x] + ring[sc][1][x]) : (R)0.5 * (ring[sc][hq][0] + ring[sc][hq][1]);
+        const R rhf = tab_rowf(Th, Ti1, Ti2, &b3);
+        kh = ldu(krh, (unsigned)((int)rhf - Ti1));
+        Th = Th - rhf;
+      }
+      // every face inside the block is checked, like k_stage3d's flux evaluations
+      bad = (live && ((hx && b0) || (hy && b1) || (k < B.nk && b2))) || ((yhalo || xhalo) && b3);
+      badc = live && badc;
+    }
+    // phase B: the next plane's loads (the last plane re-reads its own cells)
+    {
+      const unsigned step = k < k1 ? sz : 0u;
+      fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k < k1 ? k + 2 : k + 1);
+      stream(nxt, p + step, ph + step);
+    }
+    if (spend) stn(out, ppend, vpend);
+    // phase C: fluxes (Diffusive_Flux regrouped per direction, as flux_x/y/z)
+    R (*m_)[PX] = ring[sm];
+    R (*c_)[PX] = ring[sc];
+    R (*p_)[PX] = ring[sp];
+    R fz = 0;
+    {
+#pragma clang fp contract(off)
+      if (live) {
+        if (hx) {
+          const R G0 = Txr - Tc;
+          const R G1 = (c_[y + 1][x] - c_[y - 1][x] + c_[y + 1][x + 1] - c_[y - 1][x + 1]) * (R)0.25;
+          const R G2 = (p_[y][x] - m_[y][x] + p_[y][x + 1] - m_[y][x + 1]) * (R)0.25;
+          fx[ty][tx + 1] = (kx.x + Tx * kx.y) * ((G0 * w[0] + G1 * w[1]) + G2 * w[2]);
+        }
+        if (hy) {
+          const R G1 = Tyr - Tc;
+          const R G0 = (c_[y][x + 1] - c_[y][x - 1] + c_[y + 1][x + 1] - c_[y + 1][x - 1]) * (R)0.25;
+          const R G2 = (p_[y][x] - m_[y][x] + p_[y + 1][x] - m_[y + 1][x]) * (R)0.25;
+          fy[ty + 1][tx] = (ky.x + Ty * ky.y) * ((G0 * w[3] + G1 * w[4]) + G2 * w[5]);
+        }
+        if (k < B.nk) {
+          const R G2 = Tzr - Tc;
+          const R G0 = (c_[y][x + 1] - c_[y][x - 1] + p_[y][x + 1] - p_[y][x - 1]) * (R)0.25;
+          const R G1 = (c_[y + 1][x] - c_[y - 1][x] + p_[y + 1][x] - p_[y - 1][x]) * (R)0.25;
+          fz = (kz.x + Tz * kz.y) * ((G0 * w[6] + G1 * w[7]) + G2 * w[8]);
+        }
+      }
+      if (yhalo) {  // +y face of (i, j0-1): LDS row 0
+        const R G1 = c_[1][x] - c_[0][x];
+        const R G0 = (c_[0][x + 1] - c_[0][x - 1] + c_[1][x + 1] - c_[1][x - 1]) * (R)0.25;
+        const R G2 = (p_[0][x] - m_[0][x] + p_[1][x] - m_[1][x]) * (R)0.25;
+        fy[0][tx] = (kh.x + Th * kh.y) * ((G0 * wh[0] + G1 * wh[1]) + G2 * wh[2]);
+      }
+      if (xhalo) {  // +x face of (i0-1, j0+hr): LDS column 0 of row hq
+        const R G0 = c_[hq][1] - c_[hq][0];
+        const R G1 = (c_[hq + 1][0] - c_[hq - 1][0] + c_[hq + 1][1] - c_[hq - 1][1]) * (R)0.25;
+        const R G2 = (p_[hq][0] - m_[hq][0] + p_[hq][1] - m_[hq][1]) * (R)0.25;
+        fx[hr][0] = (kh.x + Th * kh.y) * ((G0 * wh[0] + G1 * wh[1]) + G2 * wh[2]);
+      }
+    }
+    if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
+    {
+      const unsigned step = k < k1 ? sz : 0u;
+      weights(p + step, ph + step);
+    }
+    __syncthreads();
+    if (live) {
+      R Rv = 0;
+      {
+#pragma clang fp contract(off)
+        if (i > 1) Rv = Rv + fx[ty][tx];
+        if (hx) Rv = Rv - fx[ty][tx + 1];
+        if (j > 1) Rv = Rv + fy[ty][tx];
+        if (hy) Rv = Rv - fy[ty + 1][tx];
+        if (k > 1) Rv = Rv + fz_prev;
+        if (k < B.nk) Rv = Rv - fz;
+      }
+      {
+#pragma clang fp contract(off)
+        if (xo >= 0) Rv = Rv - bx;
+        if (yo >= 0) Rv = Rv - by;
+        if (zo_of(k) >= 0) Rv = Rv - bz;
+      }
+      const R dt = a.use_dtg ? dtg : cur.dt;
+      R res;
+      {
+#pragma clang fp contract(off)
+        if (PRIM) {
+          const R rho = pc.x + fc * pc.y;
+          const R cp = pc.z + fc * pc.w;
+          res = (-Rv / cur.vol + cur.qv) * dt / (Tc * cp * pc.y + rho * cp + Tc * rho * pc.w);
+        } else {
+          res = (-Rv / cur.vol + cur.qv) * dt;
+        }
+      }
+      int bad2 = badc;
+      if (IRSMODE) {
+        vpend = res;
+      } else {
+        const R Tn = new_state<PRIM>(a.tab, cur.m, Tc, cur.TO, res, a.rkc, a.jam, &bad2);
+        vpend = Tn;
+        if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
+        const double d = (double)Tn - (double)cur.TO;
+        rsq += d * d;
+      }
+      ppend = p;
+      if (bad2) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+    }
+    spend = live;
+    fz_prev = fz;
+    const int s0 = sm;
+    sm = sc;
+    sc = sp;
+    sp = s0;
+    p += sz;
+    ph += sz;
+  };
+  for (int k = k0; k <= k1; k += 2) {
+    plane(sa_, sb_, k);
+    if (k + 1 <= k1) plane(sb_, sa_, k + 1);
+  }
+  if (spend) stn(out, ppend, vpend);
+  if (!IRSMODE && a.last) block_sum<TT>(rsq, a.partials, wg);
+}
+
+// ------------------------------------------------------------------------
 // Barrier-free stage kernel for orthogonal 3-D blocks: each wave owns one
 // 64-cell row (i) of the tile and marches its KC planes in k on its own.
 // i-neighbours come from the neighbouring lanes (shuffles; lane 0 and 63

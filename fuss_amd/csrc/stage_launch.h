@@ -66,13 +66,19 @@ void launch_stage3p_ring(fuss_ctx* c, Level& L, const StageArgs<R>& a, bool prim
   else launch_stage3p_t<8, TY>(c, L, a, prim, irs);
 }
 
+// blocks with tangential weights: the pipelined kernel on the 4-row columns
+template <bool UM, int KCT, typename R>
+void launch_stage3t(fuss_ctx* c, unsigned g, const StageArgs<R>& a, bool prim, bool irs) {
+  if (prim) { irs ? k_stage3t<UM, true, true, R, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3t<UM, true, false, R, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+  else { irs ? k_stage3t<UM, false, true, R, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3t<UM, false, false, R, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+}
+
 template <int KCT, typename R>
 void launch_stage3d_k(fuss_ctx* c, Level& L, const StageArgs<R>& a, bool tang, bool prim, bool irs) {
   const unsigned g = (unsigned)L.hctiles.size();
   const bool row = L.row_march >= 0 ? L.row_march == 1 : !c->stage_lds;
   if (tang) {
-    if (prim) { irs ? k_stage3d<true, true, true, R, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, true, false, R, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
-    else { irs ? k_stage3d<true, false, true, R, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<true, false, false, R, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
+    L.all_umat ? launch_stage3t<true, KCT>(c, g, a, prim, irs) : launch_stage3t<false, KCT>(c, g, a, prim, irs);
   } else if (!row && !L.dims2) {
     if (prim) { irs ? k_stage3d<false, true, true, R, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, true, false, R, KCT><<<g, TILE_T, 0, c->stream>>>(a); }
     else { irs ? k_stage3d<false, false, true, R, KCT><<<g, TILE_T, 0, c->stream>>>(a) : k_stage3d<false, false, false, R, KCT><<<g, TILE_T, 0, c->stream>>>(a); }

@@ -14,15 +14,16 @@ variables does not read the stage state (stage_state=False).
 from __future__ import annotations
 
 
-def stage_bytes(ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True, qvol=True, rect=False):
+def stage_bytes(ndir=3, tang=False, irs=True, prim=True, fp=8, mat=True, qvol=True, rect=False, dt_stream=True):
     """Fused flux + BC + newstate kernel, per cell.
 
     reads: T, dt, qvol, vol, material, own face weights (ndir normal weights,
     or 3 per face when tangential weights are present); writes the scaled
     residual (IRS on) or reads T^n and writes T (IRS off).  rect: volume and
-    weights are products of per-axis factors (no per-cell geometry read)."""
+    weights are products of per-axis factors (no per-cell geometry read).
+    dt_stream=False: a time-accurate step's global dt (one value, no stream)."""
     nw = 0 if rect else ndir * (3 if tang else 1)
-    b = fp * (1 + 1 + (1 if qvol else 0) + (0 if rect else 1) + nw) + (4 if mat else 0)
+    b = fp * (1 + (1 if dt_stream else 0) + (1 if qvol else 0) + (0 if rect else 1) + nw) + (4 if mat else 0)
     if irs:
         b += fp            # write R
     else:
