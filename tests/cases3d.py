@@ -156,3 +156,73 @@ def affine_split(axis, n=(8, 7, 6), h=(0.01, 0.012, 0.009), T_of_X=None, k=10.0,
     p = cs.Params(n_rk=n_rk, vnn=kw.pop("vnn", 0.5), prim=True, irs_beta=kw.pop("irs_beta", 0.0),
                   time_accurate=kw.pop("time_accurate", False), mgl=1, level_iter=[10**9], res_threshold=1e-10)
     return cs.Case(blocks, bcs, cases.const_table(k, 1000.0, 500.0), p, [], [np.full(bcs[0].n, -1, np.int32)])
+
+
+def chimera3d(n_rk=3, irs_beta=0.0, sheared=False):
+    """Two 3-D blocks side by side in x with different j / k counts: A (6 x 5 x
+    4 cells) receives chimera (102) ghosts on its faces 2 (i), 4 (j) and 6
+    (k) from donors among the cells of B (7 x 6 x 5); the other faces are
+    walls (302, 303, 301).  Per entry two first-layer donors (volume fractions
+    0.6, 0.4) and one second-layer donor, all interior cells of B.  The donor
+    choice is arbitrary (a parity case, not a physical overlap).  sheared:
+    both blocks on a sheared lattice (tangential weights)."""
+    dims = [(6, 5, 4), (7, 6, 5)]
+    h = (0.01, 0.012, 0.009)
+    blocks, ib, rb = [], [], []
+    chim = {}
+    rng = np.random.default_rng(21)
+    for b, (n, x0) in enumerate(zip(dims, [(0.0, 0.0, 0.0), (0.07, 0.0, 0.0)])):
+        S = SHEAR if sheared else np.eye(3)
+        nodes = affine_nodes(n, h, S=S, x0=x0)
+        T = 400.0 + 40.0 * rng.uniform(-1, 1, size=n[::-1])
+        blocks.append(cs.Block(n, nodes, T, np.ones_like(T), np.zeros_like(T)))
+    walls = [(302, cases.R(Tw=450.0)), (303, cases.R(hconv=30.0, Tref=300.0)), (301, cases.R(qw=-2e3)),
+             (302, cases.R(Tw=380.0)), (303, cases.R(hconv=60.0, Tref=320.0)), (301, cases.R())]
+    nB = dims[1]
+    for b, n in enumerate(dims):
+        for f in range(1, 7):
+            if b == 0 and f in (2, 4, 6):
+                x, y = cases.face_entries(1, n, f, 102, cases.R())
+                for r in range(x.shape[0]):
+                    i, j, k = (int(v) for v in x[r, 1:4])
+                    if f == 2:
+                        d = [(1, min(j, nB[1]), min(k, nB[2])), (2, min(j, nB[1]), min(k, nB[2])),
+                             (3, min(j, nB[1]), min(k, nB[2]))]
+                    elif f == 4:
+                        d = [(min(i, nB[0]), 1, min(k, nB[2])), (min(i, nB[0]), 2, min(k, nB[2])),
+                             (min(i, nB[0]), 3, min(k, nB[2]))]
+                    else:
+                        d = [(min(i, nB[0]), min(j, nB[1]), 1), (min(i, nB[0]), min(j, nB[1]), 2),
+                             (min(i, nB[0]), min(j, nB[1]), 3)]
+                    don = np.array([(2,) + q for q in d], np.int32)
+                    chim[sum(a.shape[0] for a in ib) + r] = (2, don, np.array([0.6, 0.4, 1.0]))
+            else:
+                x, y = cases.face_entries(b + 1, n, f, *walls[f - 1])
+            ib.append(x)
+            rb.append(y)
+    bcl = cases.bclist(np.concatenate(ib), np.concatenate(rb))
+    bcl.chim = chim
+    p = cs.Params(n_rk=n_rk, vnn=0.5, prim=True, irs_beta=irs_beta, mgl=1, level_iter=[10**9], res_threshold=1e-10)
+    return cs.Case(blocks, [bcl], cases.const_table(15.0, 3000.0, 600.0), p, [], [np.full(bcl.n, -1, np.int32)])
+
+
+def chimera_split(axis, T_of_X=None, k=10.0, n_rk=1, **kw):
+    """affine_split's two blocks with the cut faces joined by chimera (102)
+    entries instead of 101: each entry's first-layer donor is the cell of the
+    other block that its ghost cell coincides with (volume fraction 1), its
+    second-layer donor the next cell inward."""
+    c = affine_split(axis, T_of_X=T_of_X, k=k, n_rk=n_rk, **kw)
+    bcl = c.bcs[0]
+    ib = bcl.ib.copy()
+    chim = {}
+    for e in np.nonzero(ib[:, 5] == 101)[0]:
+        bs, i_s, js, ks, fs = (int(v) for v in ib[e, 6:11])
+        second = [i_s, js, ks]
+        second[axis] += 1 if fs % 2 == 1 else -1
+        don = np.array([[bs, i_s, js, ks], [bs] + second], np.int32)
+        chim[int(e)] = (1, don, np.array([1.0, 1.0]))
+        ib[e, 5] = 102
+    nb = cases.bclist(ib, bcl.rb.copy())
+    nb.chim = chim
+    c.bcs[0] = nb
+    return c

@@ -58,6 +58,14 @@ PINS = {
     "box_305": lambda: c3.box(faces=[(302, cases.R(Tw=600.0)), (305, cases.R(eps=0.7, hconv=35.0, Tref=300.0)),
                                      (301, cases.R()), (305, cases.R(eps=0.3, hconv=0.0, Tref=280.0)),
                                      (305, cases.R(eps=0.0, hconv=15.0, Tref=310.0)), (301, cases.R())], T0=520.0),
+    # chimera (102) receivers on i, j and k faces of a 3-D block (cubic ghost
+    # metric extrapolation across all three, the 3-D stage kernels and IRS
+    # next to chimera ghosts), Cartesian and sheared; the sheared two-block
+    # lattice joined by chimera entries along k
+    "chimera3d": lambda: c3.chimera3d(),
+    "chimera3d_sheared": lambda: c3.chimera3d(sheared=True),
+    "chimera3d_irs": lambda: c3.chimera3d(irs_beta=0.5, n_rk=4),
+    "chimera_split_k": lambda: c3.chimera_split(2, n_rk=3),
 }
 
 

@@ -236,3 +236,36 @@ def test_connection_exact_for_linear_field(oracle_mod, axis):
         sl[ax] = slice(1, None) if b == 1 else slice(0, -1)
         Rin = R[tuple(sl)]
         assert Rin.size and np.max(np.abs(Rin)) <= 1e-12 * fscale, (b, np.max(np.abs(Rin)), fscale)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_chimera_exact_for_linear_field(oracle_mod, axis):
+    """The same sheared two-block lattice joined by chimera (102) entries whose
+    donors are the cells the ghosts coincide with: the ghost enthalpy average
+    (Lib_Ghost.f90:218-231) returns the donor temperature, the extrapolated
+    ghost metric (BC_Extrapolate_Metrics, cubic across i, j and k faces of a
+    3-D block, Lib_Metrics.f90:379-600) is the lattice's own, and the flux
+    through Tg(1) with the boundary cell's tangential neighbours as Tg(3:6)
+    is exact for a linear field: no cell away from the outer walls heats."""
+    k = 10.0
+    grad = np.array([800.0, -600.0, 400.0])
+    case = c3.chimera_split(axis, T_of_X=lambda X: 500.0 + X @ grad, k=k)
+    o = oracle_mod.Oracle(case)
+    o.step(1)
+    faces = o.metrics()[3]
+    fscale = k * max(float(np.max(f[1])) for f in faces) * np.linalg.norm(grad)
+    ax = 2 - axis
+    for b in (1, 2):
+        R = -o.get("R", 1, b) * RHO_CP * o.get("vol", 1, b) / o.get("dt", 1, b)
+        sl = [slice(1, -1)] * 3
+        sl[ax] = slice(1, None) if b == 1 else slice(0, -1)
+        Rin = R[tuple(sl)]
+        assert Rin.size and np.max(np.abs(Rin)) <= 1e-12 * fscale, (b, np.max(np.abs(Rin)), fscale)
+    # a ghost from the wrong donor (the cell beside the coincident one) heats the cut
+    bad = c3.chimera_split(axis, T_of_X=lambda X: 500.0 + X @ grad, k=k)
+    for e, (n1, don, vf) in bad.bcs[0].chim.items():
+        don[0, 1 + (axis + 1) % 3] = max(1, don[0, 1 + (axis + 1) % 3] - 1)
+    ob = oracle_mod.Oracle(bad)
+    ob.step(1)
+    Rb = -ob.get("R", 1, 1) * RHO_CP * ob.get("vol", 1, 1) / ob.get("dt", 1, 1)
+    assert np.max(np.abs(Rb)) > 1e-6 * fscale
