@@ -86,6 +86,33 @@ def calibration_copy(nbytes=1 << 30, reps=3):
     hip.hipFree(b)
 
 
+def stream_copy_gbs(nbytes=1 << 30, reps=5):
+    """Achievable HBM bandwidth: the best of `reps` device-to-device copies of
+    nbytes (read + write), timed with HIP events; GB/s of read + written bytes."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    best = 0.0
+    for _ in range(reps):
+        hip.hipEventRecord(e0, None)
+        assert hip.hipMemcpyAsync(b, a, ctypes.c_size_t(nbytes), 3, None) == 0  # device to device
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float(0.0)
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        best = max(best, 2.0 * nbytes / (ms.value * 1e-3) / 1e9)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return best
+
+
 def cpu_baseline(n, iters):
     """The oracle's OpenMP build (the C restatement of the reference, FP64,
     its per-cell and per-line loops shared over the host's cores as the
@@ -661,9 +688,21 @@ def main():
         out["lattice"] = lattice_check(args)
     if args.calib_copy:
         calibration_copy()
+    # the achievable bandwidth beside the peak: a 1 GiB device-to-device copy
+    copy_gbs = stream_copy_gbs()
+    out["roofline"]["achievable_copy_GBs"] = copy_gbs
+    out["roofline"]["frac_of_copy"] = out["roofline"]["achieved"] / copy_gbs
+    print(f"[bench] device copy {copy_gbs:.0f} GB/s; stage kernel at {out['roofline']['frac_of_copy']:.2f} of it",
+          file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print(f"[bench] cpu baseline: oracle {args.cpu_n}^3 x {args.cpu_iters}", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_iters)
+        cb = cpu_baseline(args.cpu_n, args.cpu_iters)
+        if "time_to_1e-10" in out and args.cpu_n == args.n:
+            # the plain march's iteration count (the same trajectory) at the CPU's
+            # iteration rate: an estimate, as the GPU's plain time at this size
+            it = out["time_to_1e-10"]["fp64_plain"]["iterations_est"]
+            cb["time_to_1e-10_est_s"] = it / (cb["value"] / ncell)
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 
