@@ -45,22 +45,23 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the stage kernel measured with rocprofv3 PMC
-# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB device copy);
-# source: profiles/explicit_step_cube256_fp64.md (256^3 cube, FP64).
-# The LDS-ring stage kernel (stages 2-4 of a steady step: no dt) on 64x4
-# columns of kc planes, one material, primitive variables, IRS on; rect =
-# per-axis metric factors (rectilinear level).
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB device copy) on
+# the 256^3 cube, FP64: the LDS-ring stage kernel (stages 2-4 of a steady
+# step: no dt) on 64 x 4 or 64 x 8 columns of kc planes, one material,
+# primitive variables, IRS on; rect = per-axis metric factors (rectilinear
+# level).
 
 
 def stage_kernel(rect, kc, rows):
     return f"k_stage3p<true, true, true, {'true' if rect else 'false'}, false, double, {kc}, {rows}>"
 
 
-STAGE_TRAFFIC_MEASURED = {stage_kernel(True, kc, 4): fetch * 1024 * 2.0 + 131072 * 1024 * 1.0
-                          for kc, fetch in ((4, 153140), (8, 154981), (16, 149399), (32, 143788), (64, 140748))}
+STAGE_TRAFFIC_MEASURED = {stage_kernel(True, kc, rows): fetch * 1024 * 2.0 + 131072 * 1024 * 1.0
+                          for kc, rows, fetch in ((4, 4, 153140), (8, 4, 154981), (16, 4, 149399), (32, 4, 143788),
+                                                  (64, 4, 140748), (32, 8, 143789), (64, 8, 140803))}
 # Plain FP64 path (four-stage RK + LTS + IRS 0.5, 3-level sequencing) run to
-# residual 1e-10 on the GPU: iterations per cube size (profiles/
-# steady_convergence_cube.md).  Fit: iters = 230218 (n/64)^PLAIN_EXP.
+# residual 1e-10 on the GPU: iterations per cube size (bench runs with
+# --plain-conv-n 32, 48, 64).  Fit: iters = 230218 (n/64)^PLAIN_EXP.
 PLAIN_ITERS_MEASURED = {32: 64506, 48: 135672, 64: 230218}
 PLAIN_EXP = 1.835
 
@@ -550,7 +551,7 @@ def main():
         summarize_profile(args.summarize_profile)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding of the path is design-only", file=sys.stderr)
         sys.exit(2)
 

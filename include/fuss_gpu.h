@@ -2,7 +2,7 @@
  * fuss_gpu.h -- C-ABI of the MI355X (gfx950) explicit solver step of FUSS.
  *
  * Drop-in boundary for the reference's hot path (open-hydra/FUSS, Fortran):
- * the driver keeps FUSS_setup / FUSS_solve / FUSS_postprocess, the INI
+ * its Fortran driver keeps FUSS_setup / FUSS_solve / FUSS_postprocess, the INI
  * configuration and the BC / material files, and calls these entry points
  * through iso_c_binding (fuss_amd/fortran/fuss_gpu_m.f90) instead of the
  * OpenMP loops of Explicit_Step.  Plain C types only: int, double, pointers
