@@ -328,14 +328,15 @@ __global__ void k_set_matg(BCRec* bc, int n, const int* mat) {
 }
 
 // blocks whose qvol differs from the value of their cell (1,1,1): nonu[b] = 1
+// (work-group (b, y) scans planes y, y + gridDim.y, ... of block b)
 __global__ void k_qvol_nonuniform(const BlockInfo* blk, const double* q, int* nonu) {
   const BlockInfo B = blk[blockIdx.x];
   const double q0 = q[B.base];
+  const int nij = B.ni * B.nj;
   int bad = 0;
   for (int k = blockIdx.y; k < B.nk && !bad; k += gridDim.y)
-    for (int j = 0; j < B.nj && !bad; j++)
-      for (int i = threadIdx.x; i < B.ni; i += blockDim.x)
-        if (q[B.base + i + (long)j * B.sy + (long)k * B.sz] != q0) bad = 1;
+    for (int e = threadIdx.x; e < nij; e += blockDim.x)
+      if (q[B.base + e % B.ni + (long)(e / B.ni) * B.sy + (long)k * B.sz] != q0) bad = 1;
   if (bad) nonu[blockIdx.x] = 1;
 }
 
@@ -2003,7 +2004,7 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     Level& L = c->lev[m];
     DBuf<int> nonu;
     if (nonu.alloc(L.nb)) return FUSS_EHIP;
-    k_qvol_nonuniform<<<dim3(L.nb, 16), 64, 0, c->stream>>>(L.blk.p, L.qvol.p, nonu.p);
+    k_qvol_nonuniform<<<dim3(L.nb, 64), 256, 0, c->stream>>>(L.blk.p, L.qvol.p, nonu.p);
     HIPCHK(hipGetLastError());
     std::vector<int> hn(L.nb);
     HIPCHK(hipMemcpyAsync(hn.data(), nonu.p, sizeof(int) * L.nb, hipMemcpyDeviceToHost, c->stream));
