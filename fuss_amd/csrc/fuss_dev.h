@@ -83,6 +83,7 @@ struct StageArgs {
   R* dtw;
   double* pmin;        // per-tile dt minima
   int conn_split;      // connection (101) entries are evaluated by k_connflux, not k_bcflux
+  unsigned bcz;        // an entry of bcf that is always +0 (its last)
 };
 
 // Compute_dt (Mod_dt.f90:26-84) for the cells of tile `wg` at iteration

@@ -1305,7 +1305,7 @@ int build_bc(fuss_ctx* c, Level& L) {
   e |= L.bch.alloc(std::max(n, 1));
   e |= L.gmap.alloc(gm.size());
   e |= L.bcv.alloc(std::max(n, 1));
-  e |= L.bcf.alloc(std::max<size_t>(fmap.size(), 1));
+  e |= L.bcf.alloc(fmap.size() + 1);  // (+ an entry that stays 0)
   e |= L.fmap.alloc(std::max<size_t>(fmap.size(), 1));
   if (e) return FUSS_EHIP;
   if (n) e |= upload(L.bc.p, recs.data(), sizeof(BCRec) * n);
@@ -1352,7 +1352,7 @@ int build_bc_dev(fuss_ctx* c, Level& L, size_t nf) {
   e |= L.bch.alloc(std::max(n, 1));
   e |= L.gmap.alloc(std::max(n, 1));
   e |= L.bcv.alloc(std::max(n, 1));
-  e |= L.bcf.alloc(std::max<size_t>(nf, 1));
+  e |= L.bcf.alloc(nf + 1);  // (+ an entry that stays 0)
   e |= L.fmap.alloc(std::max<size_t>(nf, 1));
   if (e) return FUSS_EHIP;
   DBuf<int> dib, dser, bits, flags;
@@ -2009,7 +2009,11 @@ int fuss_setup(fuss_ctx* c, double initial_time) {
     std::vector<int> hn(L.nb);
     HIPCHK(hipMemcpyAsync(hn.data(), nonu.p, sizeof(int) * L.nb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (int b = 0; b < L.nb; b++) L.hblk[b].uqv = !hn[b];
+    L.all_uqv = 1;
+    for (int b = 0; b < L.nb; b++) {
+      L.hblk[b].uqv = !hn[b];
+      L.all_uqv &= L.hblk[b].uqv;
+    }
     HIPCHK(hipMemcpy(L.blk.p, L.hblk.data(), sizeof(BlockInfo) * L.nb, hipMemcpyHostToDevice));
   }
   // compact wall records, now that every level's materials are set
@@ -2248,6 +2252,7 @@ int enqueue_step_t(fuss_ctx* c, int level, const fuss_step_params* p, const Stag
     sa.mat = L.mat.p;
     for (int w = 0; w < 9; w++) sa.W[w] = V.W[w];
     sa.bc = L.bc.p; sa.fmap = L.fmap.p; sa.bcv = V.bcv; sa.bcf = V.bcf; sa.tab = dtab<R>(c);
+    sa.bcz = (unsigned)(L.bcf.n - 1);
     sa.ser = serdev(c); sa.clk = L.clk.p; sa.halt = halt;
     sa.partials = L.partials.p; sa.status = c->status.p;  // k_bcflux flags table errors through status
     sa.rect = (L.rect_w && !c->rect_off) ? L.rfac.p : nullptr;
@@ -3192,6 +3197,11 @@ int autotune(fuss_ctx* c, int level, const fuss_step_params* p, int steps, int* 
   for (int st = 4; st <= (ring ? 8 : 4); st += 4)
     for (int ty = 4; ty <= (xyz ? 8 : 4); ty += 4)
       for (int ki = 0; ki < 5 && !err && (st == 8 || ty == 8); ki++) err = trial(row4, kcs[ki], st, ty);
+  // the register-column stage kernel (rectilinear levels; column heights 8,
+  // 16 and 32) with either IRS column
+  const bool regcol = ring && L.rect_w && !c->rect_off && c->Ti1 >= 0;
+  for (int ty = 4; ty <= (xyz ? 8 : 4) && regcol; ty += 4)
+    for (int ki = 1; ki <= 3 && !err; ki++) err = trial(row4, kcs[ki], FUSS_STAGE_REGCOL, ty);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HIPCHK(hipStreamSynchronize(c->stream));  // the last restore has read `save`
@@ -3225,10 +3235,17 @@ int fuss_set_tiles(fuss_ctx* c, int level, int kc, int row_march, int stage_rows
   if (c) drop_dt_next(c);
   if (!c || !c->setup_done || level < 1 || level > c->nlev) return FUSS_EARG;
   if ((kc != 4 && kc != 8 && kc != 16 && kc != 32 && kc != 64) || row_march < -1 || row_march > 1 ||
-      (stage_rows != 4 && stage_rows != 8) || (irs_rows != 4 && irs_rows != 8))
+      (stage_rows != 4 && stage_rows != 8 && stage_rows != FUSS_STAGE_REGCOL) || (irs_rows != 4 && irs_rows != 8))
     return FUSS_EARG;
   Level& L = c->lev[level - 1];
   if (L.kp != GC) return 0;  // 2-D levels have no column tiles
+  // the register-column kernel only where it can run (rectilinear level with
+  // its per-axis factors in use, orthogonal blocks, tables from T >= 0, its
+  // column heights)
+  if (stage_rows == FUSS_STAGE_REGCOL &&
+      !(L.rect_w && !c->rect_off && c->Ti1 >= 0 && !L.tang && L.dims2 && row_march != 1 &&
+        (kc == 8 || kc == 16 || kc == 32)))
+    return FUSS_EUNSUPPORTED;
   const int e = set_kc(c, L, kc);
   if (e) return e;
   L.row_march = row_march;

@@ -77,6 +77,7 @@ struct Level {
   int dt_next = 0;                  // dt and its per-tile minima hold the next step's (fused IRS pass)
   double dt_vnn = 0.0;              // ... computed with this vnn
   int all_umat = 0;                 // every block has a single material
+  int all_uqv = 0;                  // every block has a uniform qvol
   int dims2 = 0;                    // every block has >= 2 cells in each direction
   // rectilinear levels: per-axis metric factors (RectAxes) that reproduce the
   // stored volumes and normal weights to rect_dev (relative) and sum 1/dl^2 exactly

@@ -90,11 +90,25 @@ void launch_stage3d_k(fuss_ctx* c, Level& L, const StageArgs<R>& a, bool tang, b
   }
 }
 
+// the register-column kernel (stage_reg_{d,f}.hip)
+template <typename R>
+void launch_stage3r(fuss_ctx* c, Level& L, const StageArgs<R>& a, bool prim, bool irs);
+
+// Levels the register-column kernel runs: per-axis metric factors in use,
+// table rows from T >= 0 (its row conversion saturates at 0), a column
+// height it is built for.
+template <typename R>
+inline bool rc_eligible(const Level& L, const StageArgs<R>& a) {
+  return a.rect && a.tab.Ti1 >= 0 && (L.kc == 8 || L.kc == 16 || L.kc == 32);
+}
+
 template <typename R>
 void launch_stage3d(fuss_ctx* c, Level& L, const StageArgs<R>& a, bool tang, bool prim, bool irs) {
   const bool row = L.row_march >= 0 ? L.row_march == 1 : !c->stage_lds;
   if (!tang && !row && L.dims2) {
-    L.stage_ty == 8 ? launch_stage3p_ring<R, 8>(c, L, a, prim, irs) : launch_stage3p_ring<R, 4>(c, L, a, prim, irs);
+    if (L.stage_ty == FUSS_STAGE_REGCOL && rc_eligible(L, a)) launch_stage3r<R>(c, L, a, prim, irs);
+    else if (L.stage_ty == 8) launch_stage3p_ring<R, 8>(c, L, a, prim, irs);
+    else launch_stage3p_ring<R, 4>(c, L, a, prim, irs);
     return;
   }
   if (L.kc == 4) launch_stage3d_k<4>(c, L, a, tang, prim, irs);

@@ -6,6 +6,8 @@ namespace fuss {
 // the LDS-ring kernels are instantiated in stage_ring{4,8}_d.hip
 extern template void launch_stage3p_ring<double, 4>(fuss_ctx*, Level&, const StageArgs<double>&, bool, bool);
 extern template void launch_stage3p_ring<double, 8>(fuss_ctx*, Level&, const StageArgs<double>&, bool, bool);
+// the register-column kernel in stage_reg_d.hip
+extern template void launch_stage3r<double>(fuss_ctx*, Level&, const StageArgs<double>&, bool, bool);
 template void launch_stage2d<double>(fuss_ctx*, Level&, const StageArgs<double>&, bool, bool, bool);
 template void launch_stage3d<double>(fuss_ctx*, Level&, const StageArgs<double>&, bool, bool, bool);
 template void launch_stage_raw<double>(fuss_ctx*, Level&, const StageArgs<double>&, int, bool);

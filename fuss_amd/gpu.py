@@ -31,6 +31,7 @@ FUSS_ETABLE = 3
 FUSS_EMETRIC = 4
 FUSS_EUNSUPPORTED = 5
 FUSS_EBC = 7
+FUSS_STAGE_REGCOL = 1  # fuss_set_tiles stage_rows: the register-column stage kernel
 
 # fuss_stage_fn: (user, level, irk, block, ni, nj, nk, const double *T, double *R)
 STAGE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

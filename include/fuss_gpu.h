@@ -207,11 +207,15 @@ int fuss_autotune(fuss_ctx *ctx, int level, const fuss_step_params *p, int steps
 /* Column tiles of a 3-D level without timing (no reference counterpart: the
  * reference has no tiles): kc = 4, 8, 16, 32 or 64 planes; row_march = -1
  * (context default), 0 (LDS ring) or 1 (row march); stage_rows = 4 or 8 rows
- * per column of the LDS-ring stage kernel (orthogonal blocks); irs_rows = 4
- * or 8 rows per column of the fused x-y-z IRS pass (levels without j/k
- * connections, Jacobi IRS).  fuss_autotune also chooses the rows; fuss_tiles
- * reports the level's current choice (0s on 2-D levels).  Results do not
- * depend on the tiles except for the grouping of the residual's partial sums. */
+ * per column of the LDS-ring stage kernel (orthogonal blocks), or
+ * FUSS_STAGE_REGCOL: the register-column stage kernel (one wave per 64x4
+ * column, no LDS ring and no barriers; rectilinear levels with kc = 8, 16 or
+ * 32 -- other levels run the 4-row LDS ring); irs_rows = 4 or 8 rows per
+ * column of the fused x-y-z IRS pass (levels without j/k connections, Jacobi
+ * IRS).  fuss_autotune also chooses the rows and kernels; fuss_tiles reports
+ * the level's current choice (0s on 2-D levels).  Results do not depend on
+ * the tiles except for the grouping of the residual's partial sums. */
+#define FUSS_STAGE_REGCOL 1
 int fuss_set_tiles(fuss_ctx *ctx, int level, int kc, int row_march, int stage_rows, int irs_rows);
 int fuss_tiles(fuss_ctx *ctx, int level, int *kc, int *row_march, int *stage_rows, int *irs_rows);
 
