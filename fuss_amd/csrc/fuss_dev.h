@@ -642,3 +642,38 @@ __device__ __forceinline__ int tab_row(R T, int Ti1, int Ti2, int* bad) {
   if (!(T >= (R)Ti1 && T < (R)Ti2)) *bad = 1;
   return (int)fmin(fmax(T, (R)Ti1), (R)(Ti2 - 1));
 }
+
+// Table row of T clamped to [Ti1, Ti2 - 1]: the hardware conversion
+// saturates (and maps NaN to 0), so with Ti1 >= 0 this is
+// trunc(fmin(fmax(T, Ti1), Ti2 - 1)) of tab_rowf as an integer.
+// (hi is held in a VGPR: one instruction may read one scalar register)
+__device__ __forceinline__ int med3_i32(int c, int lo, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(c), "s"(lo), "v"(hi));
+  return r;
+}
+__device__ __forceinline__ int row_sat(double T, int Ti1, int Ti2m1) {
+  int c;
+  asm("v_cvt_i32_f64 %0, %1" : "=v"(c) : "v"(T));
+  return med3_i32(c, Ti1, Ti2m1);
+}
+__device__ __forceinline__ int row_sat(float T, int Ti1, int Ti2m1) {
+  int c;
+  asm("v_cvt_i32_f32 %0, %1" : "=v"(c) : "v"(T));
+  return med3_i32(c, Ti1, Ti2m1);
+}
+// a uniform value copied to a VGPR once
+__device__ __forceinline__ int to_vgpr(int s) {
+  int v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+  return v;
+}
+
+// An opaque statement: a branch on a uniform condition that contains it
+// stays a scalar branch (it is not speculated into a select).
+__device__ __forceinline__ void keep_branch() { asm volatile(""); }
+
+// x - floor(x): for x >= 0 the interpolation fraction x - trunc(x) of a
+// table row (exact: the integer part's subtraction loses nothing)
+__device__ __forceinline__ double frac_of(double x) { return __builtin_amdgcn_fract(x); }
+__device__ __forceinline__ float frac_of(float x) { return __builtin_amdgcn_fractf(x); }

@@ -141,6 +141,11 @@ template <typename T>
 __device__ __forceinline__ T ldu(const T* __restrict__ base, unsigned idx) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * (unsigned)sizeof(T));
 }
+// (a byte offset instead of an element index)
+template <typename T>
+__device__ __forceinline__ T ldb(const void* base, unsigned off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
 template <typename T>
 __device__ __forceinline__ void stu(T* __restrict__ base, unsigned idx, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T)) = v;

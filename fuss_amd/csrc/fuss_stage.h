@@ -399,10 +399,17 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
   const int k0 = t.k0, k1 = min(B.nk, k0 + KCT - 1);
   const bool live = i <= B.ni && j <= B.nj;
   const int x = tx + 1, y = ty + 1;
-  const bool yhalo = ty == 0 && t.j0 > 1 && i <= B.ni;
+  // Rectilinear levels evaluate every face of the tile, the block's faces
+  // with weight 0 (per-axis factors f = 0 there): their fluxes are +-0 and
+  // are added unconditionally (the sum starts at +0 and never becomes -0,
+  // so adding +-0 changes nothing)
+  const bool yhalo = ty == 0 && (RECT || t.j0 > 1) && i <= B.ni;
   const int hr = tx;
-  const bool xhalo = ty == 1 && tx < TY && t.i0 > 1 && t.j0 + hr <= B.nj;
+  // (rectilinear levels: also where the column starts the block, with weight 0)
+  const bool xhalo = ty == 1 && tx < TY && (RECT || t.i0 > 1) && t.j0 + hr <= B.nj;
   const bool hx = i < B.ni, hy = j < B.nj;  // the cell's +x / +y face is interior
+  // the halo face is interior (table errors are counted for these only)
+  const bool yhalo_in = yhalo && t.j0 > 1, xhalo_in = xhalo && t.i0 > 1;
   const int hq = xhalo ? hr + 1 : 1;        // LDS row of the x-halo face (in range for every lane)
   __shared__ R ring[3][TY + 2][PX];
   __shared__ R fx[TY][TILE_X + 1];
@@ -410,12 +417,14 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
   using V2 = typename Vec2<R>::T;
   using V4 = typename Vec4<R>::T;
   const int Ti1 = a.tab.Ti1, Ti2 = a.tab.Ti2, nT = a.tab.nT;
+  const int Ti2m1 = to_vgpr(Ti2 - 1);
   const int mu = B.umat >= 0 ? B.umat : 0;
-  // first row of a material's table (row of T = Ti1)
-  const V2* kap_u = reinterpret_cast<const V2*>(a.tab.kap2) + (long)mu * nT;
-  const V4* pr_u = reinterpret_cast<const V4*>(a.tab.pr4) + (long)mu * nT;
-  const V2* kap0 = reinterpret_cast<const V2*>(a.tab.kap2);
-  const V4* pr0 = reinterpret_cast<const V4*>(a.tab.pr4);
+  // a material's rows addressed by the row number itself (row_sat)
+  const V2* kap0 = reinterpret_cast<const V2*>(a.tab.kap2) - Ti1;
+  const V4* pr0 = reinterpret_cast<const V4*>(a.tab.pr4) - Ti1;
+  const V2* kap_u = kap0 + (long)mu * nT;
+  const V4* pr_u = pr0 + (long)mu * nT;
+  auto kin = [&](R T) -> int { return !(T >= (R)Ti1 && T < (R)Ti2); };
   // the halo face's weight array: +y weights for the y-halo row, +x for the x-halo
   const R* __restrict__ Wh = yhalo ? a.W[4] : a.W[0];
   // RECT (every block rectilinear): volume and normal weights are products of
@@ -430,15 +439,19 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
     const RectAxes rx_ = rect_axes(a.rect, B);
     if (threadIdx.x < (DTC ? 3 : 2) * (KCT + 1)) {
       const int q = threadIdx.x % (KCT + 1), kk = min(k0 - 1 + q, B.nk + 1), row = threadIdx.x / (KCT + 1);
-      zfac[row][q] = row == 0 ? rx_.az[kk] : (row == 1 ? rx_.fz[kk] : rx_.qz[kk]);
+      // (f_z = 0 on the block's last plane: its +z flux is +-0)
+      zfac[row][q] = row == 0 ? rx_.az[kk] : (row == 1 ? (kk < B.nk ? rx_.fz[kk] : 0.0) : rx_.qz[kk]);
     }
     const int ci_ = live ? i : 1, cj_ = live ? j : 1;
     cvol = (R)(rx_.ax[ci_] * rx_.ay[cj_]);
-    cw0 = (R)(rx_.fx[ci_] * rx_.ay[cj_]);
+    // (0 on the block's +x face: the face's flux is then +-0 and is added
+    // unconditionally)
+    cw0 = hx ? (R)(rx_.fx[ci_] * rx_.ay[cj_]) : (R)0;
     if (DTC) qxy = rx_.qx[ci_] + rx_.qy[cj_];
-    cw4 = (R)(rx_.fy[cj_] * rx_.ax[ci_]);
+    cw4 = hy ? (R)(rx_.fy[cj_] * rx_.ax[ci_]) : (R)0;
     // the halo face: +y face of (i, j0-1) or +x face of (i0-1, j0+hr)
-    cwh = yhalo ? (R)(rx_.fy[t.j0 - 1] * rx_.ax[i]) : (xhalo ? (R)(rx_.fx[t.i0 - 1] * rx_.ay[t.j0 + hr]) : (R)0);
+    cwh = yhalo_in ? (R)(rx_.fy[t.j0 - 1] * rx_.ax[i])
+                   : (xhalo_in ? (R)(rx_.fx[t.i0 - 1] * rx_.ay[t.j0 + hr]) : (R)0);
   }
   int sm = 0, sc = 1, sp = 2;
   PlaneRegs<R> pre;
@@ -452,8 +465,11 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
   // halo face's left cell (the cell itself for other lanes)
   const int ci = live ? i : 1, cj = live ? j : 1;
   unsigned p = (unsigned)(B.base + (ci - 1) + (long)(cj - 1) * B.sy + (long)(k0 - 1) * B.sz);
-  unsigned ph = yhalo ? p - (unsigned)B.sy
-                      : (xhalo ? (unsigned)(B.base + (t.i0 - 2) + (long)(t.j0 + hr - 1) * B.sy + (long)(k0 - 1) * B.sz)
+  // (where the x-halo face is a block face, the row's own first cell: the
+  // material read for it is then valid)
+  unsigned ph = yhalo ? (t.j0 > 1 ? p - (unsigned)B.sy : p)
+                      : (xhalo ? (unsigned)(B.base + max(t.i0 - 2, 0) + (long)(t.j0 + hr - 1) * B.sy +
+                                            (long)(k0 - 1) * B.sz)
                                : p);
   // face-map positions of the lane's boundary faces (BC values in face
   // order; at most one face per axis since every block dimension is >= 2)
@@ -467,6 +483,13 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
   const int fa_t = B.face_active;
   const bool xtile = (fa_t & 3) && (t.i0 == 1 || t.i0 + TILE_X - 1 >= B.ni);
   const bool yrow = (fa_t & 12) && (j == 1 || j == B.nj);
+  // byte offsets of the lane's BC values in bcf (x and y: base + step per
+  // plane); lanes without an entry read the entry that stays 0, so a tile,
+  // row or plane with boundary faces subtracts its values unconditionally
+  constexpr unsigned SR = sizeof(R);
+  const unsigned xb8 = (xo >= 0 ? (unsigned)xo : a.bcz) * SR, xs8 = xo >= 0 ? (unsigned)B.nj * SR : 0u;
+  const unsigned yb8 = (yo >= 0 ? (unsigned)yo : a.bcz) * SR, ys8 = yo >= 0 ? (unsigned)B.ni * SR : 0u;
+  const unsigned zl8 = (zlo >= 0 ? (unsigned)zlo : a.bcz) * SR, zh8 = (zhi >= 0 ? (unsigned)zhi : a.bcz) * SR;
   const R dtg = a.use_dtg ? (R)(*a.dtg) : (R)0;
   // a block of uniform qvol: every lane reads the value of cell (1,1,1) (one
   // cached line instead of a stream)
@@ -526,45 +549,50 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
     V4 pc{};
     // BC values of the lane's boundary faces, issued with the table rows;
     // only tiles on an i face, rows on a j face and the end planes load them
-    R bx = 0, by = 0, bz = 0;
-    if (xtile) bx = ldu(a.bcf, (unsigned)max(xo >= 0 ? xo + B.nj * (k - 1) : 0, 0));
-    if (yrow) by = ldu(a.bcf, (unsigned)max(yo >= 0 ? yo + B.ni * (k - 1) : 0, 0));
+    // (each read only under the test that loads it)
+    R bx, by, bz;
     // (a lane's z face has an entry only on an end plane of a block with z entries)
-    if ((fa_t & 48) && (k == 1 || k == B.nk)) bz = ldu(a.bcf, (unsigned)max(zo_of(k), 0));
+    const bool zpl = ((fa_t & 16) && k == 1) || ((fa_t & 32) && k == B.nk);
+    if (xtile) bx = ldb<R>(a.bcf, xb8 + xs8 * (unsigned)(k - 1));
+    if (yrow) by = ldb<R>(a.bcf, yb8 + ys8 * (unsigned)(k - 1));
+    if (zpl) bz = ldb<R>(a.bcf, k == 1 ? zl8 : zh8);
     {
 #pragma clang fp contract(off)
-      // rows addressed by 32-bit offsets from the material's first row
+      // rows addressed by 32-bit offsets from the material's rows; the
+      // fraction T - row is x - floor(x) (the same for T in the table; outside
+      // it an error is flagged)
       const V2* kr = UM ? kap_u : kap0 + (long)cur.m * nT;
       int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
       Tx = (R)0.5 * (Tc + Txr);
-      const R rxf = tab_rowf(Tx, Ti1, Ti2, &b0);
-      kx = ldu(kr, (unsigned)((int)rxf - Ti1));
-      Tx = Tx - rxf;
+      kx = ldu(kr, (unsigned)row_sat(Tx, Ti1, Ti2m1));
+      if (!PRIM) b0 = kin(Tx);
+      Tx = frac_of(Tx);
       Ty = (R)0.5 * (Tc + Tyr);
-      const R ryf = tab_rowf(Ty, Ti1, Ti2, &b1);
-      ky = ldu(kr, (unsigned)((int)ryf - Ti1));
-      Ty = Ty - ryf;
+      ky = ldu(kr, (unsigned)row_sat(Ty, Ti1, Ti2m1));
+      if (!PRIM) b1 = kin(Ty);
+      Ty = frac_of(Ty);
       Tz = (R)0.5 * (Tc + Tzr);
-      const R rzf = tab_rowf(Tz, Ti1, Ti2, &b2);
-      kz = ldu(kr, (unsigned)((int)rzf - Ti1));
-      Tz = Tz - rzf;
+      kz = ldu(kr, (unsigned)row_sat(Tz, Ti1, Ti2m1));
+      if (!PRIM) b2 = kin(Tz);
+      Tz = frac_of(Tz);
       if (PRIM) {
-        const R rcf = tab_rowf(Tcl, Ti1, Ti2, &badc);
-        pc = ldu(UM ? pr_u : pr0 + (long)cur.m * nT, (unsigned)((int)rcf - Ti1));
-        if (DTC) kc = ldu(kr, (unsigned)((int)rcf - Ti1));  // kappa at the cell for its dt
-        fc = Tcl - rcf;
+        const int rc = row_sat(Tcl, Ti1, Ti2m1);
+        pc = ldu(UM ? pr_u : pr0 + (long)cur.m * nT, (unsigned)rc);
+        if (DTC) kc = ldu(kr, (unsigned)rc);  // kappa at the cell for its dt
+        badc = kin(Tcl);
+        fc = frac_of(Tcl);
       }
       if (ty < 2) {  // waves 0 and 1 hold the halo lanes (a wave-uniform test)
         const V2* krh = UM ? kap_u : kap0 + (long)cur.mh * nT;
         Th = yhalo ? (R)0.5 * (ring[sc][0][x] + ring[sc][1][x]) : (R)0.5 * (ring[sc][hq][0] + ring[sc][hq][1]);
-        const R rhf = tab_rowf(Th, Ti1, Ti2, &b3);
-        kh = ldu(krh, (unsigned)((int)rhf - Ti1));
-        Th = Th - rhf;
+        kh = ldu(krh, (unsigned)row_sat(Th, Ti1, Ti2m1));
+        if (!PRIM) b3 = kin(Th);
+        Th = frac_of(Th);
       }
       // primitive variables: every cell's own row is checked (badc), and a face
       // average lies between its two cells' temperatures, so a face outside
       // the table implies a cell outside it -- the face tests are redundant
-      bad = PRIM ? 0 : (live && ((hx && b0) || (hy && b1) || (k < B.nk && b2))) || ((yhalo || xhalo) && b3);
+      bad = PRIM ? 0 : (live && ((hx && b0) || (hy && b1) || (k < B.nk && b2))) || ((yhalo_in || xhalo_in) && b3);
       badc = live && badc;
     }
     // phase B: the next plane's loads (the last plane re-reads its own cells)
@@ -592,9 +620,9 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
     {
 #pragma clang fp contract(off)
       if (live) {
-        if (hx) fx[ty][tx + 1] = (kx.x + Tx * kx.y) * ((Txr - Tc) * cur.w0);
-        if (hy) fy[ty + 1][tx] = (ky.x + Ty * ky.y) * ((Tyr - Tc) * cur.w4);
-        if (k < B.nk) fz = (kz.x + Tz * kz.y) * ((Tzr - Tc) * cur.w8);
+        if (RECT || hx) fx[ty][tx + 1] = (kx.x + Tx * kx.y) * ((Txr - Tc) * cur.w0);
+        if (RECT || hy) fy[ty + 1][tx] = (ky.x + Ty * ky.y) * ((Tyr - Tc) * cur.w4);
+        if (RECT || k < B.nk) fz = (kz.x + Tz * kz.y) * ((Tzr - Tc) * cur.w8);
       }
       if (yhalo) fy[0][tx] = (kh.x + Th * kh.y) * ((ring[sc][1][x] - ring[sc][0][x]) * cur.wh);
       if (xhalo) fx[hr][0] = (kh.x + Th * kh.y) * ((ring[sc][hq][1] - ring[sc][hq][0]) * cur.wh);
@@ -605,19 +633,33 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
       R Rv = 0;
       {
 #pragma clang fp contract(off)
-        if (i > 1) Rv = Rv + fx[ty][tx];
-        if (hx) Rv = Rv - fx[ty][tx + 1];
-        if (j > 1) Rv = Rv + fy[ty][tx];
-        if (hy) Rv = Rv - fy[ty + 1][tx];
-        if (k > 1) Rv = Rv + fz_prev;
-        if (k < B.nk) Rv = Rv - fz;
+        // (rectilinear levels: the block's faces hold +-0; fz_prev is +0 on
+        // the block's first plane)
+        if (RECT || i > 1) Rv = Rv + fx[ty][tx];
+        if (RECT || hx) Rv = Rv - fx[ty][tx + 1];
+        if (RECT || j > 1) Rv = Rv + fy[ty][tx];
+        if (RECT || hy) Rv = Rv - fy[ty + 1][tx];
+        if (RECT || k > 1) Rv = Rv + fz_prev;
+        if (RECT || k < B.nk) Rv = Rv - fz;
       }
       {
 #pragma clang fp contract(off)
-        // BC_Fluxes contributions in face order (as apply_bc_faces)
-        if (xo >= 0) Rv = Rv - bx;
-        if (yo >= 0) Rv = Rv - by;
-        if (zo_of(k) >= 0) Rv = Rv - bz;
+        // BC_Fluxes contributions in face order (as apply_bc_faces); +0
+        // where the cell has none
+        // (scalar branches: an opaque statement keeps them from being turned
+        // into selects)
+        if (xtile) {
+          keep_branch();
+          Rv = Rv - bx;
+        }
+        if (yrow) {
+          keep_branch();
+          Rv = Rv - by;
+        }
+        if (zpl) {
+          keep_branch();
+          Rv = Rv - bz;
+        }
       }
       R dt = a.use_dtg ? dtg : cur.dt;
       if (DTC) {

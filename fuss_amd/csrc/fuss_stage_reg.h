@@ -74,36 +74,9 @@ __device__ __forceinline__ R dpp_quad_from(R src, int e) {
   }
 }
 
-// x - floor(x): for x >= 0 the interpolation fraction x - trunc(x) of a
-// table row (exact: the integer part's subtraction loses nothing)
-__device__ __forceinline__ double frac_of(double x) { return __builtin_amdgcn_fract(x); }
-__device__ __forceinline__ float frac_of(float x) { return __builtin_amdgcn_fractf(x); }
 
-// Table row of T clamped to [Ti1, Ti2 - 1]: the hardware conversion
-// saturates (and maps NaN to 0), so with Ti1 >= 0 this is
-// trunc(fmin(fmax(T, Ti1), Ti2 - 1)) of tab_rowf as an integer.
-// (hi is held in a VGPR: one instruction may read one scalar register)
-__device__ __forceinline__ int med3_i32(int c, int lo, int hi) {
-  int r;
-  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(c), "s"(lo), "v"(hi));
-  return r;
-}
-__device__ __forceinline__ int row_sat(double T, int Ti1, int Ti2m1) {
-  int c;
-  asm("v_cvt_i32_f64 %0, %1" : "=v"(c) : "v"(T));
-  return med3_i32(c, Ti1, Ti2m1);
-}
-__device__ __forceinline__ int row_sat(float T, int Ti1, int Ti2m1) {
-  int c;
-  asm("v_cvt_i32_f32 %0, %1" : "=v"(c) : "v"(T));
-  return med3_i32(c, Ti1, Ti2m1);
-}
-// a uniform value copied to a VGPR once
-__device__ __forceinline__ int to_vgpr(int s) {
-  int v;
-  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
-  return v;
-}
+
+
 
 // Raw buffer accesses: a resource per array (base address, no range
 // limit), a loop-invariant 32-bit lane offset in a VGPR and the plane's byte
@@ -131,10 +104,6 @@ __device__ __forceinline__ void bst(T v, Rsrc r, unsigned voff, unsigned soff) {
   }
 }
 
-template <typename R>
-__device__ __forceinline__ R ldb(const void* base, unsigned off) {
-  return *reinterpret_cast<const R*>(reinterpret_cast<const char*>(base) + off);
-}
 template <typename R>
 __device__ __forceinline__ R ldbn(const void* base, unsigned off) {
   return __builtin_nontemporal_load(reinterpret_cast<const R*>(reinterpret_cast<const char*>(base) + off));
