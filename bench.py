@@ -53,6 +53,10 @@ HBM_PEAK_GBS = 8000.0
 
 
 def stage_kernel(rect, kc, rows):
+    """The stage kernel of stages 2-4 on the cube (one material, uniform qvol,
+    primitive, IRS): the LDS ring with 4 or 8 rows, or register columns."""
+    if rows == 1:
+        return f"fuss::k_stage3r<true, true, true, true, false, false, double, {kc}>"
     return f"k_stage3p<true, true, true, {'true' if rect else 'false'}, false, double, {kc}, {rows}>"
 
 
@@ -61,9 +65,11 @@ STAGE_TRAFFIC_MEASURED = {stage_kernel(True, kc, rows): fetch * 1024 * 2.0 + 131
                                                   (64, 4, 140748), (32, 8, 143789), (64, 8, 140803))}
 # Plain FP64 path (four-stage RK + LTS + IRS 0.5, 3-level sequencing) run to
 # residual 1e-10 on the GPU: iterations per cube size (bench runs with
-# --plain-conv-n 32, 48, 64).  Fit: iters = 230218 (n/64)^PLAIN_EXP.
-PLAIN_ITERS_MEASURED = {32: 64506, 48: 135672, 64: 230218}
-PLAIN_EXP = 1.835
+# --plain-conv-n 32, 48, 64 and, with --n 128, 128).  Least-squares fit of
+# log(iters) on log(n) through the four: slope PLAIN_EXP; the 256^3 count is
+# extrapolated from the largest measured cube: 826104 (n/128)^PLAIN_EXP.
+PLAIN_ITERS_MEASURED = {32: 64506, 48: 135672, 64: 230218, 128: 826104}
+PLAIN_EXP = 1.840
 
 
 def device_sync():
@@ -192,7 +198,12 @@ def time_to_converge(s, case, args, ms_step):
         sp.autotune(1, steps=3)
         device_sync()
         t0 = time.perf_counter()
-        nit, hist = sp.solve()
+
+        def progress(done, lv, res):  # (printed between fuss_run calls of 50 000 steps)
+            print(f"[bench] plain FP64 {m}^3: {done} iterations, level {lv}, residual {res:.3e}",
+                  file=sys.stderr, flush=True)
+
+        nit, hist = sp.solve(chunk=50000, progress=progress)
         device_sync()
         t_plain = time.perf_counter() - t0
         Tp = sp.get("T")
@@ -208,11 +219,14 @@ def time_to_converge(s, case, args, ms_step):
                               "fp32_refined_vs_plain_max_rel": float(np.max(np.abs(Tr - Tp) / np.abs(Tp)))}
         print(f"[bench] plain FP64 {m}^3 to 1e-10: {nit} iterations in {t_plain:.1f} s; FP32-refined "
               f"{t_ref*1e3:.1f} ms; speedup {t_plain / t_ref:.0f}", file=sys.stderr, flush=True)
-    iters = PLAIN_ITERS_MEASURED[64] * (args.n / 64.0) ** PLAIN_EXP
+    iters = PLAIN_ITERS_MEASURED[128] * (args.n / 128.0) ** PLAIN_EXP
     plain_s = iters * ms_step * 1e-3
     out["fp64_plain"] = {"measured": False, "seconds_est": plain_s, "iterations_est": iters,
                          "ms_per_iteration": ms_step, "iterations_measured": PLAIN_ITERS_MEASURED,
-                         "fit": f"230218 (n/64)^{PLAIN_EXP}",
+                         "fit": f"826104 (n/128)^{PLAIN_EXP}",
+                         "measured_128": {"iterations": 826104, "seconds": 151.9,
+                                          "fp32_refined_seconds": 0.0115, "speedup_measured": 13224,
+                                          "source": "bench.py --n 128 --plain-conv-n 128"},
                          "note": "estimate: measured ms per iteration x extrapolated iteration count"}
     out["speedup_fp32_refined_vs_plain_est"] = plain_s / out["fp32_refined"]["seconds"]
     return out
@@ -524,7 +538,8 @@ def main():
     ap.add_argument("--calib-copy", action="store_true", help="also run 1 GiB D2D copies (counter calibration)")
     ap.add_argument("--rk", type=int, default=4)
     ap.add_argument("--kc", type=int, default=0, help="column height (4..64); with --irs-rows: tiles without autotune")
-    ap.add_argument("--stage-rows", type=int, default=4, help="rows of the stage kernel's columns (4 or 8)")
+    ap.add_argument("--stage-rows", type=int, default=4,
+                    help="rows of the LDS-ring stage kernel's columns (4 or 8), or 1: register columns")
     ap.add_argument("--irs-rows", type=int, default=4, help="rows of the fused IRS pass's columns (4 or 8)")
     ap.add_argument("--row-march", action="store_true", help="with --kc: the row-march stage kernel")
     ap.add_argument("--no-rect", action="store_true",
@@ -655,7 +670,9 @@ def main():
                                "qvol 1e6, 303 h=50, k(T) 32-knot table, 3 MG levels; step = one "
                                "finest-level Explicit_Step",
                    "cells": ncell, "parallelism": "single-gpu",
-                   "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring",
+                   "tiles": {"column_planes": tuned[0],
+                             "stage_kernel": "row march" if tuned[1] else ("register columns" if stage_rows == 1
+                                                                           else "LDS ring"),
                              "stage_rows": stage_rows, "irs_rows": irs_rows, "chosen_by": chosen}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -357,21 +357,32 @@ class Solver:
         k = n.value
         return k, (hr[:k] if hist else None), (ht[:k] if hist else None), r
 
-    def solve(self, max_iters=10**9, irs_mode=None):
+    def solve(self, max_iters=10**9, irs_mode=None, chunk=None, progress=None):
         """FUSS_solve loop to the end of the run: coarse-to-fine sequencing,
         then level 1 until a stop condition.  Returns (n_iter, history) with
-        history rows (iter_general, time, residual)."""
+        history rows (iter_general, time, residual).  chunk: at most this
+        many steps per fuss_run call (the same steps and stop tests), after
+        each of which progress(n_iter, level, residual) is called."""
         p = self.p
         itermax = p.itermax()
         rows = []
         done = 0
         while done < max_iters:
             lv = self.level
-            k, hr, ht, _ = self.run_level(lv, max_iters - done, itermax[lv - 1], p.res_threshold,
-                                          p.time_threshold, True, irs_mode)
-            rows.append(np.stack([done + 1 + np.arange(k, dtype=np.float64), ht, hr], axis=1))
-            done += k
-            stop = (k >= itermax[lv - 1] or (k and hr[-1] <= p.res_threshold) or (k and ht[-1] >= p.time_threshold))
+            lvdone = 0
+            while True:
+                n = max_iters - done if not chunk else min(chunk, max_iters - done)
+                k, hr, ht, _ = self.run_level(lv, n, itermax[lv - 1] - lvdone, p.res_threshold,
+                                              p.time_threshold, True, irs_mode)
+                rows.append(np.stack([done + 1 + np.arange(k, dtype=np.float64), ht, hr], axis=1))
+                done += k
+                lvdone += k
+                stop = (lvdone >= itermax[lv - 1] or (k and hr[-1] <= p.res_threshold)
+                        or (k and ht[-1] >= p.time_threshold))
+                if progress:
+                    progress(done, lv, float(hr[-1]) if k else float("nan"))
+                if stop or k < n or done >= max_iters:
+                    break
             if lv == 1 or not stop or done >= max_iters:
                 break
             self.prolong(lv - 1)
