@@ -140,12 +140,13 @@ def cpu_baseline(n, iters):
                       f"{iters} iteration(s) of the {n}^3 cube, RK4 + LTS + IRS 0.5, {dt:.1f} s wall"}
 
 
-def refined_run(s, T0, prec, max_levels, label, max_outer=300):
+def refined_run(s, T0, prec, max_levels, label, max_outer=300, coarse_cg=0):
     """FP64 defect correction with `prec` inner sweeps from T0; wall time of
     the second of two identical runs (the first loads the modules)."""
     from fuss_amd import gpu
 
-    rp = gpu.refine_params(precision=prec, max_outer=max_outer, res_threshold=1e-10, max_levels=max_levels)
+    rp = gpu.refine_params(precision=prec, max_outer=max_outer, res_threshold=1e-10, max_levels=max_levels,
+                           coarse_cg=coarse_cg)
     gpu._chk(s.L.fuss_refine_setup(s.h, gpu.C.byref(rp)), "fuss_refine_setup")
     s.set_T(T0)
     s.refine(rp, setup=False)
@@ -165,13 +166,14 @@ def time_to_converge(s, case, args, ms_step):
     FP64 Explicit_Step) from T = 300 K.
 
     Measured: the FP64 defect correction with FP32 and with FP64 inner
-    sweeps at the benchmark size (correction V-cycle to full depth; a
-    hierarchy cut at 3 levels, the depth of the configuration's plain
-    multigrid, is reported as well: its coarsest 64^3 level, only smoothed,
-    cannot carry the global heat-up mode, and 30 outer iterations leave the
-    residual far above the threshold), and at a smaller cube both the plain FP64 march (the reference's method, 3-level
-    coarse-to-fine sequencing) and the FP32-refined solve, so the speedup
-    there is a ratio of two measurements.  The plain march at the benchmark
+    sweeps at the benchmark size (correction V-cycle to full depth, and a
+    3-level V-cycle -- the configuration's multigrid depth -- whose coarsest
+    64^3 level is solved by 80 conjugate-gradient iterations; the same cut
+    with that level only smoothed is reported too: it cannot carry the global
+    heat-up mode, and 30 outer iterations leave the residual far above the
+    threshold), and at a smaller cube both the plain FP64 march (the
+    reference's method, 3-level coarse-to-fine sequencing) and the
+    FP32-refined solve, so the speedup there is a ratio of two measurements.  The plain march at the benchmark
     size takes ~3e6 iterations; its time there is an estimate (measured time
     per iteration x an iteration count extrapolated from measured runs)."""
     from fuss_amd import cases, gpu
@@ -179,12 +181,15 @@ def time_to_converge(s, case, args, ms_step):
     T0 = case.blocks[0].T
     out = {"threshold": 1e-10, "from": "initial state T = 300 K"}
     finals = {}
-    for prec, name, lv, mo in ((gpu.FUSS_FP32, "fp32_refined", 0, 300), (gpu.FUSS_FP64, "fp64_refined", 0, 300),
-                               (gpu.FUSS_FP32, "fp32_refined_3_levels", 3, 30)):
-        dt, r, hist, T = refined_run(s, T0, prec, lv, name, mo)
+    for prec, name, lv, mo, cg in ((gpu.FUSS_FP32, "fp32_refined", 0, 300, 0),
+                                   (gpu.FUSS_FP64, "fp64_refined", 0, 300, 0),
+                                   (gpu.FUSS_FP32, "fp32_refined_3_levels", 3, 300, 80),
+                                   (gpu.FUSS_FP32, "fp32_refined_3_levels_smoothed", 3, 30, 0)):
+        dt, r, hist, T = refined_run(s, T0, prec, lv, name, mo, cg)
         finals[name] = T
         out[name] = {"seconds": dt, "converged": bool(r.converged), "outer": r.outer, "vcycles": r.vcycles,
-                     "levels": r.levels, "final_residual": r.res, "defect_W": r.defect, "measured": True,
+                     "levels": r.levels, "coarsest": f"{cg} CG iterations" if cg else "direct or smoothed",
+                     "final_residual": r.res, "defect_W": r.defect, "measured": True,
                      "residual_history": [float(x) for x in hist]}
     a, b = finals["fp32_refined"], finals["fp64_refined"]
     out["fp32_vs_fp64_refined_max_rel"] = float(np.max(np.abs(a - b) / np.abs(b)))

@@ -2481,6 +2481,8 @@ void refine_free(Refine& R) {
   R.ready = 0;
 }
 
+inline unsigned cgrid(int n) { return (unsigned)((n + 255) / 256); }
+
 int refine_build(fuss_ctx* c, const fuss_refine_params* rp) {
   Refine& R = c->ref;
   Level& L = c->lev[0];
@@ -2490,8 +2492,8 @@ int refine_build(fuss_ctx* c, const fuss_refine_params* rp) {
   }
   if (rp->precision != FUSS_FP32 && rp->precision != FUSS_FP64) return FUSS_EARG;
   if (rp->n_stage < 1 || rp->n_stage > 8 || rp->nu1 < 0 || rp->nu2 < 0 || rp->vcycles < 1 || rp->max_outer < 0 ||
-      rp->max_levels < 0 || rp->ncoarse < 0 || !(rp->lam_lo > 0.0) || !(rp->lam_hi > rp->lam_lo) ||
-      !(rp->irs_beta >= 0.0))
+      rp->max_levels < 0 || rp->ncoarse < 0 || rp->coarse_cg < 0 || !(rp->lam_lo > 0.0) ||
+      !(rp->lam_hi > rp->lam_lo) || !(rp->irs_beta >= 0.0))
     return FUSS_EARG;
   refine_free(R);
   const size_t es = rp->precision == FUSS_FP32 ? 4 : 8;
@@ -2536,6 +2538,15 @@ int refine_build(fuss_ctx* c, const fuss_refine_params* rp) {
     HIPCHK(hipMemcpy(cl.blk.p, cl.hblk.data(), cl.hblk.size() * sizeof(CBlk), hipMemcpyHostToDevice));
     cl.cur = 0;
   }
+  {
+    // conjugate gradients on the coarsest level when it is not solved directly
+    CLevel& cl = R.lv.back();
+    if (rp->coarse_cg > 0 && cl.maxcells > CMG_DIRECT) {
+      const size_t one = ((size_t)cl.ncell * es + 255) / 256 * 256;
+      if (cl.cgmem.alloc(6 * one) || cl.cgpart.alloc(3 * (size_t)cgrid(cl.ncell) + 2)) return FUSS_EHIP;
+      for (int q = 0; q < 6; q++) cl.cg[q] = cl.cgmem.p + q * one;
+    }
+  }
   if (R.bcd.alloc(std::max(L.nbound, 1)) || R.red.alloc(2) || R.bres.alloc(nb) ||
       hipHostMalloc((void**)&R.hbres, nb * sizeof(double), 0) != hipSuccess)
     return FUSS_EHIP;
@@ -2558,7 +2569,6 @@ COp<S> cop(const CLevel& l) {
   return o;
 }
 
-inline unsigned cgrid(int n) { return (unsigned)((n + 255) / 256); }
 
 // nu iterations of the multistage smoother on one correction level.
 template <typename S>
@@ -2603,6 +2613,32 @@ int cmg_smooth(fuss_ctx* c, CLevel& l, int nu, const std::vector<double>& al, do
   return 0;
 }
 
+// iters conjugate-gradient iterations on a coarsest level from its current
+// correction (Jacobi preconditioner; k_cg_*)
+template <typename S>
+int cmg_cg(fuss_ctx* c, CLevel& l, int iters) {
+  const COp<S> o = cop<S>(l);
+  const unsigned gs = cgrid(l.ncell);
+  S* x = (S*)l.d[l.cur];
+  S *r = (S*)l.cg[0], *z = (S*)l.cg[1], *p0 = (S*)l.cg[2], *p1 = (S*)l.cg[3], *q = (S*)l.cg[4];
+  S* invd = (S*)l.cg[5];
+  // partials: r . z (two, alternating by iteration), p . q; the last two r . z
+  double* prz[2] = {l.cgpart.p, l.cgpart.p + gs};
+  double* ppq = l.cgpart.p + 2 * gs;
+  double* rzs = l.cgpart.p + 3 * gs;
+  k_cg_start<S><<<gs, 256, 0, c->stream>>>(o, x, r, z, p0, invd, prz[0]);
+  HIPCHK(hipGetLastError());
+  for (int it = 0; it < iters; it++) {
+    // p' = z + beta p alternates between the two direction buffers
+    const S* pold = it % 2 == 0 ? p0 : p1;
+    S* pnew = it % 2 == 0 ? p1 : p0;
+    k_cg_ap<S><<<gs, 256, 0, c->stream>>>(o, it, z, pold, pnew, q, prz[it % 2], ppq, rzs);
+    k_cg_update<S><<<gs, 256, 0, c->stream>>>(l.ncell, x, pnew, r, q, invd, z, prz[it % 2], ppq, prz[(it + 1) % 2]);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
 template <typename S>
 int cmg_vcycle(fuss_ctx* c, int l, const std::vector<double>& al) {
   Refine& R = c->ref;
@@ -2615,6 +2651,7 @@ int cmg_vcycle(fuss_ctx* c, int l, const std::vector<double>& al) {
       HIPCHK(hipGetLastError());
       return 0;
     }
+    if (R.p.coarse_cg > 0) return cmg_cg<S>(c, L, R.p.coarse_cg);
     return cmg_smooth<S>(c, L, R.p.ncoarse, al, R.p.irs_beta);
   }
   int e = cmg_smooth<S>(c, L, R.p.nu1, al, R.p.irs_beta);

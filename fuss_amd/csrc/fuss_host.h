@@ -167,6 +167,12 @@ struct CLevel {
   void* d[3] = {nullptr, nullptr, nullptr};
   void *b = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr, *D = nullptr, *s = nullptr, *s2 = nullptr;
   int cur = 0;
+  // conjugate gradients on a coarsest level (refine coarse_cg > 0): r, z,
+  // two direction buffers, A p, 1/diag; per-group partial sums (two of
+  // r . z, one of p . q) and the last two r . z
+  DBuf<unsigned char> cgmem;
+  void* cg[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  DBuf<double> cgpart;
 };
 
 struct Refine {

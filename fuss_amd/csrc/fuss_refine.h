@@ -274,6 +274,133 @@ __global__ void k_cmg_direct(COp<S> o, S* __restrict__ d) {
   for (int r = 0; r < n; r++) d[B.off + r] = (S)x[r];
 }
 
+// ---------------------------------------------------------------------------
+// Jacobi-preconditioned conjugate gradients on a coarsest level too large for
+// the direct solve (a hierarchy cut at a few levels).  A is symmetric (each
+// face conductance enters both of its cells) and positive definite wherever
+// a BC diagonal reaches the block.  Vectors in S, inner products and the
+// step scalars in FP64.  Each iteration is two kernels; each writes one
+// partial inner product per work-group, and every work-group of the next
+// kernel adds all of them in the same fixed order (no atomics, no device
+// fences, the same bits on every run and in every group).
+// ---------------------------------------------------------------------------
+// Sum of v over the 256 threads of the work-group (valid in every thread).
+__device__ __forceinline__ double cg_group_sum(double v, double* sm) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (sm[0] + sm[1]) + (sm[2] + sm[3]);
+}
+// Sum of the n partials of the previous kernel, in a fixed order.
+__device__ __forceinline__ double cg_all_sum(const double* __restrict__ part, int n, double* sm) {
+  double t = 0.0;
+  for (int q = threadIdx.x; q < n; q += 256) t += part[q];
+  return cg_group_sum(t, sm);
+}
+
+// (A v)_g with v(g') = z(g') + beta p(g'); *vc = v(g)
+template <typename S>
+__device__ __forceinline__ S cg_Ap(const COp<S>& o, const S* __restrict__ z, const S* __restrict__ p, S beta,
+                                   const CBlk& B, int g, int i, int j, int k, S* vc) {
+  const int sy = B.n[0], sz = B.n[0] * B.n[1];
+  auto v = [&](int q) -> S { return z[q] + beta * p[q]; };
+  const S c0 = v(g);
+  *vc = c0;
+  S a = o.D[g] * c0;
+  if (i > 0) a += o.cx[g] * (c0 - v(g - 1));
+  if (i < B.n[0] - 1) a += o.cx[g + 1] * (c0 - v(g + 1));
+  if (j > 0) a += o.cy[g] * (c0 - v(g - sy));
+  if (j < B.n[1] - 1) a += o.cy[g + sy] * (c0 - v(g + sy));
+  if (k > 0) a += o.cz[g] * (c0 - v(g - sz));
+  if (k < B.n[2] - 1) a += o.cz[g + sz] * (c0 - v(g + sz));
+  return a;
+}
+
+// r = b - A x, invd = 1/diag(A), z = invd r, p = z; partials of r . z
+template <typename S>
+__global__ __launch_bounds__(256) void k_cg_start(COp<S> o, const S* __restrict__ x, S* __restrict__ r,
+                                                  S* __restrict__ z, S* __restrict__ p, S* __restrict__ invd,
+                                                  double* __restrict__ prz) {
+  __shared__ double sm[4];
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (g < o.ncell) {
+    const int bi = cmg_find_block(o.blk, o.nb, g);
+    const CBlk B = o.blk[bi];
+    const CCell c = cmg_cell(B, bi, g);
+    S dg;
+    const S ax = cmg_Ad(o, x, B, g, c.i, c.j, c.k, &dg);
+    const S id = dg > (S)0 ? (S)1 / dg : (S)0;
+    const S rr = o.b[g] - ax;
+    const S zz = id * rr;
+    r[g] = rr;
+    z[g] = zz;
+    p[g] = zz;
+    invd[g] = id;
+    v = (double)rr * (double)zz;
+  }
+  v = cg_group_sum(v, sm);
+  if (threadIdx.x == 0) prz[blockIdx.x] = v;
+}
+
+// Iteration it: rz = r . z (from prz), beta = rz / rz of the previous
+// iteration (rzs[(it+1)%2]; 0 on the first), p' = z + beta p, q = A p';
+// partials of p' . q; work-group 0 keeps rz in rzs[it%2].
+template <typename S>
+__global__ __launch_bounds__(256) void k_cg_ap(COp<S> o, int it, const S* __restrict__ z, const S* __restrict__ p,
+                                               S* __restrict__ pn, S* __restrict__ q,
+                                               const double* __restrict__ prz, double* __restrict__ ppq,
+                                               double* __restrict__ rzs) {
+  __shared__ double sm[4];
+  const double rz = cg_all_sum(prz, (int)gridDim.x, sm);
+  const double rzp = rzs[(it + 1) % 2];
+  const S beta = it > 0 && rzp > 0.0 ? (S)(rz / rzp) : (S)0;
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (g < o.ncell) {
+    const int bi = cmg_find_block(o.blk, o.nb, g);
+    const CBlk B = o.blk[bi];
+    const CCell c = cmg_cell(B, bi, g);
+    S pc;
+    const S a = cg_Ap(o, z, p, beta, B, g, c.i, c.j, c.k, &pc);
+    pn[g] = pc;
+    q[g] = a;
+    v = (double)pc * (double)a;
+  }
+  v = cg_group_sum(v, sm);
+  if (threadIdx.x == 0) {
+    ppq[blockIdx.x] = v;
+    if (blockIdx.x == 0) rzs[it % 2] = rz;
+  }
+}
+
+// alpha = rz / (p' . q); x += alpha p', r -= alpha q, z = invd r; partials
+// of the new r . z into przn
+template <typename S>
+__global__ __launch_bounds__(256) void k_cg_update(int n, S* __restrict__ x, const S* __restrict__ p,
+                                                   S* __restrict__ r, const S* __restrict__ q,
+                                                   const S* __restrict__ invd, S* __restrict__ z,
+                                                   const double* __restrict__ prz, const double* __restrict__ ppq,
+                                                   double* __restrict__ przn) {
+  __shared__ double sm[4];
+  const double rz = cg_all_sum(prz, (int)gridDim.x, sm);
+  const double pq = cg_all_sum(ppq, (int)gridDim.x, sm);
+  const S alpha = pq > 0.0 ? (S)(rz / pq) : (S)0;
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (g < n) {
+    x[g] = x[g] + alpha * p[g];
+    const S rr = r[g] - alpha * q[g];
+    const S zz = invd[g] * rr;
+    r[g] = rr;
+    z[g] = zz;
+    v = (double)rr * (double)zz;
+  }
+  v = cg_group_sum(v, sm);
+  if (threadIdx.x == 0) przn[blockIdx.x] = v;
+}
+
 template <typename S>
 __global__ __launch_bounds__(256) void k_cmg_fill(S* __restrict__ p, int n, S v) {
   const int g = blockIdx.x * 256 + threadIdx.x;

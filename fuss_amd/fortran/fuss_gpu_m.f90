@@ -49,6 +49,7 @@ module fuss_gpu_m
     integer(c_int) :: max_outer = 200
     real(c_double) :: res_threshold = 1d-10
     integer(c_int) :: per_block = 0
+    integer(c_int) :: coarse_cg = 0
   end type
 
   !> outcome of fuss_refine (fuss_refine_result)

@@ -53,7 +53,7 @@ class RefineParams(C.Structure):
     _fields_ = [("precision", C.c_int), ("n_stage", C.c_int), ("lam_lo", C.c_double), ("lam_hi", C.c_double),
                 ("irs_beta", C.c_double), ("nu1", C.c_int), ("nu2", C.c_int), ("vcycles", C.c_int),
                 ("max_levels", C.c_int), ("ncoarse", C.c_int), ("max_outer", C.c_int), ("res_threshold", C.c_double),
-                ("per_block", C.c_int)]
+                ("per_block", C.c_int), ("coarse_cg", C.c_int)]
 
 
 class RefineResult(C.Structure):
@@ -62,12 +62,14 @@ class RefineResult(C.Structure):
 
 
 def refine_params(precision=FUSS_FP32, n_stage=4, lam_lo=0.3, lam_hi=2.0, irs_beta=0.0, nu1=1, nu2=1, vcycles=2,
-                  max_levels=0, ncoarse=20, max_outer=200, res_threshold=1e-10, per_block=False):
+                  max_levels=0, ncoarse=20, max_outer=200, res_threshold=1e-10, per_block=False, coarse_cg=0):
     """Defaults: FP32 sweeps, four-stage Chebyshev smoother on [0.3, 2], one
     pre- and one post-smoothing iteration, two V-cycles per outer iteration,
-    coarsening down to <= 2 cells per direction."""
+    coarsening down to <= 2 cells per direction.  coarse_cg: conjugate-
+    gradient iterations on a coarsest level larger than the direct solve
+    (with max_levels > 0), 0: ncoarse smoothing iterations."""
     return RefineParams(precision, n_stage, lam_lo, lam_hi, irs_beta, nu1, nu2, vcycles, max_levels, ncoarse,
-                        max_outer, res_threshold, int(per_block))
+                        max_outer, res_threshold, int(per_block), int(coarse_cg))
 
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

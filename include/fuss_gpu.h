@@ -249,6 +249,9 @@ typedef struct {
   double res_threshold;/* stop when the monitor residual sqrt(res_sq) <= this */
   int per_block;       /* nonzero: blocks are independent cases, each must meet res_threshold
                           with its own residual; 0: one residual over all blocks */
+  int coarse_cg;       /* > 0: this many Jacobi-preconditioned conjugate-gradient iterations
+                          solve a coarsest level too large for the direct solve (instead of
+                          ncoarse smoothing iterations) */
 } fuss_refine_params;
 
 typedef struct {

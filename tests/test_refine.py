@@ -152,3 +152,24 @@ def test_connections_rejected(gpu):
         s.refine(gpu.refine_params())
     assert ei.value.code == 5  # FUSS_EUNSUPPORTED
     s.close()
+
+
+@pytest.mark.parametrize("precision", ["FUSS_FP32", "FUSS_FP64"])
+def test_three_levels_cg_coarse(gpu, precision):
+    """A 3-level V-cycle whose coarsest level (8^3 of a 32^3 cube) is solved
+    by conjugate gradients converges to the full hierarchy's fixed point in
+    about as many outer iterations."""
+    n = 32
+    Tf, rf, _ = refined(gpu, cases.cube3d(n=n, mgl=1), getattr(gpu, precision))
+    Tc, rc, hc = refined(gpu, cases.cube3d(n=n, mgl=1), getattr(gpu, precision), max_levels=3, coarse_cg=40)
+    assert rf.converged and rc.converged and rc.levels == 3
+    assert rc.outer <= rf.outer + 3 and hc[-1] == rc.res
+    for a, b in zip(Tc, Tf):
+        assert relerr(a, b) < 1e-11
+
+
+def test_cg_coarse_rejects_negative(gpu):
+    s = gpu.Solver(cases.cube3d(n=8, mgl=1))
+    with pytest.raises(gpu.FussError):
+        s.refine(gpu.refine_params(max_levels=2, coarse_cg=-1))
+    s.close()
