@@ -652,6 +652,12 @@ __device__ __forceinline__ int med3_i32(int c, int lo, int hi) {
   asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(c), "s"(lo), "v"(hi));
   return r;
 }
+// median of (c, 0, hi) for a uniform hi (no VGPR for the bound)
+__device__ __forceinline__ int med3_i32_0s(int c, int hi) {
+  int r;
+  asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(c), "s"(hi));
+  return r;
+}
 __device__ __forceinline__ int row_sat(double T, int Ti1, int Ti2m1) {
   int c;
   asm("v_cvt_i32_f64 %0, %1" : "=v"(c) : "v"(T));

@@ -2,6 +2,8 @@
 // the FUSS explicit step on gfx950.  Included by the stage launch units.
 #pragma once
 
+#include <type_traits>
+
 #include "fuss_dev.h"
 
 template <int NDIR, bool TANG, bool PRIM, bool IRSMODE, typename R>
@@ -453,272 +455,354 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
     cwh = yhalo_in ? (R)(rx_.fy[t.j0 - 1] * rx_.ax[i])
                    : (xhalo_in ? (R)(rx_.fx[t.i0 - 1] * rx_.ay[t.j0 + hr]) : (R)0);
   }
-  int sm = 0, sc = 1, sp = 2;
-  PlaneRegs<R> pre;
-  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0 - 1);
-  store_plane<TY>(ring[sm], pre);
-  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0);
-  store_plane<TY>(ring[sc], pre);
-  fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0 + 1);
-  const unsigned sz = (unsigned)B.sz;
-  // addresses of plane k0: the cell (cell (1,1) for out-of-block lanes) and the
-  // halo face's left cell (the cell itself for other lanes)
-  const int ci = live ? i : 1, cj = live ? j : 1;
-  unsigned p = (unsigned)(B.base + (ci - 1) + (long)(cj - 1) * B.sy + (long)(k0 - 1) * B.sz);
-  // (where the x-halo face is a block face, the row's own first cell: the
-  // material read for it is then valid)
-  unsigned ph = yhalo ? (t.j0 > 1 ? p - (unsigned)B.sy : p)
-                      : (xhalo ? (unsigned)(B.base + max(t.i0 - 2, 0) + (long)(t.j0 + hr - 1) * B.sy +
-                                            (long)(k0 - 1) * B.sz)
-                               : p);
-  // face-map positions of the lane's boundary faces (BC values in face
-  // order; at most one face per axis since every block dimension is >= 2)
-  const int fa = live ? B.face_active : 0;
-  const int xo = (fa & 1) && i == 1 ? B.fmap_off[0] + (j - 1) : ((fa & 2) && i == B.ni ? B.fmap_off[1] + (j - 1) : -1);
-  const int yo = (fa & 4) && j == 1 ? B.fmap_off[2] + (i - 1) : ((fa & 8) && j == B.nj ? B.fmap_off[3] + (i - 1) : -1);
-  const int zlo = (fa & 16) ? B.fmap_off[4] + (i - 1) + B.ni * (j - 1) : -1;
-  const int zhi = (fa & 32) ? B.fmap_off[5] + (i - 1) + B.ni * (j - 1) : -1;
-  auto zo_of = [&](int kk) { return kk == 1 ? zlo : (kk == B.nk ? zhi : -1); };
-  // which loads of boundary values a tile / row needs at all
-  const int fa_t = B.face_active;
-  const bool xtile = (fa_t & 3) && (t.i0 == 1 || t.i0 + TILE_X - 1 >= B.ni);
-  const bool yrow = (fa_t & 12) && (j == 1 || j == B.nj);
-  // byte offsets of the lane's BC values in bcf (x and y: base + step per
-  // plane); lanes without an entry read the entry that stays 0, so a tile,
-  // row or plane with boundary faces subtracts its values unconditionally
-  constexpr unsigned SR = sizeof(R);
-  const unsigned xb8 = (xo >= 0 ? (unsigned)xo : a.bcz) * SR, xs8 = xo >= 0 ? (unsigned)B.nj * SR : 0u;
-  const unsigned yb8 = (yo >= 0 ? (unsigned)yo : a.bcz) * SR, ys8 = yo >= 0 ? (unsigned)B.ni * SR : 0u;
-  const unsigned zl8 = (zlo >= 0 ? (unsigned)zlo : a.bcz) * SR, zh8 = (zhi >= 0 ? (unsigned)zhi : a.bcz) * SR;
-  const R dtg = a.use_dtg ? (R)(*a.dtg) : (R)0;
-  // a block of uniform qvol: every lane reads the value of cell (1,1,1) (one
-  // cached line instead of a stream)
-  const unsigned quni = (unsigned)B.base;
-  auto stream = [&](StageStream<R>& s, unsigned pp, unsigned hh) {
-    s.qv = ldu(a.qvol, B.uqv ? quni : pp);
-    if (!DTC) s.dt = ldn(a.dt, pp);
-    if (!IRSMODE) s.TO = ldn(a.TO, pp);
-    if (!RECT) {
-      s.vol = ldn(a.vol, pp);
-      s.w0 = ldn(a.W[0], pp);
-      s.w4 = ldn(a.W[4], pp);
-      s.w8 = ldn(a.W[8], pp);
-      s.wh = ldu(Wh, hh);
-    }
-    if (!UM) {
-      s.m = ldu(a.mat, pp);
-      s.mh = ldu(a.mat, hh);
-    }
-  };
-  StageStream<R> sa_, sb_;
-  sa_.m = sa_.mh = sb_.m = sb_.mh = mu;
-  sa_.TO = sb_.TO = (R)0;
-  if (DTC) sa_.dt = sb_.dt = (R)0;
-  if (RECT) sa_.vol = sb_.vol = sa_.w0 = sb_.w0 = sa_.w4 = sb_.w4 = sa_.w8 = sb_.w8 = sa_.wh = sb_.wh = (R)0;
-  stream(sa_, p, ph);
-  __syncthreads();
-  R fz_prev = 0;
-  if (live && k0 > 1) {
-#pragma clang fp contract(off)
-    fz_prev = flux_z<false>(a, B, ring[sm], ring[sc], x, y, p - sz,
-                            RECT ? (R)(zfac[1][0] * (double)cvol) : ldu(a.W[8], p - sz));
-  }
-  double rsq = 0.0;
-  // The result of a plane is stored during the next plane, after that
-  // plane's loads are issued: stores count in the same in-order counter as
-  // loads, so a store issued just before the table loads would make the
-  // wait for the tables wait for the store's completion too.
-  R* const out = IRSMODE ? a.Rres : a.Tout;
-  R vpend = 0, dtpend = 0;
-  unsigned ppend = p;
-  bool spend = false;
-  double dtmin = 1e300;
-  // one plane: cur holds its streams, nxt receives those of the next plane
-  auto plane = [&](StageStream<R>& cur, StageStream<R>& nxt, int k) {
-    store_plane<TY>(ring[sp], pre);
+  // Steady primitive stages with IRS on levels of one material per block
+  // (W): the table rows of the column's temperatures are staged in LDS, a
+  // window of TWR rows around the first plane's rows, and every lookup reads
+  // LDS instead of memory (no table load in the in-order counter, so the
+  // waits for the rows no longer wait for the next plane's streams).  A
+  // lookup of a face or cell outside the window counts as a miss; a
+  // work-group with a miss computes its column again from memory (its
+  // stores are overwritten; the same operations give the same bits).
+  constexpr bool WIN = UM && PRIM && IRSMODE;
+  constexpr int TWR = 128;
+  __shared__ V2 wkap[WIN ? TWR : 1];
+  __shared__ V4 wpr[WIN ? TWR : 1];
+  __shared__ int wred[2][TY];
+  double rsq = 0.0, dtmin = 1e300;
+  auto column = [&](auto wtag) -> bool {
+    constexpr bool W = decltype(wtag)::value;
+    int sm = 0, sc = 1, sp = 2;
+    PlaneRegs<R> pre;
+    fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0 - 1);
+    store_plane<TY>(ring[sm], pre);
+    fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0);
+    store_plane<TY>(ring[sc], pre);
+    fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k0 + 1);
+    const unsigned sz = (unsigned)B.sz;
+    // addresses of plane k0: the cell (cell (1,1) for out-of-block lanes) and the
+    // halo face's left cell (the cell itself for other lanes)
+    const int ci = live ? i : 1, cj = live ? j : 1;
+    unsigned p = (unsigned)(B.base + (ci - 1) + (long)(cj - 1) * B.sy + (long)(k0 - 1) * B.sz);
+    // (where the x-halo face is a block face, the row's own first cell: the
+    // material read for it is then valid)
+    unsigned ph = yhalo ? (t.j0 > 1 ? p - (unsigned)B.sy : p)
+                        : (xhalo ? (unsigned)(B.base + max(t.i0 - 2, 0) + (long)(t.j0 + hr - 1) * B.sy +
+                                              (long)(k0 - 1) * B.sz)
+                                 : p);
+    // face-map positions of the lane's boundary faces (BC values in face
+    // order; at most one face per axis since every block dimension is >= 2)
+    const int fa = live ? B.face_active : 0;
+    const int xo = (fa & 1) && i == 1 ? B.fmap_off[0] + (j - 1) : ((fa & 2) && i == B.ni ? B.fmap_off[1] + (j - 1) : -1);
+    const int yo = (fa & 4) && j == 1 ? B.fmap_off[2] + (i - 1) : ((fa & 8) && j == B.nj ? B.fmap_off[3] + (i - 1) : -1);
+    const int zlo = (fa & 16) ? B.fmap_off[4] + (i - 1) + B.ni * (j - 1) : -1;
+    const int zhi = (fa & 32) ? B.fmap_off[5] + (i - 1) + B.ni * (j - 1) : -1;
+    auto zo_of = [&](int kk) { return kk == 1 ? zlo : (kk == B.nk ? zhi : -1); };
+    // which loads of boundary values a tile / row needs at all
+    const int fa_t = B.face_active;
+    const bool xtile = (fa_t & 3) && (t.i0 == 1 || t.i0 + TILE_X - 1 >= B.ni);
+    const bool yrow = (fa_t & 12) && (j == 1 || j == B.nj);
+    // byte offsets of the lane's BC values in bcf (x and y: base + step per
+    // plane); lanes without an entry read the entry that stays 0, so a tile,
+    // row or plane with boundary faces subtracts its values unconditionally
+    constexpr unsigned SR = sizeof(R);
+    const unsigned xb8 = (xo >= 0 ? (unsigned)xo : a.bcz) * SR, xs8 = xo >= 0 ? (unsigned)B.nj * SR : 0u;
+    const unsigned yb8 = (yo >= 0 ? (unsigned)yo : a.bcz) * SR, ys8 = yo >= 0 ? (unsigned)B.ni * SR : 0u;
+    const unsigned zl8 = (zlo >= 0 ? (unsigned)zlo : a.bcz) * SR, zh8 = (zhi >= 0 ? (unsigned)zhi : a.bcz) * SR;
+    const R dtg = a.use_dtg ? (R)(*a.dtg) : (R)0;
+    // a block of uniform qvol: every lane reads the value of cell (1,1,1) (one
+    // cached line instead of a stream)
+    const unsigned quni = (unsigned)B.base;
+    auto stream = [&](StageStream<R>& s, unsigned pp, unsigned hh) {
+      s.qv = ldu(a.qvol, B.uqv ? quni : pp);
+      if (!DTC) s.dt = ldn(a.dt, pp);
+      if (!IRSMODE) s.TO = ldn(a.TO, pp);
+      if (!RECT) {
+        s.vol = ldn(a.vol, pp);
+        s.w0 = ldn(a.W[0], pp);
+        s.w4 = ldn(a.W[4], pp);
+        s.w8 = ldn(a.W[8], pp);
+        s.wh = ldu(Wh, hh);
+      }
+      if (!UM) {
+        s.m = ldu(a.mat, pp);
+        s.mh = ldu(a.mat, hh);
+      }
+    };
+    StageStream<R> sa_, sb_;
+    sa_.m = sa_.mh = sb_.m = sb_.mh = mu;
+    sa_.TO = sb_.TO = (R)0;
+    if (DTC) sa_.dt = sb_.dt = (R)0;
+    if (RECT) sa_.vol = sb_.vol = sa_.w0 = sb_.w0 = sa_.w4 = sb_.w4 = sa_.w8 = sb_.w8 = sa_.wh = sb_.wh = (R)0;
+    stream(sa_, p, ph);
     __syncthreads();
-    // phase A: table rows of the plane's faces and of the cell (loads always
-    // issued, at a clamped row; errors counted for existing faces only)
-    int bad = 0, badc = 0;
-    const R Tc = ring[sc][y][x];
-    const R Txr = ring[sc][y][x + 1], Tyr = ring[sc][y + 1][x], Tzr = ring[sp][y][x];
-    // face averages, then the fractions of their table rows (T - row, the
-    // row taken as a floating value: no conversion back from the integer)
-    R Tx, Ty, Tz, Th = 0, Tcl = Tc, fc = 0;
-    V2 kx, ky, kz, kh{}, kc{};
-    V4 pc{};
-    // BC values of the lane's boundary faces, issued with the table rows;
-    // only tiles on an i face, rows on a j face and the end planes load them
-    // (each read only under the test that loads it)
-    R bx, by, bz;
-    // (a lane's z face has an entry only on an end plane of a block with z entries)
-    const bool zpl = ((fa_t & 16) && k == 1) || ((fa_t & 32) && k == B.nk);
-    if (xtile) bx = ldb<R>(a.bcf, xb8 + xs8 * (unsigned)(k - 1));
-    if (yrow) by = ldb<R>(a.bcf, yb8 + ys8 * (unsigned)(k - 1));
-    if (zpl) bz = ldb<R>(a.bcf, k == 1 ? zl8 : zh8);
-    {
-#pragma clang fp contract(off)
-      // rows addressed by 32-bit offsets from the material's rows; the
-      // fraction T - row is x - floor(x) (the same for T in the table; outside
-      // it an error is flagged)
-      const V2* kr = UM ? kap_u : kap0 + (long)cur.m * nT;
-      int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
-      Tx = (R)0.5 * (Tc + Txr);
-      kx = ldu(kr, (unsigned)row_sat(Tx, Ti1, Ti2m1));
-      if (!PRIM) b0 = kin(Tx);
-      Tx = frac_of(Tx);
-      Ty = (R)0.5 * (Tc + Tyr);
-      ky = ldu(kr, (unsigned)row_sat(Ty, Ti1, Ti2m1));
-      if (!PRIM) b1 = kin(Ty);
-      Ty = frac_of(Ty);
-      Tz = (R)0.5 * (Tc + Tzr);
-      kz = ldu(kr, (unsigned)row_sat(Tz, Ti1, Ti2m1));
-      if (!PRIM) b2 = kin(Tz);
-      Tz = frac_of(Tz);
-      if (PRIM) {
-        const int rc = row_sat(Tcl, Ti1, Ti2m1);
-        pc = ldu(UM ? pr_u : pr0 + (long)cur.m * nT, (unsigned)rc);
-        if (DTC) kc = ldu(kr, (unsigned)rc);  // kappa at the cell for its dt
-        badc = kin(Tcl);
-        fc = frac_of(Tcl);
+    // the window: TWR rows centred on the rows of plane k0's cells (every
+    // work-group of a level of one material per block reads its block's rows)
+    // (lanes with a miss, or-ed over the planes: a uniform mask)
+    int wlo = Ti1;
+    unsigned long long miss = 0;
+    if constexpr (W) {
+      const int rq = row_sat(ring[sc][y][x], Ti1, Ti2m1);
+      int rlo = live ? rq : 0x7fffffff, rhi = live ? rq : -0x7fffffff;
+      for (int o = 32; o > 0; o >>= 1) {
+        rlo = min(rlo, __shfl_xor(rlo, o, 64));
+        rhi = max(rhi, __shfl_xor(rhi, o, 64));
       }
-      if (ty < 2) {  // waves 0 and 1 hold the halo lanes (a wave-uniform test)
-        const V2* krh = UM ? kap_u : kap0 + (long)cur.mh * nT;
-        Th = yhalo ? (R)0.5 * (ring[sc][0][x] + ring[sc][1][x]) : (R)0.5 * (ring[sc][hq][0] + ring[sc][hq][1]);
-        kh = ldu(krh, (unsigned)row_sat(Th, Ti1, Ti2m1));
-        if (!PRIM) b3 = kin(Th);
-        Th = frac_of(Th);
+      if (tx == 0) {
+        wred[0][ty] = rlo;
+        wred[1][ty] = rhi;
       }
-      // primitive variables: every cell's own row is checked (badc), and a face
-      // average lies between its two cells' temperatures, so a face outside
-      // the table implies a cell outside it -- the face tests are redundant
-      bad = PRIM ? 0 : (live && ((hx && b0) || (hy && b1) || (k < B.nk && b2))) || ((yhalo_in || xhalo_in) && b3);
-      badc = live && badc;
+      __syncthreads();
+      int lo_ = wred[0][0], hi_ = wred[1][0];
+      for (int q = 1; q < TY; q++) {
+        lo_ = min(lo_, wred[0][q]);
+        hi_ = max(hi_, wred[1][q]);
+      }
+      // (a plane spanning more than half the window: the memory path)
+      if (hi_ - lo_ >= TWR / 2) return false;
+      wlo = __builtin_amdgcn_readfirstlane(max(Ti1, min((lo_ + hi_) / 2 - TWR / 2, Ti2 - TWR)));
+      for (int q = threadIdx.x; q < TWR; q += TT) {
+        wkap[q] = kap_u[wlo + q];
+        wpr[q] = pr_u[wlo + q];
+      }
+      __syncthreads();
     }
-    // phase B: the next plane's loads (the last plane re-reads its own cells)
-    {
-      const unsigned step = k < k1 ? sz : 0u;
-      fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k < k1 ? k + 2 : k + 1);
-      stream(nxt, p + step, ph + step);
+    const int wtop = __builtin_amdgcn_readfirstlane(TWR - 1);
+    // a row clamped to the table, then its place in the window clamped to
+    // the window; ms: the two differ
+    auto wrow = [&](R Tv, int& ms) {
+      const int r = row_sat(Tv, Ti1, Ti2m1) - wlo;
+      const int rw = med3_i32_0s(r, wtop);
+      ms = r != rw;
+      return rw;
+    };
+    R fz_prev = 0;
+    if (live && k0 > 1) {
+  #pragma clang fp contract(off)
+      fz_prev = flux_z<false>(a, B, ring[sm], ring[sc], x, y, p - sz,
+                              RECT ? (R)(zfac[1][0] * (double)cvol) : ldu(a.W[8], p - sz));
     }
-    if (spend) {  // the previous plane's result
+    // The result of a plane is stored during the next plane, after that
+    // plane's loads are issued: stores count in the same in-order counter as
+    // loads, so a store issued just before the table loads would make the
+    // wait for the tables wait for the store's completion too.
+    R* const out = IRSMODE ? a.Rres : a.Tout;
+    R vpend = 0, dtpend = 0;
+    unsigned ppend = p;
+    bool spend = false;
+    // one plane: cur holds its streams, nxt receives those of the next plane
+    auto plane = [&](StageStream<R>& cur, StageStream<R>& nxt, int k) {
+      store_plane<TY>(ring[sp], pre);
+      __syncthreads();
+      // phase A: table rows of the plane's faces and of the cell (loads always
+      // issued, at a clamped row; errors counted for existing faces only)
+      int bad = 0, badc = 0;
+      const R Tc = ring[sc][y][x];
+      const R Txr = ring[sc][y][x + 1], Tyr = ring[sc][y + 1][x], Tzr = ring[sp][y][x];
+      // face averages, then the fractions of their table rows (T - row, the
+      // row taken as a floating value: no conversion back from the integer)
+      R Tx, Ty, Tz, Th = 0, Tcl = Tc, fc = 0;
+      V2 kx, ky, kz, kh{}, kc{};
+      V4 pc{};
+      // BC values of the lane's boundary faces, issued with the table rows;
+      // only tiles on an i face, rows on a j face and the end planes load them
+      // (each read only under the test that loads it)
+      R bx, by, bz;
+      // (a lane's z face has an entry only on an end plane of a block with z entries)
+      const bool zpl = ((fa_t & 16) && k == 1) || ((fa_t & 32) && k == B.nk);
+      if (xtile) bx = ldb<R>(a.bcf, xb8 + xs8 * (unsigned)(k - 1));
+      if (yrow) by = ldb<R>(a.bcf, yb8 + ys8 * (unsigned)(k - 1));
+      if (zpl) bz = ldb<R>(a.bcf, k == 1 ? zl8 : zh8);
+      {
+  #pragma clang fp contract(off)
+        // rows addressed by 32-bit offsets from the material's rows; the
+        // fraction T - row is x - floor(x) (the same for T in the table; outside
+        // it an error is flagged)
+        const V2* kr = UM ? kap_u : kap0 + (long)cur.m * nT;
+        int b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+        int mx = 0, my = 0, mz = 0, mc = 0, mh = 0;
+        Tx = (R)0.5 * (Tc + Txr);
+        if constexpr (W) kx = wkap[wrow(Tx, mx)];
+        else kx = ldu(kr, (unsigned)row_sat(Tx, Ti1, Ti2m1));
+        if (!PRIM) b0 = kin(Tx);
+        Tx = frac_of(Tx);
+        Ty = (R)0.5 * (Tc + Tyr);
+        if constexpr (W) ky = wkap[wrow(Ty, my)];
+        else ky = ldu(kr, (unsigned)row_sat(Ty, Ti1, Ti2m1));
+        if (!PRIM) b1 = kin(Ty);
+        Ty = frac_of(Ty);
+        Tz = (R)0.5 * (Tc + Tzr);
+        if constexpr (W) kz = wkap[wrow(Tz, mz)];
+        else kz = ldu(kr, (unsigned)row_sat(Tz, Ti1, Ti2m1));
+        if (!PRIM) b2 = kin(Tz);
+        Tz = frac_of(Tz);
+        if (PRIM) {
+          if constexpr (W) {
+            const int qc = wrow(Tcl, mc);
+            pc = wpr[qc];
+            if (DTC) kc = wkap[qc];
+          } else {
+            const int rc = row_sat(Tcl, Ti1, Ti2m1);
+            pc = ldu(UM ? pr_u : pr0 + (long)cur.m * nT, (unsigned)rc);
+            if (DTC) kc = ldu(kr, (unsigned)rc);  // kappa at the cell for its dt
+          }
+          badc = kin(Tcl);
+          fc = frac_of(Tcl);
+        }
+        if (ty < 2) {  // waves 0 and 1 hold the halo lanes (a wave-uniform test)
+          const V2* krh = UM ? kap_u : kap0 + (long)cur.mh * nT;
+          Th = yhalo ? (R)0.5 * (ring[sc][0][x] + ring[sc][1][x]) : (R)0.5 * (ring[sc][hq][0] + ring[sc][hq][1]);
+          if constexpr (W) kh = wkap[wrow(Th, mh)];
+          else kh = ldu(krh, (unsigned)row_sat(Th, Ti1, Ti2m1));
+          if (!PRIM) b3 = kin(Th);
+          Th = frac_of(Th);
+        }
+        // primitive variables: every cell's own row is checked (badc), and a face
+        // average lies between its two cells' temperatures, so a face outside
+        // the table implies a cell outside it -- the face tests are redundant
+        bad = PRIM ? 0 : (live && ((hx && b0) || (hy && b1) || (k < B.nk && b2))) || ((yhalo_in || xhalo_in) && b3);
+        badc = live && badc;
+        // a row of a face whose flux is used (or of the cell) outside the window
+        if constexpr (W)
+          miss |= __builtin_amdgcn_ballot_w64((live && ((hx && mx) || (hy && my) || (k < B.nk && mz) || mc)) ||
+                                              ((yhalo_in || xhalo_in) && mh));
+      }
+      // phase B: the next plane's loads (the last plane re-reads its own cells)
+      {
+        const unsigned step = k < k1 ? sz : 0u;
+        fetch_plane_u<TY>(pre, a.T, B, t.i0, t.j0, k < k1 ? k + 2 : k + 1);
+        stream(nxt, p + step, ph + step);
+      }
+      if (spend) {  // the previous plane's result
+        stn(out, ppend, vpend);
+        if (DTC) stn(a.dtw, ppend, dtpend);
+      }
+      if (RECT) {
+        // the plane's geometry from the per-axis factors (same products as the
+        // setup check against the stored metrics)
+        const double azk = zfac[0][k - k0 + 1], fzk = zfac[1][k - k0 + 1];
+        cur.vol = (R)((double)cvol * azk);
+        cur.w0 = (R)((double)cw0 * azk);
+        cur.w4 = (R)((double)cw4 * azk);
+        cur.w8 = (R)(fzk * (double)cvol);
+        cur.wh = (R)((double)cwh * azk);
+      }
+      // phase C: fluxes
+      R fz = 0;
+      {
+  #pragma clang fp contract(off)
+        if (live) {
+          if (RECT || hx) fx[ty][tx + 1] = (kx.x + Tx * kx.y) * ((Txr - Tc) * cur.w0);
+          if (RECT || hy) fy[ty + 1][tx] = (ky.x + Ty * ky.y) * ((Tyr - Tc) * cur.w4);
+          if (RECT || k < B.nk) fz = (kz.x + Tz * kz.y) * ((Tzr - Tc) * cur.w8);
+        }
+        if (yhalo) fy[0][tx] = (kh.x + Th * kh.y) * ((ring[sc][1][x] - ring[sc][0][x]) * cur.wh);
+        if (xhalo) fx[hr][0] = (kh.x + Th * kh.y) * ((ring[sc][hq][1] - ring[sc][hq][0]) * cur.wh);
+      }
+      if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
+      __syncthreads();
+      if (live) {
+        R Rv = 0;
+        {
+  #pragma clang fp contract(off)
+          // (rectilinear levels: the block's faces hold +-0; fz_prev is +0 on
+          // the block's first plane)
+          if (RECT || i > 1) Rv = Rv + fx[ty][tx];
+          if (RECT || hx) Rv = Rv - fx[ty][tx + 1];
+          if (RECT || j > 1) Rv = Rv + fy[ty][tx];
+          if (RECT || hy) Rv = Rv - fy[ty + 1][tx];
+          if (RECT || k > 1) Rv = Rv + fz_prev;
+          if (RECT || k < B.nk) Rv = Rv - fz;
+        }
+        {
+  #pragma clang fp contract(off)
+          // BC_Fluxes contributions in face order (as apply_bc_faces); +0
+          // where the cell has none
+          // (scalar branches: an opaque statement keeps them from being turned
+          // into selects)
+          if (xtile) {
+            keep_branch();
+            Rv = Rv - bx;
+          }
+          if (yrow) {
+            keep_branch();
+            Rv = Rv - by;
+          }
+          if (zpl) {
+            keep_branch();
+            Rv = Rv - bz;
+          }
+        }
+        R dt = a.use_dtg ? dtg : cur.dt;
+        if (DTC) {
+          // Compute_dt (Mod_dt.f90:26-84) at T^n, the first stage's input: the
+          // kappa, rho, cp interpolations of k_dt at the cell's row (rho and cp
+          // are the residual's below), sum 1/dl^2 from the per-axis factors
+  #pragma clang fp contract(off)
+          const R fr = fc;
+          const R kap = kc.x + fr * kc.y, rho = pc.x + fr * pc.y, cp = pc.z + fr * pc.w;
+          const double alpha = (double)kap / (double)rho / (double)cp;
+          double dd = 0.5 * a.vnn / (alpha * (qxy + zfac[2][k - k0 + 1]));
+          dd = dd < 1e8 ? dd : 1e8;
+          dt = (R)dd;
+          dtpend = dt;
+          dtmin = fmin(dtmin, dd);
+        }
+        R res;
+        {
+  #pragma clang fp contract(off)
+          if (PRIM) {
+            // Compute_Prim_Residual, Lib_Newstate.f90:241-256 (as scaled_residual)
+            const R rho = pc.x + fc * pc.y;
+            const R cp = pc.z + fc * pc.w;
+            res = (-Rv / cur.vol + cur.qv) * dt / (Tc * cp * pc.y + rho * cp + Tc * rho * pc.w);
+          } else {
+            res = (-Rv / cur.vol + cur.qv) * dt;
+          }
+        }
+        int bad2 = badc;
+        if (IRSMODE) {
+          vpend = res;
+        } else {
+          const R Tn = new_state<PRIM>(a.tab, cur.m, Tc, cur.TO, res, a.rkc, a.jam, &bad2);
+          vpend = Tn;
+          if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
+          const double d = (double)Tn - (double)cur.TO;
+          rsq += d * d;
+        }
+        ppend = p;
+        if (bad2) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
+      }
+      spend = live;
+      fz_prev = fz;
+      const int s0 = sm;
+      sm = sc;
+      sc = sp;
+      sp = s0;
+      p += sz;
+      ph += sz;
+    };
+    // two planes per trip with the stream buffers swapping roles (no copy of
+    // in-flight registers)
+    for (int k = k0; k <= k1; k += 2) {
+      plane(sa_, sb_, k);
+      if (k + 1 <= k1) plane(sb_, sa_, k + 1);
+    }
+    if (spend) {
       stn(out, ppend, vpend);
       if (DTC) stn(a.dtw, ppend, dtpend);
     }
-    if (RECT) {
-      // the plane's geometry from the per-axis factors (same products as the
-      // setup check against the stored metrics)
-      const double azk = zfac[0][k - k0 + 1], fzk = zfac[1][k - k0 + 1];
-      cur.vol = (R)((double)cvol * azk);
-      cur.w0 = (R)((double)cw0 * azk);
-      cur.w4 = (R)((double)cw4 * azk);
-      cur.w8 = (R)(fzk * (double)cvol);
-      cur.wh = (R)((double)cwh * azk);
-    }
-    // phase C: fluxes
-    R fz = 0;
-    {
-#pragma clang fp contract(off)
-      if (live) {
-        if (RECT || hx) fx[ty][tx + 1] = (kx.x + Tx * kx.y) * ((Txr - Tc) * cur.w0);
-        if (RECT || hy) fy[ty + 1][tx] = (ky.x + Ty * ky.y) * ((Tyr - Tc) * cur.w4);
-        if (RECT || k < B.nk) fz = (kz.x + Tz * kz.y) * ((Tzr - Tc) * cur.w8);
-      }
-      if (yhalo) fy[0][tx] = (kh.x + Th * kh.y) * ((ring[sc][1][x] - ring[sc][0][x]) * cur.wh);
-      if (xhalo) fx[hr][0] = (kh.x + Th * kh.y) * ((ring[sc][hq][1] - ring[sc][hq][0]) * cur.wh);
-    }
-    if (bad) flag_error(a.status, FUSS_ETABLE, -1, 0, 0, 0);
-    __syncthreads();
-    if (live) {
-      R Rv = 0;
-      {
-#pragma clang fp contract(off)
-        // (rectilinear levels: the block's faces hold +-0; fz_prev is +0 on
-        // the block's first plane)
-        if (RECT || i > 1) Rv = Rv + fx[ty][tx];
-        if (RECT || hx) Rv = Rv - fx[ty][tx + 1];
-        if (RECT || j > 1) Rv = Rv + fy[ty][tx];
-        if (RECT || hy) Rv = Rv - fy[ty + 1][tx];
-        if (RECT || k > 1) Rv = Rv + fz_prev;
-        if (RECT || k < B.nk) Rv = Rv - fz;
-      }
-      {
-#pragma clang fp contract(off)
-        // BC_Fluxes contributions in face order (as apply_bc_faces); +0
-        // where the cell has none
-        // (scalar branches: an opaque statement keeps them from being turned
-        // into selects)
-        if (xtile) {
-          keep_branch();
-          Rv = Rv - bx;
-        }
-        if (yrow) {
-          keep_branch();
-          Rv = Rv - by;
-        }
-        if (zpl) {
-          keep_branch();
-          Rv = Rv - bz;
-        }
-      }
-      R dt = a.use_dtg ? dtg : cur.dt;
-      if (DTC) {
-        // Compute_dt (Mod_dt.f90:26-84) at T^n, the first stage's input: the
-        // kappa, rho, cp interpolations of k_dt at the cell's row (rho and cp
-        // are the residual's below), sum 1/dl^2 from the per-axis factors
-#pragma clang fp contract(off)
-        const R fr = fc;
-        const R kap = kc.x + fr * kc.y, rho = pc.x + fr * pc.y, cp = pc.z + fr * pc.w;
-        const double alpha = (double)kap / (double)rho / (double)cp;
-        double dd = 0.5 * a.vnn / (alpha * (qxy + zfac[2][k - k0 + 1]));
-        dd = dd < 1e8 ? dd : 1e8;
-        dt = (R)dd;
-        dtpend = dt;
-        dtmin = fmin(dtmin, dd);
-      }
-      R res;
-      {
-#pragma clang fp contract(off)
-        if (PRIM) {
-          // Compute_Prim_Residual, Lib_Newstate.f90:241-256 (as scaled_residual)
-          const R rho = pc.x + fc * pc.y;
-          const R cp = pc.z + fc * pc.w;
-          res = (-Rv / cur.vol + cur.qv) * dt / (Tc * cp * pc.y + rho * cp + Tc * rho * pc.w);
-        } else {
-          res = (-Rv / cur.vol + cur.qv) * dt;
-        }
-      }
-      int bad2 = badc;
-      if (IRSMODE) {
-        vpend = res;
-      } else {
-        const R Tn = new_state<PRIM>(a.tab, cur.m, Tc, cur.TO, res, a.rkc, a.jam, &bad2);
-        vpend = Tn;
-        if (Tn != Tn || Tn < (R)0) flag_state(a.status, a.irk, t.b, i, j, k);
-        const double d = (double)Tn - (double)cur.TO;
-        rsq += d * d;
-      }
-      ppend = p;
-      if (bad2) flag_error(a.status, FUSS_ETABLE, t.b, i, j, k);
-    }
-    spend = live;
-    fz_prev = fz;
-    const int s0 = sm;
-    sm = sc;
-    sc = sp;
-    sp = s0;
-    p += sz;
-    ph += sz;
+    // the column is done unless a lane of the work-group missed the window
+    if constexpr (W) return !__syncthreads_or(miss != 0);
+    return true;
   };
-  // two planes per trip with the stream buffers swapping roles (no copy of
-  // in-flight registers)
-  for (int k = k0; k <= k1; k += 2) {
-    plane(sa_, sb_, k);
-    if (k + 1 <= k1) plane(sb_, sa_, k + 1);
+  bool done = false;
+  if constexpr (WIN) {
+    if (Ti2 - Ti1 >= TWR) done = column(std::true_type{});
   }
-  if (spend) {
-    stn(out, ppend, vpend);
-    if (DTC) stn(a.dtw, ppend, dtpend);
+  if (!done) {
+    rsq = 0.0;
+    dtmin = 1e300;
+    column(std::false_type{});
   }
   if (!IRSMODE && a.last) block_sum<TT>(rsq, a.partials, wg);
   if (DTC) block_min<TT>(dtmin, a.pmin, wg);

@@ -172,3 +172,64 @@ def test_regcol_table_error(gpu):
         s.explicit_step(1)
     assert ei.value.code == gpu.FUSS_ETABLE
     s.close()
+
+
+def _graded(dims, axis, lo, hi, seed=5):
+    """A tall box whose temperature runs from lo to hi along one axis."""
+    c = tallbox(dims, seed=seed)
+    T = c.blocks[0].T
+    n = T.shape[2 - axis]
+    ramp = lo + (hi - lo) * np.linspace(0.0, 1.0, n)
+    shape = [1, 1, 1]
+    shape[2 - axis] = n
+    c.blocks[0].T = T - 300.0 + ramp.reshape(shape)
+    return c
+
+
+# Steady primitive stages with IRS on a level of one material stage the
+# table rows of each column in LDS (a window of rows around its first
+# plane's temperatures); a face or cell outside the window makes the
+# work-group compute its column again from memory, and a first plane whose
+# rows span half the window sends the column to memory at once.
+WINDOW_CASES = {
+    # 300 -> 700 K along k: every column leaves its window partway up
+    "ramp_k": lambda: _graded((66, 9, 70), 2, 300.0, 700.0),
+    # 300 -> 900 K along i: first planes span more than half the window
+    "ramp_i": lambda: _graded((70, 9, 20), 0, 300.0, 900.0),
+    # one hot cell in one column (that work-group alone misses)
+    "hot_cell": lambda: _hot_cell(),
+}
+
+
+def _hot_cell():
+    c = tallbox((70, 12, 40))
+    c.blocks[0].T[30, 6, 20] = 820.0
+    return c
+
+
+@pytest.mark.parametrize("stage_rows", [4, 8])
+@pytest.mark.parametrize("name", list(WINDOW_CASES))
+def test_window_misses_match_memory_path(gpu, name, stage_rows):
+    """The LDS-ring kernel with its row window (and the recomputed columns)
+    gives the register-column kernel's state (rows from memory) bit for
+    bit."""
+    c = WINDOW_CASES[name]()
+    ring = run(gpu, c, 16, stage_rows)
+    reg = run(gpu, c, 16, gpu.FUSS_STAGE_REGCOL)
+    assert np.array_equal(ring[0], reg[0])
+    np.testing.assert_allclose(ring[1], reg[1], rtol=1e-13)
+    assert np.array_equal(ring[2], reg[2])
+
+
+@pytest.mark.parametrize("name", list(WINDOW_CASES))
+def test_window_misses_vs_oracle(gpu, oracle_mod, name):
+    c = WINDOW_CASES[name]()
+    s = gpu.Solver(c)
+    s.set_tiles(1, kc=64, row_march=0, stage_rows=8, irs_rows=8)
+    o = oracle_mod.Oracle(c)
+    for _ in range(3):
+        r = s.explicit_step(1)
+        ro = o.step(1)
+        assert abs(np.sqrt(r.res_sq) - np.sqrt(ro)) <= 1e-9 * np.sqrt(ro)
+        assert relerr(s.get("T"), o.get("T")) < RTOL
+    s.close()
