@@ -62,7 +62,7 @@ def stage_kernel(rect, kc, rows):
 
 STAGE_TRAFFIC_MEASURED = {stage_kernel(True, kc, rows): fetch * 1024 * 2.0 + 131072 * 1024 * 1.0
                           for kc, rows, fetch in ((4, 4, 153140), (8, 4, 154981), (16, 4, 149399), (32, 4, 143788),
-                                                  (64, 4, 140748), (32, 8, 143789), (64, 8, 140803))}
+                                                  (64, 4, 140721), (32, 8, 143789), (64, 8, 140803))}
 # Plain FP64 path (four-stage RK + LTS + IRS 0.5, 3-level sequencing) run to
 # residual 1e-10 on the GPU: iterations per cube size (bench runs with
 # --plain-conv-n 32, 48, 64 and, with --n 128, 128).  Least-squares fit of
@@ -499,11 +499,12 @@ def c1_check(args):
     return out
 
 
-def summarize_profile(db_path, top=16):
+def summarize_profile(db_path, top=16, only=None):
     """Markdown summary of a rocprofv3 results database (kernel-trace
     statistics, or per-kernel PMC counter values when the run collected
     counters): calls, mean / min / max duration and share per kernel; for
-    counters the median and max value per dispatch."""
+    counters the median and max value per dispatch.  top: rows kept; only:
+    kernels whose name contains this text (counters)."""
     import sqlite3
 
     db = sqlite3.connect(db_path)
@@ -516,7 +517,8 @@ def summarize_profile(db_path, top=16):
         rows = cur.execute("select name, counter_name, counter_value from pmc_events").fetchall()
         by = {}
         for name, ctr, v in rows:
-            by.setdefault((name, ctr), []).append(v)
+            if only is None or only in name:
+                by.setdefault((name, ctr), []).append(v)
         for (name, ctr), vs in sorted(by.items(), key=lambda kv: -max(kv[1]))[:top]:
             lines.append(f"| `{name[:100]}` | {ctr} | {len(vs)} | {float(np.median(vs)):.1f} | {max(vs):.1f} |")
     else:
@@ -566,9 +568,11 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--adj-lbfgs", type=int, default=20)
     ap.add_argument("--summarize-profile", metavar="DB", help="print a markdown summary of a rocprofv3 database")
+    ap.add_argument("--profile-rows", type=int, default=16, help="with --summarize-profile: rows kept")
+    ap.add_argument("--profile-only", metavar="TEXT", help="with --summarize-profile: kernels whose name contains TEXT")
     args = ap.parse_args()
     if args.summarize_profile:
-        summarize_profile(args.summarize_profile)
+        summarize_profile(args.summarize_profile, args.profile_rows, args.profile_only)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); "
@@ -601,7 +605,8 @@ def main():
         tuned = s.autotune(1, steps=4)
         chosen = "fuss_autotune"
     _, _, stage_rows, irs_rows = s.tiles(1)
-    print(f"[bench] tiles ({chosen}): column height {tuned[0]}, {'row march' if tuned[1] else 'LDS ring'}, "
+    skind = "row march" if tuned[1] else ("register columns" if stage_rows == 1 else "LDS ring")
+    print(f"[bench] tiles ({chosen}): column height {tuned[0]}, {skind}, "
           f"stage columns of {stage_rows} rows, IRS columns of {irs_rows} rows, {tuned[2]:.3f} ms/step",
           file=sys.stderr, flush=True)
     # warm-up: module load, then a call of the timed size, which captures the
