@@ -92,8 +92,8 @@ CASES = {
 }
 
 
-def run(gpu, c, kc, stage_rows, irs_rows=4, steps=5):
-    s = gpu.Solver(c)
+def run(gpu, c, kc, stage_rows, irs_rows=4, steps=5, precision=None):
+    s = gpu.Solver(c) if precision is None else gpu.Solver(c, precision=precision)
     s.set_tiles(1, kc=kc, row_march=0, stage_rows=stage_rows, irs_rows=irs_rows)
     assert s.tiles(1) == (kc, 0, stage_rows, irs_rows)
     rs = [s.explicit_step(1) for _ in range(steps)]
@@ -207,15 +207,17 @@ def _hot_cell():
     return c
 
 
+@pytest.mark.parametrize("fp32", [False, True])
 @pytest.mark.parametrize("stage_rows", [4, 8])
 @pytest.mark.parametrize("name", list(WINDOW_CASES))
-def test_window_misses_match_memory_path(gpu, name, stage_rows):
+def test_window_misses_match_memory_path(gpu, name, stage_rows, fp32):
     """The LDS-ring kernel with its row window (and the recomputed columns)
     gives the register-column kernel's state (rows from memory) bit for
-    bit."""
+    bit, in FP64 and in FP32 contexts."""
     c = WINDOW_CASES[name]()
-    ring = run(gpu, c, 16, stage_rows)
-    reg = run(gpu, c, 16, gpu.FUSS_STAGE_REGCOL)
+    prec = gpu.FUSS_FP32 if fp32 else None
+    ring = run(gpu, c, 16, stage_rows, precision=prec)
+    reg = run(gpu, c, 16, gpu.FUSS_STAGE_REGCOL, precision=prec)
     assert np.array_equal(ring[0], reg[0])
     np.testing.assert_allclose(ring[1], reg[1], rtol=1e-13)
     assert np.array_equal(ring[2], reg[2])
