@@ -1,8 +1,8 @@
 """Parity of the HIP path (libfuss_gpu.so, FP64) with the oracle and with the
 reference's stored outputs.
 
-Tolerances: the north-star bar for the plain FP64 path is 1e-12 relative
-(L-inf on T).  Metrics are computed on the device with the reference's
+Tolerances: the bar for the plain FP64 path is 1e-12 relative (L-inf on
+T).  Metrics are computed on the device with the reference's
 operation order and FP contraction off, so they must agree bit for bit.
 The interior fluxes regroup A * kappa * (grad . n) as kappa * sum_c G_c w_c
 with precomputed face weights w (a different rounding, ~1 ulp per face), so
