@@ -321,6 +321,7 @@ def sweep_check(args):
 
     t0 = time.perf_counter()
     case = cases.sweep(ncase=args.sweep_cases, lo=32, hi=96, mgl=3, n_rk=args.rk)
+    gen_s = time.perf_counter() - t0
     s = gpu.Solver(case)
     setup_s = time.perf_counter() - t0
     ncell = case.ncell
@@ -336,7 +337,11 @@ def sweep_check(args):
     T0 = [b.T for b in case.blocks]
     out = {"workload": f"{args.sweep_cases} independent steady cubes, dims 32..96 (multiples of 4), "
                        f"k 5-200, h 10-500, qvol 0-1e6, RK{args.rk} + IRS 0.5, one context",
-           "cells": ncell, "setup_s": setup_s, "plain_fp64_ms_per_step": step_s * 1e3,
+           "cells": ncell, "setup_s": setup_s,
+           # (numpy generation of the synthetic cases, then the C-ABI set-up:
+           # block and BC uploads, device metrics, BC records, levels)
+           "case_generation_s": gen_s, "solver_setup_s": setup_s - gen_s,
+           "plain_fp64_ms_per_step": step_s * 1e3,
            "tiles": {"column_planes": tuned[0], "stage_kernel": "row march" if tuned[1] else "LDS ring",
                      "stage_rows": s.tiles(1)[2], "irs_rows": s.tiles(1)[3]},
            "plain_fp64_cell_updates_per_s": ncell / step_s}
