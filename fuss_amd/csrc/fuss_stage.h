@@ -537,12 +537,13 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
     __syncthreads();
     // the window: TWR rows centred on the rows of plane k0's cells (every
     // work-group of a level of one material per block reads its block's rows)
-    // (lanes with a miss, or-ed over the planes: a uniform mask)
     int wlo = Ti1;
+    // (lanes with a miss, or-ed over the planes: a uniform mask)
     unsigned long long miss = 0;
     if constexpr (W) {
       const int rq = row_sat(ring[sc][y][x], Ti1, Ti2m1);
-      int rlo = live ? rq : 0x7fffffff, rhi = live ? rq : -0x7fffffff;
+      // (sentinels for lanes outside the block: their difference cannot overflow)
+      int rlo = live ? rq : 0x3fffffff, rhi = live ? rq : -0x3fffffff;
       for (int o = 32; o > 0; o >>= 1) {
         rlo = min(rlo, __shfl_xor(rlo, o, 64));
         rhi = max(rhi, __shfl_xor(rhi, o, 64));
