@@ -74,10 +74,6 @@ __device__ __forceinline__ R dpp_quad_from(R src, int e) {
   }
 }
 
-
-
-
-
 // Raw buffer accesses: a resource per array (base address, no range
 // limit), a loop-invariant 32-bit lane offset in a VGPR and the plane's byte
 // offset in an SGPR, so no 64-bit address is formed per row and array.
