@@ -235,3 +235,34 @@ def test_window_misses_vs_oracle(gpu, oracle_mod, name):
         assert abs(np.sqrt(r.res_sq) - np.sqrt(ro)) <= 1e-9 * np.sqrt(ro)
         assert relerr(s.get("T"), o.get("T")) < RTOL
     s.close()
+
+
+def _short_table_box():
+    """A table of 100 rows (250-349 K), shorter than the row window: the
+    ring kernel reads its rows from memory."""
+    from fuss_amd import formats as fm
+
+    c = tallbox((66, 9, 20))
+    Tr = np.arange(250, 350, dtype=np.float64)
+    kap = (20.0 + 0.02 * (Tr - 300.0)).reshape(1, -1)
+    rho = np.full_like(kap, 7800.0)
+    cp = (460.0 + 0.1 * (Tr - 300.0)).reshape(1, -1)
+    c.table = fm.PropertyTable(250, cp, rho, kap, np.cumsum(rho * cp, axis=1), ["short"])
+    return c
+
+
+@pytest.mark.parametrize("stage_rows", [4, 8])
+def test_short_table_memory_path(gpu, oracle_mod, stage_rows):
+    c = _short_table_box()
+    ring = run(gpu, c, 16, stage_rows)
+    reg = run(gpu, c, 16, gpu.FUSS_STAGE_REGCOL)
+    assert np.array_equal(ring[0], reg[0])
+    assert np.array_equal(ring[2], reg[2])
+    s = gpu.Solver(c)
+    s.set_tiles(1, kc=16, row_march=0, stage_rows=stage_rows, irs_rows=8)
+    o = oracle_mod.Oracle(c)
+    for _ in range(2):
+        s.explicit_step(1)
+        o.step(1)
+    assert relerr(s.get("T"), o.get("T")) < RTOL
+    s.close()
