@@ -578,7 +578,7 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
     };
     R fz_prev = 0;
     if (live && k0 > 1) {
-  #pragma clang fp contract(off)
+#pragma clang fp contract(off)
       fz_prev = flux_z<false>(a, B, ring[sm], ring[sc], x, y, p - sz,
                               RECT ? (R)(zfac[1][0] * (double)cvol) : ldu(a.W[8], p - sz));
     }
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
       if (yrow) by = ldb<R>(a.bcf, yb8 + ys8 * (unsigned)(k - 1));
       if (zpl) bz = ldb<R>(a.bcf, k == 1 ? zl8 : zh8);
       {
-  #pragma clang fp contract(off)
+#pragma clang fp contract(off)
         // rows addressed by 32-bit offsets from the material's rows; the
         // fraction T - row is x - floor(x) (the same for T in the table; outside
         // it an error is flagged)
@@ -690,7 +690,7 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
       // phase C: fluxes
       R fz = 0;
       {
-  #pragma clang fp contract(off)
+#pragma clang fp contract(off)
         if (live) {
           if (RECT || hx) fx[ty][tx + 1] = (kx.x + Tx * kx.y) * ((Txr - Tc) * cur.w0);
           if (RECT || hy) fy[ty + 1][tx] = (ky.x + Ty * ky.y) * ((Tyr - Tc) * cur.w4);
@@ -704,7 +704,7 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
       if (live) {
         R Rv = 0;
         {
-  #pragma clang fp contract(off)
+#pragma clang fp contract(off)
           // (rectilinear levels: the block's faces hold +-0; fz_prev is +0 on
           // the block's first plane)
           if (RECT || i > 1) Rv = Rv + fx[ty][tx];
@@ -715,7 +715,7 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
           if (RECT || k < B.nk) Rv = Rv - fz;
         }
         {
-  #pragma clang fp contract(off)
+#pragma clang fp contract(off)
           // BC_Fluxes contributions in face order (as apply_bc_faces); +0
           // where the cell has none
           // (scalar branches: an opaque statement keeps them from being turned
@@ -738,7 +738,7 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
           // Compute_dt (Mod_dt.f90:26-84) at T^n, the first stage's input: the
           // kappa, rho, cp interpolations of k_dt at the cell's row (rho and cp
           // are the residual's below), sum 1/dl^2 from the per-axis factors
-  #pragma clang fp contract(off)
+#pragma clang fp contract(off)
           const R fr = fc;
           const R kap = kc.x + fr * kc.y, rho = pc.x + fr * pc.y, cp = pc.z + fr * pc.w;
           const double alpha = (double)kap / (double)rho / (double)cp;
@@ -750,7 +750,7 @@ __global__ __launch_bounds__(TILE_X * TY) void k_stage3p(StageArgs<R> a) {
         }
         R res;
         {
-  #pragma clang fp contract(off)
+#pragma clang fp contract(off)
           if (PRIM) {
             // Compute_Prim_Residual, Lib_Newstate.f90:241-256 (as scaled_residual)
             const R rho = pc.x + fc * pc.y;
